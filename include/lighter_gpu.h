@@ -1,0 +1,129 @@
+/*
+ * lighter_gpu.h — additional C entry points of liblighter_amd.so.
+ *
+ * The ltr_* API (lighter.h) is the drop-in boundary and runs a whole bake.
+ * The lgx_* functions below have no counterpart in the reference: they expose
+ * the same hot-path stages one at a time, for callers that drive the stages
+ * themselves (tests, benchmarks, tools) and for the radiosity solver forms
+ * and adjoint that the reference does not have.  Each comment names the
+ * reference code whose behaviour the stage reproduces.
+ *
+ * Conventions: plain pointers and sizes, host memory in and out; functions
+ * return 1 on success and 0 on failure, with the message available from
+ * lgx_LastError() (per calling thread).  All GPU work runs on one HIP device
+ * (the current one).
+ */
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "lighter.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+LTRAPI const char* lgx_LastError(void);
+/* 1 when a HIP device is usable, else 0 (and lgx_LastError says why). */
+LTRAPI int lgx_HasDevice(void);
+
+/* ---- staged bake of an ltr_Scene (synchronous, calling thread) -------- */
+/* Host stages "transforming spatial data" .. "generating samples"
+ * (lighter.cpp:1050-1073). No GPU needed. */
+LTRAPI int lgx_Prepare(ltr_Scene* scene);
+/* "rendering lightmaps" (lighter.cpp:1075-1096) on the GPU. */
+LTRAPI int lgx_StageDirect(ltr_Scene* scene);
+/* "calculating radiosity" .. "committing radiosity" (lighter.cpp:1098-1124). */
+LTRAPI int lgx_StageRadiosity(ltr_Scene* scene);
+/* "rendering ambient occlusion" (lighter.cpp:1126-1136). */
+LTRAPI int lgx_StageAO(ltr_Scene* scene);
+/* "exporting lightmaps" (lighter.cpp:1138-1140): fills the work outputs. */
+LTRAPI int lgx_StageFinalize(ltr_Scene* scene);
+
+LTRAPI int lgx_InstanceCount(ltr_Scene* scene);
+/* out: lightmap width, height, sample count, collision triangle count. */
+LTRAPI int lgx_InstanceInfo(ltr_Scene* scene, int inst, int32_t out[4]);
+/* Sample arrays of one instance (lighter_int.hpp:897-901); NULL skips. */
+LTRAPI int lgx_GetSamples(ltr_Scene* scene, int inst, float* pos3, float* nrm3, uint32_t* loc, float* radinfo4);
+LTRAPI int lgx_GetLightmap(ltr_Scene* scene, int inst, float* rgb3);
+LTRAPI int lgx_SetLightmap(ltr_Scene* scene, int inst, const float* rgb3);
+/* Collision triangles of one instance, 9 floats each. */
+LTRAPI int lgx_GetTriangles(ltr_Scene* scene, int inst, float* tris9);
+/* Instances light `light` renders into (lighter.cpp:1080-1093). */
+LTRAPI int lgx_LightInstances(ltr_Scene* scene, int light, int32_t* out, int cap);
+/* Radiosity links after lgx_StageRadiosity: rowmap {offset, count} per
+ * patch, then (other, factor) per link, as Int_RDGenLinks lists them. */
+LTRAPI size_t lgx_RadLinkCount(ltr_Scene* scene);
+LTRAPI int lgx_RadGetLinks(ltr_Scene* scene, uint32_t* rowmap, uint32_t* other, float* factor);
+/* Stage times in ms: direct, direct march, AO, links, bounces. */
+LTRAPI int lgx_StageTimes(ltr_Scene* scene, double out[5]);
+
+/* ---- device-level hot path over raw arrays -------------------------- */
+typedef struct lgx_Device lgx_Device;
+
+/* Light parameters as the reference stores them after ltr_LightAdd
+ * (lighter.cpp:1282-1298): dir normalised by the caller. */
+typedef struct lgx_Light {
+	int32_t type;
+	float pos[3];
+	float dir[3];
+	float color[3];
+	float range;
+	float power;
+	float radius;
+	float spot_angle_out; /* degrees */
+	float spot_angle_in;  /* degrees */
+	float spot_curve;
+} lgx_Light;
+
+/* Uploads shadow-casting triangles (9 floats each) and builds the BVH. */
+LTRAPI lgx_Device* lgx_DeviceCreate(const float* tris9, size_t ntri);
+LTRAPI void lgx_DeviceDestroy(lgx_Device* dev);
+/* out: triangles, BVH nodes, BVH depth. */
+LTRAPI int lgx_DeviceInfo(lgx_Device* dev, int64_t out[3]);
+
+/* Scene queries, one result per query:
+ *  distance: min(2, nearest shadow-triangle distance)   (lighter.cpp:183-188)
+ *  shadow:   CalcInvShadowFactor(from, to, k) + step count (lighter.cpp:190-207)
+ *  closest:  DistanceTestBBT(a, b)                       (lighter.cpp:280-289)
+ *  occluded: VisibilityTest(a, b)                        (lighter.cpp:138-147) */
+LTRAPI int lgx_QueryDistance(lgx_Device* dev, const float* p3, size_t n, float* out);
+LTRAPI int lgx_QueryShadow(lgx_Device* dev, const float* from3, const float* to3, const float* k, size_t n,
+                           float* out, uint32_t* steps);
+LTRAPI int lgx_QueryClosest(lgx_Device* dev, const float* a3, const float* b3, size_t n, float* out);
+LTRAPI int lgx_QueryOccluded(lgx_Device* dev, const float* a3, const float* b3, size_t n, uint8_t* out);
+
+/* Samples and the lightmap they accumulate into (lightmap NULL = zeros). */
+LTRAPI int lgx_SetSamples(lgx_Device* dev, const float* pos3, const float* nrm3, size_t n, const float* lightmap3);
+LTRAPI int lgx_ReadLightmap(lgx_Device* dev, float* lightmap3);
+/* Direct light (Job_LMRender_*_Inner, lighter.cpp:485-600) for work items
+ * (light << 26) | sample grouped by light; light_off has nlights+1 entries.
+ * At most 64 lights per call; contributions are added in light order. */
+LTRAPI int lgx_DirectLight(lgx_Device* dev, const lgx_Light* lights, int nlights, const uint32_t* items,
+                           size_t nitems, const uint32_t* light_off, int keep_vis);
+/* Per item: weight multiplying the light colour (bits 0xFFFFFFFF where the
+ * reference skips the texel) and, if kept, the shadow factor. */
+LTRAPI int lgx_ReadWeights(lgx_Device* dev, float* weights, float* vis, size_t n);
+/* Ambient occlusion (Job_AORender_Inner, lighter.cpp:799-841) with one
+ * rotation offset per sample. */
+LTRAPI int lgx_AmbientOcclusion(lgx_Device* dev, const float* randoff, int num_samples, float distance,
+                                float multiplier, float falloff, float effect, const float* color3);
+/* Radiosity links among N patches (Int_RDGenLinks pair loop,
+ * lighter.cpp:728-759); returns the link count through nlinks. */
+LTRAPI int lgx_RadLinks(lgx_Device* dev, const float* pos3, const float* nrm3, size_t n, size_t* nlinks);
+LTRAPI int lgx_RadReadLinks(lgx_Device* dev, uint32_t* rowmap, uint32_t* other, float* factor);
+/* bounce_count rounds of Int_RDBounce (lighter.cpp:767-797) from emitted
+ * light E; total3 receives the accumulated light per patch. */
+LTRAPI int lgx_RadBounce(lgx_Device* dev, const float* diffuse3, const float* emit3, const float* area, int bounces,
+                         float* total3);
+/* Traversal counters (node visits, triangle tests, march steps, queries)
+ * collected by the next stage when enabled. */
+LTRAPI int lgx_SetCounting(lgx_Device* dev, int on);
+LTRAPI int lgx_ReadCounters(lgx_Device* dev, uint64_t out[4]);
+/* Last stage times in ms: direct, direct march, AO, links, bounces. */
+LTRAPI int lgx_DeviceTimes(lgx_Device* dev, double out[5]);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,228 @@
+// Binned-SAH BVH builder for the gfx950 kernels; see bvh.h.
+#include "bvh.h"
+
+#include <float.h>
+#include <math.h>
+
+#include <algorithm>
+
+namespace lgx {
+
+namespace {
+
+struct Aabb {
+	float lo[3], hi[3];
+	void reset()
+	{
+		for (int a = 0; a < 3; ++a) {
+			lo[a] = FLT_MAX;
+			hi[a] = -FLT_MAX;
+		}
+	}
+	void grow(const Aabb& b)
+	{
+		for (int a = 0; a < 3; ++a) {
+			lo[a] = std::min(lo[a], b.lo[a]);
+			hi[a] = std::max(hi[a], b.hi[a]);
+		}
+	}
+	void grow(const float* p)
+	{
+		for (int a = 0; a < 3; ++a) {
+			lo[a] = std::min(lo[a], p[a]);
+			hi[a] = std::max(hi[a], p[a]);
+		}
+	}
+	float area() const
+	{
+		float d[3];
+		for (int a = 0; a < 3; ++a)
+			d[a] = std::max(0.0f, hi[a] - lo[a]);
+		return 2.0f * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+	}
+};
+
+constexpr int kBins = 32;
+// Below this depth SAH splits; deeper subtrees split at the median so the
+// total depth stays under the kernels' traversal stack (kStackSize).
+constexpr int kMedianDepth = 40;
+
+struct Builder {
+	const float* tris;
+	std::vector<Aabb> box;
+	std::vector<float> cen; // 3 per triangle
+	std::vector<uint32_t> idx;
+	std::vector<BvhNode> nodes;
+	int max_depth = 0;
+
+	// Pads a box outward so that float rounding in the kernels' box tests
+	// never rejects a point of a contained triangle.
+	static Aabb padded(const Aabb& b)
+	{
+		Aabb o = b;
+		for (int a = 0; a < 3; ++a) {
+			float m = 1e-5f * std::max(fabsf(b.lo[a]), fabsf(b.hi[a])) + 1e-6f;
+			o.lo[a] = b.lo[a] - m;
+			o.hi[a] = b.hi[a] + m;
+		}
+		return o;
+	}
+
+	void set_child(BvhNode& n, int which, const Aabb& b, int32_t ref)
+	{
+		Aabb p = padded(b);
+		if (which == 0) {
+			n.lo0x = p.lo[0]; n.lo0y = p.lo[1]; n.lo0z = p.lo[2];
+			n.hi0x = p.hi[0]; n.hi0y = p.hi[1]; n.hi0z = p.hi[2];
+			n.c0 = ref;
+		} else {
+			n.lo1x = p.lo[0]; n.lo1y = p.lo[1]; n.lo1z = p.lo[2];
+			n.hi1x = p.hi[0]; n.hi1y = p.hi[1]; n.hi1z = p.hi[2];
+			n.c1 = ref;
+		}
+	}
+
+	Aabb bounds(uint32_t b, uint32_t e) const
+	{
+		Aabb r;
+		r.reset();
+		for (uint32_t i = b; i < e; ++i)
+			r.grow(box[idx[i]]);
+		return r;
+	}
+
+	// Returns a child reference for triangles idx[b, e).
+	int32_t build(uint32_t b, uint32_t e, int depth)
+	{
+		max_depth = std::max(max_depth, depth);
+		uint32_t n = e - b;
+		if (n <= (uint32_t)kLeafMax)
+			return leaf_ref(b, n);
+		Aabb cb;
+		cb.reset();
+		for (uint32_t i = b; i < e; ++i)
+			cb.grow(&cen[3 * idx[i]]);
+		// binned SAH over the widest centroid axes
+		float best_cost = FLT_MAX;
+		int best_axis = -1, best_split = -1;
+		for (int a = 0; a < 3; ++a) {
+			float ext = cb.hi[a] - cb.lo[a];
+			if (!(ext > 0))
+				continue;
+			Aabb bb[kBins];
+			uint32_t cnt[kBins] = {0};
+			for (int k = 0; k < kBins; ++k)
+				bb[k].reset();
+			float sc = kBins / ext;
+			for (uint32_t i = b; i < e; ++i) {
+				int k = std::min(kBins - 1, (int)((cen[3 * idx[i] + a] - cb.lo[a]) * sc));
+				cnt[k]++;
+				bb[k].grow(box[idx[i]]);
+			}
+			float ra[kBins];
+			uint32_t rc[kBins];
+			Aabb acc;
+			acc.reset();
+			uint32_t c = 0;
+			for (int k = kBins - 1; k > 0; --k) {
+				acc.grow(bb[k]);
+				c += cnt[k];
+				ra[k] = acc.area();
+				rc[k] = c;
+			}
+			acc.reset();
+			c = 0;
+			for (int k = 0; k < kBins - 1; ++k) {
+				acc.grow(bb[k]);
+				c += cnt[k];
+				if (c == 0 || rc[k + 1] == 0)
+					continue;
+				float cost = acc.area() * c + ra[k + 1] * rc[k + 1];
+				if (cost < best_cost) {
+					best_cost = cost;
+					best_axis = a;
+					best_split = k;
+				}
+			}
+		}
+		uint32_t mid;
+		if (depth >= kMedianDepth) {
+			// bound the depth: balanced split along the widest centroid axis
+			int a = 0;
+			for (int k = 1; k < 3; ++k)
+				if (cb.hi[k] - cb.lo[k] > cb.hi[a] - cb.lo[a])
+					a = k;
+			mid = b + n / 2;
+			std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e,
+			                 [&](uint32_t x, uint32_t y) { return cen[3 * x + a] < cen[3 * y + a]; });
+		} else if (best_axis < 0) {
+			mid = b + n / 2; // coincident centroids: split by index
+		} else {
+			float ext = cb.hi[best_axis] - cb.lo[best_axis];
+			float sc = kBins / ext;
+			auto it = std::partition(idx.begin() + b, idx.begin() + e, [&](uint32_t t) {
+				int k = std::min(kBins - 1, (int)((cen[3 * t + best_axis] - cb.lo[best_axis]) * sc));
+				return k <= best_split;
+			});
+			mid = (uint32_t)(it - idx.begin());
+			if (mid == b || mid == e)
+				mid = b + n / 2;
+		}
+		int32_t self = (int32_t)nodes.size();
+		nodes.push_back(BvhNode{});
+		Aabb lb = bounds(b, mid), rb = bounds(mid, e);
+		int32_t l = build(b, mid, depth + 1);
+		int32_t r = build(mid, e, depth + 1);
+		set_child(nodes[self], 0, lb, l);
+		set_child(nodes[self], 1, rb, r);
+		return self;
+	}
+};
+
+} // namespace
+
+BvhBuild build_bvh(const float* tris, size_t ntri)
+{
+	Builder B;
+	B.tris = tris;
+	B.box.resize(ntri);
+	B.cen.resize(3 * ntri);
+	B.idx.resize(ntri);
+	Aabb all;
+	all.reset();
+	for (size_t t = 0; t < ntri; ++t) {
+		Aabb& b = B.box[t];
+		b.reset();
+		for (int v = 0; v < 3; ++v)
+			b.grow(tris + 9 * t + 3 * v);
+		for (int a = 0; a < 3; ++a)
+			B.cen[3 * t + a] = 0.5f * (b.lo[a] + b.hi[a]);
+		B.idx[t] = (uint32_t)t;
+		all.grow(b);
+	}
+	BvhBuild out;
+	if (ntri == 0) {
+		out.lo = mk3(0, 0, 0);
+		out.hi = mk3(0, 0, 0);
+		return out;
+	}
+	// The root (node 0) is always an inner node; a small scene gets a second
+	// child whose inverted box is never entered.
+	if (ntri <= (size_t)kLeafMax) {
+		B.nodes.push_back(BvhNode{});
+		B.set_child(B.nodes[0], 0, all, leaf_ref(0, (uint32_t)ntri));
+		B.nodes[0].lo1x = B.nodes[0].lo1y = B.nodes[0].lo1z = FLT_MAX;
+		B.nodes[0].hi1x = B.nodes[0].hi1y = B.nodes[0].hi1z = -FLT_MAX;
+		B.nodes[0].c1 = leaf_ref(0, 1);
+	} else {
+		B.build(0, (uint32_t)ntri, 0); // pushes the root first, at index 0
+	}
+	out.nodes = std::move(B.nodes);
+	out.order = std::move(B.idx);
+	out.lo = mk3(all.lo[0], all.lo[1], all.lo[2]);
+	out.hi = mk3(all.hi[0], all.hi[1], all.hi[2]);
+	out.depth = B.max_depth;
+	return out;
+}
+
+} // namespace lgx

@@ -1,0 +1,145 @@
+// Host interface of the gfx950 hot path: device-resident scene (BVH,
+// triangles, samples, lightmap) and the stage launchers used by the bake
+// pipeline and by the lgx_* entry points.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace lgx {
+
+// Light as stored after ltr_LightAdd (direction normalised), as the kernels
+// read it.
+struct DevLight {
+	int32_t type;
+	float pos[3];
+	float dir[3];
+	float color[3];
+	float range, power, radius;
+	float spot_out_rad, spot_diff; // outer angle and (inner - outer), radians
+	float spot_curve;
+};
+
+// Per-stage timing in milliseconds measured with HIP events on the stream
+// the kernels run on.
+struct StageTimes {
+	double direct_ms = 0, direct_march_ms = 0, ao_ms = 0, links_ms = 0, bounce_ms = 0;
+};
+
+// Optional traversal counters (when enabled): node visits and triangle
+// tests, for the algorithmic-bytes figure.
+struct QueryCounters {
+	unsigned long long nodes = 0, tris = 0, steps = 0, queries = 0;
+};
+
+class DeviceScene {
+public:
+	DeviceScene();
+	~DeviceScene();
+	DeviceScene(const DeviceScene&) = delete;
+	DeviceScene& operator=(const DeviceScene&) = delete;
+
+	// Shadow-casting triangles, 9 floats each (world space).
+	void set_triangles(const float* tris, size_t ntri);
+	// Samples (points + normals) and the lightmap they accumulate into.
+	void set_samples(const float* pos, const float* nrm, size_t n);
+	void set_lightmap(const float* rgb);
+	void get_lightmap(float* rgb) const;
+	size_t num_samples() const { return nsamples_; }
+	size_t num_triangles() const { return ntri_; }
+	int bvh_depth() const { return bvh_depth_; }
+	size_t bvh_nodes() const { return nnodes_; }
+
+	// --- direct light (lighter.cpp:485-600, 1075-1096) -------------------
+	// items: (light << 26) | sample, grouped by light in ascending light
+	// order; light_off has lights.size()+1 entries.  The march kernel writes
+	// one weight per item (bit pattern 0xFFFFFFFF where the reference skips
+	// the texel); the accumulation adds color * weight per light in order.
+	void direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
+	                  const std::vector<uint32_t>& light_off, bool keep_weights);
+	const float* weights_device() const { return d_w_; }
+	void get_weights(float* out, size_t n) const;
+	// shadow factor per item (1 where no march ran); kept when keep_weights
+	void get_vis(float* out, size_t n) const;
+	size_t num_items() const { return nitems_; }
+
+	// --- ambient occlusion (lighter.cpp:799-841) -----------------------
+	void ambient_occlusion(const float* randoff, int num_samples, float distance, float multiplier, float falloff,
+	                       float effect, const float* color);
+
+	// --- radiosity (lighter.cpp:667-797) ---------------------------------
+	// Patches = samples.  rad_links builds the symmetric link structure of
+	// occluded pairs passing the form-factor thresholds.
+	void rad_set_patches(const float* pos, const float* nrm, size_t n);
+	size_t rad_links();
+	size_t rad_num_links() const { return nlinks_; }
+	// Reference link list: rowmap {offset,count} per patch, other, factor.
+	void rad_get_links(uint32_t* rowmap, uint32_t* other, float* factor) const;
+	// Symmetric CSR (both directions).
+	void rad_get_csr(std::vector<uint32_t>& rowptr, std::vector<uint32_t>& col, std::vector<float>& val) const;
+	// Reference bounce state per patch and bounce loop; E = emitted light.
+	void rad_set_colors(const float* diffuse3, const float* emit3, const float* area);
+	void rad_bounce(int count);
+	void rad_get_total(float* out3) const;
+
+	// --- point queries for tests and timing --------------------------
+	void query_distance(const float* pts, size_t n, float* out);
+	void query_shadow(const float* from, const float* to, const float* k, size_t n, float* out, uint32_t* steps);
+	void query_closest(const float* a, const float* b, size_t n, float* out);
+	void query_occluded(const float* a, const float* b, size_t n, uint8_t* out);
+
+	StageTimes times;
+	QueryCounters counters;
+	bool count_queries = false;
+
+	// device pointers for the radiosity solvers
+	struct Rad {
+		float* pos = nullptr;
+		float* nrm = nullptr;
+		uint32_t* bits = nullptr; // N x words symmetric occlusion bitmask
+		size_t words = 0;
+		uint32_t* rowptr = nullptr;
+		uint32_t* col = nullptr;
+		float* val = nullptr;
+		size_t nnz = 0;
+		float* diffuse = nullptr;
+		float* area = nullptr;
+		float* total = nullptr;
+		float* out = nullptr;
+		float* in = nullptr;
+	};
+	Rad rad;
+	size_t npatches = 0;
+	void* stream() const { return stream_; }
+
+private:
+	void free_all();
+	void* stream_ = nullptr;
+	// BVH and triangles
+	void* d_nodes_ = nullptr;
+	void* d_dtri_ = nullptr;
+	void* d_rtri_ = nullptr;
+	size_t ntri_ = 0, nnodes_ = 0;
+	int bvh_depth_ = 0;
+	float lo_[3] = {0, 0, 0}, hi_[3] = {0, 0, 0};
+	// samples
+	float* d_pos_ = nullptr;
+	float* d_nrm_ = nullptr;
+	float* d_lm_ = nullptr;
+	size_t nsamples_ = 0;
+	// direct light
+	float* d_w_ = nullptr;
+	float* d_vis_ = nullptr;
+	size_t nitems_ = 0;
+	size_t nlinks_ = 0;
+	unsigned long long* d_counters_ = nullptr;
+};
+
+// Throws std::runtime_error unless a gfx950-class GPU is usable.
+void require_gpu();
+std::string device_name();
+
+} // namespace lgx

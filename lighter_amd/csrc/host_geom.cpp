@@ -1,0 +1,520 @@
+// Host geometry for the CPU-side bake stages; see host_geom.h.
+#include "host_geom.h"
+
+#include <float.h>
+#include <string.h>
+
+#include <algorithm>
+#include <unordered_set>
+
+namespace lgx {
+
+// ---------------------------------------------------------------------------
+// 4x4 inverse by cofactor expansion (lighter_math.cpp:19-147).  Element
+// (i, j) of the inverse is the (j, i) cofactor: the 3x3 minor without row j
+// and column i, expanded along its first column, each term written as
+// (column element) * (upper row factor) * (lower row factor) and summed left
+// to right.  This fixed term order keeps the result identical to the
+// reference's.
+// ---------------------------------------------------------------------------
+bool mat4_invert(const Mat4& M, Mat4& out)
+{
+	const float* a = &M.m[0][0];
+	float inv[16];
+	for (int i = 0; i < 4; ++i) {
+		for (int j = 0; j < 4; ++j) {
+			int rows[3], cols[3], nr = 0, nc = 0;
+			for (int k = 0; k < 4; ++k) {
+				if (k != j) rows[nr++] = k;
+				if (k != i) cols[nc++] = k;
+			}
+			float sign = ((i + j) & 1) ? -1.0f : 1.0f;
+			float acc = 0.0f;
+			bool first = true;
+			for (int k = 0; k < 3; ++k) {
+				int ra = rows[k == 0 ? 1 : 0], rb = rows[k == 2 ? 1 : 2];
+				float e = a[4 * rows[k] + cols[0]];
+				float s = (k == 1) ? -sign : sign;
+				float tp = e * a[4 * ra + cols[1]] * a[4 * rb + cols[2]];
+				float tn = e * a[4 * ra + cols[2]] * a[4 * rb + cols[1]];
+				if (first) {
+					acc = s > 0 ? tp : (-e) * a[4 * ra + cols[1]] * a[4 * rb + cols[2]];
+					first = false;
+				} else {
+					acc = s > 0 ? acc + tp : acc - tp;
+				}
+				acc = s > 0 ? acc - tn : acc + tn;
+			}
+			inv[4 * i + j] = acc;
+		}
+	}
+	float det = a[0] * inv[0] + a[1] * inv[4] + a[2] * inv[8] + a[3] * inv[12];
+	if (det == 0)
+		return false;
+	det = 1.0f / det;
+	float* o = &out.m[0][0];
+	for (int i = 0; i < 16; ++i)
+		o[i] = inv[i] * det;
+	return true;
+}
+
+f3 mat4_transform(const Mat4& M, f3 v, float w)
+{
+	const float(*m)[4] = M.m;
+	return mk3(v.x * m[0][0] + v.y * m[1][0] + v.z * m[2][0] + m[3][0] * w,
+	           v.x * m[0][1] + v.y * m[1][1] + v.z * m[2][1] + m[3][1] * w,
+	           v.x * m[0][2] + v.y * m[1][2] + v.z * m[2][2] + m[3][2] * w);
+}
+
+Mat4 normal_matrix(const Mat4& M)
+{
+	Mat4 o;
+	for (int r = 0; r < 3; ++r)
+		for (int c = 0; c < 3; ++c)
+			o.m[r][c] = M.m[r][c];
+	o.m[0][3] = o.m[1][3] = o.m[2][3] = 0;
+	o.m[3][0] = o.m[3][1] = o.m[3][2] = 0;
+	o.m[3][3] = 1;
+	Mat4 inv;
+	if (mat4_invert(o, inv))
+		o = inv;
+	for (int r = 0; r < 4; ++r)
+		for (int c = r + 1; c < 4; ++c)
+			std::swap(o.m[r][c], o.m[c][r]);
+	return o;
+}
+
+static float heron(float a, float b, float c)
+{
+	float p = (a + b + c) * 0.5f;
+	float pre = p * (p - a) * (p - b) * (p - c);
+	if (pre < 0)
+		return 0;
+	return sqrtf(pre);
+}
+
+float triangle_area(f3 p1, f3 p2, f3 p3) { return heron(length(p2 - p1), length(p3 - p2), length(p1 - p3)); }
+float triangle_area(f2 p1, f2 p2, f2 p3) { return heron(length2(p2 - p1), length2(p3 - p2), length2(p1 - p3)); }
+
+float sample_area(f2 t1, f2 t2, f2 t3, f3 p1, f3 p2, f3 p3)
+{
+	float lm = triangle_area(t1, t2, t3);
+	float co = triangle_area(p1, p2, p3);
+	return lm > 0 ? co / lm : 0;
+}
+
+// ---------------------------------------------------------------------------
+bool ray_box(const RaySeg& r, f3 mn, f3 mx)
+{
+	float lo = -FLT_MAX, hi = FLT_MAX;
+	if (r.inv_dir.x != 0.0f) {
+		float a = (mn.x - r.origin.x) * r.inv_dir.x, b = (mx.x - r.origin.x) * r.inv_dir.x;
+		lo = tmax(lo, tmin(a, b));
+		hi = tmin(hi, tmax(a, b));
+	}
+	if (r.inv_dir.y != 0.0f) {
+		float a = (mn.y - r.origin.y) * r.inv_dir.y, b = (mx.y - r.origin.y) * r.inv_dir.y;
+		lo = tmax(lo, tmin(a, b));
+		hi = tmin(hi, tmax(a, b));
+	}
+	if (r.inv_dir.z != 0.0f) {
+		float a = (mn.z - r.origin.z) * r.inv_dir.z, b = (mx.z - r.origin.z) * r.inv_dir.z;
+		lo = tmax(lo, tmin(a, b));
+		hi = tmin(hi, tmax(a, b));
+	}
+	return hi >= lo && r.len >= lo;
+}
+
+// ---------------------------------------------------------------------------
+// BoxTree construction (lighter_math.cpp:674-781): median split of the
+// "splittable" items (volume * 3 < node volume) along the axis of their
+// largest extent; the others stay on the node.  Leaves at <= 4 items, depth
+// 16, or fewer than 4 splittable items.
+// ---------------------------------------------------------------------------
+static constexpr size_t kMinSplit = 4;
+static constexpr int kMaxDepth = 16;
+static constexpr float kSizeFactor = 3.0f;
+
+void BoxTree::build(const Box* boxes, size_t count)
+{
+	nodes_.clear();
+	items_.clear();
+	nodes_.push_back(Node{mk3(FLT_MAX, FLT_MAX, FLT_MAX), mk3(-FLT_MAX, -FLT_MAX, -FLT_MAX), -1, -1});
+	if (count == 0)
+		return;
+	std::vector<int32_t> idx;
+	for (size_t i = 0; i < count; ++i)
+		if (boxes[i].valid())
+			idx.push_back((int32_t)i);
+	make_node(0, boxes, idx.data(), idx.size(), 0);
+}
+
+void BoxTree::make_node(int32_t node, const Box* boxes, int32_t* idx, size_t count, int depth)
+{
+	f3 bmn = mk3(FLT_MAX, FLT_MAX, FLT_MAX), bmx = mk3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+	for (size_t i = 0; i < count; ++i) {
+		bmn = vmin(bmn, boxes[idx[i]].mn);
+		bmx = vmax(bmx, boxes[idx[i]].mx);
+	}
+	nodes_[node].mn = bmn;
+	nodes_[node].mx = bmx;
+	float vol = Box{bmn, bmx}.volume();
+
+	bool leaf = true;
+	if (count > kMinSplit && depth < kMaxDepth) {
+		nodes_[node].ido = -1;
+		nodes_[node].ch = -1;
+		int splittable = 0;
+		f3 smn = mk3(FLT_MAX, FLT_MAX, FLT_MAX), smx = mk3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+		for (size_t i = 0; i < count; ++i) {
+			const Box& b = boxes[idx[i]];
+			if (b.volume() * kSizeFactor < vol) {
+				splittable++;
+				smn = vmin(smn, b.mn);
+				smx = vmax(smx, b.mx);
+			}
+		}
+		f3 ssz = smx - smn;
+		f3 axis = mk3(0, 0, 1);
+		if (ssz.x > ssz.y && ssz.x > ssz.z)
+			axis = mk3(1, 0, 0);
+		else if (ssz.y > ssz.x && ssz.y > ssz.z)
+			axis = mk3(0, 1, 0);
+		if (splittable >= (int)kMinSplit) {
+			leaf = false;
+			std::vector<int32_t> keep, sub;
+			for (size_t i = 0; i < count; ++i) {
+				if (boxes[idx[i]].volume() * kSizeFactor < vol)
+					sub.push_back(idx[i]);
+				else
+					keep.push_back(idx[i]);
+			}
+			if (!keep.empty()) {
+				nodes_[node].ido = (int32_t)items_.size();
+				items_.push_back((int32_t)keep.size());
+				items_.insert(items_.end(), keep.begin(), keep.end());
+			}
+			std::sort(sub.begin(), sub.end(), [&](int32_t a, int32_t b) {
+				return dot(axis, boxes[a].center()) < dot(axis, boxes[b].center());
+			});
+			size_t mid = sub.size() / 2;
+			nodes_.push_back(Node{});
+			make_node((int32_t)nodes_.size() - 1, boxes, sub.data(), mid, depth + 1);
+			nodes_[node].ch = (int32_t)nodes_.size();
+			nodes_.push_back(Node{});
+			make_node((int32_t)nodes_.size() - 1, boxes, sub.data() + mid, sub.size() - mid, depth + 1);
+		}
+	}
+	if (leaf) {
+		nodes_[node].ido = (int32_t)items_.size();
+		nodes_[node].ch = -1;
+		items_.push_back((int32_t)count);
+		items_.insert(items_.end(), idx, idx + count);
+	}
+}
+
+// ---------------------------------------------------------------------------
+void TriSet::set(const Tri* tris, size_t count)
+{
+	tris_.clear();
+	std::vector<Box> boxes;
+	for (size_t i = 0; i < count; ++i) {
+		if (tri_useful(tris[i])) {
+			boxes.push_back(tri_box(tris[i]));
+			tris_.push_back(tris[i]);
+		}
+	}
+	tree_.build(boxes.data(), boxes.size());
+}
+
+float TriSet::ray_dist(f3 from, f3 to, int32_t* outtid) const
+{
+	RaySeg r;
+	r.set(from, to);
+	float closest = 2.0f;
+	int32_t hit = -1;
+	auto fn = [&](const int32_t* ids, int32_t n) {
+		for (int32_t i = 0; i < n; ++i) {
+			const Tri& T = tris_[ids[i]];
+			float d = seg_tri(r.origin, r.end, T.p1, T.p2, T.p3);
+			if (d < closest) {
+				closest = d;
+				hit = ids[i];
+			}
+		}
+		return true;
+	};
+	tree_.ray_query(r, fn);
+	if (hit != -1 && outtid)
+		*outtid = hit;
+	return closest;
+}
+
+// PointProjOnTriangle (lighter_math.cpp:915-952): false when the point is
+// clearly outside the triangle's prism (beyond SMALL_FLOAT).
+static bool point_proj_on_tri(f3 pt, f3 t0, f3 t1, f3 t2)
+{
+	f3 nrm = normalized(cross(t1 - t0, t2 - t0));
+	f3 tan0 = normalized(t1 - t0), tan1 = normalized(t2 - t1), tan2 = normalized(t0 - t2);
+	float t0p = dot(tan0, pt), t0min = dot(tan0, t0), t0max = dot(tan0, t1);
+	float t1p = dot(tan1, pt), t1min = dot(tan1, t1), t1max = dot(tan1, t2);
+	float t2p = dot(tan2, pt), t2min = dot(tan2, t2), t2max = dot(tan2, t0);
+	if (t0min - kSmall > t0p && t2max + kSmall < t2p) return false;
+	if (t1min - kSmall > t1p && t0max + kSmall < t0p) return false;
+	if (t2min - kSmall > t2p && t1max + kSmall < t1p) return false;
+	f3 en0 = normalized(cross(t1 - t0, nrm));
+	f3 en1 = normalized(cross(t2 - t1, nrm));
+	f3 en2 = normalized(cross(t0 - t2, nrm));
+	float ptd0 = dot(en0, pt) - dot(en0, t0);
+	float ptd1 = dot(en1, pt) - dot(en1, t1);
+	float ptd2 = dot(en2, pt) - dot(en2, t2);
+	if (ptd0 > kSmall && t0p - kSmall > t0min && t0p + kSmall < t0max) return false;
+	if (ptd1 > kSmall && t1p - kSmall > t1min && t1p + kSmall < t1max) return false;
+	if (ptd2 > kSmall && t2p - kSmall > t2min && t2p + kSmall < t2max) return false;
+	return true;
+}
+
+void TriSet::offset_sample(f3& P, f3 N, float dist) const
+{
+	auto fn = [&](const int32_t* ids, int32_t n) {
+		for (int32_t i = 0; i < n; ++i) {
+			const Tri& T = tris_[ids[i]];
+			float nd = point_tri_dist(P, T.p1, T.p2, T.p3);
+			if (!(nd < dist))
+				continue;
+			f3 tpn = -tri_normal(T);
+			float tpd = dot(tpn, T.p1);
+			float sig = dot(tpn, P) - tpd;
+			if (!(sig <= kSmall))
+				continue;
+			f3 pex = P + normalized(tpn + N) * -sig * 1.41f;
+			if (!point_proj_on_tri(pex, T.p1, T.p2, T.p3))
+				continue;
+			float pdot = dot(tpn, N);
+			if (!(fabsf(pdot) < 0.95f))
+				continue;
+			f3 ptpn = normalized(tpn - N * pdot);
+			float dfac = 1.0f - fabsf(pdot);
+			f3 pnew = P + ptpn * (-sig / dfac + kSmall);
+			int32_t tid = -1;
+			if (ray_dist(P + (N + ptpn) * kSmall, pnew, &tid) >= 0.9f || tid == ids[i])
+				P = pnew;
+		}
+	};
+	f3 d3 = mk3(dist, dist, dist);
+	tree_.query(P - d3, P + d3, fn);
+}
+
+// ---------------------------------------------------------------------------
+// SplitTree (lighter_math.cpp:428-614)
+// ---------------------------------------------------------------------------
+static constexpr size_t kSplitMaxTris = 16;
+static constexpr int kSplitMaxDepth = 32;
+
+void SplitTree::set(const Tri* tris, size_t count)
+{
+	nodes_.clear();
+	nodes_.emplace_back();
+	nodes_[0].tris.assign(tris, tris + count);
+	if (count)
+		split(0, 0);
+}
+
+namespace {
+struct PointKey {
+	uint32_t x, y, z;
+	bool operator==(const PointKey& o) const { return x == o.x && y == o.y && z == o.z; }
+};
+struct PointKeyHash {
+	size_t operator()(const PointKey& k) const
+	{
+		uint64_t h = k.x * 0x9E3779B97F4A7C15ull;
+		h ^= (k.y + 0x632BE59BD9B4E019ull) * 0xC2B2AE3D27D4EB4Full;
+		h ^= (k.z + 0x165667B19E3779F9ull) * 0x85EBCA77C2B2AE63ull;
+		return (size_t)(h ^ (h >> 29));
+	}
+};
+inline uint32_t canon_bits(float f)
+{
+	if (f == 0.0f)
+		f = 0.0f; // +0 and -0 compare equal
+	uint32_t u;
+	memcpy(&u, &f, 4);
+	return u;
+}
+} // namespace
+
+bool SplitTree::pick_plane(Node& node)
+{
+	// unique vertices in first-seen order, equality as float ==
+	std::vector<f3> pts;
+	std::unordered_set<PointKey, PointKeyHash> seen;
+	auto add = [&](f3 p) {
+		if (p.x != p.x || p.y != p.y || p.z != p.z) {
+			pts.push_back(p); // NaN never compares equal
+			return;
+		}
+		PointKey k{canon_bits(p.x), canon_bits(p.y), canon_bits(p.z)};
+		if (seen.insert(k).second)
+			pts.push_back(p);
+	};
+	for (const Tri& t : node.tris) {
+		add(t.p1);
+		add(t.p2);
+		add(t.p3);
+	}
+	f3 center = mk3(0, 0, 0), dir = mk3(0, 0, 0);
+	float best = 0;
+	for (size_t i = 0; i < pts.size(); ++i) {
+		center += pts[i];
+		for (size_t j = i + 1; j < pts.size(); ++j) {
+			f3 nd = pts[j] - pts[i];
+			float l = lensq(nd);
+			if (l > best) {
+				dir = nd;
+				best = l;
+			}
+		}
+	}
+	if (!pts.empty())
+		center = center / (float)pts.size();
+	node.N = normalized(dir);
+	node.D = dot(node.N, center);
+	int na = 0, nb = 0;
+	for (const Tri& t : node.tris) {
+		float d1 = dot(t.p1, node.N) - node.D;
+		float d2 = dot(t.p2, node.N) - node.D;
+		float d3 = dot(t.p3, node.N) - node.D;
+		if (d1 * d2 > -kSmall && d2 * d3 > -kSmall) {
+			if (d1 > 0)
+				na++;
+			else
+				nb++;
+		} else {
+			na += (d1 > 0) + (d2 > 0) + (d3 > 0);
+			nb += (d1 <= 0) + (d2 <= 0) + (d3 <= 0);
+		}
+	}
+	if (na != 0 && nb != 0 && (float)(na + nb) < (float)node.tris.size() * 1.7f) {
+		float q = (float)na / (float)nb;
+		if (q < 1)
+			q = 1 / q;
+		return q < 3.0f;
+	}
+	return false;
+}
+
+void SplitTree::split(int32_t node, int depth)
+{
+	if (!(nodes_[node].tris.size() > kSplitMaxTris && depth < kSplitMaxDepth))
+		return;
+	if (!pick_plane(nodes_[node]))
+		return;
+	Node front, back;
+	const Node& n = nodes_[node];
+	for (const Tri& t : n.tris) {
+		float p1 = dot(n.N, t.p1) - n.D, p2 = dot(n.N, t.p2) - n.D, p3 = dot(n.N, t.p3) - n.D;
+		if (p1 * p2 >= -kSmall && p1 * p3 >= -kSmall) {
+			(p1 + p2 + p3 > 0 ? front : back).tris.push_back(t);
+		} else {
+			front.tris.push_back(t);
+			back.tris.push_back(t);
+		}
+	}
+	if (front.tris.size() + back.tris.size() > n.tris.size() * 3 / 2)
+		return;
+	int32_t fi = (int32_t)nodes_.size();
+	nodes_.push_back(std::move(front));
+	nodes_.push_back(std::move(back));
+	nodes_[node].front = fi;
+	nodes_[node].back = fi + 1;
+	std::vector<Tri>().swap(nodes_[node].tris);
+	split(fi, depth + 1);
+	split(fi + 1, depth + 1);
+}
+
+float SplitTree::node_intersect(int32_t node, f3 from, f3 to, f3* outnormal) const
+{
+	const Node& n = nodes_[node];
+	if (n.front != -1) {
+		float df = dot(from, n.N) - n.D;
+		float dt = dot(to, n.N) - n.D;
+		if (df < 0) {
+			float hit = node_intersect(n.back, from, to, outnormal);
+			if (hit < 1.0f)
+				return hit;
+			return dt > 0 ? node_intersect(n.front, from, to, outnormal) : 2.0f;
+		}
+		float hit = node_intersect(n.front, from, to, outnormal);
+		if (hit < 1.0f)
+			return hit;
+		return dt < 0 ? node_intersect(n.back, from, to, outnormal) : 2.0f;
+	}
+	float closest = 2.0f;
+	const Tri* ct = nullptr;
+	for (const Tri& t : n.tris) {
+		float hit = seg_tri(from, to, t.p1, t.p2, t.p3);
+		if (hit < closest) {
+			closest = hit;
+			ct = &t;
+		}
+	}
+	if (outnormal && ct)
+		*outnormal = normalized(cross(ct->p3 - ct->p1, ct->p2 - ct->p1));
+	return closest;
+}
+
+float SplitTree::intersect(f3 from, f3 to, f3* outnormal) const
+{
+	if (nodes_.empty())
+		return 2.0f;
+	return node_intersect(0, from, to, outnormal);
+}
+
+// ---------------------------------------------------------------------------
+void rasterize_triangle(f3* img1, f3* img2, f4* img3, int32_t width, int32_t height, float margin, f2 p1,
+                        f2 p2, f2 p3, f3 va1, f3 va2, f3 va3, f3 vb1, f3 vb2, f3 vb3, f4 vc1, f4 vc2, f4 vc3)
+{
+	int32_t maxX = (int32_t)(tmax(p1.x, tmax(p2.x, p3.x)) + margin);
+	int32_t minX = (int32_t)(tmin(p1.x, tmin(p2.x, p3.x)) - margin);
+	int32_t maxY = (int32_t)(tmax(p1.y, tmax(p2.y, p3.y)) + margin);
+	int32_t minY = (int32_t)(tmin(p1.y, tmin(p2.y, p3.y)) - margin);
+	if (maxX < 0 || minX >= width || maxY < 0 || minY >= height)
+		return;
+	if (minX < 0) minX = 0;
+	if (maxX >= width) maxX = width - 1;
+	if (minY < 0) minY = 0;
+	if (maxY >= height) maxY = height - 1;
+
+	f2 e12 = p2 - p1, e13 = p3 - p1;
+	float vc = cross2(e12, e13);
+	if (vc == 0)
+		return;
+	f2 n1 = normalized2(perp2(e12));
+	f2 n2 = normalized2(perp2(p3 - p2));
+	f2 n3 = -normalized2(perp2(e13));
+	float d1 = dot2(n1, p1), d2 = dot2(n2, p2), d3 = dot2(n3, p3);
+	f3 a12 = va2 - va1, a13 = va3 - va1;
+	f3 b12 = vb2 - vb1, b13 = vb3 - vb1;
+	f4 c12 = vc2 - vc1, c13 = vc3 - vc1;
+	for (int32_t x = minX; x <= maxX; x++) {
+		for (int32_t y = minY; y <= maxY; y++) {
+			f2 q = mk2(x - p1.x, y - p1.y);
+			float s = cross2(q, e13) / vc;
+			float t = cross2(e12, q) / vc;
+			s = tmax(tmin(s, 1.0f), 0.0f);
+			t = tmax(tmin(t, 1.0f), 0.0f);
+			f2 p = mk2((float)x, (float)y);
+			float pd1 = dot2(p, n1), pd2 = dot2(p, n2), pd3 = dot2(p, n3);
+			if ((pd1 <= d1 + margin && pd2 <= d2 + margin && pd3 <= d3 + margin) ||
+			    (pd1 + margin >= d1 && pd2 + margin >= d2 && pd3 + margin >= d3)) {
+				size_t o = (size_t)x + (size_t)width * (size_t)y;
+				img1[o] = va1 + a12 * s + a13 * t;
+				img2[o] = vb1 + b12 * s + b13 * t;
+				img3[o] = vc1 + c12 * s + c13 * t;
+			}
+		}
+	}
+}
+
+} // namespace lgx

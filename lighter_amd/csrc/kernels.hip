@@ -1,0 +1,1217 @@
+// gfx950 kernels of the baking hot path and the DeviceScene that owns their
+// device-resident data.  See device.h for the stage contracts.
+//
+// Query semantics (all order independent, hence equal to the reference's
+// tree walks):
+//  * distance(p): min(2, min over shadow triangles of PointTriangleDistance)
+//    (lighter.cpp:150-188);
+//  * closest(A,B): min over triangles of the segment hit parameter on A->B
+//    shrunk by 0.001 at both ends, 2 when nothing is hit (lighter.cpp:254-289);
+//  * occluded(A,B): some triangle hit with r < 1 on the same shrunk segment
+//    (lighter.cpp:112-147).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bvh.h"
+#include "device.h"
+#include "geom.h"
+
+namespace lgx {
+
+#define HIPCHECK(x)                                                                                          \
+	do {                                                                                                     \
+		hipError_t e_ = (x);                                                                                 \
+		if (e_ != hipSuccess)                                                                                \
+			throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " __FILE__ ":" + \
+			                         std::to_string(__LINE__) + ": " #x);                                    \
+	} while (0)
+
+constexpr int kStackSize = 64;
+constexpr int kBlock = 256;
+constexpr uint32_t kSkipBits = 0xFFFFFFFFu; // weight of a texel the reference skips
+constexpr uint32_t kSampleBits = 26;
+constexpr uint32_t kSampleMask = (1u << kSampleBits) - 1;
+
+struct View {
+	const BvhNode* __restrict__ nodes;
+	const DistTri* __restrict__ dtri;
+	const RayTri* __restrict__ rtri;
+	int has_tris;
+	// scene box grown by the distance-query radius plus a safety margin:
+	// outside it every distance query returns exactly 2
+	float xlo[3], xhi[3];
+};
+
+// ---------------------------------------------------------------------------
+// transcendental helpers: evaluated in double and rounded once, which is the
+// correctly rounded value except in vanishingly rare ties
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pow_r(float x, float y)
+{
+	if (y == 1.0f)
+		return x;
+	return (float)pow((double)x, (double)y);
+}
+__device__ __forceinline__ float acos_r(float x) { return (float)acos((double)x); }
+
+struct Node4 {
+	float4 a, b, c;
+	int4 d;
+};
+
+__device__ __forceinline__ Node4 load_node(const BvhNode* nodes, int32_t i)
+{
+	const float4* q = reinterpret_cast<const float4*>(nodes + i);
+	Node4 n;
+	n.a = q[0];
+	n.b = q[1];
+	n.c = q[2];
+	n.d = reinterpret_cast<const int4*>(q)[3];
+	return n;
+}
+
+__device__ __forceinline__ float box_d2(f3 p, float lx, float ly, float lz, float hx, float hy, float hz)
+{
+	float dx = fmaxf(fmaxf(lx - p.x, p.x - hx), 0.0f);
+	float dy = fmaxf(fmaxf(ly - p.y, p.y - hy), 0.0f);
+	float dz = fmaxf(fmaxf(lz - p.z, p.z - hz), 0.0f);
+	return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ DistTri load_dtri(const DistTri* t, uint32_t i)
+{
+	const float4* q = reinterpret_cast<const float4*>(t + i);
+	DistTri r;
+	float4* o = reinterpret_cast<float4*>(&r);
+#pragma unroll
+	for (int k = 0; k < 10; ++k)
+		o[k] = q[k];
+	return r;
+}
+
+__device__ __forceinline__ RayTri load_rtri(const RayTri* t, uint32_t i)
+{
+	const float4* q = reinterpret_cast<const float4*>(t + i);
+	RayTri r;
+	float4* o = reinterpret_cast<float4*>(&r);
+#pragma unroll
+	for (int k = 0; k < 4; ++k)
+		o[k] = q[k];
+	return r;
+}
+
+// Nearest-first traversal for the distance query.
+template <bool COUNT>
+__device__ float q_distance(const View& V, f3 p, unsigned long long* cnt)
+{
+	float best = kPenumbraSize;
+	if (!V.has_tris)
+		return best;
+	int32_t sref[kStackSize];
+	float sd[kStackSize];
+	int sp = 0;
+	int32_t ref = 0;
+	unsigned nn = 0, nt = 0;
+	for (;;) {
+		if (ref >= 0) {
+			Node4 n = load_node(V.nodes, ref);
+			nn++;
+			float d0 = box_d2(p, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
+			float d1 = box_d2(p, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
+			int32_t r0 = n.d.x, r1 = n.d.y;
+			if (d1 < d0) {
+				float t = d0; d0 = d1; d1 = t;
+				int32_t u = r0; r0 = r1; r1 = u;
+			}
+			float lim = best * best;
+			if (d0 <= lim) {
+				if (d1 <= lim) {
+					sref[sp] = r1;
+					sd[sp] = d1;
+					sp++;
+				}
+				ref = r0;
+				continue;
+			}
+		} else {
+			uint32_t f = leaf_first(ref), c = leaf_count(ref);
+			for (uint32_t i = 0; i < c; ++i) {
+				DistTri T = load_dtri(V.dtri, f + i);
+				float d = dist_tri_pre(p, T);
+				if (d < best)
+					best = d;
+			}
+			nt += c;
+		}
+		// pop the next entry still within reach
+		bool found = false;
+		while (sp > 0) {
+			--sp;
+			if (sd[sp] <= best * best) {
+				ref = sref[sp];
+				found = true;
+				break;
+			}
+		}
+		if (!found)
+			break;
+	}
+	if (COUNT) {
+		atomicAdd(cnt + 0, (unsigned long long)nn);
+		atomicAdd(cnt + 1, (unsigned long long)nt);
+		atomicAdd(cnt + 3, 1ull);
+	}
+	return best;
+}
+
+struct Seg {
+	f3 o, dir, inv;
+};
+
+__device__ __forceinline__ Seg make_seg(f3 A, f3 B)
+{
+	f3 dn = normalized(B - A);
+	f3 mA = A + dn * kSmall;
+	f3 mB = B - dn * kSmall;
+	Seg s;
+	s.o = mA;
+	s.dir = mB - mA;
+	float ex = fabsf(s.dir.x) > 1e-30f ? s.dir.x : copysignf(1e-30f, s.dir.x);
+	float ey = fabsf(s.dir.y) > 1e-30f ? s.dir.y : copysignf(1e-30f, s.dir.y);
+	float ez = fabsf(s.dir.z) > 1e-30f ? s.dir.z : copysignf(1e-30f, s.dir.z);
+	s.inv = mk3(1.0f / ex, 1.0f / ey, 1.0f / ez);
+	return s;
+}
+
+// entry parameter of the segment into a box, or +inf when [0, tlim] misses it
+__device__ __forceinline__ float slab(const Seg& s, float lx, float ly, float lz, float hx, float hy, float hz,
+                                      float tlim)
+{
+	float tx0 = (lx - s.o.x) * s.inv.x, tx1 = (hx - s.o.x) * s.inv.x;
+	float ty0 = (ly - s.o.y) * s.inv.y, ty1 = (hy - s.o.y) * s.inv.y;
+	float tz0 = (lz - s.o.z) * s.inv.z, tz1 = (hz - s.o.z) * s.inv.z;
+	float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+	float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tlim));
+	return tn <= tf ? tn : INFINITY;
+}
+
+// ANY = true: returns 0 on the first hit with r < 1 (occlusion), else 2.
+template <bool ANY, bool COUNT>
+__device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt)
+{
+	float best = 2.0f;
+	if (!V.has_tris)
+		return best;
+	Seg s = make_seg(A, B);
+	int32_t sref[kStackSize];
+	float sd[kStackSize];
+	int sp = 0;
+	int32_t ref = 0;
+	unsigned nn = 0, nt = 0;
+	for (;;) {
+		if (ref >= 0) {
+			Node4 n = load_node(V.nodes, ref);
+			nn++;
+			float tl = fminf(best, 1.0f);
+			float t0 = slab(s, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tl);
+			float t1 = slab(s, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tl);
+			int32_t r0 = n.d.x, r1 = n.d.y;
+			if (t1 < t0) {
+				float t = t0; t0 = t1; t1 = t;
+				int32_t u = r0; r0 = r1; r1 = u;
+			}
+			if (t0 != INFINITY) {
+				if (t1 != INFINITY) {
+					sref[sp] = r1;
+					sd[sp] = t1;
+					sp++;
+				}
+				ref = r0;
+				continue;
+			}
+		} else {
+			uint32_t f = leaf_first(ref), c = leaf_count(ref);
+			nt += c;
+			bool done = false;
+			for (uint32_t i = 0; i < c; ++i) {
+				RayTri T = load_rtri(V.rtri, f + i);
+				float r = seg_tri_pre(s.o, s.dir, T);
+				if (r < best)
+					best = r;
+				if (ANY && best < 1.0f) {
+					done = true;
+					break;
+				}
+			}
+			if (done)
+				break;
+		}
+		bool found = false;
+		while (sp > 0) {
+			--sp;
+			if (sd[sp] <= fminf(best, 1.0f)) {
+				ref = sref[sp];
+				found = true;
+				break;
+			}
+		}
+		if (!found)
+			break;
+	}
+	if (COUNT) {
+		atomicAdd(cnt + 0, (unsigned long long)nn);
+		atomicAdd(cnt + 1, (unsigned long long)nt);
+		atomicAdd(cnt + 3, 1ull);
+	}
+	return best;
+}
+
+// Exit parameter of the ray from + rd*t from the grown scene box, padded so
+// that points beyond it are certainly outside; -inf if the ray misses it.
+__device__ __forceinline__ float box_exit(const View& V, f3 from, f3 rd)
+{
+	float lo[3] = {V.xlo[0], V.xlo[1], V.xlo[2]}, hi[3] = {V.xhi[0], V.xhi[1], V.xhi[2]};
+	float o[3] = {from.x, from.y, from.z}, d[3] = {rd.x, rd.y, rd.z};
+	float tn = -INFINITY, tf = INFINITY;
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		if (d[a] != 0.0f) {
+			float t0 = (lo[a] - o[a]) / d[a], t1 = (hi[a] - o[a]) / d[a];
+			tn = fmaxf(tn, fminf(t0, t1));
+			tf = fminf(tf, fmaxf(t0, t1));
+		} else if (o[a] < lo[a] || o[a] > hi[a]) {
+			return -INFINITY;
+		}
+	}
+	if (tn > tf)
+		return -INFINITY;
+	return tf * 1.0001f + 0.01f;
+}
+
+// CalcInvShadowFactor (lighter.cpp:190-207).  Once the march has left the
+// grown scene box every remaining step sees h = 2 and, for k >= 0, cannot
+// lower res (2 / min(t*k, 2) >= 1 >= res), so the loop stops there with the
+// value the full loop would return.
+template <bool COUNT>
+__device__ float shadow_march(const View& V, f3 from, f3 to, float k, unsigned long long* cnt, uint32_t* nsteps)
+{
+	f3 rd = normalized(to - from);
+	float mint = 0.001f;
+	float maxt = length(to - from);
+	float res = 1.0f;
+	float t_out = (k >= 0.0f) ? box_exit(V, from, rd) : INFINITY;
+	uint32_t steps = 0;
+	for (float t = mint; t < maxt;) {
+		if (t > t_out)
+			break;
+		float h = q_distance<COUNT>(V, from + rd * t, cnt);
+		steps++;
+		if (h < 0.001f) {
+			res = 0.0f;
+			break;
+		}
+		res = tmin(res, h / tmin(t * k, kPenumbraSize));
+		h = tmin(h, kPenumbraStep);
+		t += h;
+	}
+	if (COUNT)
+		atomicAdd(cnt + 2, (unsigned long long)steps);
+	if (nsteps)
+		*nsteps = steps;
+	return res;
+}
+
+// ---------------------------------------------------------------------------
+// direct light
+// ---------------------------------------------------------------------------
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_march(View V, const float* __restrict__ pos,
+                                                  const float* __restrict__ nrm, const DevLight* __restrict__ lights,
+                                                  const uint32_t* __restrict__ items, uint32_t nitems,
+                                                  float* __restrict__ wout, float* __restrict__ vout,
+                                                  unsigned long long* cnt)
+{
+	uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= nitems)
+		return;
+	uint32_t code = items[k];
+	const DevLight& L = lights[code >> kSampleBits];
+	uint32_t s = code & kSampleMask;
+	f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
+	f3 lpos = ld3(L.pos), ldir = ld3(L.dir);
+	float w, fv = 1.0f;
+	if (L.type == 3) {
+		// Job_LMRender_Direct_Inner (lighter.cpp:586-589)
+		float fn = tmax(0.0f, dot(ldir, SN));
+		if (fn > 0) {
+			fv = shadow_march<COUNT>(V, SP + SN * kShadowOffset, SP + ldir * L.range, L.radius, cnt, nullptr);
+			w = fn * fv;
+		} else {
+			w = fn; // +-0 times a finite non-negative factor
+		}
+	} else if (L.type == 1 || L.type == 2) {
+		// Job_LMRender_Point_Inner / Spot_Inner (lighter.cpp:485-553)
+		f3 s2l = lpos - SP;
+		float dist = length(s2l);
+		if (dist)
+			s2l = s2l / dist;
+		float fd = pow_r(1 - tmin(1.0f, dist / L.range), L.power);
+		float fn = tmax(0.0f, dot(s2l, SN));
+		float fdir = 1.0f, gate;
+		if (L.type == 2) {
+			float ang = acos_r(tmin(1.0f, dot(s2l, -ldir)));
+			fdir = tmax(0.0f, tmin(1.0f, (ang - L.spot_out_rad) / L.spot_diff));
+			fdir = pow_r(fdir, L.spot_curve);
+			gate = fd * fn * fdir;
+		} else {
+			gate = fd * fn;
+		}
+		if (gate <= 0) {
+			w = __uint_as_float(kSkipBits);
+		} else {
+			fv = shadow_march<COUNT>(V, SP + SN * kShadowOffset, lpos, L.radius, cnt, nullptr);
+			w = (L.type == 2) ? fd * fn * fdir * fv : fd * fn * fv;
+		}
+	} else {
+		w = __uint_as_float(kSkipBits);
+	}
+	wout[k] = w;
+	if (vout)
+		vout[k] = fv;
+}
+
+// lightmap[s] += color * w for the items of one light (no sample repeats
+// within a light, so no two lanes touch the same texel).
+__global__ __launch_bounds__(kBlock) void k_accumulate(const uint32_t* __restrict__ items,
+                                                       const float* __restrict__ w, uint32_t begin, uint32_t end,
+                                                       float cr, float cg, float cb, float* __restrict__ lm)
+{
+	uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= end)
+		return;
+	float wk = w[k];
+	if (__float_as_uint(wk) == kSkipBits)
+		return;
+	uint32_t s = items[k] & kSampleMask;
+	f3 c = mk3(cr, cg, cb);
+	f3 v = ld3(lm + 3 * s) + c * wk;
+	st3(lm + 3 * s, v);
+}
+
+// ---------------------------------------------------------------------------
+// ambient occlusion (lighter.cpp:799-841)
+// ---------------------------------------------------------------------------
+struct AOParams {
+	int n;
+	float distance, multiplier, falloff, effect;
+	float color[3];
+};
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_ao(View V, const float* __restrict__ pos, const float* __restrict__ nrm,
+                                               const float* __restrict__ randoff, const float2* __restrict__ side,
+                                               AOParams P, uint32_t S, float* __restrict__ lm,
+                                               unsigned long long* cnt)
+{
+	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= S)
+		return;
+	f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
+	f3 origin = SP + SN * (kSmall * 2);
+	float ro = randoff[s];
+	float ao = 0;
+	for (int i = 0; i < P.n; ++i) {
+		float angle = (i + ro) * kGoldenAngle;
+		double sa, ca;
+		sincos((double)angle, &sa, &ca);
+		float2 sd = side[i];
+		f3 d = spiral_dir(SN, sd.x, sd.y, (float)ca, (float)sa) * P.distance;
+		float hit = q_segment<false, COUNT>(V, origin, origin + d, cnt);
+		if (hit < 1.0f)
+			ao += 1.0f - hit;
+	}
+	ao /= P.n;
+	ao = tmin(ao * P.multiplier, 1.0f);
+	if (P.falloff)
+		ao = pow_r(ao, P.falloff);
+	f3 out = ld3(lm + 3 * s);
+	f3 col = mk3(P.color[0], P.color[1], P.color[2]);
+	if (P.effect >= 0) {
+		out = out * (1 - ao * (1 - P.effect)) + col * ao;
+	} else {
+		out = lerp3(out, lerp3(col, col * out, -P.effect), ao);
+	}
+	st3(lm + 3 * s, out);
+}
+
+// ---------------------------------------------------------------------------
+// point-query kernels (tests, timing)
+// ---------------------------------------------------------------------------
+__global__ void k_q_distance(View V, const float* p, uint32_t n, float* out)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		out[i] = q_distance<false>(V, ld3(p + 3 * i), nullptr);
+}
+
+__global__ void k_q_shadow(View V, const float* a, const float* b, const float* k, uint32_t n, float* out,
+                           uint32_t* steps)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) {
+		uint32_t st = 0;
+		out[i] = shadow_march<false>(V, ld3(a + 3 * i), ld3(b + 3 * i), k[i], nullptr, &st);
+		steps[i] = st;
+	}
+}
+
+__global__ void k_q_closest(View V, const float* a, const float* b, uint32_t n, float* out)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		out[i] = q_segment<false, false>(V, ld3(a + 3 * i), ld3(b + 3 * i), nullptr);
+}
+
+__global__ void k_q_occluded(View V, const float* a, const float* b, uint32_t n, uint8_t* out)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		out[i] = q_segment<true, false>(V, ld3(a + 3 * i), ld3(b + 3 * i), nullptr) < 1.0f ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// radiosity links (lighter.cpp:728-759)
+// ---------------------------------------------------------------------------
+// Form factor of the pair (i < j) exactly as the reference evaluates it, or
+// -1 when the pair fails the geometric thresholds.
+__device__ __forceinline__ float pair_factor(f3 ap, f3 an, f3 bp, f3 bn)
+{
+	f3 d = bp - ap;
+	float da = dot(an, d);
+	float db = dot(bn, -d);
+	if (da <= kSmall || db <= kSmall)
+		return -1.0f;
+	float l2 = lensq(d);
+	float f = da * db / (l2 * l2 * kPiF);
+	if (f < kSmall)
+		return -1.0f;
+	return f;
+}
+
+constexpr int kPairChunk = 256;
+
+// Pass 0 counts, pass 1 writes: candidates of row i in columns
+// [c*kPairChunk, +kPairChunk) restricted to j > i, in ascending j.
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void k_pairs(const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                  uint32_t N, uint32_t nchunks, uint32_t* __restrict__ counts,
+                                                  const uint64_t* __restrict__ offsets, uint2* __restrict__ cand)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t c = blockIdx.y;
+	if (i >= N)
+		return;
+	uint32_t j0 = c * kPairChunk, j1 = min(N, j0 + kPairChunk);
+	if (j0 <= i)
+		j0 = i + 1;
+	uint32_t n = 0;
+	uint64_t base = WRITE ? offsets[(uint64_t)i * nchunks + c] : 0;
+	if (j0 < j1) {
+		f3 ap = ld3(pos + 3 * i), an = ld3(nrm + 3 * i);
+		for (uint32_t j = j0; j < j1; ++j) {
+			float f = pair_factor(ap, an, ld3(pos + 3 * j), ld3(nrm + 3 * j));
+			if (f >= 0.0f) {
+				if (WRITE)
+					cand[base + n] = make_uint2(i, j);
+				n++;
+			}
+		}
+	}
+	if (!WRITE)
+		counts[(uint64_t)i * nchunks + c] = n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_link_vis(View V, const float* __restrict__ pos,
+                                                     const uint2* __restrict__ cand, uint64_t ncand,
+                                                     uint32_t* __restrict__ bits, uint64_t words)
+{
+	uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= ncand)
+		return;
+	uint2 c = cand[k];
+	if (q_segment<true, false>(V, ld3(pos + 3 * c.x), ld3(pos + 3 * c.y), nullptr) < 1.0f) {
+		atomicOr(bits + (uint64_t)c.x * words + (c.y >> 5), 1u << (c.y & 31));
+		atomicOr(bits + (uint64_t)c.y * words + (c.x >> 5), 1u << (c.x & 31));
+	}
+}
+
+__global__ void k_row_popc(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N, uint32_t* __restrict__ cnt)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= N)
+		return;
+	uint32_t n = 0;
+	const uint32_t* row = bits + (uint64_t)i * words;
+	for (uint64_t w = 0; w < words; ++w)
+		n += __popc(row[w]);
+	cnt[i] = n;
+}
+
+__global__ void k_row_fill(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N,
+                           const uint32_t* __restrict__ rowptr, const float* __restrict__ pos,
+                           const float* __restrict__ nrm, uint32_t* __restrict__ col, float* __restrict__ val)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= N)
+		return;
+	const uint32_t* row = bits + (uint64_t)i * words;
+	uint32_t o = rowptr[i];
+	f3 ip = ld3(pos + 3 * i), in_ = ld3(nrm + 3 * i);
+	for (uint64_t w = 0; w < words; ++w) {
+		uint32_t m = row[w];
+		while (m) {
+			uint32_t b = __ffs(m) - 1;
+			m &= m - 1;
+			uint32_t j = (uint32_t)(w * 32 + b);
+			f3 jp = ld3(pos + 3 * j), jn = ld3(nrm + 3 * j);
+			// the reference evaluates each pair with the lower index as A
+			float f = (i < j) ? pair_factor(ip, in_, jp, jn) : pair_factor(jp, jn, ip, in_);
+			col[o] = j;
+			val[o] = f;
+			o++;
+		}
+	}
+}
+
+// One bounce of Int_RDBounce (lighter.cpp:767-797) as a gather over the
+// symmetric links: in_i = sum_j ((out_j * diffuse_j) * area_j) * f_ij.
+__global__ __launch_bounds__(kBlock) void k_bounce3(uint32_t N, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ col, const float* __restrict__ val,
+                                                    const float* __restrict__ diffuse, const float* __restrict__ area,
+                                                    const float* __restrict__ out, float* __restrict__ in,
+                                                    float* __restrict__ total)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= N)
+		return;
+	f3 acc = mk3(0, 0, 0);
+	for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+		uint32_t j = col[k];
+		acc += ((ld3(out + 3 * j) * ld3(diffuse + 3 * j)) * area[j]) * val[k];
+	}
+	st3(in + 3 * i, acc);
+	st3(total + 3 * i, ld3(total + 3 * i) + acc);
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of u32 counts into u64 offsets (three-phase)
+// ---------------------------------------------------------------------------
+__global__ void k_scan_blocks(const uint32_t* __restrict__ in, uint64_t n, uint64_t* __restrict__ out,
+                              uint64_t* __restrict__ block_sums)
+{
+	__shared__ uint64_t sh[kBlock];
+	constexpr int kPer = 16;
+	uint64_t base = (uint64_t)blockIdx.x * kBlock * kPer + (uint64_t)threadIdx.x * kPer;
+	uint64_t local[kPer];
+	uint64_t s = 0;
+	for (int k = 0; k < kPer; ++k) {
+		uint64_t idx = base + k;
+		local[k] = s;
+		s += (idx < n) ? in[idx] : 0;
+	}
+	sh[threadIdx.x] = s;
+	__syncthreads();
+	for (int off = 1; off < kBlock; off <<= 1) {
+		uint64_t v = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0;
+		__syncthreads();
+		sh[threadIdx.x] += v;
+		__syncthreads();
+	}
+	uint64_t excl = sh[threadIdx.x] - s;
+	for (int k = 0; k < kPer; ++k) {
+		uint64_t idx = base + k;
+		if (idx < n)
+			out[idx] = excl + local[k];
+	}
+	if (threadIdx.x == kBlock - 1)
+		block_sums[blockIdx.x] = sh[threadIdx.x];
+}
+
+__global__ void k_scan_sums(uint64_t* sums, uint32_t nb, uint64_t* total)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		uint64_t s = 0;
+		for (uint32_t b = 0; b < nb; ++b) {
+			uint64_t v = sums[b];
+			sums[b] = s;
+			s += v;
+		}
+		*total = s;
+	}
+}
+
+__global__ void k_scan_add(uint64_t* out, uint64_t n, const uint64_t* sums)
+{
+	constexpr int kPer = 16;
+	uint64_t base = (uint64_t)blockIdx.x * kBlock * kPer + (uint64_t)threadIdx.x * kPer;
+	uint64_t add = sums[blockIdx.x];
+	for (int k = 0; k < kPer; ++k)
+		if (base + k < n)
+			out[base + k] += add;
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+static inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+void require_gpu()
+{
+	int n = 0;
+	hipError_t e = hipGetDeviceCount(&n);
+	if (e != hipSuccess || n == 0)
+		throw std::runtime_error("liblighter_amd: no HIP device available (the bake stages run only on the GPU)");
+}
+
+std::string device_name()
+{
+	hipDeviceProp_t p;
+	if (hipGetDeviceProperties(&p, 0) != hipSuccess)
+		return "unknown";
+	return std::string(p.name) + " " + p.gcnArchName;
+}
+
+template <class T> static T* dalloc(size_t n)
+{
+	void* p = nullptr;
+	if (n == 0)
+		n = 1;
+	HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+	return (T*)p;
+}
+
+template <class T> static void dfree(T*& p)
+{
+	if (p)
+		(void)hipFree((void*)p);
+	p = nullptr;
+}
+
+DeviceScene::DeviceScene()
+{
+	require_gpu();
+	hipStream_t s;
+	HIPCHECK(hipStreamCreate(&s));
+	stream_ = s;
+	d_counters_ = dalloc<unsigned long long>(4);
+}
+
+DeviceScene::~DeviceScene()
+{
+	free_all();
+	dfree(d_counters_);
+	if (stream_)
+		(void)hipStreamDestroy((hipStream_t)stream_);
+}
+
+void DeviceScene::free_all()
+{
+	dfree(d_nodes_);
+	dfree(d_dtri_);
+	dfree(d_rtri_);
+	dfree(d_pos_);
+	dfree(d_nrm_);
+	dfree(d_lm_);
+	dfree(d_w_);
+	dfree(d_vis_);
+	dfree(rad.pos);
+	dfree(rad.nrm);
+	dfree(rad.bits);
+	dfree(rad.rowptr);
+	dfree(rad.col);
+	dfree(rad.val);
+	dfree(rad.diffuse);
+	dfree(rad.area);
+	dfree(rad.total);
+	dfree(rad.out);
+	dfree(rad.in);
+}
+
+static View make_view(void* nodes, void* dtri, void* rtri, size_t ntri, const float* lo, const float* hi)
+{
+	View v;
+	v.nodes = (const BvhNode*)nodes;
+	v.dtri = (const DistTri*)dtri;
+	v.rtri = (const RayTri*)rtri;
+	v.has_tris = ntri > 0;
+	for (int a = 0; a < 3; ++a) {
+		float m = kPenumbraSize * 1.001f + 0.01f + 1e-4f * fmaxf(fabsf(lo[a]), fabsf(hi[a]));
+		v.xlo[a] = lo[a] - m;
+		v.xhi[a] = hi[a] + m;
+	}
+	return v;
+}
+
+void DeviceScene::set_triangles(const float* tris, size_t ntri)
+{
+	dfree(d_nodes_);
+	dfree(d_dtri_);
+	dfree(d_rtri_);
+	ntri_ = ntri;
+	if (ntri == 0) {
+		nnodes_ = 0;
+		return;
+	}
+	if (ntri >= (1u << 27))
+		throw std::runtime_error("too many triangles for the leaf encoding");
+	BvhBuild b = build_bvh(tris, ntri);
+	if (b.depth + 2 >= kStackSize)
+		throw std::runtime_error("BVH deeper than the traversal stack");
+	bvh_depth_ = b.depth;
+	nnodes_ = b.nodes.size();
+	std::vector<DistTri> dt(ntri);
+	std::vector<RayTri> rt(ntri);
+	for (size_t i = 0; i < ntri; ++i) {
+		const float* t = tris + 9 * (size_t)b.order[i];
+		f3 a = ld3(t), c = ld3(t + 3), e = ld3(t + 6);
+		dt[i] = make_dist_tri(a, c, e);
+		rt[i] = make_ray_tri(a, c, e);
+	}
+	d_nodes_ = dalloc<BvhNode>(nnodes_);
+	d_dtri_ = dalloc<DistTri>(ntri);
+	d_rtri_ = dalloc<RayTri>(ntri);
+	hipStream_t s = (hipStream_t)stream_;
+	HIPCHECK(hipMemcpyAsync(d_nodes_, b.nodes.data(), nnodes_ * sizeof(BvhNode), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_dtri_, dt.data(), ntri * sizeof(DistTri), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_rtri_, rt.data(), ntri * sizeof(RayTri), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	lo_[0] = b.lo.x; lo_[1] = b.lo.y; lo_[2] = b.lo.z;
+	hi_[0] = b.hi.x; hi_[1] = b.hi.y; hi_[2] = b.hi.z;
+}
+
+void DeviceScene::set_samples(const float* pos, const float* nrm, size_t n)
+{
+	if (n > kSampleMask)
+		throw std::runtime_error("too many samples for the work-item encoding");
+	dfree(d_pos_);
+	dfree(d_nrm_);
+	dfree(d_lm_);
+	nsamples_ = n;
+	d_pos_ = dalloc<float>(3 * n);
+	d_nrm_ = dalloc<float>(3 * n);
+	d_lm_ = dalloc<float>(3 * n);
+	hipStream_t s = (hipStream_t)stream_;
+	if (n) {
+		HIPCHECK(hipMemcpyAsync(d_pos_, pos, 12 * n, hipMemcpyHostToDevice, s));
+		HIPCHECK(hipMemcpyAsync(d_nrm_, nrm, 12 * n, hipMemcpyHostToDevice, s));
+		HIPCHECK(hipMemsetAsync(d_lm_, 0, 12 * n, s));
+	}
+	HIPCHECK(hipStreamSynchronize(s));
+}
+
+void DeviceScene::set_lightmap(const float* rgb)
+{
+	if (nsamples_)
+		HIPCHECK(hipMemcpy(d_lm_, rgb, 12 * nsamples_, hipMemcpyHostToDevice));
+}
+
+void DeviceScene::get_lightmap(float* rgb) const
+{
+	if (nsamples_)
+		HIPCHECK(hipMemcpy(rgb, d_lm_, 12 * nsamples_, hipMemcpyDeviceToHost));
+}
+
+struct EventTimer {
+	hipEvent_t a, b;
+	hipStream_t s;
+	explicit EventTimer(hipStream_t st) : s(st)
+	{
+		HIPCHECK(hipEventCreate(&a));
+		HIPCHECK(hipEventCreate(&b));
+		HIPCHECK(hipEventRecord(a, s));
+	}
+	double stop()
+	{
+		HIPCHECK(hipEventRecord(b, s));
+		HIPCHECK(hipEventSynchronize(b));
+		float ms = 0;
+		HIPCHECK(hipEventElapsedTime(&ms, a, b));
+		return ms;
+	}
+	~EventTimer()
+	{
+		(void)hipEventDestroy(a);
+		(void)hipEventDestroy(b);
+	}
+};
+
+void DeviceScene::direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
+                               const std::vector<uint32_t>& light_off, bool keep_weights)
+{
+	hipStream_t s = (hipStream_t)stream_;
+	if (lights.size() > 64)
+		throw std::runtime_error("direct_light: at most 64 lights per batch");
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	dfree(d_w_);
+	dfree(d_vis_);
+	nitems_ = items.size();
+	d_w_ = dalloc<float>(nitems_);
+	if (keep_weights)
+		d_vis_ = dalloc<float>(nitems_);
+	uint32_t* d_items = dalloc<uint32_t>(nitems_);
+	DevLight* d_lights = dalloc<DevLight>(lights.size());
+	HIPCHECK(hipMemcpyAsync(d_items, items.data(), 4 * nitems_, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_lights, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemsetAsync(d_counters_, 0, 4 * sizeof(unsigned long long), s));
+	EventTimer all(s);
+	{
+		EventTimer tm(s);
+		if (nitems_) {
+			if (count_queries)
+				k_march<true><<<grid_for(nitems_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights, d_items,
+				                                                   (uint32_t)nitems_, d_w_, d_vis_, d_counters_);
+			else
+				k_march<false><<<grid_for(nitems_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights, d_items,
+				                                                    (uint32_t)nitems_, d_w_, d_vis_, d_counters_);
+			HIPCHECK(hipGetLastError());
+		}
+		times.direct_march_ms = tm.stop();
+	}
+	for (size_t l = 0; l < lights.size(); ++l) {
+		uint32_t b = light_off[l], e = light_off[l + 1];
+		if (e > b) {
+			k_accumulate<<<grid_for(e - b), kBlock, 0, s>>>(d_items, d_w_, b, e, lights[l].color[0],
+			                                                lights[l].color[1], lights[l].color[2], d_lm_);
+			HIPCHECK(hipGetLastError());
+		}
+	}
+	times.direct_ms = all.stop();
+	if (count_queries) {
+		unsigned long long c[4];
+		HIPCHECK(hipMemcpy(c, d_counters_, sizeof(c), hipMemcpyDeviceToHost));
+		counters.nodes = c[0];
+		counters.tris = c[1];
+		counters.steps = c[2];
+		counters.queries = c[3];
+	}
+	dfree(d_items);
+	dfree(d_lights);
+}
+
+void DeviceScene::get_weights(float* out, size_t n) const
+{
+	if (n > nitems_)
+		throw std::runtime_error("get_weights: more than available");
+	if (n)
+		HIPCHECK(hipMemcpy(out, d_w_, 4 * n, hipMemcpyDeviceToHost));
+}
+
+void DeviceScene::get_vis(float* out, size_t n) const
+{
+	if (!d_vis_ || n > nitems_)
+		throw std::runtime_error("get_vis: shadow factors were not kept");
+	if (n)
+		HIPCHECK(hipMemcpy(out, d_vis_, 4 * n, hipMemcpyDeviceToHost));
+}
+
+void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float distance, float multiplier,
+                                    float falloff, float effect, const float* color)
+{
+	hipStream_t s = (hipStream_t)stream_;
+	if (nsamples_ == 0 || num_samples <= 0)
+		return;
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	// (i + 0.5) / n, its square root and sin(acos(.)) depend only on (i, n):
+	// evaluated here with the C library the reference uses
+	std::vector<float2> side(num_samples);
+	for (int i = 0; i < num_samples; ++i) {
+		float q = (i + 0.5f) / num_samples;
+		float cs = sqrtf(q);
+		side[i] = make_float2(cs, sinf(acosf(cs)));
+	}
+	float2* d_side = dalloc<float2>(num_samples);
+	float* d_ro = dalloc<float>(nsamples_);
+	HIPCHECK(hipMemcpyAsync(d_side, side.data(), sizeof(float2) * num_samples, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_ro, randoff, 4 * nsamples_, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemsetAsync(d_counters_, 0, 4 * sizeof(unsigned long long), s));
+	AOParams P;
+	P.n = num_samples;
+	P.distance = distance;
+	P.multiplier = multiplier;
+	P.falloff = falloff;
+	P.effect = effect;
+	P.color[0] = color[0];
+	P.color[1] = color[1];
+	P.color[2] = color[2];
+	EventTimer tm(s);
+	if (count_queries)
+		k_ao<true><<<grid_for(nsamples_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P, (uint32_t)nsamples_,
+		                                                  d_lm_, d_counters_);
+	else
+		k_ao<false><<<grid_for(nsamples_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P, (uint32_t)nsamples_,
+		                                                   d_lm_, d_counters_);
+	HIPCHECK(hipGetLastError());
+	times.ao_ms = tm.stop();
+	if (count_queries) {
+		unsigned long long c[4];
+		HIPCHECK(hipMemcpy(c, d_counters_, sizeof(c), hipMemcpyDeviceToHost));
+		counters.nodes = c[0];
+		counters.tris = c[1];
+		counters.steps = c[2];
+		counters.queries = c[3];
+	}
+	dfree(d_side);
+	dfree(d_ro);
+}
+
+// ---------------------------------------------------------------------------
+void DeviceScene::query_distance(const float* pts, size_t n, float* out)
+{
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	float *dp = dalloc<float>(3 * n), *dout = dalloc<float>(n);
+	HIPCHECK(hipMemcpy(dp, pts, 12 * n, hipMemcpyHostToDevice));
+	k_q_distance<<<grid_for(n), kBlock, 0, (hipStream_t)stream_>>>(V, dp, (uint32_t)n, dout);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize((hipStream_t)stream_));
+	HIPCHECK(hipMemcpy(out, dout, 4 * n, hipMemcpyDeviceToHost));
+	dfree(dp);
+	dfree(dout);
+}
+
+void DeviceScene::query_shadow(const float* from, const float* to, const float* k, size_t n, float* out,
+                               uint32_t* steps)
+{
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	float *da = dalloc<float>(3 * n), *db = dalloc<float>(3 * n), *dk = dalloc<float>(n), *dout = dalloc<float>(n);
+	uint32_t* dst = dalloc<uint32_t>(n);
+	HIPCHECK(hipMemcpy(da, from, 12 * n, hipMemcpyHostToDevice));
+	HIPCHECK(hipMemcpy(db, to, 12 * n, hipMemcpyHostToDevice));
+	HIPCHECK(hipMemcpy(dk, k, 4 * n, hipMemcpyHostToDevice));
+	k_q_shadow<<<grid_for(n), kBlock, 0, (hipStream_t)stream_>>>(V, da, db, dk, (uint32_t)n, dout, dst);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize((hipStream_t)stream_));
+	HIPCHECK(hipMemcpy(out, dout, 4 * n, hipMemcpyDeviceToHost));
+	if (steps)
+		HIPCHECK(hipMemcpy(steps, dst, 4 * n, hipMemcpyDeviceToHost));
+	dfree(da);
+	dfree(db);
+	dfree(dk);
+	dfree(dout);
+	dfree(dst);
+}
+
+void DeviceScene::query_closest(const float* a, const float* b, size_t n, float* out)
+{
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	float *da = dalloc<float>(3 * n), *db = dalloc<float>(3 * n), *dout = dalloc<float>(n);
+	HIPCHECK(hipMemcpy(da, a, 12 * n, hipMemcpyHostToDevice));
+	HIPCHECK(hipMemcpy(db, b, 12 * n, hipMemcpyHostToDevice));
+	k_q_closest<<<grid_for(n), kBlock, 0, (hipStream_t)stream_>>>(V, da, db, (uint32_t)n, dout);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize((hipStream_t)stream_));
+	HIPCHECK(hipMemcpy(out, dout, 4 * n, hipMemcpyDeviceToHost));
+	dfree(da);
+	dfree(db);
+	dfree(dout);
+}
+
+void DeviceScene::query_occluded(const float* a, const float* b, size_t n, uint8_t* out)
+{
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	float *da = dalloc<float>(3 * n), *db = dalloc<float>(3 * n);
+	uint8_t* dout = dalloc<uint8_t>(n);
+	HIPCHECK(hipMemcpy(da, a, 12 * n, hipMemcpyHostToDevice));
+	HIPCHECK(hipMemcpy(db, b, 12 * n, hipMemcpyHostToDevice));
+	k_q_occluded<<<grid_for(n), kBlock, 0, (hipStream_t)stream_>>>(V, da, db, (uint32_t)n, dout);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize((hipStream_t)stream_));
+	HIPCHECK(hipMemcpy(out, dout, n, hipMemcpyDeviceToHost));
+	dfree(da);
+	dfree(db);
+	dfree(dout);
+}
+
+// ---------------------------------------------------------------------------
+// radiosity
+// ---------------------------------------------------------------------------
+static void scan_u32_to_u64(const uint32_t* d_in, uint64_t n, uint64_t* d_out, uint64_t* h_total, hipStream_t s)
+{
+	constexpr uint64_t per_block = (uint64_t)kBlock * 16;
+	uint32_t nb = (uint32_t)((n + per_block - 1) / per_block);
+	if (nb == 0) {
+		*h_total = 0;
+		return;
+	}
+	uint64_t* sums = dalloc<uint64_t>(nb + 1);
+	k_scan_blocks<<<nb, kBlock, 0, s>>>(d_in, n, d_out, sums);
+	k_scan_sums<<<1, 1, 0, s>>>(sums, nb, sums + nb);
+	k_scan_add<<<nb, kBlock, 0, s>>>(d_out, n, sums);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(h_total, sums + nb, 8, hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	dfree(sums);
+}
+
+void DeviceScene::rad_set_patches(const float* pos, const float* nrm, size_t n)
+{
+	dfree(rad.pos);
+	dfree(rad.nrm);
+	npatches = n;
+	rad.pos = dalloc<float>(3 * n);
+	rad.nrm = dalloc<float>(3 * n);
+	if (n) {
+		HIPCHECK(hipMemcpy(rad.pos, pos, 12 * n, hipMemcpyHostToDevice));
+		HIPCHECK(hipMemcpy(rad.nrm, nrm, 12 * n, hipMemcpyHostToDevice));
+	}
+}
+
+size_t DeviceScene::rad_links()
+{
+	hipStream_t s = (hipStream_t)stream_;
+	uint32_t N = (uint32_t)npatches;
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	dfree(rad.bits);
+	dfree(rad.rowptr);
+	dfree(rad.col);
+	dfree(rad.val);
+	rad.words = (N + 31) / 32;
+	rad.bits = dalloc<uint32_t>((size_t)N * rad.words);
+	HIPCHECK(hipMemsetAsync(rad.bits, 0, (size_t)N * rad.words * 4, s));
+	rad.rowptr = dalloc<uint32_t>(N + 1);
+	EventTimer tm(s);
+	if (N > 1) {
+		uint32_t nchunks = (N + kPairChunk - 1) / kPairChunk;
+		uint64_t ncounts = (uint64_t)N * nchunks;
+		uint32_t* counts = dalloc<uint32_t>(ncounts);
+		uint64_t* offs = dalloc<uint64_t>(ncounts);
+		dim3 g(grid_for(N), nchunks);
+		k_pairs<false><<<g, kBlock, 0, s>>>(rad.pos, rad.nrm, N, nchunks, counts, nullptr, nullptr);
+		HIPCHECK(hipGetLastError());
+		uint64_t ncand = 0;
+		scan_u32_to_u64(counts, ncounts, offs, &ncand, s);
+		uint2* cand = dalloc<uint2>(ncand);
+		k_pairs<true><<<g, kBlock, 0, s>>>(rad.pos, rad.nrm, N, nchunks, nullptr, offs, cand);
+		HIPCHECK(hipGetLastError());
+		dfree(counts);
+		dfree(offs);
+		if (ncand) {
+			k_link_vis<<<grid_for(ncand), kBlock, 0, s>>>(V, rad.pos, cand, ncand, rad.bits, rad.words);
+			HIPCHECK(hipGetLastError());
+		}
+		HIPCHECK(hipStreamSynchronize(s));
+		dfree(cand);
+	}
+	// symmetric CSR
+	uint32_t* cnt = dalloc<uint32_t>(N + 1);
+	HIPCHECK(hipMemsetAsync(cnt, 0, 4 * (N + 1), s));
+	if (N)
+		k_row_popc<<<grid_for(N), kBlock, 0, s>>>(rad.bits, rad.words, N, cnt);
+	uint64_t* rp64 = dalloc<uint64_t>(N + 1);
+	uint64_t nnz = 0;
+	scan_u32_to_u64(cnt, (uint64_t)N + 1, rp64, &nnz, s);
+	std::vector<uint64_t> h64(N + 1);
+	HIPCHECK(hipMemcpy(h64.data(), rp64, 8 * (N + 1), hipMemcpyDeviceToHost));
+	if (nnz >= (1ull << 32))
+		throw std::runtime_error("radiosity: more than 2^32 link entries");
+	std::vector<uint32_t> h32(N + 1);
+	for (uint32_t i = 0; i <= N; ++i)
+		h32[i] = (uint32_t)h64[i];
+	HIPCHECK(hipMemcpy(rad.rowptr, h32.data(), 4 * (N + 1), hipMemcpyHostToDevice));
+	dfree(cnt);
+	dfree(rp64);
+	rad.nnz = nnz;
+	rad.col = dalloc<uint32_t>(nnz);
+	rad.val = dalloc<float>(nnz);
+	if (N)
+		k_row_fill<<<grid_for(N), kBlock, 0, s>>>(rad.bits, rad.words, N, rad.rowptr, rad.pos, rad.nrm, rad.col,
+		                                          rad.val);
+	HIPCHECK(hipGetLastError());
+	times.links_ms = tm.stop();
+	nlinks_ = nnz / 2;
+	return nlinks_;
+}
+
+void DeviceScene::rad_get_csr(std::vector<uint32_t>& rowptr, std::vector<uint32_t>& col,
+                              std::vector<float>& val) const
+{
+	rowptr.resize(npatches + 1);
+	col.resize(rad.nnz);
+	val.resize(rad.nnz);
+	HIPCHECK(hipMemcpy(rowptr.data(), rad.rowptr, 4 * (npatches + 1), hipMemcpyDeviceToHost));
+	if (rad.nnz) {
+		HIPCHECK(hipMemcpy(col.data(), rad.col, 4 * rad.nnz, hipMemcpyDeviceToHost));
+		HIPCHECK(hipMemcpy(val.data(), rad.val, 4 * rad.nnz, hipMemcpyDeviceToHost));
+	}
+}
+
+void DeviceScene::rad_get_links(uint32_t* rowmap, uint32_t* other, float* factor) const
+{
+	std::vector<uint32_t> rp, col;
+	std::vector<float> val;
+	rad_get_csr(rp, col, val);
+	uint32_t o = 0;
+	for (size_t i = 0; i < npatches; ++i) {
+		rowmap[2 * i] = o;
+		uint32_t c = 0;
+		for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+			if (col[k] > i) {
+				other[o] = col[k];
+				factor[o] = val[k];
+				o++;
+				c++;
+			}
+		}
+		rowmap[2 * i + 1] = c;
+	}
+}
+
+void DeviceScene::rad_set_colors(const float* diffuse3, const float* emit3, const float* area)
+{
+	size_t n = npatches;
+	dfree(rad.diffuse);
+	dfree(rad.area);
+	dfree(rad.total);
+	dfree(rad.out);
+	dfree(rad.in);
+	rad.diffuse = dalloc<float>(3 * n);
+	rad.area = dalloc<float>(n);
+	rad.total = dalloc<float>(3 * n);
+	rad.out = dalloc<float>(3 * n);
+	rad.in = dalloc<float>(3 * n);
+	if (n) {
+		HIPCHECK(hipMemcpy(rad.diffuse, diffuse3, 12 * n, hipMemcpyHostToDevice));
+		HIPCHECK(hipMemcpy(rad.area, area, 4 * n, hipMemcpyHostToDevice));
+		HIPCHECK(hipMemcpy(rad.total, emit3, 12 * n, hipMemcpyHostToDevice));
+		HIPCHECK(hipMemcpy(rad.out, emit3, 12 * n, hipMemcpyHostToDevice));
+	}
+}
+
+void DeviceScene::rad_bounce(int count)
+{
+	hipStream_t s = (hipStream_t)stream_;
+	uint32_t N = (uint32_t)npatches;
+	EventTimer tm(s);
+	for (int b = 0; b < count && N; ++b) {
+		k_bounce3<<<grid_for(N), kBlock, 0, s>>>(N, rad.rowptr, rad.col, rad.val, rad.diffuse, rad.area, rad.out,
+		                                         rad.in, rad.total);
+		HIPCHECK(hipGetLastError());
+		std::swap(rad.out, rad.in);
+	}
+	times.bounce_ms = tm.stop();
+}
+
+void DeviceScene::rad_get_total(float* out3) const
+{
+	if (npatches)
+		HIPCHECK(hipMemcpy(out3, rad.total, 12 * npatches, hipMemcpyDeviceToHost));
+}
+
+} // namespace lgx

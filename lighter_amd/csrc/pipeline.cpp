@@ -1,0 +1,784 @@
+// The bake pipeline behind ltr_Start: host stages before and after the GPU
+// hot path, and the GPU stages themselves.
+//
+// Stage order, status strings and completion updates follow Job_MainProc
+// (lighter.cpp:1046-1145).  The host stages restate the reference's
+// instance transform, collision set-up, sample placement and lightmap
+// finalisation (lighter.cpp:292-482, 849-1044) so that the samples the
+// kernels see and the images the caller gets back are the reference's.
+#include "pipeline.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <functional>
+#include <stdexcept>
+#include <thread>
+
+namespace lgx {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static void parallel_for(int nthreads, size_t n, const std::function<void(size_t)>& fn)
+{
+	if (nthreads <= 1 || n <= 1) {
+		for (size_t i = 0; i < n; ++i)
+			fn(i);
+		return;
+	}
+	std::atomic<size_t> next{0};
+	std::vector<std::thread> th;
+	int nt = (int)std::min<size_t>((size_t)nthreads, n);
+	for (int t = 0; t < nt; ++t)
+		th.emplace_back([&]() {
+			for (;;) {
+				size_t i = next.fetch_add(1);
+				if (i >= n)
+					break;
+				fn(i);
+			}
+		});
+	for (auto& t : th)
+		t.join();
+}
+
+static void set_stage(ltr_Scene* S, const char* st)
+{
+	S->stage.store(st);
+	S->completion.store(0.0f);
+}
+
+static void check_abort(ltr_Scene* S)
+{
+	if (S->abort.load())
+		throw AbortBake();
+}
+
+// ---------------------------------------------------------------------------
+// transforming spatial data (lighter.cpp:292-341)
+// ---------------------------------------------------------------------------
+static void pre_xform(ltr_Scene* S, Instance* mi)
+{
+	if (mi->samplecont)
+		return;
+	ltr_Mesh* mesh = mi->mesh;
+	mi->vpos.resize(mesh->vpos.size());
+	mi->vnrm.resize(mesh->vnrm.size());
+	mi->vtex.resize(mesh->vtex1.size());
+	mi->ltex.resize(mesh->vtex2.size());
+	for (size_t i = 0; i < mesh->vpos.size(); ++i)
+		mi->vpos[i] = mat4_transform(mi->matrix, mesh->vpos[i], 1.0f);
+	Mat4 nm = normal_matrix(mi->matrix);
+	for (size_t i = 0; i < mesh->vnrm.size(); ++i)
+		mi->vnrm[i] = normalized(mat4_transform(nm, mesh->vnrm[i], 0.0f));
+
+	float total_area = 0.0f;
+	for (const MeshPart& mp : mesh->parts) {
+		const f3* vb = mi->vpos.data() + mp.voff;
+		const uint32_t* ib = mesh->indices.data() + mp.ioff;
+		for (uint32_t t = 0; t + 2 < mp.icount; t += 3)
+			total_area += triangle_area(vb[ib[t]], vb[ib[t + 1]], vb[ib[t + 2]]);
+	}
+	ltr_Config& cfg = S->config;
+	u32 sz[2] = {cfg.default_width, cfg.default_height};
+	if (!cfg.size_fn || !cfg.size_fn(&cfg, mesh->ident.c_str(), mesh->ident.size(), mi->ident.c_str(),
+	                                 mi->ident.size(), total_area, mi->importance, sz)) {
+		sz[0] = cfg.default_width;
+		sz[1] = cfg.default_height;
+	}
+	mi->lm_w = sz[0];
+	mi->lm_h = sz[1];
+	f2 lsize = mk2((float)mi->lm_w, (float)mi->lm_h);
+	for (size_t i = 0; i < mi->ltex.size(); ++i) {
+		mi->vtex[i] = mesh->vtex1[i];
+		mi->ltex[i] = mesh->vtex2[i] * lsize - mk2(0.5f, 0.5f);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// generating data structures (lighter.cpp:349-384, 1060-1069)
+// ---------------------------------------------------------------------------
+static void col_info(ltr_Scene* S, Instance* mi)
+{
+	(void)S;
+	if (mi->samplecont) {
+		mi->tris.set(nullptr, 0);
+		return;
+	}
+	std::vector<Tri> tris;
+	ltr_Mesh* mesh = mi->mesh;
+	for (const MeshPart& mp : mesh->parts) {
+		if (!mp.shadow)
+			continue;
+		const f3* vb = mi->vpos.data() + mp.voff;
+		const uint32_t* ib = mesh->indices.data() + mp.ioff;
+		for (uint32_t t = 0; t + 2 < mp.icount; t += 3) {
+			Tri T{vb[ib[t]], vb[ib[t + 1]], vb[ib[t + 2]]};
+			if (tri_useful(T))
+				tris.push_back(T);
+		}
+	}
+	if (!tris.empty())
+		mi->split.set(tris.data(), tris.size());
+	mi->tris.set(tris.data(), tris.size());
+}
+
+// Scene-level closest hit through the split trees (lighter.cpp:210-251).
+static float scene_split_hit(ltr_Scene* S, f3 A, f3 B, f3* outnormal)
+{
+	f3 dn = normalized(B - A);
+	f3 mA = A + dn * kSmall;
+	f3 mB = B - dn * kSmall;
+	RaySeg r;
+	r.set(mA, mB);
+	float closest = 2.0f;
+	auto fn = [&](const int32_t* ids, int32_t n) {
+		for (int32_t i = 0; i < n; ++i) {
+			Instance* mi = S->insts[ids[i]];
+			if (!mi->shadow)
+				continue;
+			f3 nrm = mk3(0, 0, 0);
+			float d = mi->split.intersect(r.origin, r.end, &nrm);
+			if (d < closest) {
+				closest = d;
+				if (outnormal)
+					*outnormal = nrm;
+			}
+		}
+		return true;
+	};
+	S->inst_tree.ray_query(r, fn);
+	return closest;
+}
+
+// ---------------------------------------------------------------------------
+// generating samples (lighter.cpp:32-66, 392-476)
+// ---------------------------------------------------------------------------
+static void rasterize_instance(Instance* mi, float margin, f3* R1, f3* R2, f4* R3)
+{
+	ltr_Mesh* mesh = mi->mesh;
+	for (uint32_t part = 0; part < mesh->parts.size(); ++part) {
+		const MeshPart& mp = mesh->parts[part];
+		uint32_t O = mp.voff;
+		const uint32_t* ib = mesh->indices.data() + mp.ioff;
+		for (uint32_t t = 0; t + 2 < mp.icount; t += 3) {
+			uint32_t a = O + ib[t], b = O + ib[t + 1], c = O + ib[t + 2];
+			float area = sample_area(mi->ltex[a], mi->ltex[b], mi->ltex[c], mi->vpos[a], mi->vpos[b], mi->vpos[c]);
+			f4 ia{mi->vtex[a].x, mi->vtex[a].y, (float)part, area};
+			f4 ibv{mi->vtex[b].x, mi->vtex[b].y, (float)part, area};
+			f4 ic{mi->vtex[c].x, mi->vtex[c].y, (float)part, area};
+			rasterize_triangle(R1, R2, R3, (int32_t)mi->lm_w, (int32_t)mi->lm_h, margin, mi->ltex[a], mi->ltex[b],
+			                   mi->ltex[c], mi->vpos[a], mi->vpos[b], mi->vpos[c], mi->vnrm[a], mi->vnrm[b],
+			                   mi->vnrm[c], ia, ibv, ic);
+		}
+	}
+}
+
+static void gen_samples(ltr_Scene* S, Instance* mi)
+{
+	const ltr_Config& cfg = S->config;
+	float corr_min_dot = cosf(cfg.max_correct_angle / 180.0f * kPiF);
+	f3 amb = ld3(cfg.ambient_color);
+	if (mi->samplecont) {
+		for (auto& c : mi->lightmap)
+			c = amb;
+		return;
+	}
+	size_t n = (size_t)mi->lm_w * mi->lm_h;
+	std::vector<f3> R1(n, mk3(0, 0, 0)), R2(n, mk3(0, 0, 0));
+	std::vector<f4> R3(n, f4{0, 0, 0, 0});
+	rasterize_instance(mi, 0.5f + kSmall, R1.data(), R2.data(), R3.data());
+	rasterize_instance(mi, 0.0f, R1.data(), R2.data(), R3.data());
+
+	size_t count = 0;
+	for (size_t i = 0; i < n; ++i)
+		if (!is_zero(R2[i]))
+			count++;
+	mi->s_pos.reserve(count);
+	mi->s_nrm.reserve(count);
+	mi->s_loc.reserve(count);
+	mi->s_info.reserve(count);
+	for (size_t i = 0; i < n; ++i) {
+		f3 N = R2[i];
+		if (is_zero(N))
+			continue;
+		N = normalized(N);
+		f3 P = R1[i];
+		f4 XD = R3[i];
+		// move the sample out of concave edges
+		for (Instance* o : S->insts)
+			o->tris.offset_sample(P, N, sqrtf(XD.w));
+		// move the sample in front of overlapping faces
+		if (cfg.max_correct_dist) {
+			int left = 100;
+			float md = cfg.max_correct_dist;
+			f3 PEnd = P + N * md;
+			while (md > kSmall && left-- > 0) {
+				f3 hn = mk3(0, 0, 0);
+				float q = scene_split_hit(S, P, PEnd, &hn);
+				if (-dot(hn, N) < corr_min_dot)
+					break;
+				if (q < kSmall)
+					q = kSmall;
+				P = P * (1.0f - q) + PEnd * q;
+				md *= (1.0f - q);
+			}
+		}
+		mi->s_pos.push_back(P);
+		mi->s_nrm.push_back(N);
+		mi->s_loc.push_back((uint32_t)i);
+		mi->s_info.push_back(XD);
+	}
+	mi->lightmap.assign(count, amb);
+}
+
+void prepare(ltr_Scene* S)
+{
+	if (S->prepared)
+		return;
+	int nt = S->nthreads;
+	set_stage(S, "transforming spatial data");
+	parallel_for(nt, S->insts.size(), [&](size_t i) { pre_xform(S, S->insts[i]); });
+	check_abort(S);
+
+	set_stage(S, "generating data structures");
+	parallel_for(nt, S->insts.size(), [&](size_t i) { col_info(S, S->insts[i]); });
+	S->completion.store(0.99f);
+	std::vector<Box> bbs;
+	for (Instance* mi : S->insts) {
+		const BoxTree::Node& r = mi->tris.tree().root();
+		bbs.push_back(Box{r.mn, r.mx});
+	}
+	S->inst_tree.build(bbs.data(), bbs.size());
+	check_abort(S);
+
+	set_stage(S, "generating samples");
+	parallel_for(nt, S->insts.size(), [&](size_t i) { gen_samples(S, S->insts[i]); });
+	check_abort(S);
+
+	S->inst_off.assign(S->insts.size() + 1, 0);
+	for (size_t i = 0; i < S->insts.size(); ++i)
+		S->inst_off[i + 1] = S->inst_off[i] + S->insts[i]->lightmap.size();
+	S->total_samples = S->inst_off.back();
+	S->prepared = true;
+}
+
+// ---------------------------------------------------------------------------
+// GPU scene
+// ---------------------------------------------------------------------------
+void ensure_device(ltr_Scene* S)
+{
+	if (S->dev)
+		return;
+	std::unique_ptr<DeviceScene> d(new DeviceScene());
+	std::vector<float> tris;
+	for (Instance* mi : S->insts) {
+		if (!mi->shadow)
+			continue;
+		for (const Tri& t : mi->tris.tris()) {
+			const float v[9] = {t.p1.x, t.p1.y, t.p1.z, t.p2.x, t.p2.y, t.p2.z, t.p3.x, t.p3.y, t.p3.z};
+			tris.insert(tris.end(), v, v + 9);
+		}
+	}
+	d->set_triangles(tris.data(), tris.size() / 9);
+	std::vector<float> pos(3 * S->total_samples), nrm(3 * S->total_samples), lm(3 * S->total_samples);
+	size_t k = 0;
+	for (Instance* mi : S->insts) {
+		for (size_t i = 0; i < mi->lightmap.size(); ++i, ++k) {
+			st3(&pos[3 * k], mi->s_pos[i]);
+			st3(&nrm[3 * k], mi->s_nrm[i]);
+			st3(&lm[3 * k], mi->lightmap[i]);
+		}
+	}
+	d->set_samples(pos.data(), nrm.data(), S->total_samples);
+	d->set_lightmap(lm.data());
+	S->dev = std::move(d);
+}
+
+void download_lightmaps(ltr_Scene* S)
+{
+	std::vector<float> lm(3 * S->total_samples);
+	S->dev->get_lightmap(lm.data());
+	size_t k = 0;
+	for (Instance* mi : S->insts)
+		for (size_t i = 0; i < mi->lightmap.size(); ++i, ++k)
+			mi->lightmap[i] = ld3(&lm[3 * k]);
+}
+
+void upload_lightmaps(ltr_Scene* S)
+{
+	std::vector<float> lm(3 * S->total_samples);
+	size_t k = 0;
+	for (Instance* mi : S->insts)
+		for (size_t i = 0; i < mi->lightmap.size(); ++i, ++k)
+			st3(&lm[3 * k], mi->lightmap[i]);
+	S->dev->set_lightmap(lm.data());
+}
+
+// Instances a light renders into, in render order (lighter.cpp:80-98,
+// 1080-1093): the sample container plus the instance-tree query, sorted and
+// de-duplicated.
+std::vector<int32_t> light_instances(ltr_Scene* S, const Light& L)
+{
+	std::vector<int32_t> ids;
+	ids.push_back(0);
+	auto add = [&](const int32_t* p, int32_t n) { ids.insert(ids.end(), p, p + n); };
+	if (L.type == LTR_LT_POINT || L.type == LTR_LT_SPOT) {
+		f3 r3 = mk3(L.range, L.range, L.range);
+		S->inst_tree.query(L.pos - r3, L.pos + r3, add);
+	} else if (L.type == LTR_LT_DIRECT) {
+		S->inst_tree.all(add);
+	}
+	std::sort(ids.begin(), ids.end());
+	ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+	return ids;
+}
+
+DevLight to_dev_light(const Light& L)
+{
+	DevLight d{};
+	d.type = (int32_t)L.type;
+	st3(d.pos, L.pos);
+	st3(d.dir, L.dir);
+	st3(d.color, L.color);
+	d.range = L.range;
+	d.power = L.power;
+	d.radius = L.radius;
+	// Int_LMRender spot set-up (lighter.cpp:641-644)
+	float out_rad = L.spot_out / 180.0f * kPiF, in_rad = L.spot_in / 180.0f * kPiF;
+	if (in_rad == out_rad)
+		in_rad -= kSmall;
+	d.spot_out_rad = out_rad;
+	d.spot_diff = in_rad - out_rad;
+	d.spot_curve = L.spot_curve;
+	return d;
+}
+
+static float brightness(f3 c) { return (c.x + c.y + c.z) * (1.0f / 3.0f); }
+
+// Normal-map light contributions of one light (lighter.cpp:505-513,
+// 543-551, 590-598, 656-664), from the shadow factors the GPU computed.
+static void collect_contribs(ltr_Scene* S, const Light& L, const std::vector<int32_t>& ids, const float* vis)
+{
+	float out_rad = L.spot_out / 180.0f * kPiF, in_rad = L.spot_in / 180.0f * kPiF;
+	if (in_rad == out_rad)
+		in_rad -= kSmall;
+	float diff = in_rad - out_rad;
+	size_t k = 0;
+	for (int32_t id : ids) {
+		Instance* mi = S->insts[id];
+		for (size_t i = 0; i < mi->lightmap.size(); ++i, ++k) {
+			if (mi->samplecont)
+				continue;
+			f3 SP = mi->s_pos[i], SN = mi->s_nrm[i];
+			float fv = vis[k];
+			if (L.type == LTR_LT_DIRECT) {
+				float fn = tmax(0.0f, dot(L.dir, SN));
+				float factor = fv * fn * brightness(L.color);
+				if (factor > 0)
+					mi->contrib.push_back(LightContrib{L.dir * factor, (int32_t)i});
+				continue;
+			}
+			f3 s2l = L.pos - SP;
+			float dist = length(s2l);
+			if (dist)
+				s2l = s2l / dist;
+			float fd = powf(1 - tmin(1.0f, dist / L.range), L.power);
+			float fn = tmax(0.0f, dot(s2l, SN));
+			float factor;
+			if (L.type == LTR_LT_SPOT) {
+				float ang = acosf(tmin(1.0f, dot(s2l, -L.dir)));
+				float fdir = tmax(0.0f, tmin(1.0f, (ang - out_rad) / diff));
+				fdir = powf(fdir, L.spot_curve);
+				if (fd * fn * fdir <= 0)
+					continue;
+				factor = fd * fdir * fv * fn * brightness(L.color);
+			} else {
+				if (fd * fn <= 0)
+					continue;
+				factor = fd * fv * fn * brightness(L.color);
+			}
+			if (factor > 0)
+				mi->contrib.push_back(LightContrib{s2l * factor, (int32_t)i});
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// rendering lightmaps (lighter.cpp:1075-1096)
+// ---------------------------------------------------------------------------
+void stage_direct(ltr_Scene* S)
+{
+	ensure_device(S);
+	const bool want_vis = S->config.generate_normalmap_data != 0;
+	S->times.direct_ms = S->times.direct_march_ms = 0;
+	for (size_t l0 = 0; l0 < S->lights.size(); l0 += 64) {
+		size_t l1 = std::min(S->lights.size(), l0 + 64);
+		std::vector<DevLight> dl;
+		std::vector<uint32_t> items, off;
+		std::vector<std::vector<int32_t>> ids_of;
+		off.push_back(0);
+		for (size_t l = l0; l < l1; ++l) {
+			const Light& L = S->lights[l];
+			dl.push_back(to_dev_light(L));
+			std::vector<int32_t> ids = light_instances(S, L);
+			for (int32_t id : ids)
+				for (size_t s = S->inst_off[id]; s < S->inst_off[id + 1]; ++s)
+					items.push_back((uint32_t)((l - l0) << 26) | (uint32_t)s);
+			off.push_back((uint32_t)items.size());
+			ids_of.push_back(std::move(ids));
+		}
+		S->dev->direct_light(dl, items, off, want_vis);
+		S->times.direct_ms += S->dev->times.direct_ms;
+		S->times.direct_march_ms += S->dev->times.direct_march_ms;
+		if (want_vis) {
+			std::vector<float> vis(items.size());
+			S->dev->get_vis(vis.data(), vis.size());
+			for (size_t l = l0; l < l1; ++l)
+				collect_contribs(S, S->lights[l], ids_of[l - l0], vis.data() + off[l - l0]);
+		}
+		S->completion.store((float)l1 / (float)S->lights.size());
+		check_abort(S);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// radiosity (lighter.cpp:667-765, 1098-1124)
+// ---------------------------------------------------------------------------
+void stage_radiosity(ltr_Scene* S)
+{
+	ensure_device(S);
+	size_t N = S->total_samples;
+	if (N > kMaxPatches)
+		throw std::runtime_error("radiosity: " + std::to_string(N) + " patches exceeds the supported " +
+		                         std::to_string(kMaxPatches));
+	download_lightmaps(S);
+	std::vector<float> pos(3 * N), nrm(3 * N), diff(3 * N), emit(3 * N), area(N);
+	const ltr_Config& cfg = S->config;
+	size_t k = 0;
+	for (Instance* mi : S->insts) {
+		for (size_t i = 0; i < mi->lightmap.size(); ++i, ++k) {
+			f3 P = mi->s_pos[i];
+			f3 Nn = normalized(mi->s_nrm[i]);
+			st3(&pos[3 * k], P);
+			st3(&nrm[3 * k], Nn);
+			f3 d = mk3(1, 1, 1);
+			f3 e = mi->lightmap[i];
+			float a = mi->samplecont ? 0 : mi->s_info[i].w;
+			if (mi->samplecont) {
+				d = mk3(0, 0, 0);
+			} else if (cfg.sample_fn) {
+				uint32_t L = mi->s_loc[i];
+				int lx = (int)(L % mi->lm_w), ly = (int)(L / mi->lm_w);
+				ltr_SampleRequest req;
+				memset(&req, 0, sizeof(req));
+				st3(req.position, P);
+				st3(req.normal, Nn);
+				req.tex0u = mi->s_info[i].x;
+				req.tex0v = mi->s_info[i].y;
+				req.tex1u = (lx + 0.5f) / mi->lm_w;
+				req.tex1v = (ly + 0.5f) / mi->lm_h;
+				req.part_id = (uint32_t)mi->s_info[i].z;
+				req.mesh_ident = mi->mesh->ident.c_str();
+				req.mesh_ident_size = mi->mesh->ident.size();
+				req.inst_ident = mi->ident.c_str();
+				req.inst_ident_size = mi->ident.size();
+				LTR_VEC3_SET(req.out_diffuse_color, 1, 1, 1);
+				LTR_VEC3_SET(req.out_emissive_color, 0, 0, 0);
+				if (cfg.sample_fn(&S->config, &req)) {
+					d = ld3(req.out_diffuse_color);
+					e += ld3(req.out_emissive_color);
+				}
+			}
+			st3(&diff[3 * k], d);
+			st3(&emit[3 * k], e);
+			area[k] = a;
+		}
+	}
+	check_abort(S);
+	DeviceScene& D = *S->dev;
+	D.rad_set_patches(pos.data(), nrm.data(), N);
+	D.rad_links();
+	S->times.links_ms = D.times.links_ms;
+	check_abort(S);
+
+	set_stage(S, "bouncing light");
+	D.rad_set_colors(diff.data(), emit.data(), area.data());
+	S->times.bounce_ms = 0;
+	for (int b = 0; b < cfg.bounce_count; ++b) {
+		D.rad_bounce(1);
+		S->times.bounce_ms += D.times.bounce_ms;
+		S->completion.store((b + 1) / (float)cfg.bounce_count);
+	}
+	check_abort(S);
+
+	set_stage(S, "committing radiosity");
+	std::vector<float> total(3 * N);
+	D.rad_get_total(total.data());
+	k = 0;
+	for (size_t m = 0; m < S->insts.size(); ++m) {
+		Instance* mi = S->insts[m];
+		for (size_t i = 0; i < mi->lightmap.size(); ++i, ++k)
+			mi->lightmap[i] = ld3(&total[3 * k]);
+		S->completion.store((m + 1) / (float)S->insts.size());
+	}
+	upload_lightmaps(S);
+}
+
+// ---------------------------------------------------------------------------
+// rendering ambient occlusion (lighter.cpp:799-841, 1126-1136)
+// ---------------------------------------------------------------------------
+void stage_ao(ltr_Scene* S)
+{
+	ensure_device(S);
+	const ltr_Config& cfg = S->config;
+	// One rotation offset per texel, drawn from the C library generator in
+	// instance order and sample order, as a single-threaded reference bake
+	// draws them (lighter.cpp:819, lighter_int.hpp:271).
+	std::vector<float> ro(S->total_samples);
+	for (size_t i = 0; i < ro.size(); ++i)
+		ro[i] = (float)rand() / (float)RAND_MAX;
+	S->dev->ambient_occlusion(ro.data(), cfg.ao_num_samples, cfg.ao_distance, cfg.ao_multiplier, cfg.ao_falloff,
+	                          cfg.ao_effect, cfg.ao_color_rgb);
+	S->times.ao_ms = S->dev->times.ao_ms;
+	S->completion.store(1.0f);
+}
+
+// ---------------------------------------------------------------------------
+// exporting lightmaps (lighter.cpp:849-1044, lighter_math.cpp:357-424)
+// ---------------------------------------------------------------------------
+static void gaussian_kernel(float* out, int ext, float radius)
+{
+	float sum = 0.0f;
+	float mc = 1.0f / sqrtf(2.0f * kPiF * radius * radius);
+	for (int i = -ext; i <= ext; ++i) {
+		float x = (float)i / radius;
+		sum += out[i + ext] = expf(-0.5f * (x * x)) * mc;
+	}
+	for (int i = -ext; i <= ext; ++i)
+		out[i + ext] /= sum;
+}
+
+static void convolve_transpose(const float* src, float* dst, uint32_t width, uint32_t height, int ext,
+                               const float* kernel, float* tmp)
+{
+	for (uint32_t y = 0; y < height; ++y) {
+		memcpy(tmp + ext * 3, src + (size_t)y * width * 3, sizeof(float) * width * 3);
+		for (int i = 0; i < ext; ++i)
+			for (int c = 0; c < 3; ++c) {
+				tmp[i * 3 + c] = tmp[ext * 3 + c];
+				tmp[(ext + width + i) * 3 + c] = tmp[(ext + width - 1) * 3 + c];
+			}
+		for (uint32_t x = 0; x < width; ++x) {
+			const float* sp = tmp + (x + ext) * 3;
+			float* dp = dst + ((size_t)y + (size_t)height * x) * 3;
+			float sum[3] = {0.0f, 0.0f, 0.0f};
+			for (int i = -ext; i <= ext; ++i) {
+				float kq = kernel[i + ext];
+				sum[0] += sp[i * 3 + 0] * kq;
+				sum[1] += sp[i * 3 + 1] * kq;
+				sum[2] += sp[i * 3 + 2] * kq;
+			}
+			dp[0] = sum[0];
+			dp[1] = sum[1];
+			dp[2] = sum[2];
+		}
+	}
+}
+
+static void downsample2x(float* dst, unsigned dw, unsigned dh, const float* src, unsigned sw, unsigned sh)
+{
+	for (unsigned y = 0; y < dh; ++y)
+		for (unsigned x = 0; x < dw; ++x) {
+			unsigned x0 = (x * 2) % sw, y0 = (y * 2) % sh, x1 = (x * 2 + 1) % sw, y1 = (y * 2 + 1) % sh;
+			f3 avg = (ld3(src + (x0 + y0 * sw) * 3) + ld3(src + (x1 + y0 * sw) * 3) + ld3(src + (x0 + y1 * sw) * 3) +
+			          ld3(src + (x1 + y1 * sw) * 3)) *
+			         0.25f;
+			st3(dst + (x + y * dw) * 3, avg);
+		}
+}
+
+static void finalize_instance(ltr_Scene* S, size_t mid)
+{
+	Instance* mi = S->insts[mid];
+	const ltr_Config& cfg = S->config;
+	uint32_t w = mi->lm_w, h = mi->lm_h;
+	size_t npx = (size_t)w * h;
+	float* rgb = new float[npx * 3]();
+	std::vector<char> set(npx * 2, 0);
+	for (size_t i = 0; i < mi->s_loc.size(); ++i) {
+		st3(rgb + (size_t)mi->s_loc[i] * 3, mi->lightmap[i]);
+		set[mi->s_loc[i]] = 1;
+	}
+	// three rounds of edge extrapolation into unset texels
+	auto isset = [&](uint32_t x, uint32_t y) { return x < w && y < h && set[x + (size_t)y * w]; };
+	auto col = [&](uint32_t x, uint32_t y) { return ld3(rgb + (x + (size_t)y * w) * 3); };
+	for (int its = 1; its <= 3; ++its) {
+		memcpy(set.data() + npx, set.data(), npx);
+		for (uint32_t y = 0; y < h; ++y) {
+			for (uint32_t x = 0; x < w; ++x) {
+				if (set[x + (size_t)y * w])
+					continue;
+				f3 c = mk3(0, 0, 0);
+				int count = 0;
+				if (isset(x - 1, y)) {
+					c += isset(x - 2, y) ? lerp3(col(x - 2, y), col(x - 1, y), 2.0f) : col(x - 1, y);
+					count++;
+				}
+				if (isset(x + 1, y)) {
+					c += isset(x + 2, y) ? lerp3(col(x + 2, y), col(x + 1, y), 2.0f) : col(x + 1, y);
+					count++;
+				}
+				if (isset(x, y - 1)) {
+					c += isset(x, y - 2) ? lerp3(col(x, y - 2), col(x, y - 1), 2.0f) : col(x, y - 1);
+					count++;
+				}
+				if (isset(x, y + 1)) {
+					c += isset(x, y + 2) ? lerp3(col(x, y + 2), col(x, y + 1), 2.0f) : col(x, y + 1);
+					count++;
+				}
+				if (count) {
+					c = c / (float)count;
+					float* o = rgb + (x + (size_t)y * w) * 3;
+					o[0] = tmax(c.x, 0.0f);
+					o[1] = tmax(c.y, 0.0f);
+					o[2] = tmax(c.z, 0.0f);
+					set[x + (size_t)y * w + npx] = 1;
+				}
+			}
+		}
+		memcpy(set.data(), set.data() + npx, npx);
+	}
+	if (cfg.blur_size) {
+		int ext = (int)ceilf(cfg.blur_size);
+		size_t bufsz = ((size_t)std::max(w, h) + ext * 2) * 3;
+		std::vector<float> tmp(npx * 3 + bufsz + ext * 2 + 1);
+		float* img_tmp = tmp.data();
+		float* blur_tmp = img_tmp + npx * 3;
+		float* kernel = blur_tmp + bufsz;
+		gaussian_kernel(kernel, ext, cfg.blur_size);
+		convolve_transpose(rgb, img_tmp, w, h, ext, kernel, blur_tmp);
+		convolve_transpose(img_tmp, rgb, h, w, ext, kernel, blur_tmp);
+	}
+	if (cfg.ds2x) {
+		unsigned hw = std::max(w / 2, 1u), hh = std::max(h / 2, 1u);
+		float* ds = new float[(size_t)hw * hh * 3];
+		downsample2x(ds, hw, hh, rgb, w, h);
+		delete[] rgb;
+		rgb = ds;
+		mi->lm_w = hw;
+		mi->lm_h = hh;
+	}
+	float* normals = nullptr;
+	if (cfg.generate_normalmap_data) {
+		struct Sum {
+			f3 normal;
+			float mindot;
+			int32_t count;
+		};
+		std::vector<Sum> cs(mi->s_loc.size(), Sum{mk3(0, 0, 0), 1, 1});
+		float ab = brightness(ld3(cfg.ambient_color));
+		for (size_t i = 0; i < mi->s_nrm.size(); ++i)
+			cs[i].normal = mi->s_nrm[i] * ab;
+		for (const LightContrib& c : mi->contrib) {
+			cs[c.sid].normal += c.normal;
+			cs[c.sid].count++;
+		}
+		for (Sum& c : cs)
+			c.normal = c.normal / (float)c.count;
+		for (const LightContrib& c : mi->contrib) {
+			Sum& t = cs[c.sid];
+			f3 cn = c.normal;
+			float d = tmax(dot(normalized(t.normal), normalized(cn)), 0.0f);
+			d = lerpf(1.0f, d, tmin(length(cn) / length(t.normal), 1.0f));
+			t.mindot *= d;
+		}
+		size_t nn = (size_t)mi->lm_w * mi->lm_h;
+		normals = new float[nn * 4]();
+		for (size_t i = 0; i < mi->s_loc.size(); ++i) {
+			size_t loc = (size_t)mi->s_loc[i];
+			if (loc >= nn)
+				continue; // after ds2x the full-resolution texel index is out of range
+			f3 N = normalized(cs[i].normal);
+			normals[loc * 4 + 0] = N.x;
+			normals[loc * 4 + 1] = N.y;
+			normals[loc * 4 + 2] = N.z;
+			normals[loc * 4 + 3] = cs[i].mindot;
+		}
+	}
+	ltr_WorkOutput wo;
+	wo.uid = (u32)mid;
+	wo.mesh_ident = mi->mesh->ident.c_str();
+	wo.mesh_ident_size = mi->mesh->ident.size();
+	wo.inst_ident = mi->ident.c_str();
+	wo.inst_ident_size = mi->ident.size();
+	wo.lightmap_rgb = rgb;
+	wo.normals_xyzf = normals;
+	wo.width = mi->lm_w;
+	wo.height = mi->lm_h;
+	S->outputs.push_back(wo);
+}
+
+void finalize(ltr_Scene* S)
+{
+	for (size_t mid = 0; mid < S->insts.size(); ++mid) {
+		Instance* mi = S->insts[mid];
+		if (mi->samplecont) {
+			for (size_t i = 0; i < mi->lightmap.size(); ++i)
+				st3(S->samples[i].out_color, mi->lightmap[i]);
+			continue;
+		}
+		finalize_instance(S, mid);
+	}
+}
+
+// ---------------------------------------------------------------------------
+void bake(ltr_Scene* S)
+{
+	try {
+		if (!S->insts.empty()) {
+			prepare(S);
+			bool gpu = !S->lights.empty() || S->config.bounce_count || S->config.ao_distance;
+			if (gpu)
+				ensure_device(S);
+			if (!S->lights.empty()) {
+				set_stage(S, "rendering lightmaps");
+				stage_direct(S);
+			}
+			if (S->config.bounce_count) {
+				set_stage(S, "calculating radiosity");
+				stage_radiosity(S);
+			}
+			if (S->config.ao_distance) {
+				set_stage(S, "rendering ambient occlusion");
+				stage_ao(S);
+			}
+			if (gpu)
+				download_lightmaps(S);
+			set_stage(S, "exporting lightmaps");
+			finalize(S);
+		}
+		S->dev.reset();
+		S->completion.store(1.0f);
+		S->stage.store(nullptr);
+	} catch (const AbortBake&) {
+		S->dev.reset();
+		S->completion.store(1.0f);
+		S->stage.store(nullptr);
+	} catch (const std::exception& e) {
+		S->error = e.what();
+		fprintf(stderr, "lighter_amd: bake failed: %s\n", e.what());
+		S->dev.reset();
+		S->failed.store(true);
+		S->completion.store(0.0f);
+		S->stage.store(nullptr);
+	}
+}
+
+} // namespace lgx

@@ -1,0 +1,515 @@
+"""Seeded synthetic scenes for the baking workloads, and a builder that issues
+the corresponding ``ltr_*`` calls on any library exporting lighter's API.
+
+Scenes are produced with numpy's PCG64 generator (seed = 0x4C540000 + config
+index by default); libc ``rand`` is never used here, because the baker itself
+draws its AO rotation offsets from libc ``rand`` (lighter.cpp:819,1300).
+
+Geometry is laid out as planar rectangular "charts" (room walls, box faces,
+terrain patches), each tessellated into quads and packed into the instance's
+UV2 atlas with a shelf packer.  Triangles wind counter-clockwise around the
+shading normal.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses as dc
+import math
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import api
+
+SEED_BASE = 0x4C540000
+
+
+@dc.dataclass
+class Part:
+    positions: np.ndarray  # (V,3) float32
+    normals: np.ndarray  # (V,3) float32
+    tex1: np.ndarray  # (V,2) float32
+    tex2: np.ndarray  # (V,2) float32
+    indices: np.ndarray  # (I,) uint32
+    shadow: int = 1
+
+
+@dc.dataclass
+class Instance:
+    matrix: np.ndarray  # (4,4) float32, ltr_MAT4 layout (translation in row 3)
+    importance: float = 1.0
+    shadow: int = 1
+    ident: str = ""
+
+
+@dc.dataclass
+class Mesh:
+    ident: str
+    parts: list
+    instances: list
+
+
+@dc.dataclass
+class SceneSpec:
+    name: str
+    meshes: list
+    lights: list  # list of dicts of LightInfo fields
+    config: dict = dc.field(default_factory=dict)
+    lm_size: Optional[int] = None  # every instance gets lm_size x lm_size
+    albedo: Optional[dict] = None  # part_id -> rgb, applied through sample_fn
+    emission: Optional[dict] = None  # part_id -> rgb, applied through sample_fn
+    samples: list = dc.field(default_factory=list)  # external samples: (pos, nrm)
+
+    @property
+    def num_triangles(self) -> int:
+        n = 0
+        for m in self.meshes:
+            tri = sum(len(p.indices) // 3 for p in m.parts)
+            n += tri * len(m.instances)
+        return n
+
+
+IDENTITY = np.eye(4, dtype=np.float32)
+
+
+# --------------------------------------------------------------------------
+# charts and atlas packing
+# --------------------------------------------------------------------------
+@dc.dataclass
+class Chart:
+    origin: np.ndarray  # corner
+    du: np.ndarray  # edge along chart u
+    dv: np.ndarray  # edge along chart v
+    nu: int
+    nv: int
+    part: int
+    height: Optional[np.ndarray] = None  # (nv+1, nu+1) displacement along +z (terrain)
+
+    @property
+    def size_m(self):
+        return float(np.linalg.norm(self.du)), float(np.linalg.norm(self.dv))
+
+
+def _shelf_pack(sizes, W, H, pad):
+    """Shelf-pack rectangles (w, h) in pixels; returns corners or None."""
+    order = sorted(range(len(sizes)), key=lambda i: -sizes[i][1])
+    out = [None] * len(sizes)
+    x = y = pad
+    row_h = 0
+    for i in order:
+        w, h = sizes[i]
+        if x + w + pad > W:
+            x = pad
+            y += row_h + pad
+            row_h = 0
+        if x + w + pad > W or y + h + pad > H:
+            return None
+        out[i] = (x, y)
+        x += w + pad
+        row_h = max(row_h, h)
+    return out
+
+
+def pack_charts(charts, W, H, pad=2):
+    """Pick the largest texels-per-metre scale at which all charts fit."""
+    lo, hi = 0.0, 4096.0
+    best = None
+    for _ in range(40):
+        s = 0.5 * (lo + hi)
+        sizes = [(max(1, int(c.size_m[0] * s)), max(1, int(c.size_m[1] * s))) for c in charts]
+        pos = _shelf_pack(sizes, W, H, pad)
+        if pos is None:
+            hi = s
+        else:
+            lo = s
+            best = (pos, sizes)
+    assert best is not None, "charts do not fit the atlas"
+    return best
+
+
+def charts_to_part_arrays(charts, W, H, pad=2):
+    """Tessellate charts and assign UV2 atlas coordinates; returns per-part arrays."""
+    pos_rect, sizes = pack_charts(charts, W, H, pad)
+    by_part = {}
+    for c, (x0, y0), (wp, hp) in zip(charts, pos_rect, sizes):
+        a = np.linspace(0.0, 1.0, c.nu + 1, dtype=np.float64)
+        b = np.linspace(0.0, 1.0, c.nv + 1, dtype=np.float64)
+        A, B = np.meshgrid(a, b)  # (nv+1, nu+1)
+        P = c.origin[None, None, :] + A[..., None] * c.du[None, None, :] + B[..., None] * c.dv[None, None, :]
+        if c.height is not None:
+            P = P.copy()
+            P[..., 2] += c.height
+        n = np.cross(c.du, c.dv)
+        n = n / np.linalg.norm(n)
+        if c.height is not None:
+            # terrain normals from central differences of the displaced grid
+            gz_u = np.gradient(P[..., 2], axis=1) / (np.linalg.norm(c.du) / c.nu)
+            gz_v = np.gradient(P[..., 2], axis=0) / (np.linalg.norm(c.dv) / c.nv)
+            N = np.stack([-gz_u, -gz_v, np.ones_like(gz_u)], -1)
+            N /= np.linalg.norm(N, axis=-1, keepdims=True)
+        else:
+            N = np.broadcast_to(n, P.shape)
+        T1 = np.stack([A, B], -1)
+        T2 = np.stack([(x0 + A * wp) / W, (y0 + B * hp) / H], -1)
+        verts = (P.reshape(-1, 3), N.reshape(-1, 3), T1.reshape(-1, 2), T2.reshape(-1, 2))
+        iu, iv = np.meshgrid(np.arange(c.nu), np.arange(c.nv))
+        v00 = (iv * (c.nu + 1) + iu).ravel()
+        v10 = v00 + 1
+        v01 = v00 + (c.nu + 1)
+        v11 = v01 + 1
+        idx = np.stack([v00, v10, v11, v11, v01, v00], -1).ravel()
+        lst = by_part.setdefault(c.part, [])
+        lst.append((verts, idx))
+    parts = {}
+    for pid, lst in by_part.items():
+        P, N, T1, T2, I = [], [], [], [], []
+        base = 0
+        for (p, n, t1, t2), idx in lst:
+            P.append(p); N.append(n); T1.append(t1); T2.append(t2)
+            I.append(idx + base)
+            base += len(p)
+        parts[pid] = Part(
+            np.ascontiguousarray(np.concatenate(P), dtype=np.float32),
+            np.ascontiguousarray(np.concatenate(N), dtype=np.float32),
+            np.ascontiguousarray(np.concatenate(T1), dtype=np.float32),
+            np.ascontiguousarray(np.concatenate(T2), dtype=np.float32),
+            np.ascontiguousarray(np.concatenate(I), dtype=np.uint32),
+        )
+    return [parts[k] for k in sorted(parts)]
+
+
+def box_charts(lo, hi, part, with_bottom=True):
+    """Six outward-facing faces of an axis-aligned box, one quad each."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    dx, dy, dz = hi - lo
+    X = np.array([dx, 0, 0]); Y = np.array([0, dy, 0]); Z = np.array([0, 0, dz])
+    ch = [
+        Chart(lo + Z, X, Y, 1, 1, part),  # top, +z
+        Chart(lo, X, Z, 1, 1, part),  # -y side
+        Chart(lo + Y + X, -X, Z, 1, 1, part),  # +y side
+        Chart(lo + Y, -Y, Z, 1, 1, part),  # -x side
+        Chart(lo + X, Y, Z, 1, 1, part),  # +x side
+    ]
+    if with_bottom:
+        ch.append(Chart(lo + Y, X, -Y, 1, 1, part))  # bottom, -z
+    return ch
+
+
+def room_charts(size=(10.0, 10.0, 4.0), tess=8, part=0):
+    """Six inward-facing walls of a room [0,sx]x[0,sy]x[0,sz]."""
+    sx, sy, sz = size
+    X = np.array([sx, 0, 0.0]); Y = np.array([0, sy, 0.0]); Z = np.array([0, 0, sz])
+    o = np.zeros(3)
+    return [
+        Chart(o, X, Y, tess, tess, part),  # floor, +z
+        Chart(o + Z + Y, X, -Y, tess, tess, part),  # ceiling, -z
+        Chart(o, Y, Z, tess, tess, part),  # x=0, +x
+        Chart(o + X + Y, -Y, Z, tess, tess, part),  # x=sx, -x
+        Chart(o + X, -X, Z, tess, tess, part),  # y=0, +y
+        Chart(o + Y, X, Z, tess, tess, part),  # y=sy, -y
+    ]
+
+
+def _check_charts_normals(charts, want):
+    for c, w in zip(charts, want):
+        n = np.cross(c.du, c.dv)
+        assert np.dot(n, w) > 0, (c, w)
+
+
+# --------------------------------------------------------------------------
+# lights
+# --------------------------------------------------------------------------
+def point_light(pos, color, rng_, power=1.0, radius=0.1, samples=4):
+    return dict(type=api.LTR_LT_POINT, position=list(map(float, pos)), color_rgb=list(map(float, color)),
+                range=float(rng_), power=float(power), light_radius=float(radius), shadow_sample_count=samples)
+
+
+def spot_light(pos, direction, color, rng_, angle_out, angle_in, curve, power=1.0, radius=0.1, samples=4):
+    return dict(type=api.LTR_LT_SPOT, position=list(map(float, pos)), direction=list(map(float, direction)),
+                up_direction=[1.0, 0.0, 0.0], color_rgb=list(map(float, color)), range=float(rng_),
+                power=float(power), light_radius=float(radius), shadow_sample_count=samples,
+                spot_angle_out=float(angle_out), spot_angle_in=float(angle_in), spot_curve=float(curve))
+
+
+def dir_light(direction, color, rng_, radius=0.1, samples=1):
+    return dict(type=api.LTR_LT_DIRECT, direction=list(map(float, direction)), color_rgb=list(map(float, color)),
+                range=float(rng_), power=1.0, light_radius=float(radius), shadow_sample_count=samples)
+
+
+# --------------------------------------------------------------------------
+# configurations
+# --------------------------------------------------------------------------
+def room_scene(seed=SEED_BASE + 0, lm=256, n_boxes=20, wall_tess=8, n_point=2, with_sun=True,
+               bounce_count=1, ao_distance=0.0, ao_samples=17, blur_size=0.5, room=(10.0, 10.0, 4.0),
+               name="C0"):
+    """Room 10x10x4 m, 6 inward walls of wall_tess^2 quads, n_boxes axis-aligned
+    boxes standing on the floor; one instance with one lm x lm atlas."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    sx, sy, sz = room
+    charts = room_charts(room, wall_tess, part=0)
+    albedo = {0: (0.8, 0.8, 0.8)}
+    for b in range(n_boxes):
+        size = g.uniform(0.2, 1.5, 3)
+        cx = g.uniform(size[0] / 2, sx - size[0] / 2)
+        cy = g.uniform(size[1] / 2, sy - size[1] / 2)
+        lo = np.array([cx - size[0] / 2, cy - size[1] / 2, 0.0])
+        hi = lo + size
+        charts += box_charts(lo, hi, part=1 + b)
+        albedo[1 + b] = tuple(float(x) for x in g.uniform(0.2, 0.9, 3))
+    parts = charts_to_part_arrays(charts, lm, lm)
+    mesh = Mesh("room", parts, [Instance(IDENTITY.copy(), 1.0, 1, "room0")])
+    lights = []
+    for _ in range(n_point):
+        p = (g.uniform(1, sx - 1), g.uniform(1, sy - 1), g.uniform(2.5, 3.5))
+        lights.append(point_light(p, g.uniform(0.5, 1.0, 3), 8.0, radius=0.1, samples=4))
+    if with_sun:
+        d = np.array([g.uniform(-1, 1), g.uniform(-1, 1), g.uniform(0.5, 1.0)])
+        diag = math.sqrt(sx * sx + sy * sy + sz * sz)
+        lights.append(dir_light(d / np.linalg.norm(d), g.uniform(0.5, 1.0, 3), diag, radius=0.1, samples=4))
+    cfg = dict(bounce_count=bounce_count, ao_distance=ao_distance, ao_num_samples=ao_samples,
+               blur_size=blur_size, max_num_threads=1)
+    return SceneSpec(name, [mesh], lights, cfg, lm_size=lm, albedo=albedo if bounce_count else None)
+
+
+def value_noise(g, n, octaves=5, base_cells=4):
+    """Fractal value noise on an (n+1)^2 vertex grid, range about [-1,1]."""
+    out = np.zeros((n + 1, n + 1))
+    amp, total = 1.0, 0.0
+    cells = base_cells
+    coords = np.linspace(0.0, 1.0, n + 1)
+    for _ in range(octaves):
+        lat = g.uniform(-1.0, 1.0, (cells + 1, cells + 1))
+        f = coords * cells
+        i0 = np.minimum(f.astype(int), cells - 1)
+        t = f - i0
+        t = t * t * (3 - 2 * t)
+        a = lat[i0][:, i0]
+        b = lat[i0][:, i0 + 1]
+        c = lat[i0 + 1][:, i0]
+        d = lat[i0 + 1][:, i0 + 1]
+        ty = t[:, None]
+        tx = t[None, :]
+        out += amp * ((a * (1 - tx) + b * tx) * (1 - ty) + (c * (1 - tx) + d * tx) * ty)
+        total += amp
+        amp *= 0.5
+        cells *= 2
+    return out / total
+
+
+def terrain_scene(seed=SEED_BASE + 1, grid=512, extent=200.0, amplitude=20.0, n_boxes=2000, tiles=16,
+                  lm_total=2048, n_lights=32, sun=True, sun_range=300.0, name="C1",
+                  box_size=(0.5, 4.0), light_range=(10.0, 40.0)):
+    """Heightfield terrain (grid^2 quads over extent m) split into tiles x tiles
+    meshes, plus n_boxes boxes merged into the tile under their centre; each
+    tile has one instance with an (lm_total/tiles)^2 atlas."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    H = value_noise(g, grid) * amplitude
+    cell = extent / grid
+    q = grid // tiles
+    lm = lm_total // tiles
+    tile_charts = [[[] for _ in range(tiles)] for _ in range(tiles)]
+    for ty in range(tiles):
+        for tx in range(tiles):
+            o = np.array([tx * q * cell, ty * q * cell, 0.0])
+            h = H[ty * q: ty * q + q + 1, tx * q: tx * q + q + 1]
+            tile_charts[ty][tx].append(Chart(o, np.array([q * cell, 0, 0]), np.array([0, q * cell, 0]), q, q, 0,
+                                             height=h.copy()))
+
+    def height_at(x, y):
+        fx, fy = x / cell, y / cell
+        i, j = min(int(fx), grid - 1), min(int(fy), grid - 1)
+        return H[j, i]
+
+    for b in range(n_boxes):
+        size = g.uniform(box_size[0], box_size[1], 3)
+        cx = g.uniform(size[0], extent - size[0])
+        cy = g.uniform(size[1], extent - size[1])
+        z0 = height_at(cx, cy) - 0.3
+        lo = np.array([cx - size[0] / 2, cy - size[1] / 2, z0])
+        tx = min(int(cx / (q * cell)), tiles - 1)
+        ty = min(int(cy / (q * cell)), tiles - 1)
+        tile_charts[ty][tx] += box_charts(lo, lo + size, part=1, with_bottom=True)
+    meshes = []
+    for ty in range(tiles):
+        for tx in range(tiles):
+            parts = charts_to_part_arrays(tile_charts[ty][tx], lm, lm)
+            ident = f"tile_{tx}_{ty}"
+            meshes.append(Mesh(ident, parts, [Instance(IDENTITY.copy(), 1.0, 1, ident)]))
+    lights = []
+    for i in range(n_lights):
+        x, y = g.uniform(5, extent - 5, 2)
+        z = height_at(x, y) + g.uniform(2.0, 8.0)
+        col = g.uniform(0.3, 1.0, 3)
+        rng_ = g.uniform(*light_range)
+        if i % 2 == 0:
+            lights.append(point_light((x, y, z), col, rng_, radius=0.5, samples=16))
+        else:
+            d = np.array([g.uniform(-0.5, 0.5), g.uniform(-0.5, 0.5), -1.0])
+            lights.append(spot_light((x, y, z), d / np.linalg.norm(d), col, rng_, 60.0, 40.0, 1.0,
+                                     radius=0.5, samples=16))
+    if sun:
+        d = np.array([0.35, -0.25, 0.9])
+        lights.append(dir_light(d / np.linalg.norm(d), (1.0, 0.95, 0.9), sun_range, radius=0.5, samples=8))
+    cfg = dict(bounce_count=0, ao_distance=0.0, max_num_threads=1)
+    return SceneSpec(name, meshes, lights, cfg, lm_size=lm)
+
+
+def basic_scene():
+    """lighter_test.cpp:183-257: a quad with three instances and one point light."""
+    pos = np.array([-1, -1, 0, 1, -1, 0, 1, 1, 0, -1, 1, 0], np.float32).reshape(4, 3)
+    nrm = np.tile(np.array([0, 0, 1], np.float32), (4, 1))
+    tex = np.array([0.1, 0.1, 0.9, 0.1, 0.9, 0.9, 0.1, 0.9], np.float32).reshape(4, 2)
+    idx = np.array([0, 1, 2, 2, 3, 0], np.uint32)
+    part = Part(pos, nrm, tex, tex.copy(), idx, 1)
+    c, s = math.cos(0.5) * 0.2, math.sin(0.5) * 0.2
+    m1 = np.eye(4, dtype=np.float32)
+    m2 = np.array([[c, s, 0, 0], [-s, c, 0, 0], [0, 0, 0.2, 0], [0, 0, 0.5, 1]], np.float32)
+    m3 = np.array([[1.9, 0, 0, 0], [0, 1.9, 0, 0], [0, 0, 1.9, 0], [0, 0, -0.5, 1]], np.float32)
+    mesh = Mesh("mesh1", [part], [Instance(m, 1.0, 1, "") for m in (m1, m2, m3)])
+    lights = [point_light((-0.4, -0.4, 1.0), (0.9, 0.1, 0.05), 4.0, 1.0, 0.1, 9)]
+    return SceneSpec("basic", [mesh], lights, dict(max_num_threads=1))
+
+
+# --------------------------------------------------------------------------
+# building a scene through the ltr_* API
+# --------------------------------------------------------------------------
+class BuiltScene:
+    """Owns the ctypes callbacks and keeps the scene pointer."""
+
+    def __init__(self, lib, spec: SceneSpec, config_overrides: Optional[dict] = None):
+        self.lib = lib
+        self.spec = spec
+        self.keep = []
+        self.scene = lib.ltr_CreateScene()
+        cfg = api.Config()
+        lib.ltr_GetConfig(C.byref(cfg), self.scene)
+        overrides = dict(spec.config)
+        overrides.update(config_overrides or {})
+        for k, v in overrides.items():
+            if isinstance(getattr(cfg, k), C.Array):
+                for i, x in enumerate(v):
+                    getattr(cfg, k)[i] = x
+            else:
+                setattr(cfg, k, v)
+        if spec.lm_size:
+            # ltr_DefaultSizeFunc refuses any size above max_lightmap_size, and
+            # the bake then uses default_width/height (lighter.cpp:19-27,325-330)
+            cfg.default_width = spec.lm_size
+            cfg.default_height = spec.lm_size
+            cfg.max_lightmap_size = 0
+        if spec.albedo is not None or spec.emission is not None:
+            alb = spec.albedo or {}
+            emi = spec.emission or {}
+
+            def sample_fn(cfgp, req):
+                r = req.contents
+                a = alb.get(r.part_id)
+                if a is not None:
+                    r.out_diffuse_color[0], r.out_diffuse_color[1], r.out_diffuse_color[2] = a
+                e = emi.get(r.part_id)
+                if e is not None:
+                    r.out_emissive_color[0], r.out_emissive_color[1], r.out_emissive_color[2] = e
+                return 1
+
+            cb = api.SAMPLE_FN(sample_fn)
+            self.keep.append(cb)
+            cfg.sample_fn = C.cast(cb, C.c_void_p).value
+        self.config = cfg
+        lib.ltr_SetConfig(self.scene, C.byref(cfg))
+        for m in spec.meshes:
+            mp = lib.ltr_CreateMesh(self.scene, m.ident.encode(), len(m.ident))
+            for p in m.parts:
+                info = api.MeshPartInfo(
+                    p.positions.ctypes.data, p.normals.ctypes.data, p.tex1.ctypes.data, p.tex2.ctypes.data,
+                    12, 12, 8, 8, p.indices.ctypes.data_as(C.POINTER(C.c_uint32)),
+                    len(p.positions), len(p.indices), p.shadow)
+                assert lib.ltr_MeshAddPart(mp, C.byref(info)) == 1
+            for inst in m.instances:
+                ii = api.MeshInstanceInfo()
+                for r in range(4):
+                    for c_ in range(4):
+                        ii.matrix[r][c_] = float(inst.matrix[r, c_])
+                ii.importance = inst.importance
+                ii.shadow = inst.shadow
+                ident = inst.ident.encode()
+                self.keep.append(ident)
+                ii.ident = ident
+                ii.ident_size = len(ident)
+                assert lib.ltr_MeshAddInstance(mp, C.byref(ii)) == 1
+        for L in spec.lights:
+            li = api.LightInfo()
+            for k, v in L.items():
+                if isinstance(getattr(li, k), C.Array):
+                    for i, x in enumerate(v):
+                        getattr(li, k)[i] = x
+                else:
+                    setattr(li, k, v)
+            lib.ltr_LightAdd(self.scene, C.byref(li))
+        for k, (p, n) in enumerate(spec.samples):
+            si = api.SampleInfo()
+            si.id = k
+            for i in range(3):
+                si.position[i] = p[i]
+                si.normal[i] = n[i]
+            lib.ltr_SampleAdd(self.scene, C.byref(si))
+
+    def bake(self, poll_ms: int = 1, status_cb: Optional[Callable] = None):
+        """ltr_Start + ltr_GetStatus polling; returns {stage: seconds}."""
+        import time
+        lib = self.lib
+        st = api.WorkStatus()
+        times = {}
+        t_last = time.perf_counter()
+        lib.ltr_Start(self.scene)
+        cur = None
+        while True:
+            busy = lib.ltr_GetStatus(self.scene, C.byref(st))
+            now = time.perf_counter()
+            stage = st.stage.decode() if st.stage else None
+            if cur is not None:
+                times[cur] = times.get(cur, 0.0) + (now - t_last)
+            t_last = now
+            cur = stage
+            if status_cb:
+                status_cb(stage, st.completion)
+            if not busy:
+                break
+            time.sleep(poll_ms / 1000.0)
+        return times
+
+    def outputs(self):
+        """List of (uid, width, height, rgb (H,W,3) copy, normals or None)."""
+        lib = self.lib
+        info = api.WorkOutputInfo()
+        lib.ltr_GetWorkOutputInfo(self.scene, C.byref(info))
+        res = []
+        for k in range(info.lightmap_count):
+            wo = api.WorkOutput()
+            assert lib.ltr_GetWorkOutput(self.scene, k, C.byref(wo)) == 1
+            n = wo.width * wo.height
+            rgb = np.ctypeslib.as_array(wo.lightmap_rgb, (n * 3,)).reshape(wo.height, wo.width, 3).copy()
+            nm = None
+            if wo.normals_xyzf:
+                nm = np.ctypeslib.as_array(wo.normals_xyzf, (n * 4,)).reshape(wo.height, wo.width, 4).copy()
+            res.append((wo.uid, wo.width, wo.height, rgb, nm))
+        return res
+
+    def sample_outputs(self):
+        info = api.WorkOutputInfo()
+        self.lib.ltr_GetWorkOutputInfo(self.scene, C.byref(info))
+        out = np.zeros((info.sample_count, 3), np.float32)
+        for i in range(info.sample_count):
+            out[i] = info.samples[i].out_color[:]
+        return out
+
+    def destroy(self):
+        if self.scene:
+            self.lib.ltr_DestroyScene(self.scene)
+            self.scene = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

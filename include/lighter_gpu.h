@@ -102,6 +102,11 @@ LTRAPI int lgx_ReadLightmap(lgx_Device* dev, float* lightmap3);
  * At most 64 lights per call; contributions are added in light order. */
 LTRAPI int lgx_DirectLight(lgx_Device* dev, const lgx_Light* lights, int nlights, const uint32_t* items,
                            size_t nitems, const uint32_t* light_off, int keep_vis);
+/* The same in two steps: upload the work items once, then run them (each run
+ * adds the lights' contributions to the lightmap again). */
+LTRAPI int lgx_DirectLightSetup(lgx_Device* dev, const lgx_Light* lights, int nlights, const uint32_t* items,
+                                size_t nitems, const uint32_t* light_off, int keep_vis);
+LTRAPI int lgx_DirectLightRun(lgx_Device* dev);
 /* Per item: weight multiplying the light colour (bits 0xFFFFFFFF where the
  * reference skips the texel) and, if kept, the shadow factor. */
 LTRAPI int lgx_ReadWeights(lgx_Device* dev, float* weights, float* vis, size_t n);
@@ -117,10 +122,10 @@ LTRAPI int lgx_RadReadLinks(lgx_Device* dev, uint32_t* rowmap, uint32_t* other, 
  * light E; total3 receives the accumulated light per patch. */
 LTRAPI int lgx_RadBounce(lgx_Device* dev, const float* diffuse3, const float* emit3, const float* area, int bounces,
                          float* total3);
-/* Traversal counters (node visits, triangle tests, march steps, queries)
- * collected by the next stage when enabled. */
+/* Traversal counters (node visits, triangle tests, march steps, queries,
+ * marches) collected by the next stage when enabled. */
 LTRAPI int lgx_SetCounting(lgx_Device* dev, int on);
-LTRAPI int lgx_ReadCounters(lgx_Device* dev, uint64_t out[4]);
+LTRAPI int lgx_ReadCounters(lgx_Device* dev, uint64_t out[5]);
 /* Last stage times in ms: direct, direct march, AO, links, bounces. */
 LTRAPI int lgx_DeviceTimes(lgx_Device* dev, double out[5]);
 

@@ -541,6 +541,17 @@ int lgx_ReadLightmap(lgx_Device* d, float* lm)
 int lgx_DirectLight(lgx_Device* d, const lgx_Light* lights, int nlights, const uint32_t* items, size_t nitems,
                     const uint32_t* light_off, int keep_vis)
 {
+	return lgx_DirectLightSetup(d, lights, nlights, items, nitems, light_off, keep_vis) && lgx_DirectLightRun(d);
+}
+
+int lgx_DirectLightRun(lgx_Device* d)
+{
+	return guarded([&] { d->dev.run_direct(); });
+}
+
+int lgx_DirectLightSetup(lgx_Device* d, const lgx_Light* lights, int nlights, const uint32_t* items, size_t nitems,
+                         const uint32_t* light_off, int keep_vis)
+{
 	return guarded([&] {
 		if (nlights < 0 || nlights > 64)
 			throw std::runtime_error("lgx_DirectLight: 0..64 lights per call");
@@ -565,7 +576,7 @@ int lgx_DirectLight(lgx_Device* d, const lgx_Light* lights, int nlights, const u
 		for (uint32_t v : it)
 			if ((v >> 26) >= (uint32_t)nlights || (v & ((1u << 26) - 1)) >= d->dev.num_samples())
 				throw std::runtime_error("lgx_DirectLight: work item out of range");
-		d->dev.direct_light(dl, it, off, keep_vis != 0);
+		d->dev.set_direct_work(dl, it, off, keep_vis != 0);
 	});
 }
 
@@ -619,12 +630,13 @@ int lgx_SetCounting(lgx_Device* d, int on)
 	return 1;
 }
 
-int lgx_ReadCounters(lgx_Device* d, uint64_t out[4])
+int lgx_ReadCounters(lgx_Device* d, uint64_t out[5])
 {
 	out[0] = d->dev.counters.nodes;
 	out[1] = d->dev.counters.tris;
 	out[2] = d->dev.counters.steps;
 	out[3] = d->dev.counters.queries;
+	out[4] = d->dev.counters.marches;
 	return 1;
 }
 

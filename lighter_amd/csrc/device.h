@@ -32,7 +32,7 @@ struct StageTimes {
 // Optional traversal counters (when enabled): node visits and triangle
 // tests, for the algorithmic-bytes figure.
 struct QueryCounters {
-	unsigned long long nodes = 0, tris = 0, steps = 0, queries = 0;
+	unsigned long long nodes = 0, tris = 0, steps = 0, queries = 0, marches = 0;
 };
 
 class DeviceScene {
@@ -60,6 +60,10 @@ public:
 	// the texel); the accumulation adds color * weight per light in order.
 	void direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
 	                  const std::vector<uint32_t>& light_off, bool keep_weights);
+	// the same in two steps: upload the work once, run it (repeatedly)
+	void set_direct_work(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
+	                     const std::vector<uint32_t>& light_off, bool keep_weights);
+	void run_direct();
 	const float* weights_device() const { return d_w_; }
 	void get_weights(float* out, size_t n) const;
 	// shadow factor per item (1 where no march ran); kept when keep_weights
@@ -117,6 +121,7 @@ public:
 
 private:
 	void free_all();
+	void read_counters();
 	void* stream_ = nullptr;
 	// BVH and triangles
 	void* d_nodes_ = nullptr;
@@ -133,6 +138,10 @@ private:
 	// direct light
 	float* d_w_ = nullptr;
 	float* d_vis_ = nullptr;
+	uint32_t* d_items_ = nullptr;
+	DevLight* d_lights_ = nullptr;
+	std::vector<DevLight> h_lights_;
+	std::vector<uint32_t> h_light_off_;
 	size_t nitems_ = 0;
 	size_t nlinks_ = 0;
 	unsigned long long* d_counters_ = nullptr;

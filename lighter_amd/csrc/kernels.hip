@@ -36,6 +36,7 @@ namespace lgx {
 
 constexpr int kStackSize = 64;
 constexpr int kBlock = 256;
+constexpr int kNumCounters = 5;
 constexpr uint32_t kSkipBits = 0xFFFFFFFFu; // weight of a texel the reference skips
 constexpr uint32_t kSampleBits = 26;
 constexpr uint32_t kSampleMask = (1u << kSampleBits) - 1;
@@ -322,8 +323,10 @@ __device__ float shadow_march(const View& V, f3 from, f3 to, float k, unsigned l
 		h = tmin(h, kPenumbraStep);
 		t += h;
 	}
-	if (COUNT)
+	if (COUNT) {
 		atomicAdd(cnt + 2, (unsigned long long)steps);
+		atomicAdd(cnt + 4, 1ull);
+	}
 	if (nsteps)
 		*nsteps = steps;
 	return res;
@@ -711,7 +714,7 @@ DeviceScene::DeviceScene()
 	hipStream_t s;
 	HIPCHECK(hipStreamCreate(&s));
 	stream_ = s;
-	d_counters_ = dalloc<unsigned long long>(4);
+	d_counters_ = dalloc<unsigned long long>(kNumCounters);
 }
 
 DeviceScene::~DeviceScene()
@@ -732,6 +735,8 @@ void DeviceScene::free_all()
 	dfree(d_lm_);
 	dfree(d_w_);
 	dfree(d_vis_);
+	dfree(d_items_);
+	dfree(d_lights_);
 	dfree(rad.pos);
 	dfree(rad.nrm);
 	dfree(rad.bits);
@@ -853,57 +858,80 @@ struct EventTimer {
 	}
 };
 
-void DeviceScene::direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
-                               const std::vector<uint32_t>& light_off, bool keep_weights)
+void DeviceScene::set_direct_work(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
+                                  const std::vector<uint32_t>& light_off, bool keep_weights)
 {
 	hipStream_t s = (hipStream_t)stream_;
 	if (lights.size() > 64)
 		throw std::runtime_error("direct_light: at most 64 lights per batch");
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	if (light_off.size() != lights.size() + 1 || light_off.back() != items.size())
+		throw std::runtime_error("direct_light: light offsets do not match the work items");
 	dfree(d_w_);
 	dfree(d_vis_);
+	dfree(d_items_);
+	dfree(d_lights_);
 	nitems_ = items.size();
 	d_w_ = dalloc<float>(nitems_);
 	if (keep_weights)
 		d_vis_ = dalloc<float>(nitems_);
-	uint32_t* d_items = dalloc<uint32_t>(nitems_);
-	DevLight* d_lights = dalloc<DevLight>(lights.size());
-	HIPCHECK(hipMemcpyAsync(d_items, items.data(), 4 * nitems_, hipMemcpyHostToDevice, s));
-	HIPCHECK(hipMemcpyAsync(d_lights, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice, s));
-	HIPCHECK(hipMemsetAsync(d_counters_, 0, 4 * sizeof(unsigned long long), s));
+	d_items_ = dalloc<uint32_t>(nitems_);
+	d_lights_ = dalloc<DevLight>(lights.size());
+	HIPCHECK(hipMemcpyAsync(d_items_, items.data(), 4 * nitems_, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_lights_, lights.data(), sizeof(DevLight) * lights.size(), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	h_lights_ = lights;
+	h_light_off_ = light_off;
+}
+
+void DeviceScene::run_direct()
+{
+	hipStream_t s = (hipStream_t)stream_;
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	HIPCHECK(hipMemsetAsync(d_counters_, 0, kNumCounters * sizeof(unsigned long long), s));
 	EventTimer all(s);
 	{
 		EventTimer tm(s);
 		if (nitems_) {
 			if (count_queries)
-				k_march<true><<<grid_for(nitems_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights, d_items,
+				k_march<true><<<grid_for(nitems_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_,
 				                                                   (uint32_t)nitems_, d_w_, d_vis_, d_counters_);
 			else
-				k_march<false><<<grid_for(nitems_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights, d_items,
+				k_march<false><<<grid_for(nitems_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_,
 				                                                    (uint32_t)nitems_, d_w_, d_vis_, d_counters_);
 			HIPCHECK(hipGetLastError());
 		}
 		times.direct_march_ms = tm.stop();
 	}
-	for (size_t l = 0; l < lights.size(); ++l) {
-		uint32_t b = light_off[l], e = light_off[l + 1];
+	for (size_t l = 0; l < h_lights_.size(); ++l) {
+		uint32_t b = h_light_off_[l], e = h_light_off_[l + 1];
 		if (e > b) {
-			k_accumulate<<<grid_for(e - b), kBlock, 0, s>>>(d_items, d_w_, b, e, lights[l].color[0],
-			                                                lights[l].color[1], lights[l].color[2], d_lm_);
+			k_accumulate<<<grid_for(e - b), kBlock, 0, s>>>(d_items_, d_w_, b, e, h_lights_[l].color[0],
+			                                                h_lights_[l].color[1], h_lights_[l].color[2], d_lm_);
 			HIPCHECK(hipGetLastError());
 		}
 	}
 	times.direct_ms = all.stop();
-	if (count_queries) {
-		unsigned long long c[4];
-		HIPCHECK(hipMemcpy(c, d_counters_, sizeof(c), hipMemcpyDeviceToHost));
-		counters.nodes = c[0];
-		counters.tris = c[1];
-		counters.steps = c[2];
-		counters.queries = c[3];
-	}
-	dfree(d_items);
-	dfree(d_lights);
+	read_counters();
+}
+
+void DeviceScene::read_counters()
+{
+	if (!count_queries)
+		return;
+	unsigned long long c[kNumCounters];
+	HIPCHECK(hipMemcpy(c, d_counters_, sizeof(c), hipMemcpyDeviceToHost));
+	counters.nodes = c[0];
+	counters.tris = c[1];
+	counters.steps = c[2];
+	counters.queries = c[3];
+	counters.marches = c[4];
+}
+
+void DeviceScene::direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
+                               const std::vector<uint32_t>& light_off, bool keep_weights)
+{
+	set_direct_work(lights, items, light_off, keep_weights);
+	run_direct();
 }
 
 void DeviceScene::get_weights(float* out, size_t n) const
@@ -941,7 +969,7 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 	float* d_ro = dalloc<float>(nsamples_);
 	HIPCHECK(hipMemcpyAsync(d_side, side.data(), sizeof(float2) * num_samples, hipMemcpyHostToDevice, s));
 	HIPCHECK(hipMemcpyAsync(d_ro, randoff, 4 * nsamples_, hipMemcpyHostToDevice, s));
-	HIPCHECK(hipMemsetAsync(d_counters_, 0, 4 * sizeof(unsigned long long), s));
+	HIPCHECK(hipMemsetAsync(d_counters_, 0, kNumCounters * sizeof(unsigned long long), s));
 	AOParams P;
 	P.n = num_samples;
 	P.distance = distance;
@@ -960,14 +988,7 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 		                                                   d_lm_, d_counters_);
 	HIPCHECK(hipGetLastError());
 	times.ao_ms = tm.stop();
-	if (count_queries) {
-		unsigned long long c[4];
-		HIPCHECK(hipMemcpy(c, d_counters_, sizeof(c), hipMemcpyDeviceToHost));
-		counters.nodes = c[0];
-		counters.tris = c[1];
-		counters.steps = c[2];
-		counters.queries = c[3];
-	}
+	read_counters();
 	dfree(d_side);
 	dfree(d_ro);
 }

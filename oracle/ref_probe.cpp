@@ -36,6 +36,56 @@ int rp_prepare(ltr_Scene* S, int nthreads)
 	return 1;
 }
 
+// PreXForm, ColInfo and the instance tree only: the collision structures the
+// scene queries need, without sample placement.
+int rp_prepare_collision(ltr_Scene* S, int nthreads)
+{
+	if (S->m_meshInstances.empty())
+		return 0;
+	S->m_worker.Init(nthreads < 1 ? 1 : nthreads);
+	S->m_worker.DoWork(S, NULL, 0, S->m_meshInstances.size(), ltr_Scene::Job_PreXForm);
+	S->m_worker.DoWork(S, NULL, 0, S->m_meshInstances.size(), ltr_Scene::Job_ColInfo);
+	std::vector<AABB3> bbs;
+	for (size_t i = 0; i < S->m_meshInstances.size(); ++i) {
+		AABBTree::Node& N = S->m_meshInstances[i]->m_triTree.m_bbTree.m_nodes[0];
+		AABB3 bb = {N.bbmin, N.bbmax};
+		bbs.push_back(bb);
+	}
+	S->m_instTree.SetAABBs(&bbs[0], bbs.size());
+	return 1;
+}
+
+// CalcInvShadowFactor over a batch on nthreads threads (contiguous chunks),
+// for timing the reference's shadow march on the host cores.
+struct ShadowJob {
+	ltr_Scene* S;
+	const float *from, *to, *k;
+	float* out;
+	size_t b, e;
+};
+static void* shadow_job(void* p)
+{
+	ShadowJob* j = (ShadowJob*)p;
+	for (size_t i = j->b; i < j->e; ++i)
+		j->out[i] = j->S->CalcInvShadowFactor(V3P(j->from + 3 * i), V3P(j->to + 3 * i), j->k[i]);
+	return NULL;
+}
+void rp_shadow_factor_batch_mt(ltr_Scene* S, size_t n, const float* from, const float* to, const float* k,
+                               float* out, int nthreads)
+{
+	if (nthreads < 1)
+		nthreads = 1;
+	std::vector<pthread_t> th(nthreads);
+	std::vector<ShadowJob> jobs(nthreads);
+	for (int t = 0; t < nthreads; ++t) {
+		ShadowJob j = {S, from, to, k, out, n * t / nthreads, n * (t + 1) / nthreads};
+		jobs[t] = j;
+		pthread_create(&th[t], NULL, shadow_job, &jobs[t]);
+	}
+	for (int t = 0; t < nthreads; ++t)
+		pthread_join(th[t], NULL);
+}
+
 // Instance ids a light is rendered into, in render order (lighter.cpp:1080-1093).
 int rp_light_instances(ltr_Scene* S, int light, int32_t* out, int cap)
 {

@@ -1,0 +1,315 @@
+"""Pinning the C oracle (CPU, no GPU).
+
+Two kinds of checks:
+* against the reference compiled from its own sources (oracle/_ref), bit for
+  bit, on seeded inputs (tests marked `ref`);
+* against what the mathematics fixes, independent of any restatement:
+  FP64 segment/triangle intersection and closest-point distances, closed
+  forms of the shadow march, spiral moments, the radiosity series as FP64
+  matrix powers.  Each is chosen so that a dropped term, a wrong sign or a
+  swapped operand in the oracle fails it.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import all_samples_and_tris, bits, prepared
+from lighter_amd import scenes
+from oracle.checkers import normalize_f32, orc_light_from_info
+
+fp = C.POINTER(C.c_float)
+
+
+def _p(a):
+    return np.ascontiguousarray(a, np.float32).ctypes.data_as(fp)
+
+
+# ---------------------------------------------------------------------------
+# closed forms / FP64 geometry
+# ---------------------------------------------------------------------------
+def _seg_tri_f64(l1, l2, p1, p2, p3):
+    l1, l2, p1, p2, p3 = (np.asarray(x, np.float64) for x in (l1, l2, p1, p2, p3))
+    d = l2 - l1
+    e1, e2 = p2 - p1, p3 - p1
+    M = np.stack([-d, e1, e2], 1)
+    try:
+        r, s, t = np.linalg.solve(M, l1 - p1)
+    except np.linalg.LinAlgError:
+        return None
+    return r, s, t
+
+
+def test_seg_tri_matches_fp64_solution(orc):
+    rng = np.random.default_rng(10)
+    n_hit = n_checked = 0
+    for _ in range(4000):
+        p = rng.uniform(-1, 1, (3, 3))
+        l1 = rng.uniform(-2, 2, 3)
+        l2 = rng.uniform(-2, 2, 3)
+        sol = _seg_tri_f64(l1, l2, *p)
+        got = orc.seg_tri(l1, l2, *p)
+        if sol is None:
+            continue
+        r, s, t = sol
+        margin = min(abs(r), abs(1 - r), abs(s), abs(t), abs(1 - s - t))
+        n_ = np.cross(p[1] - p[0], p[2] - p[0])
+        if margin < 1e-4 or abs(np.dot(n_, l2 - l1)) < 0.01:
+            continue  # threshold band: behaviour fixed only by the reference
+        n_checked += 1
+        inside = 0 <= r <= 1 and s >= 0 and t >= 0 and s + t <= 1
+        if inside:
+            n_hit += 1
+            assert abs(got - r) < 1e-4
+        else:
+            assert got == 2.0
+    assert n_checked > 3000 and n_hit > 100
+
+
+def test_seg_tri_rules(orc):
+    """Exact rules of IntersectLineSegmentTriangle (lighter_math.cpp:315-354)."""
+    t = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    assert orc.seg_tri((0.2, 0.2, 1), (0.2, 0.2, -1), *t) == 0.5
+    assert orc.seg_tri((0.2, 0.2, -1), (0.2, 0.2, 1), *t) == 0.5  # two-sided
+    assert orc.seg_tri((0.2, 0.2, 0), (0.2, 0.2, -1), *t) == 0.0  # r = 0 counts
+    assert orc.seg_tri((0.2, 0.2, 1), (0.2, 0.2, 0), *t) == 1.0  # r = 1 counts
+    assert orc.seg_tri((0.2, 0.2, 1), (0.2, 0.2, 0.5), *t) == 2.0  # segment stops short
+    assert orc.seg_tri((0.7, 0.7, 1), (0.7, 0.7, -1), *t) == 2.0  # s + t > 1
+    assert orc.seg_tri((-2, 0.2, 0.0), (2, 0.2, 0.0), *t) == 2.0  # parallel: |n.dir| < 0.001
+    assert orc.seg_tri((0.2, 0.2, 1), (0.2, 0.2, -1), (0, 0, 0), (1e-4, 0, 0), (0, 1e-4, 0)) == 2.0  # tiny
+
+
+def _closest_point_dist(p, a, b, c):
+    """Ericson, Real-Time Collision Detection 5.1.5, in FP64."""
+    p, a, b, c = (np.asarray(x, np.float64) for x in (p, a, b, c))
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = ab @ ap, ac @ ap
+    if d1 <= 0 and d2 <= 0:
+        return np.linalg.norm(p - a)
+    bp = p - b
+    d3, d4 = ab @ bp, ac @ bp
+    if d3 >= 0 and d4 <= d3:
+        return np.linalg.norm(p - b)
+    vc = d1 * d4 - d3 * d2
+    if vc <= 0 and d1 >= 0 and d3 <= 0:
+        v = d1 / (d1 - d3)
+        return np.linalg.norm(p - (a + v * ab))
+    cp = p - c
+    d5, d6 = ab @ cp, ac @ cp
+    if d6 >= 0 and d5 <= d6:
+        return np.linalg.norm(p - c)
+    vb = d5 * d2 - d1 * d6
+    if vb <= 0 and d2 >= 0 and d6 <= 0:
+        w = d2 / (d2 - d6)
+        return np.linalg.norm(p - (a + w * ac))
+    va = d3 * d6 - d5 * d4
+    if va <= 0 and (d4 - d3) >= 0 and (d5 - d6) >= 0:
+        w = (d4 - d3) / ((d4 - d3) + (d5 - d6))
+        return np.linalg.norm(p - (b + w * (c - b)))
+    denom = 1.0 / (va + vb + vc)
+    v, w = vb * denom, vc * denom
+    return np.linalg.norm(p - (a + ab * v + ac * w))
+
+
+def test_point_triangle_distance_matches_closest_point(orc):
+    rng = np.random.default_rng(11)
+    for k in range(3000):
+        if k % 3 == 0:  # obtuse triangles
+            a, b = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+            c = 0.5 * (a + b) + rng.uniform(-0.1, 0.1, 3)
+        else:
+            a, b, c = rng.uniform(-1, 1, (3, 3))
+        p = rng.uniform(-2, 2, 3)
+        a, b, c, p = (x.astype(np.float32) for x in (a, b, c, p))
+        exp = _closest_point_dist(p, a, b, c)
+        got = orc.point_tri_dist(p, a, b, c)
+        assert abs(got - exp) <= 2e-5 * max(1.0, exp), (k, got, exp)
+
+
+def test_shadow_march_closed_forms(orc):
+    floor = np.array([[-50, -50, 0, 50, -50, 0, 50, 50, 0], [-50, -50, 0, 50, 50, 0, -50, 50, 0]], np.float32)
+    # unoccluded, hard shadow (k = 0): factor 1
+    v, steps = orc.shadow_factor(floor, (0, 0, 0.005), (3, 4, 10), 0.0)
+    assert v == 1.0
+    # a blocker on the segment: fully shadowed
+    blocker = np.array([[0, -1, 2, 2, -1, 2, 1, 1, 2]], np.float32)
+    v, _ = orc.shadow_factor(np.concatenate([floor, blocker]), (0.5, 0, 0.005), (0.5, 0, 10), 0.1)
+    assert v == 0.0
+    # straight up from a plane: h = t + 0.005 at every step, res = h/min(t k, 2)
+    k = 0.1
+    v, steps = orc.shadow_factor(floor, (0, 0, 0.005), (0, 0, 30), k)
+    assert v == 1.0
+    # at least ceil(maxt / 1 m) steps (step capped at 1 m)
+    assert steps >= 30
+    # grazing: distance grows as t*sin(theta); the factor is bounded by it
+    th = 0.05
+    d = np.array([np.cos(th), 0, np.sin(th)], np.float32)
+    v, _ = orc.shadow_factor(floor, (0, 0, 0.005), (d * 40).astype(np.float32), 1.0)
+    assert 0 < v <= np.sin(th) * 1.05 + 0.005
+
+
+def test_spiral_moments(orc):
+    """Cosine-weighted hemisphere: unit vectors, E[cos] = 2/3, E[cos^2] = 1/2."""
+    rng = np.random.default_rng(12)
+    n = 64
+    for _ in range(20):
+        d = normalize_f32(rng.normal(size=3))
+        v = np.array([orc.spiral(d, rng.uniform(), i, n) for i in range(n)])
+        np.testing.assert_allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-5)
+        c = v @ d
+        assert np.all(c > 0)
+        assert abs(c.mean() - 2 / 3) < 0.01
+        assert abs((c * c).mean() - 0.5) < 0.01
+
+
+def test_radiosity_series_is_matrix_powers(orc):
+    """orc_rad_series_f64 = sum_k (F diag(a d_c))^k E with F symmetric."""
+    rng = np.random.default_rng(13)
+    N = 40
+    links = [(i, j) for i in range(N) for j in range(i + 1, N) if rng.uniform() < 0.2]
+    rowmap = np.zeros(2 * N, np.uint32)
+    other, fac = [], []
+    for i in range(N):
+        rowmap[2 * i] = len(other)
+        for (a, b) in links:
+            if a == i:
+                other.append(b)
+                fac.append(rng.uniform(0.001, 0.3))
+        rowmap[2 * i + 1] = len(other) - rowmap[2 * i]
+    other = np.array(other, np.uint32)
+    fac = np.array(fac, np.float32)
+    F = np.zeros((N, N))
+    for i in range(N):
+        for k in range(rowmap[2 * i], rowmap[2 * i] + rowmap[2 * i + 1]):
+            F[i, other[k]] = F[other[k], i] = fac[k]
+    area = rng.uniform(0.1, 1, N)
+    diff = rng.uniform(0.2, 0.9, (N, 3))
+    E = rng.uniform(0, 1, (N, 6))
+    got = orc.rad_series_f64(rowmap, other, fac, area, diff, E, 3)
+    for r in range(6):
+        K = F @ np.diag(area * diff[:, r % 3])
+        x = E[:, r].copy()
+        tot = x.copy()
+        for _ in range(3):
+            x = K @ x
+            tot += x
+        np.testing.assert_allclose(got[:, r], tot, rtol=1e-12)
+    # the FP32 bounce loop in reference order agrees with the series
+    c13 = np.concatenate([diff, E[:, :3], E[:, :3], np.zeros((N, 3)), area[:, None]], 1).astype(np.float32)
+    for _ in range(3):
+        c13 = orc.rad_bounce_f32(rowmap, other, fac, c13)
+    np.testing.assert_allclose(c13[:, 3:6], got[:, :3], rtol=1e-5)
+
+
+def test_ao_open_plane_is_zero(orc):
+    floor = np.array([[-50, -50, 0, 50, -50, 0, 50, 50, 0], [-50, -50, 0, 50, 50, 0, -50, 50, 0]], np.float32)
+    pos = np.array([[0, 0, 0], [3, 1, 0]], np.float32)
+    nrm = np.array([[0, 0, 1], [0, 0, 1]], np.float32)
+    lm = np.ones((2, 3), np.float32)
+    out, fac = orc.ao(floor, pos, nrm, np.array([0.3, 0.9], np.float32), lm, 2.0, n=32)
+    assert np.all(fac == 0) and np.all(out == 1)
+    # a ceiling 1 m above: every ray of length 2 hits it at r ~ (1/cos)/2 <= 1
+    ceil_ = floor.copy()
+    ceil_[:, 2::3] = 1.0
+    out, fac = orc.ao(np.concatenate([floor, ceil_]), pos, nrm, np.array([0.3, 0.9], np.float32), lm, 2.0,
+                      multiplier=1.0, n=32)
+    assert np.all(fac > 0.2) and np.all(fac < 0.6)
+
+
+# ---------------------------------------------------------------------------
+# against the reference build
+# ---------------------------------------------------------------------------
+pytestmark_ref = pytest.mark.ref
+
+
+@pytest.mark.ref
+def test_primitives_bitexact_with_reference(orc, ref):
+    rng = np.random.default_rng(20)
+    L = ref.lib
+    for _ in range(3000):
+        p = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        a, b = rng.uniform(-2, 2, (2, 3)).astype(np.float32)
+        if rng.uniform() < 0.3:
+            b = (a + (p[0] - a) * rng.uniform(0.5, 2)).astype(np.float32)  # aimed at a vertex region
+        r1 = orc.seg_tri(a, b, *p)
+        r2 = L.rp_seg_tri(_p(a), _p(b), _p(p[0]), _p(p[1]), _p(p[2]))
+        assert bits(np.float32(r1)) == bits(np.float32(r2))
+        q = rng.uniform(-2, 2, 3).astype(np.float32)
+        d1 = orc.point_tri_dist(q, *p)
+        d2 = L.rp_point_tri_dist_capped(_p(q), _p(p[0]), _p(p[1]), _p(p[2]))
+        if orc.lib.orc_tri_useful(_p(p.reshape(-1))):
+            assert bits(np.float32(d1)) == bits(np.float32(d2))
+    for _ in range(500):
+        d = normalize_f32(rng.normal(size=3))
+        ro = np.float32(rng.uniform())
+        i, n = int(rng.integers(0, 64)), 64
+        s1 = orc.spiral(d, ro, i, n)
+        s2 = np.zeros(3, np.float32)
+        L.rp_spiral(_p(d), ro, i, n, _p(s2))
+        assert np.array_equal(bits(s1), bits(s2))
+
+
+@pytest.fixture(scope="module")
+def ref_room(lib, ref):
+    import ctypes
+    ctypes.CDLL(None).srand(5)
+    spec = scenes.room_scene(lm=40, n_point=2, bounce_count=1, ao_distance=2.0)
+    br = scenes.BuiltScene(ref.lib, spec)
+    ref.lib.rp_prepare(br.scene, 1)
+    yield spec, br
+    br.destroy()
+
+
+@pytest.mark.ref
+def test_queries_bitexact_with_reference(orc, ref, ref_room):
+    spec, br = ref_room
+    pos, nrm, _, _ = ref.samples(br.scene, 1)
+    tris = ref.shadow_tris(br.scene)
+    rng = np.random.default_rng(21)
+    L = ref.lib
+    for i in rng.choice(len(pos), 150, replace=False):
+        p = (pos[i] + nrm[i] * np.float32(rng.uniform(0, 1))).astype(np.float32)
+        assert bits(np.float32(orc.scene_distance(tris, p))) == bits(np.float32(L.rp_distance(br.scene, _p(p))))
+        a = (pos[i] + nrm[i] * np.float32(0.005)).astype(np.float32)
+        lp = np.asarray(spec.lights[0]["position"], np.float32)
+        v, _ = orc.shadow_factor(tris, a, lp, 0.1)
+        assert bits(np.float32(v)) == bits(np.float32(L.rp_shadow_factor(br.scene, _p(a), _p(lp), 0.1)))
+        d = normalize_f32(rng.normal(size=3))
+        b = (a + d * np.float32(2)).astype(np.float32)
+        assert bits(np.float32(orc.closest_hit(tris, a, b))) == bits(np.float32(L.rp_closest_hit(br.scene, _p(a), _p(b))))
+        j = int(rng.integers(len(pos)))
+        assert orc.occluded(tris, pos[i], pos[j]) == L.rp_visibility(br.scene, _p(pos[i]), _p(pos[j]))
+
+
+@pytest.mark.ref
+def test_direct_ao_links_bitexact_with_reference(orc, ref, ref_room, libc):
+    spec, br = ref_room
+    pos, nrm, _, _ = ref.samples(br.scene, 1)
+    tris = ref.shadow_tris(br.scene)
+    lm = ref.lightmap(br.scene, 1)
+    for li in spec.lights:
+        lm, _, _ = orc.direct_light(tris, pos, nrm, orc_light_from_info(li), lm)
+    ref.lib.rp_direct(br.scene, 0, len(spec.lights))
+    assert np.array_equal(bits(lm), bits(ref.lightmap(br.scene, 1)))
+    # radiosity links on the reference's own patch geometry
+    ref.lib.rp_rad_links(br.scene)
+    geo = ref.rad_geoms(br.scene)
+    rm, ot, fa = orc.rad_links(tris, geo[:, :3], geo[:, 3:])
+    rm2, ot2, fa2 = ref.links(br.scene)
+    assert np.array_equal(rm, rm2) and np.array_equal(ot, ot2) and np.array_equal(bits(fa), bits(fa2))
+    # one bounce in the reference's order
+    c0 = ref.rad_colors(br.scene)
+    ref.lib.rp_rad_bounce(br.scene, 1)
+    c1 = orc.rad_bounce_f32(rm, ot, fa, c0)
+    assert np.array_equal(bits(c1), bits(ref.rad_colors(br.scene)))
+    ref.lib.rp_rad_commit(br.scene)
+    # AO: the reference draws one rand() per texel in sample order
+    lm = ref.lightmap(br.scene, 1)
+    libc.srand(99)
+    ro = np.array([np.float32(libc.rand()) / np.float32(2147483647) for _ in range(len(pos))], np.float32)
+    libc.srand(99)
+    ref.lib.rp_ao(br.scene)
+    cfg = br.config
+    out, _ = orc.ao(tris, pos, nrm, ro, lm, cfg.ao_distance, cfg.ao_multiplier, cfg.ao_falloff, cfg.ao_effect,
+                    list(cfg.ao_color_rgb), cfg.ao_num_samples)
+    assert np.array_equal(bits(out), bits(ref.lightmap(br.scene, 1)))

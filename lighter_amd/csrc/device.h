@@ -140,6 +140,12 @@ private:
 	float* d_vis_ = nullptr;
 	uint32_t* d_items_ = nullptr;
 	DevLight* d_lights_ = nullptr;
+	float* d_gate_ = nullptr;     // shading factor per item (marched items)
+	uint32_t* d_mlist_ = nullptr; // compacted march list
+	uint32_t* d_mcount_ = nullptr; // [0] march count, [1] persistent-kernel cursor
+	uint32_t nmarch_ = 0;
+	unsigned persistent_blocks();
+	unsigned persist_blocks_ = 0;
 	std::vector<DevLight> h_lights_;
 	std::vector<uint32_t> h_light_off_;
 	size_t nitems_ = 0;

@@ -109,11 +109,12 @@ __device__ __forceinline__ RayTri load_rtri(const RayTri* t, uint32_t i)
 	return r;
 }
 
-// Nearest-first traversal for the distance query.
+// Nearest-first traversal for the distance query: min(radius, distance to the
+// nearest triangle closer than radius).
 template <bool COUNT>
-__device__ float q_distance(const View& V, f3 p, unsigned long long* cnt)
+__device__ float q_distance(const View& V, f3 p, float radius, unsigned long long* cnt)
 {
-	float best = kPenumbraSize;
+	float best = radius;
 	if (!V.has_tris)
 		return best;
 	int32_t sref[kStackSize];
@@ -297,98 +298,209 @@ __device__ __forceinline__ float box_exit(const View& V, f3 from, f3 rd)
 	return tf * 1.0001f + 0.01f;
 }
 
-// CalcInvShadowFactor (lighter.cpp:190-207).  Once the march has left the
-// grown scene box every remaining step sees h = 2 and, for k >= 0, cannot
-// lower res (2 / min(t*k, 2) >= 1 >= res), so the loop stops there with the
-// value the full loop would return.
+// CalcInvShadowFactor (lighter.cpp:190-207) as a resumable state machine, one
+// distance query per step.  Two shortcuts leave the result unchanged:
+//  * once the march has left the grown scene box every remaining step sees
+//    h = 2 and, for k >= 0, cannot lower res (2 / min(t*k, 2) >= 1 >= res),
+//    so the march stops there;
+//  * the distance to the scene is 1-Lipschitz, so the next query's search
+//    radius starts at min(2, h + step + margin) instead of 2: every
+//    triangle closer than that is still found, hence the same minimum.
+// A march whose t no longer advances (t + step == t, where the reference
+// would loop forever) ends with the current factor.
+struct March {
+	f3 from, rd;
+	float t, maxt, k, res, t_out, radius;
+	uint32_t steps;
+};
+
+__device__ __forceinline__ void march_init(March& M, const View& V, f3 from, f3 to, float k)
+{
+	M.from = from;
+	M.rd = normalized(to - from);
+	M.t = 0.001f;
+	M.maxt = length(to - from);
+	M.k = k;
+	M.res = 1.0f;
+	M.t_out = (k >= 0.0f) ? box_exit(V, from, M.rd) : INFINITY;
+	M.radius = kPenumbraSize;
+	M.steps = 0;
+}
+
+// margin of the next search radius: covers the rounding of the computed
+// distances (a few ulps of the coordinates) and of the march position
+__device__ __forceinline__ float lipschitz_margin(f3 p)
+{
+	return 1e-4f + 1e-5f * fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z));
+}
+
+// Advances one step; returns true when the march is over (result in M.res).
+template <bool COUNT>
+__device__ __forceinline__ bool march_step(March& M, const View& V, unsigned long long* cnt)
+{
+	if (!(M.t < M.maxt) || M.t > M.t_out)
+		return true;
+	f3 p = M.from + M.rd * M.t;
+	float h = q_distance<COUNT>(V, p, M.radius, cnt);
+	M.steps++;
+	if (h < 0.001f) {
+		M.res = 0.0f;
+		return true;
+	}
+	M.res = tmin(M.res, h / tmin(M.t * M.k, kPenumbraSize));
+	float st = tmin(h, kPenumbraStep);
+	float tn = M.t + st;
+	if (tn == M.t)
+		return true;
+	M.radius = fminf(h + st + lipschitz_margin(p), kPenumbraSize);
+	M.t = tn;
+	return false;
+}
+
 template <bool COUNT>
 __device__ float shadow_march(const View& V, f3 from, f3 to, float k, unsigned long long* cnt, uint32_t* nsteps)
 {
-	f3 rd = normalized(to - from);
-	float mint = 0.001f;
-	float maxt = length(to - from);
-	float res = 1.0f;
-	float t_out = (k >= 0.0f) ? box_exit(V, from, rd) : INFINITY;
-	uint32_t steps = 0;
-	for (float t = mint; t < maxt;) {
-		if (t > t_out)
-			break;
-		float h = q_distance<COUNT>(V, from + rd * t, cnt);
-		steps++;
-		if (h < 0.001f) {
-			res = 0.0f;
-			break;
-		}
-		res = tmin(res, h / tmin(t * k, kPenumbraSize));
-		h = tmin(h, kPenumbraStep);
-		t += h;
+	March M;
+	march_init(M, V, from, to, k);
+	while (!march_step<COUNT>(M, V, cnt)) {
 	}
 	if (COUNT) {
-		atomicAdd(cnt + 2, (unsigned long long)steps);
+		atomicAdd(cnt + 2, (unsigned long long)M.steps);
 		atomicAdd(cnt + 4, 1ull);
 	}
 	if (nsteps)
-		*nsteps = steps;
-	return res;
+		*nsteps = M.steps;
+	return M.res;
 }
 
 // ---------------------------------------------------------------------------
 // direct light
 // ---------------------------------------------------------------------------
+// Wave-aggregated append: one atomic per wave; returns this lane's slot.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter)
+{
+	const int lane = threadIdx.x & 63;
+	uint64_t m = __ballot(want);
+	if (!m)
+		return 0;
+	int leader = __ffsll((unsigned long long)m) - 1;
+	uint32_t base = 0;
+	if (lane == leader)
+		base = atomicAdd(counter, (uint32_t)__popcll(m));
+	base = (uint32_t)__shfl((int)base, leader);
+	return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Shading gate of each (light, texel) item (Job_LMRender_*_Inner up to the
+// shadow march, lighter.cpp:485-553, 586): items with a zero shading factor
+// get their final weight here; the others are appended to the march list
+// with the factor the shadow term multiplies.
+__global__ __launch_bounds__(kBlock) void k_direct_gate(const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                        const DevLight* __restrict__ lights,
+                                                        const uint32_t* __restrict__ items, uint32_t nitems,
+                                                        float* __restrict__ gate, float* __restrict__ wout,
+                                                        float* __restrict__ vout, uint32_t* __restrict__ mlist,
+                                                        uint32_t* __restrict__ mcount)
+{
+	uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	bool march = false;
+	if (k < nitems) {
+		uint32_t code = items[k];
+		const DevLight& L = lights[code >> kSampleBits];
+		uint32_t s = code & kSampleMask;
+		f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
+		float w = __uint_as_float(kSkipBits), g = 0.0f;
+		if (L.type == 3) {
+			float fn = tmax(0.0f, dot(ld3(L.dir), SN));
+			if (fn > 0) {
+				march = true;
+				g = fn;
+			} else {
+				w = fn; // +-0 times a finite non-negative shadow factor
+			}
+		} else if (L.type == 1 || L.type == 2) {
+			f3 s2l = ld3(L.pos) - SP;
+			float dist = length(s2l);
+			if (dist)
+				s2l = s2l / dist;
+			float fd = pow_r(1 - tmin(1.0f, dist / L.range), L.power);
+			float fn = tmax(0.0f, dot(s2l, SN));
+			float gt;
+			if (L.type == 2) {
+				float ang = acos_r(tmin(1.0f, dot(s2l, -ld3(L.dir))));
+				float fdir = tmax(0.0f, tmin(1.0f, (ang - L.spot_out_rad) / L.spot_diff));
+				fdir = pow_r(fdir, L.spot_curve);
+				gt = fd * fn * fdir;
+			} else {
+				gt = fd * fn;
+			}
+			if (!(gt <= 0)) {
+				march = true;
+				g = gt;
+			}
+		}
+		if (march) {
+			gate[k] = g;
+		} else {
+			wout[k] = w;
+			if (vout)
+				vout[k] = 1.0f;
+		}
+	}
+	uint32_t slot = wave_append(march, mcount);
+	if (march)
+		mlist[slot] = k;
+}
+
+// Persistent shadow-march kernel: every lane pulls a new march from the list
+// as soon as its current one ends (one atomic per wave per refill), so the
+// very uneven march lengths do not idle the wave.  Every wave leaves once the
+// list is exhausted and its lanes' marches have ended.
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_march(View V, const float* __restrict__ pos,
                                                   const float* __restrict__ nrm, const DevLight* __restrict__ lights,
-                                                  const uint32_t* __restrict__ items, uint32_t nitems,
-                                                  float* __restrict__ wout, float* __restrict__ vout,
+                                                  const uint32_t* __restrict__ items,
+                                                  const uint32_t* __restrict__ mlist, uint32_t nmarch,
+                                                  const float* __restrict__ gate, float* __restrict__ wout,
+                                                  float* __restrict__ vout, uint32_t* __restrict__ next,
                                                   unsigned long long* cnt)
 {
-	uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= nitems)
-		return;
-	uint32_t code = items[k];
-	const DevLight& L = lights[code >> kSampleBits];
-	uint32_t s = code & kSampleMask;
-	f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
-	f3 lpos = ld3(L.pos), ldir = ld3(L.dir);
-	float w, fv = 1.0f;
-	if (L.type == 3) {
-		// Job_LMRender_Direct_Inner (lighter.cpp:586-589)
-		float fn = tmax(0.0f, dot(ldir, SN));
-		if (fn > 0) {
-			fv = shadow_march<COUNT>(V, SP + SN * kShadowOffset, SP + ldir * L.range, L.radius, cnt, nullptr);
-			w = fn * fv;
-		} else {
-			w = fn; // +-0 times a finite non-negative factor
+	bool active = false, done = false;
+	uint32_t k = 0;
+	March M;
+	for (;;) {
+		bool need = !active && !done;
+		if (__ballot(need)) {
+			uint32_t idx = wave_append(need, next);
+			if (need) {
+				if (idx < nmarch) {
+					k = mlist[idx];
+					uint32_t code = items[k];
+					const DevLight& L = lights[code >> kSampleBits];
+					uint32_t s = code & kSampleMask;
+					f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
+					f3 to = (L.type == 3) ? SP + ld3(L.dir) * L.range : ld3(L.pos);
+					march_init(M, V, SP + SN * kShadowOffset, to, L.radius);
+					active = true;
+				} else {
+					done = true;
+				}
+			}
 		}
-	} else if (L.type == 1 || L.type == 2) {
-		// Job_LMRender_Point_Inner / Spot_Inner (lighter.cpp:485-553)
-		f3 s2l = lpos - SP;
-		float dist = length(s2l);
-		if (dist)
-			s2l = s2l / dist;
-		float fd = pow_r(1 - tmin(1.0f, dist / L.range), L.power);
-		float fn = tmax(0.0f, dot(s2l, SN));
-		float fdir = 1.0f, gate;
-		if (L.type == 2) {
-			float ang = acos_r(tmin(1.0f, dot(s2l, -ldir)));
-			fdir = tmax(0.0f, tmin(1.0f, (ang - L.spot_out_rad) / L.spot_diff));
-			fdir = pow_r(fdir, L.spot_curve);
-			gate = fd * fn * fdir;
-		} else {
-			gate = fd * fn;
+		if (__all(done))
+			break;
+		if (active && march_step<COUNT>(M, V, cnt)) {
+			// weight = (shading factor) * shadow factor, in the reference's order
+			wout[k] = gate[k] * M.res;
+			if (vout)
+				vout[k] = M.res;
+			if (COUNT) {
+				atomicAdd(cnt + 2, (unsigned long long)M.steps);
+				atomicAdd(cnt + 4, 1ull);
+			}
+			active = false;
 		}
-		if (gate <= 0) {
-			w = __uint_as_float(kSkipBits);
-		} else {
-			fv = shadow_march<COUNT>(V, SP + SN * kShadowOffset, lpos, L.radius, cnt, nullptr);
-			w = (L.type == 2) ? fd * fn * fdir * fv : fd * fn * fv;
-		}
-	} else {
-		w = __uint_as_float(kSkipBits);
 	}
-	wout[k] = w;
-	if (vout)
-		vout[k] = fv;
 }
 
 // lightmap[s] += color * w for the items of one light (no sample repeats
@@ -462,7 +574,7 @@ __global__ void k_q_distance(View V, const float* p, uint32_t n, float* out)
 {
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n)
-		out[i] = q_distance<false>(V, ld3(p + 3 * i), nullptr);
+		out[i] = q_distance<false>(V, ld3(p + 3 * i), kPenumbraSize, nullptr);
 }
 
 __global__ void k_q_shadow(View V, const float* a, const float* b, const float* k, uint32_t n, float* out,
@@ -715,12 +827,28 @@ DeviceScene::DeviceScene()
 	HIPCHECK(hipStreamCreate(&s));
 	stream_ = s;
 	d_counters_ = dalloc<unsigned long long>(kNumCounters);
+	d_mcount_ = dalloc<uint32_t>(2);
+}
+
+// Blocks of the persistent march kernel: as many as can be resident at once.
+unsigned DeviceScene::persistent_blocks()
+{
+	if (!persist_blocks_) {
+		int dev = 0, per_cu = 0;
+		hipDeviceProp_t p;
+		HIPCHECK(hipGetDevice(&dev));
+		HIPCHECK(hipGetDeviceProperties(&p, dev));
+		HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<false>, kBlock, 0));
+		persist_blocks_ = (unsigned)std::max(1, per_cu) * (unsigned)p.multiProcessorCount;
+	}
+	return persist_blocks_;
 }
 
 DeviceScene::~DeviceScene()
 {
 	free_all();
 	dfree(d_counters_);
+	dfree(d_mcount_);
 	if (stream_)
 		(void)hipStreamDestroy((hipStream_t)stream_);
 }
@@ -737,6 +865,8 @@ void DeviceScene::free_all()
 	dfree(d_vis_);
 	dfree(d_items_);
 	dfree(d_lights_);
+	dfree(d_gate_);
+	dfree(d_mlist_);
 	dfree(rad.pos);
 	dfree(rad.nrm);
 	dfree(rad.bits);
@@ -870,10 +1000,14 @@ void DeviceScene::set_direct_work(const std::vector<DevLight>& lights, const std
 	dfree(d_vis_);
 	dfree(d_items_);
 	dfree(d_lights_);
+	dfree(d_gate_);
+	dfree(d_mlist_);
 	nitems_ = items.size();
 	d_w_ = dalloc<float>(nitems_);
 	if (keep_weights)
 		d_vis_ = dalloc<float>(nitems_);
+	d_gate_ = dalloc<float>(nitems_);
+	d_mlist_ = dalloc<uint32_t>(nitems_);
 	d_items_ = dalloc<uint32_t>(nitems_);
 	d_lights_ = dalloc<DevLight>(lights.size());
 	HIPCHECK(hipMemcpyAsync(d_items_, items.data(), 4 * nitems_, hipMemcpyHostToDevice, s));
@@ -889,15 +1023,24 @@ void DeviceScene::run_direct()
 	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
 	HIPCHECK(hipMemsetAsync(d_counters_, 0, kNumCounters * sizeof(unsigned long long), s));
 	EventTimer all(s);
-	{
+	if (nitems_) {
+		HIPCHECK(hipMemsetAsync(d_mcount_, 0, 2 * sizeof(uint32_t), s));
+		k_direct_gate<<<grid_for(nitems_), kBlock, 0, s>>>(d_pos_, d_nrm_, d_lights_, d_items_, (uint32_t)nitems_,
+		                                                   d_gate_, d_w_, d_vis_, d_mlist_, d_mcount_);
+		HIPCHECK(hipGetLastError());
+		uint32_t nmarch = 0;
+		HIPCHECK(hipMemcpyAsync(&nmarch, d_mcount_, 4, hipMemcpyDeviceToHost, s));
+		HIPCHECK(hipStreamSynchronize(s));
+		nmarch_ = nmarch;
 		EventTimer tm(s);
-		if (nitems_) {
+		if (nmarch) {
+			unsigned blocks = persistent_blocks();
 			if (count_queries)
-				k_march<true><<<grid_for(nitems_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_,
-				                                                   (uint32_t)nitems_, d_w_, d_vis_, d_counters_);
+				k_march<true><<<blocks, kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch,
+				                                        d_gate_, d_w_, d_vis_, d_mcount_ + 1, d_counters_);
 			else
-				k_march<false><<<grid_for(nitems_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_,
-				                                                    (uint32_t)nitems_, d_w_, d_vis_, d_counters_);
+				k_march<false><<<blocks, kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch,
+				                                         d_gate_, d_w_, d_vis_, d_mcount_ + 1, d_counters_);
 			HIPCHECK(hipGetLastError());
 		}
 		times.direct_march_ms = tm.stop();

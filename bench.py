@@ -171,6 +171,126 @@ def cpu_baseline(spec, pos, nrm, items, w, lights, target_s, nthreads):
                        f"(collision set-up {prep_s:.1f} s not counted)"), (sel, out)
 
 
+FORMS = {0: "sparse_f64", 1: "sparse_f32", 2: "dense_f32", 3: "dense_bf16_mfma", 4: "dense_f16_mfma"}
+
+
+def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
+    """C2: radiosity over the room's texels as patches, R right-hand sides
+    (setups x RGB) built from the lights' direct contributions."""
+    from oracle.checkers import normalize_f32
+    lm = 200
+    while True:
+        spec = scenes.room_scene(lm=lm, bounce_count=bounces, ao_distance=2.0, ao_samples=64, name="C2")
+        b = scenes.BuiltScene(lib, spec, dict(max_num_threads=nth))
+        _lib.check(lib.lgx_Prepare(b.scene), lib)
+        info = (C.c_int32 * 4)()
+        lib.lgx_InstanceInfo(b.scene, 1, info)
+        if info[2] >= n_patches or lm >= 512:
+            break
+        b.destroy()
+        lm += 8
+    n_all, nt = info[2], info[3]
+    pos = np.zeros((n_all, 3), np.float32)
+    nrm = np.zeros((n_all, 3), np.float32)
+    rad = np.zeros((n_all, 4), np.float32)
+    lib.lgx_GetSamples(b.scene, 1, P(pos), P(nrm), None, P(rad))
+    tris = np.zeros((nt, 9), np.float32)
+    lib.lgx_GetTriangles(b.scene, 1, P(tris))
+    N = min(n_patches, n_all)
+    pos, nrm, rad = pos[:N].copy(), nrm[:N].copy(), rad[:N].copy()
+    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    # direct light of each light on the patches
+    _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), N, None), lib)
+    lights = make_lights(spec)
+    L = len(lights)
+    items = np.concatenate([(np.uint32(l) << np.uint32(26)) | np.arange(N, dtype=np.uint32) for l in range(L)])
+    off = (np.arange(L + 1) * N).astype(np.uint32)
+    _lib.check(lib.lgx_DirectLight(dev, lights, L, items.ctypes.data_as(u32p), len(items),
+                                   off.ctypes.data_as(u32p), 0), lib)
+    w = np.zeros(len(items), np.float32)
+    lib.lgx_ReadWeights(dev, P(w), None, len(items))
+    w[w.view(np.uint32) == 0xFFFFFFFF] = 0
+    W = w.reshape(L, N).T.astype(np.float64)
+    rng = np.random.default_rng(0x4C540002)
+    cols = rng.uniform(0.5, 1.0, (setups, L, 3))
+    E = np.zeros((N, 3 * setups))
+    for s in range(setups):
+        for c in range(3):
+            E[:, 3 * s + c] = W @ cols[s, :, c]
+    E = np.ascontiguousarray(E[:, :R])
+    # links (patch normals normalised again, as Int_RDGenLinks does)
+    pn = np.array([normalize_f32(v) for v in nrm], np.float32)
+    nl = C.c_size_t(0)
+    t0 = time.perf_counter()
+    _lib.check(lib.lgx_RadLinks(dev, P(pos), P(pn), N, C.byref(nl)), lib)
+    tm = (C.c_double * 5)()
+    lib.lgx_DeviceTimes(dev, tm)
+    link_ms = tm[3]
+    log(f"[bench] C2: {N} patches, {nl.value} links ({link_ms:.1f} ms GPU, {time.perf_counter() - t0:.1f} s wall)")
+    alb = spec.albedo
+    diff = np.array([alb.get(int(p), (1.0, 1.0, 1.0)) for p in rad[:, 2]], np.float32)
+    area = np.ascontiguousarray(rad[:, 3])
+    dp = C.POINTER(C.c_double)
+
+    def run(form, refine=0):
+        out = np.zeros_like(E)
+        t = (C.c_double * 4)()
+        _lib.check(lib.lgx_RadSolve(dev, E.ctypes.data_as(dp), R, P(diff), P(area), form, bounces, refine,
+                                    out.ctypes.data_as(dp), t), lib)
+        return out, list(t)
+
+    ref64, t64 = run(0)
+    res = {}
+    # pin the GPU FP64 series against the C oracle on the first setup (RGB)
+    pin = None
+    try:
+        from oracle.checkers import Oracle
+        orc = Oracle()
+        rowmap = np.zeros(2 * N, np.uint32)
+        other = np.zeros(nl.value, np.uint32)
+        fac = np.zeros(nl.value, np.float32)
+        lib.lgx_RadReadLinks(dev, rowmap.ctypes.data_as(u32p), other.ctypes.data_as(u32p), P(fac))
+        t0 = time.perf_counter()
+        exp = orc.rad_series_f64(rowmap, other, fac, area, diff, E[:, :3], bounces)
+        pin = float(np.linalg.norm(ref64[:, :3] - exp) / np.linalg.norm(exp))
+        # the reference's own FP32 bounce loop (one RGB setup), restated in C
+        c13 = np.concatenate([diff, E[:, :3], E[:, :3], np.zeros((N, 3)), area[:, None]], 1).astype(np.float32)
+        t0 = time.perf_counter()
+        c13 = orc.rad_bounce_f32(rowmap, other, fac, c13)
+        cpu_bounce_s = time.perf_counter() - t0
+    except Exception as e:  # the checker is optional on a bare install
+        log(f"[bench] oracle pin skipped: {e}")
+        cpu_bounce_s = None
+    nrm64 = np.linalg.norm(ref64)
+    for form, refine in [(0, 0), (1, 0), (2, 0), (3, 0), (3, 1), (4, 0), (4, 1)]:
+        out, t = (ref64, t64) if form == 0 else run(form, refine)
+        key = FORMS[form] + ("_refined" if refine else "")
+        per_bounce = t[2] if not refine else (t[1] + t[3]) / bounces
+        res[key] = dict(s_per_bounce=per_bounce / 1000.0, rel_l2_err_vs_fp64=float(np.linalg.norm(out - ref64) / nrm64),
+                        series_ms=round(t[1], 3), refine_ms=round(t[3], 3), build_ms=round(t[0], 3))
+        log(f"[bench] {key}: {per_bounce:.3f} ms/bounce, err {res[key]['rel_l2_err_vs_fp64']:.2e}")
+    lib.lgx_DeviceDestroy(dev)
+    b.destroy()
+    Np = ((N + 127) // 128) * 128
+    mf = res["dense_bf16_mfma"]["s_per_bounce"]
+    gemm_bytes = Np * Np * 2 + 2 * 192 * Np * 2
+    out = dict(workload=f"C2 room radiosity: {N} patches (texels of the 10x10x4 m room), {R} right-hand sides "
+                        f"({setups} seeded lighting setups x RGB), {bounces} bounces", patches=N, rhs=R,
+               bounces=bounces, links=int(nl.value), link_density=round(2 * nl.value / (N * (N - 1)), 5),
+               link_generation_s=link_ms / 1000.0, forms=res,
+               fp64_vs_oracle_rel_l2=pin,
+               mfma_bf16_speedup_vs_dense_f32=round(res["dense_f32"]["s_per_bounce"] / mf, 2),
+               mfma_f16_refined_speedup_vs_dense_f32=round(
+                   res["dense_f32"]["s_per_bounce"] / res["dense_f16_mfma_refined"]["s_per_bounce"], 2),
+               mfma_roofline=dict(bound="hbm", achieved=round(gemm_bytes / mf / 1e9, 1), peak=HBM_PEAK_GBS,
+                                  unit="GB/s", frac=round(gemm_bytes / mf / 1e9 / HBM_PEAK_GBS, 4),
+                                  note="BF16 matrix once per bounce + right-hand sides in and out"),
+               cpu_port_s_per_bounce_rgb=cpu_bounce_s,
+               cpu_note="the reference's Int_RDBounce loop restated in C (oracle), one RGB setup, one host thread, "
+                        "on the same link list; the reference itself runs it single-threaded")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +298,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--small", action="store_true", help="reduced terrain for a quick check")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity (C2) leg")
+    ap.add_argument("--radiosity-patches", type=int, default=32768)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (scales the HBM byte counters)")
@@ -271,6 +393,9 @@ def main():
             log(f"[bench] cpu baseline {cpu['value']:.0f} rays/s; sampled parity {parity}")
     lib.lgx_DeviceDestroy(dev)
     built.destroy()
+    radio = None
+    if not args.no_radiosity:
+        radio = radiosity_bench(lib, nth, n_patches=args.radiosity_patches)
 
     out = dict(metric=METRIC, value=round(value, 1), unit="shadow rays/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(ms_per_step, 3), higher_is_better=True, scaling="weak",
@@ -281,7 +406,7 @@ def main():
                            triangles=int(info[0]), samples=int(len(pos)), lights=len(spec.lights),
                            work_items=int(len(items)), shadow_rays_per_step=int(marches),
                            march_steps_per_step=int(steps), march_kernel_ms=round(march_avg, 3)),
-               roofline=roofline, cpu_baseline=cpu, parity_sample=parity)
+               roofline=roofline, cpu_baseline=cpu, parity_sample=parity, radiosity=radio)
     print(json.dumps(out), flush=True)
 
 

@@ -122,6 +122,17 @@ LTRAPI int lgx_RadReadLinks(lgx_Device* dev, uint32_t* rowmap, uint32_t* other, 
  * light E; total3 receives the accumulated light per patch. */
 LTRAPI int lgx_RadBounce(lgx_Device* dev, const float* diffuse3, const float* emit3, const float* area, int bounces,
                          float* total3);
+/* Radiosity for R right-hand sides (1..192) over the links of the last
+ * lgx_RadLinks: total = sum_{k=0..bounces} K^k E with
+ * K = F diag(area) diag(diffuse_{c % 3}) for column c, the closed form of
+ * bounce_count rounds of Int_RDBounce (lighter.cpp:767-797, 1104-1123).
+ * E and total are N x R row-major FP64.  form: 0 sparse FP64 gather,
+ * 1 sparse FP32 gather, 2 dense FP32, 3 dense BF16 on MFMA, 4 dense FP16 on
+ * MFMA (FP32 accumulation); refine: refinement sweeps for the dense forms
+ * with the exact link operator's residual in FP64.  times_ms: dense build,
+ * series, per bounce, refinement. */
+LTRAPI int lgx_RadSolve(lgx_Device* dev, const double* E, uint32_t R, const float* diffuse3, const float* area,
+                        int form, int bounces, int refine, double* total, double times_ms[4]);
 /* Traversal counters (node visits, triangle tests, march steps, queries,
  * marches) collected by the next stage when enabled. */
 LTRAPI int lgx_SetCounting(lgx_Device* dev, int on);

@@ -60,6 +60,8 @@ LGX_SIGNATURES = {
     "lgx_RadLinks": (C.c_int, [vp, fp, fp, C.c_size_t, C.POINTER(C.c_size_t)]),
     "lgx_RadReadLinks": (C.c_int, [vp, u32p, u32p, fp]),
     "lgx_RadBounce": (C.c_int, [vp, fp, fp, fp, C.c_int, fp]),
+    "lgx_RadSolve": (C.c_int, [vp, C.POINTER(C.c_double), C.c_uint32, fp, fp, C.c_int, C.c_int, C.c_int,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "lgx_SetCounting": (C.c_int, [vp, C.c_int]),
     "lgx_ReadCounters": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
     "lgx_DeviceTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),

@@ -11,6 +11,7 @@
 
 #include "../../include/lighter_gpu.h"
 #include "pipeline.h"
+#include "radiosity.h"
 
 using namespace lgx;
 
@@ -478,6 +479,7 @@ int lgx_StageTimes(ltr_Scene* S, double out[5])
 // ---------------------------------------------------------------------------
 struct lgx_Device {
 	DeviceScene dev;
+	std::unique_ptr<RadSolver> rad;
 };
 
 lgx_Device* lgx_DeviceCreate(const float* tris9, size_t ntri)
@@ -621,6 +623,27 @@ int lgx_RadBounce(lgx_Device* d, const float* diffuse3, const float* emit3, cons
 		d->dev.rad_set_colors(diffuse3, emit3, area);
 		d->dev.rad_bounce(bounces);
 		d->dev.rad_get_total(total3);
+	});
+}
+
+int lgx_RadSolve(lgx_Device* d, const double* E, uint32_t R, const float* diffuse3, const float* area, int form,
+                 int bounces, int refine, double* total, double times_ms[4])
+{
+	return guarded([&] {
+		if (form < kSparseF64 || form > kDenseF16)
+			throw std::runtime_error("lgx_RadSolve: unknown form");
+		if (bounces < 0 || refine < 0)
+			throw std::runtime_error("lgx_RadSolve: negative count");
+		if (!d->rad || d->rad->N != d->dev.npatches)
+			d->rad.reset(new RadSolver(d->dev));
+		d->rad->set_problem(E, R, diffuse3, area);
+		d->rad->solve(form, bounces, refine, total);
+		if (times_ms) {
+			times_ms[0] = d->rad->times.build_ms;
+			times_ms[1] = d->rad->times.series_ms;
+			times_ms[2] = d->rad->times.per_bounce_ms;
+			times_ms[3] = d->rad->times.refine_ms;
+		}
 	});
 }
 

@@ -107,10 +107,20 @@ __device__ __forceinline__ float link_factor(const float* pos, const float* nrm,
 // per 32-column word.  FP32 (P = -1) or 16-bit with a power-of-two scale.
 // ---------------------------------------------------------------------------
 template <int P>
+__device__ __forceinline__ float from16(uint16_t v)
+{
+	if (P == 0)
+		return __uint_as_float((uint32_t)v << 16);
+	_Float16 h;
+	memcpy(&h, &v, 2);
+	return (float)h;
+}
+
+template <int P>
 __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N,
                                                     uint32_t Np, const float* __restrict__ pos,
                                                     const float* __restrict__ nrm, const float* __restrict__ area,
-                                                    float scale, void* __restrict__ G)
+                                                    float scale, void* __restrict__ G, uint16_t* __restrict__ Glo)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	uint64_t wpr = Np / 32; // words per padded row
@@ -130,15 +140,26 @@ __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__
 		for (int q = 0; q < 8; ++q)
 			o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 	} else {
-		uint32_t pk[16];
+		constexpr int PP = P < 0 ? 0 : P;
+		uint32_t pk[16], pl[16];
 #pragma unroll
-		for (int q = 0; q < 16; ++q)
-			pk[q] = (uint32_t)Prec<(P < 0 ? 0 : P)>::cvt(v[2 * q] * scale) |
-			        ((uint32_t)Prec<(P < 0 ? 0 : P)>::cvt(v[2 * q + 1] * scale) << 16);
+		for (int q = 0; q < 16; ++q) {
+			float a = v[2 * q] * scale, b = v[2 * q + 1] * scale;
+			uint16_t ha = Prec<PP>::cvt(a), hb = Prec<PP>::cvt(b);
+			pk[q] = (uint32_t)ha | ((uint32_t)hb << 16);
+			// remainder plane: what the 16-bit value left out
+			pl[q] = (uint32_t)Prec<PP>::cvt(a - from16<PP>(ha)) | ((uint32_t)Prec<PP>::cvt(b - from16<PP>(hb)) << 16);
+		}
 		uint4* o = reinterpret_cast<uint4*>((uint16_t*)G + (uint64_t)i * Np + (uint64_t)w * 32);
 #pragma unroll
 		for (int q = 0; q < 4; ++q)
 			o[q] = make_uint4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
+		if (Glo) {
+			uint4* ol = reinterpret_cast<uint4*>(Glo + (uint64_t)i * Np + (uint64_t)w * 32);
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				ol[q] = make_uint4(pl[4 * q], pl[4 * q + 1], pl[4 * q + 2], pl[4 * q + 3]);
+		}
 	}
 }
 
@@ -161,7 +182,7 @@ __global__ void k_max_link(uint32_t N, const uint32_t* __restrict__ rowptr, cons
 template <int P>
 __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, const float* __restrict__ diffuse,
                                                  const float* __restrict__ cscale, uint32_t N, uint32_t Np,
-                                                 uint32_t R, uint16_t* __restrict__ xq)
+                                                 uint32_t R, uint16_t* __restrict__ xq, uint16_t* __restrict__ xlo)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t >= (uint64_t)kRPad * Np)
@@ -170,7 +191,10 @@ __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, c
 	float v = 0.0f;
 	if (c < R && j < N)
 		v = xT[(uint64_t)c * Np + j] * diffuse[3 * j + c % 3] * cscale[c];
-	xq[t] = Prec<P>::cvt(v);
+	uint16_t h = Prec<P>::cvt(v);
+	xq[t] = h;
+	if (xlo)
+		xlo[t] = Prec<P>::cvt(v - from16<P>(h));
 }
 
 __global__ __launch_bounds__(kB) void k_prep_x32(const float* __restrict__ xT, const float* __restrict__ diffuse,
@@ -192,13 +216,38 @@ __global__ __launch_bounds__(kB) void k_prep_x32(const float* __restrict__ xT, c
 // tile sits at row 4*(l>>4)+i, column l & 15.
 // The right-hand-side chunk (192 x 64) is staged through LDS per k stage.
 // ---------------------------------------------------------------------------
-constexpr int kLdsStride = kKStage + 8; // bf16 elements per LDS column (pad)
+constexpr int kLdsStride = kKStage + 8; // 16-bit elements per LDS column (pad)
+constexpr int kXPieces = (kRPad * kKStage / 8) / kB; // 16-byte pieces per thread per stage
 
 template <int P>
-__global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G, const uint16_t* __restrict__ xq,
+__device__ __forceinline__ f32x4 mfma16(uint4 a, uint4 b, f32x4 c)
+{
+	if (P == 0) {
+		bf16x8 av, bv;
+		memcpy(&av, &a, 16);
+		memcpy(&bv, &b, 16);
+		return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+	} else {
+		f16x8 av, bv;
+		memcpy(&av, &a, 16);
+		memcpy(&bv, &b, 16);
+		return __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, c, 0, 0, 0);
+	}
+}
+
+// SPLIT: the matrix and the right-hand side come as hi + lo planes and the
+// kernel forms G_hi X_hi + G_hi X_lo + G_lo X_hi (the lo x lo term is below
+// FP32 resolution), an error-compensated product.
+// The next k stage's loads (matrix fragments to registers, right-hand-side
+// chunk to registers then to the other LDS buffer) are issued before the
+// current stage's MFMAs, so HBM traffic overlaps the math.
+template <int P, bool SPLIT>
+__global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G, const uint16_t* __restrict__ Glo,
+                                                  const uint16_t* __restrict__ xq, const uint16_t* __restrict__ xlo,
                                                   uint32_t Np, uint32_t ksplit, float* __restrict__ part)
 {
-	__shared__ uint16_t xs[kRPad * kLdsStride];
+	constexpr int NP = SPLIT ? 2 : 1; // planes
+	__shared__ uint16_t xs[2][NP][kRPad * kLdsStride];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const uint32_t i0 = blockIdx.x * kTileM + wave * 32;
 	const uint32_t klen = Np / ksplit;
@@ -210,47 +259,71 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 #pragma unroll
 		for (int t = 0; t < kColTiles; ++t)
 			acc[a][t] = f32x4{0, 0, 0, 0};
-	const uint16_t* grow0 = G + (uint64_t)(i0 + r) * Np;
-	const uint16_t* grow1 = G + (uint64_t)(i0 + 16 + r) * Np;
-	for (uint32_t kb = kbeg; kb < kend; kb += kKStage) {
-		__syncthreads();
-		// stage Xq[:, kb:kb+64] (192 columns x 128 bytes) into LDS
+	const uint64_t row0 = (uint64_t)(i0 + r) * Np, row1 = (uint64_t)(i0 + 16 + r) * Np;
+	// matrix fragments [plane][row tile][k half]; x chunk pieces [plane][piece]
+	uint4 ga[NP][2][2], xr[NP][kXPieces];
+	auto load_stage = [&](uint32_t kb) {
 #pragma unroll
-		for (int q = 0; q < (kRPad * kKStage / 8) / kB; ++q) {
-			int piece = q * kB + tid;           // 16-byte piece
-			int c = piece / (kKStage / 8), o = piece % (kKStage / 8);
-			uint4 v = *reinterpret_cast<const uint4*>(xq + (uint64_t)c * Np + kb + o * 8);
-			*reinterpret_cast<uint4*>(xs + c * kLdsStride + o * 8) = v;
+		for (int pl = 0; pl < NP; ++pl) {
+			const uint16_t* M = pl ? Glo : G;
+			const uint16_t* X = pl ? xlo : xq;
+			ga[pl][0][0] = *reinterpret_cast<const uint4*>(M + row0 + kb + 8 * g);
+			ga[pl][0][1] = *reinterpret_cast<const uint4*>(M + row0 + kb + 32 + 8 * g);
+			ga[pl][1][0] = *reinterpret_cast<const uint4*>(M + row1 + kb + 8 * g);
+			ga[pl][1][1] = *reinterpret_cast<const uint4*>(M + row1 + kb + 32 + 8 * g);
+#pragma unroll
+			for (int q = 0; q < kXPieces; ++q) {
+				int piece = q * kB + tid, c = piece / (kKStage / 8), o = piece % (kKStage / 8);
+				xr[pl][q] = *reinterpret_cast<const uint4*>(X + (uint64_t)c * Np + kb + o * 8);
+			}
 		}
-		// A fragments straight from HBM: rows i0+r and i0+16+r, k = kb + 8g + [0,8) and +32
-		uint4 a00 = *reinterpret_cast<const uint4*>(grow0 + kb + 8 * g);
-		uint4 a01 = *reinterpret_cast<const uint4*>(grow0 + kb + 32 + 8 * g);
-		uint4 a10 = *reinterpret_cast<const uint4*>(grow1 + kb + 8 * g);
-		uint4 a11 = *reinterpret_cast<const uint4*>(grow1 + kb + 32 + 8 * g);
-		__syncthreads();
+	};
+	auto store_x = [&](int buf) {
+#pragma unroll
+		for (int pl = 0; pl < NP; ++pl)
+#pragma unroll
+			for (int q = 0; q < kXPieces; ++q) {
+				int piece = q * kB + tid, c = piece / (kKStage / 8), o = piece % (kKStage / 8);
+				*reinterpret_cast<uint4*>(&xs[buf][pl][c * kLdsStride + o * 8]) = xr[pl][q];
+			}
+	};
+	load_stage(kbeg);
+	store_x(0);
+	__syncthreads();
+	int buf = 0;
+	for (uint32_t kb = kbeg; kb < kend; kb += kKStage) {
+		uint4 cur[NP][2][2];
+#pragma unroll
+		for (int pl = 0; pl < NP; ++pl)
+#pragma unroll
+			for (int a = 0; a < 2; ++a)
+#pragma unroll
+				for (int h = 0; h < 2; ++h)
+					cur[pl][a][h] = ga[pl][a][h];
+		const bool more = kb + kKStage < kend;
+		if (more)
+			load_stage(kb + kKStage);
 #pragma unroll
 		for (int h = 0; h < 2; ++h) {
-			uint4 A0 = h ? a01 : a00, A1 = h ? a11 : a10;
 #pragma unroll
 			for (int t = 0; t < kColTiles; ++t) {
-				uint4 Bv = *reinterpret_cast<const uint4*>(xs + (t * 16 + r) * kLdsStride + h * 32 + 8 * g);
-				if (P == 0) {
-					bf16x8 av0, av1, bv;
-					memcpy(&av0, &A0, 16);
-					memcpy(&av1, &A1, 16);
-					memcpy(&bv, &Bv, 16);
-					acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av0, bv, acc[0][t], 0, 0, 0);
-					acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av1, bv, acc[1][t], 0, 0, 0);
-				} else {
-					f16x8 av0, av1, bv;
-					memcpy(&av0, &A0, 16);
-					memcpy(&av1, &A1, 16);
-					memcpy(&bv, &Bv, 16);
-					acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av0, bv, acc[0][t], 0, 0, 0);
-					acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av1, bv, acc[1][t], 0, 0, 0);
+				const int off = (t * 16 + r) * kLdsStride + h * 32 + 8 * g;
+				uint4 Bv = *reinterpret_cast<const uint4*>(&xs[buf][0][off]);
+				acc[0][t] = mfma16<P>(cur[0][0][h], Bv, acc[0][t]);
+				acc[1][t] = mfma16<P>(cur[0][1][h], Bv, acc[1][t]);
+				if (SPLIT) {
+					uint4 Bl = *reinterpret_cast<const uint4*>(&xs[buf][NP - 1][off]);
+					acc[0][t] = mfma16<P>(cur[0][0][h], Bl, acc[0][t]);
+					acc[1][t] = mfma16<P>(cur[0][1][h], Bl, acc[1][t]);
+					acc[0][t] = mfma16<P>(cur[NP - 1][0][h], Bv, acc[0][t]);
+					acc[1][t] = mfma16<P>(cur[NP - 1][1][h], Bv, acc[1][t]);
 				}
 			}
 		}
+		if (more)
+			store_x(buf ^ 1);
+		__syncthreads();
+		buf ^= 1;
 	}
 	// part[s][c][i]: column-major per split
 	float* out = part + (uint64_t)blockIdx.y * kRPad * Np;
@@ -400,6 +473,13 @@ __global__ void k_to_rowmajor(const A* __restrict__ xT, uint32_t N, uint32_t Np,
 	x[t] = (B)xT[(uint64_t)c * Np + j];
 }
 
+template <class A, class B> __global__ void k_copy_cast(uint64_t n, const A* __restrict__ x, B* __restrict__ y)
+{
+	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
+	if (t < n)
+		y[t] = (B)x[t];
+}
+
 template <class A, class B> __global__ void k_axpy(uint64_t n, const A* __restrict__ x, B* __restrict__ y)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
@@ -407,13 +487,13 @@ template <class A, class B> __global__ void k_axpy(uint64_t n, const A* __restri
 		y[t] += (B)x[t];
 }
 
-// r = a - b (FP64)
-__global__ void k_sub64(uint64_t n, const double* __restrict__ a, const double* __restrict__ b,
-                        double* __restrict__ r)
+// r = a - b, formed in FP64
+__global__ void k_resid64(uint64_t n, const float* __restrict__ a, const float* __restrict__ b,
+                          double* __restrict__ r)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t < n)
-		r[t] = a[t] - b[t];
+		r[t] = (double)a[t] - (double)b[t];
 }
 
 // ===========================================================================
@@ -460,6 +540,7 @@ void RadSolver::release()
 {
 	dfree(g32_);
 	dfree(g16_);
+	dfree(g16lo_);
 	dfree(E_);
 	dfree(diff_);
 	dfree(area_);
@@ -470,6 +551,7 @@ void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3,
 	if (R_ == 0 || R_ > (uint32_t)kRPad)
 		throw std::runtime_error("RadSolver: 1..192 right-hand sides");
 	R = R_;
+	built_form_ = -1; // G carries the areas: rebuild for a new problem
 	hipStream_t s = (hipStream_t)s_;
 	dfree(E_);
 	dfree(diff_);
@@ -493,10 +575,12 @@ void RadSolver::build_dense(int form)
 		if (!g32_)
 			g32_ = dalloc<float>((uint64_t)Np * Np);
 		k_dense_build<-1><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-		                                                dev.rad.nrm, area_, 1.0f, g32_);
+		                                                dev.rad.nrm, area_, 1.0f, g32_, nullptr);
 	} else {
 		if (!g16_)
 			g16_ = dalloc<uint16_t>((uint64_t)Np * Np);
+		if (!g16lo_)
+			g16lo_ = dalloc<uint16_t>((uint64_t)Np * Np);
 		gscale_ = 1.0f;
 		if (form == kDenseF16) {
 			// scale so that the largest entry sits near 2^14: small factors
@@ -512,10 +596,10 @@ void RadSolver::build_dense(int form)
 			memcpy(&mx, &hm, 4);
 			gscale_ = mx > 0 ? ldexpf(1.0f, 14 - ilogbf(mx) - 1) : 1.0f;
 			k_dense_build<1><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-			                                               dev.rad.nrm, area_, gscale_, g16_);
+			                                               dev.rad.nrm, area_, gscale_, g16_, g16lo_);
 		} else {
 			k_dense_build<0><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-			                                               dev.rad.nrm, area_, 1.0f, g16_);
+			                                               dev.rad.nrm, area_, 1.0f, g16_, g16lo_);
 		}
 	}
 	HIPCHECK(hipGetLastError());
@@ -532,8 +616,11 @@ uint32_t RadSolver::ksplit() const
 	return s;
 }
 
-// y (column-major, R x Np, FP32) = K x for the dense forms
-void RadSolver::apply_dense(int form, const float* xT, float* yT, float* part, void* xq)
+// y (column-major, R x Np, FP32) = K x for the dense forms.  split: the
+// error-compensated product of the 16-bit forms (hi + lo planes); xqlo is
+// the right-hand side's lo plane.
+void RadSolver::apply_dense(int form, const float* xT, float* yT, float* part, void* xq, uint16_t* xqlo,
+                            bool split)
 {
 	hipStream_t s = (hipStream_t)s_;
 	uint32_t ks = ksplit();
@@ -545,14 +632,21 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, float* part, v
 		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part, ks, Np, R, nullptr, yT);
 	} else {
 		// per-column power-of-two scale keeps FP16 operands in range
-		if (form == kDenseF16)
-			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, cscale_, N, Np, R, (uint16_t*)xq);
-		else
-			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, cscale_, N, Np, R, (uint16_t*)xq);
-		if (form == kDenseF16)
-			k_gemm_mfma<1><<<grid, kB, 0, s>>>(g16_, (const uint16_t*)xq, Np, ks, part);
-		else
-			k_gemm_mfma<0><<<grid, kB, 0, s>>>(g16_, (const uint16_t*)xq, Np, ks, part);
+		uint16_t* lo = split ? xqlo : nullptr;
+		const uint16_t* q = (const uint16_t*)xq;
+		if (form == kDenseF16) {
+			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, cscale_, N, Np, R, (uint16_t*)xq, lo);
+			if (split)
+				k_gemm_mfma<1, true><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part);
+			else
+				k_gemm_mfma<1, false><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part);
+		} else {
+			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, cscale_, N, Np, R, (uint16_t*)xq, lo);
+			if (split)
+				k_gemm_mfma<0, true><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part);
+			else
+				k_gemm_mfma<0, false><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part);
+		}
 		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part, ks, Np, R, cunscale_, yT);
 	}
 	HIPCHECK(hipGetLastError());
@@ -608,14 +702,7 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 			dfree(y);
 		} else {
 			float *x = dalloc<float>(nR), *y = dalloc<float>(nR);
-			k_to_colmajor<double, float><<<gridn(nR), kB, 0, s>>>(E_, N, N, R, x); // placeholder layout fix below
-			// row-major FP32 copy of E
-			k_to_rowmajor<float, float><<<1, 1, 0, s>>>(x, 0, N, R, y);           // no-op keeps types instantiated
-			std::vector<double> he(nR);
-			HIPCHECK(hipMemcpyAsync(he.data(), E_, 8 * nR, hipMemcpyDeviceToHost, s));
-			HIPCHECK(hipStreamSynchronize(s));
-			std::vector<float> hf(he.begin(), he.end());
-			HIPCHECK(hipMemcpy(x, hf.data(), 4 * nR, hipMemcpyHostToDevice));
+			k_copy_cast<double, float><<<gridn(nR), kB, 0, s>>>(nR, E_, x);
 			Ev ev2(s);
 			for (int b = 0; b < bounces; ++b) {
 				k_spmm<float><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, x, y);
@@ -639,8 +726,9 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		for (auto& p : z)
 			p = dalloc<float>(nRp);
 		float* part = dalloc<float>((uint64_t)ksplit() * kRPad * Np);
-		void* xq = form == kDenseF32 ? (void*)dalloc<float>((uint64_t)kRPad * Np)
-		                             : (void*)dalloc<uint16_t>((uint64_t)kRPad * Np);
+		const bool half = form != kDenseF32;
+		void* xq = half ? (void*)dalloc<uint16_t>((uint64_t)kRPad * Np) : (void*)dalloc<float>((uint64_t)kRPad * Np);
+		uint16_t* xqlo = half ? dalloc<uint16_t>((uint64_t)kRPad * Np) : nullptr;
 		cscale_ = dalloc<float>(kRPad);
 		cunscale_ = dalloc<float>(kRPad);
 		k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(E_, N, Np, R, z[0]);
@@ -648,14 +736,19 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		{
 			Ev ev(s);
 			for (int b = 1; b <= bounces; ++b)
-				apply_dense(form, z[b - 1], z[b], part, xq);
+				apply_dense(form, z[b - 1], z[b], part, xq, nullptr, false);
 			times.series_ms = ev.stop();
 		}
 		times.per_bounce_ms = times.series_ms / std::max(1, bounces);
-		// iterative refinement on the block system z_k = K z_{k-1}, z_0 = E:
-		// residual with the exact link operator in FP64, correction with the
-		// dense low-precision operator
-		double *zr = dalloc<double>(nR), *kz = dalloc<double>(nR), *rho = dalloc<double>(nR);
+		// Iterative refinement on the block system z_b = K z_{b-1}, z_0 = E.
+		// One sweep: residuals rho_b = K z_{b-1} - z_b of the current iterate,
+		// with K z_{b-1} from the error-compensated (hi + lo) MFMA product and
+		// the difference taken in FP64; corrections d_b = Kdense d_{b-1} + rho_b
+		// (d_0 = 0: z_0 = E is exact) with the plain 16-bit product; then
+		// z += d.  z_{b-1} is corrected only after rho_b has used its old value.
+		double* zr = dalloc<double>(nR);
+		double* rho = dalloc<double>(nRp);
+		float* kz = dalloc<float>(nRp);
 		float* d = dalloc<float>(nRp);
 		float* dprev = dalloc<float>(nRp);
 		{
@@ -663,22 +756,18 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 			for (int it = 0; it < refine; ++it) {
 				HIPCHECK(hipMemsetAsync(dprev, 0, 4 * nRp, s));
 				for (int b = 1; b <= bounces; ++b) {
-					k_to_rowmajor<float, double><<<gridn(nR), kB, 0, s>>>(z[b - 1], N, Np, R, zr);
-					k_spmm<double><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, zr, kz);
-					k_to_rowmajor<float, double><<<gridn(nR), kB, 0, s>>>(z[b], N, Np, R, zr);
-					k_sub64<<<gridn(nR), kB, 0, s>>>(nR, kz, zr, rho);
-					// d_b = Kdense d_{b-1} + rho_b
-					apply_dense(form, dprev, d, part, xq);
-					k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(rho, N, Np, R, dprev);
-					k_axpy<float, float><<<gridn(nRp), kB, 0, s>>>(nRp, d, dprev);
-					// dprev now holds d_b; z_{b-1} stays as used by the next residual
-					std::swap(d, dprev);
-					k_axpy<float, float><<<gridn(nRp), kB, 0, s>>>(nRp, d, z[b]);
+					apply_dense(form, z[b - 1], kz, part, xq, xqlo, half);               // K z_{b-1}
+					k_resid64<<<gridn(nRp), kB, 0, s>>>(nRp, kz, z[b], rho);             // rho_b (FP64)
+					apply_dense(form, dprev, d, part, xq, nullptr, false);               // Kdense d_{b-1}
+					k_axpy<double, float><<<gridn(nRp), kB, 0, s>>>(nRp, rho, d);        // d_b
+					k_axpy<float, float><<<gridn(nRp), kB, 0, s>>>(nRp, dprev, z[b - 1]); // apply d_{b-1}
 					std::swap(d, dprev);
 				}
+				k_axpy<float, float><<<gridn(nRp), kB, 0, s>>>(nRp, dprev, z[bounces]);
 			}
 			times.refine_ms = ev.stop();
 		}
+		dfree(xqlo);
 		HIPCHECK(hipGetLastError());
 		// total = sum_k z_k in FP64 (z_0 = E exactly)
 		for (int b = 1; b <= bounces; ++b) {

@@ -39,7 +39,7 @@ public:
 
 private:
 	void release();
-	uint32_t ksplit() const;
+	uint32_t ksplit(uint32_t rows_per_block, uint32_t min_blocks) const;
 	void apply_dense(int form, const float* xT, float* yT, float* part, void* xq, uint16_t* xqlo, bool split);
 	void column_scales(int form, const float* xT);
 	DeviceScene& dev;
@@ -48,6 +48,7 @@ private:
 	uint16_t* g16_ = nullptr;   // 16-bit matrix (hi plane)
 	uint16_t* g16lo_ = nullptr; // remainder plane for the compensated product
 	int built_form_ = -1;
+	uint32_t ncu_ = 256; // compute units of the device
 	float gscale_ = 1.0f;
 	double* E_ = nullptr;
 	float* diff_ = nullptr;

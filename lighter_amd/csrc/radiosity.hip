@@ -15,7 +15,7 @@
 //
 // Layouts: dense G is row-major Np x Np (rows = receiving patch), the dense
 // right-hand sides are column-major (R x Np, each column contiguous), the
-// sparse ones row-major (Np x R).  Np is N rounded up to kTileM.
+// sparse ones row-major (Np x R).  Np is N rounded up to kPadM.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -23,6 +23,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "device.h"
@@ -42,10 +43,11 @@ namespace lgx {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short i16x8 __attribute__((ext_vector_type(8))); // 16-bit MFMA operand fragment as loaded
 
 constexpr int kB = 256;          // threads per block
-constexpr int kTileM = 128;      // rows per block of the dense GEMMs
-constexpr int kKStage = 64;      // k per LDS stage
+constexpr int kTileM = 128;      // rows per block of the FP32 GEMM
+constexpr int kPadM = 256;       // Np is a multiple of this (rows per block of the plain MFMA GEMM)
 constexpr int kColTiles = 12;    // 16-wide column tiles: R padded to 192
 constexpr int kRPad = 16 * kColTiles;
 
@@ -209,134 +211,232 @@ __global__ __launch_bounds__(kB) void k_prep_x32(const float* __restrict__ xT, c
 
 // ---------------------------------------------------------------------------
 // MFMA GEMM: part[s][c][i] = sum_{j in split s} G[i][j] * Xq[c][j]
-// Block: 128 rows x 192 columns, 4 waves of 32 rows; v_mfma_f32_16x16x32
-// (bf16 or f16) with FP32 accumulators.  Lane l supplies row (l & 15) of A
-// and column (l & 15) of B with the same eight k indices 8*(l>>4)+e, so the
-// k order inside an instruction is immaterial; output element i of a 16x16
-// tile sits at row 4*(l>>4)+i, column l & 15.
-// The right-hand-side chunk (192 x 64) is staged through LDS per k stage.
+//
+// Block: 4 waves, each RT row tiles of 16 x all 12 column tiles (R padded to
+// 192), v_mfma_f32_16x16x32 (bf16 or f16) with FP32 accumulators.  Lane l
+// supplies row (l & 15) of A and column (l & 15) of B with the same eight k
+// indices 8*(l>>4)+e, so the k order inside an instruction is immaterial;
+// output element i of a 16x16 tile sits at row 4*(l>>4)+i, column l & 15.
+//
+// k advances in stages of 32.  A stage (the block's matrix rows x 32 and the
+// right-hand-side chunk 192 x 32, per plane) goes HBM -> LDS by
+// global_load_lds_dwordx4 (no register staging), PD stages in flight in a
+// ring of PD LDS buffers.  LDS holds 16-row x 64-byte groups of 1 KB, one
+// DMA instruction each; the 16-byte pieces of a row are permuted by
+// piece_swz(row) so that the rows one ds_read_b128 serves together fall in
+// distinct banks.  Each wave fetches its own matrix rows and a quarter of
+// the right-hand-side chunk, waits on its own DMA count, then a barrier makes
+// every wave's part visible.
 // ---------------------------------------------------------------------------
-constexpr int kLdsStride = kKStage + 8; // 16-bit elements per LDS column (pad)
-constexpr int kXPieces = (kRPad * kKStage / 8) / kB; // 16-byte pieces per thread per stage
+constexpr int kKS = 32; // k per stage
 
 template <int P>
-__device__ __forceinline__ f32x4 mfma16(uint4 a, uint4 b, f32x4 c)
+__device__ __forceinline__ f32x4 mfma16(i16x8 a, i16x8 b, f32x4 c)
 {
-	if (P == 0) {
-		bf16x8 av, bv;
-		memcpy(&av, &a, 16);
-		memcpy(&bv, &b, 16);
-		return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
-	} else {
-		f16x8 av, bv;
-		memcpy(&av, &a, 16);
-		memcpy(&bv, &b, 16);
-		return __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, c, 0, 0, 0);
+	if (P == 0)
+		return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+		                                               0, 0, 0);
+	else
+		return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c,
+		                                              0, 0, 0);
+}
+
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ int piece_swz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+
+// wait until at most N vector-memory operations of this wave are outstanding
+template <int N>
+__device__ __forceinline__ void wait_vm()
+{
+	static_assert(N >= 0 && N < 64, "vmcnt range");
+	asm volatile("" ::: "memory");
+	__builtin_amdgcn_s_waitcnt(0x0F70 | (N & 15) | ((N >> 4) << 14));
+	asm volatile("" ::: "memory");
+}
+
+__device__ __forceinline__ void block_barrier()
+{
+	asm volatile("" ::: "memory");
+	__builtin_amdgcn_s_barrier();
+	asm volatile("" ::: "memory");
+}
+
+// 16 bytes per lane from global memory to LDS at dst + 16 * lane
+__device__ __forceinline__ void dma16(const uint16_t* src, unsigned char* dst)
+{
+	__builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)dst, 16, 0, 0);
+}
+
+template <int NPL, int RT>
+struct GemmShape {
+	static constexpr int kRows = 64 * RT;                       // block rows
+	static constexpr int kXBytes = kRPad * kKS * 2;             // right-hand-side plane per stage
+	static constexpr int kGBytes = kRows * kKS * 2;             // matrix plane per stage
+	static constexpr int kStage = NPL * (kXBytes + kGBytes);    // LDS bytes per stage
+	static constexpr int kOps = NPL * (kColTiles / 4 + RT);     // DMA instructions per wave per stage
+};
+
+// issue stage st (k offset kofs from the split's start) into LDS buffer base
+template <int NPL, int RT>
+__device__ __forceinline__ void gemm_issue(const uint16_t* G, const uint16_t* Glo, const uint16_t* xq,
+                                           const uint16_t* xlo, uint64_t xsrc, uint64_t gsrc, uint64_t Np,
+                                           uint32_t kofs, unsigned char* base, int wave)
+{
+	typedef GemmShape<NPL, RT> S;
+#pragma unroll
+	for (int pl = 0; pl < NPL; ++pl) {
+		const uint16_t* X = pl ? xlo : xq;
+		const uint16_t* M = pl ? Glo : G;
+		unsigned char* xb = base + pl * S::kXBytes + wave * (kColTiles / 4) * 1024;
+		unsigned char* gb = base + NPL * S::kXBytes + pl * S::kGBytes + wave * RT * 1024;
+#pragma unroll
+		for (int j = 0; j < kColTiles / 4; ++j)
+			dma16(X + xsrc + (uint64_t)j * 16 * Np + kofs, xb + j * 1024);
+#pragma unroll
+		for (int t = 0; t < RT; ++t)
+			dma16(M + gsrc + (uint64_t)t * 16 * Np + kofs, gb + t * 1024);
 	}
 }
 
-// SPLIT: the matrix and the right-hand side come as hi + lo planes and the
-// kernel forms G_hi X_hi + G_hi X_lo + G_lo X_hi (the lo x lo term is below
-// FP32 resolution), an error-compensated product.
-// The next k stage's loads (matrix fragments to registers, right-hand-side
-// chunk to registers then to the other LDS buffer) are issued before the
-// current stage's MFMAs, so HBM traffic overlaps the math.
-template <int P, bool SPLIT>
+template <int NPL, int RT>
+struct GemmLane {
+	uint64_t xsrc, gsrc; // DMA source offsets (elements) at the split's first k
+	int wave, rofs;      // wave index; ds_read_b128 offset of the fragment in a 1 KB group
+	unsigned char* buf[4]; // the LDS stage ring
+};
+
+// one k stage (buffer B of PD): wait for it, free the previous buffer, issue
+// the stage PD - 1 ahead into that buffer, multiply
+template <int P, int NPL, int RT, int PD, int B>
+__device__ __forceinline__ void gemm_step(int s, int nst, const GemmLane<NPL, RT>& L, const uint16_t* G,
+                                          const uint16_t* Glo, const uint16_t* xq, const uint16_t* xlo,
+                                          uint64_t Np, f32x4 (&acc)[RT][kColTiles])
+{
+	typedef GemmShape<NPL, RT> S;
+	// stage s has landed once at most the later PD - 2 stages' DMAs are pending
+	wait_vm<S::kOps * (PD - 2)>();
+	block_barrier(); // every wave's part of stage s is in; stage s-1's buffer is free
+	// all of this stage's fragments are read before the next DMA is issued:
+	// the compiler cannot tell that DMA's buffer from this one and would
+	// otherwise wait for every DMA in flight
+	const unsigned char* base = L.buf[B];
+	i16x8 a[NPL][RT], bx[NPL][kColTiles];
+#pragma unroll
+	for (int pl = 0; pl < NPL; ++pl) {
+#pragma unroll
+		for (int t = 0; t < RT; ++t)
+			a[pl][t] = *reinterpret_cast<const i16x8*>(base + NPL * S::kXBytes + pl * S::kGBytes +
+			                                           (L.wave * RT + t) * 1024 + L.rofs);
+#pragma unroll
+		for (int c = 0; c < kColTiles; ++c)
+			bx[pl][c] = *reinterpret_cast<const i16x8*>(base + pl * S::kXBytes + c * 1024 + L.rofs);
+	}
+	// always PD - 1 stages in flight: past the end, the last stage is fetched
+	// again into the freed buffer (never read), which keeps the wait counts
+	// above exact
+	constexpr int BN = (B + PD - 1) % PD;
+	const int ahead = s + PD - 1 < nst ? s + PD - 1 : nst - 1;
+	gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, ahead * kKS, L.buf[BN], L.wave);
+#pragma unroll
+	for (int c = 0; c < kColTiles; ++c) {
+#pragma unroll
+		for (int t = 0; t < RT; ++t)
+			acc[t][c] = mfma16<P>(a[0][t], bx[0][c], acc[t][c]);
+		if (NPL == 2) {
+			// error-compensated: + G_hi X_lo + G_lo X_hi
+#pragma unroll
+			for (int t = 0; t < RT; ++t) {
+				acc[t][c] = mfma16<P>(a[0][t], bx[NPL - 1][c], acc[t][c]);
+				acc[t][c] = mfma16<P>(a[NPL - 1][t], bx[0][c], acc[t][c]);
+			}
+		}
+	}
+}
+
+template <int P, int NPL, int RT, int PD, int... Bs>
+__device__ __forceinline__ void gemm_steps(int s, const GemmLane<NPL, RT>& L, const uint16_t* G,
+                                           const uint16_t* Glo, const uint16_t* xq, const uint16_t* xlo,
+                                           uint64_t Np, f32x4 (&acc)[RT][kColTiles], int nst,
+                                           std::integer_sequence<int, Bs...>)
+{
+	(gemm_step<P, NPL, RT, PD, Bs>(s + Bs, nst, L, G, Glo, xq, xlo, Np, acc), ...);
+}
+
+template <int P, int NPL, int RT, int PD>
 __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G, const uint16_t* __restrict__ Glo,
                                                   const uint16_t* __restrict__ xq, const uint16_t* __restrict__ xlo,
                                                   uint32_t Np, uint32_t ksplit, float* __restrict__ part)
 {
-	constexpr int NP = SPLIT ? 2 : 1; // planes
-	__shared__ uint16_t xs[2][NP][kRPad * kLdsStride];
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const uint32_t i0 = blockIdx.x * kTileM + wave * 32;
-	const uint32_t klen = Np / ksplit;
-	const uint32_t kbeg = blockIdx.y * klen, kend = kbeg + klen;
+	typedef GemmShape<NPL, RT> S;
+	static_assert(PD >= 2, "pipeline depth");
+	const int tid = threadIdx.x, lane = tid & 63;
 	const int r = lane & 15, g = lane >> 4;
-	f32x4 acc[2][kColTiles];
+	// one variable per stage buffer: the compiler's wait-count tracking can
+	// then tell the DMA into one buffer from the reads of another (with a
+	// single array every LDS read would wait for every DMA in flight)
+	__shared__ __align__(1024) unsigned char lds0[S::kStage], lds1[S::kStage], lds2[PD > 2 ? S::kStage : 16],
+	    lds3[PD > 3 ? S::kStage : 16];
+	GemmLane<NPL, RT> L;
+	L.buf[0] = lds0;
+	L.buf[1] = lds1;
+	L.buf[2] = lds2;
+	L.buf[3] = lds3;
+	L.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const uint32_t i0 = blockIdx.x * S::kRows + L.wave * 16 * RT; // this wave's first row
+	const uint32_t klen = Np / ksplit, kbeg = blockIdx.y * klen;
+	const int nst = (int)(klen / kKS);
+	// DMA sources: lane l fills piece (l & 3) of row l >> 2 of a 16-row group,
+	// i.e. k piece (l & 3) ^ piece_swz(l >> 2)
+	const int drow = lane >> 2, dk = 8 * ((lane & 3) ^ piece_swz(lane >> 2));
+	L.xsrc = (uint64_t)(16 * (kColTiles / 4) * L.wave + drow) * Np + kbeg + dk;
+	L.gsrc = (uint64_t)(i0 + drow) * Np + kbeg + dk;
+	L.rofs = r * 64 + 16 * (g ^ piece_swz(r));
+
+	f32x4 acc[RT][kColTiles];
 #pragma unroll
-	for (int a = 0; a < 2; ++a)
+	for (int t = 0; t < RT; ++t)
 #pragma unroll
-		for (int t = 0; t < kColTiles; ++t)
-			acc[a][t] = f32x4{0, 0, 0, 0};
-	const uint64_t row0 = (uint64_t)(i0 + r) * Np, row1 = (uint64_t)(i0 + 16 + r) * Np;
-	// matrix fragments [plane][row tile][k half]; x chunk pieces [plane][piece]
-	uint4 ga[NP][2][2], xr[NP][kXPieces];
-	auto load_stage = [&](uint32_t kb) {
-#pragma unroll
-		for (int pl = 0; pl < NP; ++pl) {
-			const uint16_t* M = pl ? Glo : G;
-			const uint16_t* X = pl ? xlo : xq;
-			ga[pl][0][0] = *reinterpret_cast<const uint4*>(M + row0 + kb + 8 * g);
-			ga[pl][0][1] = *reinterpret_cast<const uint4*>(M + row0 + kb + 32 + 8 * g);
-			ga[pl][1][0] = *reinterpret_cast<const uint4*>(M + row1 + kb + 8 * g);
-			ga[pl][1][1] = *reinterpret_cast<const uint4*>(M + row1 + kb + 32 + 8 * g);
-#pragma unroll
-			for (int q = 0; q < kXPieces; ++q) {
-				int piece = q * kB + tid, c = piece / (kKStage / 8), o = piece % (kKStage / 8);
-				xr[pl][q] = *reinterpret_cast<const uint4*>(X + (uint64_t)c * Np + kb + o * 8);
-			}
-		}
-	};
-	auto store_x = [&](int buf) {
-#pragma unroll
-		for (int pl = 0; pl < NP; ++pl)
-#pragma unroll
-			for (int q = 0; q < kXPieces; ++q) {
-				int piece = q * kB + tid, c = piece / (kKStage / 8), o = piece % (kKStage / 8);
-				*reinterpret_cast<uint4*>(&xs[buf][pl][c * kLdsStride + o * 8]) = xr[pl][q];
-			}
-	};
-	load_stage(kbeg);
-	store_x(0);
-	__syncthreads();
-	int buf = 0;
-	for (uint32_t kb = kbeg; kb < kend; kb += kKStage) {
-		uint4 cur[NP][2][2];
-#pragma unroll
-		for (int pl = 0; pl < NP; ++pl)
-#pragma unroll
-			for (int a = 0; a < 2; ++a)
-#pragma unroll
-				for (int h = 0; h < 2; ++h)
-					cur[pl][a][h] = ga[pl][a][h];
-		const bool more = kb + kKStage < kend;
-		if (more)
-			load_stage(kb + kKStage);
-#pragma unroll
-		for (int h = 0; h < 2; ++h) {
-#pragma unroll
-			for (int t = 0; t < kColTiles; ++t) {
-				const int off = (t * 16 + r) * kLdsStride + h * 32 + 8 * g;
-				uint4 Bv = *reinterpret_cast<const uint4*>(&xs[buf][0][off]);
-				acc[0][t] = mfma16<P>(cur[0][0][h], Bv, acc[0][t]);
-				acc[1][t] = mfma16<P>(cur[0][1][h], Bv, acc[1][t]);
-				if (SPLIT) {
-					uint4 Bl = *reinterpret_cast<const uint4*>(&xs[buf][NP - 1][off]);
-					acc[0][t] = mfma16<P>(cur[0][0][h], Bl, acc[0][t]);
-					acc[1][t] = mfma16<P>(cur[0][1][h], Bl, acc[1][t]);
-					acc[0][t] = mfma16<P>(cur[NP - 1][0][h], Bv, acc[0][t]);
-					acc[1][t] = mfma16<P>(cur[NP - 1][1][h], Bv, acc[1][t]);
-				}
-			}
-		}
-		if (more)
-			store_x(buf ^ 1);
-		__syncthreads();
-		buf ^= 1;
-	}
+		for (int c = 0; c < kColTiles; ++c)
+			acc[t][c] = f32x4{0, 0, 0, 0};
+
+	// prologue: stages 0 .. PD-2 into buffers 0 .. PD-2 (nst >= 1)
+	gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, 0, L.buf[0], L.wave);
+	if (PD > 2)
+		gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, (nst > 1 ? 1 : 0) * kKS,
+		                    L.buf[1], L.wave);
+	if (PD > 3)
+		gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, (nst > 2 ? 2 : nst - 1) * kKS,
+		                    L.buf[2], L.wave);
+	static_assert(PD <= 4, "prologue covers up to four buffers");
+	int s = 0;
+	for (; s + PD <= nst; s += PD)
+		gemm_steps<P, NPL, RT, PD>(s, L, G, Glo, xq, xlo, Np, acc, nst, std::make_integer_sequence<int, PD>{});
+	// fewer than PD stages left
+	if (s < nst)
+		gemm_step<P, NPL, RT, PD, 0>(s, nst, L, G, Glo, xq, xlo, Np, acc);
+	if (PD > 2 && s + 1 < nst)
+		gemm_step<P, NPL, RT, PD, (PD > 2 ? 1 : 0)>(s + 1, nst, L, G, Glo, xq, xlo, Np, acc);
+	if (PD > 3 && s + 2 < nst)
+		gemm_step<P, NPL, RT, PD, (PD > 3 ? 2 : 0)>(s + 2, nst, L, G, Glo, xq, xlo, Np, acc);
+	wait_vm<0>(); // no DMA may land after the block has left its LDS
+
 	// part[s][c][i]: column-major per split
 	float* out = part + (uint64_t)blockIdx.y * kRPad * Np;
 #pragma unroll
-	for (int a = 0; a < 2; ++a)
+	for (int t = 0; t < RT; ++t)
 #pragma unroll
-		for (int t = 0; t < kColTiles; ++t) {
-			uint32_t c = t * 16 + r;
-			uint32_t i = i0 + a * 16 + 4 * g;
-			*reinterpret_cast<float4*>(out + (uint64_t)c * Np + i) =
-			    make_float4(acc[a][t][0], acc[a][t][1], acc[a][t][2], acc[a][t][3]);
+		for (int c = 0; c < kColTiles; ++c) {
+			const uint32_t col = c * 16 + r, i = i0 + t * 16 + 4 * g;
+			*reinterpret_cast<float4*>(out + (uint64_t)col * Np + i) =
+			    make_float4(acc[t][c][0], acc[t][c][1], acc[t][c][2], acc[t][c][3]);
 		}
 }
+
+// the two instantiations: plain (64 rows per wave, 4 stages in flight) and
+// error-compensated hi + lo (32 rows per wave, 3 stages)
+constexpr int kPlainRT = 4, kPlainPD = 4, kSplitRT = 2, kSplitPD = 3;
 
 // ---------------------------------------------------------------------------
 // plain FP32 GEMM with the same blocking: 256 threads, each 8 rows x 12
@@ -528,10 +628,14 @@ RadSolver::RadSolver(DeviceScene& d) : dev(d)
 	if (!d.rad.rowptr)
 		throw std::runtime_error("RadSolver: build the links first");
 	N = (uint32_t)d.npatches;
-	Np = ((N + kTileM - 1) / kTileM) * kTileM;
+	Np = ((N + kPadM - 1) / kPadM) * kPadM;
 	if (Np < 512)
 		Np = 512;
 	s_ = d.stream();
+	int dv = 0, ncu = 0;
+	HIPCHECK(hipGetDevice(&dv));
+	HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dv));
+	ncu_ = ncu > 0 ? (uint32_t)ncu : 256;
 }
 
 RadSolver::~RadSolver() { release(); }
@@ -607,11 +711,11 @@ void RadSolver::build_dense(int form)
 	built_form_ = form;
 }
 
-uint32_t RadSolver::ksplit() const
+uint32_t RadSolver::ksplit(uint32_t rows_per_block, uint32_t min_blocks) const
 {
-	// enough blocks to fill the chip: split k until >= 2048 blocks
-	uint32_t rows = Np / kTileM, s = 1;
-	while (rows * s < 2048 && (Np / (s * 2)) % kKStage == 0 && s < 64)
+	// split k (in whole stages) until every CU has a block
+	uint32_t rows = Np / rows_per_block, s = 1;
+	while (rows * s < min_blocks && (Np / (s * 2)) % 32 == 0 && s < 64)
 		s *= 2;
 	return s;
 }
@@ -623,10 +727,10 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, float* part, v
                             bool split)
 {
 	hipStream_t s = (hipStream_t)s_;
-	uint32_t ks = ksplit();
-	dim3 grid(Np / kTileM, ks);
 	uint64_t nq = (uint64_t)kRPad * Np;
 	if (form == kDenseF32) {
+		uint32_t ks = ksplit(kTileM, 2048);
+		dim3 grid(Np / kTileM, ks);
 		k_prep_x32<<<gridn(nq), kB, 0, s>>>(xT, diff_, N, Np, R, (float*)xq);
 		k_gemm_f32<<<grid, kB, 0, s>>>(g32_, (const float*)xq, Np, ks, part);
 		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part, ks, Np, R, nullptr, yT);
@@ -634,19 +738,21 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, float* part, v
 		// per-column power-of-two scale keeps FP16 operands in range
 		uint16_t* lo = split ? xqlo : nullptr;
 		const uint16_t* q = (const uint16_t*)xq;
-		if (form == kDenseF16) {
+		if (form == kDenseF16)
 			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, cscale_, N, Np, R, (uint16_t*)xq, lo);
-			if (split)
-				k_gemm_mfma<1, true><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part);
-			else
-				k_gemm_mfma<1, false><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part);
-		} else {
+		else
 			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, cscale_, N, Np, R, (uint16_t*)xq, lo);
-			if (split)
-				k_gemm_mfma<0, true><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part);
-			else
-				k_gemm_mfma<0, false><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part);
-		}
+		const uint32_t rows = 64 * (split ? kSplitRT : kPlainRT);
+		const uint32_t ks = ksplit(rows, ncu_);
+		dim3 grid(Np / rows, ks);
+		if (form == kDenseF16 && split)
+			k_gemm_mfma<1, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part);
+		else if (form == kDenseF16)
+			k_gemm_mfma<1, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part);
+		else if (split)
+			k_gemm_mfma<0, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part);
+		else
+			k_gemm_mfma<0, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part);
 		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part, ks, Np, R, cunscale_, yT);
 	}
 	HIPCHECK(hipGetLastError());
@@ -725,7 +831,8 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		std::vector<float*> z(bounces + 1);
 		for (auto& p : z)
 			p = dalloc<float>(nRp);
-		float* part = dalloc<float>((uint64_t)ksplit() * kRPad * Np);
+		const uint32_t kmax = std::max(ksplit(kTileM, 2048), std::max(ksplit(64 * kPlainRT, ncu_), ksplit(64 * kSplitRT, ncu_)));
+		float* part = dalloc<float>((uint64_t)kmax * kRPad * Np);
 		const bool half = form != kDenseF32;
 		void* xq = half ? (void*)dalloc<uint16_t>((uint64_t)kRPad * Np) : (void*)dalloc<float>((uint64_t)kRPad * Np);
 		uint16_t* xqlo = half ? dalloc<uint16_t>((uint64_t)kRPad * Np) : nullptr;

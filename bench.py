@@ -223,7 +223,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
     nl = C.c_size_t(0)
     t0 = time.perf_counter()
     _lib.check(lib.lgx_RadLinks(dev, P(pos), P(pn), N, C.byref(nl)), lib)
-    tm = (C.c_double * 5)()
+    tm = (C.c_double * 6)()
     lib.lgx_DeviceTimes(dev, tm)
     link_ms = tm[3]
     log(f"[bench] C2: {N} patches, {nl.value} links ({link_ms:.1f} ms GPU, {time.perf_counter() - t0:.1f} s wall)")
@@ -352,7 +352,7 @@ def main():
     log(f"[bench] BVH {info[0]} tris, {info[1]} nodes, depth {info[2]}; marches {marches}, steps {steps}, "
         f"node visits {nodes}, triangle tests {ntris}")
 
-    tm = (C.c_double * 5)()
+    tm = (C.c_double * 6)()
     for _ in range(args.warmup):
         _lib.check(lib.lgx_DirectLightRun(dev), lib)
     march_ms = []

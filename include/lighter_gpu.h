@@ -129,16 +129,33 @@ LTRAPI int lgx_RadBounce(lgx_Device* dev, const float* diffuse3, const float* em
  * E and total are N x R row-major FP64.  form: 0 sparse FP64 gather,
  * 1 sparse FP32 gather, 2 dense FP32, 3 dense BF16 on MFMA, 4 dense FP16 on
  * MFMA (FP32 accumulation); refine: refinement sweeps for the dense forms
- * with the exact link operator's residual in FP64.  times_ms: dense build,
- * series, per bounce, refinement. */
+ * with an FP64 residual from the error-compensated (hi + lo) product.
+ * times_ms: dense build, series, per bounce, refinement. */
 LTRAPI int lgx_RadSolve(lgx_Device* dev, const double* E, uint32_t R, const float* diffuse3, const float* area,
                         int form, int bounces, int refine, double* total, double times_ms[4]);
+/* Reverse mode of lgx_RadSolve (no counterpart in the reference, which has
+ * no derivatives): for the upstream gradient g = dL/dtotal (N x R FP64),
+ * total again, lam = dL/dE (which is also dL/d emission) and
+ * dalb = dL/d diffuse per column (column c holds the derivative through the
+ * albedo of channel c % 3; sum the columns of one channel).  The adjoint
+ * sweep applies K^T = diag(diffuse) diag(area) F with the same kernels.
+ * times_ms: dense build, forward series, reverse per bounce, reverse sweep. */
+LTRAPI int lgx_RadSolveGrad(lgx_Device* dev, const double* E, uint32_t R, const float* diffuse3,
+                            const float* area, int form, int bounces, const double* g, double* total, double* lam,
+                            double* dalb, double times_ms[4]);
+/* Reverse mode of the last lgx_DirectLight / lgx_DirectLightRun (no
+ * counterpart in the reference): the lightmap gains colour * weight per work
+ * item (Job_LMRender_*_Inner, lighter.cpp:485-589), so for lam3 = dL/dlightmap
+ * (3 FP64 per device sample) dcolor3[3 l + c] = sum over light l's items of
+ * weight * lam3[3 s + c]. */
+LTRAPI int lgx_DirectLightGrad(lgx_Device* dev, const double* lam3, double* dcolor3);
 /* Traversal counters (node visits, triangle tests, march steps, queries,
  * marches) collected by the next stage when enabled. */
 LTRAPI int lgx_SetCounting(lgx_Device* dev, int on);
 LTRAPI int lgx_ReadCounters(lgx_Device* dev, uint64_t out[5]);
-/* Last stage times in ms: direct, direct march, AO, links, bounces. */
-LTRAPI int lgx_DeviceTimes(lgx_Device* dev, double out[5]);
+/* Last stage times in ms: direct, direct march, AO, links, bounces,
+ * direct-light gradient. */
+LTRAPI int lgx_DeviceTimes(lgx_Device* dev, double out[6]);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,10 @@ LGX_SIGNATURES = {
     "lgx_RadBounce": (C.c_int, [vp, fp, fp, fp, C.c_int, fp]),
     "lgx_RadSolve": (C.c_int, [vp, C.POINTER(C.c_double), C.c_uint32, fp, fp, C.c_int, C.c_int, C.c_int,
                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "lgx_RadSolveGrad": (C.c_int, [vp, C.POINTER(C.c_double), C.c_uint32, fp, fp, C.c_int, C.c_int,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "lgx_DirectLightGrad": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "lgx_SetCounting": (C.c_int, [vp, C.c_int]),
     "lgx_ReadCounters": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
     "lgx_DeviceTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),

@@ -604,6 +604,7 @@ int lgx_RadLinks(lgx_Device* d, const float* pos, const float* nrm, size_t n, si
 	return guarded([&] {
 		if (n > kMaxPatches)
 			throw std::runtime_error("lgx_RadLinks: too many patches");
+		d->rad.reset(); // its dense matrix belongs to the old links
 		d->dev.rad_set_patches(pos, nrm, n);
 		size_t nl = d->dev.rad_links();
 		if (nlinks)
@@ -647,6 +648,39 @@ int lgx_RadSolve(lgx_Device* d, const double* E, uint32_t R, const float* diffus
 	});
 }
 
+int lgx_RadSolveGrad(lgx_Device* d, const double* E, uint32_t R, const float* diffuse3, const float* area,
+                     int form, int bounces, const double* g, double* total, double* lam, double* dalb,
+                     double times_ms[4])
+{
+	return guarded([&] {
+		if (form < kSparseF64 || form > kDenseF16)
+			throw std::runtime_error("lgx_RadSolveGrad: unknown form");
+		if (bounces < 0)
+			throw std::runtime_error("lgx_RadSolveGrad: negative bounce count");
+		if (!g || !total || !lam || !dalb)
+			throw std::runtime_error("lgx_RadSolveGrad: null output");
+		if (!d->rad || d->rad->N != d->dev.npatches)
+			d->rad.reset(new RadSolver(d->dev));
+		d->rad->set_problem(E, R, diffuse3, area);
+		d->rad->solve_grad(form, bounces, g, total, lam, dalb);
+		if (times_ms) {
+			times_ms[0] = d->rad->times.build_ms;
+			times_ms[1] = d->rad->times.series_ms;
+			times_ms[2] = d->rad->times.per_bounce_ms;
+			times_ms[3] = d->rad->times.refine_ms;
+		}
+	});
+}
+
+int lgx_DirectLightGrad(lgx_Device* d, const double* lam3, double* dcolor3)
+{
+	return guarded([&] {
+		if (!lam3 || !dcolor3)
+			throw std::runtime_error("lgx_DirectLightGrad: null argument");
+		d->dev.direct_grad(lam3, dcolor3);
+	});
+}
+
 int lgx_SetCounting(lgx_Device* d, int on)
 {
 	d->dev.count_queries = on != 0;
@@ -663,7 +697,7 @@ int lgx_ReadCounters(lgx_Device* d, uint64_t out[5])
 	return 1;
 }
 
-int lgx_DeviceTimes(lgx_Device* d, double out[5])
+int lgx_DeviceTimes(lgx_Device* d, double out[6])
 {
 	const StageTimes& t = d->dev.times;
 	out[0] = t.direct_ms;
@@ -671,6 +705,7 @@ int lgx_DeviceTimes(lgx_Device* d, double out[5])
 	out[2] = t.ao_ms;
 	out[3] = t.links_ms;
 	out[4] = t.bounce_ms;
+	out[5] = t.grad_ms;
 	return 1;
 }
 

@@ -26,7 +26,7 @@ struct DevLight {
 // Per-stage timing in milliseconds measured with HIP events on the stream
 // the kernels run on.
 struct StageTimes {
-	double direct_ms = 0, direct_march_ms = 0, ao_ms = 0, links_ms = 0, bounce_ms = 0;
+	double direct_ms = 0, direct_march_ms = 0, ao_ms = 0, links_ms = 0, bounce_ms = 0, grad_ms = 0;
 };
 
 // Optional traversal counters (when enabled): node visits and triangle
@@ -66,6 +66,9 @@ public:
 	void run_direct();
 	const float* weights_device() const { return d_w_; }
 	void get_weights(float* out, size_t n) const;
+	// reverse mode of the last direct-light run: dcolor3[l] = sum over light
+	// l's items of weight * lam3[sample] (lam3: 3 per device sample)
+	void direct_grad(const double* lam3, double* dcolor3);
 	// shadow factor per item (1 where no march ran); kept when keep_weights
 	void get_vis(float* out, size_t n) const;
 	size_t num_items() const { return nitems_; }

@@ -524,6 +524,50 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const uint32_t* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Reverse mode of the accumulation above: the lightmap gains color_l * w
+// per item (lighter.cpp:505-513), so dL/dcolor_l[c] = sum over light l's
+// items of w * lam[s][c], lam = dL/dlightmap per sample.  kGradBlocks blocks
+// per light write FP64 partial sums, reduced in a fixed order on the host.
+// ---------------------------------------------------------------------------
+constexpr int kGradBlocks = 64;
+
+__global__ __launch_bounds__(kBlock) void k_color_grad(const uint32_t* __restrict__ items,
+                                                       const float* __restrict__ w, uint32_t begin, uint32_t end,
+                                                       const double* __restrict__ lam, double* __restrict__ part)
+{
+	double a0 = 0, a1 = 0, a2 = 0;
+	for (uint32_t k = begin + blockIdx.x * kBlock + threadIdx.x; k < end; k += kGradBlocks * kBlock) {
+		float wk = w[k];
+		if (__float_as_uint(wk) == kSkipBits)
+			continue;
+		uint32_t s = items[k] & kSampleMask;
+		a0 += (double)wk * lam[3 * (uint64_t)s];
+		a1 += (double)wk * lam[3 * (uint64_t)s + 1];
+		a2 += (double)wk * lam[3 * (uint64_t)s + 2];
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) {
+		a0 += __shfl_xor(a0, o);
+		a1 += __shfl_xor(a1, o);
+		a2 += __shfl_xor(a2, o);
+	}
+	__shared__ double red[kBlock / 64][3];
+	const int wave = threadIdx.x >> 6;
+	if ((threadIdx.x & 63) == 0) {
+		red[wave][0] = a0;
+		red[wave][1] = a1;
+		red[wave][2] = a2;
+	}
+	__syncthreads();
+	if (threadIdx.x < 3) {
+		double t = 0;
+		for (int q = 0; q < kBlock / 64; ++q)
+			t += red[q][threadIdx.x];
+		part[3 * blockIdx.x + threadIdx.x] = t;
+	}
+}
+
+// ---------------------------------------------------------------------------
 // ambient occlusion (lighter.cpp:799-841)
 // ---------------------------------------------------------------------------
 struct AOParams {
@@ -1077,6 +1121,39 @@ void DeviceScene::direct_light(const std::vector<DevLight>& lights, const std::v
 {
 	set_direct_work(lights, items, light_off, keep_weights);
 	run_direct();
+}
+
+void DeviceScene::direct_grad(const double* lam3, double* dcolor3)
+{
+	hipStream_t s = (hipStream_t)stream_;
+	const size_t L = h_lights_.size();
+	if (!d_w_ && nitems_)
+		throw std::runtime_error("direct_grad: no direct-light work on the device");
+	double* dlam = dalloc<double>(3 * std::max<size_t>(nsamples_, 1));
+	double* dpart = dalloc<double>(3 * kGradBlocks * std::max<size_t>(L, 1));
+	HIPCHECK(hipMemcpyAsync(dlam, lam3, 24 * nsamples_, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemsetAsync(dpart, 0, 24 * kGradBlocks * std::max<size_t>(L, 1), s));
+	EventTimer tm(s);
+	for (size_t l = 0; l < L; ++l) {
+		uint32_t b = h_light_off_[l], e = h_light_off_[l + 1];
+		if (e > b) {
+			k_color_grad<<<kGradBlocks, kBlock, 0, s>>>(d_items_, d_w_, b, e, dlam, dpart + 3 * kGradBlocks * l);
+			HIPCHECK(hipGetLastError());
+		}
+	}
+	times.grad_ms = tm.stop();
+	std::vector<double> hp(3 * kGradBlocks * L);
+	if (L)
+		HIPCHECK(hipMemcpy(hp.data(), dpart, 8 * hp.size(), hipMemcpyDeviceToHost));
+	dfree(dlam);
+	dfree(dpart);
+	for (size_t l = 0; l < L; ++l)
+		for (int c = 0; c < 3; ++c) {
+			double t = 0;
+			for (int b = 0; b < kGradBlocks; ++b)
+				t += hp[3 * (kGradBlocks * l + b) + c];
+			dcolor3[3 * l + c] = t;
+		}
 }
 
 void DeviceScene::get_weights(float* out, size_t n) const

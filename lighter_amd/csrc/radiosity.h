@@ -1,8 +1,10 @@
 // Radiosity solver forms (radiosity.hip): many right-hand sides at once over
-// the link structure of a DeviceScene.
+// the link structure of a DeviceScene, forward and reverse mode.
 #pragma once
 
 #include <stdint.h>
+
+#include <vector>
 
 #include "device.h"
 
@@ -11,16 +13,16 @@ namespace lgx {
 enum RadForm {
 	kSparseF64 = 0, // gather over the links, FP64 (the accuracy reference on the GPU)
 	kSparseF32 = 1, // gather over the links, FP32
-	kDenseF32 = 2,  // dense G = F diag(area) in FP32, plain FP32 FMAs
-	kDenseBF16 = 3, // dense G in BF16, MFMA with FP32 accumulation
-	kDenseF16 = 4,  // dense G in FP16 (power-of-two scaled), MFMA with FP32 accumulation
+	kDenseF32 = 2,  // dense H = F in FP32, plain FP32 FMAs
+	kDenseBF16 = 3, // dense H in BF16, MFMA with FP32 accumulation
+	kDenseF16 = 4,  // dense H in FP16 (power-of-two scaled), MFMA with FP32 accumulation
 };
 
 struct RadTimes {
 	double build_ms = 0;      // dense matrix construction
 	double series_ms = 0;     // the bounces (all right-hand sides)
-	double per_bounce_ms = 0; // series_ms / bounces
-	double refine_ms = 0;     // refinement sweeps
+	double per_bounce_ms = 0; // series_ms / bounces (reverse sweep / bounces for solve_grad)
+	double refine_ms = 0;     // refinement sweeps (the reverse sweep for solve_grad)
 };
 
 class RadSolver {
@@ -31,17 +33,23 @@ public:
 	void set_problem(const double* E, uint32_t R, const float* diffuse3, const float* area);
 	void build_dense(int form);
 	// total (N x R row-major) = sum_{k=0..bounces} K^k E; dense forms may add
-	// `refine` refinement sweeps with the FP64 link residual.
+	// `refine` refinement sweeps with an FP64 residual.
 	void solve(int form, int bounces, int refine, double* total);
+	// Reverse mode: for the upstream gradient g = dL/dtotal (N x R), the
+	// total again, lam = dL/dE and dalb = dL/d diffuse per column (the
+	// albedo of column c's channel; sum the columns of a channel).
+	void solve_grad(int form, int bounces, const double* g, double* total, double* lam, double* dalb);
 	RadTimes times;
 	double build_ms = 0;
 	uint32_t N = 0, Np = 0, R = 0;
 
 private:
 	void release();
+	void ensure_work();
 	uint32_t ksplit(uint32_t rows_per_block, uint32_t min_blocks) const;
-	void apply_dense(int form, const float* xT, float* yT, float* part, void* xq, uint16_t* xqlo, bool split);
-	void column_scales(int form, const float* xT);
+	void set_scales(int form, int which, const std::vector<float>& colmax);
+	void apply_dense(int form, const float* xT, float* yT, bool split, bool pre, int which);
+	void apply_op(int form, const double* x, double* y, bool pre, int which);
 	DeviceScene& dev;
 	void* s_ = nullptr;
 	float* g32_ = nullptr;
@@ -53,8 +61,17 @@ private:
 	double* E_ = nullptr;
 	float* diff_ = nullptr;
 	float* area_ = nullptr;
-	float* cscale_ = nullptr;
-	float* cunscale_ = nullptr;
+	std::vector<float> emax_; // column maxima of the forward operand
+	// dense-product work buffers
+	float* part_ = nullptr;
+	float* xq_ = nullptr;
+	uint16_t* xqlo_ = nullptr;
+	float* xT_ = nullptr;
+	float* yT_ = nullptr;
+	float* rm32a_ = nullptr;
+	float* rm32b_ = nullptr;
+	float* cs_[2] = {nullptr, nullptr}; // column scales: forward operand, adjoint operand
+	float* cu_[2] = {nullptr, nullptr}; // their unscales (times the matrix scale)
 };
 
 } // namespace lgx

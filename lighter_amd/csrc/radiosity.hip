@@ -105,8 +105,9 @@ __device__ __forceinline__ float link_factor(const float* pos, const float* nrm,
 }
 
 // ---------------------------------------------------------------------------
-// dense matrix G[i][j] = f_ij * area_j from the occlusion bitmask; one thread
-// per 32-column word.  FP32 (P = -1) or 16-bit with a power-of-two scale.
+// dense matrix H[i][j] = f_ij (the symmetric link factors, zero off the
+// links) from the occlusion bitmask; one thread per 32-column word.  FP32
+// (P = -1) or 16-bit with a power-of-two scale, plus the remainder plane.
 // ---------------------------------------------------------------------------
 template <int P>
 __device__ __forceinline__ float from16(uint16_t v)
@@ -121,8 +122,8 @@ __device__ __forceinline__ float from16(uint16_t v)
 template <int P>
 __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N,
                                                     uint32_t Np, const float* __restrict__ pos,
-                                                    const float* __restrict__ nrm, const float* __restrict__ area,
-                                                    float scale, void* __restrict__ G, uint16_t* __restrict__ Glo)
+                                                    const float* __restrict__ nrm, float scale, void* __restrict__ G,
+                                                    uint16_t* __restrict__ Glo)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	uint64_t wpr = Np / 32; // words per padded row
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__
 #pragma unroll
 	for (int b = 0; b < 32; ++b) {
 		uint32_t j = w * 32 + b;
-		v[b] = ((m >> b) & 1u) ? link_factor(pos, nrm, i, j) * area[j] : 0.0f;
+		v[b] = ((m >> b) & 1u) ? link_factor(pos, nrm, i, j) : 0.0f;
 	}
 	if (P < 0) {
 		float4* o = reinterpret_cast<float4*>((float*)G + (uint64_t)i * Np + (uint64_t)w * 32);
@@ -165,34 +166,42 @@ __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__
 	}
 }
 
-// largest |G| entry (for the FP16 scale): max over links of f * area_j
-__global__ void k_max_link(uint32_t N, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
-                           const float* __restrict__ val, const float* __restrict__ area, unsigned* out)
+// largest link factor (for the FP16 scale); grid-stride over the stored
+// links, one atomic per wave
+__global__ __launch_bounds__(kB) void k_max_link(uint64_t nnz, const float* __restrict__ val, unsigned* out)
 {
-	uint32_t i = blockIdx.x * kB + threadIdx.x;
-	if (i >= N)
-		return;
 	float m = 0.0f;
-	for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
-		m = fmaxf(m, val[k] * area[col[k]]);
-	atomicMax(out, __float_as_uint(m)); // non-negative floats order like their bits
+	for (uint64_t k = (uint64_t)blockIdx.x * kB + threadIdx.x; k < nnz; k += (uint64_t)gridDim.x * kB)
+		m = fmaxf(m, val[k]);
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		m = fmaxf(m, __shfl_xor(m, o));
+	if ((threadIdx.x & 63) == 0)
+		atomicMax(out, __float_as_uint(m)); // non-negative floats order like their bits
 }
 
 // ---------------------------------------------------------------------------
-// right-hand-side preparation: Xq[c][j] = x[c][j] * diffuse[j][c % 3] * s_c
+// right-hand-side preparation: Xq[c][j] = (x[c][j] * diffuse[j][c % 3]) *
+// area_j * s_c for K = H diag(area) diag(diffuse_c) (pre), or x[c][j] * s_c
+// for the bare H (the adjoint sweep); s_c is a power-of-two column scale
 // ---------------------------------------------------------------------------
 template <int P>
 __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, const float* __restrict__ diffuse,
-                                                 const float* __restrict__ cscale, uint32_t N, uint32_t Np,
-                                                 uint32_t R, uint16_t* __restrict__ xq, uint16_t* __restrict__ xlo)
+                                                 const float* __restrict__ area, const float* __restrict__ cscale,
+                                                 uint32_t N, uint32_t Np, uint32_t R, int pre,
+                                                 uint16_t* __restrict__ xq, uint16_t* __restrict__ xlo)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t >= (uint64_t)kRPad * Np)
 		return;
 	uint32_t c = (uint32_t)(t / Np), j = (uint32_t)(t % Np);
 	float v = 0.0f;
-	if (c < R && j < N)
-		v = xT[(uint64_t)c * Np + j] * diffuse[3 * j + c % 3] * cscale[c];
+	if (c < R && j < N) {
+		v = xT[(uint64_t)c * Np + j];
+		if (pre)
+			v = (v * diffuse[3 * j + c % 3]) * area[j];
+		v *= cscale[c];
+	}
 	uint16_t h = Prec<P>::cvt(v);
 	xq[t] = h;
 	if (xlo)
@@ -200,13 +209,20 @@ __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, c
 }
 
 __global__ __launch_bounds__(kB) void k_prep_x32(const float* __restrict__ xT, const float* __restrict__ diffuse,
-                                                 uint32_t N, uint32_t Np, uint32_t R, float* __restrict__ xq)
+                                                 const float* __restrict__ area, uint32_t N, uint32_t Np, uint32_t R,
+                                                 int pre, float* __restrict__ xq)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t >= (uint64_t)kRPad * Np)
 		return;
 	uint32_t c = (uint32_t)(t / Np), j = (uint32_t)(t % Np);
-	xq[t] = (c < R && j < N) ? xT[(uint64_t)c * Np + j] * diffuse[3 * j + c % 3] : 0.0f;
+	float v = 0.0f;
+	if (c < R && j < N) {
+		v = xT[(uint64_t)c * Np + j];
+		if (pre)
+			v = (v * diffuse[3 * j + c % 3]) * area[j];
+	}
+	xq[t] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -528,8 +544,9 @@ __global__ __launch_bounds__(kB) void k_reduce_split(const float* __restrict__ p
 // ---------------------------------------------------------------------------
 // sparse gather over the symmetric CSR links, one wave per row, lanes over
 // columns: y[i][c] = sum_k ((x[j][c] * diffuse[j][c%3]) * area_j) * f_k
+// (PRE, the reference's operation order) or sum_k x[j][c] * f_k
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, bool PRE>
 __global__ __launch_bounds__(kB) void k_spmm(uint32_t N, uint32_t R, const uint32_t* __restrict__ rowptr,
                                              const uint32_t* __restrict__ col, const float* __restrict__ val,
                                              const float* __restrict__ diffuse, const float* __restrict__ area,
@@ -545,7 +562,10 @@ __global__ __launch_bounds__(kB) void k_spmm(uint32_t N, uint32_t R, const uint3
 		if (c < R) {
 			for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
 				uint32_t j = col[k];
-				acc += ((x[(uint64_t)j * R + c] * (T)diffuse[3 * j + c % 3]) * (T)area[j]) * (T)val[k];
+				if (PRE)
+					acc += ((x[(uint64_t)j * R + c] * (T)diffuse[3 * j + c % 3]) * (T)area[j]) * (T)val[k];
+				else
+					acc += x[(uint64_t)j * R + c] * (T)val[k];
 			}
 			y[(uint64_t)i * R + c] = acc;
 		}
@@ -585,6 +605,23 @@ template <class A, class B> __global__ void k_axpy(uint64_t n, const A* __restri
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t < n)
 		y[t] += (B)x[t];
+}
+
+// one step of the adjoint sweep (row-major N x R, FP64), with u = H ybar_k:
+// dalb += area * u * S_{k-1}, ybar_{k-1} = diffuse_c * area * u, lam += ybar_{k-1}
+__global__ void k_adj_step(uint32_t N, uint32_t R, const double* __restrict__ u, const double* __restrict__ sprev,
+                           const float* __restrict__ diffuse, const float* __restrict__ area,
+                           double* __restrict__ dalb, double* __restrict__ ybar, double* __restrict__ lam)
+{
+	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
+	if (t >= (uint64_t)N * R)
+		return;
+	uint32_t i = (uint32_t)(t / R), c = (uint32_t)(t % R);
+	double au = (double)area[i] * u[t];
+	dalb[t] += au * sprev[t];
+	double yb = (double)diffuse[3 * i + c % 3] * au;
+	ybar[t] = yb;
+	lam[t] += yb;
 }
 
 // r = a - b, formed in FP64
@@ -648,14 +685,28 @@ void RadSolver::release()
 	dfree(E_);
 	dfree(diff_);
 	dfree(area_);
+	dfree(part_);
+	dfree(xq_);
+	dfree(xqlo_);
+	dfree(xT_);
+	dfree(yT_);
+	dfree(rm32a_);
+	dfree(rm32b_);
+	for (int k = 0; k < 2; ++k) {
+		dfree(cs_[k]);
+		dfree(cu_[k]);
+	}
 }
 
 void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3, const float* area)
 {
 	if (R_ == 0 || R_ > (uint32_t)kRPad)
 		throw std::runtime_error("RadSolver: 1..192 right-hand sides");
+	if (R_ != R) {
+		dfree(rm32a_);
+		dfree(rm32b_);
+	}
 	R = R_;
-	built_form_ = -1; // G carries the areas: rebuild for a new problem
 	hipStream_t s = (hipStream_t)s_;
 	dfree(E_);
 	dfree(diff_);
@@ -667,6 +718,11 @@ void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3,
 	HIPCHECK(hipMemcpyAsync(diff_, diffuse3, 12 * (size_t)N, hipMemcpyHostToDevice, s));
 	HIPCHECK(hipMemsetAsync(area_, 0, 4 * (size_t)Np, s));
 	HIPCHECK(hipMemcpyAsync(area_, area, 4 * (size_t)N, hipMemcpyHostToDevice, s));
+	// column maxima of the forward operand (x * diffuse) * area, for the FP16 scales
+	emax_.assign(kRPad, 0.0f);
+	for (uint32_t j = 0; j < N; ++j)
+		for (uint32_t c = 0; c < R; ++c)
+			emax_[c] = fmaxf(emax_[c], fabsf((float)E[(size_t)j * R + c] * diffuse3[3 * j + c % 3] * area[j]));
 	HIPCHECK(hipStreamSynchronize(s));
 }
 
@@ -679,7 +735,7 @@ void RadSolver::build_dense(int form)
 		if (!g32_)
 			g32_ = dalloc<float>((uint64_t)Np * Np);
 		k_dense_build<-1><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-		                                                dev.rad.nrm, area_, 1.0f, g32_, nullptr);
+		                                                dev.rad.nrm, 1.0f, g32_, nullptr);
 	} else {
 		if (!g16_)
 			g16_ = dalloc<uint16_t>((uint64_t)Np * Np);
@@ -687,11 +743,11 @@ void RadSolver::build_dense(int form)
 			g16lo_ = dalloc<uint16_t>((uint64_t)Np * Np);
 		gscale_ = 1.0f;
 		if (form == kDenseF16) {
-			// scale so that the largest entry sits near 2^14: small factors
+			// scale so that the largest factor sits near 2^13: small factors
 			// stay in FP16's normal range
 			unsigned* dm = dalloc<unsigned>(1);
 			HIPCHECK(hipMemsetAsync(dm, 0, 4, s));
-			k_max_link<<<gridn(N), kB, 0, s>>>(N, dev.rad.rowptr, dev.rad.col, dev.rad.val, area_, dm);
+			k_max_link<<<2048, kB, 0, s>>>(dev.rad.nnz, dev.rad.val, dm);
 			unsigned hm = 0;
 			HIPCHECK(hipMemcpyAsync(&hm, dm, 4, hipMemcpyDeviceToHost, s));
 			HIPCHECK(hipStreamSynchronize(s));
@@ -700,10 +756,10 @@ void RadSolver::build_dense(int form)
 			memcpy(&mx, &hm, 4);
 			gscale_ = mx > 0 ? ldexpf(1.0f, 14 - ilogbf(mx) - 1) : 1.0f;
 			k_dense_build<1><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-			                                               dev.rad.nrm, area_, gscale_, g16_, g16lo_);
+			                                               dev.rad.nrm, gscale_, g16_, g16lo_);
 		} else {
 			k_dense_build<0><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-			                                               dev.rad.nrm, area_, 1.0f, g16_, g16lo_);
+			                                               dev.rad.nrm, 1.0f, g16_, g16lo_);
 		}
 	}
 	HIPCHECK(hipGetLastError());
@@ -720,66 +776,108 @@ uint32_t RadSolver::ksplit(uint32_t rows_per_block, uint32_t min_blocks) const
 	return s;
 }
 
-// y (column-major, R x Np, FP32) = K x for the dense forms.  split: the
-// error-compensated product of the 16-bit forms (hi + lo planes); xqlo is
-// the right-hand side's lo plane.
-void RadSolver::apply_dense(int form, const float* xT, float* yT, float* part, void* xq, uint16_t* xqlo,
-                            bool split)
+// work buffers of the dense products (kept across solves)
+void RadSolver::ensure_work()
+{
+	const uint64_t n = (uint64_t)kRPad * Np;
+	if (!part_) {
+		const uint32_t kmax =
+		    std::max(ksplit(kTileM, 2048), std::max(ksplit(64 * kPlainRT, ncu_), ksplit(64 * kSplitRT, ncu_)));
+		part_ = dalloc<float>((uint64_t)kmax * n);
+		xq_ = dalloc<float>(n); // FP32 operand, or the 16-bit one in its first half
+		xqlo_ = dalloc<uint16_t>(n);
+		xT_ = dalloc<float>(n);
+		yT_ = dalloc<float>(n);
+		for (int k = 0; k < 2; ++k) {
+			cs_[k] = dalloc<float>(kRPad);
+			cu_[k] = dalloc<float>(kRPad);
+		}
+	}
+	if (!rm32a_) {
+		rm32a_ = dalloc<float>((uint64_t)Np * R);
+		rm32b_ = dalloc<float>((uint64_t)Np * R);
+	}
+}
+
+// power-of-two column scales of the 16-bit operand (FP16 needs the column
+// maxima of the operand; BF16 has FP32's range) and the matching unscale
+void RadSolver::set_scales(int form, int which, const std::vector<float>& colmax)
+{
+	std::vector<float> sc(kRPad, 1.0f), us(kRPad, 1.0f);
+	for (uint32_t c = 0; c < kRPad; ++c) {
+		float e = 1.0f;
+		if (form == kDenseF16 && c < colmax.size() && colmax[c] > 0)
+			e = ldexpf(1.0f, 14 - ilogbf(colmax[c]) - 1);
+		sc[c] = e;
+		us[c] = 1.0f / (e * gscale_);
+	}
+	HIPCHECK(hipMemcpy(cs_[which], sc.data(), 4 * kRPad, hipMemcpyHostToDevice));
+	HIPCHECK(hipMemcpy(cu_[which], us.data(), 4 * kRPad, hipMemcpyHostToDevice));
+}
+
+// y (column-major, R x Np, FP32) = K x (pre) or H x for the dense forms.
+// split: the error-compensated product of the 16-bit forms (hi + lo planes).
+// which: the column-scale set (0: forward operand, 1: adjoint operand).
+void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bool pre, int which)
 {
 	hipStream_t s = (hipStream_t)s_;
 	uint64_t nq = (uint64_t)kRPad * Np;
 	if (form == kDenseF32) {
 		uint32_t ks = ksplit(kTileM, 2048);
 		dim3 grid(Np / kTileM, ks);
-		k_prep_x32<<<gridn(nq), kB, 0, s>>>(xT, diff_, N, Np, R, (float*)xq);
-		k_gemm_f32<<<grid, kB, 0, s>>>(g32_, (const float*)xq, Np, ks, part);
-		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part, ks, Np, R, nullptr, yT);
+		k_prep_x32<<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, N, Np, R, pre ? 1 : 0, xq_);
+		k_gemm_f32<<<grid, kB, 0, s>>>(g32_, xq_, Np, ks, part_);
+		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, Np, R, nullptr, yT);
 	} else {
-		// per-column power-of-two scale keeps FP16 operands in range
-		uint16_t* lo = split ? xqlo : nullptr;
-		const uint16_t* q = (const uint16_t*)xq;
+		uint16_t* q = reinterpret_cast<uint16_t*>(xq_);
+		uint16_t* lo = split ? xqlo_ : nullptr;
 		if (form == kDenseF16)
-			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, cscale_, N, Np, R, (uint16_t*)xq, lo);
+			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cs_[which], N, Np, R, pre ? 1 : 0, q, lo);
 		else
-			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, cscale_, N, Np, R, (uint16_t*)xq, lo);
+			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cs_[which], N, Np, R, pre ? 1 : 0, q, lo);
 		const uint32_t rows = 64 * (split ? kSplitRT : kPlainRT);
 		const uint32_t ks = ksplit(rows, ncu_);
 		dim3 grid(Np / rows, ks);
 		if (form == kDenseF16 && split)
-			k_gemm_mfma<1, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part);
+			k_gemm_mfma<1, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part_);
 		else if (form == kDenseF16)
-			k_gemm_mfma<1, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part);
+			k_gemm_mfma<1, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part_);
 		else if (split)
-			k_gemm_mfma<0, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part);
+			k_gemm_mfma<0, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part_);
 		else
-			k_gemm_mfma<0, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part);
-		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part, ks, Np, R, cunscale_, yT);
+			k_gemm_mfma<0, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part_);
+		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, Np, R, cu_[which], yT);
 	}
 	HIPCHECK(hipGetLastError());
 }
 
-// per-column scales for the 16-bit operand: FP16 needs the column maxima
-void RadSolver::column_scales(int form, const float* xT)
+// y = K x (pre) or H x, row-major N x R FP64 in and out, in the given form
+void RadSolver::apply_op(int form, const double* x, double* y, bool pre, int which)
 {
-	std::vector<float> hx((size_t)R * Np), sc(kRPad, 1.0f), us(kRPad, 1.0f);
-	if (form == kDenseF16) {
-		HIPCHECK(hipMemcpy(hx.data(), xT, 4 * (size_t)R * Np, hipMemcpyDeviceToHost));
-		std::vector<float> hd(3 * (size_t)N);
-		HIPCHECK(hipMemcpy(hd.data(), diff_, 12 * (size_t)N, hipMemcpyDeviceToHost));
-		for (uint32_t c = 0; c < R; ++c) {
-			float m = 0;
-			for (uint32_t j = 0; j < N; ++j)
-				m = fmaxf(m, fabsf(hx[(size_t)c * Np + j] * hd[3 * j + c % 3]));
-			float e = m > 0 ? ldexpf(1.0f, 14 - ilogbf(m) - 1) : 1.0f;
-			sc[c] = e;
-			us[c] = 1.0f / (e * gscale_);
-		}
+	hipStream_t s = (hipStream_t)s_;
+	const uint64_t nR = (uint64_t)N * R, nRp = (uint64_t)R * Np;
+	const unsigned sgrid = gridn(N, kB / 64);
+	const uint32_t* rp = dev.rad.rowptr;
+	const uint32_t* cl = dev.rad.col;
+	const float* vl = dev.rad.val;
+	if (form == kSparseF64) {
+		if (pre)
+			k_spmm<double, true><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, x, y);
+		else
+			k_spmm<double, false><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, x, y);
+	} else if (form == kSparseF32) {
+		k_copy_cast<double, float><<<gridn(nR), kB, 0, s>>>(nR, x, rm32a_);
+		if (pre)
+			k_spmm<float, true><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, rm32a_, rm32b_);
+		else
+			k_spmm<float, false><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, rm32a_, rm32b_);
+		k_copy_cast<float, double><<<gridn(nR), kB, 0, s>>>(nR, rm32b_, y);
 	} else {
-		for (uint32_t c = 0; c < R; ++c)
-			us[c] = 1.0f / gscale_;
+		k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(x, N, Np, R, xT_);
+		apply_dense(form, xT_, yT_, false, pre, which);
+		k_to_rowmajor<float, double><<<gridn(nR), kB, 0, s>>>(yT_, N, Np, R, y);
 	}
-	HIPCHECK(hipMemcpy(cscale_, sc.data(), 4 * kRPad, hipMemcpyHostToDevice));
-	HIPCHECK(hipMemcpy(cunscale_, us.data(), 4 * kRPad, hipMemcpyHostToDevice));
+	HIPCHECK(hipGetLastError());
 }
 
 void RadSolver::solve(int form, int bounces, int refine, double* total_out)
@@ -787,63 +885,37 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 	hipStream_t s = (hipStream_t)s_;
 	uint64_t nR = (uint64_t)N * R, nRp = (uint64_t)R * Np;
 	times = RadTimes{};
-	// FP64 running total and the exact sparse operator's workspace
+	ensure_work();
+	// FP64 running total
 	double* tot = dalloc<double>(nR);
 	HIPCHECK(hipMemcpyAsync(tot, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
-	const uint32_t* rp = dev.rad.rowptr;
-	const uint32_t* cl = dev.rad.col;
-	const float* vl = dev.rad.val;
-	unsigned sgrid = gridn(N, kB / 64);
 	if (form == kSparseF64 || form == kSparseF32) {
+		double *x = dalloc<double>(nR), *y = dalloc<double>(nR);
+		HIPCHECK(hipMemcpyAsync(x, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
 		Ev ev(s);
-		if (form == kSparseF64) {
-			double *x = dalloc<double>(nR), *y = dalloc<double>(nR);
-			HIPCHECK(hipMemcpyAsync(x, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
-			for (int b = 0; b < bounces; ++b) {
-				k_spmm<double><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, x, y);
-				k_axpy<double, double><<<gridn(nR), kB, 0, s>>>(nR, y, tot);
-				std::swap(x, y);
-			}
-			dfree(x);
-			dfree(y);
-		} else {
-			float *x = dalloc<float>(nR), *y = dalloc<float>(nR);
-			k_copy_cast<double, float><<<gridn(nR), kB, 0, s>>>(nR, E_, x);
-			Ev ev2(s);
-			for (int b = 0; b < bounces; ++b) {
-				k_spmm<float><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, x, y);
-				k_axpy<float, double><<<gridn(nR), kB, 0, s>>>(nR, y, tot);
-				std::swap(x, y);
-			}
-			times.series_ms = ev2.stop();
-			dfree(x);
-			dfree(y);
+		for (int b = 0; b < bounces; ++b) {
+			apply_op(form, x, y, true, 0);
+			k_axpy<double, double><<<gridn(nR), kB, 0, s>>>(nR, y, tot);
+			std::swap(x, y);
 		}
-		double ms = ev.stop();
-		if (form == kSparseF64)
-			times.series_ms = ms;
+		times.series_ms = ev.stop();
 		times.per_bounce_ms = times.series_ms / std::max(1, bounces);
+		dfree(x);
+		dfree(y);
 	} else {
 		if (built_form_ != form)
 			build_dense(form);
 		times.build_ms = build_ms;
+		set_scales(form, 0, emax_);
 		// z_k (column-major FP32), k = 0..b
 		std::vector<float*> z(bounces + 1);
 		for (auto& p : z)
 			p = dalloc<float>(nRp);
-		const uint32_t kmax = std::max(ksplit(kTileM, 2048), std::max(ksplit(64 * kPlainRT, ncu_), ksplit(64 * kSplitRT, ncu_)));
-		float* part = dalloc<float>((uint64_t)kmax * kRPad * Np);
-		const bool half = form != kDenseF32;
-		void* xq = half ? (void*)dalloc<uint16_t>((uint64_t)kRPad * Np) : (void*)dalloc<float>((uint64_t)kRPad * Np);
-		uint16_t* xqlo = half ? dalloc<uint16_t>((uint64_t)kRPad * Np) : nullptr;
-		cscale_ = dalloc<float>(kRPad);
-		cunscale_ = dalloc<float>(kRPad);
 		k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(E_, N, Np, R, z[0]);
-		column_scales(form, z[0]);
 		{
 			Ev ev(s);
 			for (int b = 1; b <= bounces; ++b)
-				apply_dense(form, z[b - 1], z[b], part, xq, nullptr, false);
+				apply_dense(form, z[b - 1], z[b], false, true, 0);
 			times.series_ms = ev.stop();
 		}
 		times.per_bounce_ms = times.series_ms / std::max(1, bounces);
@@ -853,19 +925,20 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		// the difference taken in FP64; corrections d_b = Kdense d_{b-1} + rho_b
 		// (d_0 = 0: z_0 = E is exact) with the plain 16-bit product; then
 		// z += d.  z_{b-1} is corrected only after rho_b has used its old value.
+		const bool half = form != kDenseF32;
 		double* zr = dalloc<double>(nR);
-		double* rho = dalloc<double>(nRp);
-		float* kz = dalloc<float>(nRp);
-		float* d = dalloc<float>(nRp);
-		float* dprev = dalloc<float>(nRp);
+		double* rho = refine > 0 ? dalloc<double>(nRp) : nullptr;
+		float* kz = refine > 0 ? dalloc<float>(nRp) : nullptr;
+		float* d = refine > 0 ? dalloc<float>(nRp) : nullptr;
+		float* dprev = refine > 0 ? dalloc<float>(nRp) : nullptr;
 		{
 			Ev ev(s);
 			for (int it = 0; it < refine; ++it) {
 				HIPCHECK(hipMemsetAsync(dprev, 0, 4 * nRp, s));
 				for (int b = 1; b <= bounces; ++b) {
-					apply_dense(form, z[b - 1], kz, part, xq, xqlo, half);               // K z_{b-1}
+					apply_dense(form, z[b - 1], kz, half, true, 0);                       // K z_{b-1}
 					k_resid64<<<gridn(nRp), kB, 0, s>>>(nRp, kz, z[b], rho);             // rho_b (FP64)
-					apply_dense(form, dprev, d, part, xq, nullptr, false);               // Kdense d_{b-1}
+					apply_dense(form, dprev, d, false, true, 0);                          // Kdense d_{b-1}
 					k_axpy<double, float><<<gridn(nRp), kB, 0, s>>>(nRp, rho, d);        // d_b
 					k_axpy<float, float><<<gridn(nRp), kB, 0, s>>>(nRp, dprev, z[b - 1]); // apply d_{b-1}
 					std::swap(d, dprev);
@@ -874,7 +947,6 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 			}
 			times.refine_ms = ev.stop();
 		}
-		dfree(xqlo);
 		HIPCHECK(hipGetLastError());
 		// total = sum_k z_k in FP64 (z_0 = E exactly)
 		for (int b = 1; b <= bounces; ++b) {
@@ -883,13 +955,6 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		}
 		for (auto& p : z)
 			dfree(p);
-		dfree(part);
-		if (form == kDenseF32)
-			dfree(*(float**)&xq);
-		else
-			dfree(*(uint16_t**)&xq);
-		dfree(cscale_);
-		dfree(cunscale_);
 		dfree(zr);
 		dfree(kz);
 		dfree(rho);
@@ -900,6 +965,81 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 	HIPCHECK(hipMemcpyAsync(total_out, tot, 8 * nR, hipMemcpyDeviceToHost, s));
 	HIPCHECK(hipStreamSynchronize(s));
 	dfree(tot);
+}
+
+// Reverse mode of T = sum_{k=0..B} K^k E, K = H diag(area) diag(diffuse_c).
+// Forward partial sums S_k = sum_{m<=k} K^m E are kept (T = S_B is the Horner
+// recurrence S_k = E + K S_{k-1}); its adjoint, from ybar_B = g:
+//   u_k = H ybar_k            (H symmetric, so K^T = diag(diffuse_c) diag(area) H)
+//   dalb += area * u_k * S_{k-1}
+//   ybar_{k-1} = diffuse_c * area * u_k
+//   lam = sum_k ybar_k        (= dL/dE; also dL/d emission)
+void RadSolver::solve_grad(int form, int bounces, const double* g, double* total_out, double* lam_out,
+                           double* dalb_out)
+{
+	hipStream_t s = (hipStream_t)s_;
+	const uint64_t nR = (uint64_t)N * R;
+	times = RadTimes{};
+	ensure_work();
+	if (form >= kDenseF32 && built_form_ != form)
+		build_dense(form);
+	if (form >= kDenseF32) {
+		set_scales(form, 0, emax_);
+		std::vector<float> gmax(kRPad, 0.0f);
+		for (uint32_t j = 0; j < N; ++j)
+			for (uint32_t c = 0; c < R; ++c)
+				gmax[c] = fmaxf(gmax[c], fabsf((float)g[(size_t)j * R + c]));
+		set_scales(form, 1, gmax);
+	}
+	const int B = std::max(0, bounces);
+	std::vector<double*> S(std::max(1, B));
+	for (auto& p : S)
+		p = dalloc<double>(nR);
+	double* z = dalloc<double>(nR);
+	double* zn = dalloc<double>(nR);
+	double* tot = dalloc<double>(nR);
+	double* ybar = dalloc<double>(nR);
+	double* u = dalloc<double>(nR);
+	double* lam = dalloc<double>(nR);
+	double* dalb = dalloc<double>(nR);
+	Ev ev(s);
+	// forward: S_0 = E, z_k = K z_{k-1}, S_k = S_{k-1} + z_k
+	HIPCHECK(hipMemcpyAsync(S[0], E_, 8 * nR, hipMemcpyDeviceToDevice, s));
+	HIPCHECK(hipMemcpyAsync(z, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
+	HIPCHECK(hipMemcpyAsync(tot, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
+	for (int k = 1; k <= B; ++k) {
+		apply_op(form, z, zn, true, 0);
+		k_axpy<double, double><<<gridn(nR), kB, 0, s>>>(nR, zn, tot);
+		if (k < B)
+			HIPCHECK(hipMemcpyAsync(S[k], tot, 8 * nR, hipMemcpyDeviceToDevice, s));
+		std::swap(z, zn);
+	}
+	times.series_ms = ev.stop();
+	// reverse sweep
+	Ev ev2(s);
+	HIPCHECK(hipMemcpyAsync(ybar, g, 8 * nR, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(lam, ybar, 8 * nR, hipMemcpyDeviceToDevice, s));
+	HIPCHECK(hipMemsetAsync(dalb, 0, 8 * nR, s));
+	for (int k = B; k >= 1; --k) {
+		apply_op(form, ybar, u, false, 1);
+		k_adj_step<<<gridn(nR), kB, 0, s>>>(N, R, u, S[k - 1], diff_, area_, dalb, ybar, lam);
+	}
+	times.refine_ms = ev2.stop(); // the reverse sweep's time
+	times.per_bounce_ms = times.refine_ms / std::max(1, B);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(total_out, tot, 8 * nR, hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipMemcpyAsync(lam_out, lam, 8 * nR, hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipMemcpyAsync(dalb_out, dalb, 8 * nR, hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	for (auto& p : S)
+		dfree(p);
+	dfree(z);
+	dfree(zn);
+	dfree(tot);
+	dfree(ybar);
+	dfree(u);
+	dfree(lam);
+	dfree(dalb);
 }
 
 } // namespace lgx

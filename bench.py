@@ -174,21 +174,21 @@ def cpu_baseline(spec, pos, nrm, items, w, lights, target_s, nthreads):
 FORMS = {0: "sparse_f64", 1: "sparse_f32", 2: "dense_f32", 3: "dense_bf16_mfma", 4: "dense_f16_mfma"}
 
 
-def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
-    """C2: radiosity over the room's texels as patches, R right-hand sides
-    (setups x RGB) built from the lights' direct contributions."""
-    from oracle.checkers import normalize_f32
-    lm = 200
+def room_patches(lib, nth, n_patches, lm0, **room_kw):
+    """The room's texels as radiosity patches: grow the atlas until the room
+    instance has n_patches samples, keep the first n_patches."""
+    lm = lm0
     while True:
-        spec = scenes.room_scene(lm=lm, bounce_count=bounces, ao_distance=2.0, ao_samples=64, name="C2")
+        spec = scenes.room_scene(lm=lm, **room_kw)
         b = scenes.BuiltScene(lib, spec, dict(max_num_threads=nth))
         _lib.check(lib.lgx_Prepare(b.scene), lib)
         info = (C.c_int32 * 4)()
         lib.lgx_InstanceInfo(b.scene, 1, info)
-        if info[2] >= n_patches or lm >= 512:
+        if info[2] >= n_patches or lm >= 2048:
             break
         b.destroy()
-        lm += 8
+        # covered texels grow with the atlas area
+        lm = max(lm + 8, int(np.ceil(lm * np.sqrt(n_patches / max(info[2], 1)) * 1.01 / 8)) * 8)
     n_all, nt = info[2], info[3]
     pos = np.zeros((n_all, 3), np.float32)
     nrm = np.zeros((n_all, 3), np.float32)
@@ -197,7 +197,15 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
     tris = np.zeros((nt, 9), np.float32)
     lib.lgx_GetTriangles(b.scene, 1, P(tris))
     N = min(n_patches, n_all)
-    pos, nrm, rad = pos[:N].copy(), nrm[:N].copy(), rad[:N].copy()
+    return spec, b, pos[:N].copy(), nrm[:N].copy(), rad[:N].copy(), tris, N, lm
+
+
+def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
+    """C2: radiosity over the room's texels as patches, R right-hand sides
+    (setups x RGB) built from the lights' direct contributions."""
+    from oracle.checkers import normalize_f32
+    spec, b, pos, nrm, rad, tris, N, _ = room_patches(lib, nth, n_patches, 200, bounce_count=bounces,
+                                                      ao_distance=2.0, ao_samples=64, name="C2")
     dev = lib.lgx_DeviceCreate(P(tris), len(tris))
     # direct light of each light on the patches
     _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), N, None), lib)
@@ -271,7 +279,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
         log(f"[bench] {key}: {per_bounce:.3f} ms/bounce, err {res[key]['rel_l2_err_vs_fp64']:.2e}")
     lib.lgx_DeviceDestroy(dev)
     b.destroy()
-    Np = ((N + 127) // 128) * 128
+    Np = ((N + 255) // 256) * 256
     mf = res["dense_bf16_mfma"]["s_per_bounce"]
     gemm_bytes = Np * Np * 2 + 2 * 192 * Np * 2
     out = dict(workload=f"C2 room radiosity: {N} patches (texels of the 10x10x4 m room), {R} right-hand sides "
@@ -291,6 +299,140 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
     return out
 
 
+def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
+    """C3: gradient of an L2 bake loss with respect to 64 light colours (RGB)
+    and 32 albedo groups (RGB), 288 parameters, through 4 bounces.
+
+    The room (about 20k triangles, 64 point lights) with its texels as
+    patches; the target is baked from parameters perturbed by +-30 %.  One
+    gradient evaluation = forward bake (E = sum_l colour_l w_l from the
+    checkpointed direct-light weights, then the bounces) + reverse sweep
+    (lgx_RadSolveGrad, which recomputes the forward partial sums) + colour
+    gradient (lgx_DirectLightGrad).  The dense FP16 MFMA form is checked
+    against the sparse FP64 form's gradient.  Second part: the colour
+    gradient over the checkpointed visibility of a ckpt_lm^2 atlas (every
+    texel x 64 lights)."""
+    from oracle.checkers import normalize_f32
+    kw = dict(seed=scenes.SEED_BASE + 3, n_boxes=31, wall_tess=40, n_point=64, with_sun=False, bounce_count=bounces,
+              name="C3")
+    spec, b, pos, nrm, rad, tris, N, lm = room_patches(lib, nth, n_patches, 256, **kw)
+    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), N, None), lib)
+    lights = make_lights(spec)
+    L = len(lights)
+    items = np.concatenate([(np.uint32(l) << np.uint32(26)) | np.arange(N, dtype=np.uint32) for l in range(L)])
+    off = (np.arange(L + 1) * N).astype(np.uint32)
+    _lib.check(lib.lgx_DirectLight(dev, lights, L, items.ctypes.data_as(u32p), len(items),
+                                   off.ctypes.data_as(u32p), 0), lib)
+    tm = (C.c_double * 6)()
+    lib.lgx_DeviceTimes(dev, tm)
+    direct_ms = tm[0]
+    w = np.zeros(len(items), np.float32)
+    lib.lgx_ReadWeights(dev, P(w), None, len(items))
+    w[w.view(np.uint32) == 0xFFFFFFFF] = 0
+    W = w.reshape(L, N).T.astype(np.float64)  # N x L
+    pn = np.array([normalize_f32(v) for v in nrm], np.float32)
+    nl = C.c_size_t(0)
+    _lib.check(lib.lgx_RadLinks(dev, P(pos), P(pn), N, C.byref(nl)), lib)
+    lib.lgx_DeviceTimes(dev, tm)
+    link_ms = tm[3]
+    area = np.ascontiguousarray(rad[:, 3])
+    groups = rad[:, 2].astype(np.int64) % 32
+    col0 = np.array([[lights[l].color[c] for c in range(3)] for l in range(L)], np.float64)
+    alb0 = np.array([spec.albedo.get(g, (0.8, 0.8, 0.8)) for g in range(32)], np.float64)
+    rng = np.random.default_rng(0x4C540003)
+    col_t = col0 * rng.uniform(0.7, 1.3, col0.shape)
+    alb_t = np.clip(alb0 * rng.uniform(0.7, 1.3, alb0.shape), 0.0, 0.95)
+    dp = C.POINTER(C.c_double)
+
+    def forward(form, col, alb):
+        E = np.ascontiguousarray(W @ col)
+        diff = np.ascontiguousarray(alb[groups], np.float32)
+        T = np.zeros_like(E)
+        t = (C.c_double * 4)()
+        _lib.check(lib.lgx_RadSolve(dev, E.ctypes.data_as(dp), 3, P(diff), P(area), form, bounces, 0,
+                                    T.ctypes.data_as(dp), t), lib)
+        return E, diff, T, list(t)
+
+    _, _, target, _ = forward(4, col_t, alb_t)
+
+    def gradient(form):
+        t0 = time.perf_counter()
+        E, diff, T, tf = forward(form, col0, alb0)
+        g = np.ascontiguousarray(T - target)
+        tot, lam, dalb = np.zeros_like(E), np.zeros_like(E), np.zeros_like(E)
+        t = (C.c_double * 4)()
+        _lib.check(lib.lgx_RadSolveGrad(dev, E.ctypes.data_as(dp), 3, P(diff), P(area), form, bounces,
+                                        g.ctypes.data_as(dp), tot.ctypes.data_as(dp), lam.ctypes.data_as(dp),
+                                        dalb.ctypes.data_as(dp), t), lib)
+        dcol = np.zeros((L, 3))
+        _lib.check(lib.lgx_DirectLightGrad(dev, np.ascontiguousarray(lam).ctypes.data_as(dp),
+                                           dcol.ctypes.data_as(dp)), lib)
+        lib.lgx_DeviceTimes(dev, tm)
+        dgrp = np.zeros((32, 3))
+        np.add.at(dgrp, groups, dalb)
+        wall = time.perf_counter() - t0
+        loss = 0.5 * float((g * g).sum())
+        return dict(loss=loss, dcol=dcol, dalb=dgrp, wall_s=wall, fwd_series_ms=tf[1], rev_fwd_ms=t[1],
+                    rev_sweep_ms=t[3], color_grad_ms=tm[5], build_ms=tf[0])
+
+    g4 = gradient(4)   # builds the FP16 matrix
+    g4 = gradient(4)   # timed with the matrix resident
+    g0 = gradient(0)
+    rel = lambda a, b_: float(np.linalg.norm(a - b_) / max(np.linalg.norm(b_), 1e-300))
+    gpu_ms = g4["fwd_series_ms"] + g4["rev_fwd_ms"] + g4["rev_sweep_ms"] + g4["color_grad_ms"]
+    log(f"[bench] C3: {N} patches, {len(tris)} tris, {L} lights, {nl.value} links; gradient (FP16 MFMA) "
+        f"{gpu_ms:.2f} ms GPU / {g4['wall_s'] * 1e3:.1f} ms wall; vs FP64: colours {rel(g4['dcol'], g0['dcol']):.2e}, "
+        f"albedo {rel(g4['dalb'], g0['dalb']):.2e}")
+    lib.lgx_DeviceDestroy(dev)
+    b.destroy()
+    out = dict(workload=f"C3 room gradient: {len(tris)} tris, {N} patches (texels, {lm}^2 atlas), {L} point "
+                        f"lights, 288 parameters (64 light RGB + 32 albedo groups RGB), L2 loss against a bake "
+                        f"from +-30 % perturbed parameters, {bounces} bounces",
+               patches=N, triangles=len(tris), links=int(nl.value), direct_light_ms=round(direct_ms, 3),
+               link_generation_ms=round(link_ms, 3),
+               gradient_ms_gpu=round(gpu_ms, 3), gradient_ms_wall=round(g4["wall_s"] * 1e3, 2),
+               breakdown_ms=dict(forward_series=round(g4["fwd_series_ms"], 3),
+                                 reverse_recompute=round(g4["rev_fwd_ms"], 3),
+                                 reverse_sweep=round(g4["rev_sweep_ms"], 3),
+                                 colour_gradient=round(g4["color_grad_ms"], 3)),
+               form="dense FP16 MFMA (FP32 accumulation)",
+               rel_err_vs_fp64=dict(colours=rel(g4["dcol"], g0["dcol"]), albedo=rel(g4["dalb"], g0["dalb"])),
+               fp64_gradient_ms_gpu=round(g0["fwd_series_ms"] + g0["rev_fwd_ms"] + g0["rev_sweep_ms"]
+                                          + g0["color_grad_ms"], 3))
+    # colour gradient over the checkpointed visibility of a ckpt_lm^2 atlas
+    if ckpt_lm:
+        spec2 = scenes.room_scene(lm=ckpt_lm, **kw)
+        b2, pos2, nrm2, tris2, items2, off2 = prepare_workload(lib, spec2, nth)
+        S = len(pos2)
+        dev2 = lib.lgx_DeviceCreate(P(tris2), len(tris2))
+        _lib.check(lib.lgx_SetSamples(dev2, P(pos2), P(nrm2), S, None), lib)
+        lights2 = make_lights(spec2)
+        _lib.check(lib.lgx_DirectLight(dev2, lights2, len(lights2), items2.ctypes.data_as(u32p), len(items2),
+                                       off2.ctypes.data_as(u32p), 0), lib)
+        lib.lgx_DeviceTimes(dev2, tm)
+        d2_ms = tm[0]
+        lam2 = np.ascontiguousarray(rng.normal(size=(S, 3)))
+        dcol2 = np.zeros((len(lights2), 3))
+        ms = []
+        for _ in range(3):
+            _lib.check(lib.lgx_DirectLightGrad(dev2, lam2.ctypes.data_as(dp), dcol2.ctypes.data_as(dp)), lib)
+            lib.lgx_DeviceTimes(dev2, tm)
+            ms.append(tm[5])
+        gms = float(min(ms))
+        ck_bytes = len(items2) * 4
+        stream = len(items2) * 8 + S * 24  # weights + items streamed, lam gathered once per item in cache
+        out["checkpoint"] = dict(atlas=f"{ckpt_lm}^2", samples=S, lights=len(lights2), items=int(len(items2)),
+                                 visibility_bytes=ck_bytes, direct_light_ms=round(d2_ms, 3),
+                                 colour_gradient_ms=round(gms, 4),
+                                 achieved_gbs=round(stream / (gms / 1e3) / 1e9, 1))
+        log(f"[bench] C3 checkpoint {ckpt_lm}^2: {S} samples x {len(lights2)} lights, {ck_bytes / 1e6:.0f} MB "
+            f"weights; direct {d2_ms:.1f} ms, colour gradient {gms:.3f} ms")
+        lib.lgx_DeviceDestroy(dev2)
+        b2.destroy()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +442,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity (C2) leg")
     ap.add_argument("--radiosity-patches", type=int, default=32768)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (C3) leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (scales the HBM byte counters)")
@@ -396,6 +539,9 @@ def main():
     radio = None
     if not args.no_radiosity:
         radio = radiosity_bench(lib, nth, n_patches=args.radiosity_patches)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_bench(lib, nth)
 
     out = dict(metric=METRIC, value=round(value, 1), unit="shadow rays/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(ms_per_step, 3), higher_is_better=True, scaling="weak",
@@ -406,7 +552,7 @@ def main():
                            triangles=int(info[0]), samples=int(len(pos)), lights=len(spec.lights),
                            work_items=int(len(items)), shadow_rays_per_step=int(marches),
                            march_steps_per_step=int(steps), march_kernel_ms=round(march_avg, 3)),
-               roofline=roofline, cpu_baseline=cpu, parity_sample=parity, radiosity=radio)
+               roofline=roofline, cpu_baseline=cpu, parity_sample=parity, radiosity=radio, adjoint=adjoint)
     print(json.dumps(out), flush=True)
 
 

@@ -526,15 +526,19 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const uint32_t* __restric
 // ---------------------------------------------------------------------------
 // Reverse mode of the accumulation above: the lightmap gains color_l * w
 // per item (lighter.cpp:505-513), so dL/dcolor_l[c] = sum over light l's
-// items of w * lam[s][c], lam = dL/dlightmap per sample.  kGradBlocks blocks
-// per light write FP64 partial sums, reduced in a fixed order on the host.
+// items of w * lam[s][c], lam = dL/dlightmap per sample.  One launch, grid
+// (kGradBlocks, lights); each block writes FP64 partial sums, reduced in a
+// fixed order on the host.
 // ---------------------------------------------------------------------------
-constexpr int kGradBlocks = 64;
+constexpr int kGradBlocks = 256; // blocks per light
 
 __global__ __launch_bounds__(kBlock) void k_color_grad(const uint32_t* __restrict__ items,
-                                                       const float* __restrict__ w, uint32_t begin, uint32_t end,
+                                                       const float* __restrict__ w,
+                                                       const uint32_t* __restrict__ light_off,
                                                        const double* __restrict__ lam, double* __restrict__ part)
 {
+	const uint32_t l = blockIdx.y;
+	const uint32_t begin = light_off[l], end = light_off[l + 1];
 	double a0 = 0, a1 = 0, a2 = 0;
 	for (uint32_t k = begin + blockIdx.x * kBlock + threadIdx.x; k < end; k += kGradBlocks * kBlock) {
 		float wk = w[k];
@@ -563,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_color_grad(const uint32_t* __restric
 		double t = 0;
 		for (int q = 0; q < kBlock / 64; ++q)
 			t += red[q][threadIdx.x];
-		part[3 * blockIdx.x + threadIdx.x] = t;
+		part[3 * ((uint64_t)l * kGradBlocks + blockIdx.x) + threadIdx.x] = t;
 	}
 }
 
@@ -1040,8 +1044,17 @@ void DeviceScene::set_direct_work(const std::vector<DevLight>& lights, const std
 	hipStream_t s = (hipStream_t)stream_;
 	if (lights.size() > 64)
 		throw std::runtime_error("direct_light: at most 64 lights per batch");
-	if (light_off.size() != lights.size() + 1 || light_off.back() != items.size())
+	if (light_off.size() != lights.size() + 1 || light_off.back() != items.size() || light_off[0] != 0)
 		throw std::runtime_error("direct_light: light offsets do not match the work items");
+	// every item must name its segment's light and a sample on the device
+	for (size_t l = 0; l < lights.size(); ++l) {
+		if (light_off[l + 1] < light_off[l])
+			throw std::runtime_error("direct_light: light offsets must not decrease");
+		for (uint32_t k = light_off[l]; k < light_off[l + 1]; ++k)
+			if ((items[k] >> kSampleBits) != l || (items[k] & kSampleMask) >= nsamples_)
+				throw std::runtime_error("direct_light: work item " + std::to_string(k) +
+				                         " names another light or a sample beyond the device's samples");
+	}
 	dfree(d_w_);
 	dfree(d_vis_);
 	dfree(d_items_);
@@ -1133,13 +1146,12 @@ void DeviceScene::direct_grad(const double* lam3, double* dcolor3)
 	double* dpart = dalloc<double>(3 * kGradBlocks * std::max<size_t>(L, 1));
 	HIPCHECK(hipMemcpyAsync(dlam, lam3, 24 * nsamples_, hipMemcpyHostToDevice, s));
 	HIPCHECK(hipMemsetAsync(dpart, 0, 24 * kGradBlocks * std::max<size_t>(L, 1), s));
+	uint32_t* doff = dalloc<uint32_t>(L + 1);
+	HIPCHECK(hipMemcpyAsync(doff, h_light_off_.data(), 4 * (L + 1), hipMemcpyHostToDevice, s));
 	EventTimer tm(s);
-	for (size_t l = 0; l < L; ++l) {
-		uint32_t b = h_light_off_[l], e = h_light_off_[l + 1];
-		if (e > b) {
-			k_color_grad<<<kGradBlocks, kBlock, 0, s>>>(d_items_, d_w_, b, e, dlam, dpart + 3 * kGradBlocks * l);
-			HIPCHECK(hipGetLastError());
-		}
+	if (L) {
+		k_color_grad<<<dim3(kGradBlocks, (unsigned)L), kBlock, 0, s>>>(d_items_, d_w_, doff, dlam, dpart);
+		HIPCHECK(hipGetLastError());
 	}
 	times.grad_ms = tm.stop();
 	std::vector<double> hp(3 * kGradBlocks * L);
@@ -1147,6 +1159,7 @@ void DeviceScene::direct_grad(const double* lam3, double* dcolor3)
 		HIPCHECK(hipMemcpy(hp.data(), dpart, 8 * hp.size(), hipMemcpyDeviceToHost));
 	dfree(dlam);
 	dfree(dpart);
+	dfree(doff);
 	for (size_t l = 0; l < L; ++l)
 		for (int c = 0; c < 3; ++c) {
 			double t = 0;

@@ -59,6 +59,7 @@ class SceneSpec:
     albedo: Optional[dict] = None  # part_id -> rgb, applied through sample_fn
     emission: Optional[dict] = None  # part_id -> rgb, applied through sample_fn
     samples: list = dc.field(default_factory=list)  # external samples: (pos, nrm)
+    sample_cb: Optional[Callable] = None  # sample_fn body: called with the ltr_SampleRequest, after albedo/emission
 
     @property
     def num_triangles(self) -> int:
@@ -362,7 +363,12 @@ def basic_scene():
     tex = np.array([0.1, 0.1, 0.9, 0.1, 0.9, 0.9, 0.1, 0.9], np.float32).reshape(4, 2)
     idx = np.array([0, 1, 2, 2, 3, 0], np.uint32)
     part = Part(pos, nrm, tex, tex.copy(), idx, 1)
-    c, s = math.cos(0.5) * 0.2, math.sin(0.5) * 0.2
+    # cos(0.5f) * 0.2f in float arithmetic with the C library's cosf/sinf
+    libm = C.CDLL("libm.so.6")
+    libm.cosf.restype = libm.sinf.restype = C.c_float
+    libm.cosf.argtypes = libm.sinf.argtypes = [C.c_float]
+    c = np.float32(libm.cosf(0.5)) * np.float32(0.2)
+    s = np.float32(libm.sinf(0.5)) * np.float32(0.2)
     m1 = np.eye(4, dtype=np.float32)
     m2 = np.array([[c, s, 0, 0], [-s, c, 0, 0], [0, 0, 0.2, 0], [0, 0, 0.5, 1]], np.float32)
     m3 = np.array([[1.9, 0, 0, 0], [0, 1.9, 0, 0], [0, 0, 1.9, 0], [0, 0, -0.5, 1]], np.float32)
@@ -398,9 +404,10 @@ class BuiltScene:
             cfg.default_width = spec.lm_size
             cfg.default_height = spec.lm_size
             cfg.max_lightmap_size = 0
-        if spec.albedo is not None or spec.emission is not None:
+        if spec.albedo is not None or spec.emission is not None or spec.sample_cb is not None:
             alb = spec.albedo or {}
             emi = spec.emission or {}
+            extra = spec.sample_cb
 
             def sample_fn(cfgp, req):
                 r = req.contents
@@ -410,6 +417,8 @@ class BuiltScene:
                 e = emi.get(r.part_id)
                 if e is not None:
                     r.out_emissive_color[0], r.out_emissive_color[1], r.out_emissive_color[2] = e
+                if extra is not None:
+                    extra(r)
                 return 1
 
             cb = api.SAMPLE_FN(sample_fn)

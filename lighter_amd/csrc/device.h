@@ -51,6 +51,8 @@ public:
 	size_t num_samples() const { return nsamples_; }
 	size_t num_triangles() const { return ntri_; }
 	int bvh_depth() const { return bvh_depth_; }
+	int march_stack_depth() const;
+	size_t march_lds_bytes() const;
 	size_t bvh_nodes() const { return nnodes_; }
 
 	// --- direct light (lighter.cpp:485-600, 1075-1096) -------------------

@@ -109,16 +109,74 @@ __device__ __forceinline__ RayTri load_rtri(const RayTri* t, uint32_t i)
 	return r;
 }
 
+// Traversal stacks.  The traversals below are nearest-first: at an inner
+// node they push at most the farther child, so the stack holds at most one
+// entry per level of the current path and the BVH's depth bounds it.
+// Entry = node reference + key (squared box distance, or the segment's entry
+// parameter); a popped entry is skipped when its key exceeds the current
+// bound.
+struct PrivStack { // per-lane arrays (private segment)
+	int32_t ref[kStackSize];
+	float key[kStackSize];
+	__device__ __forceinline__ void put(int i, int32_t r, float k)
+	{
+		ref[i] = r;
+		key[i] = k;
+	}
+	__device__ __forceinline__ int32_t r(int i) const { return ref[i]; }
+	__device__ __forceinline__ float k(int i) const { return key[i]; }
+};
+
+// A column of LDS per lane: references [level][lane] as int32, keys as FP16
+// rounded toward zero.  Keys are non-negative, so the stored key never
+// exceeds the true one and skipping stays conservative: at worst a node is
+// visited that the exact key would have skipped, which changes no result.
+__device__ __forceinline__ _Float16 f16_down(float x)
+{
+	_Float16 h = (_Float16)x;
+	if ((float)h > x) {
+		uint16_t b;
+		memcpy(&b, &h, 2);
+		b = (uint16_t)(b - 1); // x >= 0 and h > x imply h > 0
+		memcpy(&h, &b, 2);
+	}
+	return h;
+}
+
+// The first kLdsLevels levels in LDS, deeper ones (rare: only a path that
+// pushed at more than kLdsLevels of its levels) in a per-lane array.
+constexpr int kLdsLevels = 12;
+
+struct LdsStack {
+	int32_t* ref;  // &refs[0][lane]
+	_Float16* key; // &keys[0][lane]
+	int32_t oref[kStackSize - kLdsLevels];
+	float okey[kStackSize - kLdsLevels];
+	__device__ __forceinline__ void put(int i, int32_t r, float k)
+	{
+		if (i < kLdsLevels) {
+			ref[i * kBlock] = r;
+			key[i * kBlock] = f16_down(k);
+		} else {
+			oref[i - kLdsLevels] = r;
+			okey[i - kLdsLevels] = k;
+		}
+	}
+	__device__ __forceinline__ int32_t r(int i) const { return i < kLdsLevels ? ref[i * kBlock] : oref[i - kLdsLevels]; }
+	__device__ __forceinline__ float k(int i) const
+	{
+		return i < kLdsLevels ? (float)key[i * kBlock] : okey[i - kLdsLevels];
+	}
+};
+
 // Nearest-first traversal for the distance query: min(radius, distance to the
 // nearest triangle closer than radius).
-template <bool COUNT>
-__device__ float q_distance(const View& V, f3 p, float radius, unsigned long long* cnt)
+template <bool COUNT, class Stack>
+__device__ float q_distance(const View& V, f3 p, float radius, unsigned long long* cnt, Stack& st)
 {
 	float best = radius;
 	if (!V.has_tris)
 		return best;
-	int32_t sref[kStackSize];
-	float sd[kStackSize];
 	int sp = 0;
 	int32_t ref = 0;
 	unsigned nn = 0, nt = 0;
@@ -136,8 +194,7 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 			float lim = best * best;
 			if (d0 <= lim) {
 				if (d1 <= lim) {
-					sref[sp] = r1;
-					sd[sp] = d1;
+					st.put(sp, r1, d1);
 					sp++;
 				}
 				ref = r0;
@@ -157,8 +214,8 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 		bool found = false;
 		while (sp > 0) {
 			--sp;
-			if (sd[sp] <= best * best) {
-				ref = sref[sp];
+			if (st.k(sp) <= best * best) {
+				ref = st.r(sp);
 				found = true;
 				break;
 			}
@@ -172,6 +229,13 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 		atomicAdd(cnt + 3, 1ull);
 	}
 	return best;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ float q_distance(const View& V, f3 p, float radius, unsigned long long* cnt)
+{
+	PrivStack st;
+	return q_distance<COUNT>(V, p, radius, cnt, st);
 }
 
 struct Seg {
@@ -206,15 +270,13 @@ __device__ __forceinline__ float slab(const Seg& s, float lx, float ly, float lz
 }
 
 // ANY = true: returns 0 on the first hit with r < 1 (occlusion), else 2.
-template <bool ANY, bool COUNT>
-__device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt)
+template <bool ANY, bool COUNT, class Stack>
+__device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt, Stack& st)
 {
 	float best = 2.0f;
 	if (!V.has_tris)
 		return best;
 	Seg s = make_seg(A, B);
-	int32_t sref[kStackSize];
-	float sd[kStackSize];
 	int sp = 0;
 	int32_t ref = 0;
 	unsigned nn = 0, nt = 0;
@@ -232,8 +294,7 @@ __device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt)
 			}
 			if (t0 != INFINITY) {
 				if (t1 != INFINITY) {
-					sref[sp] = r1;
-					sd[sp] = t1;
+					st.put(sp, r1, t1);
 					sp++;
 				}
 				ref = r0;
@@ -259,8 +320,8 @@ __device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt)
 		bool found = false;
 		while (sp > 0) {
 			--sp;
-			if (sd[sp] <= fminf(best, 1.0f)) {
-				ref = sref[sp];
+			if (st.k(sp) <= fminf(best, 1.0f)) {
+				ref = st.r(sp);
 				found = true;
 				break;
 			}
@@ -274,6 +335,13 @@ __device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt)
 		atomicAdd(cnt + 3, 1ull);
 	}
 	return best;
+}
+
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt)
+{
+	PrivStack st;
+	return q_segment<ANY, COUNT>(V, A, B, cnt, st);
 }
 
 // Exit parameter of the ray from + rd*t from the grown scene box, padded so
@@ -335,13 +403,13 @@ __device__ __forceinline__ float lipschitz_margin(f3 p)
 }
 
 // Advances one step; returns true when the march is over (result in M.res).
-template <bool COUNT>
-__device__ __forceinline__ bool march_step(March& M, const View& V, unsigned long long* cnt)
+template <bool COUNT, class Stack>
+__device__ __forceinline__ bool march_step(March& M, const View& V, unsigned long long* cnt, Stack& stack)
 {
 	if (!(M.t < M.maxt) || M.t > M.t_out)
 		return true;
 	f3 p = M.from + M.rd * M.t;
-	float h = q_distance<COUNT>(V, p, M.radius, cnt);
+	float h = q_distance<COUNT>(V, p, M.radius, cnt, stack);
 	M.steps++;
 	if (h < 0.001f) {
 		M.res = 0.0f;
@@ -362,7 +430,8 @@ __device__ float shadow_march(const View& V, f3 from, f3 to, float k, unsigned l
 {
 	March M;
 	march_init(M, V, from, to, k);
-	while (!march_step<COUNT>(M, V, cnt)) {
+	PrivStack st;
+	while (!march_step<COUNT>(M, V, cnt, st)) {
 	}
 	if (COUNT) {
 		atomicAdd(cnt + 2, (unsigned long long)M.steps);
@@ -458,6 +527,9 @@ __global__ __launch_bounds__(kBlock) void k_direct_gate(const float* __restrict_
 // list is exhausted and its lanes' marches have ended.  (A variant that
 // interleaves single traversal actions of all lanes in one loop measured
 // 37% slower on the terrain workload.)
+// The traversal stack's first levels live in LDS (dynamic, kBlock * depth * 6
+// bytes: an int32 reference and an FP16 key per level and lane), depth =
+// min(kLdsLevels, levels of the BVH plus margin).
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_march(View V, const float* __restrict__ pos,
                                                   const float* __restrict__ nrm, const DevLight* __restrict__ lights,
@@ -465,8 +537,12 @@ __global__ __launch_bounds__(kBlock) void k_march(View V, const float* __restric
                                                   const uint32_t* __restrict__ mlist, uint32_t nmarch,
                                                   const float* __restrict__ gate, float* __restrict__ wout,
                                                   float* __restrict__ vout, uint32_t* __restrict__ next,
-                                                  unsigned long long* cnt)
+                                                  unsigned long long* cnt, int depth)
 {
+	extern __shared__ __align__(16) unsigned char lds_stack[];
+	LdsStack st;
+	st.ref = reinterpret_cast<int32_t*>(lds_stack) + threadIdx.x;
+	st.key = reinterpret_cast<_Float16*>(lds_stack + 4 * (size_t)kBlock * depth) + threadIdx.x;
 	bool active = false, done = false;
 	uint32_t k = 0;
 	March M;
@@ -491,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_march(View V, const float* __restric
 		}
 		if (__all(done))
 			break;
-		if (active && march_step<COUNT>(M, V, cnt)) {
+		if (active && march_step<COUNT>(M, V, cnt, st)) {
 			// weight = (shading factor) * shadow factor, in the reference's order
 			wout[k] = gate[k] * M.res;
 			if (vout)
@@ -881,6 +957,12 @@ DeviceScene::DeviceScene()
 }
 
 // Blocks of the persistent march kernel: as many as can be resident at once.
+// stack levels of the march kernel: a leaf at depth d has at most d entries
+// below it on the stack
+int DeviceScene::march_stack_depth() const { return std::min(kLdsLevels, bvh_depth_ + 2); }
+
+size_t DeviceScene::march_lds_bytes() const { return (size_t)kBlock * march_stack_depth() * 6; }
+
 unsigned DeviceScene::persistent_blocks()
 {
 	if (!persist_blocks_) {
@@ -888,7 +970,13 @@ unsigned DeviceScene::persistent_blocks()
 		hipDeviceProp_t p;
 		HIPCHECK(hipGetDevice(&dev));
 		HIPCHECK(hipGetDeviceProperties(&p, dev));
-		HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<false>, kBlock, 0));
+		if (march_lds_bytes() > 65536) {
+			HIPCHECK(hipFuncSetAttribute((const void*)k_march<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+			                             (int)march_lds_bytes()));
+			HIPCHECK(hipFuncSetAttribute((const void*)k_march<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+			                             (int)march_lds_bytes()));
+		}
+		HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<false>, kBlock, march_lds_bytes()));
 		persist_blocks_ = (unsigned)std::max(1, per_cu) * (unsigned)p.multiProcessorCount;
 	}
 	return persist_blocks_;
@@ -961,6 +1049,7 @@ void DeviceScene::set_triangles(const float* tris, size_t ntri)
 	if (b.depth + 2 >= kStackSize)
 		throw std::runtime_error("BVH deeper than the traversal stack");
 	bvh_depth_ = b.depth;
+	persist_blocks_ = 0; // the march kernel's LDS stack follows the depth
 	nnodes_ = b.nodes.size();
 	std::vector<DistTri> dt(ntri);
 	std::vector<RayTri> rt(ntri);
@@ -1094,12 +1183,15 @@ void DeviceScene::run_direct()
 		EventTimer tm(s);
 		if (nmarch) {
 			unsigned blocks = persistent_blocks();
+			const size_t lds = march_lds_bytes();
 			if (count_queries)
-				k_march<true><<<blocks, kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch,
-				                                        d_gate_, d_w_, d_vis_, d_mcount_ + 1, d_counters_);
+				k_march<true><<<blocks, kBlock, lds, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch,
+				                                          d_gate_, d_w_, d_vis_, d_mcount_ + 1, d_counters_,
+				                                          march_stack_depth());
 			else
-				k_march<false><<<blocks, kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch,
-				                                         d_gate_, d_w_, d_vis_, d_mcount_ + 1, d_counters_);
+				k_march<false><<<blocks, kBlock, lds, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch,
+				                                           d_gate_, d_w_, d_vis_, d_mcount_ + 1, d_counters_,
+				                                           march_stack_depth());
 			HIPCHECK(hipGetLastError());
 		}
 		times.direct_march_ms = tm.stop();

@@ -433,6 +433,22 @@ def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
     return out
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor, KiB per
+    dispatch), when it was measured on this same workload; else None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t[kernel]
+        if k["workload"] != workload:
+            return None
+        return int(round((k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"]) * 1024))
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -458,7 +474,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path "
-              "is a design described in DESIGN.md and is not run.", file=sys.stderr)
+              "is a design only and is not run.", file=sys.stderr)
         sys.exit(2)
 
     lib = _lib.load()
@@ -515,7 +531,9 @@ def main():
     alg_bytes = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + io_bytes
     achieved = alg_bytes / (march_avg / 1000.0) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic("k_march", dict(triangles=int(info[0]), work_items=int(len(items)),
+                                                             marches=int(marches))),
                     kernel="k_march", alg_bytes_per_launch=int(alg_bytes),
                     note="node visits x 64 B + distance-triangle tests x 160 B + per-item I/O, counted by an "
                          "instrumented pass; the working set (BVH + triangles) is cache resident")

@@ -14,10 +14,11 @@
  * for x86-64 without -march flags.  Transcendentals (powf, acosf, sinf, cosf)
  * come from the same C library the reference calls.
  *
- * Pinning (tests/test_oracle_*.py): each function is checked against the
- * reference compiled from its own sources (oracle/_ref) on seeded inputs and
- * against closed forms / brute-force FP64 geometry; see DESIGN.md for the
- * per-function table.
+ * Pinning (tests/test_oracle.py, tests/test_golden.py): each function is
+ * checked against the reference compiled from its own sources (oracle/_ref)
+ * on seeded inputs, against golden fixtures of the reference's own test
+ * scenarios (tests/golden), and against closed forms / brute-force FP64
+ * geometry.
  */
 #ifndef LIGHTER_ORACLE_H
 #define LIGHTER_ORACLE_H

@@ -58,7 +58,10 @@ def host_threads():
         return max(1, min(16, os.cpu_count() or 1))
 
 
-def prepare_workload(lib, spec, nthreads):
+def prepare_workload(lib, spec, nthreads, radinfo=None):
+    """Host stages through the product; samples, shadow triangles and the
+    direct-light work items of every light.  radinfo (a list) receives the
+    per-texel radiosity info (uv, part id, world area) when given."""
     b = scenes.BuiltScene(lib, spec, dict(max_num_threads=nthreads))
     _lib.check(lib.lgx_Prepare(b.scene), lib)
     n_inst = lib.lgx_InstanceCount(b.scene)
@@ -69,7 +72,10 @@ def prepare_workload(lib, spec, nthreads):
         n, nt = info[2], info[3]
         p = np.zeros((n, 3), np.float32)
         q = np.zeros((n, 3), np.float32)
-        lib.lgx_GetSamples(b.scene, i, P(p), P(q), None, None)
+        r = np.zeros((n, 4), np.float32) if radinfo is not None else None
+        lib.lgx_GetSamples(b.scene, i, P(p), P(q), None, P(r) if r is not None else None)
+        if radinfo is not None:
+            radinfo.append(r)
         t = np.zeros((nt, 9), np.float32)
         if nt:
             lib.lgx_GetTriangles(b.scene, i, P(t))
@@ -433,6 +439,160 @@ def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
     return out
 
 
+def prepare_with_progress(lib, spec, nth, every_s=20.0, radinfo=None):
+    """prepare_workload, logging the host stage every every_s seconds (the
+    2M-triangle scene spends minutes in the reference's sample generation)."""
+    import threading
+    res = {}
+    th = threading.Thread(target=lambda: res.update(out=prepare_workload(lib, spec, nth, radinfo)))
+    t0 = time.perf_counter()
+    th.start()
+    while th.is_alive():
+        th.join(every_s)
+        if th.is_alive():
+            log(f"[bench] host stages running ({time.perf_counter() - t0:.0f} s)")
+    return res["out"]
+
+
+def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=65536, bounces=4):
+    """C4, full bake at scale: 2M-triangle city block, 4096^2 atlas, 64 lights:
+    direct-light shadow rays/s (with the reference's CPU march on a sample),
+    64 AO rays per texel, 65,536-patch radiosity in the MFMA forms with
+    refinement, and the gradient with respect to the 64 light colours."""
+    t0 = time.perf_counter()
+    spec = scenes.city_scene()
+    log(f"[bench] C4 scene: {spec.num_triangles} tris, {len(spec.lights)} lights ({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    radinfo = []
+    built, pos, nrm, tris, items, light_off = prepare_with_progress(lib, spec, nth, radinfo=radinfo)
+    radinfo = np.concatenate(radinfo)
+    host_s = time.perf_counter() - t0
+    S = len(pos)
+    log(f"[bench] C4 host stages: {S} samples, {len(items)} work items ({host_s:.1f} s)")
+    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    if not dev:
+        raise RuntimeError(_lib.last_error(lib))
+    info = (C.c_int64 * 3)()
+    lib.lgx_DeviceInfo(dev, info)
+    _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), S, None), lib)
+    lights = make_lights(spec)
+    _lib.check(lib.lgx_DirectLightSetup(dev, lights, len(lights), items.ctypes.data_as(u32p), len(items),
+                                        light_off.ctypes.data_as(u32p), 1), lib)
+    lib.lgx_SetCounting(dev, 1)
+    _lib.check(lib.lgx_DirectLightRun(dev), lib)
+    cnt = (C.c_uint64 * 5)()
+    lib.lgx_ReadCounters(dev, cnt)
+    nodes, ntris, qsteps, queries, marches = list(cnt)
+    lib.lgx_SetCounting(dev, 0)
+    tm = (C.c_double * 6)()
+    for _ in range(warmup):
+        _lib.check(lib.lgx_DirectLightRun(dev), lib)
+    march_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _lib.check(lib.lgx_DirectLightRun(dev), lib)
+        lib.lgx_DeviceTimes(dev, tm)
+        march_ms.append(tm[1])
+    ms_step = (time.perf_counter() - t0) * 1000.0 / steps
+    rays = marches / (ms_step / 1000.0)
+    march_avg = float(np.mean(march_ms))
+    alg = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + len(items) * 36
+    log(f"[bench] C4 direct: BVH {info[1]} nodes depth {info[2]}; {marches} marches, {ms_step:.1f} ms/step "
+        f"(march {march_avg:.1f} ms): {rays / 1e6:.1f} M rays/s")
+    w = np.zeros(len(items), np.float32)
+    vis = np.zeros(len(items), np.float32)
+    _lib.check(lib.lgx_ReadWeights(dev, P(w), P(vis), len(items)), lib)
+    out = dict(workload=f"C4 city block: {spec.num_triangles} tris (1 km heightfield + 155,000 lognormal boxes in "
+                        f"16 x 16 tiles), 4096^2 atlas ({S} texels), 64 point/spot lights",
+               triangles=int(info[0]), samples=S, work_items=int(len(items)), host_stages_s=round(host_s, 1),
+               direct=dict(shadow_rays_per_s=round(rays, 1), marches=int(marches), ms_per_step=round(ms_step, 3),
+                           march_kernel_ms=round(march_avg, 3),
+                           roofline=dict(bound="hbm", achieved=round(alg / (march_avg / 1e3) / 1e9, 1),
+                                         peak=HBM_PEAK_GBS, unit="GB/s",
+                                         frac=round(alg / (march_avg / 1e3) / 1e9 / HBM_PEAK_GBS, 4))))
+    # ambient occlusion: 64 rays per texel, 2 m
+    rng = np.random.default_rng(0x4C540004)
+    ro = rng.uniform(0, 1, S).astype(np.float32)
+    col = np.zeros(3, np.float32)
+    _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
+    _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
+    lib.lgx_DeviceTimes(dev, tm)
+    out["ao"] = dict(rays=S * 64, ms=round(tm[2], 3), rays_per_s=round(S * 64 / (tm[2] / 1e3), 1))
+    log(f"[bench] C4 AO: {S * 64} rays in {tm[2]:.1f} ms")
+    # colour gradient of L = 1/2 |direct - target|^2 over every texel (64 lights)
+    lm = np.zeros((S, 3), np.float32)
+    lib.lgx_ReadLightmap(dev, P(lm))
+    lam = np.ascontiguousarray(lm.astype(np.float64) * rng.uniform(-0.3, 0.3, (S, 1)))
+    dcol = np.zeros((len(lights), 3))
+    dp = C.POINTER(C.c_double)
+    _lib.check(lib.lgx_DirectLightSetup(dev, lights, len(lights), items.ctypes.data_as(u32p), len(items),
+                                        light_off.ctypes.data_as(u32p), 0), lib)
+    _lib.check(lib.lgx_DirectLightRun(dev), lib)
+    _lib.check(lib.lgx_DirectLightGrad(dev, lam.ctypes.data_as(dp), dcol.ctypes.data_as(dp)), lib)
+    lib.lgx_DeviceTimes(dev, tm)
+    out["colour_gradient"] = dict(ms=round(tm[5], 4), items=int(len(items)),
+                                  weights_bytes=int(len(items) * 4))
+    log(f"[bench] C4 colour gradient over {len(items)} items: {tm[5]:.3f} ms")
+    # radiosity on the first n_patches texels, 192 right-hand sides from the lights' weights
+    from oracle.checkers import normalize_f32
+    N = min(n_patches, S)
+    pn = np.array([normalize_f32(v) for v in nrm[:N]], np.float32)
+    ppos = np.ascontiguousarray(pos[:N])
+    rad_dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    _lib.check(lib.lgx_SetSamples(rad_dev, P(ppos), P(np.ascontiguousarray(nrm[:N])), N, None), lib)
+    nl = C.c_size_t(0)
+    _lib.check(lib.lgx_RadLinks(rad_dev, P(ppos), P(pn), N, C.byref(nl)), lib)
+    lib.lgx_DeviceTimes(rad_dev, tm)
+    link_ms = tm[3]
+    lid = (items >> 26).astype(np.int64)
+    smp = (items & ((1 << 26) - 1)).astype(np.int64)
+    wv = np.where(w.view(np.uint32) == 0xFFFFFFFF, 0.0, w.astype(np.float64))
+    sel = smp < N
+    Wp = np.zeros((N, len(lights)))
+    np.add.at(Wp, (smp[sel], lid[sel]), wv[sel])
+    cols = rng.uniform(0.5, 1.0, (64, len(lights), 3))
+    # the lights' direct contributions plus a small uniform emission, so that
+    # no column is empty where the lights do not reach the first patches
+    E = np.ascontiguousarray(np.concatenate([Wp @ cols[s] for s in range(64)], 1) + 0.05)
+    diff = np.full((N, 3), 0.6, np.float32)
+    area = np.ascontiguousarray(radinfo[:N, 3])  # world area per texel, as the reference's patches carry
+    R = E.shape[1]
+
+    def run(form, refine=0):
+        o = np.zeros_like(E)
+        t = (C.c_double * 4)()
+        _lib.check(lib.lgx_RadSolve(rad_dev, E.ctypes.data_as(dp), R, P(diff), P(area), form, bounces, refine,
+                                    o.ctypes.data_as(dp), t), lib)
+        return o, list(t)
+
+    ref64, t64 = run(0)
+    forms = {}
+    for form, refine in [(0, 0), (2, 0), (3, 0), (3, 1), (4, 0), (4, 1)]:
+        o, t = (ref64, t64) if form == 0 else run(form, refine)
+        key = FORMS[form] + ("_refined" if refine else "")
+        per = t[2] if not refine else (t[1] + t[3]) / bounces
+        forms[key] = dict(s_per_bounce=per / 1e3, rel_l2_err_vs_fp64=float(np.linalg.norm(o - ref64) / np.linalg.norm(ref64)))
+        log(f"[bench] C4 radiosity {key}: {per:.3f} ms/bounce, err {forms[key]['rel_l2_err_vs_fp64']:.2e}")
+    out["radiosity"] = dict(patches=N, rhs=R, bounces=bounces, links=int(nl.value), link_generation_ms=round(link_ms, 1),
+                            forms=forms)
+    lib.lgx_DeviceDestroy(rad_dev)
+    if with_cpu:
+        cpu, samp = cpu_baseline(spec, pos, nrm, items, w, lights, cpu_seconds, nth)
+        out["cpu_baseline"] = cpu
+        if cpu:
+            out["direct"]["speedup_vs_cpu"] = round(rays / cpu["value"], 1)
+            sel_, refvis = samp
+            out["parity_sample"] = dict(sampled=int(len(sel_)),
+                                        bitexact_fraction=float(np.mean(vis[sel_].view(np.uint32)
+                                                                        == refvis.view(np.uint32))),
+                                        shadowed_agreement=float(np.mean((vis[sel_] == 0) == (refvis == 0))))
+            log(f"[bench] C4 cpu baseline {cpu['value']:.0f} rays/s; GPU/CPU {rays / cpu['value']:.0f}x; "
+                f"parity {out['parity_sample']}")
+    lib.lgx_DeviceDestroy(dev)
+    built.destroy()
+    return out
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor, KiB per
@@ -459,6 +619,8 @@ def main():
     ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity (C2) leg")
     ap.add_argument("--radiosity-patches", type=int, default=32768)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (C3) leg")
+    ap.add_argument("--c4", action="store_true",
+                    help="only the 2M-triangle full-bake configuration (several minutes of host stages)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (scales the HBM byte counters)")
@@ -480,6 +642,10 @@ def main():
     lib = _lib.load()
     _lib.check(lib.lgx_HasDevice(), lib)
     nth = host_threads()
+    if args.c4:
+        out = c4_bench(lib, nth, args.steps, args.warmup, args.cpu_seconds, with_cpu=not args.no_cpu_baseline)
+        print(json.dumps(dict(metric=METRIC, config="C4", n_gpus=1, c4=out)), flush=True)
+        return
     t0 = time.perf_counter()
     if args.small:
         spec = scenes.terrain_scene(grid=128, extent=50.0, n_boxes=100, tiles=4, lm_total=512, n_lights=8,

@@ -169,6 +169,16 @@ struct LdsStack {
 	}
 };
 
+// The calling lane's LDS stack in the kernel's dynamic LDS (kBlock x depth
+// int32 references, then kBlock x depth FP16 keys).
+__device__ __forceinline__ LdsStack lds_stack_of_lane(unsigned char* lds, int depth)
+{
+	LdsStack st;
+	st.ref = reinterpret_cast<int32_t*>(lds) + threadIdx.x;
+	st.key = reinterpret_cast<_Float16*>(lds + 4 * (size_t)kBlock * depth) + threadIdx.x;
+	return st;
+}
+
 // Nearest-first traversal for the distance query: min(radius, distance to the
 // nearest triangle closer than radius).
 template <bool COUNT, class Stack>
@@ -540,9 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_march(View V, const float* __restric
                                                   unsigned long long* cnt, int depth)
 {
 	extern __shared__ __align__(16) unsigned char lds_stack[];
-	LdsStack st;
-	st.ref = reinterpret_cast<int32_t*>(lds_stack) + threadIdx.x;
-	st.key = reinterpret_cast<_Float16*>(lds_stack + 4 * (size_t)kBlock * depth) + threadIdx.x;
+	LdsStack st = lds_stack_of_lane(lds_stack, depth);
 	bool active = false, done = false;
 	uint32_t k = 0;
 	March M;
@@ -660,8 +668,10 @@ template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_ao(View V, const float* __restrict__ pos, const float* __restrict__ nrm,
                                                const float* __restrict__ randoff, const float2* __restrict__ side,
                                                AOParams P, uint32_t S, float* __restrict__ lm,
-                                               unsigned long long* cnt)
+                                               unsigned long long* cnt, int depth)
 {
+	extern __shared__ __align__(16) unsigned char lds_stack[];
+	LdsStack st = lds_stack_of_lane(lds_stack, depth);
 	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= S)
 		return;
@@ -675,7 +685,7 @@ __global__ __launch_bounds__(kBlock) void k_ao(View V, const float* __restrict__
 		sincos((double)angle, &sa, &ca);
 		float2 sd = side[i];
 		f3 d = spiral_dir(SN, sd.x, sd.y, (float)ca, (float)sa) * P.distance;
-		float hit = q_segment<false, COUNT>(V, origin, origin + d, cnt);
+		float hit = q_segment<false, COUNT>(V, origin, origin + d, cnt, st);
 		if (hit < 1.0f)
 			ao += 1.0f - hit;
 	}
@@ -782,13 +792,15 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const float* __restrict__ pos,
 
 __global__ __launch_bounds__(kBlock) void k_link_vis(View V, const float* __restrict__ pos,
                                                      const uint2* __restrict__ cand, uint64_t ncand,
-                                                     uint32_t* __restrict__ bits, uint64_t words)
+                                                     uint32_t* __restrict__ bits, uint64_t words, int depth)
 {
+	extern __shared__ __align__(16) unsigned char lds_stack[];
+	LdsStack st = lds_stack_of_lane(lds_stack, depth);
 	uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= ncand)
 		return;
 	uint2 c = cand[k];
-	if (q_segment<true, false>(V, ld3(pos + 3 * c.x), ld3(pos + 3 * c.y), nullptr) < 1.0f) {
+	if (q_segment<true, false>(V, ld3(pos + 3 * c.x), ld3(pos + 3 * c.y), nullptr, st) < 1.0f) {
 		atomicOr(bits + (uint64_t)c.x * words + (c.y >> 5), 1u << (c.y & 31));
 		atomicOr(bits + (uint64_t)c.y * words + (c.x >> 5), 1u << (c.x & 31));
 	}
@@ -957,8 +969,9 @@ DeviceScene::DeviceScene()
 }
 
 // Blocks of the persistent march kernel: as many as can be resident at once.
-// stack levels of the march kernel: a leaf at depth d has at most d entries
-// below it on the stack
+// LDS stack levels of the traversal kernels (march, AO, link visibility): a
+// leaf at depth d has at most d entries below it on the stack; at most
+// kLdsLevels levels (18 KB per block) live in LDS
 int DeviceScene::march_stack_depth() const { return std::min(kLdsLevels, bvh_depth_ + 2); }
 
 size_t DeviceScene::march_lds_bytes() const { return (size_t)kBlock * march_stack_depth() * 6; }
@@ -970,12 +983,6 @@ unsigned DeviceScene::persistent_blocks()
 		hipDeviceProp_t p;
 		HIPCHECK(hipGetDevice(&dev));
 		HIPCHECK(hipGetDeviceProperties(&p, dev));
-		if (march_lds_bytes() > 65536) {
-			HIPCHECK(hipFuncSetAttribute((const void*)k_march<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-			                             (int)march_lds_bytes()));
-			HIPCHECK(hipFuncSetAttribute((const void*)k_march<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-			                             (int)march_lds_bytes()));
-		}
 		HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<false>, kBlock, march_lds_bytes()));
 		persist_blocks_ = (unsigned)std::max(1, per_cu) * (unsigned)p.multiProcessorCount;
 	}
@@ -1308,11 +1315,13 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 	P.color[2] = color[2];
 	EventTimer tm(s);
 	if (count_queries)
-		k_ao<true><<<grid_for(nsamples_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P, (uint32_t)nsamples_,
-		                                                  d_lm_, d_counters_);
+		k_ao<true><<<grid_for(nsamples_), kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
+		                                                                  (uint32_t)nsamples_, d_lm_, d_counters_,
+		                                                                  march_stack_depth());
 	else
-		k_ao<false><<<grid_for(nsamples_), kBlock, 0, s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P, (uint32_t)nsamples_,
-		                                                   d_lm_, d_counters_);
+		k_ao<false><<<grid_for(nsamples_), kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
+		                                                                   (uint32_t)nsamples_, d_lm_, d_counters_,
+		                                                                   march_stack_depth());
 	HIPCHECK(hipGetLastError());
 	times.ao_ms = tm.stop();
 	read_counters();
@@ -1451,7 +1460,8 @@ size_t DeviceScene::rad_links()
 		dfree(counts);
 		dfree(offs);
 		if (ncand) {
-			k_link_vis<<<grid_for(ncand), kBlock, 0, s>>>(V, rad.pos, cand, ncand, rad.bits, rad.words);
+			k_link_vis<<<grid_for(ncand), kBlock, march_lds_bytes(), s>>>(V, rad.pos, cand, ncand, rad.bits,
+			                                                              rad.words, march_stack_depth());
 			HIPCHECK(hipGetLastError());
 		}
 		HIPCHECK(hipStreamSynchronize(s));

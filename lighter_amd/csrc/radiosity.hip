@@ -245,6 +245,7 @@ __global__ __launch_bounds__(kB) void k_prep_x32(const float* __restrict__ xT, c
 // every wave's part visible.
 // ---------------------------------------------------------------------------
 constexpr int kKS = 32; // k per stage
+constexpr int kMaxPD = 6; // deepest LDS ring
 
 template <int P>
 __device__ __forceinline__ f32x4 mfma16(i16x8 a, i16x8 b, f32x4 c)
@@ -320,7 +321,7 @@ template <int NPL, int RT>
 struct GemmLane {
 	uint64_t xsrc, gsrc; // DMA source offsets (elements) at the split's first k
 	int wave, rofs;      // wave index; ds_read_b128 offset of the fragment in a 1 KB group
-	unsigned char* buf[4]; // the LDS stage ring
+	unsigned char* buf[kMaxPD]; // the LDS stage ring
 };
 
 // one k stage (buffer B of PD): wait for it, free the previous buffer, issue
@@ -380,6 +381,25 @@ __device__ __forceinline__ void gemm_steps(int s, const GemmLane<NPL, RT>& L, co
 	(gemm_step<P, NPL, RT, PD, Bs>(s + Bs, nst, L, G, Glo, xq, xlo, Np, acc), ...);
 }
 
+template <int NPL, int RT, int... Bs>
+__device__ __forceinline__ void gemm_prologue(int nst, const GemmLane<NPL, RT>& L, const uint16_t* G,
+                                              const uint16_t* Glo, const uint16_t* xq, const uint16_t* xlo,
+                                              uint64_t Np, std::integer_sequence<int, Bs...>)
+{
+	// stage Bs into buffer Bs (the last stage again where there are fewer)
+	(gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, (Bs < nst ? Bs : nst - 1) * kKS, L.buf[Bs], L.wave),
+	 ...);
+}
+
+template <int P, int NPL, int RT, int PD, int... Bs>
+__device__ __forceinline__ void gemm_tail(int s, int nst, const GemmLane<NPL, RT>& L, const uint16_t* G,
+                                          const uint16_t* Glo, const uint16_t* xq, const uint16_t* xlo,
+                                          uint64_t Np, f32x4 (&acc)[RT][kColTiles],
+                                          std::integer_sequence<int, Bs...>)
+{
+	((s + Bs < nst ? gemm_step<P, NPL, RT, PD, Bs>(s + Bs, nst, L, G, Glo, xq, xlo, Np, acc) : void()), ...);
+}
+
 template <int P, int NPL, int RT, int PD>
 __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G, const uint16_t* __restrict__ Glo,
                                                   const uint16_t* __restrict__ xq, const uint16_t* __restrict__ xlo,
@@ -392,13 +412,16 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 	// one variable per stage buffer: the compiler's wait-count tracking can
 	// then tell the DMA into one buffer from the reads of another (with a
 	// single array every LDS read would wait for every DMA in flight)
+	static_assert(PD <= kMaxPD, "pipeline depth");
 	__shared__ __align__(1024) unsigned char lds0[S::kStage], lds1[S::kStage], lds2[PD > 2 ? S::kStage : 16],
-	    lds3[PD > 3 ? S::kStage : 16];
+	    lds3[PD > 3 ? S::kStage : 16], lds4[PD > 4 ? S::kStage : 16], lds5[PD > 5 ? S::kStage : 16];
 	GemmLane<NPL, RT> L;
 	L.buf[0] = lds0;
 	L.buf[1] = lds1;
 	L.buf[2] = lds2;
 	L.buf[3] = lds3;
+	L.buf[4] = lds4;
+	L.buf[5] = lds5;
 	L.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const uint32_t i0 = blockIdx.x * S::kRows + L.wave * 16 * RT; // this wave's first row
 	const uint32_t klen = Np / ksplit, kbeg = blockIdx.y * klen;
@@ -418,24 +441,12 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 			acc[t][c] = f32x4{0, 0, 0, 0};
 
 	// prologue: stages 0 .. PD-2 into buffers 0 .. PD-2 (nst >= 1)
-	gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, 0, L.buf[0], L.wave);
-	if (PD > 2)
-		gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, (nst > 1 ? 1 : 0) * kKS,
-		                    L.buf[1], L.wave);
-	if (PD > 3)
-		gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, (nst > 2 ? 2 : nst - 1) * kKS,
-		                    L.buf[2], L.wave);
-	static_assert(PD <= 4, "prologue covers up to four buffers");
+	gemm_prologue<NPL, RT>(nst, L, G, Glo, xq, xlo, Np, std::make_integer_sequence<int, PD - 1>{});
 	int s = 0;
 	for (; s + PD <= nst; s += PD)
 		gemm_steps<P, NPL, RT, PD>(s, L, G, Glo, xq, xlo, Np, acc, nst, std::make_integer_sequence<int, PD>{});
 	// fewer than PD stages left
-	if (s < nst)
-		gemm_step<P, NPL, RT, PD, 0>(s, nst, L, G, Glo, xq, xlo, Np, acc);
-	if (PD > 2 && s + 1 < nst)
-		gemm_step<P, NPL, RT, PD, (PD > 2 ? 1 : 0)>(s + 1, nst, L, G, Glo, xq, xlo, Np, acc);
-	if (PD > 3 && s + 2 < nst)
-		gemm_step<P, NPL, RT, PD, (PD > 3 ? 2 : 0)>(s + 2, nst, L, G, Glo, xq, xlo, Np, acc);
+	gemm_tail<P, NPL, RT, PD>(s, nst, L, G, Glo, xq, xlo, Np, acc, std::make_integer_sequence<int, PD - 1>{});
 	wait_vm<0>(); // no DMA may land after the block has left its LDS
 
 	// part[s][c][i]: column-major per split
@@ -452,7 +463,7 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 
 // the two instantiations: plain (64 rows per wave, 4 stages in flight) and
 // error-compensated hi + lo (32 rows per wave, 3 stages)
-constexpr int kPlainRT = 4, kPlainPD = 4, kSplitRT = 2, kSplitPD = 3;
+constexpr int kPlainRT = 4, kPlainPD = 5, kSplitRT = 2, kSplitPD = 3;
 
 // ---------------------------------------------------------------------------
 // plain FP32 GEMM with the same blocking: 256 threads, each 8 rows x 12

@@ -13,6 +13,7 @@ shading normal.
 from __future__ import annotations
 
 import ctypes as C
+import functools
 import dataclasses as dc
 import math
 from typing import Callable, Optional
@@ -86,7 +87,7 @@ class Chart:
     part: int
     height: Optional[np.ndarray] = None  # (nv+1, nu+1) displacement along +z (terrain)
 
-    @property
+    @functools.cached_property
     def size_m(self):
         return float(np.linalg.norm(self.du)), float(np.linalg.norm(self.dv))
 
@@ -115,9 +116,10 @@ def pack_charts(charts, W, H, pad=2):
     """Pick the largest texels-per-metre scale at which all charts fit."""
     lo, hi = 0.0, 4096.0
     best = None
+    metric = [c.size_m for c in charts]
     for _ in range(40):
         s = 0.5 * (lo + hi)
-        sizes = [(max(1, int(c.size_m[0] * s)), max(1, int(c.size_m[1] * s))) for c in charts]
+        sizes = [(max(1, int(mu * s)), max(1, int(mv * s))) for mu, mv in metric]
         pos = _shelf_pack(sizes, W, H, pad)
         if pos is None:
             hi = s
@@ -128,11 +130,42 @@ def pack_charts(charts, W, H, pad=2):
     return best
 
 
+def _quad_batch(lst, W, H):
+    """Vertices and indices of many 1x1 charts at once, in the per-chart
+    layout: vertices (a, b) = (0,0), (1,0), (0,1), (1,1), triangles
+    (v00, v10, v11), (v11, v01, v00)."""
+    n = len(lst)
+    O = np.array([c.origin for c, *_ in lst], np.float64)
+    DU = np.array([c.du for c, *_ in lst], np.float64)
+    DV = np.array([c.dv for c, *_ in lst], np.float64)
+    rect = np.array([(x0, y0, wp, hp) for _, x0, y0, wp, hp in lst], np.float64)
+    A = np.array([0.0, 1.0, 0.0, 1.0])
+    B = np.array([0.0, 0.0, 1.0, 1.0])
+    P = O[:, None, :] + A[None, :, None] * DU[:, None, :] + B[None, :, None] * DV[:, None, :]
+    nrm = np.cross(DU, DV)
+    nrm = nrm / np.sqrt((nrm * nrm).sum(-1, keepdims=True))
+    N = np.broadcast_to(nrm[:, None, :], P.shape)
+    T1 = np.broadcast_to(np.stack([A, B], -1)[None], (n, 4, 2))
+    T2 = np.stack([(rect[:, 0:1] + A[None] * rect[:, 2:3]) / W, (rect[:, 1:2] + B[None] * rect[:, 3:4]) / H], -1)
+    idx = (np.array([0, 1, 3, 3, 2, 0])[None, :] + 4 * np.arange(n)[:, None]).ravel()
+    verts = (P.reshape(-1, 3), N.reshape(-1, 3), T1.reshape(-1, 2), T2.reshape(-1, 2))
+    return verts, idx
+
+
 def charts_to_part_arrays(charts, W, H, pad=2):
     """Tessellate charts and assign UV2 atlas coordinates; returns per-part arrays."""
     pos_rect, sizes = pack_charts(charts, W, H, pad)
     by_part = {}
+    # parts made only of single flat quads (boxes) are tessellated in one batch
+    quad_parts = {}
+    for c in charts:
+        q = c.nu == 1 and c.nv == 1 and c.height is None
+        quad_parts[c.part] = quad_parts.get(c.part, True) and q
+    batch = {}
     for c, (x0, y0), (wp, hp) in zip(charts, pos_rect, sizes):
+        if quad_parts[c.part]:
+            batch.setdefault(c.part, []).append((c, x0, y0, wp, hp))
+            continue
         a = np.linspace(0.0, 1.0, c.nu + 1, dtype=np.float64)
         b = np.linspace(0.0, 1.0, c.nv + 1, dtype=np.float64)
         A, B = np.meshgrid(a, b)  # (nv+1, nu+1)
@@ -161,6 +194,8 @@ def charts_to_part_arrays(charts, W, H, pad=2):
         idx = np.stack([v00, v10, v11, v11, v01, v00], -1).ravel()
         lst = by_part.setdefault(c.part, [])
         lst.append((verts, idx))
+    for pid, lst in batch.items():
+        by_part[pid] = [_quad_batch(lst, W, H)]
     parts = {}
     for pid, lst in by_part.items():
         P, N, T1, T2, I = [], [], [], [], []
@@ -352,6 +387,67 @@ def terrain_scene(seed=SEED_BASE + 1, grid=512, extent=200.0, amplitude=20.0, n_
     if sun:
         d = np.array([0.35, -0.25, 0.9])
         lights.append(dir_light(d / np.linalg.norm(d), (1.0, 0.95, 0.9), sun_range, radius=0.5, samples=8))
+    cfg = dict(bounce_count=0, ao_distance=0.0, max_num_threads=1)
+    return SceneSpec(name, meshes, lights, cfg, lm_size=lm)
+
+
+def city_scene(seed=SEED_BASE + 4, extent=1000.0, grid=256, amplitude=6.0, n_boxes=155_000, tiles=16,
+               lm_total=4096, n_lights=64, name="C4", light_range=(20.0, 80.0)):
+    """Seeded city block at scale: a gentle heightfield over extent m (grid^2
+    quads) plus n_boxes boxes with lognormal(mu=1, sigma=0.6) m sides on a
+    jittered grid, merged into tiles x tiles meshes (one instance and one
+    (lm_total/tiles)^2 atlas each); n_lights point/spot lights above the
+    roofs.  With the defaults: 131,072 + 1,860,000 = 1,991,072 triangles."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    H = value_noise(g, grid) * amplitude
+    cell = extent / grid
+    q = grid // tiles
+    lm = lm_total // tiles
+    tile_charts = [[[] for _ in range(tiles)] for _ in range(tiles)]
+    for ty in range(tiles):
+        for tx in range(tiles):
+            o = np.array([tx * q * cell, ty * q * cell, 0.0])
+            h = H[ty * q: ty * q + q + 1, tx * q: tx * q + q + 1]
+            tile_charts[ty][tx].append(Chart(o, np.array([q * cell, 0, 0]), np.array([0, q * cell, 0]), q, q, 0,
+                                             height=h.copy()))
+
+    def height_at(x, y):
+        i, j = min(int(x / cell), grid - 1), min(int(y / cell), grid - 1)
+        return H[j, i]
+
+    side = int(np.ceil(np.sqrt(n_boxes)))
+    pitch = extent / side
+    sizes = np.minimum(g.lognormal(1.0, 0.6, (n_boxes, 3)), 0.9 * pitch * np.array([1, 1, 8]))
+    jitter = g.uniform(-0.5, 0.5, (n_boxes, 2))
+    tile_m = q * cell
+    for b in range(n_boxes):
+        gx, gy = b % side, b // side
+        size = sizes[b]
+        slack = np.maximum(pitch - size[:2], 0.0)
+        cx = (gx + 0.5) * pitch + jitter[b, 0] * slack[0]
+        cy = (gy + 0.5) * pitch + jitter[b, 1] * slack[1]
+        lo = np.array([cx - size[0] / 2, cy - size[1] / 2, height_at(cx, cy) - 0.3])
+        tx = min(int(cx / tile_m), tiles - 1)
+        ty = min(int(cy / tile_m), tiles - 1)
+        tile_charts[ty][tx] += box_charts(lo, lo + size, part=1, with_bottom=True)
+    meshes = []
+    for ty in range(tiles):
+        for tx in range(tiles):
+            parts = charts_to_part_arrays(tile_charts[ty][tx], lm, lm)
+            ident = f"block_{tx}_{ty}"
+            meshes.append(Mesh(ident, parts, [Instance(IDENTITY.copy(), 1.0, 1, ident)]))
+    lights = []
+    for i in range(n_lights):
+        x, y = g.uniform(10, extent - 10, 2)
+        z = height_at(x, y) + g.uniform(6.0, 20.0)
+        col = g.uniform(0.3, 1.0, 3)
+        rng_ = g.uniform(*light_range)
+        if i % 2 == 0:
+            lights.append(point_light((x, y, z), col, rng_, radius=0.5, samples=16))
+        else:
+            d = np.array([g.uniform(-0.5, 0.5), g.uniform(-0.5, 0.5), -1.0])
+            lights.append(spot_light((x, y, z), d / np.linalg.norm(d), col, rng_, 60.0, 40.0, 1.0,
+                                     radius=0.5, samples=16))
     cfg = dict(bounce_count=0, ao_distance=0.0, max_num_threads=1)
     return SceneSpec(name, meshes, lights, cfg, lm_size=lm)
 

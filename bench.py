@@ -567,7 +567,7 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
 
     ref64, t64 = run(0)
     forms = {}
-    for form, refine in [(0, 0), (2, 0), (3, 0), (3, 1), (4, 0), (4, 1)]:
+    for form, refine in [(0, 0), (1, 0), (2, 0), (3, 0), (3, 1), (4, 0), (4, 1)]:
         o, t = (ref64, t64) if form == 0 else run(form, refine)
         key = FORMS[form] + ("_refined" if refine else "")
         per = t[2] if not refine else (t[1] + t[3]) / bounces

@@ -47,7 +47,6 @@ private:
 	void release();
 	void ensure_work();
 	uint32_t ksplit(uint32_t rows_per_block, uint32_t min_blocks) const;
-	void set_scales(int form, int which, const std::vector<float>& colmax);
 	void apply_dense(int form, const float* xT, float* yT, bool split, bool pre, int which);
 	void apply_op(int form, const double* x, double* y, bool pre, int which);
 	DeviceScene& dev;
@@ -61,7 +60,6 @@ private:
 	double* E_ = nullptr;
 	float* diff_ = nullptr;
 	float* area_ = nullptr;
-	std::vector<float> emax_; // column maxima of the forward operand
 	// dense-product work buffers
 	float* part_ = nullptr;
 	float* xq_ = nullptr;
@@ -70,7 +68,7 @@ private:
 	float* yT_ = nullptr;
 	float* rm32a_ = nullptr;
 	float* rm32b_ = nullptr;
-	float* cs_[2] = {nullptr, nullptr}; // column scales: forward operand, adjoint operand
+	float* cs_[2] = {nullptr, nullptr}; // column scales of the 16-bit operand, set per product on the device
 	float* cu_[2] = {nullptr, nullptr}; // their unscales (times the matrix scale)
 };
 

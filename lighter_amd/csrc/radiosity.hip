@@ -208,6 +208,42 @@ __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, c
 		xlo[t] = Prec<P>::cvt(v - from16<P>(h));
 }
 
+// Per-column power-of-two scale of the 16-bit operand, from the operand
+// itself (one block per column): FP16 (p16) puts each column's largest
+// entry near 2^13, so neither growth of the iterates nor small values leave
+// its range; BF16 has FP32's exponent range and keeps scale 1.  unscale =
+// 1 / (scale * matrix scale) for the product's reduction.
+__global__ __launch_bounds__(kB) void k_col_scale(const float* __restrict__ xT, const float* __restrict__ diffuse,
+                                                  const float* __restrict__ area, uint32_t N, uint32_t Np, int pre,
+                                                  int p16, float gscale, float* __restrict__ cscale,
+                                                  float* __restrict__ cunscale)
+{
+	const uint32_t c = blockIdx.x;
+	float m = 0.0f;
+	if (p16) {
+		for (uint32_t j = threadIdx.x; j < N; j += kB) {
+			float v = xT[(uint64_t)c * Np + j];
+			if (pre)
+				v = (v * diffuse[3 * j + c % 3]) * area[j];
+			m = fmaxf(m, fabsf(v));
+		}
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1)
+			m = fmaxf(m, __shfl_xor(m, o));
+	}
+	__shared__ float red[kB / 64];
+	if ((threadIdx.x & 63) == 0)
+		red[threadIdx.x >> 6] = m;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int q = 1; q < kB / 64; ++q)
+			m = fmaxf(m, red[q]);
+		float e = (m > 0.0f && m < INFINITY) ? ldexpf(1.0f, 14 - ilogbf(m) - 1) : 1.0f;
+		cscale[c] = e;
+		cunscale[c] = 1.0f / (e * gscale);
+	}
+}
+
 __global__ __launch_bounds__(kB) void k_prep_x32(const float* __restrict__ xT, const float* __restrict__ diffuse,
                                                  const float* __restrict__ area, uint32_t N, uint32_t Np, uint32_t R,
                                                  int pre, float* __restrict__ xq)
@@ -463,7 +499,7 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 
 // the two instantiations: plain (64 rows per wave, 4 stages in flight) and
 // error-compensated hi + lo (32 rows per wave, 3 stages)
-constexpr int kPlainRT = 4, kPlainPD = 5, kSplitRT = 2, kSplitPD = 3;
+constexpr int kPlainRT = 4, kPlainPD = 4, kSplitRT = 2, kSplitPD = 3;
 
 // ---------------------------------------------------------------------------
 // plain FP32 GEMM with the same blocking: 256 threads, each 8 rows x 12
@@ -729,11 +765,6 @@ void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3,
 	HIPCHECK(hipMemcpyAsync(diff_, diffuse3, 12 * (size_t)N, hipMemcpyHostToDevice, s));
 	HIPCHECK(hipMemsetAsync(area_, 0, 4 * (size_t)Np, s));
 	HIPCHECK(hipMemcpyAsync(area_, area, 4 * (size_t)N, hipMemcpyHostToDevice, s));
-	// column maxima of the forward operand (x * diffuse) * area, for the FP16 scales
-	emax_.assign(kRPad, 0.0f);
-	for (uint32_t j = 0; j < N; ++j)
-		for (uint32_t c = 0; c < R; ++c)
-			emax_[c] = fmaxf(emax_[c], fabsf((float)E[(size_t)j * R + c] * diffuse3[3 * j + c % 3] * area[j]));
 	HIPCHECK(hipStreamSynchronize(s));
 }
 
@@ -810,22 +841,6 @@ void RadSolver::ensure_work()
 	}
 }
 
-// power-of-two column scales of the 16-bit operand (FP16 needs the column
-// maxima of the operand; BF16 has FP32's range) and the matching unscale
-void RadSolver::set_scales(int form, int which, const std::vector<float>& colmax)
-{
-	std::vector<float> sc(kRPad, 1.0f), us(kRPad, 1.0f);
-	for (uint32_t c = 0; c < kRPad; ++c) {
-		float e = 1.0f;
-		if (form == kDenseF16 && c < colmax.size() && colmax[c] > 0)
-			e = ldexpf(1.0f, 14 - ilogbf(colmax[c]) - 1);
-		sc[c] = e;
-		us[c] = 1.0f / (e * gscale_);
-	}
-	HIPCHECK(hipMemcpy(cs_[which], sc.data(), 4 * kRPad, hipMemcpyHostToDevice));
-	HIPCHECK(hipMemcpy(cu_[which], us.data(), 4 * kRPad, hipMemcpyHostToDevice));
-}
-
 // y (column-major, R x Np, FP32) = K x (pre) or H x for the dense forms.
 // split: the error-compensated product of the 16-bit forms (hi + lo planes).
 // which: the column-scale set (0: forward operand, 1: adjoint operand).
@@ -842,6 +857,8 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 	} else {
 		uint16_t* q = reinterpret_cast<uint16_t*>(xq_);
 		uint16_t* lo = split ? xqlo_ : nullptr;
+		k_col_scale<<<R, kB, 0, s>>>(xT, diff_, area_, N, Np, pre ? 1 : 0, form == kDenseF16 ? 1 : 0, gscale_,
+		                             cs_[which], cu_[which]);
 		if (form == kDenseF16)
 			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cs_[which], N, Np, R, pre ? 1 : 0, q, lo);
 		else
@@ -917,7 +934,6 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		if (built_form_ != form)
 			build_dense(form);
 		times.build_ms = build_ms;
-		set_scales(form, 0, emax_);
 		// z_k (column-major FP32), k = 0..b
 		std::vector<float*> z(bounces + 1);
 		for (auto& p : z)
@@ -994,14 +1010,6 @@ void RadSolver::solve_grad(int form, int bounces, const double* g, double* total
 	ensure_work();
 	if (form >= kDenseF32 && built_form_ != form)
 		build_dense(form);
-	if (form >= kDenseF32) {
-		set_scales(form, 0, emax_);
-		std::vector<float> gmax(kRPad, 0.0f);
-		for (uint32_t j = 0; j < N; ++j)
-			for (uint32_t c = 0; c < R; ++c)
-				gmax[c] = fmaxf(gmax[c], fabsf((float)g[(size_t)j * R + c]));
-		set_scales(form, 1, gmax);
-	}
 	const int B = std::max(0, bounces);
 	std::vector<double*> S(std::max(1, B));
 	for (auto& p : S)

@@ -40,6 +40,7 @@ u32p = C.POINTER(C.c_uint32)
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 64
 DIST_TRI_BYTES = 160
+TRI_BOX_BYTES = 48  # the vertex quads a distance query's box pre-test reads
 METRIC = "shadow rays/s + s per radiosity bounce @ stated error; GPU plain vs MFMA/adjoint"
 
 
@@ -480,9 +481,9 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
                                         light_off.ctypes.data_as(u32p), 1), lib)
     lib.lgx_SetCounting(dev, 1)
     _lib.check(lib.lgx_DirectLightRun(dev), lib)
-    cnt = (C.c_uint64 * 5)()
+    cnt = (C.c_uint64 * 6)()
     lib.lgx_ReadCounters(dev, cnt)
-    nodes, ntris, qsteps, queries, marches = list(cnt)
+    nodes, ntris, qsteps, queries, marches, npre = list(cnt)
     lib.lgx_SetCounting(dev, 0)
     tm = (C.c_double * 6)()
     for _ in range(warmup):
@@ -496,7 +497,7 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
     ms_step = (time.perf_counter() - t0) * 1000.0 / steps
     rays = marches / (ms_step / 1000.0)
     march_avg = float(np.mean(march_ms))
-    alg = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + len(items) * 36
+    alg = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + npre * TRI_BOX_BYTES + len(items) * 36
     log(f"[bench] C4 direct: BVH {info[1]} nodes depth {info[2]}; {marches} marches, {ms_step:.1f} ms/step "
         f"(march {march_avg:.1f} ms): {rays / 1e6:.1f} M rays/s")
     w = np.zeros(len(items), np.float32)
@@ -670,12 +671,12 @@ def main():
     # instrumented pass: traversal counters for the algorithmic byte count
     lib.lgx_SetCounting(dev, 1)
     _lib.check(lib.lgx_DirectLightRun(dev), lib)
-    cnt = (C.c_uint64 * 5)()
+    cnt = (C.c_uint64 * 6)()
     lib.lgx_ReadCounters(dev, cnt)
-    nodes, ntris, steps, queries, marches = list(cnt)
+    nodes, ntris, steps, queries, marches, npre = list(cnt)
     lib.lgx_SetCounting(dev, 0)
     log(f"[bench] BVH {info[0]} tris, {info[1]} nodes, depth {info[2]}; marches {marches}, steps {steps}, "
-        f"node visits {nodes}, triangle tests {ntris}")
+        f"node visits {nodes}, triangle tests {ntris} (+{npre} culled by the box pre-test)")
 
     tm = (C.c_double * 6)()
     for _ in range(args.warmup):
@@ -694,14 +695,16 @@ def main():
 
     # algorithmic bytes of one march-kernel launch
     io_bytes = len(items) * (4 + 24 + 4 + 4)  # item, sample pos+nrm, weight, shadow factor
-    alg_bytes = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + io_bytes
+    alg_bytes = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + npre * TRI_BOX_BYTES + io_bytes
     achieved = alg_bytes / (march_avg / 1000.0) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=measured_traffic("k_march", dict(triangles=int(info[0]), work_items=int(len(items)),
-                                                             marches=int(marches))),
+                                                             marches=int(marches), node_visits=int(nodes),
+                                                             triangle_tests=int(ntris))),
                     kernel="k_march", alg_bytes_per_launch=int(alg_bytes),
-                    note="node visits x 64 B + distance-triangle tests x 160 B + per-item I/O, counted by an "
+                    note="node visits x 64 B + full distance-triangle tests x 160 B + box pre-tests x 48 B + "
+                         "per-item I/O, counted by an "
                          "instrumented pass; the working set (BVH + triangles) is cache resident")
 
     w = np.zeros(len(items), np.float32)

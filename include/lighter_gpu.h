@@ -149,10 +149,11 @@ LTRAPI int lgx_RadSolveGrad(lgx_Device* dev, const double* E, uint32_t R, const 
  * (3 FP64 per device sample) dcolor3[3 l + c] = sum over light l's items of
  * weight * lam3[3 s + c]. */
 LTRAPI int lgx_DirectLightGrad(lgx_Device* dev, const double* lam3, double* dcolor3);
-/* Traversal counters (node visits, triangle tests, march steps, queries,
- * marches) collected by the next stage when enabled. */
+/* Traversal counters (node visits, full triangle tests, march steps,
+ * queries, marches, triangles culled by the distance query's box pre-test)
+ * collected by the next stage when enabled. */
 LTRAPI int lgx_SetCounting(lgx_Device* dev, int on);
-LTRAPI int lgx_ReadCounters(lgx_Device* dev, uint64_t out[5]);
+LTRAPI int lgx_ReadCounters(lgx_Device* dev, uint64_t out[6]);
 /* Last stage times in ms: direct, direct march, AO, links, bounces,
  * direct-light gradient. */
 LTRAPI int lgx_DeviceTimes(lgx_Device* dev, double out[6]);

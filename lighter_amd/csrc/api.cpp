@@ -687,13 +687,14 @@ int lgx_SetCounting(lgx_Device* d, int on)
 	return 1;
 }
 
-int lgx_ReadCounters(lgx_Device* d, uint64_t out[5])
+int lgx_ReadCounters(lgx_Device* d, uint64_t out[6])
 {
 	out[0] = d->dev.counters.nodes;
 	out[1] = d->dev.counters.tris;
 	out[2] = d->dev.counters.steps;
 	out[3] = d->dev.counters.queries;
 	out[4] = d->dev.counters.marches;
+	out[5] = d->dev.counters.pretests;
 	return 1;
 }
 

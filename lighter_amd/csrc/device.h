@@ -33,6 +33,7 @@ struct StageTimes {
 // tests, for the algorithmic-bytes figure.
 struct QueryCounters {
 	unsigned long long nodes = 0, tris = 0, steps = 0, queries = 0, marches = 0;
+	unsigned long long pretests = 0; // triangles the box pre-test culled (distance queries)
 };
 
 class DeviceScene {

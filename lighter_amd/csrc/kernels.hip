@@ -36,7 +36,7 @@ namespace lgx {
 
 constexpr int kStackSize = 64;
 constexpr int kBlock = 256;
-constexpr int kNumCounters = 5;
+constexpr int kNumCounters = 6;
 constexpr uint32_t kSkipBits = 0xFFFFFFFFu; // weight of a texel the reference skips
 constexpr uint32_t kSampleBits = 26;
 constexpr uint32_t kSampleMask = (1u << kSampleBits) - 1;
@@ -107,6 +107,27 @@ __device__ __forceinline__ RayTri load_rtri(const RayTri* t, uint32_t i)
 	for (int k = 0; k < 4; ++k)
 		o[k] = q[k];
 	return r;
+}
+
+// True when the triangle cannot lower the distance bound `best`: the box of
+// its three vertices (DistTri quads 4-6) lies farther than best + margin.
+// The distance the reference's formula returns is the distance to a point of
+// the triangle (a vertex, a point of an edge line inside the edge's span, or
+// the projection into the face), up to rounding in its region tests of a few
+// ulps of the coordinates, so it is never below the box distance minus that;
+// the margin (1e-5 of the coordinates' magnitude, hundreds of ulps) covers
+// it, and a skipped triangle could only have returned a value >= best, which
+// leaves the minimum unchanged.
+__device__ __forceinline__ bool tri_box_beyond(f3 p, const DistTri* t, float best)
+{
+	const float4* q = reinterpret_cast<const float4*>(t);
+	const float4 qa = q[4], qb = q[5], qc = q[6];
+	// vertices (qa.w, qb.x, qb.y), (qb.z, qb.w, qc.x), (qc.y, qc.z, qc.w)
+	float lx = fminf(fminf(qa.w, qb.z), qc.y), hx = fmaxf(fmaxf(qa.w, qb.z), qc.y);
+	float ly = fminf(fminf(qb.x, qb.w), qc.z), hy = fmaxf(fmaxf(qb.x, qb.w), qc.z);
+	float lz = fminf(fminf(qb.y, qc.x), qc.w), hz = fmaxf(fmaxf(qb.y, qc.x), qc.w);
+	float lim = best + 1e-5f * (1.0f + fabsf(p.x) + fabsf(p.y) + fabsf(p.z));
+	return box_d2(p, lx, ly, lz, hx, hy, hz) > lim * lim;
 }
 
 // Traversal stacks.  The traversals below are nearest-first: at an inner
@@ -189,7 +210,7 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 		return best;
 	int sp = 0;
 	int32_t ref = 0;
-	unsigned nn = 0, nt = 0;
+	unsigned nn = 0, nt = 0, ncull = 0;
 	for (;;) {
 		if (ref >= 0) {
 			Node4 n = load_node(V.nodes, ref);
@@ -213,12 +234,16 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 		} else {
 			uint32_t f = leaf_first(ref), c = leaf_count(ref);
 			for (uint32_t i = 0; i < c; ++i) {
+				if (tri_box_beyond(p, V.dtri + f + i, best)) {
+					ncull++;
+					continue;
+				}
 				DistTri T = load_dtri(V.dtri, f + i);
 				float d = dist_tri_pre(p, T);
+				nt++;
 				if (d < best)
 					best = d;
 			}
-			nt += c;
 		}
 		// pop the next entry still within reach
 		bool found = false;
@@ -236,6 +261,7 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 	if (COUNT) {
 		atomicAdd(cnt + 0, (unsigned long long)nn);
 		atomicAdd(cnt + 1, (unsigned long long)nt);
+		atomicAdd(cnt + 5, (unsigned long long)ncull);
 		atomicAdd(cnt + 3, 1ull);
 	}
 	return best;
@@ -1226,6 +1252,7 @@ void DeviceScene::read_counters()
 	counters.steps = c[2];
 	counters.queries = c[3];
 	counters.marches = c[4];
+	counters.pretests = c[5];
 }
 
 void DeviceScene::direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,

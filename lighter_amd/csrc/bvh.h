@@ -31,7 +31,7 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "node size");
 
-constexpr int kLeafMax = 4;
+constexpr int kLeafMax = 2; // measured on the terrain workload: 2 beats 1, 4 and 8
 
 LHD int32_t leaf_ref(uint32_t first, uint32_t count) { return ~(int32_t)((first << 4) | (count - 1)); }
 LHD uint32_t leaf_first(int32_t ref) { return ((uint32_t)~ref) >> 4; }

@@ -53,6 +53,11 @@ constexpr int kRPad = 16 * kColTiles;
 
 static inline unsigned gridn(uint64_t n, unsigned b = kB) { return (unsigned)((n + b - 1) / b); }
 
+// 16-byte piece permutation inside a 16-row x 64-byte group of the 16-bit
+// GEMM operands' LDS images: the rows one ds_read_b128 serves together land
+// in distinct banks
+__device__ __forceinline__ int piece_swz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+
 template <class T> static T* dalloc(size_t n)
 {
 	void* p = nullptr;
@@ -153,15 +158,21 @@ __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__
 			// remainder plane: what the 16-bit value left out
 			pl[q] = (uint32_t)Prec<PP>::cvt(a - from16<PP>(ha)) | ((uint32_t)Prec<PP>::cvt(b - from16<PP>(hb)) << 16);
 		}
-		uint4* o = reinterpret_cast<uint4*>((uint16_t*)G + (uint64_t)i * Np + (uint64_t)w * 32);
+		// LDS-image layout: word w is k stage w; its four 8-element pieces go to
+		// the row's 64 bytes of 1 KB group (i & 255) >> 4, piece q at slot
+		// q ^ piece_swz(i & 15)
+		const uint64_t nks = Np / 32;
+		const uint64_t grp = ((uint64_t)(i >> 8) * nks + w) * 16 + ((i & 255) >> 4);
+		const uint32_t r = i & 15;
+		uint4* o = reinterpret_cast<uint4*>((uint16_t*)G + grp * 512 + r * 32);
 #pragma unroll
 		for (int q = 0; q < 4; ++q)
-			o[q] = make_uint4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
+			o[q ^ piece_swz(r)] = make_uint4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
 		if (Glo) {
-			uint4* ol = reinterpret_cast<uint4*>(Glo + (uint64_t)i * Np + (uint64_t)w * 32);
+			uint4* ol = reinterpret_cast<uint4*>(Glo + grp * 512 + r * 32);
 #pragma unroll
 			for (int q = 0; q < 4; ++q)
-				ol[q] = make_uint4(pl[4 * q], pl[4 * q + 1], pl[4 * q + 2], pl[4 * q + 3]);
+				ol[q ^ piece_swz(r)] = make_uint4(pl[4 * q], pl[4 * q + 1], pl[4 * q + 2], pl[4 * q + 3]);
 		}
 	}
 }
@@ -203,9 +214,14 @@ __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, c
 		v *= cscale[c];
 	}
 	uint16_t h = Prec<P>::cvt(v);
-	xq[t] = h;
+	// LDS-image layout: stage j / 32, column group c / 16, row c % 16, piece
+	// (j % 32) / 8 at slot piece ^ piece_swz(c % 16), element j % 8
+	const uint32_t kk = j & 31, r = c & 15;
+	const uint64_t o = (((uint64_t)(j >> 5) * (kRPad / 16) + (c >> 4)) * 64 + r * 4 + ((kk >> 3) ^ piece_swz(r))) * 8 +
+	                   (kk & 7);
+	xq[o] = h;
 	if (xlo)
-		xlo[t] = Prec<P>::cvt(v - from16<P>(h));
+		xlo[o] = Prec<P>::cvt(v - from16<P>(h));
 }
 
 // Per-column power-of-two scale of the 16-bit operand, from the operand
@@ -297,7 +313,6 @@ __device__ __forceinline__ f32x4 mfma16(i16x8 a, i16x8 b, f32x4 c)
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-__device__ __forceinline__ int piece_swz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
 
 // wait until at most N vector-memory operations of this wave are outstanding
 template <int N>
@@ -332,10 +347,18 @@ struct GemmShape {
 };
 
 // issue stage st (k offset kofs from the split's start) into LDS buffer base
+// Issue stage st (counted from the split's first stage) into LDS buffer
+// base.  The 16-bit matrix and operand are stored as LDS images (see
+// k_dense_build / k_prep_x16): every DMA instruction copies one contiguous
+// KB, and a block's stage is one contiguous tile.
+constexpr uint64_t kTileElems = 512;             // 16-bit elements per 1 KB group
+constexpr uint64_t kGStageElems = 16 * kTileElems; // a 256-row stage of the matrix
+constexpr uint64_t kXStageElems = (kRPad / 16) * kTileElems; // a stage of the operand
+
 template <int NPL, int RT>
 __device__ __forceinline__ void gemm_issue(const uint16_t* G, const uint16_t* Glo, const uint16_t* xq,
-                                           const uint16_t* xlo, uint64_t xsrc, uint64_t gsrc, uint64_t Np,
-                                           uint32_t kofs, unsigned char* base, int wave)
+                                           const uint16_t* xlo, uint64_t xsrc, uint64_t gsrc, uint64_t gstride,
+                                           uint32_t st, unsigned char* base, int wave)
 {
 	typedef GemmShape<NPL, RT> S;
 #pragma unroll
@@ -346,16 +369,17 @@ __device__ __forceinline__ void gemm_issue(const uint16_t* G, const uint16_t* Gl
 		unsigned char* gb = base + NPL * S::kXBytes + pl * S::kGBytes + wave * RT * 1024;
 #pragma unroll
 		for (int j = 0; j < kColTiles / 4; ++j)
-			dma16(X + xsrc + (uint64_t)j * 16 * Np + kofs, xb + j * 1024);
+			dma16(X + xsrc + (uint64_t)st * kXStageElems + j * kTileElems, xb + j * 1024);
 #pragma unroll
 		for (int t = 0; t < RT; ++t)
-			dma16(M + gsrc + (uint64_t)t * 16 * Np + kofs, gb + t * 1024);
+			dma16(M + gsrc + (uint64_t)st * gstride + t * kTileElems, gb + t * 1024);
 	}
 }
 
 template <int NPL, int RT>
 struct GemmLane {
-	uint64_t xsrc, gsrc; // DMA source offsets (elements) at the split's first k
+	uint64_t xsrc, gsrc; // DMA source offsets (elements) of this lane at the split's first stage
+	uint64_t gstride;    // matrix elements from one stage to the next
 	int wave, rofs;      // wave index; ds_read_b128 offset of the fragment in a 1 KB group
 	unsigned char* buf[kMaxPD]; // the LDS stage ring
 };
@@ -391,7 +415,7 @@ __device__ __forceinline__ void gemm_step(int s, int nst, const GemmLane<NPL, RT
 	// above exact
 	constexpr int BN = (B + PD - 1) % PD;
 	const int ahead = s + PD - 1 < nst ? s + PD - 1 : nst - 1;
-	gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, ahead * kKS, L.buf[BN], L.wave);
+	gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, L.gstride, ahead, L.buf[BN], L.wave);
 #pragma unroll
 	for (int c = 0; c < kColTiles; ++c) {
 #pragma unroll
@@ -423,7 +447,7 @@ __device__ __forceinline__ void gemm_prologue(int nst, const GemmLane<NPL, RT>& 
                                               uint64_t Np, std::integer_sequence<int, Bs...>)
 {
 	// stage Bs into buffer Bs (the last stage again where there are fewer)
-	(gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, Np, (Bs < nst ? Bs : nst - 1) * kKS, L.buf[Bs], L.wave),
+	(gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, L.gstride, Bs < nst ? Bs : nst - 1, L.buf[Bs], L.wave),
 	 ...);
 }
 
@@ -462,11 +486,16 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 	const uint32_t i0 = blockIdx.x * S::kRows + L.wave * 16 * RT; // this wave's first row
 	const uint32_t klen = Np / ksplit, kbeg = blockIdx.y * klen;
 	const int nst = (int)(klen / kKS);
-	// DMA sources: lane l fills piece (l & 3) of row l >> 2 of a 16-row group,
-	// i.e. k piece (l & 3) ^ piece_swz(l >> 2)
-	const int drow = lane >> 2, dk = 8 * ((lane & 3) ^ piece_swz(lane >> 2));
-	L.xsrc = (uint64_t)(16 * (kColTiles / 4) * L.wave + drow) * Np + kbeg + dk;
-	L.gsrc = (uint64_t)(i0 + drow) * Np + kbeg + dk;
+	// DMA sources in the LDS-image layouts: lane l copies the l-th 16 bytes
+	// of each 1 KB group.  Matrix tile (256-row block rb, stage ks, group g)
+	// at ((rb * nks + ks) * 16 + g) KB; operand tile (stage ks, column
+	// group cg) at (ks * 12 + cg) KB.
+	const uint64_t nks = Np / kKS, ks0 = kbeg / kKS;
+	const uint32_t row0 = blockIdx.x * S::kRows; // the block's first row
+	const uint64_t rb = row0 >> 8, gbase = (row0 & 255) >> 4;
+	L.gstride = kGStageElems;
+	L.gsrc = ((rb * nks + ks0) * 16 + gbase + (uint64_t)L.wave * RT) * kTileElems + 8 * lane;
+	L.xsrc = (ks0 * (kRPad / 16) + (uint64_t)(kColTiles / 4) * L.wave) * kTileElems + 8 * lane;
 	L.rofs = r * 64 + 16 * (g ^ piece_swz(r));
 
 	f32x4 acc[RT][kColTiles];

@@ -207,7 +207,43 @@ def room_patches(lib, nth, n_patches, lm0, **room_kw):
     return spec, b, pos[:N].copy(), nrm[:N].copy(), rad[:N].copy(), tris, N, lm
 
 
-def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
+def ao_bench(lib, dev, spec, S, nth, with_cpu):
+    """C2 ambient occlusion: 64 cosine-weighted rays of 2 m per texel on the
+    device's S texels (U2 = one ray), beside the reference's AO stage on all
+    texels of the same room on the host threads."""
+    rng = np.random.default_rng(0x4C540012)
+    ro = rng.uniform(0, 1, S).astype(np.float32)
+    col = np.zeros(3, np.float32)
+    tm = (C.c_double * 6)()
+    ms = []
+    for _ in range(3):
+        _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
+        lib.lgx_DeviceTimes(dev, tm)
+        ms.append(tm[2])
+    best = float(np.median(ms))
+    out = dict(texels=S, rays=S * 64, ms=round(best, 3), rays_per_s=round(S * 64 / (best / 1e3), 1))
+    if with_cpu:
+        from oracle.checkers import Reference, reference_available
+        if reference_available():
+            ref = Reference()
+            br = scenes.BuiltScene(ref.lib, spec, dict(max_num_threads=nth))
+            ref.lib.rp_prepare(br.scene, nth)
+            n_ref = sum(i[2] for i in ref.instances(br.scene))
+            t0 = time.perf_counter()
+            ref.lib.rp_ao(br.scene)
+            dt = time.perf_counter() - t0
+            br.destroy()
+            out["cpu_baseline"] = dict(value=round(n_ref * 64 / dt, 1), unit="AO rays/s", cores=nth,
+                                       kind="reference",
+                                       sample=f"the reference's AO stage over all {n_ref} texels of the room, "
+                                              f"64 rays of 2 m each, {dt:.2f} s")
+            out["speedup_vs_cpu"] = round(out["rays_per_s"] / out["cpu_baseline"]["value"], 1)
+    log(f"[bench] C2 AO: {S * 64} rays in {best:.2f} ms ({out['rays_per_s'] / 1e9:.2f} G rays/s)"
+        + (f"; CPU {out['cpu_baseline']['value'] / 1e6:.2f} M rays/s" if "cpu_baseline" in out else ""))
+    return out
+
+
+def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with_cpu=True):
     """C2: radiosity over the room's texels as patches, R right-hand sides
     (setups x RGB) built from the lights' direct contributions."""
     from oracle.checkers import normalize_f32
@@ -226,6 +262,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
     lib.lgx_ReadWeights(dev, P(w), None, len(items))
     w[w.view(np.uint32) == 0xFFFFFFFF] = 0
     W = w.reshape(L, N).T.astype(np.float64)
+    ao = ao_bench(lib, dev, spec, N, nth, with_cpu)
     rng = np.random.default_rng(0x4C540002)
     cols = rng.uniform(0.5, 1.0, (setups, L, 3))
     E = np.zeros((N, 3 * setups))
@@ -300,7 +337,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64):
                mfma_roofline=dict(bound="hbm", achieved=round(gemm_bytes / mf / 1e9, 1), peak=HBM_PEAK_GBS,
                                   unit="GB/s", frac=round(gemm_bytes / mf / 1e9 / HBM_PEAK_GBS, 4),
                                   note="BF16 matrix once per bounce + right-hand sides in and out"),
-               cpu_port_s_per_bounce_rgb=cpu_bounce_s,
+               ao=ao, cpu_port_s_per_bounce_rgb=cpu_bounce_s,
                cpu_note="the reference's Int_RDBounce loop restated in C (oracle), one RGB setup, one host thread, "
                         "on the same link list; the reference itself runs it single-threaded")
     return out
@@ -725,7 +762,7 @@ def main():
     built.destroy()
     radio = None
     if not args.no_radiosity:
-        radio = radiosity_bench(lib, nth, n_patches=args.radiosity_patches)
+        radio = radiosity_bench(lib, nth, n_patches=args.radiosity_patches, with_cpu=not args.no_cpu_baseline)
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_bench(lib, nth)

@@ -690,6 +690,10 @@ struct AOParams {
 	float color[3];
 };
 
+// One wave per texel, one lane per ray (rays beyond 64 in further rounds of
+// the wave): the rays of a texel start together and stay coherent, and a
+// small scene still fills the GPU.  The rays' terms are then added in ray
+// order, the reference's summation order, so the sum is the same.
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_ao(View V, const float* __restrict__ pos, const float* __restrict__ nrm,
                                                const float* __restrict__ randoff, const float2* __restrict__ side,
@@ -698,23 +702,34 @@ __global__ __launch_bounds__(kBlock) void k_ao(View V, const float* __restrict__
 {
 	extern __shared__ __align__(16) unsigned char lds_stack[];
 	LdsStack st = lds_stack_of_lane(lds_stack, depth);
-	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const uint32_t s = blockIdx.x * (kBlock / 64) + wave;
 	if (s >= S)
-		return;
+		return; // whole waves: no barrier below
 	f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
 	f3 origin = SP + SN * (kSmall * 2);
 	float ro = randoff[s];
 	float ao = 0;
-	for (int i = 0; i < P.n; ++i) {
-		float angle = (i + ro) * kGoldenAngle;
-		double sa, ca;
-		sincos((double)angle, &sa, &ca);
-		float2 sd = side[i];
-		f3 d = spiral_dir(SN, sd.x, sd.y, (float)ca, (float)sa) * P.distance;
-		float hit = q_segment<false, COUNT>(V, origin, origin + d, cnt, st);
-		if (hit < 1.0f)
-			ao += 1.0f - hit;
+	for (int i0 = 0; i0 < P.n; i0 += 64) {
+		const int i = i0 + lane;
+		float term = 0.0f;
+		if (i < P.n) {
+			float angle = (i + ro) * kGoldenAngle;
+			double sa, ca;
+			sincos((double)angle, &sa, &ca);
+			float2 sd = side[i];
+			f3 d = spiral_dir(SN, sd.x, sd.y, (float)ca, (float)sa) * P.distance;
+			float hit = q_segment<false, COUNT>(V, origin, origin + d, cnt, st);
+			if (hit < 1.0f)
+				term = 1.0f - hit; // a miss adds nothing (ao + 0 = ao)
+		}
+		// every lane adds the 64 terms in ray order (the same sum on every lane)
+		const int m = P.n - i0 < 64 ? P.n - i0 : 64;
+		for (int k = 0; k < m; ++k)
+			ao += __shfl(term, k);
 	}
+	if (lane != 0)
+		return;
 	ao /= P.n;
 	ao = tmin(ao * P.multiplier, 1.0f);
 	if (P.falloff)
@@ -1340,13 +1355,14 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 	P.color[0] = color[0];
 	P.color[1] = color[1];
 	P.color[2] = color[2];
+	const unsigned ao_grid = (unsigned)((nsamples_ + kBlock / 64 - 1) / (kBlock / 64)); // a wave per texel
 	EventTimer tm(s);
 	if (count_queries)
-		k_ao<true><<<grid_for(nsamples_), kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
+		k_ao<true><<<ao_grid, kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
 		                                                                  (uint32_t)nsamples_, d_lm_, d_counters_,
 		                                                                  march_stack_depth());
 	else
-		k_ao<false><<<grid_for(nsamples_), kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
+		k_ao<false><<<ao_grid, kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
 		                                                                   (uint32_t)nsamples_, d_lm_, d_counters_,
 		                                                                   march_stack_depth());
 	HIPCHECK(hipGetLastError());

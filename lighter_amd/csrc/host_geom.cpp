@@ -318,6 +318,19 @@ void SplitTree::set(const Tri* tris, size_t count)
 	nodes_[0].tris.assign(tris, tris + count);
 	if (count)
 		split(0, 0);
+	// a triangle whose normal is near zero never reports a hit, so the leaf
+	// lists keep only the others, in order, with their operands precomputed
+	for (Node& n : nodes_) {
+		if (n.front != -1)
+			continue;
+		for (size_t i = 0; i < n.tris.size(); ++i) {
+			const Tri& t = n.tris[i];
+			if (!tri_useful(t))
+				continue;
+			n.rt.push_back(make_ray_tri(t.p1, t.p2, t.p3));
+			n.rti.push_back((int32_t)i);
+		}
+	}
 }
 
 namespace {
@@ -452,11 +465,12 @@ float SplitTree::node_intersect(int32_t node, f3 from, f3 to, f3* outnormal) con
 	}
 	float closest = 2.0f;
 	const Tri* ct = nullptr;
-	for (const Tri& t : n.tris) {
-		float hit = seg_tri(from, to, t.p1, t.p2, t.p3);
+	const f3 dir = to - from;
+	for (size_t k = 0; k < n.rt.size(); ++k) {
+		float hit = seg_tri_pre(from, dir, n.rt[k]); // = seg_tri(from, to, ...)
 		if (hit < closest) {
 			closest = hit;
-			ct = &t;
+			ct = &n.tris[n.rti[k]];
 		}
 	}
 	if (outnormal && ct)

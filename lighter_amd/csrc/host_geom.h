@@ -186,6 +186,10 @@ private:
 		float D;
 		int32_t front = -1, back = -1;
 		std::vector<Tri> tris;
+		// leaves: the segment-test operands of the triangles with a usable
+		// normal, in list order, and their positions in tris
+		std::vector<RayTri> rt;
+		std::vector<int32_t> rti;
 	};
 	void split(int32_t node, int depth);
 	bool pick_plane(Node& n);

@@ -191,6 +191,11 @@ static void gen_samples(ltr_Scene* S, Instance* mi)
 		return;
 	}
 	size_t n = (size_t)mi->lm_w * mi->lm_h;
+	std::vector<Box> roots(S->insts.size());
+	for (size_t k = 0; k < S->insts.size(); ++k) {
+		const BoxTree::Node& r = S->insts[k]->tris.tree().root();
+		roots[k] = Box{r.mn, r.mx};
+	}
 	std::vector<f3> R1(n, mk3(0, 0, 0)), R2(n, mk3(0, 0, 0));
 	std::vector<f4> R3(n, f4{0, 0, 0, 0});
 	rasterize_instance(mi, 0.5f + kSmall, R1.data(), R2.data(), R3.data());
@@ -211,9 +216,16 @@ static void gen_samples(ltr_Scene* S, Instance* mi)
 		N = normalized(N);
 		f3 P = R1[i];
 		f4 XD = R3[i];
-		// move the sample out of concave edges
-		for (Instance* o : S->insts)
-			o->tris.offset_sample(P, N, sqrtf(XD.w));
+		// move the sample out of concave edges; an instance whose tree root is
+		// disjoint from the query cube contributes nothing (its query stops at
+		// the root), so that test is made here first, on the current P
+		const float dist = sqrtf(XD.w);
+		for (size_t k = 0; k < S->insts.size(); ++k) {
+			const f3 d3 = mk3(dist, dist, dist);
+			if (box_disjoint(P - d3, P + d3, roots[k].mn, roots[k].mx))
+				continue;
+			S->insts[k]->tris.offset_sample(P, N, dist);
+		}
 		// move the sample in front of overlapping faces
 		if (cfg.max_correct_dist) {
 			int left = 100;

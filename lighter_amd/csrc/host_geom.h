@@ -71,6 +71,10 @@ struct RaySeg {
 	float len;
 	f3 inv_dir;
 	f3 end;
+	// the segment's bounding box, padded by 1e-3 (1 + largest |coordinate|):
+	// a triangle the segment test accepts holds a point within rounding of the
+	// segment, so a box disjoint from this one holds no such triangle
+	f3 qmn, qmx;
 	void set(f3 r0, f3 r1)
 	{
 		origin = r0;
@@ -78,6 +82,12 @@ struct RaySeg {
 		len = length(r1 - r0);
 		f3 d = normalized(r1 - r0);
 		inv_dir = mk3(d.x ? 1.0f / d.x : 0.0f, d.y ? 1.0f / d.y : 0.0f, d.z ? 1.0f / d.z : 0.0f);
+		float m = tmax(tmax(tmax(fabsf(r0.x), fabsf(r0.y)), fabsf(r0.z)),
+		               tmax(tmax(fabsf(r1.x), fabsf(r1.y)), fabsf(r1.z)));
+		float pad = 1e-3f * (1.0f + m);
+		f3 p3 = mk3(pad, pad, pad);
+		qmn = vmin(r0, r1) - p3;
+		qmx = vmax(r0, r1) + p3;
 	}
 };
 
@@ -122,7 +132,12 @@ public:
 			}
 		}
 	}
-	// Items of every node the segment's slab test accepts, until fn returns false.
+	// Items of every node the segment's slab test accepts, until fn returns
+	// false.  The slab test skips the axes where the direction is zero, so an
+	// axis-aligned segment accepts every box in a whole slab of space; the
+	// callers look for triangles the segment meets, so boxes disjoint from the
+	// segment's padded bounding box are passed over as well (same visit order
+	// for the rest, same answers).
 	template <class F> void ray_query(const RaySeg& r, F& fn) const
 	{
 		int32_t stack[128];
@@ -131,7 +146,7 @@ public:
 		while (sp) {
 			int32_t n = stack[--sp];
 			const Node& N = nodes_[n];
-			if (!ray_box(r, N.mn, N.mx))
+			if (box_disjoint(r.qmn, r.qmx, N.mn, N.mx) || !ray_box(r, N.mn, N.mx))
 				continue;
 			if (N.ido != -1 && !fn(&items_[N.ido + 1], items_[N.ido]))
 				return;

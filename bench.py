@@ -181,6 +181,28 @@ def cpu_baseline(spec, pos, nrm, items, w, lights, target_s, nthreads):
 FORMS = {0: "sparse_f64", 1: "sparse_f32", 2: "dense_f32", 3: "dense_bf16_mfma", 4: "dense_f16_mfma"}
 
 
+def read_links(lib, dev, N, nlinks):
+    """The link list of the last lgx_RadLinks: per-row (offset, count), the
+    other patch and the factor of each link (rows i < j, as the reference
+    stores them)."""
+    rowmap = np.zeros(2 * N, np.uint32)
+    other = np.zeros(nlinks, np.uint32)
+    fac = np.zeros(nlinks, np.float32)
+    _lib.check(lib.lgx_RadReadLinks(dev, rowmap.ctypes.data_as(u32p), other.ctypes.data_as(u32p), P(fac)), lib)
+    return rowmap, other, fac
+
+
+def tile_fraction(N, rowmap, other):
+    """Fraction of the 256-row x 32-column tiles of the symmetric link matrix
+    that hold a link: the tiles the 16-bit forms store and multiply."""
+    Np = max(512, -(-N // 256) * 256)
+    i = np.repeat(np.arange(N, dtype=np.int64), rowmap[1::2].astype(np.int64))
+    j = other.astype(np.int64)
+    nks = Np // 32
+    t = np.concatenate([(i >> 8) * nks + (j >> 5), (j >> 8) * nks + (i >> 5)])
+    return len(np.unique(t)) / ((Np // 256) * nks)
+
+
 def room_patches(lib, nth, n_patches, lm0, **room_kw):
     """The room's texels as radiosity patches: grow the atlas until the room
     instance has n_patches samples, keep the first n_patches."""
@@ -293,15 +315,13 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
 
     ref64, t64 = run(0)
     res = {}
+    rowmap, other, fac = read_links(lib, dev, N, nl.value)
+    tiles = tile_fraction(N, rowmap, other)
     # pin the GPU FP64 series against the C oracle on the first setup (RGB)
     pin = None
     try:
         from oracle.checkers import Oracle
         orc = Oracle()
-        rowmap = np.zeros(2 * N, np.uint32)
-        other = np.zeros(nl.value, np.uint32)
-        fac = np.zeros(nl.value, np.float32)
-        lib.lgx_RadReadLinks(dev, rowmap.ctypes.data_as(u32p), other.ctypes.data_as(u32p), P(fac))
         t0 = time.perf_counter()
         exp = orc.rad_series_f64(rowmap, other, fac, area, diff, E[:, :3], bounces)
         pin = float(np.linalg.norm(ref64[:, :3] - exp) / np.linalg.norm(exp))
@@ -323,20 +343,20 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
         log(f"[bench] {key}: {per_bounce:.3f} ms/bounce, err {res[key]['rel_l2_err_vs_fp64']:.2e}")
     lib.lgx_DeviceDestroy(dev)
     b.destroy()
-    Np = ((N + 255) // 256) * 256
+    Np = max(512, ((N + 255) // 256) * 256)
     mf = res["dense_bf16_mfma"]["s_per_bounce"]
-    gemm_bytes = Np * Np * 2 + 2 * 192 * Np * 2
+    gemm_bytes = tiles * Np * Np * 2 + 2 * 192 * Np * 2
     out = dict(workload=f"C2 room radiosity: {N} patches (texels of the 10x10x4 m room), {R} right-hand sides "
                         f"({setups} seeded lighting setups x RGB), {bounces} bounces", patches=N, rhs=R,
                bounces=bounces, links=int(nl.value), link_density=round(2 * nl.value / (N * (N - 1)), 5),
-               link_generation_s=link_ms / 1000.0, forms=res,
+               stored_tile_fraction=round(tiles, 4), link_generation_s=link_ms / 1000.0, forms=res,
                fp64_vs_oracle_rel_l2=pin,
                mfma_bf16_speedup_vs_dense_f32=round(res["dense_f32"]["s_per_bounce"] / mf, 2),
                mfma_f16_refined_speedup_vs_dense_f32=round(
                    res["dense_f32"]["s_per_bounce"] / res["dense_f16_mfma_refined"]["s_per_bounce"], 2),
                mfma_roofline=dict(bound="hbm", achieved=round(gemm_bytes / mf / 1e9, 1), peak=HBM_PEAK_GBS,
                                   unit="GB/s", frac=round(gemm_bytes / mf / 1e9 / HBM_PEAK_GBS, 4),
-                                  note="BF16 matrix once per bounce + right-hand sides in and out"),
+                                  note="BF16 matrix (its stored tiles) once per bounce + right-hand sides in and out"),
                ao=ao, cpu_port_s_per_bounce_rgb=cpu_bounce_s,
                cpu_note="the reference's Int_RDBounce loop restated in C (oracle), one RGB setup, one host thread, "
                         "on the same link list; the reference itself runs it single-threaded")
@@ -611,8 +631,18 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
         per = t[2] if not refine else (t[1] + t[3]) / bounces
         forms[key] = dict(s_per_bounce=per / 1e3, rel_l2_err_vs_fp64=float(np.linalg.norm(o - ref64) / np.linalg.norm(ref64)))
         log(f"[bench] C4 radiosity {key}: {per:.3f} ms/bounce, err {forms[key]['rel_l2_err_vs_fp64']:.2e}")
+    rowmap, other, _ = read_links(lib, rad_dev, N, nl.value)
+    tiles = tile_fraction(N, rowmap, other)
     out["radiosity"] = dict(patches=N, rhs=R, bounces=bounces, links=int(nl.value), link_generation_ms=round(link_ms, 1),
-                            forms=forms)
+                            stored_tile_fraction=round(tiles, 4), forms=forms)
+    sp = forms["sparse_f32"]["s_per_bounce"]
+    best = min((v["s_per_bounce"], k) for k, v in forms.items()
+               if "mfma" in k and v["rel_l2_err_vs_fp64"] <= 1e-4)
+    out["radiosity"]["mfma_at_1e-4"] = dict(form=best[1], s_per_bounce=best[0],
+                                            speedup_vs_sparse_f32_gather=round(sp / best[0], 2),
+                                            speedup_vs_dense_f32=round(forms["dense_f32"]["s_per_bounce"] / best[0], 2))
+    log(f"[bench] C4 radiosity: {tiles:.4f} of the 256 x 32 tiles stored; best MFMA form at <= 1e-4: {best[1]}, "
+        f"{sp / best[0]:.2f}x the sparse FP32 gather")
     lib.lgx_DeviceDestroy(rad_dev)
     if with_cpu:
         cpu, samp = cpu_baseline(spec, pos, nrm, items, w, lights, cpu_seconds, nth)

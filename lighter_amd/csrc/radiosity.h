@@ -14,8 +14,8 @@ enum RadForm {
 	kSparseF64 = 0, // gather over the links, FP64 (the accuracy reference on the GPU)
 	kSparseF32 = 1, // gather over the links, FP32
 	kDenseF32 = 2,  // dense H = F in FP32, plain FP32 FMAs
-	kDenseBF16 = 3, // dense H in BF16, MFMA with FP32 accumulation
-	kDenseF16 = 4,  // dense H in FP16 (power-of-two scaled), MFMA with FP32 accumulation
+	kDenseBF16 = 3, // H in BF16 (the tiles that hold links), MFMA with FP32 accumulation
+	kDenseF16 = 4,  // H in FP16 (power-of-two scaled, the tiles that hold links), MFMA with FP32 accumulation
 };
 
 struct RadTimes {
@@ -42,9 +42,11 @@ public:
 	RadTimes times;
 	double build_ms = 0;
 	uint32_t N = 0, Np = 0, R = 0;
+	uint64_t ntiles() const { return ntiles_; } // stored 256 x 32 tiles of the 16-bit forms
 
 private:
 	void release();
+	void build_tiles();
 	void ensure_work();
 	uint32_t ksplit(uint32_t rows_per_block, uint32_t min_blocks) const;
 	void apply_dense(int form, const float* xT, float* yT, bool split, bool pre, int which);
@@ -54,6 +56,10 @@ private:
 	float* g32_ = nullptr;
 	uint16_t* g16_ = nullptr;   // 16-bit matrix (hi plane)
 	uint16_t* g16lo_ = nullptr; // remainder plane for the compensated product
+	int32_t* tmap_ = nullptr;   // tile (256-row block, k stage) -> place in the 16-bit store, or -1
+	uint32_t* toff_ = nullptr;  // per 256-row block: first entry of its tiles in tks_
+	uint32_t* tks_ = nullptr;   // k stage of each stored tile
+	uint64_t ntiles_ = 0;
 	int built_form_ = -1;
 	uint32_t ncu_ = 256; // compute units of the device
 	float gscale_ = 1.0f;

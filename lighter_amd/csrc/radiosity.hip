@@ -127,7 +127,8 @@ __device__ __forceinline__ float from16(uint16_t v)
 template <int P>
 __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N,
                                                     uint32_t Np, const float* __restrict__ pos,
-                                                    const float* __restrict__ nrm, float scale, void* __restrict__ G,
+                                                    const float* __restrict__ nrm, float scale,
+                                                    const int32_t* __restrict__ tmap, void* __restrict__ G,
                                                     uint16_t* __restrict__ Glo)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
@@ -135,6 +136,12 @@ __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__
 	if (t >= (uint64_t)Np * wpr)
 		return;
 	uint32_t i = (uint32_t)(t / wpr), w = (uint32_t)(t % wpr);
+	int32_t tile = 0;
+	if (P >= 0) {
+		tile = tmap[(uint64_t)(i >> 8) * wpr + w];
+		if (tile < 0)
+			return; // a tile without links is not stored
+	}
 	uint32_t m = (i < N && w < words) ? bits[(uint64_t)i * words + w] : 0u;
 	float v[32];
 #pragma unroll
@@ -158,11 +165,11 @@ __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__
 			// remainder plane: what the 16-bit value left out
 			pl[q] = (uint32_t)Prec<PP>::cvt(a - from16<PP>(ha)) | ((uint32_t)Prec<PP>::cvt(b - from16<PP>(hb)) << 16);
 		}
-		// LDS-image layout: word w is k stage w; its four 8-element pieces go to
-		// the row's 64 bytes of 1 KB group (i & 255) >> 4, piece q at slot
-		// q ^ piece_swz(i & 15)
-		const uint64_t nks = Np / 32;
-		const uint64_t grp = ((uint64_t)(i >> 8) * nks + w) * 16 + ((i & 255) >> 4);
+		// LDS-image layout: the tile (256-row block i >> 8, k stage w) is
+		// 16 KB at its place in the tile list; its four 8-element pieces of
+		// row i go to the row's 64 bytes of 1 KB group (i & 255) >> 4, piece q
+		// at slot q ^ piece_swz(i & 15)
+		const uint64_t grp = (uint64_t)tile * 16 + ((i & 255) >> 4);
 		const uint32_t r = i & 15;
 		uint4* o = reinterpret_cast<uint4*>((uint16_t*)G + grp * 512 + r * 32);
 #pragma unroll
@@ -175,6 +182,25 @@ __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__
 				ol[q ^ piece_swz(r)] = make_uint4(pl[4 * q], pl[4 * q + 1], pl[4 * q + 2], pl[4 * q + 3]);
 		}
 	}
+}
+
+// which tiles (256-row block rb, 32-column k stage w) hold a link: the OR of
+// the pair bitmask's word w over the block's rows; one thread per (rb, w)
+__global__ __launch_bounds__(kB) void k_tile_flags(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N,
+                                                   uint32_t Np, uint8_t* __restrict__ flags)
+{
+	const uint32_t nks = Np / 32;
+	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
+	if (t >= (uint64_t)(Np / 256) * nks)
+		return;
+	const uint32_t rb = (uint32_t)(t / nks), w = (uint32_t)(t % nks);
+	uint32_t m = 0;
+	if (w < words) {
+		const uint32_t i1 = min(N, (rb + 1) * 256);
+		for (uint32_t i = rb * 256; i < i1; ++i)
+			m |= bits[(uint64_t)i * words + w];
+	}
+	flags[t] = m != 0u;
 }
 
 // largest link factor (for the FP16 scale); grid-stride over the stored
@@ -358,9 +384,10 @@ constexpr uint64_t kXStageElems = (kRPad / 16) * kTileElems; // a stage of the o
 template <int NPL, int RT>
 __device__ __forceinline__ void gemm_issue(const uint16_t* G, const uint16_t* Glo, const uint16_t* xq,
                                            const uint16_t* xlo, uint64_t xsrc, uint64_t gsrc, uint64_t gstride,
-                                           uint32_t st, unsigned char* base, int wave)
+                                           const uint32_t* lst, uint32_t st, unsigned char* base, int wave)
 {
 	typedef GemmShape<NPL, RT> S;
+	const uint64_t ks = lst[st]; // k stage of the block's st-th listed tile
 #pragma unroll
 	for (int pl = 0; pl < NPL; ++pl) {
 		const uint16_t* X = pl ? xlo : xq;
@@ -369,7 +396,7 @@ __device__ __forceinline__ void gemm_issue(const uint16_t* G, const uint16_t* Gl
 		unsigned char* gb = base + NPL * S::kXBytes + pl * S::kGBytes + wave * RT * 1024;
 #pragma unroll
 		for (int j = 0; j < kColTiles / 4; ++j)
-			dma16(X + xsrc + (uint64_t)st * kXStageElems + j * kTileElems, xb + j * 1024);
+			dma16(X + xsrc + ks * kXStageElems + j * kTileElems, xb + j * 1024);
 #pragma unroll
 		for (int t = 0; t < RT; ++t)
 			dma16(M + gsrc + (uint64_t)st * gstride + t * kTileElems, gb + t * 1024);
@@ -380,6 +407,7 @@ template <int NPL, int RT>
 struct GemmLane {
 	uint64_t xsrc, gsrc; // DMA source offsets (elements) of this lane at the split's first stage
 	uint64_t gstride;    // matrix elements from one stage to the next
+	const uint32_t* lst; // k stages of the split's tiles
 	int wave, rofs;      // wave index; ds_read_b128 offset of the fragment in a 1 KB group
 	unsigned char* buf[kMaxPD]; // the LDS stage ring
 };
@@ -415,7 +443,7 @@ __device__ __forceinline__ void gemm_step(int s, int nst, const GemmLane<NPL, RT
 	// above exact
 	constexpr int BN = (B + PD - 1) % PD;
 	const int ahead = s + PD - 1 < nst ? s + PD - 1 : nst - 1;
-	gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, L.gstride, ahead, L.buf[BN], L.wave);
+	gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, L.gstride, L.lst, ahead, L.buf[BN], L.wave);
 #pragma unroll
 	for (int c = 0; c < kColTiles; ++c) {
 #pragma unroll
@@ -447,7 +475,8 @@ __device__ __forceinline__ void gemm_prologue(int nst, const GemmLane<NPL, RT>& 
                                               uint64_t Np, std::integer_sequence<int, Bs...>)
 {
 	// stage Bs into buffer Bs (the last stage again where there are fewer)
-	(gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, L.gstride, Bs < nst ? Bs : nst - 1, L.buf[Bs], L.wave),
+	(gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, L.gstride, L.lst, Bs < nst ? Bs : nst - 1, L.buf[Bs],
+	                     L.wave),
 	 ...);
 }
 
@@ -460,9 +489,13 @@ __device__ __forceinline__ void gemm_tail(int s, int nst, const GemmLane<NPL, RT
 	((s + Bs < nst ? gemm_step<P, NPL, RT, PD, Bs>(s + Bs, nst, L, G, Glo, xq, xlo, Np, acc) : void()), ...);
 }
 
+// y = G x over the stored tiles: block (x, y) takes rows x * kRows ..
+// and split y of its 256-row block's tile list (toff[rb] .. toff[rb + 1] in
+// tks, the k stage of each stored tile, ascending)
 template <int P, int NPL, int RT, int PD>
 __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G, const uint16_t* __restrict__ Glo,
                                                   const uint16_t* __restrict__ xq, const uint16_t* __restrict__ xlo,
+                                                  const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tks,
                                                   uint32_t Np, uint32_t ksplit, float* __restrict__ part)
 {
 	typedef GemmShape<NPL, RT> S;
@@ -484,18 +517,19 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 	L.buf[5] = lds5;
 	L.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const uint32_t i0 = blockIdx.x * S::kRows + L.wave * 16 * RT; // this wave's first row
-	const uint32_t klen = Np / ksplit, kbeg = blockIdx.y * klen;
-	const int nst = (int)(klen / kKS);
+	const uint32_t row0 = blockIdx.x * S::kRows;                    // the block's first row
+	const uint32_t rb = row0 >> 8, gbase = (row0 & 255) >> 4;
+	const uint32_t t0 = toff[rb], cnt = toff[rb + 1] - t0;
+	const uint32_t pbeg = (uint32_t)((uint64_t)cnt * blockIdx.y / ksplit);
+	const uint32_t pend = (uint32_t)((uint64_t)cnt * (blockIdx.y + 1) / ksplit);
+	const int nst = (int)(pend - pbeg);
 	// DMA sources in the LDS-image layouts: lane l copies the l-th 16 bytes
-	// of each 1 KB group.  Matrix tile (256-row block rb, stage ks, group g)
-	// at ((rb * nks + ks) * 16 + g) KB; operand tile (stage ks, column
-	// group cg) at (ks * 12 + cg) KB.
-	const uint64_t nks = Np / kKS, ks0 = kbeg / kKS;
-	const uint32_t row0 = blockIdx.x * S::kRows; // the block's first row
-	const uint64_t rb = row0 >> 8, gbase = (row0 & 255) >> 4;
+	// of each 1 KB group.  Stored tile t (16 KB) holds group g at
+	// (t * 16 + g) KB; operand stage ks, column group cg at (ks * 12 + cg) KB.
+	L.lst = tks + t0 + pbeg;
 	L.gstride = kGStageElems;
-	L.gsrc = ((rb * nks + ks0) * 16 + gbase + (uint64_t)L.wave * RT) * kTileElems + 8 * lane;
-	L.xsrc = (ks0 * (kRPad / 16) + (uint64_t)(kColTiles / 4) * L.wave) * kTileElems + 8 * lane;
+	L.gsrc = ((uint64_t)(t0 + pbeg) * 16 + gbase + (uint64_t)L.wave * RT) * kTileElems + 8 * lane;
+	L.xsrc = ((uint64_t)(kColTiles / 4) * L.wave) * kTileElems + 8 * lane;
 	L.rofs = r * 64 + 16 * (g ^ piece_swz(r));
 
 	f32x4 acc[RT][kColTiles];
@@ -505,14 +539,16 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 		for (int c = 0; c < kColTiles; ++c)
 			acc[t][c] = f32x4{0, 0, 0, 0};
 
-	// prologue: stages 0 .. PD-2 into buffers 0 .. PD-2 (nst >= 1)
-	gemm_prologue<NPL, RT>(nst, L, G, Glo, xq, xlo, Np, std::make_integer_sequence<int, PD - 1>{});
-	int s = 0;
-	for (; s + PD <= nst; s += PD)
-		gemm_steps<P, NPL, RT, PD>(s, L, G, Glo, xq, xlo, Np, acc, nst, std::make_integer_sequence<int, PD>{});
-	// fewer than PD stages left
-	gemm_tail<P, NPL, RT, PD>(s, nst, L, G, Glo, xq, xlo, Np, acc, std::make_integer_sequence<int, PD - 1>{});
-	wait_vm<0>(); // no DMA may land after the block has left its LDS
+	if (nst > 0) { // a split without tiles writes zeros
+		// prologue: stages 0 .. PD-2 into buffers 0 .. PD-2
+		gemm_prologue<NPL, RT>(nst, L, G, Glo, xq, xlo, Np, std::make_integer_sequence<int, PD - 1>{});
+		int s = 0;
+		for (; s + PD <= nst; s += PD)
+			gemm_steps<P, NPL, RT, PD>(s, L, G, Glo, xq, xlo, Np, acc, nst, std::make_integer_sequence<int, PD>{});
+		// fewer than PD stages left
+		gemm_tail<P, NPL, RT, PD>(s, nst, L, G, Glo, xq, xlo, Np, acc, std::make_integer_sequence<int, PD - 1>{});
+		wait_vm<0>(); // no DMA may land after the block has left its LDS
+	}
 
 	// part[s][c][i]: column-major per split
 	float* out = part + (uint64_t)blockIdx.y * kRPad * Np;
@@ -758,6 +794,9 @@ void RadSolver::release()
 	dfree(g32_);
 	dfree(g16_);
 	dfree(g16lo_);
+	dfree(tmap_);
+	dfree(toff_);
+	dfree(tks_);
 	dfree(E_);
 	dfree(diff_);
 	dfree(area_);
@@ -797,6 +836,47 @@ void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3,
 	HIPCHECK(hipStreamSynchronize(s));
 }
 
+// The 16-bit forms store only the tiles (256 rows x one 32-column k stage)
+// that hold a link: per 256-row block, the ascending list of its stored
+// tiles' k stages (tks_, from toff_[rb]) and, for the build, each tile's
+// place in the store (tmap_, -1 for none).  A room whose patches all see
+// each other stores every tile; a city block, where links stay within tens
+// of metres, a few percent of them.
+void RadSolver::build_tiles()
+{
+	if (toff_)
+		return;
+	hipStream_t s = (hipStream_t)s_;
+	const uint32_t nrb = Np / 256, nks = Np / 32;
+	const uint64_t n = (uint64_t)nrb * nks;
+	uint8_t* df = dalloc<uint8_t>(n);
+	k_tile_flags<<<gridn(n), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, df);
+	HIPCHECK(hipGetLastError());
+	std::vector<uint8_t> f(n);
+	HIPCHECK(hipMemcpyAsync(f.data(), df, n, hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	dfree(df);
+	std::vector<int32_t> map(n, -1);
+	std::vector<uint32_t> off(nrb + 1, 0), ks;
+	for (uint32_t rb = 0; rb < nrb; ++rb) {
+		for (uint32_t w = 0; w < nks; ++w)
+			if (f[(uint64_t)rb * nks + w]) {
+				map[(uint64_t)rb * nks + w] = (int32_t)ks.size();
+				ks.push_back(w);
+			}
+		off[rb + 1] = (uint32_t)ks.size();
+	}
+	ntiles_ = ks.size();
+	tmap_ = dalloc<int32_t>(n);
+	toff_ = dalloc<uint32_t>(nrb + 1);
+	tks_ = dalloc<uint32_t>(ntiles_);
+	HIPCHECK(hipMemcpyAsync(tmap_, map.data(), 4 * n, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(toff_, off.data(), 4 * (nrb + 1), hipMemcpyHostToDevice, s));
+	if (ntiles_)
+		HIPCHECK(hipMemcpyAsync(tks_, ks.data(), 4 * ntiles_, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipStreamSynchronize(s));
+}
+
 void RadSolver::build_dense(int form)
 {
 	hipStream_t s = (hipStream_t)s_;
@@ -806,12 +886,13 @@ void RadSolver::build_dense(int form)
 		if (!g32_)
 			g32_ = dalloc<float>((uint64_t)Np * Np);
 		k_dense_build<-1><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-		                                                dev.rad.nrm, 1.0f, g32_, nullptr);
+		                                                dev.rad.nrm, 1.0f, nullptr, g32_, nullptr);
 	} else {
+		build_tiles();
 		if (!g16_)
-			g16_ = dalloc<uint16_t>((uint64_t)Np * Np);
+			g16_ = dalloc<uint16_t>(ntiles_ * kGStageElems);
 		if (!g16lo_)
-			g16lo_ = dalloc<uint16_t>((uint64_t)Np * Np);
+			g16lo_ = dalloc<uint16_t>(ntiles_ * kGStageElems);
 		gscale_ = 1.0f;
 		if (form == kDenseF16) {
 			// scale so that the largest factor sits near 2^13: small factors
@@ -827,10 +908,10 @@ void RadSolver::build_dense(int form)
 			memcpy(&mx, &hm, 4);
 			gscale_ = mx > 0 ? ldexpf(1.0f, 14 - ilogbf(mx) - 1) : 1.0f;
 			k_dense_build<1><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-			                                               dev.rad.nrm, gscale_, g16_, g16lo_);
+			                                               dev.rad.nrm, gscale_, tmap_, g16_, g16lo_);
 		} else {
 			k_dense_build<0><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-			                                               dev.rad.nrm, 1.0f, g16_, g16lo_);
+			                                               dev.rad.nrm, 1.0f, tmap_, g16_, g16lo_);
 		}
 	}
 	HIPCHECK(hipGetLastError());
@@ -896,13 +977,13 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 		const uint32_t ks = ksplit(rows, ncu_);
 		dim3 grid(Np / rows, ks);
 		if (form == kDenseF16 && split)
-			k_gemm_mfma<1, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part_);
+			k_gemm_mfma<1, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, toff_, tks_, Np, ks, part_);
 		else if (form == kDenseF16)
-			k_gemm_mfma<1, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part_);
+			k_gemm_mfma<1, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, toff_, tks_, Np, ks, part_);
 		else if (split)
-			k_gemm_mfma<0, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, Np, ks, part_);
+			k_gemm_mfma<0, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, toff_, tks_, Np, ks, part_);
 		else
-			k_gemm_mfma<0, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, Np, ks, part_);
+			k_gemm_mfma<0, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, toff_, tks_, Np, ks, part_);
 		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, Np, R, cu_[which], yT);
 	}
 	HIPCHECK(hipGetLastError());

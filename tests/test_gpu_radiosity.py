@@ -22,9 +22,7 @@ TOL = {0: 1e-12, 1: 1e-5, 2: 1e-5, 3: 3e-2, 4: 3e-3}
 TOL_REFINED = {3: 1e-4, 4: 1e-5}
 
 
-@pytest.fixture(scope="module")
-def links(lib):
-    spec = scenes.room_scene(lm=40, n_boxes=6, bounce_count=1)
+def scene_links(lib, spec):
     b = prepared(lib, spec)
     pos, nrm, tris = all_samples_and_tris(lib, b)
     pn = np.array([normalize_f32(v) for v in nrm], np.float32)
@@ -37,7 +35,25 @@ def links(lib):
     fac = np.zeros(nl.value, np.float32)
     u32p = C.POINTER(C.c_uint32)
     _lib.check(lib.lgx_RadReadLinks(dev.h, rowmap.ctypes.data_as(u32p), other.ctypes.data_as(u32p), P(fac)), lib)
-    yield dev, N, rowmap, other, fac
+    return b, dev, (dev, N, rowmap, other, fac)
+
+
+@pytest.fixture(scope="module")
+def links(lib):
+    b, dev, out = scene_links(lib, scenes.room_scene(lm=40, n_boxes=6, bounce_count=1))
+    yield out
+    dev.close()
+    b.destroy()
+
+
+@pytest.fixture(scope="module")
+def city_links(lib):
+    """A 200 m city block in 4 x 4 tiles (16,988 patches): links reach a few
+    tens of metres, so most 256-row x 32-column tiles of H hold none."""
+    spec = scenes.city_scene(extent=200.0, grid=32, amplitude=2.0, n_boxes=144, tiles=4, lm_total=160,
+                             n_lights=2)
+    b, dev, out = scene_links(lib, spec)
+    yield out
     dev.close()
     b.destroy()
 
@@ -88,3 +104,46 @@ def test_full_width_192_columns(lib, orc, links):
     for form, tol in [(2, 1e-5), (3, 3e-2), (4, 3e-3)]:
         got, _ = solve(lib, dev, E, diff, area, form, 3)
         assert np.linalg.norm(got - exp) / np.linalg.norm(exp) < tol, form
+
+
+def tile_fraction(N, rowmap, other):
+    """Fraction of the 256 x 32 tiles of the symmetric link matrix that hold
+    a link (the tiles the 16-bit forms store and read)."""
+    Np = max(512, -(-N // 256) * 256)
+    i = np.repeat(np.arange(N, dtype=np.int64), rowmap[1::2].astype(np.int64))
+    j = other.astype(np.int64)
+    nks = Np // 32
+    t = np.concatenate([(i >> 8) * nks + (j >> 5), (j >> 8) * nks + (i >> 5)])
+    return len(np.unique(t)) / ((Np // 256) * nks)
+
+
+@pytest.mark.parametrize("form", [3, 4])
+def test_tile_skipping_on_a_city_block(lib, orc, city_links, form):
+    """The 16-bit forms multiply only the tiles that hold links; on a city
+    block most tiles are empty, and the series and its refinement still meet
+    the dense tolerances (and the product equals the FP32 form's to the
+    16-bit rounding)."""
+    dev, N, rowmap, other, fac = city_links
+    frac = tile_fraction(N, rowmap, other)
+    assert 0.0 < frac < 0.5, frac
+    rng = np.random.default_rng(50 + form)
+    R = 24
+    diff = rng.uniform(0.2, 0.9, (N, 3)).astype(np.float32)
+    area = rng.uniform(0.01, 0.06, N).astype(np.float32)
+    E = rng.uniform(0, 1, (N, R))
+    E[rng.uniform(size=N) < 0.5] *= 1e-3
+    bounces = 4
+    exp = orc.rad_series_f64(rowmap, other, fac, area, diff, E, bounces)
+    assert np.linalg.norm(exp - E) / np.linalg.norm(E) > 1e-3
+    got, _ = solve(lib, dev, E, diff, area, form, bounces)
+    rel = np.linalg.norm(got - exp) / np.linalg.norm(exp)
+    assert rel < TOL[form], (form, rel)
+    got1, _ = solve(lib, dev, E, diff, area, form, bounces, refine=1)
+    rel1 = np.linalg.norm(got1 - exp) / np.linalg.norm(exp)
+    assert rel1 < TOL_REFINED[form], (form, rel, rel1)
+    # rows without links keep their emission exactly
+    lonely = np.ones(N, bool)
+    lonely[np.repeat(np.arange(N), rowmap[1::2])] = False
+    lonely[other] = False
+    if lonely.any():
+        assert np.array_equal(got[lonely], E[lonely])

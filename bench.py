@@ -192,15 +192,12 @@ def read_links(lib, dev, N, nlinks):
     return rowmap, other, fac
 
 
-def tile_fraction(N, rowmap, other):
-    """Fraction of the 256-row x 32-column tiles of the symmetric link matrix
-    that hold a link: the tiles the 16-bit forms store and multiply."""
-    Np = max(512, -(-N // 256) * 256)
-    i = np.repeat(np.arange(N, dtype=np.int64), rowmap[1::2].astype(np.int64))
-    j = other.astype(np.int64)
-    nks = Np // 32
-    t = np.concatenate([(i >> 8) * nks + (j >> 5), (j >> 8) * nks + (i >> 5)])
-    return len(np.unique(t)) / ((Np // 256) * nks)
+def tile_fraction(lib, dev):
+    """Fraction of the 256-row x 32-column tiles of the link matrix that the
+    16-bit forms store and multiply (those holding a link, Morton order)."""
+    out = (C.c_uint64 * 2)()
+    _lib.check(lib.lgx_RadTiles(dev, out), lib)
+    return out[0] / out[1] if out[1] else 0.0
 
 
 def room_patches(lib, nth, n_patches, lm0, **room_kw):
@@ -316,7 +313,6 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
     ref64, t64 = run(0)
     res = {}
     rowmap, other, fac = read_links(lib, dev, N, nl.value)
-    tiles = tile_fraction(N, rowmap, other)
     # pin the GPU FP64 series against the C oracle on the first setup (RGB)
     pin = None
     try:
@@ -341,6 +337,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
         res[key] = dict(s_per_bounce=per_bounce / 1000.0, rel_l2_err_vs_fp64=float(np.linalg.norm(out - ref64) / nrm64),
                         series_ms=round(t[1], 3), refine_ms=round(t[3], 3), build_ms=round(t[0], 3))
         log(f"[bench] {key}: {per_bounce:.3f} ms/bounce, err {res[key]['rel_l2_err_vs_fp64']:.2e}")
+    tiles = tile_fraction(lib, dev)
     lib.lgx_DeviceDestroy(dev)
     b.destroy()
     Np = max(512, ((N + 255) // 256) * 256)
@@ -631,8 +628,7 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
         per = t[2] if not refine else (t[1] + t[3]) / bounces
         forms[key] = dict(s_per_bounce=per / 1e3, rel_l2_err_vs_fp64=float(np.linalg.norm(o - ref64) / np.linalg.norm(ref64)))
         log(f"[bench] C4 radiosity {key}: {per:.3f} ms/bounce, err {forms[key]['rel_l2_err_vs_fp64']:.2e}")
-    rowmap, other, _ = read_links(lib, rad_dev, N, nl.value)
-    tiles = tile_fraction(N, rowmap, other)
+    tiles = tile_fraction(lib, rad_dev)
     out["radiosity"] = dict(patches=N, rhs=R, bounces=bounces, links=int(nl.value), link_generation_ms=round(link_ms, 1),
                             stored_tile_fraction=round(tiles, 4), forms=forms)
     sp = forms["sparse_f32"]["s_per_bounce"]

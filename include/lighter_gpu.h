@@ -143,6 +143,11 @@ LTRAPI int lgx_RadSolve(lgx_Device* dev, const double* E, uint32_t R, const floa
 LTRAPI int lgx_RadSolveGrad(lgx_Device* dev, const double* E, uint32_t R, const float* diffuse3,
                             const float* area, int form, int bounces, const double* g, double* total, double* lam,
                             double* dalb, double times_ms[4]);
+/* Tile store of the 16-bit forms after an lgx_RadSolve / lgx_RadSolveGrad in
+ * form 3 or 4 (no counterpart in the reference): out[0] = stored 256 x 32
+ * tiles (those holding a link, patches in Morton order), out[1] = all tiles
+ * of the padded matrix.  Zeros before such a solve. */
+LTRAPI int lgx_RadTiles(lgx_Device* dev, uint64_t out[2]);
 /* Reverse mode of the last lgx_DirectLight / lgx_DirectLightRun (no
  * counterpart in the reference): the lightmap gains colour * weight per work
  * item (Job_LMRender_*_Inner, lighter.cpp:485-589), so for lam3 = dL/dlightmap

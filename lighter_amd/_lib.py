@@ -62,6 +62,7 @@ LGX_SIGNATURES = {
     "lgx_RadBounce": (C.c_int, [vp, fp, fp, fp, C.c_int, fp]),
     "lgx_RadSolve": (C.c_int, [vp, C.POINTER(C.c_double), C.c_uint32, fp, fp, C.c_int, C.c_int, C.c_int,
                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "lgx_RadTiles": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
     "lgx_RadSolveGrad": (C.c_int, [vp, C.POINTER(C.c_double), C.c_uint32, fp, fp, C.c_int, C.c_int,
                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -672,6 +672,19 @@ int lgx_RadSolveGrad(lgx_Device* d, const double* E, uint32_t R, const float* di
 	});
 }
 
+int lgx_RadTiles(lgx_Device* d, uint64_t out[2])
+{
+	return guarded([&] {
+		if (!out)
+			throw std::runtime_error("lgx_RadTiles: null output");
+		out[0] = out[1] = 0;
+		if (d->rad && d->rad->ntiles_built()) {
+			out[0] = d->rad->ntiles();
+			out[1] = (uint64_t)(d->rad->Np / 256) * (d->rad->Np / 32);
+		}
+	});
+}
+
 int lgx_DirectLightGrad(lgx_Device* d, const double* lam3, double* dcolor3)
 {
 	return guarded([&] {

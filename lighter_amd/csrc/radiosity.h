@@ -43,6 +43,7 @@ public:
 	double build_ms = 0;
 	uint32_t N = 0, Np = 0, R = 0;
 	uint64_t ntiles() const { return ntiles_; } // stored 256 x 32 tiles of the 16-bit forms
+	bool ntiles_built() const { return toff_ != nullptr; }
 
 private:
 	void release();
@@ -60,6 +61,8 @@ private:
 	uint32_t* toff_ = nullptr;  // per 256-row block: first entry of its tiles in tks_
 	uint32_t* tks_ = nullptr;   // k stage of each stored tile
 	uint64_t ntiles_ = 0;
+	uint32_t* perm_ = nullptr; // position -> patch (Morton order, padding last)
+	uint32_t* pinv_ = nullptr; // patch -> position
 	int built_form_ = -1;
 	uint32_t ncu_ = 256; // compute units of the device
 	float gscale_ = 1.0f;

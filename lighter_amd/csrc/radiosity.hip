@@ -7,20 +7,22 @@
 // right-hand sides at once (columns of E; column c uses channel c % 3):
 //
 //  * sparse gathers over the symmetric CSR links, FP32 or FP64;
-//  * dense G = F diag(area) stored FP32 and multiplied with plain FP32 FMAs;
-//  * dense G stored BF16 or FP16 (power-of-two scaled) and multiplied with
-//    gfx950 MFMA instructions accumulating in FP32;
+//  * dense H = F stored FP32 and multiplied with plain FP32 FMAs;
+//  * H stored BF16 or FP16 (power-of-two scaled) as the 256 x 32 tiles that
+//    hold links, the patches in Morton order, and multiplied with gfx950
+//    MFMA instructions accumulating in FP32;
 //  * the MFMA form followed by iterative refinement sweeps whose residual
 //    r_k = K z_{k-1} - z_k uses the exact link operator in FP64.
 //
-// Layouts: dense G is row-major Np x Np (rows = receiving patch), the dense
-// right-hand sides are column-major (R x Np, each column contiguous), the
-// sparse ones row-major (Np x R).  Np is N rounded up to kPadM.
+// Layouts: dense FP32 H is row-major Np x Np (rows = receiving patch), the
+// dense right-hand sides are column-major (R x Np, each column contiguous),
+// the sparse ones row-major (Np x R).  Np is N rounded up to kPadM.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -109,11 +111,6 @@ __device__ __forceinline__ float link_factor(const float* pos, const float* nrm,
 	return da * db / (l2 * l2 * kPiF);
 }
 
-// ---------------------------------------------------------------------------
-// dense matrix H[i][j] = f_ij (the symmetric link factors, zero off the
-// links) from the occlusion bitmask; one thread per 32-column word.  FP32
-// (P = -1) or 16-bit with a power-of-two scale, plus the remainder plane.
-// ---------------------------------------------------------------------------
 template <int P>
 __device__ __forceinline__ float from16(uint16_t v)
 {
@@ -124,24 +121,20 @@ __device__ __forceinline__ float from16(uint16_t v)
 	return (float)h;
 }
 
-template <int P>
+// ---------------------------------------------------------------------------
+// dense FP32 matrix H[i][j] = f_ij (the symmetric link factors, zero off the
+// links), row-major Np x Np, from the occlusion bitmask; one thread per
+// 32-column word
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N,
                                                     uint32_t Np, const float* __restrict__ pos,
-                                                    const float* __restrict__ nrm, float scale,
-                                                    const int32_t* __restrict__ tmap, void* __restrict__ G,
-                                                    uint16_t* __restrict__ Glo)
+                                                    const float* __restrict__ nrm, float* __restrict__ G)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	uint64_t wpr = Np / 32; // words per padded row
 	if (t >= (uint64_t)Np * wpr)
 		return;
 	uint32_t i = (uint32_t)(t / wpr), w = (uint32_t)(t % wpr);
-	int32_t tile = 0;
-	if (P >= 0) {
-		tile = tmap[(uint64_t)(i >> 8) * wpr + w];
-		if (tile < 0)
-			return; // a tile without links is not stored
-	}
 	uint32_t m = (i < N && w < words) ? bits[(uint64_t)i * words + w] : 0u;
 	float v[32];
 #pragma unroll
@@ -149,58 +142,60 @@ __global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__
 		uint32_t j = w * 32 + b;
 		v[b] = ((m >> b) & 1u) ? link_factor(pos, nrm, i, j) : 0.0f;
 	}
-	if (P < 0) {
-		float4* o = reinterpret_cast<float4*>((float*)G + (uint64_t)i * Np + (uint64_t)w * 32);
+	float4* o = reinterpret_cast<float4*>(G + (uint64_t)i * Np + (uint64_t)w * 32);
 #pragma unroll
-		for (int q = 0; q < 8; ++q)
-			o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
-	} else {
-		constexpr int PP = P < 0 ? 0 : P;
-		uint32_t pk[16], pl[16];
-#pragma unroll
-		for (int q = 0; q < 16; ++q) {
-			float a = v[2 * q] * scale, b = v[2 * q + 1] * scale;
-			uint16_t ha = Prec<PP>::cvt(a), hb = Prec<PP>::cvt(b);
-			pk[q] = (uint32_t)ha | ((uint32_t)hb << 16);
-			// remainder plane: what the 16-bit value left out
-			pl[q] = (uint32_t)Prec<PP>::cvt(a - from16<PP>(ha)) | ((uint32_t)Prec<PP>::cvt(b - from16<PP>(hb)) << 16);
-		}
-		// LDS-image layout: the tile (256-row block i >> 8, k stage w) is
-		// 16 KB at its place in the tile list; its four 8-element pieces of
-		// row i go to the row's 64 bytes of 1 KB group (i & 255) >> 4, piece q
-		// at slot q ^ piece_swz(i & 15)
-		const uint64_t grp = (uint64_t)tile * 16 + ((i & 255) >> 4);
-		const uint32_t r = i & 15;
-		uint4* o = reinterpret_cast<uint4*>((uint16_t*)G + grp * 512 + r * 32);
-#pragma unroll
-		for (int q = 0; q < 4; ++q)
-			o[q ^ piece_swz(r)] = make_uint4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
-		if (Glo) {
-			uint4* ol = reinterpret_cast<uint4*>(Glo + grp * 512 + r * 32);
-#pragma unroll
-			for (int q = 0; q < 4; ++q)
-				ol[q ^ piece_swz(r)] = make_uint4(pl[4 * q], pl[4 * q + 1], pl[4 * q + 2], pl[4 * q + 3]);
-		}
-	}
+	for (int q = 0; q < 8; ++q)
+		o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 }
 
-// which tiles (256-row block rb, 32-column k stage w) hold a link: the OR of
-// the pair bitmask's word w over the block's rows; one thread per (rb, w)
-__global__ __launch_bounds__(kB) void k_tile_flags(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N,
-                                                   uint32_t Np, uint8_t* __restrict__ flags)
+// ---------------------------------------------------------------------------
+// 16-bit forms: H' = Pi H Pi^T for the patch order Pi (pinv[i] = position of
+// patch i), stored as the 256-row x 32-column tiles that hold a link.
+// Tile flags and values come from the symmetric CSR links; one wave per row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kB) void k_tile_flags(uint32_t N, uint32_t Np, const uint32_t* __restrict__ rowptr,
+                                                   const uint32_t* __restrict__ col,
+                                                   const uint32_t* __restrict__ pinv, uint8_t* __restrict__ flags)
 {
-	const uint32_t nks = Np / 32;
-	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
-	if (t >= (uint64_t)(Np / 256) * nks)
+	const uint32_t i = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (i >= N)
 		return;
-	const uint32_t rb = (uint32_t)(t / nks), w = (uint32_t)(t % nks);
-	uint32_t m = 0;
-	if (w < words) {
-		const uint32_t i1 = min(N, (rb + 1) * 256);
-		for (uint32_t i = rb * 256; i < i1; ++i)
-			m |= bits[(uint64_t)i * words + w];
+	const uint64_t rowbase = (uint64_t)(pinv[i] >> 8) * (Np / 32);
+	for (uint32_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64)
+		flags[rowbase + (pinv[col[k]] >> 5)] = 1; // every writer stores 1
+}
+
+// Values into the tile store (zeroed first), scaled by a power of two, and
+// the remainder planes.  LDS-image layout of a tile: row r = i' & 255 in
+// 1 KB group r >> 4 at 64 bytes per row; column kk = j' & 31 in 16-byte piece
+// kk >> 3 at slot (kk >> 3) ^ piece_swz(r & 15), element kk & 7.
+template <int P>
+__global__ __launch_bounds__(kB) void k_tile_scatter(uint32_t N, uint32_t Np, const uint32_t* __restrict__ rowptr,
+                                                     const uint32_t* __restrict__ col,
+                                                     const float* __restrict__ val,
+                                                     const uint32_t* __restrict__ pinv,
+                                                     const int32_t* __restrict__ tmap, float scale,
+                                                     uint16_t* __restrict__ G, uint16_t* __restrict__ Glo)
+{
+	const uint32_t i = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (i >= N)
+		return;
+	const uint32_t ip = pinv[i], r = ip & 255, rr = ip & 15;
+	const uint64_t rowbase = (uint64_t)(ip >> 8) * (Np / 32);
+	const uint64_t inrow = (uint64_t)(r >> 4) * 512 + rr * 32;
+	for (uint32_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
+		const uint32_t jp = pinv[col[k]], kk = jp & 31;
+		const int32_t tile = tmap[rowbase + (jp >> 5)];
+		if (tile < 0)
+			continue; // not reached: the flags came from the same links
+		const uint64_t o = (uint64_t)tile * 8192 + inrow + (uint32_t)(((kk >> 3) ^ piece_swz(rr)) * 8) + (kk & 7);
+		const float a = val[k] * scale;
+		const uint16_t h = Prec<P>::cvt(a);
+		G[o] = h;
+		Glo[o] = Prec<P>::cvt(a - from16<P>(h));
 	}
-	flags[t] = m != 0u;
 }
 
 // largest link factor (for the FP16 scale); grid-stride over the stored
@@ -218,25 +213,28 @@ __global__ __launch_bounds__(kB) void k_max_link(uint64_t nnz, const float* __re
 }
 
 // ---------------------------------------------------------------------------
-// right-hand-side preparation: Xq[c][j] = (x[c][j] * diffuse[j][c % 3]) *
+// right-hand-side preparation: Xq[c][j'] = (x[c][j] * diffuse[j][c % 3]) *
 // area_j * s_c for K = H diag(area) diag(diffuse_c) (pre), or x[c][j] * s_c
-// for the bare H (the adjoint sweep); s_c is a power-of-two column scale
+// for the bare H (the adjoint sweep), at position j' of patch j = perm[j'];
+// s_c is a power-of-two column scale
 // ---------------------------------------------------------------------------
 template <int P>
 __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, const float* __restrict__ diffuse,
                                                  const float* __restrict__ area, const float* __restrict__ cscale,
-                                                 uint32_t N, uint32_t Np, uint32_t R, int pre,
-                                                 uint16_t* __restrict__ xq, uint16_t* __restrict__ xlo)
+                                                 const uint32_t* __restrict__ perm, uint32_t N, uint32_t Np,
+                                                 uint32_t R, int pre, uint16_t* __restrict__ xq,
+                                                 uint16_t* __restrict__ xlo)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t >= (uint64_t)kRPad * Np)
 		return;
 	uint32_t c = (uint32_t)(t / Np), j = (uint32_t)(t % Np);
+	const uint32_t src = perm[j];
 	float v = 0.0f;
-	if (c < R && j < N) {
-		v = xT[(uint64_t)c * Np + j];
+	if (c < R && src < N) {
+		v = xT[(uint64_t)c * Np + src];
 		if (pre)
-			v = (v * diffuse[3 * j + c % 3]) * area[j];
+			v = (v * diffuse[3 * src + c % 3]) * area[src];
 		v *= cscale[c];
 	}
 	uint16_t h = Prec<P>::cvt(v);
@@ -372,10 +370,9 @@ struct GemmShape {
 	static constexpr int kOps = NPL * (kColTiles / 4 + RT);     // DMA instructions per wave per stage
 };
 
-// issue stage st (k offset kofs from the split's start) into LDS buffer base
 // Issue stage st (counted from the split's first stage) into LDS buffer
 // base.  The 16-bit matrix and operand are stored as LDS images (see
-// k_dense_build / k_prep_x16): every DMA instruction copies one contiguous
+// k_tile_scatter / k_prep_x16): every DMA instruction copies one contiguous
 // KB, and a block's stage is one contiguous tile.
 constexpr uint64_t kTileElems = 512;             // 16-bit elements per 1 KB group
 constexpr uint64_t kGStageElems = 16 * kTileElems; // a 256-row stage of the matrix
@@ -638,19 +635,20 @@ __global__ __launch_bounds__(kB) void k_gemm_f32(const float* __restrict__ G, co
 	}
 }
 
-// y[c][i] = (sum_s part[s][c][i]) * unscale_c, fixed order over s
+// y[c][perm[i]] = (sum_s part[s][c][i]) * unscale_c, fixed order over s
+// (perm null: the identity)
 __global__ __launch_bounds__(kB) void k_reduce_split(const float* __restrict__ part, uint32_t ksplit, uint32_t Np,
                                                      uint32_t R, const float* __restrict__ unscale,
-                                                     float* __restrict__ y)
+                                                     const uint32_t* __restrict__ perm, float* __restrict__ y)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t >= (uint64_t)R * Np)
 		return;
-	uint32_t c = (uint32_t)(t / Np);
+	uint32_t c = (uint32_t)(t / Np), i = (uint32_t)(t % Np);
 	float s = 0.0f;
 	for (uint32_t q = 0; q < ksplit; ++q)
 		s += part[(uint64_t)q * kRPad * Np + t];
-	y[t] = s * (unscale ? unscale[c] : 1.0f);
+	y[perm ? (uint64_t)c * Np + perm[i] : t] = s * (unscale ? unscale[c] : 1.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -797,6 +795,8 @@ void RadSolver::release()
 	dfree(tmap_);
 	dfree(toff_);
 	dfree(tks_);
+	dfree(perm_);
+	dfree(pinv_);
 	dfree(E_);
 	dfree(diff_);
 	dfree(area_);
@@ -836,12 +836,26 @@ void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3,
 	HIPCHECK(hipStreamSynchronize(s));
 }
 
-// The 16-bit forms store only the tiles (256 rows x one 32-column k stage)
-// that hold a link: per 256-row block, the ascending list of its stored
-// tiles' k stages (tks_, from toff_[rb]) and, for the build, each tile's
-// place in the store (tmap_, -1 for none).  A room whose patches all see
-// each other stores every tile; a city block, where links stay within tens
-// of metres, a few percent of them.
+// Spread the bits of a 10-bit integer to every third bit.
+static inline uint32_t spread3(uint32_t v)
+{
+	v &= 0x3ffu;
+	v = (v | (v << 16)) & 0x030000ffu;
+	v = (v | (v << 8)) & 0x0300f00fu;
+	v = (v | (v << 4)) & 0x030c30c3u;
+	v = (v | (v << 2)) & 0x09249249u;
+	return v;
+}
+
+// The 16-bit forms multiply H' = Pi H Pi^T, the patches taken along a Morton
+// curve of their positions (ties in index order, the padding last), so that
+// a 256-row block and a 32-column stage are each a compact patch of space.
+// Only the tiles (256 rows x one 32-column k stage) that hold a link are
+// stored: per 256-row block, the ascending list of its stored tiles' k
+// stages (tks_, from toff_[rb]) and, for the build, each tile's place in the
+// store (tmap_, -1 for none).  Links reach some metres (f >= 0.001), so
+// where patches are spread over space most tiles are empty; a small room
+// whose patches all see each other stores nearly every tile.
 void RadSolver::build_tiles()
 {
 	if (toff_)
@@ -849,8 +863,44 @@ void RadSolver::build_tiles()
 	hipStream_t s = (hipStream_t)s_;
 	const uint32_t nrb = Np / 256, nks = Np / 32;
 	const uint64_t n = (uint64_t)nrb * nks;
+	{
+		std::vector<float> pos(3 * (size_t)N);
+		if (N)
+			HIPCHECK(hipMemcpyAsync(pos.data(), dev.rad.pos, 12 * (size_t)N, hipMemcpyDeviceToHost, s));
+		HIPCHECK(hipStreamSynchronize(s));
+		float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+		for (uint32_t i = 0; i < N; ++i)
+			for (int a = 0; a < 3; ++a) {
+				mn[a] = fminf(mn[a], pos[3 * i + a]);
+				mx[a] = fmaxf(mx[a], pos[3 * i + a]);
+			}
+		const float ext = fmaxf(fmaxf(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]);
+		const float q = ext > 0.0f ? 1023.0f / ext : 0.0f;
+		std::vector<uint64_t> key(N);
+		for (uint32_t i = 0; i < N; ++i) {
+			uint32_t c = 0;
+			for (int a = 0; a < 3; ++a) {
+				float f = (pos[3 * i + a] - mn[a]) * q;
+				uint32_t u = f > 0.0f ? (uint32_t)fminf(f, 1023.0f) : 0u; // NaN -> 0
+				c |= spread3(u) << a;
+			}
+			key[i] = ((uint64_t)c << 32) | i;
+		}
+		std::sort(key.begin(), key.end());
+		std::vector<uint32_t> perm(Np), pinv(Np);
+		for (uint32_t p = 0; p < Np; ++p)
+			perm[p] = p < N ? (uint32_t)key[p] : p;
+		for (uint32_t p = 0; p < Np; ++p)
+			pinv[perm[p]] = p;
+		perm_ = dalloc<uint32_t>(Np);
+		pinv_ = dalloc<uint32_t>(Np);
+		HIPCHECK(hipMemcpyAsync(perm_, perm.data(), 4 * (size_t)Np, hipMemcpyHostToDevice, s));
+		HIPCHECK(hipMemcpyAsync(pinv_, pinv.data(), 4 * (size_t)Np, hipMemcpyHostToDevice, s));
+		HIPCHECK(hipStreamSynchronize(s));
+	}
 	uint8_t* df = dalloc<uint8_t>(n);
-	k_tile_flags<<<gridn(n), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, df);
+	HIPCHECK(hipMemsetAsync(df, 0, n, s));
+	k_tile_flags<<<gridn(N, kB / 64), kB, 0, s>>>(N, Np, dev.rad.rowptr, dev.rad.col, pinv_, df);
 	HIPCHECK(hipGetLastError());
 	std::vector<uint8_t> f(n);
 	HIPCHECK(hipMemcpyAsync(f.data(), df, n, hipMemcpyDeviceToHost, s));
@@ -885,14 +935,16 @@ void RadSolver::build_dense(int form)
 	if (form == kDenseF32) {
 		if (!g32_)
 			g32_ = dalloc<float>((uint64_t)Np * Np);
-		k_dense_build<-1><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-		                                                dev.rad.nrm, 1.0f, nullptr, g32_, nullptr);
+		k_dense_build<<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos, dev.rad.nrm,
+		                                            g32_);
 	} else {
 		build_tiles();
 		if (!g16_)
 			g16_ = dalloc<uint16_t>(ntiles_ * kGStageElems);
 		if (!g16lo_)
 			g16lo_ = dalloc<uint16_t>(ntiles_ * kGStageElems);
+		HIPCHECK(hipMemsetAsync(g16_, 0, 2 * ntiles_ * kGStageElems, s));
+		HIPCHECK(hipMemsetAsync(g16lo_, 0, 2 * ntiles_ * kGStageElems, s));
 		gscale_ = 1.0f;
 		if (form == kDenseF16) {
 			// scale so that the largest factor sits near 2^13: small factors
@@ -907,11 +959,11 @@ void RadSolver::build_dense(int form)
 			float mx;
 			memcpy(&mx, &hm, 4);
 			gscale_ = mx > 0 ? ldexpf(1.0f, 14 - ilogbf(mx) - 1) : 1.0f;
-			k_dense_build<1><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-			                                               dev.rad.nrm, gscale_, tmap_, g16_, g16lo_);
+			k_tile_scatter<1><<<gridn(N, kB / 64), kB, 0, s>>>(N, Np, dev.rad.rowptr, dev.rad.col, dev.rad.val,
+			                                                   pinv_, tmap_, gscale_, g16_, g16lo_);
 		} else {
-			k_dense_build<0><<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos,
-			                                               dev.rad.nrm, 1.0f, tmap_, g16_, g16lo_);
+			k_tile_scatter<0><<<gridn(N, kB / 64), kB, 0, s>>>(N, Np, dev.rad.rowptr, dev.rad.col, dev.rad.val,
+			                                                   pinv_, tmap_, 1.0f, g16_, g16lo_);
 		}
 	}
 	HIPCHECK(hipGetLastError());
@@ -963,16 +1015,16 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 		dim3 grid(Np / kTileM, ks);
 		k_prep_x32<<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, N, Np, R, pre ? 1 : 0, xq_);
 		k_gemm_f32<<<grid, kB, 0, s>>>(g32_, xq_, Np, ks, part_);
-		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, Np, R, nullptr, yT);
+		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, Np, R, nullptr, nullptr, yT);
 	} else {
 		uint16_t* q = reinterpret_cast<uint16_t*>(xq_);
 		uint16_t* lo = split ? xqlo_ : nullptr;
 		k_col_scale<<<R, kB, 0, s>>>(xT, diff_, area_, N, Np, pre ? 1 : 0, form == kDenseF16 ? 1 : 0, gscale_,
 		                             cs_[which], cu_[which]);
 		if (form == kDenseF16)
-			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cs_[which], N, Np, R, pre ? 1 : 0, q, lo);
+			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cs_[which], perm_, N, Np, R, pre ? 1 : 0, q, lo);
 		else
-			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cs_[which], N, Np, R, pre ? 1 : 0, q, lo);
+			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cs_[which], perm_, N, Np, R, pre ? 1 : 0, q, lo);
 		const uint32_t rows = 64 * (split ? kSplitRT : kPlainRT);
 		const uint32_t ks = ksplit(rows, ncu_);
 		dim3 grid(Np / rows, ks);
@@ -984,7 +1036,7 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 			k_gemm_mfma<0, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, toff_, tks_, Np, ks, part_);
 		else
 			k_gemm_mfma<0, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, toff_, tks_, Np, ks, part_);
-		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, Np, R, cu_[which], yT);
+		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, Np, R, cu_[which], perm_, yT);
 	}
 	HIPCHECK(hipGetLastError());
 }

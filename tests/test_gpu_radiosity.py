@@ -106,15 +106,23 @@ def test_full_width_192_columns(lib, orc, links):
         assert np.linalg.norm(got - exp) / np.linalg.norm(exp) < tol, form
 
 
-def tile_fraction(N, rowmap, other):
-    """Fraction of the 256 x 32 tiles of the symmetric link matrix that hold
-    a link (the tiles the 16-bit forms store and read)."""
+def stored_tiles(lib, dev):
+    out = (C.c_uint64 * 2)()
+    _lib.check(lib.lgx_RadTiles(dev.h, out), lib)
+    return int(out[0]), int(out[1])
+
+
+def tiles_holding_links(N, rowmap, other, order):
+    """256 x 32 tiles of the symmetric link matrix that hold a link, with the
+    patches at positions order (position -> patch)."""
     Np = max(512, -(-N // 256) * 256)
-    i = np.repeat(np.arange(N, dtype=np.int64), rowmap[1::2].astype(np.int64))
-    j = other.astype(np.int64)
+    pos = np.empty(N, np.int64)
+    pos[order] = np.arange(N)
+    i = pos[np.repeat(np.arange(N, dtype=np.int64), rowmap[1::2].astype(np.int64))]
+    j = pos[other.astype(np.int64)]
     nks = Np // 32
     t = np.concatenate([(i >> 8) * nks + (j >> 5), (j >> 8) * nks + (i >> 5)])
-    return len(np.unique(t)) / ((Np // 256) * nks)
+    return len(np.unique(t)), (Np // 256) * nks
 
 
 @pytest.mark.parametrize("form", [3, 4])
@@ -124,8 +132,6 @@ def test_tile_skipping_on_a_city_block(lib, orc, city_links, form):
     the dense tolerances (and the product equals the FP32 form's to the
     16-bit rounding)."""
     dev, N, rowmap, other, fac = city_links
-    frac = tile_fraction(N, rowmap, other)
-    assert 0.0 < frac < 0.5, frac
     rng = np.random.default_rng(50 + form)
     R = 24
     diff = rng.uniform(0.2, 0.9, (N, 3)).astype(np.float32)
@@ -138,6 +144,10 @@ def test_tile_skipping_on_a_city_block(lib, orc, city_links, form):
     got, _ = solve(lib, dev, E, diff, area, form, bounces)
     rel = np.linalg.norm(got - exp) / np.linalg.norm(exp)
     assert rel < TOL[form], (form, rel)
+    stored, total = stored_tiles(lib, dev)
+    # the Morton order stores fewer tiles than the patches' own order would
+    unordered, total2 = tiles_holding_links(N, rowmap, other, np.arange(N))
+    assert total == total2 and 0 < stored < unordered and stored < 0.5 * total, (stored, unordered, total)
     got1, _ = solve(lib, dev, E, diff, area, form, bounces, refine=1)
     rel1 = np.linalg.norm(got1 - exp) / np.linalg.norm(exp)
     assert rel1 < TOL_REFINED[form], (form, rel, rel1)

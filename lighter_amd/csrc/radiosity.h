@@ -43,24 +43,26 @@ public:
 	double build_ms = 0;
 	uint32_t N = 0, Np = 0, R = 0;
 	uint64_t ntiles() const { return ntiles_; } // stored 256 x 32 tiles of the 16-bit forms
-	bool ntiles_built() const { return toff_ != nullptr; }
+	bool ntiles_built() const { return tiles_built_; }
 
 private:
 	void release();
 	void build_tiles();
 	void ensure_work();
 	uint32_t ksplit(uint32_t rows_per_block, uint32_t min_blocks) const;
-	void apply_dense(int form, const float* xT, float* yT, bool split, bool pre, int which);
-	void apply_op(int form, const double* x, double* y, bool pre, int which);
+	void apply_dense(int form, const float* xT, float* yT, bool split, bool pre);
+	void apply_op(int form, const double* x, double* y, bool pre);
 	DeviceScene& dev;
 	void* s_ = nullptr;
 	float* g32_ = nullptr;
 	uint16_t* g16_ = nullptr;   // 16-bit matrix (hi plane)
 	uint16_t* g16lo_ = nullptr; // remainder plane for the compensated product
 	int32_t* tmap_ = nullptr;   // tile (256-row block, k stage) -> place in the 16-bit store, or -1
-	uint32_t* toff_ = nullptr;  // per 256-row block: first entry of its tiles in tks_
-	uint32_t* tks_ = nullptr;   // k stage of each stored tile
-	uint64_t ntiles_ = 0;
+	uint32_t* tks_ = nullptr;   // k stage of each stored tile (row blocks in order, stages ascending)
+	uint32_t* chunks_ = nullptr; // work runs {row block, first tile, end tile, slot}
+	uint32_t* nslot_ = nullptr;  // runs (partial-sum slots) per row block
+	uint64_t ntiles_ = 0, nchunks_ = 0;
+	bool tiles_built_ = false;
 	uint32_t* perm_ = nullptr; // position -> patch (Morton order, padding last)
 	uint32_t* pinv_ = nullptr; // patch -> position
 	int built_form_ = -1;
@@ -77,8 +79,7 @@ private:
 	float* yT_ = nullptr;
 	float* rm32a_ = nullptr;
 	float* rm32b_ = nullptr;
-	float* cs_[2] = {nullptr, nullptr}; // column scales of the 16-bit operand, set per product on the device
-	float* cu_[2] = {nullptr, nullptr}; // their unscales (times the matrix scale)
+	unsigned* cmax_ = nullptr; // column maxima of the FP16 operand (its scales), per product
 };
 
 } // namespace lgx

@@ -213,14 +213,53 @@ __global__ __launch_bounds__(kB) void k_max_link(uint64_t nnz, const float* __re
 }
 
 // ---------------------------------------------------------------------------
-// right-hand-side preparation: Xq[c][j'] = (x[c][j] * diffuse[j][c % 3]) *
-// area_j * s_c for K = H diag(area) diag(diffuse_c) (pre), or x[c][j] * s_c
-// for the bare H (the adjoint sweep), at position j' of patch j = perm[j'];
-// s_c is a power-of-two column scale
+// Power-of-two column scale of the FP16 operand from its column maximum:
+// each column's largest entry lands near 2^13, so neither growth of the
+// iterates nor small values leave FP16's range.  BF16 has FP32's exponent
+// range and keeps scale 1 (cmax null).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float col_scale_of(const unsigned* cmax, uint32_t c)
+{
+	if (!cmax)
+		return 1.0f;
+	const float m = __uint_as_float(cmax[c]);
+	return (m > 0.0f && m < INFINITY) ? ldexpf(1.0f, 14 - ilogbf(m) - 1) : 1.0f;
+}
+
+// Column maxima of the operand before scaling: block (x, c) takes positions
+// x * 2048 .. of column c.  A maximum does not depend on the order of the
+// atomics, so the result is deterministic; cmax starts at zero.
+__global__ __launch_bounds__(kB) void k_col_max(const float* __restrict__ xT, const float* __restrict__ diffuse,
+                                                const float* __restrict__ area, const uint32_t* __restrict__ perm,
+                                                uint32_t N, uint32_t Np, int pre, unsigned* __restrict__ cmax)
+{
+	const uint32_t c = blockIdx.y, p0 = blockIdx.x * 2048, p1 = min(Np, p0 + 2048);
+	float m = 0.0f;
+	for (uint32_t p = p0 + threadIdx.x; p < p1; p += kB) {
+		const uint32_t src = perm[p];
+		if (src < N) {
+			float v = xT[(uint64_t)c * Np + p];
+			if (pre)
+				v = (v * diffuse[3 * src + c % 3]) * area[src];
+			m = fmaxf(m, fabsf(v));
+		}
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		m = fmaxf(m, __shfl_xor(m, o));
+	if ((threadIdx.x & 63) == 0)
+		atomicMax(cmax + c, __float_as_uint(m)); // non-negative floats order like their bits
+}
+
+// ---------------------------------------------------------------------------
+// right-hand-side preparation at position j (patch perm[j], Morton order):
+// Xq[c][j] = (x[c][j] * diffuse[perm[j]][c % 3]) * area[perm[j]] * s_c for
+// K = H diag(area) diag(diffuse_c) (pre), or x[c][j] * s_c for the bare H
+// (the adjoint sweep); s_c is the column scale
 // ---------------------------------------------------------------------------
 template <int P>
 __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, const float* __restrict__ diffuse,
-                                                 const float* __restrict__ area, const float* __restrict__ cscale,
+                                                 const float* __restrict__ area, const unsigned* __restrict__ cmax,
                                                  const uint32_t* __restrict__ perm, uint32_t N, uint32_t Np,
                                                  uint32_t R, int pre, uint16_t* __restrict__ xq,
                                                  uint16_t* __restrict__ xlo)
@@ -232,10 +271,10 @@ __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, c
 	const uint32_t src = perm[j];
 	float v = 0.0f;
 	if (c < R && src < N) {
-		v = xT[(uint64_t)c * Np + src];
+		v = xT[(uint64_t)c * Np + j];
 		if (pre)
 			v = (v * diffuse[3 * src + c % 3]) * area[src];
-		v *= cscale[c];
+		v *= col_scale_of(cmax, c);
 	}
 	uint16_t h = Prec<P>::cvt(v);
 	// LDS-image layout: stage j / 32, column group c / 16, row c % 16, piece
@@ -248,42 +287,8 @@ __global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, c
 		xlo[o] = Prec<P>::cvt(v - from16<P>(h));
 }
 
-// Per-column power-of-two scale of the 16-bit operand, from the operand
-// itself (one block per column): FP16 (p16) puts each column's largest
-// entry near 2^13, so neither growth of the iterates nor small values leave
-// its range; BF16 has FP32's exponent range and keeps scale 1.  unscale =
-// 1 / (scale * matrix scale) for the product's reduction.
-__global__ __launch_bounds__(kB) void k_col_scale(const float* __restrict__ xT, const float* __restrict__ diffuse,
-                                                  const float* __restrict__ area, uint32_t N, uint32_t Np, int pre,
-                                                  int p16, float gscale, float* __restrict__ cscale,
-                                                  float* __restrict__ cunscale)
-{
-	const uint32_t c = blockIdx.x;
-	float m = 0.0f;
-	if (p16) {
-		for (uint32_t j = threadIdx.x; j < N; j += kB) {
-			float v = xT[(uint64_t)c * Np + j];
-			if (pre)
-				v = (v * diffuse[3 * j + c % 3]) * area[j];
-			m = fmaxf(m, fabsf(v));
-		}
-#pragma unroll
-		for (int o = 32; o > 0; o >>= 1)
-			m = fmaxf(m, __shfl_xor(m, o));
-	}
-	__shared__ float red[kB / 64];
-	if ((threadIdx.x & 63) == 0)
-		red[threadIdx.x >> 6] = m;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		for (int q = 1; q < kB / 64; ++q)
-			m = fmaxf(m, red[q]);
-		float e = (m > 0.0f && m < INFINITY) ? ldexpf(1.0f, 14 - ilogbf(m) - 1) : 1.0f;
-		cscale[c] = e;
-		cunscale[c] = 1.0f / (e * gscale);
-	}
-}
-
+// FP32 operand of the dense FP32 form, patch order: Xq[c][j] =
+// (x[c][j] * diffuse[j][c % 3]) * area_j (pre) or x[c][j]
 __global__ __launch_bounds__(kB) void k_prep_x32(const float* __restrict__ xT, const float* __restrict__ diffuse,
                                                  const float* __restrict__ area, uint32_t N, uint32_t Np, uint32_t R,
                                                  int pre, float* __restrict__ xq)
@@ -486,14 +491,15 @@ __device__ __forceinline__ void gemm_tail(int s, int nst, const GemmLane<NPL, RT
 	((s + Bs < nst ? gemm_step<P, NPL, RT, PD, Bs>(s + Bs, nst, L, G, Glo, xq, xlo, Np, acc) : void()), ...);
 }
 
-// y = G x over the stored tiles: block (x, y) takes rows x * kRows ..
-// and split y of its 256-row block's tile list (toff[rb] .. toff[rb + 1] in
-// tks, the k stage of each stored tile, ascending)
+// y = G x over the stored tiles.  A chunk {rb, tbeg, tend, slot} is a run of
+// 256-row block rb's stored tiles (tks[t]: the k stage of tile t); blocks
+// 256 / kRows at a time take its rows and write their partial sums to the
+// slot's plane of part.
 template <int P, int NPL, int RT, int PD>
 __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G, const uint16_t* __restrict__ Glo,
                                                   const uint16_t* __restrict__ xq, const uint16_t* __restrict__ xlo,
-                                                  const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tks,
-                                                  uint32_t Np, uint32_t ksplit, float* __restrict__ part)
+                                                  const uint4* __restrict__ chunks, const uint32_t* __restrict__ tks,
+                                                  uint32_t Np, float* __restrict__ part)
 {
 	typedef GemmShape<NPL, RT> S;
 	static_assert(PD >= 2, "pipeline depth");
@@ -513,19 +519,18 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 	L.buf[4] = lds4;
 	L.buf[5] = lds5;
 	L.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-	const uint32_t i0 = blockIdx.x * S::kRows + L.wave * 16 * RT; // this wave's first row
-	const uint32_t row0 = blockIdx.x * S::kRows;                    // the block's first row
-	const uint32_t rb = row0 >> 8, gbase = (row0 & 255) >> 4;
-	const uint32_t t0 = toff[rb], cnt = toff[rb + 1] - t0;
-	const uint32_t pbeg = (uint32_t)((uint64_t)cnt * blockIdx.y / ksplit);
-	const uint32_t pend = (uint32_t)((uint64_t)cnt * (blockIdx.y + 1) / ksplit);
-	const int nst = (int)(pend - pbeg);
+	constexpr uint32_t kBPC = 256 / S::kRows; // blocks per chunk
+	const uint4 ch = chunks[blockIdx.x / kBPC];
+	const uint32_t row0 = ch.x * 256 + (blockIdx.x % kBPC) * S::kRows; // the block's first row
+	const uint32_t i0 = row0 + L.wave * 16 * RT;                       // this wave's first row
+	const uint32_t gbase = (row0 & 255) >> 4;
+	const int nst = (int)(ch.z - ch.y);
 	// DMA sources in the LDS-image layouts: lane l copies the l-th 16 bytes
 	// of each 1 KB group.  Stored tile t (16 KB) holds group g at
 	// (t * 16 + g) KB; operand stage ks, column group cg at (ks * 12 + cg) KB.
-	L.lst = tks + t0 + pbeg;
+	L.lst = tks + ch.y;
 	L.gstride = kGStageElems;
-	L.gsrc = ((uint64_t)(t0 + pbeg) * 16 + gbase + (uint64_t)L.wave * RT) * kTileElems + 8 * lane;
+	L.gsrc = ((uint64_t)ch.y * 16 + gbase + (uint64_t)L.wave * RT) * kTileElems + 8 * lane;
 	L.xsrc = ((uint64_t)(kColTiles / 4) * L.wave) * kTileElems + 8 * lane;
 	L.rofs = r * 64 + 16 * (g ^ piece_swz(r));
 
@@ -547,8 +552,8 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 		wait_vm<0>(); // no DMA may land after the block has left its LDS
 	}
 
-	// part[s][c][i]: column-major per split
-	float* out = part + (uint64_t)blockIdx.y * kRPad * Np;
+	// part[slot][c][i]: column-major per slot
+	float* out = part + (uint64_t)ch.w * kRPad * Np;
 #pragma unroll
 	for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -562,6 +567,7 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 // the two instantiations: plain (64 rows per wave, 4 stages in flight) and
 // error-compensated hi + lo (32 rows per wave, 3 stages)
 constexpr int kPlainRT = 4, kPlainPD = 4, kSplitRT = 2, kSplitPD = 3;
+constexpr uint32_t kMaxSlots = 16; // partial-sum slots per 256-row block
 
 // ---------------------------------------------------------------------------
 // plain FP32 GEMM with the same blocking: 256 threads, each 8 rows x 12
@@ -635,20 +641,22 @@ __global__ __launch_bounds__(kB) void k_gemm_f32(const float* __restrict__ G, co
 	}
 }
 
-// y[c][perm[i]] = (sum_s part[s][c][i]) * unscale_c, fixed order over s
-// (perm null: the identity)
-__global__ __launch_bounds__(kB) void k_reduce_split(const float* __restrict__ part, uint32_t ksplit, uint32_t Np,
-                                                     uint32_t R, const float* __restrict__ unscale,
-                                                     const uint32_t* __restrict__ perm, float* __restrict__ y)
+// y[c][i] = (sum_q part[q][c][i]) / (s_c * gscale), q in slot order: ksplit
+// slots, or nslot[i / 256] for the chunks of 256-row block i / 256
+__global__ __launch_bounds__(kB) void k_reduce_split(const float* __restrict__ part, uint32_t ksplit,
+                                                     const uint32_t* __restrict__ nslot, uint32_t Np, uint32_t R,
+                                                     const unsigned* __restrict__ cmax, float gscale,
+                                                     float* __restrict__ y)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t >= (uint64_t)R * Np)
 		return;
-	uint32_t c = (uint32_t)(t / Np), i = (uint32_t)(t % Np);
+	const uint32_t c = (uint32_t)(t / Np), i = (uint32_t)(t % Np);
+	const uint32_t ns = nslot ? nslot[i >> 8] : ksplit;
 	float s = 0.0f;
-	for (uint32_t q = 0; q < ksplit; ++q)
+	for (uint32_t q = 0; q < ns; ++q)
 		s += part[(uint64_t)q * kRPad * Np + t];
-	y[perm ? (uint64_t)c * Np + perm[i] : t] = s * (unscale ? unscale[c] : 1.0f);
+	y[t] = s * (1.0f / (col_scale_of(cmax, c) * gscale)); // powers of two: exact
 }
 
 // ---------------------------------------------------------------------------
@@ -683,24 +691,28 @@ __global__ __launch_bounds__(kB) void k_spmm(uint32_t N, uint32_t R, const uint3
 }
 
 // layout changes between the row-major (sparse) and column-major (dense) sets
+// (position p holds patch perm[p]; null perm / pinv: the identity)
 template <class A, class B>
-__global__ void k_to_colmajor(const A* __restrict__ x, uint32_t N, uint32_t Np, uint32_t R, B* __restrict__ xT)
+__global__ void k_to_colmajor(const A* __restrict__ x, uint32_t N, uint32_t Np, uint32_t R,
+                              const uint32_t* __restrict__ perm, B* __restrict__ xT)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t >= (uint64_t)R * Np)
 		return;
-	uint32_t c = (uint32_t)(t / Np), j = (uint32_t)(t % Np);
+	uint32_t c = (uint32_t)(t / Np), p = (uint32_t)(t % Np);
+	uint32_t j = perm ? perm[p] : p;
 	xT[t] = j < N ? (B)x[(uint64_t)j * R + c] : (B)0;
 }
 
 template <class A, class B>
-__global__ void k_to_rowmajor(const A* __restrict__ xT, uint32_t N, uint32_t Np, uint32_t R, B* __restrict__ x)
+__global__ void k_to_rowmajor(const A* __restrict__ xT, uint32_t N, uint32_t Np, uint32_t R,
+                              const uint32_t* __restrict__ pinv, B* __restrict__ x)
 {
 	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
 	if (t >= (uint64_t)N * R)
 		return;
 	uint32_t j = (uint32_t)(t / R), c = (uint32_t)(t % R);
-	x[t] = (B)xT[(uint64_t)c * Np + j];
+	x[t] = (B)xT[(uint64_t)c * Np + (pinv ? pinv[j] : j)];
 }
 
 template <class A, class B> __global__ void k_copy_cast(uint64_t n, const A* __restrict__ x, B* __restrict__ y)
@@ -793,8 +805,9 @@ void RadSolver::release()
 	dfree(g16_);
 	dfree(g16lo_);
 	dfree(tmap_);
-	dfree(toff_);
 	dfree(tks_);
+	dfree(chunks_);
+	dfree(nslot_);
 	dfree(perm_);
 	dfree(pinv_);
 	dfree(E_);
@@ -807,10 +820,7 @@ void RadSolver::release()
 	dfree(yT_);
 	dfree(rm32a_);
 	dfree(rm32b_);
-	for (int k = 0; k < 2; ++k) {
-		dfree(cs_[k]);
-		dfree(cu_[k]);
-	}
+	dfree(cmax_);
 }
 
 void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3, const float* area)
@@ -858,7 +868,7 @@ static inline uint32_t spread3(uint32_t v)
 // whose patches all see each other stores nearly every tile.
 void RadSolver::build_tiles()
 {
-	if (toff_)
+	if (tiles_built_)
 		return;
 	hipStream_t s = (hipStream_t)s_;
 	const uint32_t nrb = Np / 256, nks = Np / 32;
@@ -908,6 +918,7 @@ void RadSolver::build_tiles()
 	dfree(df);
 	std::vector<int32_t> map(n, -1);
 	std::vector<uint32_t> off(nrb + 1, 0), ks;
+	uint32_t maxcnt = 0;
 	for (uint32_t rb = 0; rb < nrb; ++rb) {
 		for (uint32_t w = 0; w < nks; ++w)
 			if (f[(uint64_t)rb * nks + w]) {
@@ -915,16 +926,40 @@ void RadSolver::build_tiles()
 				ks.push_back(w);
 			}
 		off[rb + 1] = (uint32_t)ks.size();
+		maxcnt = std::max(maxcnt, off[rb + 1] - off[rb]);
 	}
 	ntiles_ = ks.size();
+	// Work chunks: a row block's list in runs of about ntiles / (2 CUs)
+	// tiles (at most kMaxSlots runs per block), so that the blocks with the
+	// most links do not set the product's time; each run writes its own
+	// partial-sum slot.
+	const uint64_t run = std::max<uint64_t>(
+	    {8, (ntiles_ + 2 * ncu_ - 1) / (2 * ncu_), ((uint64_t)maxcnt + kMaxSlots - 1) / kMaxSlots});
+	std::vector<uint32_t> ch, nsl(nrb, 0);
+	for (uint32_t rb = 0; rb < nrb; ++rb) {
+		const uint64_t cnt = off[rb + 1] - off[rb];
+		const uint32_t nch = (uint32_t)((cnt + run - 1) / run);
+		for (uint32_t q = 0; q < nch; ++q) {
+			ch.push_back(rb);
+			ch.push_back(off[rb] + (uint32_t)(cnt * q / nch));
+			ch.push_back(off[rb] + (uint32_t)(cnt * (q + 1) / nch));
+			ch.push_back(q);
+		}
+		nsl[rb] = nch;
+	}
+	nchunks_ = ch.size() / 4;
 	tmap_ = dalloc<int32_t>(n);
-	toff_ = dalloc<uint32_t>(nrb + 1);
 	tks_ = dalloc<uint32_t>(ntiles_);
+	chunks_ = dalloc<uint32_t>(ch.size());
+	nslot_ = dalloc<uint32_t>(nrb);
 	HIPCHECK(hipMemcpyAsync(tmap_, map.data(), 4 * n, hipMemcpyHostToDevice, s));
-	HIPCHECK(hipMemcpyAsync(toff_, off.data(), 4 * (nrb + 1), hipMemcpyHostToDevice, s));
 	if (ntiles_)
 		HIPCHECK(hipMemcpyAsync(tks_, ks.data(), 4 * ntiles_, hipMemcpyHostToDevice, s));
+	if (nchunks_)
+		HIPCHECK(hipMemcpyAsync(chunks_, ch.data(), 4 * ch.size(), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(nslot_, nsl.data(), 4 * (size_t)nrb, hipMemcpyHostToDevice, s));
 	HIPCHECK(hipStreamSynchronize(s));
+	tiles_built_ = true;
 }
 
 void RadSolver::build_dense(int form)
@@ -985,17 +1020,13 @@ void RadSolver::ensure_work()
 {
 	const uint64_t n = (uint64_t)kRPad * Np;
 	if (!part_) {
-		const uint32_t kmax =
-		    std::max(ksplit(kTileM, 2048), std::max(ksplit(64 * kPlainRT, ncu_), ksplit(64 * kSplitRT, ncu_)));
+		const uint32_t kmax = std::max(ksplit(kTileM, 2048), kMaxSlots);
 		part_ = dalloc<float>((uint64_t)kmax * n);
 		xq_ = dalloc<float>(n); // FP32 operand, or the 16-bit one in its first half
 		xqlo_ = dalloc<uint16_t>(n);
 		xT_ = dalloc<float>(n);
 		yT_ = dalloc<float>(n);
-		for (int k = 0; k < 2; ++k) {
-			cs_[k] = dalloc<float>(kRPad);
-			cu_[k] = dalloc<float>(kRPad);
-		}
+		cmax_ = dalloc<unsigned>(kRPad);
 	}
 	if (!rm32a_) {
 		rm32a_ = dalloc<float>((uint64_t)Np * R);
@@ -1003,10 +1034,11 @@ void RadSolver::ensure_work()
 	}
 }
 
-// y (column-major, R x Np, FP32) = K x (pre) or H x for the dense forms.
+// y (column-major, R x Np, FP32) = K x (pre) or H x for the dense forms; the
+// 16-bit forms take and give their vectors in Morton order (position p holds
+// patch perm_[p]), the FP32 form in patch order.
 // split: the error-compensated product of the 16-bit forms (hi + lo planes).
-// which: the column-scale set (0: forward operand, 1: adjoint operand).
-void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bool pre, int which)
+void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bool pre)
 {
 	hipStream_t s = (hipStream_t)s_;
 	uint64_t nq = (uint64_t)kRPad * Np;
@@ -1015,34 +1047,41 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 		dim3 grid(Np / kTileM, ks);
 		k_prep_x32<<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, N, Np, R, pre ? 1 : 0, xq_);
 		k_gemm_f32<<<grid, kB, 0, s>>>(g32_, xq_, Np, ks, part_);
-		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, Np, R, nullptr, nullptr, yT);
+		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, nullptr, Np, R, nullptr, 1.0f, yT);
 	} else {
 		uint16_t* q = reinterpret_cast<uint16_t*>(xq_);
 		uint16_t* lo = split ? xqlo_ : nullptr;
-		k_col_scale<<<R, kB, 0, s>>>(xT, diff_, area_, N, Np, pre ? 1 : 0, form == kDenseF16 ? 1 : 0, gscale_,
-		                             cs_[which], cu_[which]);
-		if (form == kDenseF16)
-			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cs_[which], perm_, N, Np, R, pre ? 1 : 0, q, lo);
-		else
-			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cs_[which], perm_, N, Np, R, pre ? 1 : 0, q, lo);
-		const uint32_t rows = 64 * (split ? kSplitRT : kPlainRT);
-		const uint32_t ks = ksplit(rows, ncu_);
-		dim3 grid(Np / rows, ks);
-		if (form == kDenseF16 && split)
-			k_gemm_mfma<1, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, toff_, tks_, Np, ks, part_);
-		else if (form == kDenseF16)
-			k_gemm_mfma<1, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, toff_, tks_, Np, ks, part_);
-		else if (split)
-			k_gemm_mfma<0, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, toff_, tks_, Np, ks, part_);
-		else
-			k_gemm_mfma<0, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, toff_, tks_, Np, ks, part_);
-		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, Np, R, cu_[which], perm_, yT);
+		const unsigned* cm = nullptr;
+		if (form == kDenseF16) {
+			HIPCHECK(hipMemsetAsync(cmax_, 0, 4 * kRPad, s));
+			k_col_max<<<dim3(gridn(Np, 2048), R), kB, 0, s>>>(xT, diff_, area_, perm_, N, Np, pre ? 1 : 0, cmax_);
+			cm = cmax_;
+			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cm, perm_, N, Np, R, pre ? 1 : 0, q, lo);
+		} else {
+			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cm, perm_, N, Np, R, pre ? 1 : 0, q, lo);
+		}
+		if (nchunks_) {
+			const uint32_t bpc = 256 / (64 * (split ? kSplitRT : kPlainRT));
+			const dim3 grid((unsigned)(nchunks_ * bpc));
+			const uint4* chs = reinterpret_cast<const uint4*>(chunks_);
+			if (form == kDenseF16 && split)
+				k_gemm_mfma<1, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, chs, tks_, Np, part_);
+			else if (form == kDenseF16)
+				k_gemm_mfma<1, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, chs, tks_, Np,
+				                                                          part_);
+			else if (split)
+				k_gemm_mfma<0, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, chs, tks_, Np, part_);
+			else
+				k_gemm_mfma<0, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, chs, tks_, Np,
+				                                                          part_);
+		}
+		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, 0, nslot_, Np, R, cm, gscale_, yT);
 	}
 	HIPCHECK(hipGetLastError());
 }
 
 // y = K x (pre) or H x, row-major N x R FP64 in and out, in the given form
-void RadSolver::apply_op(int form, const double* x, double* y, bool pre, int which)
+void RadSolver::apply_op(int form, const double* x, double* y, bool pre)
 {
 	hipStream_t s = (hipStream_t)s_;
 	const uint64_t nR = (uint64_t)N * R, nRp = (uint64_t)R * Np;
@@ -1063,9 +1102,10 @@ void RadSolver::apply_op(int form, const double* x, double* y, bool pre, int whi
 			k_spmm<float, false><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, rm32a_, rm32b_);
 		k_copy_cast<float, double><<<gridn(nR), kB, 0, s>>>(nR, rm32b_, y);
 	} else {
-		k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(x, N, Np, R, xT_);
-		apply_dense(form, xT_, yT_, false, pre, which);
-		k_to_rowmajor<float, double><<<gridn(nR), kB, 0, s>>>(yT_, N, Np, R, y);
+		const bool mo = form != kDenseF32; // Morton order
+		k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(x, N, Np, R, mo ? perm_ : nullptr, xT_);
+		apply_dense(form, xT_, yT_, false, pre);
+		k_to_rowmajor<float, double><<<gridn(nR), kB, 0, s>>>(yT_, N, Np, R, mo ? pinv_ : nullptr, y);
 	}
 	HIPCHECK(hipGetLastError());
 }
@@ -1084,7 +1124,7 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		HIPCHECK(hipMemcpyAsync(x, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
 		Ev ev(s);
 		for (int b = 0; b < bounces; ++b) {
-			apply_op(form, x, y, true, 0);
+			apply_op(form, x, y, true);
 			k_axpy<double, double><<<gridn(nR), kB, 0, s>>>(nR, y, tot);
 			std::swap(x, y);
 		}
@@ -1100,11 +1140,14 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		std::vector<float*> z(bounces + 1);
 		for (auto& p : z)
 			p = dalloc<float>(nRp);
-		k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(E_, N, Np, R, z[0]);
+		// the 16-bit forms keep z in Morton order
+		const uint32_t* po = form != kDenseF32 ? perm_ : nullptr;
+		const uint32_t* pi = form != kDenseF32 ? pinv_ : nullptr;
+		k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(E_, N, Np, R, po, z[0]);
 		{
 			Ev ev(s);
 			for (int b = 1; b <= bounces; ++b)
-				apply_dense(form, z[b - 1], z[b], false, true, 0);
+				apply_dense(form, z[b - 1], z[b], false, true);
 			times.series_ms = ev.stop();
 		}
 		times.per_bounce_ms = times.series_ms / std::max(1, bounces);
@@ -1125,9 +1168,9 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 			for (int it = 0; it < refine; ++it) {
 				HIPCHECK(hipMemsetAsync(dprev, 0, 4 * nRp, s));
 				for (int b = 1; b <= bounces; ++b) {
-					apply_dense(form, z[b - 1], kz, half, true, 0);                       // K z_{b-1}
+					apply_dense(form, z[b - 1], kz, half, true);                          // K z_{b-1}
 					k_resid64<<<gridn(nRp), kB, 0, s>>>(nRp, kz, z[b], rho);             // rho_b (FP64)
-					apply_dense(form, dprev, d, false, true, 0);                          // Kdense d_{b-1}
+					apply_dense(form, dprev, d, false, true);                             // Kdense d_{b-1}
 					k_axpy<double, float><<<gridn(nRp), kB, 0, s>>>(nRp, rho, d);        // d_b
 					k_axpy<float, float><<<gridn(nRp), kB, 0, s>>>(nRp, dprev, z[b - 1]); // apply d_{b-1}
 					std::swap(d, dprev);
@@ -1139,7 +1182,7 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		HIPCHECK(hipGetLastError());
 		// total = sum_k z_k in FP64 (z_0 = E exactly)
 		for (int b = 1; b <= bounces; ++b) {
-			k_to_rowmajor<float, double><<<gridn(nR), kB, 0, s>>>(z[b], N, Np, R, zr);
+			k_to_rowmajor<float, double><<<gridn(nR), kB, 0, s>>>(z[b], N, Np, R, pi, zr);
 			k_axpy<double, double><<<gridn(nR), kB, 0, s>>>(nR, zr, tot);
 		}
 		for (auto& p : z)
@@ -1189,7 +1232,7 @@ void RadSolver::solve_grad(int form, int bounces, const double* g, double* total
 	HIPCHECK(hipMemcpyAsync(z, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
 	HIPCHECK(hipMemcpyAsync(tot, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
 	for (int k = 1; k <= B; ++k) {
-		apply_op(form, z, zn, true, 0);
+		apply_op(form, z, zn, true);
 		k_axpy<double, double><<<gridn(nR), kB, 0, s>>>(nR, zn, tot);
 		if (k < B)
 			HIPCHECK(hipMemcpyAsync(S[k], tot, 8 * nR, hipMemcpyDeviceToDevice, s));
@@ -1202,7 +1245,7 @@ void RadSolver::solve_grad(int form, int bounces, const double* g, double* total
 	HIPCHECK(hipMemcpyAsync(lam, ybar, 8 * nR, hipMemcpyDeviceToDevice, s));
 	HIPCHECK(hipMemsetAsync(dalb, 0, 8 * nR, s));
 	for (int k = B; k >= 1; --k) {
-		apply_op(form, ybar, u, false, 1);
+		apply_op(form, ybar, u, false);
 		k_adj_step<<<gridn(nR), kB, 0, s>>>(N, R, u, S[k - 1], diff_, area_, dalb, ybar, lam);
 	}
 	times.refine_ms = ev2.stop(); // the reverse sweep's time

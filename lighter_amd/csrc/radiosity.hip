@@ -520,8 +520,14 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 	L.buf[5] = lds5;
 	L.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	constexpr uint32_t kBPC = 256 / S::kRows; // blocks per chunk
-	const uint4 ch = chunks[blockIdx.x / kBPC];
-	const uint32_t row0 = ch.x * 256 + (blockIdx.x % kBPC) * S::kRows; // the block's first row
+	// XCD-aware order: block b runs on XCD b % 8, so XCD x is given the x-th
+	// contiguous eighth of the work; neighbouring chunks (neighbouring
+	// patches in Morton order, hence overlapping operand stages) then share
+	// that XCD's L2
+	const uint32_t nb = gridDim.x, x = blockIdx.x % 8, per = nb / 8, rem = nb % 8;
+	const uint32_t bid = x * per + (x < rem ? x : rem) + blockIdx.x / 8;
+	const uint4 ch = chunks[bid / kBPC];
+	const uint32_t row0 = ch.x * 256 + (bid % kBPC) * S::kRows; // the block's first row
 	const uint32_t i0 = row0 + L.wave * 16 * RT;                       // this wave's first row
 	const uint32_t gbase = (row0 & 255) >> 4;
 	const int nst = (int)(ch.z - ch.y);

@@ -168,9 +168,16 @@ __device__ __forceinline__ _Float16 f16_down(float x)
 // pushed at more than kLdsLevels of its levels) in a per-lane array.
 constexpr int kLdsLevels = 12;
 
+// LDS-qualified element types: through a generic pointer the stack would be
+// accessed with FLAT instructions, which count against both the vector-memory
+// and the LDS wait counters (every node fetch would then wait for the stack's
+// stores and vice versa)
+typedef __attribute__((address_space(3))) int32_t lds_int32_t;
+typedef __attribute__((address_space(3))) _Float16 lds_half_t;
+
 struct LdsStack {
-	int32_t* ref;  // &refs[0][lane]
-	_Float16* key; // &keys[0][lane]
+	lds_int32_t* ref; // &refs[0][lane]
+	lds_half_t* key;  // &keys[0][lane]
 	int32_t oref[kStackSize - kLdsLevels];
 	float okey[kStackSize - kLdsLevels];
 	__device__ __forceinline__ void put(int i, int32_t r, float k)
@@ -195,8 +202,8 @@ struct LdsStack {
 __device__ __forceinline__ LdsStack lds_stack_of_lane(unsigned char* lds, int depth)
 {
 	LdsStack st;
-	st.ref = reinterpret_cast<int32_t*>(lds) + threadIdx.x;
-	st.key = reinterpret_cast<_Float16*>(lds + 4 * (size_t)kBlock * depth) + threadIdx.x;
+	st.ref = (lds_int32_t*)(reinterpret_cast<int32_t*>(lds) + threadIdx.x);
+	st.key = (lds_half_t*)(reinterpret_cast<_Float16*>(lds + 4 * (size_t)kBlock * depth) + threadIdx.x);
 	return st;
 }
 

@@ -157,3 +157,29 @@ def test_tile_skipping_on_a_city_block(lib, orc, city_links, form):
     lonely[other] = False
     if lonely.any():
         assert np.array_equal(got[lonely], E[lonely])
+
+
+def test_patches_without_links_keep_their_emission(lib):
+    """Coplanar patches facing the same way never link (a <= 0.001): every
+    form returns E itself, and the 16-bit forms store no tile at all (their
+    product has no work and must still write zeros)."""
+    n = 300  # not a multiple of 256
+    g = np.random.default_rng(7)
+    pos = np.zeros((n, 3), np.float32)
+    pos[:, :2] = g.uniform(-5, 5, (n, 2))
+    nrm = np.tile(np.array([0, 0, 1], np.float32), (n, 1))
+    tri = np.array([[-10, -10, -1, 10, -10, -1, 0, 10, -1]], np.float32)
+    dev = Device(lib, tri)
+    try:
+        nl = C.c_size_t(0)
+        _lib.check(lib.lgx_RadLinks(dev.h, P(pos), P(nrm), n, C.byref(nl)), lib)
+        assert nl.value == 0
+        E = g.uniform(0, 1, (n, 5))
+        diff = np.full((n, 3), 0.5, np.float32)
+        area = np.full(n, 0.1, np.float32)
+        for form in range(5):
+            got, _ = solve(lib, dev, E, diff, area, form, 3, refine=1 if form >= 2 else 0)
+            assert np.array_equal(got, E), form
+        assert stored_tiles(lib, dev)[0] == 0
+    finally:
+        dev.close()

@@ -38,7 +38,7 @@ fp = C.POINTER(C.c_float)
 u32p = C.POINTER(C.c_uint32)
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 64
+NODE_BYTES = 112  # 4-wide node: six box planes + child references (seven dwordx4 loads)
 DIST_TRI_BYTES = 160
 TRI_BOX_BYTES = 48  # the vertex quads a distance query's box pre-test reads
 METRIC = "shadow rays/s + s per radiosity bounce @ stated error; GPU plain vs MFMA/adjoint"
@@ -766,7 +766,7 @@ def main():
                                                              marches=int(marches), node_visits=int(nodes),
                                                              triangle_tests=int(ntris))),
                     kernel="k_march", alg_bytes_per_launch=int(alg_bytes),
-                    note="node visits x 64 B + full distance-triangle tests x 160 B + box pre-tests x 48 B + "
+                    note="4-wide node visits x 112 B + full distance-triangle tests x 160 B + box pre-tests x 48 B + "
                          "per-item I/O, counted by an "
                          "instrumented pass; the working set (BVH + triangles) is cache resident")
 

@@ -181,6 +181,88 @@ struct Builder {
 
 } // namespace
 
+namespace {
+
+// Collapse binary node b into a wide node: start from its two children and
+// open the inner candidate of largest surface area until four candidates
+// (or only leaves) remain; then collapse the inner candidates in turn.
+struct Collapser {
+	const std::vector<BvhNode>& bin;
+	std::vector<WideNode>& out;
+	int max_depth = 0;
+
+	struct Cand {
+		float lo[3], hi[3];
+		int32_t ref;
+		float area() const
+		{
+			float d[3];
+			for (int a = 0; a < 3; ++a)
+				d[a] = std::max(0.0f, hi[a] - lo[a]);
+			return d[0] * d[1] + d[1] * d[2] + d[2] * d[0];
+		}
+	};
+
+	static Cand child(const BvhNode& n, int k)
+	{
+		Cand c;
+		if (k == 0) {
+			c.lo[0] = n.lo0x, c.lo[1] = n.lo0y, c.lo[2] = n.lo0z;
+			c.hi[0] = n.hi0x, c.hi[1] = n.hi0y, c.hi[2] = n.hi0z;
+			c.ref = n.c0;
+		} else {
+			c.lo[0] = n.lo1x, c.lo[1] = n.lo1y, c.lo[2] = n.lo1z;
+			c.hi[0] = n.hi1x, c.hi[1] = n.hi1y, c.hi[2] = n.hi1z;
+			c.ref = n.c1;
+		}
+		return c;
+	}
+
+	int32_t collapse(int32_t b, int depth)
+	{
+		max_depth = std::max(max_depth, depth + 1);
+		Cand c[4];
+		int n = 0;
+		c[n++] = child(bin[b], 0);
+		c[n++] = child(bin[b], 1);
+		while (n < 4) {
+			int pick = -1;
+			float best = -1.0f;
+			for (int i = 0; i < n; ++i)
+				if (c[i].ref >= 0 && c[i].area() > best) {
+					best = c[i].area();
+					pick = i;
+				}
+			if (pick < 0)
+				break;
+			const BvhNode& m = bin[c[pick].ref];
+			c[pick] = child(m, 0);
+			c[n++] = child(m, 1);
+		}
+		const int32_t self = (int32_t)out.size();
+		out.push_back(WideNode{});
+		int32_t refs[4];
+		for (int i = 0; i < n; ++i)
+			refs[i] = c[i].ref >= 0 ? collapse(c[i].ref, depth + 1) : c[i].ref;
+		WideNode& w = out[self];
+		for (int i = 0; i < 4; ++i) {
+			// an unused slot, or a box that holds nothing (lo > hi), gets the
+			// far degenerate box and a harmless real leaf
+			const bool use = i < n && c[i].lo[0] <= c[i].hi[0] && c[i].lo[1] <= c[i].hi[1] && c[i].lo[2] <= c[i].hi[2];
+			w.lox[i] = use ? c[i].lo[0] : kFarBox;
+			w.loy[i] = use ? c[i].lo[1] : kFarBox;
+			w.loz[i] = use ? c[i].lo[2] : kFarBox;
+			w.hix[i] = use ? c[i].hi[0] : kFarBox;
+			w.hiy[i] = use ? c[i].hi[1] : kFarBox;
+			w.hiz[i] = use ? c[i].hi[2] : kFarBox;
+			w.child[i] = use ? refs[i] : leaf_ref(0, 1);
+		}
+		return self;
+	}
+};
+
+} // namespace
+
 BvhBuild build_bvh(const float* tris, size_t ntri)
 {
 	Builder B;
@@ -202,6 +284,7 @@ BvhBuild build_bvh(const float* tris, size_t ntri)
 	}
 	BvhBuild out;
 	if (ntri == 0) {
+		out.nodes.clear();
 		out.lo = mk3(0, 0, 0);
 		out.hi = mk3(0, 0, 0);
 		return out;
@@ -217,11 +300,13 @@ BvhBuild build_bvh(const float* tris, size_t ntri)
 	} else {
 		B.build(0, (uint32_t)ntri, 0); // pushes the root first, at index 0
 	}
-	out.nodes = std::move(B.nodes);
+	Collapser W{B.nodes, out.nodes};
+	out.nodes.reserve(B.nodes.size() / 2 + 1);
+	W.collapse(0, 0);
 	out.order = std::move(B.idx);
 	out.lo = mk3(all.lo[0], all.lo[1], all.lo[2]);
 	out.hi = mk3(all.hi[0], all.hi[1], all.hi[2]);
-	out.depth = B.max_depth;
+	out.depth = W.max_depth;
 	return out;
 }
 

@@ -1,12 +1,16 @@
 // Bounding volume hierarchy used by the gfx950 kernels.
 //
-// Binary BVH built on the host with binned SAH, flattened depth-first into
-// 64-byte nodes.  Each node stores the boxes of both children, so one node
-// fetch (four dwordx4 loads) lets a lane order and cull both children.  A
-// child reference >= 0 is an inner node; < 0 encodes a leaf as
-// ~((first_triangle << 4) | (count - 1)), count <= 16.  Triangles are stored
-// in leaf order in two precomputed forms (DistTri for distance queries,
-// RayTri for segment queries).
+// Binary BVH built on the host with binned SAH, then collapsed into 4-wide
+// nodes of 128 bytes flattened depth-first: the four children's boxes as
+// six float4 planes (lo x, lo y, lo z, hi x, hi y, hi z over the children)
+// and their references, so one node fetch (seven dwordx4 loads) lets a lane
+// order and cull four children, and a query visits about half as many
+// nodes as in the binary tree.  A child reference >= 0 is an inner node;
+// < 0 encodes a leaf as ~((first_triangle << 4) | (count - 1)),
+// count <= 16.  Unused child slots hold a degenerate box far outside any
+// scene (never entered by either box test).  Triangles are stored in leaf
+// order in two precomputed forms (DistTri for distance queries, RayTri for
+// segment queries).
 //
 // Queries are order independent (minimum distance, minimum hit parameter,
 // any hit), so this structure gives the same answers as the reference's
@@ -31,6 +35,15 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "node size");
 
+struct alignas(16) WideNode {
+	float lox[4], loy[4], loz[4];
+	float hix[4], hiy[4], hiz[4];
+	int32_t child[4];
+	int32_t pad[4];
+};
+static_assert(sizeof(WideNode) == 128, "wide node size");
+constexpr float kFarBox = 1e37f; // corner of the unused slots' degenerate box
+
 constexpr int kLeafMax = 2; // measured on the terrain workload: 2 beats 1, 4 and 8
 
 LHD int32_t leaf_ref(uint32_t first, uint32_t count) { return ~(int32_t)((first << 4) | (count - 1)); }
@@ -38,10 +51,10 @@ LHD uint32_t leaf_first(int32_t ref) { return ((uint32_t)~ref) >> 4; }
 LHD uint32_t leaf_count(int32_t ref) { return (((uint32_t)~ref) & 15u) + 1u; }
 
 struct BvhBuild {
-	std::vector<BvhNode> nodes;
+	std::vector<WideNode> nodes; // 4-wide, root at 0
 	std::vector<uint32_t> order; // leaf-order position -> input triangle index
 	f3 lo, hi;                   // scene bounds (unpadded)
-	int depth = 0;
+	int depth = 0;               // levels of wide nodes on the longest path
 };
 
 // tris: 9 floats per triangle.

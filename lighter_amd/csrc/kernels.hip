@@ -34,7 +34,7 @@ namespace lgx {
 			                         std::to_string(__LINE__) + ": " #x);                                    \
 	} while (0)
 
-constexpr int kStackSize = 64;
+constexpr int kStackSize = 96; // >= 3 x (wide BVH depth) + 1, checked at build
 constexpr int kBlock = 256;
 constexpr int kNumCounters = 6;
 constexpr uint32_t kSkipBits = 0xFFFFFFFFu; // weight of a texel the reference skips
@@ -42,7 +42,7 @@ constexpr uint32_t kSampleBits = 26;
 constexpr uint32_t kSampleMask = (1u << kSampleBits) - 1;
 
 struct View {
-	const BvhNode* __restrict__ nodes;
+	const WideNode* __restrict__ nodes;
 	const DistTri* __restrict__ dtri;
 	const RayTri* __restrict__ rtri;
 	int has_tris;
@@ -63,20 +63,44 @@ __device__ __forceinline__ float pow_r(float x, float y)
 }
 __device__ __forceinline__ float acos_r(float x) { return (float)acos((double)x); }
 
-struct Node4 {
-	float4 a, b, c;
-	int4 d;
+struct WNode {
+	float4 lx, ly, lz, hx, hy, hz; // the four children's boxes, one plane per float4
+	int4 c;                        // their references
 };
 
-__device__ __forceinline__ Node4 load_node(const BvhNode* nodes, int32_t i)
+__device__ __forceinline__ WNode load_node(const WideNode* nodes, int32_t i)
 {
 	const float4* q = reinterpret_cast<const float4*>(nodes + i);
-	Node4 n;
-	n.a = q[0];
-	n.b = q[1];
-	n.c = q[2];
-	n.d = reinterpret_cast<const int4*>(q)[3];
+	WNode n;
+	n.lx = q[0];
+	n.ly = q[1];
+	n.lz = q[2];
+	n.hx = q[3];
+	n.hy = q[4];
+	n.hz = q[5];
+	n.c = reinterpret_cast<const int4*>(q)[6];
 	return n;
+}
+
+// ascending order of (key, reference) pairs
+__device__ __forceinline__ void cswap(float& ka, int32_t& ra, float& kb, int32_t& rb)
+{
+	const bool sw = kb < ka;
+	const float k = sw ? kb : ka;
+	kb = sw ? ka : kb;
+	ka = k;
+	const int32_t r = sw ? rb : ra;
+	rb = sw ? ra : rb;
+	ra = r;
+}
+
+__device__ __forceinline__ void sort4(float (&k)[4], int32_t (&r)[4])
+{
+	cswap(k[0], r[0], k[1], r[1]);
+	cswap(k[2], r[2], k[3], r[3]);
+	cswap(k[0], r[0], k[2], r[2]);
+	cswap(k[1], r[1], k[3], r[3]);
+	cswap(k[1], r[1], k[2], r[2]);
 }
 
 __device__ __forceinline__ float box_d2(f3 p, float lx, float ly, float lz, float hx, float hy, float hz)
@@ -131,8 +155,10 @@ __device__ __forceinline__ bool tri_box_beyond(f3 p, const DistTri* t, float bes
 }
 
 // Traversal stacks.  The traversals below are nearest-first: at an inner
-// node they push at most the farther child, so the stack holds at most one
-// entry per level of the current path and the BVH's depth bounds it.
+// node they descend into the nearest child and push the other (at most
+// three) children that are still within reach, so the stack holds at most
+// three entries per level of the current path and 3 x the BVH's depth + 1
+// bounds it.
 // Entry = node reference + key (squared box distance, or the segment's entry
 // parameter); a popped entry is skipped when its key exceeds the current
 // bound.
@@ -220,22 +246,24 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 	unsigned nn = 0, nt = 0, ncull = 0;
 	for (;;) {
 		if (ref >= 0) {
-			Node4 n = load_node(V.nodes, ref);
+			WNode n = load_node(V.nodes, ref);
 			nn++;
-			float d0 = box_d2(p, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y);
-			float d1 = box_d2(p, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w);
-			int32_t r0 = n.d.x, r1 = n.d.y;
-			if (d1 < d0) {
-				float t = d0; d0 = d1; d1 = t;
-				int32_t u = r0; r0 = r1; r1 = u;
-			}
+			float d[4] = {box_d2(p, n.lx.x, n.ly.x, n.lz.x, n.hx.x, n.hy.x, n.hz.x),
+			              box_d2(p, n.lx.y, n.ly.y, n.lz.y, n.hx.y, n.hy.y, n.hz.y),
+			              box_d2(p, n.lx.z, n.ly.z, n.lz.z, n.hx.z, n.hy.z, n.hz.z),
+			              box_d2(p, n.lx.w, n.ly.w, n.lz.w, n.hx.w, n.hy.w, n.hz.w)};
+			int32_t r[4] = {n.c.x, n.c.y, n.c.z, n.c.w};
+			sort4(d, r);
 			float lim = best * best;
-			if (d0 <= lim) {
-				if (d1 <= lim) {
-					st.put(sp, r1, d1);
-					sp++;
-				}
-				ref = r0;
+			if (d[0] <= lim) {
+				// farthest first, so that the nearer ones pop first
+				if (d[3] <= lim)
+					st.put(sp++, r[3], d[3]);
+				if (d[2] <= lim)
+					st.put(sp++, r[2], d[2]);
+				if (d[1] <= lim)
+					st.put(sp++, r[1], d[1]);
+				ref = r[0];
 				continue;
 			}
 		} else {
@@ -325,22 +353,23 @@ __device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt, S
 	unsigned nn = 0, nt = 0;
 	for (;;) {
 		if (ref >= 0) {
-			Node4 n = load_node(V.nodes, ref);
+			WNode n = load_node(V.nodes, ref);
 			nn++;
 			float tl = fminf(best, 1.0f);
-			float t0 = slab(s, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tl);
-			float t1 = slab(s, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tl);
-			int32_t r0 = n.d.x, r1 = n.d.y;
-			if (t1 < t0) {
-				float t = t0; t0 = t1; t1 = t;
-				int32_t u = r0; r0 = r1; r1 = u;
-			}
-			if (t0 != INFINITY) {
-				if (t1 != INFINITY) {
-					st.put(sp, r1, t1);
-					sp++;
-				}
-				ref = r0;
+			float t[4] = {slab(s, n.lx.x, n.ly.x, n.lz.x, n.hx.x, n.hy.x, n.hz.x, tl),
+			              slab(s, n.lx.y, n.ly.y, n.lz.y, n.hx.y, n.hy.y, n.hz.y, tl),
+			              slab(s, n.lx.z, n.ly.z, n.lz.z, n.hx.z, n.hy.z, n.hz.z, tl),
+			              slab(s, n.lx.w, n.ly.w, n.lz.w, n.hx.w, n.hy.w, n.hz.w, tl)};
+			int32_t r[4] = {n.c.x, n.c.y, n.c.z, n.c.w};
+			sort4(t, r);
+			if (t[0] != INFINITY) {
+				if (t[3] != INFINITY)
+					st.put(sp++, r[3], t[3]);
+				if (t[2] != INFINITY)
+					st.put(sp++, r[2], t[2]);
+				if (t[1] != INFINITY)
+					st.put(sp++, r[1], t[1]);
+				ref = r[0];
 				continue;
 			}
 		} else {
@@ -1020,7 +1049,7 @@ DeviceScene::DeviceScene()
 // LDS stack levels of the traversal kernels (march, AO, link visibility): a
 // leaf at depth d has at most d entries below it on the stack; at most
 // kLdsLevels levels (18 KB per block) live in LDS
-int DeviceScene::march_stack_depth() const { return std::min(kLdsLevels, bvh_depth_ + 2); }
+int DeviceScene::march_stack_depth() const { return std::min(kLdsLevels, 3 * bvh_depth_ + 1); }
 
 size_t DeviceScene::march_lds_bytes() const { return (size_t)kBlock * march_stack_depth() * 6; }
 
@@ -1076,7 +1105,7 @@ void DeviceScene::free_all()
 static View make_view(void* nodes, void* dtri, void* rtri, size_t ntri, const float* lo, const float* hi)
 {
 	View v;
-	v.nodes = (const BvhNode*)nodes;
+	v.nodes = (const WideNode*)nodes;
 	v.dtri = (const DistTri*)dtri;
 	v.rtri = (const RayTri*)rtri;
 	v.has_tris = ntri > 0;
@@ -1101,7 +1130,7 @@ void DeviceScene::set_triangles(const float* tris, size_t ntri)
 	if (ntri >= (1u << 27))
 		throw std::runtime_error("too many triangles for the leaf encoding");
 	BvhBuild b = build_bvh(tris, ntri);
-	if (b.depth + 2 >= kStackSize)
+	if (3 * b.depth + 1 > kStackSize)
 		throw std::runtime_error("BVH deeper than the traversal stack");
 	bvh_depth_ = b.depth;
 	persist_blocks_ = 0; // the march kernel's LDS stack follows the depth
@@ -1114,11 +1143,11 @@ void DeviceScene::set_triangles(const float* tris, size_t ntri)
 		dt[i] = make_dist_tri(a, c, e);
 		rt[i] = make_ray_tri(a, c, e);
 	}
-	d_nodes_ = dalloc<BvhNode>(nnodes_);
+	d_nodes_ = dalloc<WideNode>(nnodes_);
 	d_dtri_ = dalloc<DistTri>(ntri);
 	d_rtri_ = dalloc<RayTri>(ntri);
 	hipStream_t s = (hipStream_t)stream_;
-	HIPCHECK(hipMemcpyAsync(d_nodes_, b.nodes.data(), nnodes_ * sizeof(BvhNode), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_nodes_, b.nodes.data(), nnodes_ * sizeof(WideNode), hipMemcpyHostToDevice, s));
 	HIPCHECK(hipMemcpyAsync(d_dtri_, dt.data(), ntri * sizeof(DistTri), hipMemcpyHostToDevice, s));
 	HIPCHECK(hipMemcpyAsync(d_rtri_, rt.data(), ntri * sizeof(RayTri), hipMemcpyHostToDevice, s));
 	HIPCHECK(hipStreamSynchronize(s));

@@ -445,7 +445,14 @@ __device__ __forceinline__ float box_exit(const View& V, f3 from, f3 rd)
 //    so the march stops there;
 //  * the distance to the scene is 1-Lipschitz, so the next query's search
 //    radius starts at min(2, h + step + margin) instead of 2: every
-//    triangle closer than that is still found, hence the same minimum.
+//    triangle closer than that is still found, hence the same minimum;
+//  * for k >= 0 a step's distance h matters only below
+//    need = max(1, res m (1 + 2^-20)), m = min(t k, 2): at or beyond it the
+//    step is min(h, 1) = 1 and h / m >= res after rounding (the factor
+//    1 + 2^-20 is far above the two roundings of res m and h / m), so res
+//    keeps its value.  The query radius is therefore min(2, Lipschitz bound,
+//    need); when it was set by need and nothing lies closer, the step is 1,
+//    res is unchanged and the next Lipschitz bound is unknown (2).
 // A march whose t no longer advances (t + step == t, where the reference
 // would loop forever) ends with the current factor.
 struct March {
@@ -481,13 +488,25 @@ __device__ __forceinline__ bool march_step(March& M, const View& V, unsigned lon
 	if (!(M.t < M.maxt) || M.t > M.t_out)
 		return true;
 	f3 p = M.from + M.rd * M.t;
-	float h = q_distance<COUNT>(V, p, M.radius, cnt, stack);
+	const float m = tmin(M.t * M.k, kPenumbraSize);
+	const float need = M.k >= 0.0f ? fmaxf(kPenumbraStep, M.res * m * (1.0f + 0x1p-20f)) : kPenumbraSize;
+	const float R = fminf(M.radius, need);
+	float h = q_distance<COUNT>(V, p, R, cnt, stack);
 	M.steps++;
+	if (need < M.radius && h >= R) {
+		// nothing closer than need (>= 1): step 1, res unchanged
+		float tn = M.t + kPenumbraStep;
+		if (tn == M.t)
+			return true;
+		M.radius = kPenumbraSize;
+		M.t = tn;
+		return false;
+	}
 	if (h < 0.001f) {
 		M.res = 0.0f;
 		return true;
 	}
-	M.res = tmin(M.res, h / tmin(M.t * M.k, kPenumbraSize));
+	M.res = tmin(M.res, h / m);
 	float st = tmin(h, kPenumbraStep);
 	float tn = M.t + st;
 	if (tn == M.t)

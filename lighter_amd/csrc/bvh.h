@@ -27,6 +27,7 @@
 
 namespace lgx {
 
+// binary node of the build (both children's boxes and references)
 struct alignas(16) BvhNode {
 	float lo0x, lo0y, lo0z, hi0x;
 	float hi0y, hi0z, lo1x, lo1y;

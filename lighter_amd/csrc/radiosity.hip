@@ -867,9 +867,9 @@ static inline uint32_t spread3(uint32_t v)
 // curve of their positions (ties in index order, the padding last), so that
 // a 256-row block and a 32-column stage are each a compact patch of space.
 // Only the tiles (256 rows x one 32-column k stage) that hold a link are
-// stored: per 256-row block, the ascending list of its stored tiles' k
-// stages (tks_, from toff_[rb]) and, for the build, each tile's place in the
-// store (tmap_, -1 for none).  Links reach some metres (f >= 0.001), so
+// stored: the k stages of the stored tiles (tks_, row blocks in order,
+// stages ascending), for the build each tile's place in the store (tmap_,
+// -1 for none), and the work runs over them (chunks_, nslot_).  Links reach some metres (f >= 0.001), so
 // where patches are spread over space most tiles are empty; a small room
 // whose patches all see each other stores nearly every tile.
 void RadSolver::build_tiles()

@@ -750,7 +750,9 @@ struct AOParams {
 // small scene still fills the GPU.  The rays' terms are then added in ray
 // order, the reference's summation order, so the sum is the same.
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_ao(View V, const float* __restrict__ pos, const float* __restrict__ nrm,
+// at most 80 VGPRs (6 waves per SIMD instead of 5): -3 % on the terrain's
+// 191 M rays; 8 waves spill and lose
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_ao(View V, const float* __restrict__ pos, const float* __restrict__ nrm,
                                                const float* __restrict__ randoff, const float2* __restrict__ side,
                                                AOParams P, uint32_t S, float* __restrict__ lm,
                                                unsigned long long* cnt, int depth)

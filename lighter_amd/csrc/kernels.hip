@@ -554,17 +554,18 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter)
 // Shading gate of each (light, texel) item (Job_LMRender_*_Inner up to the
 // shadow march, lighter.cpp:485-553, 586): items with a zero shading factor
 // get their final weight here; the others are appended to the march list
-// with the factor the shadow term multiplies.
-__global__ __launch_bounds__(kBlock) void k_direct_gate(const float* __restrict__ pos, const float* __restrict__ nrm,
-                                                        const DevLight* __restrict__ lights,
-                                                        const uint32_t* __restrict__ items, uint32_t nitems,
-                                                        float* __restrict__ gate, float* __restrict__ wout,
-                                                        float* __restrict__ vout, uint32_t* __restrict__ mlist,
-                                                        uint32_t* __restrict__ mcount)
+// with the factor the shadow term multiplies.  A block takes kGateRounds x
+// kBlock consecutive items, collects its marches in LDS and appends them
+// with one atomic (one per wave on a single counter serialised in L2).
+constexpr int kGateRounds = 8;
+
+__device__ __forceinline__ bool gate_item(const float* __restrict__ pos, const float* __restrict__ nrm,
+                                          const DevLight* __restrict__ lights, const uint32_t* __restrict__ items,
+                                          uint32_t k, float* __restrict__ gate, float* __restrict__ wout,
+                                          float* __restrict__ vout)
 {
-	uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
 	bool march = false;
-	if (k < nitems) {
+	{
 		uint32_t code = items[k];
 		const DevLight& L = lights[code >> kSampleBits];
 		uint32_t s = code & kSampleMask;
@@ -607,9 +608,40 @@ __global__ __launch_bounds__(kBlock) void k_direct_gate(const float* __restrict_
 				vout[k] = 1.0f;
 		}
 	}
-	uint32_t slot = wave_append(march, mcount);
-	if (march)
-		mlist[slot] = k;
+	return march;
+}
+
+__global__ __launch_bounds__(kBlock) void k_direct_gate(const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                        const DevLight* __restrict__ lights,
+                                                        const uint32_t* __restrict__ items, uint32_t nitems,
+                                                        float* __restrict__ gate, float* __restrict__ wout,
+                                                        float* __restrict__ vout, uint32_t* __restrict__ mlist,
+                                                        uint32_t* __restrict__ mcount)
+{
+	__shared__ uint32_t list[kGateRounds * kBlock];
+	__shared__ uint32_t n, base;
+	if (threadIdx.x == 0)
+		n = 0;
+	__syncthreads();
+	const int lane = threadIdx.x & 63;
+	const uint64_t k0 = (uint64_t)blockIdx.x * (kGateRounds * kBlock);
+	for (int r = 0; r < kGateRounds; ++r) {
+		const uint64_t k = k0 + (uint64_t)r * kBlock + threadIdx.x;
+		const bool march = k < nitems && gate_item(pos, nrm, lights, items, (uint32_t)k, gate, wout, vout);
+		const uint64_t m = __ballot(march);
+		uint32_t wbase = 0;
+		if (lane == 0 && m)
+			wbase = atomicAdd(&n, (uint32_t)__popcll(m)); // LDS atomic, one per wave
+		wbase = (uint32_t)__shfl((int)wbase, 0);
+		if (march)
+			list[wbase + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)k;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0)
+		base = n ? atomicAdd(mcount, n) : 0u;
+	__syncthreads();
+	for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+		mlist[base + i] = list[i];
 }
 
 // Persistent shadow-march kernel: every lane pulls a new march from the list
@@ -1278,8 +1310,8 @@ void DeviceScene::run_direct()
 	EventTimer all(s);
 	if (nitems_) {
 		HIPCHECK(hipMemsetAsync(d_mcount_, 0, 2 * sizeof(uint32_t), s));
-		k_direct_gate<<<grid_for(nitems_), kBlock, 0, s>>>(d_pos_, d_nrm_, d_lights_, d_items_, (uint32_t)nitems_,
-		                                                   d_gate_, d_w_, d_vis_, d_mlist_, d_mcount_);
+		k_direct_gate<<<(unsigned)((nitems_ + kGateRounds * kBlock - 1) / (kGateRounds * kBlock)), kBlock, 0, s>>>(
+		    d_pos_, d_nrm_, d_lights_, d_items_, (uint32_t)nitems_, d_gate_, d_w_, d_vis_, d_mlist_, d_mcount_);
 		HIPCHECK(hipGetLastError());
 		uint32_t nmarch = 0;
 		HIPCHECK(hipMemcpyAsync(&nmarch, d_mcount_, 4, hipMemcpyDeviceToHost, s));

@@ -558,6 +558,7 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter)
 // kBlock consecutive items, collects its marches in LDS and appends them
 // with one atomic (one per wave on a single counter serialised in L2).
 constexpr int kGateRounds = 8;
+constexpr uint32_t kMarchBatch = 64; // march-list positions a wave claims at a time
 
 __device__ __forceinline__ bool gate_item(const float* __restrict__ pos, const float* __restrict__ nrm,
                                           const DevLight* __restrict__ lights, const uint32_t* __restrict__ items,
@@ -667,10 +668,32 @@ __global__ __launch_bounds__(kBlock) void k_march(View V, const float* __restric
 	bool active = false, done = false;
 	uint32_t k = 0;
 	March M;
+	// the wave's reservation [rbeg, rend) of march-list positions: refills
+	// take from it and claim kMarchBatch more with one atomic when it runs
+	// short, so a wave keeps working on neighbouring items (one texel region
+	// and light) and the list counter sees one atomic per kMarchBatch marches
+	const int lane = threadIdx.x & 63;
+	uint32_t rbeg = 0, rend = 0;
 	for (;;) {
 		bool need = !active && !done;
-		if (__ballot(need)) {
-			uint32_t idx = wave_append(need, next);
+		const uint64_t nm = __ballot(need);
+		if (nm) {
+			const uint32_t cnt_need = (uint32_t)__popcll(nm);
+			const uint32_t rank = (uint32_t)__popcll(nm & ((1ull << lane) - 1ull));
+			const uint32_t avail = rend - rbeg;
+			uint32_t idx;
+			if (avail >= cnt_need) {
+				idx = rbeg + rank;
+				rbeg += cnt_need;
+			} else {
+				uint32_t nb = 0;
+				if (lane == 0)
+					nb = atomicAdd(next, kMarchBatch);
+				nb = (uint32_t)__shfl((int)nb, 0);
+				idx = rank < avail ? rbeg + rank : nb + (rank - avail);
+				rbeg = nb + (cnt_need - avail);
+				rend = nb + kMarchBatch;
+			}
 			if (need) {
 				if (idx < nmarch) {
 					k = mlist[idx];

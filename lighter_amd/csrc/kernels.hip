@@ -559,6 +559,7 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter)
 // with one atomic (one per wave on a single counter serialised in L2).
 constexpr int kGateRounds = 8;
 constexpr uint32_t kMarchBatch = 64; // march-list positions a wave claims at a time
+static_assert(kMarchBatch >= 64, "a refill of all 64 lanes must fit in one claim");
 
 __device__ __forceinline__ bool gate_item(const float* __restrict__ pos, const float* __restrict__ nrm,
                                           const DevLight* __restrict__ lights, const uint32_t* __restrict__ items,

@@ -138,6 +138,7 @@ private:
 	float lo_[3] = {0, 0, 0}, hi_[3] = {0, 0, 0};
 	// samples
 	float* d_pos_ = nullptr;
+	uint32_t* d_skey_ = nullptr; // 30-bit Morton code of each sample position (march ordering)
 	float* d_nrm_ = nullptr;
 	float* d_lm_ = nullptr;
 	size_t nsamples_ = 0;

@@ -635,6 +635,10 @@ int lgx_RadSolve(lgx_Device* d, const double* E, uint32_t R, const float* diffus
 			throw std::runtime_error("lgx_RadSolve: unknown form");
 		if (bounces < 0 || refine < 0)
 			throw std::runtime_error("lgx_RadSolve: negative count");
+		if (times_ms)
+			times_ms[0] = times_ms[1] = times_ms[2] = times_ms[3] = 0.0;
+		if (d->dev.npatches == 0)
+			return; // no patches: nothing to solve
 		if (!d->rad || d->rad->N != d->dev.npatches)
 			d->rad.reset(new RadSolver(d->dev));
 		d->rad->set_problem(E, R, diffuse3, area);
@@ -659,6 +663,10 @@ int lgx_RadSolveGrad(lgx_Device* d, const double* E, uint32_t R, const float* di
 			throw std::runtime_error("lgx_RadSolveGrad: negative bounce count");
 		if (!g || !total || !lam || !dalb)
 			throw std::runtime_error("lgx_RadSolveGrad: null output");
+		if (times_ms)
+			times_ms[0] = times_ms[1] = times_ms[2] = times_ms[3] = 0.0;
+		if (d->dev.npatches == 0)
+			return; // no patches: nothing to differentiate
 		if (!d->rad || d->rad->N != d->dev.npatches)
 			d->rad.reset(new RadSolver(d->dev));
 		d->rad->set_problem(E, R, diffuse3, area);

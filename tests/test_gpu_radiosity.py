@@ -183,3 +183,25 @@ def test_patches_without_links_keep_their_emission(lib):
         assert stored_tiles(lib, dev)[0] == 0
     finally:
         dev.close()
+
+
+@pytest.mark.parametrize("n", [0, 1])
+def test_zero_and_one_patch(lib, n):
+    """Degenerate patch sets: no patch at all (every form succeeds and writes
+    nothing), and a single patch (no pairs: every form returns E)."""
+    tri = np.array([[-10, -10, -1, 10, -10, -1, 0, 10, -1]], np.float32)
+    dev = Device(lib, tri)
+    try:
+        pos = np.zeros((max(n, 1), 3), np.float32)
+        nrm = np.tile(np.array([0, 0, 1], np.float32), (max(n, 1), 1))
+        nl = C.c_size_t(7)
+        _lib.check(lib.lgx_RadLinks(dev.h, P(pos), P(nrm), n, C.byref(nl)), lib)
+        assert nl.value == 0
+        E = np.full((n, 3), 0.25)
+        diff = np.full((max(n, 1), 3), 0.5, np.float32)
+        area = np.full(max(n, 1), 0.1, np.float32)
+        for form in range(5):
+            got, _ = solve(lib, dev, E, diff, area, form, 2, refine=1 if form >= 2 else 0)
+            assert got.shape == (n, 3) and np.array_equal(got, E), form
+    finally:
+        dev.close()

@@ -536,21 +536,6 @@ __device__ float shadow_march(const View& V, f3 from, f3 to, float k, unsigned l
 // ---------------------------------------------------------------------------
 // direct light
 // ---------------------------------------------------------------------------
-// Wave-aggregated append: one atomic per wave; returns this lane's slot.
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter)
-{
-	const int lane = threadIdx.x & 63;
-	uint64_t m = __ballot(want);
-	if (!m)
-		return 0;
-	int leader = __ffsll((unsigned long long)m) - 1;
-	uint32_t base = 0;
-	if (lane == leader)
-		base = atomicAdd(counter, (uint32_t)__popcll(m));
-	base = (uint32_t)__shfl((int)base, leader);
-	return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-
 // Shading gate of each (light, texel) item (Job_LMRender_*_Inner up to the
 // shadow march, lighter.cpp:485-553, 586): items with a zero shading factor
 // get their final weight here; the others are appended to the march list

@@ -616,7 +616,9 @@ static void downsample2x(float* dst, unsigned dw, unsigned dh, const float* src,
 		}
 }
 
-static void finalize_instance(ltr_Scene* S, size_t mid)
+// One instance's output (the instance's own data only, so instances run in
+// parallel).
+static ltr_WorkOutput finalize_instance(ltr_Scene* S, size_t mid)
 {
 	Instance* mi = S->insts[mid];
 	const ltr_Config& cfg = S->config;
@@ -734,20 +736,27 @@ static void finalize_instance(ltr_Scene* S, size_t mid)
 	wo.normals_xyzf = normals;
 	wo.width = mi->lm_w;
 	wo.height = mi->lm_h;
-	S->outputs.push_back(wo);
+	return wo;
 }
 
 void finalize(ltr_Scene* S)
 {
-	for (size_t mid = 0; mid < S->insts.size(); ++mid) {
+	const size_t n = S->insts.size();
+	std::vector<ltr_WorkOutput> outs(n);
+	for (size_t mid = 0; mid < n; ++mid) {
 		Instance* mi = S->insts[mid];
-		if (mi->samplecont) {
+		if (mi->samplecont)
 			for (size_t i = 0; i < mi->lightmap.size(); ++i)
 				st3(S->samples[i].out_color, mi->lightmap[i]);
-			continue;
-		}
-		finalize_instance(S, mid);
 	}
+	parallel_for(S->nthreads, n, [&](size_t mid) {
+		if (!S->insts[mid]->samplecont)
+			outs[mid] = finalize_instance(S, mid);
+	});
+	// outputs in instance order (uid = instance index), as the reference's
+	for (size_t mid = 0; mid < n; ++mid)
+		if (!S->insts[mid]->samplecont)
+			S->outputs.push_back(outs[mid]);
 }
 
 // ---------------------------------------------------------------------------

@@ -217,14 +217,14 @@ void BoxTree::make_node(int32_t node, const Box* boxes, int32_t* idx, size_t cou
 void TriSet::set(const Tri* tris, size_t count)
 {
 	tris_.clear();
-	std::vector<Box> boxes;
+	boxes_.clear();
 	for (size_t i = 0; i < count; ++i) {
 		if (tri_useful(tris[i])) {
-			boxes.push_back(tri_box(tris[i]));
+			boxes_.push_back(tri_box(tris[i]));
 			tris_.push_back(tris[i]);
 		}
 	}
-	tree_.build(boxes.data(), boxes.size());
+	tree_.build(boxes_.data(), boxes_.size());
 }
 
 float TriSet::ray_dist(f3 from, f3 to, int32_t* outtid) const
@@ -279,6 +279,15 @@ void TriSet::offset_sample(f3& P, f3 N, float dist) const
 	auto fn = [&](const int32_t* ids, int32_t n) {
 		for (int32_t i = 0; i < n; ++i) {
 			const Tri& T = tris_[ids[i]];
+			// The distance the formula returns is that to a point of the
+			// triangle up to a few ulps of the coordinates, so a triangle
+			// whose box misses the cube around the current P of half side
+			// dist + 1e-5 (1 + |P|) returns nd >= dist and is passed over
+			// (the tree hands over whole item lists of the nodes it enters).
+			const float m = dist + 1e-5f * (1.0f + fmaxf(fmaxf(fabsf(P.x), fabsf(P.y)), fabsf(P.z)));
+			const f3 m3 = mk3(m, m, m);
+			if (box_disjoint(P - m3, P + m3, boxes_[ids[i]].mn, boxes_[ids[i]].mx))
+				continue;
 			float nd = point_tri_dist(P, T.p1, T.p2, T.p3);
 			if (!(nd < dist))
 				continue;

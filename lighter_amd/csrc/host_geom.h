@@ -182,6 +182,7 @@ public:
 private:
 	BoxTree tree_;
 	std::vector<Tri> tris_;
+	std::vector<Box> boxes_; // per triangle, as the tree was built from
 };
 
 // ---------------------------------------------------------------------------

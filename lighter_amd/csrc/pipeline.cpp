@@ -143,6 +143,12 @@ static float scene_split_hit(ltr_Scene* S, f3 A, f3 B, f3* outnormal)
 			Instance* mi = S->insts[ids[i]];
 			if (!mi->shadow)
 				continue;
+			// the tree hands over every instance of a node the segment
+			// enters; one whose own box (that of the triangles its split
+			// tree holds) misses the segment's padded box cannot be hit
+			const BoxTree::Node& root = mi->tris.tree().root();
+			if (box_disjoint(r.qmn, r.qmx, root.mn, root.mx))
+				continue;
 			f3 nrm = mk3(0, 0, 0);
 			float d = mi->split.intersect(r.origin, r.end, &nrm);
 			if (d < closest) {

@@ -226,6 +226,52 @@ def room_patches(lib, nth, n_patches, lm0, **room_kw):
     return spec, b, pos[:N].copy(), nrm[:N].copy(), rad[:N].copy(), tris, N, lm
 
 
+def links_bench(lib, nth, lm=64):
+    """Radiosity link generation (Int_RDGenLinks, lighter.cpp:667-765) on the
+    room at a smaller atlas: the reference's own pass (serial, as it runs)
+    timed beside lgx_RadLinks on the same patches and triangles (U = one
+    patch pair examined)."""
+    try:
+        from oracle.checkers import Reference
+        ref = Reference()
+    except Exception as e:  # the reference build is optional on a bare install
+        log(f"[bench] link baseline skipped: {e}")
+        return None
+    spec = scenes.room_scene(lm=lm, bounce_count=1, name="C2-links")
+    br = scenes.BuiltScene(ref.lib, spec)
+    ref.lib.rp_prepare(br.scene, nth)
+    t0 = time.perf_counter()
+    ref.lib.rp_rad_links(br.scene)
+    cpu_s = time.perf_counter() - t0
+    geo = ref.rad_geoms(br.scene)
+    nl_ref = int(ref.lib.rp_num_links(br.scene))
+    tris = np.ascontiguousarray(ref.shadow_tris(br.scene), np.float32)
+    br.destroy()
+    N = len(geo)
+    pairs = N * (N - 1) // 2
+    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    pos = np.ascontiguousarray(geo[:, :3])
+    nrm = np.ascontiguousarray(geo[:, 3:])
+    nl = C.c_size_t(0)
+    tm = (C.c_double * 6)()
+    ms = []
+    for _ in range(3):
+        _lib.check(lib.lgx_RadLinks(dev, P(pos), P(nrm), N, C.byref(nl)), lib)
+        lib.lgx_DeviceTimes(dev, tm)
+        ms.append(tm[3])
+    lib.lgx_DeviceDestroy(dev)
+    gpu_ms = float(np.median(ms))
+    out = dict(patches=N, pairs=pairs, links=int(nl.value), reference_links=nl_ref, gpu_ms=round(gpu_ms, 3),
+               gpu_pairs_per_s=pairs / (gpu_ms / 1e3),
+               cpu_baseline=dict(value=pairs / cpu_s, unit="patch pairs/s", cores=1, kind="reference",
+                                 sample=f"the reference's Int_RDGenLinks over all {N} patches of the room at a "
+                                        f"{lm}^2 atlas ({pairs} pairs), one thread as it runs, {cpu_s:.1f} s"))
+    out["speedup_vs_cpu"] = round(out["gpu_pairs_per_s"] / out["cpu_baseline"]["value"], 1)
+    log(f"[bench] links: {N} patches, {nl.value} links (reference {nl_ref}); GPU {gpu_ms:.2f} ms, "
+        f"reference {cpu_s:.2f} s: {out['speedup_vs_cpu']}x")
+    return out
+
+
 def ao_bench(lib, dev, spec, S, nth, with_cpu):
     """C2 ambient occlusion: 64 cosine-weighted rays of 2 m per texel on the
     device's S texels (U2 = one ray), beside the reference's AO stage on all
@@ -298,6 +344,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
     lib.lgx_DeviceTimes(dev, tm)
     link_ms = tm[3]
     log(f"[bench] C2: {N} patches, {nl.value} links ({link_ms:.1f} ms GPU, {time.perf_counter() - t0:.1f} s wall)")
+    links_cmp = links_bench(lib, nth) if with_cpu else None
     alb = spec.albedo
     diff = np.array([alb.get(int(p), (1.0, 1.0, 1.0)) for p in rad[:, 2]], np.float32)
     area = np.ascontiguousarray(rad[:, 3])
@@ -354,7 +401,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
                mfma_roofline=dict(bound="hbm", achieved=round(gemm_bytes / mf / 1e9, 1), peak=HBM_PEAK_GBS,
                                   unit="GB/s", frac=round(gemm_bytes / mf / 1e9 / HBM_PEAK_GBS, 4),
                                   note="BF16 matrix (its stored tiles) once per bounce + right-hand sides in and out"),
-               ao=ao, cpu_port_s_per_bounce_rgb=cpu_bounce_s,
+               ao=ao, links_vs_reference=links_cmp, cpu_port_s_per_bounce_rgb=cpu_bounce_s,
                cpu_note="the reference's Int_RDBounce loop restated in C (oracle), one RGB setup, one host thread, "
                         "on the same link list; the reference itself runs it single-threaded")
     return out

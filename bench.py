@@ -686,6 +686,27 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
                                             speedup_vs_dense_f32=round(forms["dense_f32"]["s_per_bounce"] / best[0], 2))
     log(f"[bench] C4 radiosity: {tiles:.4f} of the 256 x 32 tiles stored; best MFMA form at <= 1e-4: {best[1]}, "
         f"{sp / best[0]:.2f}x the sparse FP32 gather")
+    if with_cpu:
+        # the reference's FP32 bounce loop (Int_RDBounce, restated in the
+        # oracle) on the same links for one RGB setup, one thread as it runs
+        try:
+            from oracle.checkers import Oracle
+            orc = Oracle()
+            rowmap, other, fac = read_links(lib, rad_dev, N, nl.value)
+            c13 = np.concatenate([diff, E[:, :3], E[:, :3], np.zeros((N, 3)), area[:, None]], 1).astype(np.float32)
+            t0 = time.perf_counter()
+            orc.rad_bounce_f32(rowmap, other, fac, c13)
+            cpu_b = time.perf_counter() - t0
+            sets = R / 3.0
+            out["radiosity"]["cpu_port_s_per_bounce_rgb"] = cpu_b
+            out["radiosity"]["mfma_vs_cpu_bounce"] = dict(
+                note=f"GPU seconds per bounce for all {R} right-hand sides ({sets:.0f} RGB setups) against the "
+                     "reference loop's seconds per bounce for one RGB setup times the number of setups",
+                speedup=round(cpu_b * sets / best[0], 1))
+            log(f"[bench] C4 reference bounce loop: {cpu_b * 1e3:.1f} ms per RGB bounce; "
+                f"{best[1]} {cpu_b * sets / best[0]:.0f}x for {sets:.0f} setups")
+        except Exception as e:  # the checker is optional on a bare install
+            log(f"[bench] C4 bounce baseline skipped: {e}")
     lib.lgx_DeviceDestroy(rad_dev)
     if with_cpu:
         cpu, samp = cpu_baseline(spec, pos, nrm, items, w, lights, cpu_seconds, nth)

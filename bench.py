@@ -120,9 +120,13 @@ def make_lights(spec):
     return arr
 
 
-def cpu_baseline(spec, pos, nrm, items, w, lights, target_s, nthreads):
+def cpu_baseline(spec, pos, nrm, items, w, lights, target_s, nthreads, ao=None):
     """Reference CalcInvShadowFactor (built from the reference sources) on a
-    seeded random sample of the marched work items."""
+    seeded random sample of the marched work items.  With ao = (randoff,
+    rays per texel, distance, seconds, dict), the reference's AO ray loop
+    (Job_AORender_Inner, lighter.cpp:812-828) is also timed on a seeded
+    sample of texels of the same scene, and the dict receives the baseline
+    and the sampled texels' AO factors."""
     from oracle.checkers import Reference, reference_available
     if not reference_available():
         return None, None
@@ -171,11 +175,37 @@ def cpu_baseline(spec, pos, nrm, items, w, lights, target_s, nthreads):
     t0 = time.perf_counter()
     ref.lib.rp_shadow_factor_batch_mt(b.scene, len(frm), P(frm), P(to), P(k), P(out), nthreads)
     dt = time.perf_counter() - t0
+    if ao is not None:
+        ao_baseline(ref, b, pos, nrm, nthreads, *ao)
     b.destroy()
     return dict(value=len(frm) / dt, unit="shadow rays/s", cores=nthreads, kind="reference",
                 sample=f"{len(frm)} of {len(marched)} marched (texel, light) pairs, seeded uniform sample; "
                        f"reference CalcInvShadowFactor on {nthreads} host threads, {dt:.1f} s "
                        f"(collision set-up {prep_s:.1f} s not counted)"), (sel, out)
+
+
+def ao_baseline(ref, b, pos, nrm, nthreads, ro, nsamples, distance, target_s, out):
+    """The reference's AO rays (multiplier 1, no falloff) on a seeded sample
+    of texels, sized to about target_s on nthreads host threads."""
+    order = np.random.default_rng(777).permutation(len(pos))
+
+    def run(n):
+        idx = np.sort(order[:n])
+        p, q, r = (np.ascontiguousarray(a[idx]) for a in (pos, nrm, ro))
+        fac = np.zeros(n, np.float32)
+        t0 = time.perf_counter()
+        ref.lib.rp_ao_factor_batch_mt(b.scene, n, P(p), P(q), P(r), nsamples, distance, 1.0, P(fac), nthreads)
+        return idx, fac, max(time.perf_counter() - t0, 1e-6)
+
+    n0 = min(len(pos), 1024)
+    _, _, dt = run(n0)
+    n = int(min(len(pos), max(n0, n0 * target_s / dt)))
+    idx, fac, dt = run(n)
+    out["baseline"] = dict(value=round(n * nsamples / dt, 1), unit="AO rays/s", cores=nthreads, kind="reference",
+                           sample=f"{n} of {len(pos)} texels, seeded uniform sample, {nsamples} rays of "
+                                  f"{distance:g} m each; the reference's AO ray loop (CreateSpiralDirVector, "
+                                  f"DistanceTestBBT) on {nthreads} host threads, {dt:.1f} s")
+    out["texels"], out["factor"] = idx, fac
 
 
 FORMS = {0: "sparse_f64", 1: "sparse_f32", 2: "dense_f32", 3: "dense_bf16_mfma", 4: "dense_f16_mfma"}
@@ -709,8 +739,27 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
             log(f"[bench] C4 bounce baseline skipped: {e}")
     lib.lgx_DeviceDestroy(rad_dev)
     if with_cpu:
-        cpu, samp = cpu_baseline(spec, pos, nrm, items, w, lights, cpu_seconds, nth)
+        ao_ref = {}
+        cpu, samp = cpu_baseline(spec, pos, nrm, items, w, lights, cpu_seconds, nth,
+                                 ao=(ro, 64, 2.0, cpu_seconds, ao_ref))
         out["cpu_baseline"] = cpu
+        if ao_ref:
+            # the device's AO over a lightmap of ones (multiplier 1, no
+            # falloff, effect 0, black) leaves 1 - factor in every channel
+            _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), S, P(np.ones((S, 3), np.float32))), lib)
+            _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
+            lm1 = np.zeros((S, 3), np.float32)
+            lib.lgx_ReadLightmap(dev, P(lm1))
+            got = lm1[ao_ref["texels"], 0]
+            exp = np.float32(1) - ao_ref["factor"]
+            out["ao"]["cpu_baseline"] = ao_ref["baseline"]
+            out["ao"]["speedup_vs_cpu"] = round(out["ao"]["rays_per_s"] / ao_ref["baseline"]["value"], 1)
+            out["ao"]["parity_sample"] = dict(texels=int(len(got)),
+                                              bitexact_fraction=float(np.mean(got.view(np.uint32)
+                                                                              == exp.view(np.uint32))),
+                                              max_abs_diff=float(np.max(np.abs(got - exp))))
+            log(f"[bench] C4 AO cpu baseline {ao_ref['baseline']['value'] / 1e6:.2f} M rays/s; GPU/CPU "
+                f"{out['ao']['speedup_vs_cpu']}x; parity {out['ao']['parity_sample']}")
         if cpu:
             out["direct"]["speedup_vs_cpu"] = round(rays / cpu["value"], 1)
             sel_, refvis = samp

@@ -212,6 +212,8 @@ class Reference:
             ("rp_prepare", C.c_int, [vp, C.c_int]),
             ("rp_prepare_collision", C.c_int, [vp, C.c_int]),
             ("rp_shadow_factor_batch_mt", None, [vp, C.c_size_t, fp, fp, fp, fp, C.c_int]),
+            ("rp_ao_factor_batch_mt", None, [vp, C.c_size_t, fp, fp, fp, C.c_int, C.c_float, C.c_float, fp,
+                                             C.c_int]),
             ("rp_light_instances", C.c_int, [vp, C.c_int, C.POINTER(C.c_int32), C.c_int]),
             ("rp_direct", None, [vp, C.c_int, C.c_int]),
             ("rp_rad_links", None, [vp]),

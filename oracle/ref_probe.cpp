@@ -86,6 +86,53 @@ void rp_shadow_factor_batch_mt(ltr_Scene* S, size_t n, const float* from, const 
 		pthread_join(th[t], NULL);
 }
 
+// The AO factor of Job_AORender_Inner (lighter.cpp:812-828) for a batch of
+// texels given as position, normal and spiral offset, on nthreads threads
+// (contiguous chunks): the per-texel ray loop over the reference's own
+// CreateSpiralDirVector and DistanceTestBBT, up to the clamp by the
+// multiplier (falloff and the colour blend are left to the caller).
+struct AoJob {
+	ltr_Scene* S;
+	const float *pos, *nrm, *randoff;
+	int nsamples;
+	float distance, multiplier;
+	float* out;
+	size_t b, e;
+};
+static void* ao_job(void* p)
+{
+	AoJob* j = (AoJob*)p;
+	for (size_t i = j->b; i < j->e; ++i) {
+		Vec3 SP = V3P(j->pos + 3 * i), SN = V3P(j->nrm + 3 * i);
+		Vec3 origin = SP + SN * (SMALL_FLOAT * 2);
+		float f = 0;
+		for (int s = 0; s < j->nsamples; ++s) {
+			Vec3 dir = Vec3::CreateSpiralDirVector(SN, j->randoff[i], s, j->nsamples) * j->distance;
+			float hit = j->S->DistanceTestBBT(origin, origin + dir);
+			if (hit < 1.0f)
+				f += 1.0f - hit;
+		}
+		f /= j->nsamples;
+		j->out[i] = TMIN(f * j->multiplier, 1.0f);
+	}
+	return NULL;
+}
+void rp_ao_factor_batch_mt(ltr_Scene* S, size_t n, const float* pos, const float* nrm, const float* randoff,
+                           int nsamples, float distance, float multiplier, float* out, int nthreads)
+{
+	if (nthreads < 1)
+		nthreads = 1;
+	std::vector<pthread_t> th(nthreads);
+	std::vector<AoJob> jobs(nthreads);
+	for (int t = 0; t < nthreads; ++t) {
+		AoJob j = {S, pos, nrm, randoff, nsamples, distance, multiplier, out, n * t / nthreads, n * (t + 1) / nthreads};
+		jobs[t] = j;
+		pthread_create(&th[t], NULL, ao_job, &jobs[t]);
+	}
+	for (int t = 0; t < nthreads; ++t)
+		pthread_join(th[t], NULL);
+}
+
 // Instance ids a light is rendered into, in render order (lighter.cpp:1080-1093).
 int rp_light_instances(ltr_Scene* S, int light, int32_t* out, int cap)
 {

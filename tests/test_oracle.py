@@ -313,3 +313,20 @@ def test_direct_ao_links_bitexact_with_reference(orc, ref, ref_room, libc):
     out, _ = orc.ao(tris, pos, nrm, ro, lm, cfg.ao_distance, cfg.ao_multiplier, cfg.ao_falloff, cfg.ao_effect,
                     list(cfg.ao_color_rgb), cfg.ao_num_samples)
     assert np.array_equal(bits(out), bits(ref.lightmap(br.scene, 1)))
+
+
+@pytest.mark.ref
+def test_threaded_ao_factor_matches_oracle(orc, ref, ref_room):
+    """The probe's threaded AO factor (used to time the reference's AO rays on
+    the host cores) against the oracle's, on the reference's own samples."""
+    spec, br = ref_room
+    pos, nrm, _, _ = ref.samples(br.scene, 1)
+    tris = ref.shadow_tris(br.scene)
+    rng = np.random.default_rng(31)
+    ro = rng.uniform(0, 1, len(pos)).astype(np.float32)
+    got = np.zeros(len(pos), np.float32)
+    ref.lib.rp_ao_factor_batch_mt(br.scene, len(pos), _p(pos), _p(nrm), _p(ro), 24, 2.0, 1.2, _p(got), 4)
+    lm = np.ones((len(pos), 3), np.float32)
+    _, fac = orc.ao(tris, pos, nrm, ro, lm, 2.0, 1.2, 0.0, 0.0, (0, 0, 0), 24)
+    assert np.array_equal(bits(got), bits(fac))
+    assert 0.05 < np.mean(got > 0) < 1.0

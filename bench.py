@@ -64,7 +64,10 @@ def prepare_workload(lib, spec, nthreads, radinfo=None):
     direct-light work items of every light.  radinfo (a list) receives the
     per-texel radiosity info (uv, part id, world area) when given."""
     b = scenes.BuiltScene(lib, spec, dict(max_num_threads=nthreads))
+    t0 = time.perf_counter()
     _lib.check(lib.lgx_Prepare(b.scene), lib)
+    b.prepare_s = time.perf_counter() - t0
+    b.placement = placement_info(lib, b)
     n_inst = lib.lgx_InstanceCount(b.scene)
     pos, nrm, tris, counts = [], [], [], []
     for i in range(n_inst):
@@ -96,6 +99,41 @@ def prepare_workload(lib, spec, nthreads, radinfo=None):
         light_off.append(light_off[-1] + len(s))
     return (b, np.concatenate(pos), np.concatenate(nrm), np.concatenate(tris),
             np.concatenate(items).astype(np.uint32), np.array(light_off, np.uint32))
+
+
+def placement_info(lib, b):
+    """Times of the last GPU sample placement of scene b."""
+    t = (C.c_double * 5)()
+    lib.lgx_PlacementTimes(b.scene, t)
+    return dict(samples=int(t[4]), kernel_ms=round(t[2], 3), flatten_ms=round(t[0], 1), upload_ms=round(t[1], 1),
+                download_ms=round(t[3], 1))
+
+
+def placement_vs_host(lib, spec, nthreads, b_gpu, pos_gpu):
+    """The preparation again with the host restatement of the sample
+    placement: its wall time beside the GPU-placed run's, and whether every
+    sample position is the same."""
+    b = scenes.BuiltScene(lib, spec, dict(max_num_threads=nthreads))
+    _lib.check(lib.lgx_SetSamplePlacement(b.scene, 1), lib)
+    t0 = time.perf_counter()
+    _lib.check(lib.lgx_Prepare(b.scene), lib)
+    host_s = time.perf_counter() - t0
+    pos = []
+    for i in range(lib.lgx_InstanceCount(b.scene)):
+        info = (C.c_int32 * 4)()
+        lib.lgx_InstanceInfo(b.scene, i, info)
+        p = np.zeros((info[2], 3), np.float32)
+        lib.lgx_GetSamples(b.scene, i, P(p), None, None, None)
+        pos.append(p)
+    b.destroy()
+    pos = np.concatenate(pos)
+    same = pos.shape == pos_gpu.shape and bool(np.array_equal(pos.view(np.uint32), pos_gpu.view(np.uint32)))
+    out = dict(b_gpu.placement, prepare_s_gpu_placement=round(b_gpu.prepare_s, 2),
+               prepare_s_host_placement=round(host_s, 2), host_threads=nthreads, identical_samples=same)
+    log(f"[bench] sample placement: GPU kernel {out['kernel_ms']} ms for {out['samples']} samples; preparation "
+        f"{b_gpu.prepare_s:.2f} s (GPU placement) vs {host_s:.2f} s (host placement, {nthreads} threads); "
+        f"identical: {same}")
+    return out
 
 
 def make_lights(spec):
@@ -771,6 +809,7 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
                 f"parity {out['parity_sample']}")
     lib.lgx_DeviceDestroy(dev)
     built.destroy()
+    out["sample_placement"] = placement_vs_host(lib, spec, nth, built, pos)
     return out
 
 
@@ -903,6 +942,7 @@ def main():
             log(f"[bench] cpu baseline {cpu['value']:.0f} rays/s; sampled parity {parity}")
     lib.lgx_DeviceDestroy(dev)
     built.destroy()
+    placement = placement_vs_host(lib, spec, nth, built, pos)
     radio = None
     if not args.no_radiosity:
         radio = radiosity_bench(lib, nth, n_patches=args.radiosity_patches, with_cpu=not args.no_cpu_baseline)
@@ -919,7 +959,8 @@ def main():
                            triangles=int(info[0]), samples=int(len(pos)), lights=len(spec.lights),
                            work_items=int(len(items)), shadow_rays_per_step=int(marches),
                            march_steps_per_step=int(steps), march_kernel_ms=round(march_avg, 3)),
-               roofline=roofline, cpu_baseline=cpu, parity_sample=parity, radiosity=radio, adjoint=adjoint)
+               roofline=roofline, cpu_baseline=cpu, parity_sample=parity, sample_placement=placement,
+               radiosity=radio, adjoint=adjoint)
     print(json.dumps(out), flush=True)
 
 

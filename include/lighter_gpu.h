@@ -29,9 +29,22 @@ LTRAPI const char* lgx_LastError(void);
 LTRAPI int lgx_HasDevice(void);
 
 /* ---- staged bake of an ltr_Scene (synchronous, calling thread) -------- */
-/* Host stages "transforming spatial data" .. "generating samples"
- * (lighter.cpp:1050-1073). No GPU needed. */
+/* Stages "transforming spatial data" .. "generating samples"
+ * (lighter.cpp:1050-1073): transforms, collision set-up and rasterisation
+ * on the host threads; the sample placement (OffsetSample and the
+ * correction in front of overlapping faces, lighter.cpp:436-466) on the GPU
+ * unless lgx_SetSamplePlacement selected the host. Scenes with no texel to
+ * place need no GPU. */
 LTRAPI int lgx_Prepare(ltr_Scene* scene);
+/* No counterpart in the reference (its placement runs in Job_Samples,
+ * lighter.cpp:392-476, on the worker threads). where = 0: GPU (the
+ * default, also for ltr_Start); 1: the host restatement the GPU placement is
+ * tested against. Takes effect at the next preparation. */
+LTRAPI int lgx_SetSamplePlacement(ltr_Scene* scene, int where);
+/* No counterpart in the reference. Last GPU placement: out[0] flatten ms,
+ * out[1] upload ms, out[2] kernel ms (HIP events), out[3] download ms,
+ * out[4] samples placed (zeros after a host placement). */
+LTRAPI int lgx_PlacementTimes(ltr_Scene* scene, double out[5]);
 /* "rendering lightmaps" (lighter.cpp:1075-1096) on the GPU. */
 LTRAPI int lgx_StageDirect(ltr_Scene* scene);
 /* "calculating radiosity" .. "committing radiosity" (lighter.cpp:1098-1124). */

@@ -370,6 +370,28 @@ int lgx_StageFinalize(ltr_Scene* S)
 	});
 }
 
+int lgx_SetSamplePlacement(ltr_Scene* S, int where)
+{
+	return guarded([&] {
+		if (where != 0 && where != 1)
+			throw std::runtime_error("lgx_SetSamplePlacement: where must be 0 (GPU) or 1 (host)");
+		if (S->worker.joinable() && S->stage.load() != nullptr)
+			throw std::runtime_error("a bake started with ltr_Start is running");
+		S->host_place = where == 1;
+	});
+}
+
+int lgx_PlacementTimes(ltr_Scene* S, double out[5])
+{
+	const PlaceTimes& t = S->place_times;
+	out[0] = t.flatten_ms;
+	out[1] = t.upload_ms;
+	out[2] = t.kernel_ms;
+	out[3] = t.download_ms;
+	out[4] = (double)t.samples;
+	return 1;
+}
+
 int lgx_InstanceCount(ltr_Scene* S) { return (int)S->insts.size(); }
 
 int lgx_InstanceInfo(ltr_Scene* S, int inst, int32_t out[4])

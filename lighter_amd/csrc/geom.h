@@ -8,6 +8,8 @@
 //    uses, so both forms return identical bits.
 #pragma once
 
+#include <float.h>
+
 #include "vmath.h"
 
 namespace lgx {
@@ -191,6 +193,140 @@ LHD f3 spiral_dir(f3 dir, float cos_side, float sin_side, float cos_around, floa
 	f3 up = normalized(cross(dir, diffvec));
 	f3 rt = cross(dir, up);
 	return (cos_around * sin_side) * rt + (sin_around * sin_side) * up + cos_side * dir;
+}
+
+// ---------------------------------------------------------------------------
+// Sample placement (lighter.cpp:392-470, lighter_math.cpp:618-650, 915-1044),
+// shared by the host restatement and the gfx950 placement kernel so that
+// both evaluate the same operations.
+// ---------------------------------------------------------------------------
+LHD bool box_disjoint(f3 qmn, f3 qmx, f3 mn, f3 mx)
+{
+	return qmn.x > mx.x || qmx.x < mn.x || qmn.y > mx.y || qmx.y < mn.y || qmn.z > mx.z || qmx.z < mn.z;
+}
+
+// Segment query parameters (lighter_int.hpp:680-702).
+struct RaySeg {
+	f3 origin;
+	float len;
+	f3 inv_dir;
+	f3 end;
+	// the segment's bounding box, padded by 1e-3 (1 + largest |coordinate|):
+	// a triangle the segment test accepts holds a point within rounding of the
+	// segment, so a box disjoint from this one holds no such triangle
+	f3 qmn, qmx;
+	LHD void set(f3 r0, f3 r1)
+	{
+		origin = r0;
+		end = r1;
+		len = length(r1 - r0);
+		f3 d = normalized(r1 - r0);
+		inv_dir = mk3(d.x ? 1.0f / d.x : 0.0f, d.y ? 1.0f / d.y : 0.0f, d.z ? 1.0f / d.z : 0.0f);
+		float m = tmax(tmax(tmax(fabsf(r0.x), fabsf(r0.y)), fabsf(r0.z)),
+		               tmax(tmax(fabsf(r1.x), fabsf(r1.y)), fabsf(r1.z)));
+		float pad = 1e-3f * (1.0f + m);
+		f3 p3 = mk3(pad, pad, pad);
+		qmn = vmin(r0, r1) - p3;
+		qmx = vmax(r0, r1) + p3;
+	}
+};
+
+// The conservative slab test (lighter_math.cpp:618-650): axes whose
+// direction component is zero are skipped; no tmax >= 0 check.
+LHD bool ray_box(const RaySeg& r, f3 mn, f3 mx)
+{
+	float lo = -FLT_MAX, hi = FLT_MAX;
+	if (r.inv_dir.x != 0.0f) {
+		float a = (mn.x - r.origin.x) * r.inv_dir.x, b = (mx.x - r.origin.x) * r.inv_dir.x;
+		lo = tmax(lo, tmin(a, b));
+		hi = tmin(hi, tmax(a, b));
+	}
+	if (r.inv_dir.y != 0.0f) {
+		float a = (mn.y - r.origin.y) * r.inv_dir.y, b = (mx.y - r.origin.y) * r.inv_dir.y;
+		lo = tmax(lo, tmin(a, b));
+		hi = tmin(hi, tmax(a, b));
+	}
+	if (r.inv_dir.z != 0.0f) {
+		float a = (mn.z - r.origin.z) * r.inv_dir.z, b = (mx.z - r.origin.z) * r.inv_dir.z;
+		lo = tmax(lo, tmin(a, b));
+		hi = tmin(hi, tmax(a, b));
+	}
+	return hi >= lo && r.len >= lo;
+}
+
+// PointProjOnTriangle (lighter_math.cpp:915-952): false when the point is
+// clearly outside the triangle's prism (beyond SMALL_FLOAT).
+LHD bool point_proj_on_tri(f3 pt, f3 t0, f3 t1, f3 t2)
+{
+	f3 nrm = normalized(cross(t1 - t0, t2 - t0));
+	f3 tan0 = normalized(t1 - t0), tan1 = normalized(t2 - t1), tan2 = normalized(t0 - t2);
+	float t0p = dot(tan0, pt), t0min = dot(tan0, t0), t0max = dot(tan0, t1);
+	float t1p = dot(tan1, pt), t1min = dot(tan1, t1), t1max = dot(tan1, t2);
+	float t2p = dot(tan2, pt), t2min = dot(tan2, t2), t2max = dot(tan2, t0);
+	if (t0min - kSmall > t0p && t2max + kSmall < t2p) return false;
+	if (t1min - kSmall > t1p && t0max + kSmall < t0p) return false;
+	if (t2min - kSmall > t2p && t1max + kSmall < t1p) return false;
+	f3 en0 = normalized(cross(t1 - t0, nrm));
+	f3 en1 = normalized(cross(t2 - t1, nrm));
+	f3 en2 = normalized(cross(t0 - t2, nrm));
+	float ptd0 = dot(en0, pt) - dot(en0, t0);
+	float ptd1 = dot(en1, pt) - dot(en1, t1);
+	float ptd2 = dot(en2, pt) - dot(en2, t2);
+	if (ptd0 > kSmall && t0p - kSmall > t0min && t0p + kSmall < t0max) return false;
+	if (ptd1 > kSmall && t1p - kSmall > t1min && t1p + kSmall < t1max) return false;
+	if (ptd2 > kSmall && t2p - kSmall > t2min && t2p + kSmall < t2max) return false;
+	return true;
+}
+
+// OffsetSample's box pre-test: the distance the formula returns is that to a
+// point of the triangle up to a few ulps of the coordinates, so a triangle
+// whose box (bmn, bmx) misses the cube around the current P of half side
+// dist + 1e-5 (1 + |P|) returns a distance >= dist and can be passed over.
+LHD bool offset_box_skip(f3 P, float dist, f3 bmn, f3 bmx)
+{
+	const float m = dist + 1e-5f * (1.0f + fmaxf(fmaxf(fabsf(P.x), fabsf(P.y)), fabsf(P.z)));
+	const f3 m3 = mk3(m, m, m);
+	return box_disjoint(P - m3, P + m3, bmn, bmx);
+}
+
+// One triangle of OffsetSample (lighter_math.cpp:1000-1040) up to its
+// visibility check: true when the sample at P, normal N, would move off the
+// triangle (p1, p2, p3) to *pnew; the move is taken when the closest hit in
+// the instance along *from -> *pnew is >= 0.9 or is this triangle.
+LHD bool offset_candidate(f3 P, f3 N, float dist, f3 p1, f3 p2, f3 p3, f3* from, f3* pnew)
+{
+	float nd = point_tri_dist(P, p1, p2, p3);
+	if (!(nd < dist))
+		return false;
+	f3 tpn = -normalized(cross(p3 - p1, p2 - p1));
+	float tpd = dot(tpn, p1);
+	float sig = dot(tpn, P) - tpd;
+	if (!(sig <= kSmall))
+		return false;
+	f3 pex = P + normalized(tpn + N) * -sig * 1.41f;
+	if (!point_proj_on_tri(pex, p1, p2, p3))
+		return false;
+	float pdot = dot(tpn, N);
+	if (!(fabsf(pdot) < 0.95f))
+		return false;
+	f3 ptpn = normalized(tpn - N * pdot);
+	float dfac = 1.0f - fabsf(pdot);
+	*pnew = P + ptpn * (-sig / dfac + kSmall);
+	*from = P + (N + ptpn) * kSmall;
+	return true;
+}
+
+// Correction of a sample in front of overlapping faces (lighter.cpp:445-461):
+// one step towards PEnd after a hit at q with normal hn; false ends the loop.
+LHD bool correction_step(f3& P, f3 PEnd, f3 N, float& md, float q, f3 hn, float corr_min_dot)
+{
+	if (-dot(hn, N) < corr_min_dot)
+		return false;
+	if (q < kSmall)
+		q = kSmall;
+	P = P * (1.0f - q) + PEnd * q;
+	md *= (1.0f - q);
+	return true;
 }
 
 } // namespace lgx

@@ -104,28 +104,6 @@ float sample_area(f2 t1, f2 t2, f2 t3, f3 p1, f3 p2, f3 p3)
 }
 
 // ---------------------------------------------------------------------------
-bool ray_box(const RaySeg& r, f3 mn, f3 mx)
-{
-	float lo = -FLT_MAX, hi = FLT_MAX;
-	if (r.inv_dir.x != 0.0f) {
-		float a = (mn.x - r.origin.x) * r.inv_dir.x, b = (mx.x - r.origin.x) * r.inv_dir.x;
-		lo = tmax(lo, tmin(a, b));
-		hi = tmin(hi, tmax(a, b));
-	}
-	if (r.inv_dir.y != 0.0f) {
-		float a = (mn.y - r.origin.y) * r.inv_dir.y, b = (mx.y - r.origin.y) * r.inv_dir.y;
-		lo = tmax(lo, tmin(a, b));
-		hi = tmin(hi, tmax(a, b));
-	}
-	if (r.inv_dir.z != 0.0f) {
-		float a = (mn.z - r.origin.z) * r.inv_dir.z, b = (mx.z - r.origin.z) * r.inv_dir.z;
-		lo = tmax(lo, tmin(a, b));
-		hi = tmin(hi, tmax(a, b));
-	}
-	return hi >= lo && r.len >= lo;
-}
-
-// ---------------------------------------------------------------------------
 // BoxTree construction (lighter_math.cpp:674-781): median split of the
 // "splittable" items (volume * 3 < node volume) along the axis of their
 // largest extent; the others stay on the node.  Leaves at <= 4 items, depth
@@ -213,6 +191,22 @@ void BoxTree::make_node(int32_t node, const Box* boxes, int32_t* idx, size_t cou
 	}
 }
 
+int BoxTree::depth() const
+{
+	int depth = 0;
+	std::vector<std::pair<int32_t, int>> st{{0, 1}};
+	while (!st.empty()) {
+		auto [k, d] = st.back();
+		st.pop_back();
+		depth = std::max(depth, d);
+		if (nodes_[k].ch != -1) {
+			st.push_back({nodes_[k].ch, d + 1});
+			st.push_back({k + 1, d + 1});
+		}
+	}
+	return depth;
+}
+
 // ---------------------------------------------------------------------------
 void TriSet::set(const Tri* tris, size_t count)
 {
@@ -250,63 +244,21 @@ float TriSet::ray_dist(f3 from, f3 to, int32_t* outtid) const
 	return closest;
 }
 
-// PointProjOnTriangle (lighter_math.cpp:915-952): false when the point is
-// clearly outside the triangle's prism (beyond SMALL_FLOAT).
-static bool point_proj_on_tri(f3 pt, f3 t0, f3 t1, f3 t2)
-{
-	f3 nrm = normalized(cross(t1 - t0, t2 - t0));
-	f3 tan0 = normalized(t1 - t0), tan1 = normalized(t2 - t1), tan2 = normalized(t0 - t2);
-	float t0p = dot(tan0, pt), t0min = dot(tan0, t0), t0max = dot(tan0, t1);
-	float t1p = dot(tan1, pt), t1min = dot(tan1, t1), t1max = dot(tan1, t2);
-	float t2p = dot(tan2, pt), t2min = dot(tan2, t2), t2max = dot(tan2, t0);
-	if (t0min - kSmall > t0p && t2max + kSmall < t2p) return false;
-	if (t1min - kSmall > t1p && t0max + kSmall < t0p) return false;
-	if (t2min - kSmall > t2p && t1max + kSmall < t1p) return false;
-	f3 en0 = normalized(cross(t1 - t0, nrm));
-	f3 en1 = normalized(cross(t2 - t1, nrm));
-	f3 en2 = normalized(cross(t0 - t2, nrm));
-	float ptd0 = dot(en0, pt) - dot(en0, t0);
-	float ptd1 = dot(en1, pt) - dot(en1, t1);
-	float ptd2 = dot(en2, pt) - dot(en2, t2);
-	if (ptd0 > kSmall && t0p - kSmall > t0min && t0p + kSmall < t0max) return false;
-	if (ptd1 > kSmall && t1p - kSmall > t1min && t1p + kSmall < t1max) return false;
-	if (ptd2 > kSmall && t2p - kSmall > t2min && t2p + kSmall < t2max) return false;
-	return true;
-}
-
+// OffsetSample (lighter_math.cpp:991-1044): the triangles of every node the
+// cube around the starting P overlaps, in tree order, each possibly moving P.
 void TriSet::offset_sample(f3& P, f3 N, float dist) const
 {
 	auto fn = [&](const int32_t* ids, int32_t n) {
 		for (int32_t i = 0; i < n; ++i) {
+			const Box& b = boxes_[ids[i]];
+			if (offset_box_skip(P, dist, b.mn, b.mx))
+				continue;
 			const Tri& T = tris_[ids[i]];
-			// The distance the formula returns is that to a point of the
-			// triangle up to a few ulps of the coordinates, so a triangle
-			// whose box misses the cube around the current P of half side
-			// dist + 1e-5 (1 + |P|) returns nd >= dist and is passed over
-			// (the tree hands over whole item lists of the nodes it enters).
-			const float m = dist + 1e-5f * (1.0f + fmaxf(fmaxf(fabsf(P.x), fabsf(P.y)), fabsf(P.z)));
-			const f3 m3 = mk3(m, m, m);
-			if (box_disjoint(P - m3, P + m3, boxes_[ids[i]].mn, boxes_[ids[i]].mx))
+			f3 from, pnew;
+			if (!offset_candidate(P, N, dist, T.p1, T.p2, T.p3, &from, &pnew))
 				continue;
-			float nd = point_tri_dist(P, T.p1, T.p2, T.p3);
-			if (!(nd < dist))
-				continue;
-			f3 tpn = -tri_normal(T);
-			float tpd = dot(tpn, T.p1);
-			float sig = dot(tpn, P) - tpd;
-			if (!(sig <= kSmall))
-				continue;
-			f3 pex = P + normalized(tpn + N) * -sig * 1.41f;
-			if (!point_proj_on_tri(pex, T.p1, T.p2, T.p3))
-				continue;
-			float pdot = dot(tpn, N);
-			if (!(fabsf(pdot) < 0.95f))
-				continue;
-			f3 ptpn = normalized(tpn - N * pdot);
-			float dfac = 1.0f - fabsf(pdot);
-			f3 pnew = P + ptpn * (-sig / dfac + kSmall);
 			int32_t tid = -1;
-			if (ray_dist(P + (N + ptpn) * kSmall, pnew, &tid) >= 0.9f || tid == ids[i])
+			if (ray_dist(from, pnew, &tid) >= 0.9f || tid == ids[i])
 				P = pnew;
 		}
 	};
@@ -485,6 +437,41 @@ float SplitTree::node_intersect(int32_t node, f3 from, f3 to, f3* outnormal) con
 	if (outnormal && ct)
 		*outnormal = normalized(cross(ct->p3 - ct->p1, ct->p2 - ct->p1));
 	return closest;
+}
+
+int SplitTree::flatten(std::vector<FlatSplitNode>& out, std::vector<FlatSplitTri>& tris) const
+{
+	const size_t base = out.size(), tbase = tris.size();
+	int depth = 0;
+	std::vector<std::pair<int32_t, int>> st;
+	if (!nodes_.empty())
+		st.push_back({0, 1});
+	out.resize(base + nodes_.size());
+	while (!st.empty()) {
+		auto [k, d] = st.back();
+		st.pop_back();
+		depth = std::max(depth, d);
+		const Node& n = nodes_[k];
+		FlatSplitNode& f = out[base + k];
+		f.nx = n.N.x; f.ny = n.N.y; f.nz = n.N.z; f.d = n.D;
+		f.front = n.front;
+		f.back = n.back;
+		f.leaf_off = f.leaf_cnt = 0;
+		if (n.front != -1) {
+			st.push_back({n.front, d + 1});
+			st.push_back({n.back, d + 1});
+			continue;
+		}
+		f.leaf_off = (int32_t)(tris.size() - tbase);
+		f.leaf_cnt = (int32_t)n.rt.size();
+		for (size_t q = 0; q < n.rt.size(); ++q) {
+			const Tri& t = n.tris[n.rti[q]];
+			// the normal node_intersect reports for this triangle
+			f3 hn = normalized(cross(t.p3 - t.p1, t.p2 - t.p1));
+			tris.push_back(FlatSplitTri{n.rt[q], hn.x, hn.y, hn.z, 0.0f});
+		}
+	}
+	return depth;
 }
 
 float SplitTree::intersect(f3 from, f3 to, f3* outnormal) const

@@ -8,8 +8,9 @@
 // layouts and traversal orders:
 //  * BoxTree   - lighter_int.hpp:706-821, lighter_math.cpp:653-781
 //  * SplitTree - lighter_int.hpp:623-677, lighter_math.cpp:428-614
-// The GPU path uses its own BVH (bvh.cpp); these exist only for the host
-// stages.
+// The hot-path kernels use their own BVH (bvh.cpp); these trees serve the
+// sample placement, on the host or flattened for the placement kernel
+// (place.hip), which walks them in the same orders.
 #pragma once
 
 #include <stdint.h>
@@ -63,42 +64,6 @@ inline Box tri_box(const Tri& t)
 }
 
 // ---------------------------------------------------------------------------
-// Segment query parameters and the conservative slab test
-// (lighter_int.hpp:680-702, lighter_math.cpp:618-650).
-// ---------------------------------------------------------------------------
-struct RaySeg {
-	f3 origin;
-	float len;
-	f3 inv_dir;
-	f3 end;
-	// the segment's bounding box, padded by 1e-3 (1 + largest |coordinate|):
-	// a triangle the segment test accepts holds a point within rounding of the
-	// segment, so a box disjoint from this one holds no such triangle
-	f3 qmn, qmx;
-	void set(f3 r0, f3 r1)
-	{
-		origin = r0;
-		end = r1;
-		len = length(r1 - r0);
-		f3 d = normalized(r1 - r0);
-		inv_dir = mk3(d.x ? 1.0f / d.x : 0.0f, d.y ? 1.0f / d.y : 0.0f, d.z ? 1.0f / d.z : 0.0f);
-		float m = tmax(tmax(tmax(fabsf(r0.x), fabsf(r0.y)), fabsf(r0.z)),
-		               tmax(tmax(fabsf(r1.x), fabsf(r1.y)), fabsf(r1.z)));
-		float pad = 1e-3f * (1.0f + m);
-		f3 p3 = mk3(pad, pad, pad);
-		qmn = vmin(r0, r1) - p3;
-		qmx = vmax(r0, r1) + p3;
-	}
-};
-
-bool ray_box(const RaySeg& r, f3 mn, f3 mx);
-
-inline bool box_disjoint(f3 qmn, f3 qmx, f3 mn, f3 mx)
-{
-	return qmn.x > mx.x || qmx.x < mn.x || qmn.y > mx.y || qmx.y < mn.y || qmn.z > mx.z || qmx.z < mn.z;
-}
-
-// ---------------------------------------------------------------------------
 // BoxTree: binary tree in depth-first order; node n's children are n+1 and
 // ch; item lists (count-prefixed) hang off leaves and, for items too large
 // to split, off inner nodes.
@@ -112,6 +77,10 @@ public:
 	};
 	void build(const Box* boxes, size_t count);
 	const Node& root() const { return nodes_[0]; }
+	const std::vector<Node>& nodes() const { return nodes_; }
+	const std::vector<int32_t>& items() const { return items_; }
+	// nodes on the longest root-to-leaf path (the root alone is 1)
+	int depth() const;
 
 	// Items of every node whose box overlaps [qmn, qmx]; fn(ids, count).
 	template <class F> void query(f3 qmn, f3 qmx, F& fn) const
@@ -174,6 +143,7 @@ public:
 	void set(const Tri* tris, size_t count);
 	const std::vector<Tri>& tris() const { return tris_; }
 	const BoxTree& tree() const { return tree_; }
+	const std::vector<Box>& boxes() const { return boxes_; }
 	// Closest segment hit; hit index through outtid (lighter_math.cpp:835-871).
 	float ray_dist(f3 from, f3 to, int32_t* outtid) const;
 	// Push a sample out of concave edges (lighter_math.cpp:991-1044).
@@ -189,9 +159,26 @@ private:
 // SplitTree: the plane-split tree used to move samples in front of
 // overlapping faces.  Triangles that straddle a plane go to both sides.
 // ---------------------------------------------------------------------------
+// Flat form of a SplitTree for the placement kernel: inner nodes hold the
+// plane and the child indices, leaves (front == -1) a range of leaf
+// triangles with the segment-test operands and the normal a hit reports;
+// node and triangle indices are local to the tree.
+struct FlatSplitNode {
+	float nx, ny, nz, d;
+	int32_t front, back, leaf_off, leaf_cnt;
+};
+struct FlatSplitTri {
+	RayTri rt;
+	float nx, ny, nz, pad;
+};
+
 class SplitTree {
 public:
 	void set(const Tri* tris, size_t count);
+	bool empty() const { return nodes_.empty(); }
+	// Appends the flat form (indices local to this tree); returns the depth
+	// (nodes on the longest root-to-leaf path, 0 for an empty tree).
+	int flatten(std::vector<FlatSplitNode>& nodes, std::vector<FlatSplitTri>& tris) const;
 	// Closest hit in the first subtree along the segment that has one
 	// (lighter_math.cpp:490-533); 2 when nothing is hit.
 	float intersect(f3 from, f3 to, f3* outnormal) const;

@@ -186,46 +186,60 @@ static void rasterize_instance(Instance* mi, float margin, f3* R1, f3* R2, f4* R
 	}
 }
 
-static void gen_samples(ltr_Scene* S, Instance* mi)
+// Rasterised texels of one instance with a nonzero normal, in texel order:
+// position, normalised normal, (uv0, part, area) and the texel index.
+struct RasterSamples {
+	std::vector<f3> pos, nrm;
+	std::vector<f4> info;
+	std::vector<uint32_t> loc;
+	std::vector<float> dist; // square root of the world area per texel
+};
+
+static void raster_samples(Instance* mi, RasterSamples& out)
 {
-	const ltr_Config& cfg = S->config;
-	float corr_min_dot = cosf(cfg.max_correct_angle / 180.0f * kPiF);
-	f3 amb = ld3(cfg.ambient_color);
-	if (mi->samplecont) {
-		for (auto& c : mi->lightmap)
-			c = amb;
-		return;
-	}
 	size_t n = (size_t)mi->lm_w * mi->lm_h;
-	std::vector<Box> roots(S->insts.size());
-	for (size_t k = 0; k < S->insts.size(); ++k) {
-		const BoxTree::Node& r = S->insts[k]->tris.tree().root();
-		roots[k] = Box{r.mn, r.mx};
-	}
 	std::vector<f3> R1(n, mk3(0, 0, 0)), R2(n, mk3(0, 0, 0));
 	std::vector<f4> R3(n, f4{0, 0, 0, 0});
 	rasterize_instance(mi, 0.5f + kSmall, R1.data(), R2.data(), R3.data());
 	rasterize_instance(mi, 0.0f, R1.data(), R2.data(), R3.data());
-
 	size_t count = 0;
 	for (size_t i = 0; i < n; ++i)
 		if (!is_zero(R2[i]))
 			count++;
-	mi->s_pos.reserve(count);
-	mi->s_nrm.reserve(count);
-	mi->s_loc.reserve(count);
-	mi->s_info.reserve(count);
+	out.pos.reserve(count);
+	out.nrm.reserve(count);
+	out.info.reserve(count);
+	out.loc.reserve(count);
+	out.dist.reserve(count);
 	for (size_t i = 0; i < n; ++i) {
-		f3 N = R2[i];
-		if (is_zero(N))
+		if (is_zero(R2[i]))
 			continue;
-		N = normalized(N);
-		f3 P = R1[i];
-		f4 XD = R3[i];
+		out.pos.push_back(R1[i]);
+		out.nrm.push_back(normalized(R2[i]));
+		out.info.push_back(R3[i]);
+		out.loc.push_back((uint32_t)i);
+		out.dist.push_back(sqrtf(R3[i].w));
+	}
+}
+
+// The host restatement of the placement of one instance's samples
+// (lighter.cpp:436-466); the GPU placement (place.hip) gives the same bits.
+static void place_host(ltr_Scene* S, RasterSamples& r)
+{
+	const ltr_Config& cfg = S->config;
+	float corr_min_dot = cosf(cfg.max_correct_angle / 180.0f * kPiF);
+	std::vector<Box> roots(S->insts.size());
+	for (size_t k = 0; k < S->insts.size(); ++k) {
+		const BoxTree::Node& rt = S->insts[k]->tris.tree().root();
+		roots[k] = Box{rt.mn, rt.mx};
+	}
+	for (size_t i = 0; i < r.pos.size(); ++i) {
+		f3 P = r.pos[i];
+		const f3 N = r.nrm[i];
 		// move the sample out of concave edges; an instance whose tree root is
 		// disjoint from the query cube contributes nothing (its query stops at
 		// the root), so that test is made here first, on the current P
-		const float dist = sqrtf(XD.w);
+		const float dist = r.dist[i];
 		for (size_t k = 0; k < S->insts.size(); ++k) {
 			const f3 d3 = mk3(dist, dist, dist);
 			if (box_disjoint(P - d3, P + d3, roots[k].mn, roots[k].mx))
@@ -240,20 +254,80 @@ static void gen_samples(ltr_Scene* S, Instance* mi)
 			while (md > kSmall && left-- > 0) {
 				f3 hn = mk3(0, 0, 0);
 				float q = scene_split_hit(S, P, PEnd, &hn);
-				if (-dot(hn, N) < corr_min_dot)
+				if (!correction_step(P, PEnd, N, md, q, hn, corr_min_dot))
 					break;
-				if (q < kSmall)
-					q = kSmall;
-				P = P * (1.0f - q) + PEnd * q;
-				md *= (1.0f - q);
 			}
 		}
-		mi->s_pos.push_back(P);
-		mi->s_nrm.push_back(N);
-		mi->s_loc.push_back((uint32_t)i);
-		mi->s_info.push_back(XD);
+		r.pos[i] = P;
 	}
-	mi->lightmap.assign(count, amb);
+}
+
+// Every instance's rasterised samples placed in one GPU launch.
+static void place_gpu(ltr_Scene* S, std::vector<RasterSamples>& raw)
+{
+	size_t total = 0;
+	for (const RasterSamples& r : raw)
+		total += r.pos.size();
+	S->place_times = PlaceTimes{};
+	if (total == 0)
+		return;
+	require_gpu();
+	std::vector<float> pos(3 * total), nrm(3 * total), dist(total);
+	size_t k = 0;
+	for (const RasterSamples& r : raw) {
+		for (size_t i = 0; i < r.pos.size(); ++i, ++k) {
+			st3(&pos[3 * k], r.pos[i]);
+			st3(&nrm[3 * k], r.nrm[i]);
+			dist[k] = r.dist[i];
+		}
+	}
+	std::vector<PlaceInstance> pi;
+	for (Instance* mi : S->insts)
+		pi.push_back(PlaceInstance{&mi->tris, &mi->split, mi->shadow});
+	const ltr_Config& cfg = S->config;
+	float corr_min_dot = cosf(cfg.max_correct_angle / 180.0f * kPiF);
+	place_samples_gpu(pi, S->inst_tree, pos.data(), nrm.data(), dist.data(), total, cfg.max_correct_dist,
+	                  corr_min_dot, &S->place_times);
+	k = 0;
+	for (RasterSamples& r : raw)
+		for (size_t i = 0; i < r.pos.size(); ++i, ++k)
+			r.pos[i] = ld3(&pos[3 * k]);
+}
+
+static void commit_samples(ltr_Scene* S, Instance* mi, RasterSamples& r)
+{
+	mi->s_pos = std::move(r.pos);
+	mi->s_nrm = std::move(r.nrm);
+	mi->s_loc = std::move(r.loc);
+	mi->s_info = std::move(r.info);
+	mi->lightmap.assign(mi->s_pos.size(), ld3(S->config.ambient_color));
+}
+
+// generating samples (lighter.cpp:392-476): rasterisation and compaction per
+// instance on the host threads, placement on the GPU (or, when selected,
+// the host restatement)
+static void gen_samples(ltr_Scene* S)
+{
+	const int nt = S->nthreads;
+	std::vector<RasterSamples> raw(S->insts.size());
+	parallel_for(nt, S->insts.size(), [&](size_t i) {
+		Instance* mi = S->insts[i];
+		if (mi->samplecont) {
+			for (auto& c : mi->lightmap)
+				c = ld3(S->config.ambient_color);
+			return;
+		}
+		raster_samples(mi, raw[i]);
+	});
+	check_abort(S);
+	if (S->host_place)
+		parallel_for(nt, S->insts.size(), [&](size_t i) { place_host(S, raw[i]); });
+	else
+		place_gpu(S, raw);
+	parallel_for(nt, S->insts.size(), [&](size_t i) {
+		if (!S->insts[i]->samplecont)
+			commit_samples(S, S->insts[i], raw[i]);
+	});
 }
 
 void prepare(ltr_Scene* S)
@@ -277,7 +351,7 @@ void prepare(ltr_Scene* S)
 	check_abort(S);
 
 	set_stage(S, "generating samples");
-	parallel_for(nt, S->insts.size(), [&](size_t i) { gen_samples(S, S->insts[i]); });
+	gen_samples(S);
 	check_abort(S);
 
 	S->inst_off.assign(S->insts.size() + 1, 0);

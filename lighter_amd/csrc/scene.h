@@ -11,6 +11,7 @@
 #include "../../include/lighter.h"
 #include "device.h"
 #include "host_geom.h"
+#include "place.h"
 
 namespace lgx {
 
@@ -90,4 +91,7 @@ struct ltr_Scene {
 	bool prepared = false;
 	std::vector<float> vis; // per work item shadow factor, when normal-map data is requested
 	lgx::StageTimes times;
+	// sample placement: on the GPU unless the host restatement is selected
+	bool host_place = false;
+	lgx::PlaceTimes place_times;
 };

@@ -47,9 +47,15 @@ def prod_lightmap(lib, scene, i):
     return out
 
 
-def prepared(lib, spec):
-    """Build spec through the product library and run the host stages."""
-    b = scenes.BuiltScene(lib, spec)
+GPU_PLACEMENT, HOST_PLACEMENT = 0, 1
+
+
+def prepared(lib, spec, placement=GPU_PLACEMENT, **config):
+    """Build spec through the product library and run the preparation
+    stages, placing the samples on the GPU (the default) or with the host
+    restatement."""
+    b = scenes.BuiltScene(lib, spec, config or None)
+    _lib.check(lib.lgx_SetSamplePlacement(b.scene, placement), lib)
     _lib.check(lib.lgx_Prepare(b.scene), lib)
     return b
 

@@ -122,13 +122,15 @@ def test_empty_scene_bake_finishes_without_gpu(lib):
 
 
 def test_samples_only_scene_without_lights_finishes(lib):
-    """No lights, no radiosity, no AO: host stages only; external samples
-    come back with the ambient colour (lighter.cpp:394-399, 854-862)."""
+    """No lights, no radiosity, no AO: with the host sample placement no
+    GPU stage runs; external samples come back with the ambient colour
+    (lighter.cpp:394-399, 854-862)."""
     spec = scenes.basic_scene()
     spec.lights = []
     spec.samples = [((0, 0, 1), (0, 0, 1)), ((1, 1, 1), (0, 1, 0))]
     spec.config = dict(spec.config, ambient_color=[0.25, 0.5, 0.75])
     b = scenes.BuiltScene(lib, spec)
+    lib.lgx_SetSamplePlacement(b.scene, 1)
     times = b.bake()
     assert "rendering lightmaps" not in times
     outs = b.outputs()
@@ -159,6 +161,7 @@ def test_abort_stops_bake(lib):
     spec = scenes.room_scene(lm=64)
     spec.lights = []
     b = scenes.BuiltScene(lib, spec)
+    lib.lgx_SetSamplePlacement(b.scene, 1)  # host placement: no GPU needed here
     lib.ltr_Start(b.scene)
     lib.ltr_Abort(b.scene)
     st = api.WorkStatus()

@@ -3,10 +3,11 @@
 from the reference built from its sources.  These tests need neither the
 reference checkout nor its build.
 
-* samples per instance after preprocessing (rasterisation, OffsetSample,
-  BSP correction: host code of the product) must equal the reference's bit
-  for bit; rad1's count is also the 2,782 texels a probe of the reference's
-  rad1 mesh at importance 0.4 reported;
+* samples per instance after preprocessing (rasterisation on the host,
+  OffsetSample and BSP correction on the GPU and, on the CPU, with the
+  product's host restatement) must equal the reference's bit for bit; rad1's
+  count is also the 2,782 texels a probe of the reference's rad1 mesh at
+  importance 0.4 reported;
 * direct light per instance (GPU): bit-exact for point lights; the spot
   light of mesh1 evaluates acos and pow in double rounded once where the
   reference calls acosf and powf, so up to 4 ulp there (at least 90 % of
@@ -20,7 +21,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import bits, prepared, prod_lightmap, prod_samples
+from helpers import GPU_PLACEMENT, HOST_PLACEMENT, bits, prepared, prod_lightmap, prod_samples
 from lighter_amd import _lib, scenes
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
@@ -43,10 +44,12 @@ def test_fixture_shapes(case):
         assert fx["out0_rgb"].shape == (64, 64, 3)
 
 
-def test_samples_match_reference(lib, libc, case):
+@pytest.mark.parametrize("placement", [pytest.param(HOST_PLACEMENT, id="host"),
+                                       pytest.param(GPU_PLACEMENT, id="gpu", marks=pytest.mark.gpu)])
+def test_samples_match_reference(lib, libc, case, placement):
     name, fx, spec = case
     libc.srand(1)
-    b = prepared(lib, spec)
+    b = prepared(lib, spec, placement)
     ni = lib.lgx_InstanceCount(b.scene)
     assert ni == int(fx["n_instances"][0])
     for k in range(1, ni):

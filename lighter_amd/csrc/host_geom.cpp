@@ -439,20 +439,28 @@ float SplitTree::node_intersect(int32_t node, f3 from, f3 to, f3* outnormal) con
 	return closest;
 }
 
-int SplitTree::flatten(std::vector<FlatSplitNode>& out, std::vector<FlatSplitTri>& tris) const
+size_t SplitTree::num_leaf_tris() const
 {
-	const size_t base = out.size(), tbase = tris.size();
+	size_t n = 0;
+	for (const Node& nd : nodes_)
+		if (nd.front == -1)
+			n += nd.rt.size();
+	return n;
+}
+
+int SplitTree::flatten(FlatSplitNode* out, FlatSplitTri* tris) const
+{
 	int depth = 0;
+	size_t nt = 0;
 	std::vector<std::pair<int32_t, int>> st;
 	if (!nodes_.empty())
 		st.push_back({0, 1});
-	out.resize(base + nodes_.size());
 	while (!st.empty()) {
 		auto [k, d] = st.back();
 		st.pop_back();
 		depth = std::max(depth, d);
 		const Node& n = nodes_[k];
-		FlatSplitNode& f = out[base + k];
+		FlatSplitNode& f = out[k];
 		f.nx = n.N.x; f.ny = n.N.y; f.nz = n.N.z; f.d = n.D;
 		f.front = n.front;
 		f.back = n.back;
@@ -462,13 +470,13 @@ int SplitTree::flatten(std::vector<FlatSplitNode>& out, std::vector<FlatSplitTri
 			st.push_back({n.back, d + 1});
 			continue;
 		}
-		f.leaf_off = (int32_t)(tris.size() - tbase);
+		f.leaf_off = (int32_t)nt;
 		f.leaf_cnt = (int32_t)n.rt.size();
 		for (size_t q = 0; q < n.rt.size(); ++q) {
 			const Tri& t = n.tris[n.rti[q]];
 			// the normal node_intersect reports for this triangle
 			f3 hn = normalized(cross(t.p3 - t.p1, t.p2 - t.p1));
-			tris.push_back(FlatSplitTri{n.rt[q], hn.x, hn.y, hn.z, 0.0f});
+			tris[nt++] = FlatSplitTri{n.rt[q], hn.x, hn.y, hn.z, 0.0f};
 		}
 	}
 	return depth;

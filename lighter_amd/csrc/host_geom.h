@@ -176,9 +176,12 @@ class SplitTree {
 public:
 	void set(const Tri* tris, size_t count);
 	bool empty() const { return nodes_.empty(); }
-	// Appends the flat form (indices local to this tree); returns the depth
-	// (nodes on the longest root-to-leaf path, 0 for an empty tree).
-	int flatten(std::vector<FlatSplitNode>& nodes, std::vector<FlatSplitTri>& tris) const;
+	size_t num_nodes() const { return nodes_.size(); }
+	size_t num_leaf_tris() const;
+	// Writes the flat form (num_nodes() nodes, num_leaf_tris() triangles;
+	// indices local to this tree); returns the depth (nodes on the longest
+	// root-to-leaf path, 0 for an empty tree).
+	int flatten(FlatSplitNode* nodes, FlatSplitTri* tris) const;
 	// Closest hit in the first subtree along the segment that has one
 	// (lighter_math.cpp:490-533); 2 when nothing is hit.
 	float intersect(f3 from, f3 to, f3* outnormal) const;

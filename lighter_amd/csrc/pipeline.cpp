@@ -287,7 +287,7 @@ static void place_gpu(ltr_Scene* S, std::vector<RasterSamples>& raw)
 	const ltr_Config& cfg = S->config;
 	float corr_min_dot = cosf(cfg.max_correct_angle / 180.0f * kPiF);
 	place_samples_gpu(pi, S->inst_tree, pos.data(), nrm.data(), dist.data(), total, cfg.max_correct_dist,
-	                  corr_min_dot, &S->place_times);
+	                  corr_min_dot, S->nthreads, &S->place_times);
 	k = 0;
 	for (RasterSamples& r : raw)
 		for (size_t i = 0; i < r.pos.size(); ++i, ++k)

@@ -32,10 +32,11 @@ struct PlaceTimes {
 // pos: 3 floats per sample, moved in place; nrm: 3 floats per sample
 // (normalised); dist: per sample, the square root of the texel's world area.
 // max_corr and corr_min_dot as the config gives them (corr_min_dot is the
-// cosine of max_correct_angle).  Throws std::runtime_error on HIP errors or
-// when a tree is deeper than the kernel's stacks.
+// cosine of max_correct_angle).  The trees are flattened on nthreads host
+// threads.  Throws std::runtime_error on HIP errors or when a tree is deeper
+// than the kernel's stacks.
 void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, float* pos,
                        const float* nrm, const float* dist, size_t n, float max_corr, float corr_min_dot,
-                       PlaceTimes* times);
+                       int nthreads, PlaceTimes* times);
 
 } // namespace lgx

@@ -17,9 +17,14 @@
 // its cost is dominated by divergent tree walks, not by memory traffic.
 #include <hip/hip_runtime.h>
 
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "place.h"
 
@@ -284,23 +289,29 @@ namespace {
 using Clock = std::chrono::steady_clock;
 double ms_since(Clock::time_point t0) { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
 
-template <class T> struct DevBuf {
-	T* p = nullptr;
-	DevBuf() = default;
-	DevBuf(const DevBuf&) = delete;
-	DevBuf& operator=(const DevBuf&) = delete;
-	~DevBuf()
+// Pinned host block (hipHostMalloc: the upload runs at DMA speed).
+struct PinnedBlock {
+	char* p = nullptr;
+	explicit PinnedBlock(size_t bytes) { PLACE_CHECK(hipHostMalloc((void**)&p, bytes ? bytes : 1, hipHostMallocDefault)); }
+	PinnedBlock(const PinnedBlock&) = delete;
+	PinnedBlock& operator=(const PinnedBlock&) = delete;
+	~PinnedBlock()
+	{
+		if (p)
+			(void)hipHostFree(p);
+	}
+};
+
+struct DevBlock {
+	char* p = nullptr;
+	explicit DevBlock(size_t bytes) { PLACE_CHECK(hipMalloc((void**)&p, bytes ? bytes : 1)); }
+	DevBlock(const DevBlock&) = delete;
+	DevBlock& operator=(const DevBlock&) = delete;
+	~DevBlock()
 	{
 		if (p)
 			(void)hipFree(p);
 	}
-	void upload(const T* src, size_t n)
-	{
-		PLACE_CHECK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
-		if (n)
-			PLACE_CHECK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
-	}
-	void upload(const std::vector<T>& v) { upload(v.data(), v.size()); }
 };
 
 void check_depth(int depth, int cap, const char* what)
@@ -310,30 +321,31 @@ void check_depth(int depth, int cap, const char* what)
 		                         " exceeds the kernel's stack of " + std::to_string(cap));
 }
 
-// The per-instance trees and triangles in shared arrays, instance ranges
-// in PlaceInst.
-struct PlaceData {
-	std::vector<PlaceInst> inst;
-	std::vector<BoxTree::Node> nodes;
-	std::vector<int32_t> items;
-	std::vector<PlaceTri> tris;
-	std::vector<FlatSplitNode> snodes;
-	std::vector<FlatSplitTri> stris;
+// All placement inputs in one block (one pinned host copy, one device copy):
+// the instance records, the instances' BoxTree nodes, item lists and
+// triangles, their split trees, the instance tree and the samples.  Byte
+// offsets of each array, aligned to 256.
+struct PlaceLayout {
+	size_t inst, nodes, items, tris, snodes, stris, inodes, iitems, pos, nrm, dist, bytes;
 };
 
-void flatten(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, PlaceData& D)
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Sizes, offsets and the instance records (serial; sizes only).
+PlaceLayout plan(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, size_t n,
+                 std::vector<PlaceInst>& inst)
 {
-	// flatten: per-instance ranges into shared arrays, sizes first
-	D.inst.resize(insts.size());
-	std::vector<PlaceInst>& inst = D.inst;
-	size_t nn = 0, ni = 0, nt = 0;
+	inst.assign(insts.size(), PlaceInst{});
+	size_t nn = 0, ni = 0, nt = 0, sn = 0, stt = 0;
 	for (size_t k = 0; k < insts.size(); ++k) {
 		const BoxTree& bt = insts[k].tris->tree();
-		check_depth(bt.depth(), kBoxStack, "triangle tree");
 		PlaceInst& I = inst[k];
 		I.node_off = (int32_t)nn;
 		I.item_off = (int32_t)ni;
 		I.tri_off = (int32_t)nt;
+		I.split_off = (int32_t)sn;
+		I.stri_off = (int32_t)stt;
+		I.nsplit = (int32_t)insts[k].split->num_nodes();
 		I.shadow = insts[k].shadow ? 1 : 0;
 		I.pad = 0;
 		I.mn = bt.root().mn;
@@ -341,43 +353,96 @@ void flatten(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, 
 		nn += bt.nodes().size();
 		ni += bt.items().size();
 		nt += insts[k].tris->tris().size();
+		sn += insts[k].split->num_nodes();
+		stt += insts[k].split->num_leaf_tris();
+		if (nn > 0x7FFFFFFF || ni > 0x7FFFFFFF || nt > 0x7FFFFFFF || sn > 0x7FFFFFFF || stt > 0x7FFFFFFF)
+			throw std::runtime_error("sample placement: collision data beyond 2^31 entries");
 	}
-	if (nn > 0x7FFFFFFF || ni > 0x7FFFFFFF || nt > 0x7FFFFFFF)
-		throw std::runtime_error("sample placement: collision data beyond 2^31 entries");
-	D.nodes.resize(nn);
-	D.items.resize(ni);
-	D.tris.resize(nt);
-	std::vector<BoxTree::Node>& nodes = D.nodes;
-	std::vector<int32_t>& items = D.items;
-	std::vector<PlaceTri>& tris = D.tris;
-	std::vector<FlatSplitNode>& snodes = D.snodes;
-	std::vector<FlatSplitTri>& stris = D.stris;
-	for (size_t k = 0; k < insts.size(); ++k) {
-		const TriSet& ts = *insts[k].tris;
-		const BoxTree& bt = ts.tree();
-		PlaceInst& I = inst[k];
-		std::copy(bt.nodes().begin(), bt.nodes().end(), nodes.begin() + I.node_off);
-		std::copy(bt.items().begin(), bt.items().end(), items.begin() + I.item_off);
-		const std::vector<Tri>& tv = ts.tris();
-		const std::vector<Box>& bv = ts.boxes();
-		for (size_t j = 0; j < tv.size(); ++j)
-			tris[I.tri_off + j] = PlaceTri{tv[j].p1, tv[j].p2, tv[j].p3, bv[j].mn, bv[j].mx};
-		I.split_off = (int32_t)snodes.size();
-		I.stri_off = (int32_t)stris.size();
-		const int sd = insts[k].split->flatten(snodes, stris);
-		check_depth(sd, kSplitStack, "split tree");
-		I.nsplit = (int32_t)(snodes.size() - I.split_off);
-	}
-	if (snodes.size() > 0x7FFFFFFF || stris.size() > 0x7FFFFFFF)
-		throw std::runtime_error("sample placement: split trees beyond 2^31 entries");
-	check_depth(inst_tree.depth(), kBoxStack, "instance tree");
+	PlaceLayout L;
+	size_t o = 0;
+	auto take = [&](size_t bytes) {
+		size_t at = o;
+		o = align256(o + bytes);
+		return at;
+	};
+	L.inst = take(inst.size() * sizeof(PlaceInst));
+	L.nodes = take(nn * sizeof(BoxTree::Node));
+	L.items = take(ni * sizeof(int32_t));
+	L.tris = take(nt * sizeof(PlaceTri));
+	L.snodes = take(sn * sizeof(FlatSplitNode));
+	L.stris = take(stt * sizeof(FlatSplitTri));
+	L.inodes = take(inst_tree.nodes().size() * sizeof(BoxTree::Node));
+	L.iitems = take(inst_tree.items().size() * sizeof(int32_t));
+	L.pos = take(3 * n * sizeof(float));
+	L.nrm = take(3 * n * sizeof(float));
+	L.dist = take(n * sizeof(float));
+	L.bytes = o;
+	return L;
+}
+
+// Writes the block at base (instances in parallel on nthreads threads);
+// throws when a tree is deeper than the kernel's stacks.
+void fill(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, const std::vector<PlaceInst>& inst,
+          const PlaceLayout& L, char* base, int nthreads)
+{
+	memcpy(base + L.inst, inst.data(), inst.size() * sizeof(PlaceInst));
+	std::copy(inst_tree.nodes().begin(), inst_tree.nodes().end(), (BoxTree::Node*)(base + L.inodes));
+	std::copy(inst_tree.items().begin(), inst_tree.items().end(), (int32_t*)(base + L.iitems));
+	std::atomic<size_t> next{0};
+	std::atomic<int> box_depth{inst_tree.depth()}, split_depth{0};
+	auto work = [&]() {
+		for (size_t k; (k = next.fetch_add(1)) < insts.size();) {
+			const TriSet& ts = *insts[k].tris;
+			const BoxTree& bt = ts.tree();
+			const PlaceInst& I = inst[k];
+			std::copy(bt.nodes().begin(), bt.nodes().end(), (BoxTree::Node*)(base + L.nodes) + I.node_off);
+			std::copy(bt.items().begin(), bt.items().end(), (int32_t*)(base + L.items) + I.item_off);
+			const std::vector<Tri>& tv = ts.tris();
+			const std::vector<Box>& bv = ts.boxes();
+			PlaceTri* tris = (PlaceTri*)(base + L.tris) + I.tri_off;
+			for (size_t j = 0; j < tv.size(); ++j)
+				tris[j] = PlaceTri{tv[j].p1, tv[j].p2, tv[j].p3, bv[j].mn, bv[j].mx};
+			const int sd = insts[k].split->flatten((FlatSplitNode*)(base + L.snodes) + I.split_off,
+			                                       (FlatSplitTri*)(base + L.stris) + I.stri_off);
+			const int bd = bt.depth();
+			for (int cur = box_depth.load(); bd > cur && !box_depth.compare_exchange_weak(cur, bd);) {
+			}
+			for (int cur = split_depth.load(); sd > cur && !split_depth.compare_exchange_weak(cur, sd);) {
+			}
+		}
+	};
+	const int nt = std::max(1, std::min<int>(nthreads, (int)insts.size()));
+	std::vector<std::thread> th;
+	for (int t = 1; t < nt; ++t)
+		th.emplace_back(work);
+	work();
+	for (auto& t : th)
+		t.join();
+	check_depth(box_depth.load(), kBoxStack, "triangle or instance tree");
+	check_depth(split_depth.load(), kSplitStack, "split tree");
+}
+
+// A view of the block at base (host or device).
+PlaceView view_of(const char* base, const PlaceLayout& L, int32_t ninst)
+{
+	PlaceView V;
+	V.nodes = (const BoxTree::Node*)(base + L.nodes);
+	V.items = (const int32_t*)(base + L.items);
+	V.tris = (const PlaceTri*)(base + L.tris);
+	V.snodes = (const FlatSplitNode*)(base + L.snodes);
+	V.stris = (const FlatSplitTri*)(base + L.stris);
+	V.inst = (const PlaceInst*)(base + L.inst);
+	V.inodes = (const BoxTree::Node*)(base + L.inodes);
+	V.iitems = (const int32_t*)(base + L.iitems);
+	V.ninst = ninst;
+	return V;
 }
 
 } // namespace
 
 void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, float* pos,
                        const float* nrm, const float* dist, size_t n, float max_corr, float corr_min_dot,
-                       PlaceTimes* times)
+                       int nthreads, PlaceTimes* times)
 {
 	PlaceTimes tm;
 	tm.samples = n;
@@ -389,41 +454,22 @@ void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& i
 	if (n > 0xFFFFFFFFull)
 		throw std::runtime_error("sample placement: more than 2^32 samples");
 	auto t0 = Clock::now();
-	PlaceData D;
-	flatten(insts, inst_tree, D);
+	std::vector<PlaceInst> inst;
+	const PlaceLayout L = plan(insts, inst_tree, n, inst);
+	PinnedBlock host(L.bytes);
+	fill(insts, inst_tree, inst, L, host.p, nthreads);
+	memcpy(host.p + L.pos, pos, 3 * n * sizeof(float));
+	memcpy(host.p + L.nrm, nrm, 3 * n * sizeof(float));
+	memcpy(host.p + L.dist, dist, n * sizeof(float));
 	tm.flatten_ms = ms_since(t0);
 
 	t0 = Clock::now();
-	DevBuf<BoxTree::Node> d_nodes, d_inodes;
-	DevBuf<int32_t> d_items, d_iitems;
-	DevBuf<PlaceTri> d_tris;
-	DevBuf<FlatSplitNode> d_snodes;
-	DevBuf<FlatSplitTri> d_stris;
-	DevBuf<PlaceInst> d_inst;
-	DevBuf<float> d_pos, d_nrm, d_dist;
-	d_nodes.upload(D.nodes);
-	d_items.upload(D.items);
-	d_tris.upload(D.tris);
-	d_snodes.upload(D.snodes);
-	d_stris.upload(D.stris);
-	d_inst.upload(D.inst);
-	d_inodes.upload(inst_tree.nodes());
-	d_iitems.upload(inst_tree.items());
-	d_pos.upload(pos, 3 * n);
-	d_nrm.upload(nrm, 3 * n);
-	d_dist.upload(dist, n);
+	DevBlock dev(L.bytes);
+	PLACE_CHECK(hipMemcpy(dev.p, host.p, L.bytes, hipMemcpyHostToDevice));
 	tm.upload_ms = ms_since(t0);
 
-	PlaceView V;
-	V.nodes = d_nodes.p;
-	V.items = d_items.p;
-	V.tris = d_tris.p;
-	V.snodes = d_snodes.p;
-	V.stris = d_stris.p;
-	V.inst = d_inst.p;
-	V.inodes = d_inodes.p;
-	V.iitems = d_iitems.p;
-	V.ninst = (int32_t)insts.size();
+	const PlaceView V = view_of(dev.p, L, (int32_t)insts.size());
+	float* d_pos = (float*)(dev.p + L.pos);
 	hipStream_t s;
 	PLACE_CHECK(hipStreamCreate(&s));
 	hipEvent_t e0, e1;
@@ -431,7 +477,8 @@ void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& i
 	PLACE_CHECK(hipEventCreate(&e1));
 	const unsigned grid = (unsigned)((n + kPlaceBlock - 1) / kPlaceBlock);
 	PLACE_CHECK(hipEventRecord(e0, s));
-	k_place<<<grid, kPlaceBlock, 0, s>>>(V, d_pos.p, d_nrm.p, d_dist.p, (uint32_t)n, max_corr, corr_min_dot);
+	k_place<<<grid, kPlaceBlock, 0, s>>>(V, d_pos, (const float*)(dev.p + L.nrm), (const float*)(dev.p + L.dist),
+	                                     (uint32_t)n, max_corr, corr_min_dot);
 	hipError_t le = hipGetLastError();
 	PLACE_CHECK(hipEventRecord(e1, s));
 	hipError_t se = hipStreamSynchronize(s);
@@ -445,7 +492,8 @@ void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& i
 	PLACE_CHECK(se);
 	tm.kernel_ms = kms;
 	t0 = Clock::now();
-	PLACE_CHECK(hipMemcpy(pos, d_pos.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+	PLACE_CHECK(hipMemcpy(host.p + L.pos, d_pos, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+	memcpy(pos, host.p + L.pos, 3 * n * sizeof(float));
 	tm.download_ms = ms_since(t0);
 	if (times)
 		*times = tm;

@@ -114,7 +114,7 @@ def placement_vs_host(lib, spec, nthreads, b_gpu, pos_gpu):
     placement: its wall time beside the GPU-placed run's, and whether every
     sample position is the same."""
     b = scenes.BuiltScene(lib, spec, dict(max_num_threads=nthreads))
-    _lib.check(lib.lgx_SetSamplePlacement(b.scene, 1), lib)
+    _lib.check(lib.lgx_SetHostStages(b.scene, 1), lib)  # host placement
     t0 = time.perf_counter()
     _lib.check(lib.lgx_Prepare(b.scene), lib)
     host_s = time.perf_counter() - t0
@@ -133,6 +133,35 @@ def placement_vs_host(lib, spec, nthreads, b_gpu, pos_gpu):
     log(f"[bench] sample placement: GPU kernel {out['kernel_ms']} ms for {out['samples']} samples; preparation "
         f"{b_gpu.prepare_s:.2f} s (GPU placement) vs {host_s:.2f} s (host placement, {nthreads} threads); "
         f"identical: {same}")
+    return out
+
+
+def export_vs_host(lib, b):
+    """The lightmap export ("exporting lightmaps") of prepared scene b on the
+    GPU, then again with the host restatement (its outputs follow the
+    first call's): both wall times and whether every image is identical.
+    The scene's lightmaps hold the ambient colour here (the device-level
+    light runs of this bench do not write them back), so this times the
+    stage; tests/test_gpu_finalize.py compares lit bakes."""
+    t0 = time.perf_counter()
+    _lib.check(lib.lgx_StageFinalize(b.scene), lib)
+    gpu_s = time.perf_counter() - t0
+    t = (C.c_double * 5)()
+    lib.lgx_FinalizeTimes(b.scene, t)
+    _lib.check(lib.lgx_SetHostStages(b.scene, 2), lib)  # host export
+    t0 = time.perf_counter()
+    _lib.check(lib.lgx_StageFinalize(b.scene), lib)
+    host_s = time.perf_counter() - t0
+    outs = b.outputs()
+    n = len(outs) // 2
+    same = len(outs) == 2 * n and all(
+        outs[k][:3] == outs[n + k][:3] and np.array_equal(outs[k][3].view(np.uint32), outs[n + k][3].view(np.uint32))
+        for k in range(n))
+    out = dict(images=n, texels=int(t[4]), kernels_ms=round(t[2], 3), pack_ms=round(t[0], 1),
+               upload_ms=round(t[1], 1), download_ms=round(t[3], 1), stage_s_gpu=round(gpu_s, 3),
+               stage_s_host=round(host_s, 3), identical_images=bool(same))
+    log(f"[bench] lightmap export: {n} images, {out['texels']} texels; GPU kernels {out['kernels_ms']} ms, stage "
+        f"{gpu_s:.3f} s vs host {host_s:.3f} s; identical: {same}")
     return out
 
 
@@ -808,6 +837,7 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
             log(f"[bench] C4 cpu baseline {cpu['value']:.0f} rays/s; GPU/CPU {rays / cpu['value']:.0f}x; "
                 f"parity {out['parity_sample']}")
     lib.lgx_DeviceDestroy(dev)
+    out["lightmap_export"] = export_vs_host(lib, built)
     built.destroy()
     out["sample_placement"] = placement_vs_host(lib, spec, nth, built, pos)
     return out
@@ -941,6 +971,7 @@ def main():
                           shadowed_agreement=round(hitmiss, 6))
             log(f"[bench] cpu baseline {cpu['value']:.0f} rays/s; sampled parity {parity}")
     lib.lgx_DeviceDestroy(dev)
+    export = export_vs_host(lib, built)
     built.destroy()
     placement = placement_vs_host(lib, spec, nth, built, pos)
     radio = None
@@ -960,6 +991,7 @@ def main():
                            work_items=int(len(items)), shadow_rays_per_step=int(marches),
                            march_steps_per_step=int(steps), march_kernel_ms=round(march_avg, 3)),
                roofline=roofline, cpu_baseline=cpu, parity_sample=parity, sample_placement=placement,
+               lightmap_export=export,
                radiosity=radio, adjoint=adjoint)
     print(json.dumps(out), flush=True)
 

@@ -33,25 +33,36 @@ LTRAPI int lgx_HasDevice(void);
  * (lighter.cpp:1050-1073): transforms, collision set-up and rasterisation
  * on the host threads; the sample placement (OffsetSample and the
  * correction in front of overlapping faces, lighter.cpp:436-466) on the GPU
- * unless lgx_SetSamplePlacement selected the host. Scenes with no texel to
+ * unless lgx_SetHostStages selected the host. Scenes with no texel to
  * place need no GPU. */
 LTRAPI int lgx_Prepare(ltr_Scene* scene);
-/* No counterpart in the reference (its placement runs in Job_Samples,
- * lighter.cpp:392-476, on the worker threads). where = 0: GPU (the
- * default, also for ltr_Start); 1: the host restatement the GPU placement is
- * tested against. Takes effect at the next preparation. */
-LTRAPI int lgx_SetSamplePlacement(ltr_Scene* scene, int where);
+/* No counterpart in the reference (it runs these stages on its worker
+ * threads). Two host stages of the reference also run on the GPU here: the
+ * sample placement (lighter.cpp:436-466) and the lightmap export's image
+ * work (extrapolation, blur, downsample; lighter.cpp:849-958). Both use the
+ * GPU by default, also under ltr_Start; a bit set in mask selects the host
+ * restatement instead, which the GPU forms are tested against. Takes effect
+ * at the next preparation or export. */
+#define LGX_HOST_PLACEMENT 1
+#define LGX_HOST_FINALIZE 2
+LTRAPI int lgx_SetHostStages(ltr_Scene* scene, int mask);
 /* No counterpart in the reference. Last GPU placement: out[0] flatten ms,
  * out[1] upload ms, out[2] kernel ms (HIP events), out[3] download ms,
  * out[4] samples placed (zeros after a host placement). */
 LTRAPI int lgx_PlacementTimes(ltr_Scene* scene, double out[5]);
+/* No counterpart in the reference. Last GPU export: out[0] pack ms, out[1]
+ * upload ms, out[2] kernels ms (HIP events), out[3] download and unpack ms,
+ * out[4] atlas texels (zeros after a host export). */
+LTRAPI int lgx_FinalizeTimes(ltr_Scene* scene, double out[5]);
 /* "rendering lightmaps" (lighter.cpp:1075-1096) on the GPU. */
 LTRAPI int lgx_StageDirect(ltr_Scene* scene);
 /* "calculating radiosity" .. "committing radiosity" (lighter.cpp:1098-1124). */
 LTRAPI int lgx_StageRadiosity(ltr_Scene* scene);
 /* "rendering ambient occlusion" (lighter.cpp:1126-1136). */
 LTRAPI int lgx_StageAO(ltr_Scene* scene);
-/* "exporting lightmaps" (lighter.cpp:1138-1140): fills the work outputs. */
+/* "exporting lightmaps" (lighter.cpp:1138-1140, 849-1044): fills the work
+ * outputs; the images (extrapolation, blur, downsample) on the GPU unless
+ * lgx_SetHostStages selected the host, the normal-map data on the host. */
 LTRAPI int lgx_StageFinalize(ltr_Scene* scene);
 
 LTRAPI int lgx_InstanceCount(ltr_Scene* scene);

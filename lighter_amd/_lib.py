@@ -29,8 +29,9 @@ LGX_SIGNATURES = {
     "lgx_LastError": (C.c_char_p, []),
     "lgx_HasDevice": (C.c_int, []),
     "lgx_Prepare": (C.c_int, [vp]),
-    "lgx_SetSamplePlacement": (C.c_int, [vp, C.c_int]),
+    "lgx_SetHostStages": (C.c_int, [vp, C.c_int]),
     "lgx_PlacementTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
+    "lgx_FinalizeTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
     "lgx_StageDirect": (C.c_int, [vp]),
     "lgx_StageRadiosity": (C.c_int, [vp]),
     "lgx_StageAO": (C.c_int, [vp]),

@@ -370,15 +370,27 @@ int lgx_StageFinalize(ltr_Scene* S)
 	});
 }
 
-int lgx_SetSamplePlacement(ltr_Scene* S, int where)
+int lgx_SetHostStages(ltr_Scene* S, int mask)
 {
 	return guarded([&] {
-		if (where != 0 && where != 1)
-			throw std::runtime_error("lgx_SetSamplePlacement: where must be 0 (GPU) or 1 (host)");
+		if (mask & ~(LGX_HOST_PLACEMENT | LGX_HOST_FINALIZE))
+			throw std::runtime_error("lgx_SetHostStages: unknown bits in mask");
 		if (S->worker.joinable() && S->stage.load() != nullptr)
 			throw std::runtime_error("a bake started with ltr_Start is running");
-		S->host_place = where == 1;
+		S->host_place = (mask & LGX_HOST_PLACEMENT) != 0;
+		S->host_finalize = (mask & LGX_HOST_FINALIZE) != 0;
 	});
+}
+
+int lgx_FinalizeTimes(ltr_Scene* S, double out[5])
+{
+	const FinTimes& t = S->fin_times;
+	out[0] = t.pack_ms;
+	out[1] = t.upload_ms;
+	out[2] = t.kernels_ms;
+	out[3] = t.download_ms;
+	out[4] = (double)t.texels;
+	return 1;
 }
 
 int lgx_PlacementTimes(ltr_Scene* S, double out[5])

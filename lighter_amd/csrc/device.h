@@ -49,6 +49,9 @@ public:
 	void set_samples(const float* pos, const float* nrm, size_t n);
 	void set_lightmap(const float* rgb);
 	void get_lightmap(float* rgb) const;
+	// the device lightmap (3 floats per sample, in sample order); valid
+	// once the stage that wrote it has completed
+	const float* lightmap_device() const { return d_lm_; }
 	size_t num_samples() const { return nsamples_; }
 	size_t num_triangles() const { return ntri_; }
 	int bvh_depth() const { return bvh_depth_; }

@@ -696,9 +696,10 @@ static void downsample2x(float* dst, unsigned dw, unsigned dh, const float* src,
 		}
 }
 
-// One instance's output (the instance's own data only, so instances run in
-// parallel).
-static ltr_WorkOutput finalize_instance(ltr_Scene* S, size_t mid)
+// The image of one instance on the host (lighter.cpp:866-958): the samples'
+// colours in their texels, extrapolation, blur, downsample (which updates
+// the instance's lightmap size).  Instances run in parallel.
+static float* image_host(ltr_Scene* S, size_t mid)
 {
 	Instance* mi = S->insts[mid];
 	const ltr_Config& cfg = S->config;
@@ -769,6 +770,15 @@ static ltr_WorkOutput finalize_instance(ltr_Scene* S, size_t mid)
 		mi->lm_w = hw;
 		mi->lm_h = hh;
 	}
+	return rgb;
+}
+
+// Normal-map data (lighter.cpp:959-1030) and the output record of one
+// instance around its finished image.
+static ltr_WorkOutput output_of(ltr_Scene* S, size_t mid, float* rgb)
+{
+	Instance* mi = S->insts[mid];
+	const ltr_Config& cfg = S->config;
 	float* normals = nullptr;
 	if (cfg.generate_normalmap_data) {
 		struct Sum {
@@ -819,6 +829,49 @@ static ltr_WorkOutput finalize_instance(ltr_Scene* S, size_t mid)
 	return wo;
 }
 
+// The images of every instance on the GPU (finalize.hip), identical to
+// image_host's.
+static void images_gpu(ltr_Scene* S, std::vector<float*>& rgb)
+{
+	const ltr_Config& cfg = S->config;
+	std::vector<FinImage> imgs;
+	std::vector<size_t> ids;
+	for (size_t mid = 0; mid < S->insts.size(); ++mid) {
+		Instance* mi = S->insts[mid];
+		if (mi->samplecont)
+			continue;
+		uint32_t fw = mi->lm_w, fh = mi->lm_h;
+		if (cfg.ds2x) {
+			fw = std::max(mi->lm_w / 2, 1u);
+			fh = std::max(mi->lm_h / 2, 1u);
+		}
+		rgb[mid] = new float[(size_t)fw * fh * 3];
+		imgs.push_back(FinImage{mi->lm_w, mi->lm_h, mi->s_loc.size(), (const float*)mi->lightmap.data(),
+		                        mi->s_loc.data(), rgb[mid], fw, fh});
+		ids.push_back(mid);
+	}
+	S->fin_times = FinTimes{};
+	if (imgs.empty())
+		return;
+	require_gpu();
+	std::vector<float> kernel;
+	if (cfg.blur_size) {
+		int ext = (int)ceilf(cfg.blur_size);
+		kernel.resize(2 * ext + 1);
+		gaussian_kernel(kernel.data(), ext, cfg.blur_size);
+	}
+	// the colours stay on the device when a GPU stage left them there: the
+	// images' samples follow the sample container's in the device order
+	const float* d_rgb = nullptr;
+	if (S->dev && S->dev->num_samples() == S->total_samples && S->inst_off.size() > 1)
+		d_rgb = S->dev->lightmap_device() + 3 * S->inst_off[1];
+	finalize_gpu(imgs, d_rgb, cfg.blur_size, kernel, cfg.ds2x != 0, S->nthreads, &S->fin_times);
+	for (size_t q = 0; q < ids.size(); ++q) {
+		S->insts[ids[q]]->lm_w = imgs[q].final_w;
+		S->insts[ids[q]]->lm_h = imgs[q].final_h;
+	}
+}
+
 void finalize(ltr_Scene* S)
 {
 	const size_t n = S->insts.size();
@@ -829,9 +882,25 @@ void finalize(ltr_Scene* S)
 			for (size_t i = 0; i < mi->lightmap.size(); ++i)
 				st3(S->samples[i].out_color, mi->lightmap[i]);
 	}
+	std::vector<float*> rgb(n, nullptr);
+	if (S->host_finalize) {
+		S->fin_times = FinTimes{};
+		parallel_for(S->nthreads, n, [&](size_t mid) {
+			if (!S->insts[mid]->samplecont)
+				rgb[mid] = image_host(S, mid);
+		});
+	} else {
+		try {
+			images_gpu(S, rgb);
+		} catch (...) {
+			for (float* p : rgb)
+				delete[] p;
+			throw;
+		}
+	}
 	parallel_for(S->nthreads, n, [&](size_t mid) {
 		if (!S->insts[mid]->samplecont)
-			outs[mid] = finalize_instance(S, mid);
+			outs[mid] = output_of(S, mid, rgb[mid]);
 	});
 	// outputs in instance order (uid = instance index), as the reference's
 	for (size_t mid = 0; mid < n; ++mid)

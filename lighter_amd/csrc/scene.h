@@ -11,6 +11,7 @@
 #include "../../include/lighter.h"
 #include "device.h"
 #include "host_geom.h"
+#include "finalize.h"
 #include "place.h"
 
 namespace lgx {
@@ -91,7 +92,9 @@ struct ltr_Scene {
 	bool prepared = false;
 	std::vector<float> vis; // per work item shadow factor, when normal-map data is requested
 	lgx::StageTimes times;
-	// sample placement: on the GPU unless the host restatement is selected
-	bool host_place = false;
+	// sample placement and lightmap export: on the GPU unless the host
+	// restatement is selected
+	bool host_place = false, host_finalize = false;
 	lgx::PlaceTimes place_times;
+	lgx::FinTimes fin_times;
 };

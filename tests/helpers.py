@@ -47,15 +47,17 @@ def prod_lightmap(lib, scene, i):
     return out
 
 
-GPU_PLACEMENT, HOST_PLACEMENT = 0, 1
+# lgx_SetHostStages bits (include/lighter_gpu.h)
+GPU_PLACEMENT, HOST_PLACEMENT, HOST_FINALIZE = 0, 1, 2
+HOST_STAGES = HOST_PLACEMENT | HOST_FINALIZE
 
 
 def prepared(lib, spec, placement=GPU_PLACEMENT, **config):
     """Build spec through the product library and run the preparation
     stages, placing the samples on the GPU (the default) or with the host
-    restatement."""
+    restatement (placement = HOST_PLACEMENT)."""
     b = scenes.BuiltScene(lib, spec, config or None)
-    _lib.check(lib.lgx_SetSamplePlacement(b.scene, placement), lib)
+    _lib.check(lib.lgx_SetHostStages(b.scene, placement), lib)
     _lib.check(lib.lgx_Prepare(b.scene), lib)
     return b
 

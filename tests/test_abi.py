@@ -130,7 +130,7 @@ def test_samples_only_scene_without_lights_finishes(lib):
     spec.samples = [((0, 0, 1), (0, 0, 1)), ((1, 1, 1), (0, 1, 0))]
     spec.config = dict(spec.config, ambient_color=[0.25, 0.5, 0.75])
     b = scenes.BuiltScene(lib, spec)
-    lib.lgx_SetSamplePlacement(b.scene, 1)
+    lib.lgx_SetHostStages(b.scene, 3)  # placement and export on the host
     times = b.bake()
     assert "rendering lightmaps" not in times
     outs = b.outputs()
@@ -161,7 +161,7 @@ def test_abort_stops_bake(lib):
     spec = scenes.room_scene(lm=64)
     spec.lights = []
     b = scenes.BuiltScene(lib, spec)
-    lib.lgx_SetSamplePlacement(b.scene, 1)  # host placement: no GPU needed here
+    lib.lgx_SetHostStages(b.scene, 3)  # placement and export on the host: no GPU needed here
     lib.ltr_Start(b.scene)
     lib.ltr_Abort(b.scene)
     st = api.WorkStatus()

@@ -668,9 +668,12 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
     host_s = time.perf_counter() - t0
     S = len(pos)
     log(f"[bench] C4 host stages: {S} samples, {len(items)} work items ({host_s:.1f} s)")
+    t0 = time.perf_counter()
     dev = lib.lgx_DeviceCreate(P(tris), len(tris))
     if not dev:
         raise RuntimeError(_lib.last_error(lib))
+    device_create_s = time.perf_counter() - t0
+    log(f"[bench] C4 device scene (BVH build, triangle forms, upload): {device_create_s:.2f} s")
     info = (C.c_int64 * 3)()
     lib.lgx_DeviceInfo(dev, info)
     _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), S, None), lib)
@@ -704,6 +707,7 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
     out = dict(workload=f"C4 city block: {spec.num_triangles} tris (1 km heightfield + 155,000 lognormal boxes in "
                         f"16 x 16 tiles), 4096^2 atlas ({S} texels), 64 point/spot lights",
                triangles=int(info[0]), samples=S, work_items=int(len(items)), host_stages_s=round(host_s, 1),
+               device_create_s=round(device_create_s, 2),
                direct=dict(shadow_rays_per_s=round(rays, 1), marches=int(marches), ms_per_step=round(ms_step, 3),
                            march_kernel_ms=round(march_avg, 3),
                            roofline=dict(bound="hbm", achieved=round(alg / (march_avg / 1e3) / 1e9, 1),
@@ -908,11 +912,14 @@ def main():
     built, pos, nrm, tris, items, light_off = prepare_workload(lib, spec, nth)
     log(f"[bench] host stages: {len(pos)} samples, {len(items)} work items ({time.perf_counter() - t0:.1f} s)")
 
+    t0 = time.perf_counter()
     dev = lib.lgx_DeviceCreate(P(tris), len(tris))
     if not dev:
         raise RuntimeError(_lib.last_error(lib))
+    device_create_s = time.perf_counter() - t0
     info = (C.c_int64 * 3)()
     lib.lgx_DeviceInfo(dev, info)
+    log(f"[bench] device scene (BVH build, triangle forms, upload): {device_create_s:.2f} s")
     _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), len(pos), None), lib)
     lights = make_lights(spec)
     _lib.check(lib.lgx_DirectLightSetup(dev, lights, len(lights), items.ctypes.data_as(u32p), len(items),
@@ -989,7 +996,8 @@ def main():
                                      "(range 300 m)") if not args.small else "C1-small",
                            triangles=int(info[0]), samples=int(len(pos)), lights=len(spec.lights),
                            work_items=int(len(items)), shadow_rays_per_step=int(marches),
-                           march_steps_per_step=int(steps), march_kernel_ms=round(march_avg, 3)),
+                           march_steps_per_step=int(steps), march_kernel_ms=round(march_avg, 3),
+                           host_stages_s=round(built.prepare_s, 2), device_create_s=round(device_create_s, 2)),
                roofline=roofline, cpu_baseline=cpu, parity_sample=parity, sample_placement=placement,
                lightmap_export=export,
                radiosity=radio, adjoint=adjoint)

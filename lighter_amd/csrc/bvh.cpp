@@ -5,6 +5,9 @@
 #include <math.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <thread>
 
 namespace lgx {
 
@@ -55,6 +58,62 @@ struct Builder {
 	std::vector<BvhNode> nodes;
 	int max_depth = 0;
 
+	// Subtrees below the top levels are built as independent tasks (their
+	// triangle ranges are disjoint) and spliced back in depth-first order,
+	// which gives the same node array as building everything in one pass.
+	struct Child {
+		int kind; // 0: final reference (leaf), 1: top node, 2: task
+		int32_t ref;
+	};
+	struct TopNode {
+		Aabb lb, rb;
+		Child c[2];
+	};
+	struct Task {
+		uint32_t b, e;
+		int depth;
+		std::vector<BvhNode> nodes;
+		int32_t ref = 0;
+		int maxd = 0;
+	};
+	std::vector<TopNode> top;
+	std::vector<Task> tasks;
+
+	std::mutex mu; // guards top, tasks and max_depth while the top levels build
+
+	// A pass over idx[b, e) on up to par threads (bounds and bins merge
+	// exactly: minima, maxima and counts).
+	static constexpr uint32_t kParMin = 1u << 16;
+
+	template <class T, class Fn, class Merge>
+	T reduce(uint32_t b, uint32_t e, int par, const T& init, Fn fn, Merge merge) const
+	{
+		const uint32_t n = e - b;
+		const int nt = (n >= kParMin && par > 1) ? par : 1;
+		if (nt == 1) {
+			T r = init;
+			fn(b, e, r);
+			return r;
+		}
+		std::vector<T> part(nt, init);
+		std::vector<std::thread> th;
+		for (int t = 0; t < nt; ++t) {
+			const uint32_t cb = b + (uint32_t)((uint64_t)n * t / nt), ce = b + (uint32_t)((uint64_t)n * (t + 1) / nt);
+			th.emplace_back([&, t, cb, ce] { fn(cb, ce, part[t]); });
+		}
+		for (auto& x : th)
+			x.join();
+		T r = init;
+		for (const T& q : part)
+			merge(r, q);
+		return r;
+	}
+
+	struct Bins {
+		Aabb bb[3][kBins];
+		uint32_t cnt[3][kBins];
+	};
+
 	// Pads a box outward so that float rounding in the kernels' box tests
 	// never rejects a point of a contained triangle.
 	static Aabb padded(const Aabb& b)
@@ -82,43 +141,76 @@ struct Builder {
 		}
 	}
 
-	Aabb bounds(uint32_t b, uint32_t e) const
+	Aabb bounds(uint32_t b, uint32_t e, int par = 1) const
 	{
-		Aabb r;
-		r.reset();
-		for (uint32_t i = b; i < e; ++i)
-			r.grow(box[idx[i]]);
-		return r;
+		Aabb init;
+		init.reset();
+		return reduce(
+		    b, e, par, init,
+		    [&](uint32_t cb, uint32_t ce, Aabb& r) {
+			    for (uint32_t i = cb; i < ce; ++i)
+				    r.grow(box[idx[i]]);
+		    },
+		    [](Aabb& r, const Aabb& q) { r.grow(q); });
 	}
 
-	// Returns a child reference for triangles idx[b, e).
-	int32_t build(uint32_t b, uint32_t e, int depth)
+	// Partitions idx[b, e) (more than kLeafMax triangles) and returns the
+	// split position.
+	uint32_t split(uint32_t b, uint32_t e, int depth, int par = 1)
 	{
-		max_depth = std::max(max_depth, depth);
 		uint32_t n = e - b;
-		if (n <= (uint32_t)kLeafMax)
-			return leaf_ref(b, n);
-		Aabb cb;
-		cb.reset();
-		for (uint32_t i = b; i < e; ++i)
-			cb.grow(&cen[3 * idx[i]]);
-		// binned SAH over the widest centroid axes
+		Aabb init;
+		init.reset();
+		const Aabb cb = reduce(
+		    b, e, par, init,
+		    [&](uint32_t rb, uint32_t re, Aabb& r) {
+			    for (uint32_t i = rb; i < re; ++i)
+				    r.grow(&cen[3 * idx[i]]);
+		    },
+		    [](Aabb& r, const Aabb& q) { r.grow(q); });
+		// bins of every axis with a centroid extent, in one pass
+		Bins b0;
+		for (int a = 0; a < 3; ++a)
+			for (int k = 0; k < kBins; ++k) {
+				b0.bb[a][k].reset();
+				b0.cnt[a][k] = 0;
+			}
+		bool use[3];
+		float sc[3];
+		for (int a = 0; a < 3; ++a) {
+			float ext = cb.hi[a] - cb.lo[a];
+			use[a] = ext > 0;
+			sc[a] = use[a] ? kBins / ext : 0.0f;
+		}
+		const Bins bins = reduce(
+		    b, e, par, b0,
+		    [&](uint32_t rb, uint32_t re, Bins& r) {
+			    for (uint32_t i = rb; i < re; ++i) {
+				    const uint32_t t = idx[i];
+				    for (int a = 0; a < 3; ++a) {
+					    if (!use[a])
+						    continue;
+					    int k = std::min(kBins - 1, (int)((cen[3 * t + a] - cb.lo[a]) * sc[a]));
+					    r.cnt[a][k]++;
+					    r.bb[a][k].grow(box[t]);
+				    }
+			    }
+		    },
+		    [](Bins& r, const Bins& q) {
+			    for (int a = 0; a < 3; ++a)
+				    for (int k = 0; k < kBins; ++k) {
+					    r.bb[a][k].grow(q.bb[a][k]);
+					    r.cnt[a][k] += q.cnt[a][k];
+				    }
+		    });
+		// binned SAH over the axes with a centroid extent
 		float best_cost = FLT_MAX;
 		int best_axis = -1, best_split = -1;
 		for (int a = 0; a < 3; ++a) {
-			float ext = cb.hi[a] - cb.lo[a];
-			if (!(ext > 0))
+			if (!use[a])
 				continue;
-			Aabb bb[kBins];
-			uint32_t cnt[kBins] = {0};
-			for (int k = 0; k < kBins; ++k)
-				bb[k].reset();
-			float sc = kBins / ext;
-			for (uint32_t i = b; i < e; ++i) {
-				int k = std::min(kBins - 1, (int)((cen[3 * idx[i] + a] - cb.lo[a]) * sc));
-				cnt[k]++;
-				bb[k].grow(box[idx[i]]);
-			}
+			const Aabb* bb = bins.bb[a];
+			const uint32_t* cnt = bins.cnt[a];
 			float ra[kBins];
 			uint32_t rc[kBins];
 			Aabb acc;
@@ -168,14 +260,117 @@ struct Builder {
 			if (mid == b || mid == e)
 				mid = b + n / 2;
 		}
-		int32_t self = (int32_t)nodes.size();
-		nodes.push_back(BvhNode{});
+		return mid;
+	}
+
+	// Returns a child reference for triangles idx[b, e), appending the
+	// subtree's nodes to out (indices relative to out).
+	int32_t build(uint32_t b, uint32_t e, int depth, std::vector<BvhNode>& out, int& maxd)
+	{
+		maxd = std::max(maxd, depth);
+		uint32_t n = e - b;
+		if (n <= (uint32_t)kLeafMax)
+			return leaf_ref(b, n);
+		const uint32_t mid = split(b, e, depth);
+		int32_t self = (int32_t)out.size();
+		out.push_back(BvhNode{});
 		Aabb lb = bounds(b, mid), rb = bounds(mid, e);
-		int32_t l = build(b, mid, depth + 1);
-		int32_t r = build(mid, e, depth + 1);
-		set_child(nodes[self], 0, lb, l);
-		set_child(nodes[self], 1, rb, r);
+		int32_t l = build(b, mid, depth + 1, out, maxd);
+		int32_t r = build(mid, e, depth + 1, out, maxd);
+		set_child(out[self], 0, lb, l);
+		set_child(out[self], 1, rb, r);
 		return self;
+	}
+
+	// The top levels: subtrees of at most grain triangles become tasks.  par
+	// threads share the work: the passes over a range, then the two halves
+	// (the left one on a new thread).  Only the tree structure, not the
+	// order in which top nodes and tasks are recorded, decides the output.
+	Child build_top(uint32_t b, uint32_t e, int depth, uint32_t grain, int par)
+	{
+		uint32_t n = e - b;
+		{
+			std::lock_guard<std::mutex> g(mu);
+			max_depth = std::max(max_depth, depth);
+			if (n > (uint32_t)kLeafMax && n <= grain) {
+				tasks.push_back(Task{b, e, depth, {}, 0, 0});
+				return Child{2, (int32_t)tasks.size() - 1};
+			}
+		}
+		if (n <= (uint32_t)kLeafMax)
+			return Child{0, leaf_ref(b, n)};
+		const uint32_t mid = split(b, e, depth, par);
+		TopNode node;
+		node.lb = bounds(b, mid, par);
+		node.rb = bounds(mid, e, par);
+		Child l, r;
+		if (par > 1) {
+			const int pl = par / 2;
+			std::thread th([&] { l = build_top(b, mid, depth + 1, grain, pl); });
+			r = build_top(mid, e, depth + 1, grain, par - pl);
+			th.join();
+		} else {
+			l = build_top(b, mid, depth + 1, grain, 1);
+			r = build_top(mid, e, depth + 1, grain, 1);
+		}
+		node.c[0] = l;
+		node.c[1] = r;
+		std::lock_guard<std::mutex> g(mu);
+		top.push_back(node);
+		return Child{1, (int32_t)top.size() - 1};
+	}
+
+	// Depth-first splice of the top nodes and the tasks' node blocks.
+	int32_t emit(Child c)
+	{
+		if (c.kind == 0)
+			return c.ref;
+		if (c.kind == 2) {
+			const Task& t = tasks[c.ref];
+			const int32_t off = (int32_t)nodes.size();
+			for (BvhNode nd : t.nodes) {
+				if (nd.c0 >= 0)
+					nd.c0 += off;
+				if (nd.c1 >= 0)
+					nd.c1 += off;
+				nodes.push_back(nd);
+			}
+			return t.ref >= 0 ? t.ref + off : t.ref;
+		}
+		const TopNode& T = top[c.ref];
+		const int32_t self = (int32_t)nodes.size();
+		nodes.push_back(BvhNode{});
+		const int32_t l = emit(T.c[0]);
+		const int32_t r = emit(T.c[1]);
+		set_child(nodes[self], 0, T.lb, l);
+		set_child(nodes[self], 1, T.rb, r);
+		return self;
+	}
+
+	void build_all(uint32_t ntri, int nthreads)
+	{
+		const uint32_t grain = std::max<uint32_t>(4096, ntri / (uint32_t)(8 * std::max(nthreads, 1)));
+		const Child root = build_top(0, ntri, 0, grain, nthreads);
+		std::atomic<size_t> next{0};
+		auto work = [&]() {
+			for (size_t k; (k = next.fetch_add(1)) < tasks.size();) {
+				Task& t = tasks[k];
+				t.ref = build(t.b, t.e, t.depth, t.nodes, t.maxd);
+			}
+		};
+		std::vector<std::thread> th;
+		for (int i = 1; i < nthreads && (size_t)i < tasks.size(); ++i)
+			th.emplace_back(work);
+		work();
+		for (auto& x : th)
+			x.join();
+		for (const Task& t : tasks)
+			max_depth = std::max(max_depth, t.maxd);
+		size_t total = top.size();
+		for (const Task& t : tasks)
+			total += t.nodes.size();
+		nodes.reserve(total);
+		emit(root); // the root is an inner node (more than kLeafMax triangles): index 0
 	}
 };
 
@@ -298,7 +493,8 @@ BvhBuild build_bvh(const float* tris, size_t ntri)
 		B.nodes[0].hi1x = B.nodes[0].hi1y = B.nodes[0].hi1z = -FLT_MAX;
 		B.nodes[0].c1 = leaf_ref(0, 1);
 	} else {
-		B.build(0, (uint32_t)ntri, 0); // pushes the root first, at index 0
+		const unsigned hw = std::thread::hardware_concurrency();
+		B.build_all((uint32_t)ntri, (int)std::min(16u, std::max(1u, hw)));
 	}
 	Collapser W{B.nodes, out.nodes};
 	out.nodes.reserve(B.nodes.size() / 2 + 1);

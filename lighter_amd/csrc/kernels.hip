@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "bvh.h"
@@ -1234,21 +1235,43 @@ void DeviceScene::set_triangles(const float* tris, size_t ntri)
 	bvh_depth_ = b.depth;
 	persist_blocks_ = 0; // the march kernel's LDS stack follows the depth
 	nnodes_ = b.nodes.size();
-	std::vector<DistTri> dt(ntri);
-	std::vector<RayTri> rt(ntri);
-	for (size_t i = 0; i < ntri; ++i) {
-		const float* t = tris + 9 * (size_t)b.order[i];
-		f3 a = ld3(t), c = ld3(t + 3), e = ld3(t + 6);
-		dt[i] = make_dist_tri(a, c, e);
-		rt[i] = make_ray_tri(a, c, e);
+	// the triangles' precomputed forms, in leaf order, written by the host
+	// threads straight into pinned memory (one DMA each)
+	const size_t nb = nnodes_ * sizeof(WideNode), db = ntri * sizeof(DistTri), rb = ntri * sizeof(RayTri);
+	char* pin = nullptr;
+	HIPCHECK(hipHostMalloc((void**)&pin, nb + db + rb, hipHostMallocDefault));
+	struct PinGuard {
+		char* p;
+		~PinGuard() { (void)hipHostFree(p); }
+	} pg{pin};
+	memcpy(pin, b.nodes.data(), nb);
+	DistTri* dt = (DistTri*)(pin + nb);
+	RayTri* rt = (RayTri*)(pin + nb + db);
+	{
+		const unsigned hw = std::thread::hardware_concurrency();
+		const size_t nt = std::min<size_t>(std::min(16u, std::max(1u, hw)), (ntri + 4095) / 4096);
+		auto fill = [&](size_t t) {
+			for (size_t i = ntri * t / nt; i < ntri * (t + 1) / nt; ++i) {
+				const float* p = tris + 9 * (size_t)b.order[i];
+				f3 a = ld3(p), c = ld3(p + 3), e = ld3(p + 6);
+				dt[i] = make_dist_tri(a, c, e);
+				rt[i] = make_ray_tri(a, c, e);
+			}
+		};
+		std::vector<std::thread> th;
+		for (size_t t = 1; t < nt; ++t)
+			th.emplace_back(fill, t);
+		fill(0);
+		for (auto& x : th)
+			x.join();
 	}
 	d_nodes_ = dalloc<WideNode>(nnodes_);
 	d_dtri_ = dalloc<DistTri>(ntri);
 	d_rtri_ = dalloc<RayTri>(ntri);
 	hipStream_t s = (hipStream_t)stream_;
-	HIPCHECK(hipMemcpyAsync(d_nodes_, b.nodes.data(), nnodes_ * sizeof(WideNode), hipMemcpyHostToDevice, s));
-	HIPCHECK(hipMemcpyAsync(d_dtri_, dt.data(), ntri * sizeof(DistTri), hipMemcpyHostToDevice, s));
-	HIPCHECK(hipMemcpyAsync(d_rtri_, rt.data(), ntri * sizeof(RayTri), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_nodes_, pin, nb, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_dtri_, dt, db, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_rtri_, rt, rb, hipMemcpyHostToDevice, s));
 	HIPCHECK(hipStreamSynchronize(s));
 	lo_[0] = b.lo.x; lo_[1] = b.lo.y; lo_[2] = b.lo.z;
 	hi_[0] = b.hi.x; hi_[1] = b.hi.y; hi_[2] = b.hi.z;

@@ -39,6 +39,7 @@ u32p = C.POINTER(C.c_uint32)
 
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 112  # 4-wide node: six box planes + child references (seven dwordx4 loads)
+RAY_TRI_BYTES = 64  # RayTri: the precomputed segment-test operands
 DIST_TRI_BYTES = 160
 TRI_BOX_BYTES = 48  # the vertex quads a distance query's box pre-test reads
 METRIC = "shadow rays/s + s per radiosity bounce @ stated error; GPU plain vs MFMA/adjoint"
@@ -369,6 +370,28 @@ def links_bench(lib, nth, lm=64):
     return out
 
 
+def ao_roofline(lib, dev, ro, S, n, distance, ms):
+    """Algorithmic bytes of one AO launch (wide-node visits x 112 B + segment
+    tests x 64 B + per texel 52 B: position, normal, spiral offset, lightmap
+    read and write), counted by an instrumented launch, over the timed
+    launch's duration ms."""
+    col = np.zeros(3, np.float32)
+    lib.lgx_SetCounting(dev, 1)
+    _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), n, distance, 1.0, 0.0, 0.0, P(col)), lib)
+    cnt = (C.c_uint64 * 6)()
+    lib.lgx_ReadCounters(dev, cnt)
+    lib.lgx_SetCounting(dev, 0)
+    nodes, tests = int(cnt[0]), int(cnt[1])
+    alg = nodes * NODE_BYTES + tests * RAY_TRI_BYTES + S * 52
+    achieved = alg / (ms / 1e3) / 1e9
+    return dict(bound="cache", achieved=round(achieved, 1), unit="GB/s", hbm_peak=HBM_PEAK_GBS,
+                x_hbm_peak=round(achieved / HBM_PEAK_GBS, 4), kernel="k_ao", node_visits=nodes,
+                triangle_tests=tests, alg_bytes_per_launch=int(alg),
+                note="2 m rays visit about 12 wide nodes each near their origin; the nodes and segment operands "
+                     "they read stay in L2 and the Infinity Cache, so the algorithmic bytes are consumed faster "
+                     "than HBM could supply them and HBM is not the bound")
+
+
 def ao_bench(lib, dev, spec, S, nth, with_cpu):
     """C2 ambient occlusion: 64 cosine-weighted rays of 2 m per texel on the
     device's S texels (U2 = one ray), beside the reference's AO stage on all
@@ -383,7 +406,8 @@ def ao_bench(lib, dev, spec, S, nth, with_cpu):
         lib.lgx_DeviceTimes(dev, tm)
         ms.append(tm[2])
     best = float(np.median(ms))
-    out = dict(texels=S, rays=S * 64, ms=round(best, 3), rays_per_s=round(S * 64 / (best / 1e3), 1))
+    out = dict(texels=S, rays=S * 64, ms=round(best, 3), rays_per_s=round(S * 64 / (best / 1e3), 1),
+               roofline=ao_roofline(lib, dev, ro, S, 64, 2.0, best))
     if with_cpu:
         from oracle.checkers import Reference, reference_available
         if reference_available():
@@ -720,8 +744,11 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
     _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
     _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
     lib.lgx_DeviceTimes(dev, tm)
-    out["ao"] = dict(rays=S * 64, ms=round(tm[2], 3), rays_per_s=round(S * 64 / (tm[2] / 1e3), 1))
-    log(f"[bench] C4 AO: {S * 64} rays in {tm[2]:.1f} ms")
+    out["ao"] = dict(rays=S * 64, ms=round(tm[2], 3), rays_per_s=round(S * 64 / (tm[2] / 1e3), 1),
+                     roofline=ao_roofline(lib, dev, ro, S, 64, 2.0, tm[2]))
+    log(f"[bench] C4 AO: {S * 64} rays in {tm[2]:.1f} ms; {out['ao']['roofline']['achieved']} GB/s of "
+        f"algorithmic bytes ({out['ao']['roofline']['node_visits']} node visits, "
+        f"{out['ao']['roofline']['triangle_tests']} segment tests)")
     # colour gradient of L = 1/2 |direct - target|^2 over every texel (64 lights)
     lm = np.zeros((S, 3), np.float32)
     lib.lgx_ReadLightmap(dev, P(lm))

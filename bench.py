@@ -722,9 +722,13 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
     ms_step = (time.perf_counter() - t0) * 1000.0 / steps
     rays = marches / (ms_step / 1000.0)
     march_avg = float(np.mean(march_ms))
-    alg = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + npre * TRI_BOX_BYTES + len(items) * 36
+    refc = (C.c_uint64 * 5)()
+    _lib.check(lib.lgx_ReferenceCounts(dev, P(tris), len(tris), refc), lib)
+    rvis, rtri, rq, rbad, rm = (int(x) for x in refc)
+    alg = rvis * 32 + rtri * 36 + len(items) * 36
     log(f"[bench] C4 direct: BVH {info[1]} nodes depth {info[2]}; {marches} marches, {ms_step:.1f} ms/step "
-        f"(march {march_avg:.1f} ms): {rays / 1e6:.1f} M rays/s")
+        f"(march {march_avg:.1f} ms): {rays / 1e6:.1f} M rays/s; reference-shaped counts {rvis} node visits, "
+        f"{rtri} triangle tests, {rq} queries ({rbad} factors differ)")
     w = np.zeros(len(items), np.float32)
     vis = np.zeros(len(items), np.float32)
     _lib.check(lib.lgx_ReadWeights(dev, P(w), P(vis), len(items)), lib)
@@ -736,7 +740,10 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
                            march_kernel_ms=round(march_avg, 3),
                            roofline=dict(bound="hbm", achieved=round(alg / (march_avg / 1e3) / 1e9, 1),
                                          peak=HBM_PEAK_GBS, unit="GB/s",
-                                         frac=round(alg / (march_avg / 1e3) / 1e9 / HBM_PEAK_GBS, 4))))
+                                         frac=round(alg / (march_avg / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                                         reference_counts=dict(node_visits=rvis, triangle_tests=rtri,
+                                                               distance_queries=rq, marches=rm,
+                                                               factors_differing=rbad))))
     # ambient occlusion: 64 rays per texel, 2 m
     rng = np.random.default_rng(0x4C540004)
     ro = rng.uniform(0, 1, S).astype(np.float32)
@@ -976,19 +983,37 @@ def main():
     value = marches / (ms_per_step / 1000.0)
     log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, march kernel {march_avg:.2f} ms avg")
 
-    # algorithmic bytes of one march-kernel launch
+    # algorithmic bytes of one march-kernel launch: the reference-shaped
+    # count (the same marches on a plain binary SAH tree, 16 bins, <= 4
+    # triangles per leaf, a radius-2 query per step: node visits x 32 B +
+    # triangle tests x 36 B) plus the per-item I/O
     io_bytes = len(items) * (4 + 24 + 4 + 4)  # item, sample pos+nrm, weight, shadow factor
-    alg_bytes = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + npre * TRI_BOX_BYTES + io_bytes
+    refc = (C.c_uint64 * 5)()
+    _lib.check(lib.lgx_ReferenceCounts(dev, P(tris), len(tris), refc), lib)
+    rvis, rtri, rq, rbad, rm = (int(x) for x in refc)
+    alg_bytes = rvis * 32 + rtri * 36 + io_bytes
     achieved = alg_bytes / (march_avg / 1000.0) / 1e9
+    own_bytes = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + npre * TRI_BOX_BYTES + io_bytes
+    log(f"[bench] reference-shaped counts: {rvis} node visits, {rtri} triangle tests, {rq} queries over {rm} "
+        f"marches ({rbad} shadow factors differ from the product's); this build's tree: {nodes} wide-node visits, "
+        f"{ntris} triangle tests, {steps} queries")
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=measured_traffic("k_march", dict(triangles=int(info[0]), work_items=int(len(items)),
                                                              marches=int(marches), node_visits=int(nodes),
                                                              triangle_tests=int(ntris))),
                     kernel="k_march", alg_bytes_per_launch=int(alg_bytes),
-                    note="4-wide node visits x 112 B + full distance-triangle tests x 160 B + box pre-tests x 48 B + "
-                         "per-item I/O, counted by an "
-                         "instrumented pass; the working set (BVH + triangles) is cache resident")
+                    note="reference-shaped count: the same marches on a plain binary SAH tree (16 bins, <= 4 "
+                         "triangles per leaf) with one radius-2 distance query per step, node visits x 32 B + "
+                         "triangle tests x 36 B, plus 36 B of I/O per item; HBM traffic per launch from "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE",
+                    reference_counts=dict(node_visits=rvis, triangle_tests=rtri, distance_queries=rq, marches=rm,
+                                          factors_differing=rbad),
+                    this_build=dict(node_visits=int(nodes), triangle_tests=int(ntris), box_pretests=int(npre),
+                                    distance_queries=int(steps), bytes_per_launch=int(own_bytes),
+                                    achieved_gbs=round(own_bytes / (march_avg / 1000.0) / 1e9, 1),
+                                    note="4-wide nodes x 112 B + full triangle tests x 160 B + box pre-tests x "
+                                         "48 B + I/O on this build's own tree and query bounds"))
 
     w = np.zeros(len(items), np.float32)
     vis = np.zeros(len(items), np.float32)

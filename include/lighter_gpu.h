@@ -183,6 +183,15 @@ LTRAPI int lgx_DirectLightGrad(lgx_Device* dev, const double* lam3, double* dcol
  * collected by the next stage when enabled. */
 LTRAPI int lgx_SetCounting(lgx_Device* dev, int on);
 LTRAPI int lgx_ReadCounters(lgx_Device* dev, uint64_t out[6]);
+/* No counterpart in the reference (measurement only). The last
+ * lgx_DirectLightRun's marches (run with kept weights) again, with the
+ * reference's query semantics (lighter.cpp:150-207: a distance query of
+ * radius 2 per step) on a plain binary SAH tree of 16 bins and at most 4
+ * triangles per leaf, counting out[0] node visits (boxes tested), out[1]
+ * triangle tests, out[2] distance queries, out[3] shadow factors that
+ * differ from the product's bits, out[4] marches. tris: the device's
+ * triangles, as given to lgx_DeviceCreate. */
+LTRAPI int lgx_ReferenceCounts(lgx_Device* dev, const float* tris, size_t ntri, uint64_t out[5]);
 /* Last stage times in ms: direct, direct march, AO, links, bounces,
  * direct-light gradient. */
 LTRAPI int lgx_DeviceTimes(lgx_Device* dev, double out[6]);

@@ -72,6 +72,7 @@ LGX_SIGNATURES = {
     "lgx_DirectLightGrad": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "lgx_SetCounting": (C.c_int, [vp, C.c_int]),
     "lgx_ReadCounters": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
+    "lgx_ReferenceCounts": (C.c_int, [vp, fp, C.c_size_t, C.POINTER(C.c_uint64)]),
     "lgx_DeviceTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
 }
 

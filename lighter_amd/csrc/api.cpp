@@ -742,6 +742,11 @@ int lgx_SetCounting(lgx_Device* d, int on)
 	return 1;
 }
 
+int lgx_ReferenceCounts(lgx_Device* d, const float* tris, size_t ntri, uint64_t out[5])
+{
+	return guarded([&] { d->dev.reference_counts(tris, ntri, out); });
+}
+
 int lgx_ReadCounters(lgx_Device* d, uint64_t out[6])
 {
 	out[0] = d->dev.counters.nodes;

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <mutex>
+#include <stdexcept>
 #include <thread>
 
 namespace lgx {
@@ -45,7 +46,8 @@ struct Aabb {
 	}
 };
 
-constexpr int kBins = 32;
+constexpr int kBins = 32; // bins of the build the kernels traverse (at most kMaxBins)
+constexpr int kMaxBins = 32;
 // Below this depth SAH splits; deeper subtrees split at the median so the
 // total depth stays under the kernels' traversal stack (kStackSize).
 constexpr int kMedianDepth = 40;
@@ -57,6 +59,8 @@ struct Builder {
 	std::vector<uint32_t> idx;
 	std::vector<BvhNode> nodes;
 	int max_depth = 0;
+	int bins = kBins;          // SAH bins per axis
+	int leaf_max = kLeafMax;   // triangles per leaf at most
 
 	// Subtrees below the top levels are built as independent tasks (their
 	// triangle ranges are disjoint) and spliced back in depth-first order,
@@ -110,8 +114,8 @@ struct Builder {
 	}
 
 	struct Bins {
-		Aabb bb[3][kBins];
-		uint32_t cnt[3][kBins];
+		Aabb bb[3][kMaxBins];
+		uint32_t cnt[3][kMaxBins];
 	};
 
 	// Pads a box outward so that float rounding in the kernels' box tests
@@ -171,7 +175,7 @@ struct Builder {
 		// bins of every axis with a centroid extent, in one pass
 		Bins b0;
 		for (int a = 0; a < 3; ++a)
-			for (int k = 0; k < kBins; ++k) {
+			for (int k = 0; k < kMaxBins; ++k) {
 				b0.bb[a][k].reset();
 				b0.cnt[a][k] = 0;
 			}
@@ -180,9 +184,9 @@ struct Builder {
 		for (int a = 0; a < 3; ++a) {
 			float ext = cb.hi[a] - cb.lo[a];
 			use[a] = ext > 0;
-			sc[a] = use[a] ? kBins / ext : 0.0f;
+			sc[a] = use[a] ? bins / ext : 0.0f;
 		}
-		const Bins bins = reduce(
+		const Bins hist = reduce(
 		    b, e, par, b0,
 		    [&](uint32_t rb, uint32_t re, Bins& r) {
 			    for (uint32_t i = rb; i < re; ++i) {
@@ -190,7 +194,7 @@ struct Builder {
 				    for (int a = 0; a < 3; ++a) {
 					    if (!use[a])
 						    continue;
-					    int k = std::min(kBins - 1, (int)((cen[3 * t + a] - cb.lo[a]) * sc[a]));
+					    int k = std::min(bins - 1, (int)((cen[3 * t + a] - cb.lo[a]) * sc[a]));
 					    r.cnt[a][k]++;
 					    r.bb[a][k].grow(box[t]);
 				    }
@@ -198,7 +202,7 @@ struct Builder {
 		    },
 		    [](Bins& r, const Bins& q) {
 			    for (int a = 0; a < 3; ++a)
-				    for (int k = 0; k < kBins; ++k) {
+				    for (int k = 0; k < kMaxBins; ++k) {
 					    r.bb[a][k].grow(q.bb[a][k]);
 					    r.cnt[a][k] += q.cnt[a][k];
 				    }
@@ -209,14 +213,14 @@ struct Builder {
 		for (int a = 0; a < 3; ++a) {
 			if (!use[a])
 				continue;
-			const Aabb* bb = bins.bb[a];
-			const uint32_t* cnt = bins.cnt[a];
-			float ra[kBins];
-			uint32_t rc[kBins];
+			const Aabb* bb = hist.bb[a];
+			const uint32_t* cnt = hist.cnt[a];
+			float ra[kMaxBins];
+			uint32_t rc[kMaxBins];
 			Aabb acc;
 			acc.reset();
 			uint32_t c = 0;
-			for (int k = kBins - 1; k > 0; --k) {
+			for (int k = bins - 1; k > 0; --k) {
 				acc.grow(bb[k]);
 				c += cnt[k];
 				ra[k] = acc.area();
@@ -224,7 +228,7 @@ struct Builder {
 			}
 			acc.reset();
 			c = 0;
-			for (int k = 0; k < kBins - 1; ++k) {
+			for (int k = 0; k < bins - 1; ++k) {
 				acc.grow(bb[k]);
 				c += cnt[k];
 				if (c == 0 || rc[k + 1] == 0)
@@ -251,9 +255,9 @@ struct Builder {
 			mid = b + n / 2; // coincident centroids: split by index
 		} else {
 			float ext = cb.hi[best_axis] - cb.lo[best_axis];
-			float sc = kBins / ext;
+			float sc = bins / ext;
 			auto it = std::partition(idx.begin() + b, idx.begin() + e, [&](uint32_t t) {
-				int k = std::min(kBins - 1, (int)((cen[3 * t + best_axis] - cb.lo[best_axis]) * sc));
+				int k = std::min(bins - 1, (int)((cen[3 * t + best_axis] - cb.lo[best_axis]) * sc));
 				return k <= best_split;
 			});
 			mid = (uint32_t)(it - idx.begin());
@@ -269,7 +273,7 @@ struct Builder {
 	{
 		maxd = std::max(maxd, depth);
 		uint32_t n = e - b;
-		if (n <= (uint32_t)kLeafMax)
+		if (n <= (uint32_t)leaf_max)
 			return leaf_ref(b, n);
 		const uint32_t mid = split(b, e, depth);
 		int32_t self = (int32_t)out.size();
@@ -292,12 +296,12 @@ struct Builder {
 		{
 			std::lock_guard<std::mutex> g(mu);
 			max_depth = std::max(max_depth, depth);
-			if (n > (uint32_t)kLeafMax && n <= grain) {
+			if (n > (uint32_t)leaf_max && n <= grain) {
 				tasks.push_back(Task{b, e, depth, {}, 0, 0});
 				return Child{2, (int32_t)tasks.size() - 1};
 			}
 		}
-		if (n <= (uint32_t)kLeafMax)
+		if (n <= (uint32_t)leaf_max)
 			return Child{0, leaf_ref(b, n)};
 		const uint32_t mid = split(b, e, depth, par);
 		TopNode node;
@@ -370,7 +374,7 @@ struct Builder {
 		for (const Task& t : tasks)
 			total += t.nodes.size();
 		nodes.reserve(total);
-		emit(root); // the root is an inner node (more than kLeafMax triangles): index 0
+		emit(root); // the root is an inner node (more than leaf_max triangles): index 0
 	}
 };
 
@@ -458,9 +462,10 @@ struct Collapser {
 
 } // namespace
 
-BvhBuild build_bvh(const float* tris, size_t ntri)
+// Boxes, centroids and the identity order of the input triangles; returns
+// the scene bounds.
+static Aabb prepare(Builder& B, const float* tris, size_t ntri)
 {
-	Builder B;
 	B.tris = tris;
 	B.box.resize(ntri);
 	B.cen.resize(3 * ntri);
@@ -477,16 +482,14 @@ BvhBuild build_bvh(const float* tris, size_t ntri)
 		B.idx[t] = (uint32_t)t;
 		all.grow(b);
 	}
-	BvhBuild out;
-	if (ntri == 0) {
-		out.nodes.clear();
-		out.lo = mk3(0, 0, 0);
-		out.hi = mk3(0, 0, 0);
-		return out;
-	}
-	// The root (node 0) is always an inner node; a small scene gets a second
-	// child whose inverted box is never entered.
-	if (ntri <= (size_t)kLeafMax) {
+	return all;
+}
+
+// The root (node 0) is always an inner node; a small scene gets a second
+// child whose inverted box is never entered.
+static void build_binary(Builder& B, const Aabb& all, size_t ntri)
+{
+	if (ntri <= (size_t)B.leaf_max) {
 		B.nodes.push_back(BvhNode{});
 		B.set_child(B.nodes[0], 0, all, leaf_ref(0, (uint32_t)ntri));
 		B.nodes[0].lo1x = B.nodes[0].lo1y = B.nodes[0].lo1z = FLT_MAX;
@@ -496,6 +499,20 @@ BvhBuild build_bvh(const float* tris, size_t ntri)
 		const unsigned hw = std::thread::hardware_concurrency();
 		B.build_all((uint32_t)ntri, (int)std::min(16u, std::max(1u, hw)));
 	}
+}
+
+BvhBuild build_bvh(const float* tris, size_t ntri)
+{
+	Builder B;
+	const Aabb all = prepare(B, tris, ntri);
+	BvhBuild out;
+	if (ntri == 0) {
+		out.nodes.clear();
+		out.lo = mk3(0, 0, 0);
+		out.hi = mk3(0, 0, 0);
+		return out;
+	}
+	build_binary(B, all, ntri);
 	Collapser W{B.nodes, out.nodes};
 	out.nodes.reserve(B.nodes.size() / 2 + 1);
 	W.collapse(0, 0);
@@ -503,6 +520,26 @@ BvhBuild build_bvh(const float* tris, size_t ntri)
 	out.lo = mk3(all.lo[0], all.lo[1], all.lo[2]);
 	out.hi = mk3(all.hi[0], all.hi[1], all.hi[2]);
 	out.depth = W.max_depth;
+	return out;
+}
+
+BinaryBvh build_bvh_binary(const float* tris, size_t ntri, int bins, int leaf_max)
+{
+	if (bins < 2 || bins > kMaxBins || leaf_max < 1 || leaf_max > 16)
+		throw std::runtime_error("build_bvh_binary: bins must be 2..32 and leaves 1..16 triangles");
+	Builder B;
+	B.bins = bins;
+	B.leaf_max = leaf_max;
+	const Aabb all = prepare(B, tris, ntri);
+	BinaryBvh out;
+	out.lo = mk3(all.lo[0], all.lo[1], all.lo[2]);
+	out.hi = mk3(all.hi[0], all.hi[1], all.hi[2]);
+	if (ntri == 0)
+		return out;
+	build_binary(B, all, ntri);
+	out.nodes = std::move(B.nodes);
+	out.order = std::move(B.idx);
+	out.depth = B.max_depth + 1;
 	return out;
 }
 

@@ -61,4 +61,15 @@ struct BvhBuild {
 // tris: 9 floats per triangle.
 BvhBuild build_bvh(const float* tris, size_t ntri);
 
+// A plain binary binned-SAH tree (bins per axis, at most leaf_max triangles
+// per leaf), not collapsed: the reference tree shape the traversal byte
+// counts are stated on.  depth counts node levels, leaves included.
+struct BinaryBvh {
+	std::vector<BvhNode> nodes; // root at 0
+	std::vector<uint32_t> order;
+	f3 lo, hi;
+	int depth = 0;
+};
+BinaryBvh build_bvh_binary(const float* tris, size_t ntri, int bins, int leaf_max);
+
 } // namespace lgx

@@ -104,6 +104,13 @@ public:
 	void query_closest(const float* a, const float* b, size_t n, float* out);
 	void query_occluded(const float* a, const float* b, size_t n, uint8_t* out);
 
+	// Traversal counts of the last direct-light run's marches on a plain
+	// binary SAH tree with the reference's query semantics (measurement
+	// only; tris must be this scene's triangles): out[0] node visits,
+	// out[1] triangle tests, out[2] distance queries, out[3] shadow factors
+	// that differ from the product's, out[4] marches.
+	void reference_counts(const float* tris, size_t ntri, uint64_t out[5]);
+
 	StageTimes times;
 	QueryCounters counters;
 	bool count_queries = false;

@@ -535,6 +535,128 @@ __device__ float shadow_march(const View& V, f3 from, f3 to, float k, unsigned l
 }
 
 // ---------------------------------------------------------------------------
+// Reference-shaped traversal counts (measurement only).  The byte count a
+// roofline fraction is stated on must not depend on this build's own tree,
+// so it is taken on a plain binary SAH tree (16 bins, at most 4 triangles
+// per leaf) with the reference's query semantics: every march step of
+// CalcInvShadowFactor (lighter.cpp:190-207) is a fresh distance query of
+// radius MAX_PENUMBRA_SIZE (no exit test, no Lipschitz or need bound), the
+// nearer child is visited first and the radius shrinks with every triangle
+// distance found.  A node visit is one box tested (32 bytes of node in the
+// reference layout), a triangle test one point-triangle distance (36 bytes
+// of vertices).  The march's factor is compared with the product's, bit for
+// bit, as a by-product.
+// ---------------------------------------------------------------------------
+struct CanonView {
+	const BvhNode* __restrict__ nodes;
+	const DistTri* __restrict__ dtri;
+	float lo[3], hi[3]; // scene box (the root's)
+};
+constexpr int kCanonStack = 64; // >= the binary tree's depth, checked at build
+
+__device__ float canon_distance(const CanonView& C, f3 p, unsigned long long& nvis, unsigned long long& ntri)
+{
+	float best = kPenumbraSize;
+	nvis++;
+	if (box_d2(p, C.lo[0], C.lo[1], C.lo[2], C.hi[0], C.hi[1], C.hi[2]) > best * best)
+		return best;
+	int32_t st[kCanonStack];
+	float sk[kCanonStack];
+	int sp = 0;
+	int32_t ref = 0;
+	for (;;) {
+		if (ref >= 0) {
+			const BvhNode n = C.nodes[ref];
+			float da = box_d2(p, n.lo0x, n.lo0y, n.lo0z, n.hi0x, n.hi0y, n.hi0z);
+			float db = box_d2(p, n.lo1x, n.lo1y, n.lo1z, n.hi1x, n.hi1y, n.hi1z);
+			int32_t a = n.c0, b = n.c1;
+			nvis += 2;
+			if (db < da) {
+				float t = da;
+				da = db;
+				db = t;
+				int32_t u = a;
+				a = b;
+				b = u;
+			}
+			const float lim = best * best;
+			if (da <= lim) {
+				if (db <= lim) {
+					st[sp] = b;
+					sk[sp] = db;
+					sp++;
+				}
+				ref = a;
+				continue;
+			}
+		} else {
+			const uint32_t f = leaf_first(ref), c = leaf_count(ref);
+			for (uint32_t i = 0; i < c; ++i) {
+				const float d = dist_tri_pre(p, load_dtri(C.dtri, f + i));
+				ntri++;
+				if (d < best)
+					best = d;
+			}
+		}
+		bool found = false;
+		while (sp > 0) {
+			--sp;
+			if (sk[sp] <= best * best) {
+				ref = st[sp];
+				found = true;
+				break;
+			}
+		}
+		if (!found)
+			break;
+	}
+	return best;
+}
+
+// One lane per entry of the last direct-light run's march list.  out:
+// node visits, triangle tests, distance queries, factors that differ from
+// the product's (vis).
+__global__ __launch_bounds__(kBlock) void k_march_canon(CanonView C, View V, const float* __restrict__ pos,
+                                                        const float* __restrict__ nrm,
+                                                        const DevLight* __restrict__ lights,
+                                                        const uint32_t* __restrict__ items,
+                                                        const uint32_t* __restrict__ mlist, uint32_t nmarch,
+                                                        const float* __restrict__ vis, unsigned long long* out)
+{
+	const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+	if (idx >= nmarch)
+		return;
+	const uint32_t k = mlist[idx];
+	const uint32_t code = items[k];
+	const DevLight& L = lights[code >> kSampleBits];
+	const uint32_t s = code & kSampleMask;
+	const f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
+	const f3 to = (L.type == 3) ? SP + ld3(L.dir) * L.range : ld3(L.pos);
+	March M;
+	march_init(M, V, SP + SN * kShadowOffset, to, L.radius);
+	unsigned long long nv = 0, nt = 0, nq = 0;
+	float res = 1.0f;
+	for (float t = M.t; t < M.maxt;) {
+		const float h = canon_distance(C, M.from + M.rd * t, nv, nt);
+		nq++;
+		if (h < 0.001f) {
+			res = 0.0f;
+			break;
+		}
+		res = tmin(res, h / tmin(t * M.k, kPenumbraSize));
+		const float tn = t + tmin(h, kPenumbraStep);
+		if (tn == t)
+			break;
+		t = tn;
+	}
+	atomicAdd(out + 0, nv);
+	atomicAdd(out + 1, nt);
+	atomicAdd(out + 2, nq);
+	if (vis && __float_as_uint(res) != __float_as_uint(vis[k]))
+		atomicAdd(out + 3, 1ull);
+}
+
+// ---------------------------------------------------------------------------
 // direct light
 // ---------------------------------------------------------------------------
 // Shading gate of each (light, texel) item (Job_LMRender_*_Inner up to the
@@ -1444,6 +1566,59 @@ void DeviceScene::run_direct()
 	}
 	times.direct_ms = all.stop();
 	read_counters();
+}
+
+void DeviceScene::reference_counts(const float* tris, size_t ntri, uint64_t out[5])
+{
+	for (int i = 0; i < 5; ++i)
+		out[i] = 0;
+	if (ntri != ntri_)
+		throw std::runtime_error("reference_counts: the triangles must be those of the device scene");
+	if (!d_vis_)
+		throw std::runtime_error("reference_counts: run the direct light with kept weights first");
+	if (nmarch_ == 0 || ntri == 0)
+		return;
+	BinaryBvh b = build_bvh_binary(tris, ntri, 16, 4);
+	if (b.depth > kCanonStack)
+		throw std::runtime_error("reference_counts: binary tree deeper than the counting stack");
+	std::vector<DistTri> dt(ntri);
+	for (size_t i = 0; i < ntri; ++i) {
+		const float* p = tris + 9 * (size_t)b.order[i];
+		dt[i] = make_dist_tri(ld3(p), ld3(p + 3), ld3(p + 6));
+	}
+	BvhNode* d_nodes = dalloc<BvhNode>(b.nodes.size());
+	DistTri* d_dt = dalloc<DistTri>(ntri);
+	unsigned long long* d_out = dalloc<unsigned long long>(4);
+	struct Free {
+		BvhNode*& a;
+		DistTri*& b;
+		unsigned long long*& c;
+		~Free()
+		{
+			dfree(a);
+			dfree(b);
+			dfree(c);
+		}
+	} fr{d_nodes, d_dt, d_out};
+	hipStream_t s = (hipStream_t)stream_;
+	HIPCHECK(hipMemcpyAsync(d_nodes, b.nodes.data(), b.nodes.size() * sizeof(BvhNode), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_dt, dt.data(), ntri * sizeof(DistTri), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemsetAsync(d_out, 0, 4 * sizeof(unsigned long long), s));
+	CanonView C;
+	C.nodes = d_nodes;
+	C.dtri = d_dt;
+	C.lo[0] = b.lo.x; C.lo[1] = b.lo.y; C.lo[2] = b.lo.z;
+	C.hi[0] = b.hi.x; C.hi[1] = b.hi.y; C.hi[2] = b.hi.z;
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	k_march_canon<<<grid_for(nmarch_), kBlock, 0, s>>>(C, V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch_,
+	                                                   d_vis_, d_out);
+	HIPCHECK(hipGetLastError());
+	unsigned long long h[4];
+	HIPCHECK(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	for (int i = 0; i < 4; ++i)
+		out[i] = h[i];
+	out[4] = nmarch_;
 }
 
 void DeviceScene::read_counters()

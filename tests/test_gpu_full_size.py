@@ -81,3 +81,65 @@ def test_full_size_march_properties(terrain):
     # a real mix of lit, shadowed and penumbra marches
     r = res[0.5]
     assert (r == 0).mean() > 0.05 and (r == 1).mean() > 0.05 and ((r > 0) & (r < 1)).mean() > 0.05
+
+
+def test_every_march_equals_the_reference_semantics(lib, terrain):
+    """All 3.8 M marches of the workload's direct light, marched again on
+    the GPU with the reference's query semantics (a radius-2 distance query
+    per step on a plain binary SAH tree: no scene-box exit, no Lipschitz or
+    need bound, a different tree): every shadow factor of the product's
+    march is bit-identical, which pins the exactness of its shortcuts at
+    full size.  The same pass gives the reference-shaped traversal counts
+    the roofline figure is stated on."""
+    import ctypes as C
+
+    from helpers import P
+    from lighter_amd import _lib
+    spec, pos, nrm, tris, dev = terrain
+    b = prepared(lib, spec)
+    n_inst = lib.lgx_InstanceCount(b.scene)
+    counts = []
+    for i in range(n_inst):
+        info = (C.c_int32 * 4)()
+        lib.lgx_InstanceInfo(b.scene, i, info)
+        counts.append(info[2])
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    items, light_off = [], [0]
+    buf = (C.c_int32 * (n_inst + 1))()
+    for li in range(len(spec.lights)):
+        k = lib.lgx_LightInstances(b.scene, li, buf, n_inst + 1)
+        s = np.concatenate([np.arange(off[i], off[i + 1], dtype=np.uint32) for i in buf[:k]])
+        items.append((np.uint32(li) << np.uint32(26)) | s)
+        light_off.append(light_off[-1] + len(s))
+    b.destroy()
+    items = np.concatenate(items).astype(np.uint32)
+    light_off = np.array(light_off, np.uint32)
+    arr = (_lib.LgxLight * len(spec.lights))()
+    for i, L in enumerate(spec.lights):
+        d = np.asarray(L.get("direction", [0, 0, 0]), np.float32)
+        l2 = np.float32(np.float32(d[0] * d[0]) + np.float32(d[1] * d[1])) + np.float32(d[2] * d[2])
+        if l2 > 0:
+            d = (d * (np.float32(1) / np.sqrt(np.float32(l2)))).astype(np.float32)
+        a = arr[i]
+        a.type = L["type"]
+        a.pos[:] = L.get("position", [0, 0, 0])
+        a.dir[:] = [float(x) for x in d]
+        a.color[:] = L.get("color_rgb", [0, 0, 0])
+        a.range = L.get("range", 0)
+        a.power = L.get("power", 0)
+        a.radius = L.get("light_radius", 0)
+        a.spot_angle_out = L.get("spot_angle_out", 0)
+        a.spot_angle_in = L.get("spot_angle_in", 0)
+        a.spot_curve = L.get("spot_curve", 0)
+    u32p = C.POINTER(C.c_uint32)
+    _lib.check(lib.lgx_SetSamples(dev.h, P(pos), P(nrm), len(pos), None), lib)
+    _lib.check(lib.lgx_DirectLightSetup(dev.h, arr, len(arr), items.ctypes.data_as(u32p), len(items),
+                                        light_off.ctypes.data_as(u32p), 1), lib)
+    _lib.check(lib.lgx_DirectLightRun(dev.h), lib)
+    out = (C.c_uint64 * 5)()
+    _lib.check(lib.lgx_ReferenceCounts(dev.h, P(tris), len(tris), out), lib)
+    visits, tests, queries, differ, marches = (int(x) for x in out)
+    print(f"{marches} marches, {queries} queries, {visits} node visits, {tests} triangle tests, {differ} differ")
+    assert marches > 3_000_000
+    assert differ == 0
+    assert queries > marches and visits > queries and tests > 0

@@ -169,3 +169,28 @@ def test_abort_stops_bake(lib):
         lib.ltr_Sleep(1)
     assert st.stage in (b"finished",)
     b.destroy()
+
+
+def test_host_stage_switch(lib):
+    """lgx_SetHostStages takes the two documented bits and refuses others;
+    a scene baked with both host forms and no lights needs no GPU stage, and
+    reports no GPU placement or export times."""
+    spec = scenes.room_scene(lm=16)
+    spec.lights = []
+    b = scenes.BuiltScene(lib, spec)
+    assert lib.lgx_SetHostStages(b.scene, 4) == 0
+    assert "unknown bits" in _lib.last_error(lib)
+    assert lib.lgx_SetHostStages(b.scene, 3) == 1
+    st = api.WorkStatus()
+    lib.ltr_Start(b.scene)
+    while lib.ltr_GetStatus(b.scene, C.byref(st)):
+        lib.ltr_Sleep(1)
+    assert st.stage == b"finished"
+    t = (C.c_double * 5)()
+    lib.lgx_PlacementTimes(b.scene, t)
+    assert list(t) == [0.0] * 5
+    lib.lgx_FinalizeTimes(b.scene, t)
+    assert list(t) == [0.0] * 5
+    outs = b.outputs()
+    assert len(outs) == 1 and outs[0][1] == outs[0][2] == 16
+    b.destroy()

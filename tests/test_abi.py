@@ -175,7 +175,7 @@ def test_host_stage_switch(lib):
     """lgx_SetHostStages takes the two documented bits and refuses others;
     a scene baked with both host forms and no lights needs no GPU stage, and
     reports no GPU placement or export times."""
-    spec = scenes.room_scene(lm=16)
+    spec = scenes.room_scene(lm=64)
     spec.lights = []
     b = scenes.BuiltScene(lib, spec)
     assert lib.lgx_SetHostStages(b.scene, 4) == 0
@@ -192,5 +192,5 @@ def test_host_stage_switch(lib):
     lib.lgx_FinalizeTimes(b.scene, t)
     assert list(t) == [0.0] * 5
     outs = b.outputs()
-    assert len(outs) == 1 and outs[0][1] == outs[0][2] == 16
+    assert len(outs) == 1 and outs[0][1] == outs[0][2] == 64
     b.destroy()

@@ -175,7 +175,7 @@ def test_host_stage_switch(lib):
     """lgx_SetHostStages takes the two documented bits and refuses others;
     a scene baked with both host forms and no lights needs no GPU stage, and
     reports no GPU placement or export times."""
-    spec = scenes.room_scene(lm=64)
+    spec = scenes.room_scene(lm=64, bounce_count=0)
     spec.lights = []
     b = scenes.BuiltScene(lib, spec)
     assert lib.lgx_SetHostStages(b.scene, 4) == 0

@@ -370,11 +370,14 @@ def links_bench(lib, nth, lm=64):
     return out
 
 
-def ao_roofline(lib, dev, ro, S, n, distance, ms):
-    """Algorithmic bytes of one AO launch (wide-node visits x 112 B + segment
-    tests x 64 B + per texel 52 B: position, normal, spiral offset, lightmap
-    read and write), counted by an instrumented launch, over the timed
-    launch's duration ms."""
+def ao_roofline(lib, dev, ro, S, n, distance, ms, tris):
+    """Algorithmic bytes of one AO launch over the timed launch's duration
+    ms: reference-shaped (the same rays with the canonical closest-hit
+    semantics on a plain binary SAH tree, node visits x 32 B + triangle
+    tests x 36 B, lgx_ReferenceCountsAO, which also compares every texel's
+    factor with the product's), and on this build's tree (wide-node visits x
+    112 B + segment tests x 64 B); both plus 52 B per texel (position,
+    normal, spiral offset, lightmap read and write)."""
     col = np.zeros(3, np.float32)
     lib.lgx_SetCounting(dev, 1)
     _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), n, distance, 1.0, 0.0, 0.0, P(col)), lib)
@@ -382,17 +385,25 @@ def ao_roofline(lib, dev, ro, S, n, distance, ms):
     lib.lgx_ReadCounters(dev, cnt)
     lib.lgx_SetCounting(dev, 0)
     nodes, tests = int(cnt[0]), int(cnt[1])
-    alg = nodes * NODE_BYTES + tests * RAY_TRI_BYTES + S * 52
+    own = nodes * NODE_BYTES + tests * RAY_TRI_BYTES + S * 52
+    refc = (C.c_uint64 * 5)()
+    _lib.check(lib.lgx_ReferenceCountsAO(dev, P(tris), len(tris), P(ro), n, distance, 1.0, 0.0, refc), lib)
+    rvis, rtri, rrays, rdiff, rtex = (int(x) for x in refc)
+    alg = rvis * 32 + rtri * 36 + S * 52
     achieved = alg / (ms / 1e3) / 1e9
     return dict(bound="cache", achieved=round(achieved, 1), unit="GB/s", hbm_peak=HBM_PEAK_GBS,
-                x_hbm_peak=round(achieved / HBM_PEAK_GBS, 4), kernel="k_ao", node_visits=nodes,
-                triangle_tests=tests, alg_bytes_per_launch=int(alg),
-                note="2 m rays visit about 12 wide nodes each near their origin; the nodes and segment operands "
-                     "they read stay in L2 and the Infinity Cache, so the algorithmic bytes are consumed faster "
-                     "than HBM could supply them and HBM is not the bound")
+                x_hbm_peak=round(achieved / HBM_PEAK_GBS, 4), kernel="k_ao", alg_bytes_per_launch=int(alg),
+                reference_counts=dict(node_visits=rvis, triangle_tests=rtri, rays=rrays, texels=rtex,
+                                      factors_differing=rdiff),
+                this_build=dict(node_visits=nodes, triangle_tests=tests, bytes_per_launch=int(own),
+                                achieved_gbs=round(own / (ms / 1e3) / 1e9, 1)),
+                note="reference-shaped bytes: node visits x 32 B + triangle tests x 36 B on a plain binary SAH "
+                     "tree (16 bins, <= 4 triangles per leaf), closest hit shrinking the segment, + 52 B per "
+                     "texel; 2 m rays stay near their origin, so the nodes and triangles they read are served "
+                     "by L2 and the Infinity Cache and HBM is not the bound")
 
 
-def ao_bench(lib, dev, spec, S, nth, with_cpu):
+def ao_bench(lib, dev, spec, S, nth, with_cpu, tris):
     """C2 ambient occlusion: 64 cosine-weighted rays of 2 m per texel on the
     device's S texels (U2 = one ray), beside the reference's AO stage on all
     texels of the same room on the host threads."""
@@ -407,7 +418,7 @@ def ao_bench(lib, dev, spec, S, nth, with_cpu):
         ms.append(tm[2])
     best = float(np.median(ms))
     out = dict(texels=S, rays=S * 64, ms=round(best, 3), rays_per_s=round(S * 64 / (best / 1e3), 1),
-               roofline=ao_roofline(lib, dev, ro, S, 64, 2.0, best))
+               roofline=ao_roofline(lib, dev, ro, S, 64, 2.0, best, tris))
     if with_cpu:
         from oracle.checkers import Reference, reference_available
         if reference_available():
@@ -448,7 +459,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
     lib.lgx_ReadWeights(dev, P(w), None, len(items))
     w[w.view(np.uint32) == 0xFFFFFFFF] = 0
     W = w.reshape(L, N).T.astype(np.float64)
-    ao = ao_bench(lib, dev, spec, N, nth, with_cpu)
+    ao = ao_bench(lib, dev, spec, N, nth, with_cpu, tris)
     rng = np.random.default_rng(0x4C540002)
     cols = rng.uniform(0.5, 1.0, (setups, L, 3))
     E = np.zeros((N, 3 * setups))
@@ -752,10 +763,11 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
     _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
     lib.lgx_DeviceTimes(dev, tm)
     out["ao"] = dict(rays=S * 64, ms=round(tm[2], 3), rays_per_s=round(S * 64 / (tm[2] / 1e3), 1),
-                     roofline=ao_roofline(lib, dev, ro, S, 64, 2.0, tm[2]))
+                     roofline=ao_roofline(lib, dev, ro, S, 64, 2.0, tm[2], tris))
+    rc = out['ao']['roofline']['reference_counts']
     log(f"[bench] C4 AO: {S * 64} rays in {tm[2]:.1f} ms; {out['ao']['roofline']['achieved']} GB/s of "
-        f"algorithmic bytes ({out['ao']['roofline']['node_visits']} node visits, "
-        f"{out['ao']['roofline']['triangle_tests']} segment tests)")
+        f"reference-shaped bytes ({rc['node_visits']} node visits, {rc['triangle_tests']} triangle tests; "
+        f"{rc['factors_differing']} of {rc['texels']} factors differ from the product's)")
     # colour gradient of L = 1/2 |direct - target|^2 over every texel (64 lights)
     lm = np.zeros((S, 3), np.float32)
     lib.lgx_ReadLightmap(dev, P(lm))

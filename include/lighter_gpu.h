@@ -192,6 +192,15 @@ LTRAPI int lgx_ReadCounters(lgx_Device* dev, uint64_t out[6]);
  * differ from the product's bits, out[4] marches. tris: the device's
  * triangles, as given to lgx_DeviceCreate. */
 LTRAPI int lgx_ReferenceCounts(lgx_Device* dev, const float* tris, size_t ntri, uint64_t out[5]);
+/* No counterpart in the reference (measurement only). The AO factors of
+ * every sample (lighter.cpp:812-831; the lightmap is left as it was) with
+ * the rays traced with the canonical closest-hit semantics (nearest child
+ * first, the segment's end shrinking to the closest hit) on the same plain
+ * binary SAH tree: out[0] node visits, out[1] triangle tests, out[2] rays,
+ * out[3] samples whose factor differs from the product's bits, out[4]
+ * samples. */
+LTRAPI int lgx_ReferenceCountsAO(lgx_Device* dev, const float* tris, size_t ntri, const float* randoff,
+                                 int num_samples, float distance, float multiplier, float falloff, uint64_t out[5]);
 /* Last stage times in ms: direct, direct march, AO, links, bounces,
  * direct-light gradient. */
 LTRAPI int lgx_DeviceTimes(lgx_Device* dev, double out[6]);

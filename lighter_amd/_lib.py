@@ -73,6 +73,8 @@ LGX_SIGNATURES = {
     "lgx_SetCounting": (C.c_int, [vp, C.c_int]),
     "lgx_ReadCounters": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
     "lgx_ReferenceCounts": (C.c_int, [vp, fp, C.c_size_t, C.POINTER(C.c_uint64)]),
+    "lgx_ReferenceCountsAO": (C.c_int, [vp, fp, C.c_size_t, fp, C.c_int, C.c_float, C.c_float, C.c_float,
+                                        C.POINTER(C.c_uint64)]),
     "lgx_DeviceTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
 }
 

@@ -747,6 +747,14 @@ int lgx_ReferenceCounts(lgx_Device* d, const float* tris, size_t ntri, uint64_t 
 	return guarded([&] { d->dev.reference_counts(tris, ntri, out); });
 }
 
+int lgx_ReferenceCountsAO(lgx_Device* d, const float* tris, size_t ntri, const float* randoff, int num_samples,
+                          float distance, float multiplier, float falloff, uint64_t out[5])
+{
+	return guarded([&] {
+		d->dev.reference_counts_ao(tris, ntri, randoff, num_samples, distance, multiplier, falloff, out);
+	});
+}
+
 int lgx_ReadCounters(lgx_Device* d, uint64_t out[6])
 {
 	out[0] = d->dev.counters.nodes;

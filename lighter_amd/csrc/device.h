@@ -80,8 +80,17 @@ public:
 	size_t num_items() const { return nitems_; }
 
 	// --- ambient occlusion (lighter.cpp:799-841) -----------------------
+	// fac_out (device, one float per sample, optional) receives each
+	// texel's AO factor before the colour blend
 	void ambient_occlusion(const float* randoff, int num_samples, float distance, float multiplier, float falloff,
-	                       float effect, const float* color);
+	                       float effect, const float* color, float* fac_out = nullptr);
+	// The AO factors again with the rays traced on a plain binary SAH tree
+	// with the canonical closest-hit semantics (measurement only; the
+	// lightmap is left as it was): out[0] node visits, out[1] triangle
+	// tests, out[2] rays, out[3] texels whose factor differs from the
+	// product's, out[4] texels.
+	void reference_counts_ao(const float* tris, size_t ntri, const float* randoff, int num_samples, float distance,
+	                         float multiplier, float falloff, uint64_t out[5]);
 
 	// --- radiosity (lighter.cpp:667-797) ---------------------------------
 	// Patches = samples.  rad_links builds the symmetric link structure of

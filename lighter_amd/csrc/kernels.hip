@@ -955,7 +955,7 @@ template <bool COUNT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_ao(View V, const float* __restrict__ pos, const float* __restrict__ nrm,
                                                const float* __restrict__ randoff, const float2* __restrict__ side,
                                                AOParams P, uint32_t S, float* __restrict__ lm,
-                                               unsigned long long* cnt, int depth)
+                                               unsigned long long* cnt, int depth, float* __restrict__ fac_out)
 {
 	extern __shared__ __align__(16) unsigned char lds_stack[];
 	LdsStack st = lds_stack_of_lane(lds_stack, depth);
@@ -991,6 +991,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
 	ao = tmin(ao * P.multiplier, 1.0f);
 	if (P.falloff)
 		ao = pow_r(ao, P.falloff);
+	if (fac_out)
+		fac_out[s] = ao;
 	f3 out = ld3(lm + 3 * s);
 	f3 col = mk3(P.color[0], P.color[1], P.color[2]);
 	if (P.effect >= 0) {
@@ -999,6 +1001,120 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
 		out = lerp3(out, lerp3(col, col * out, -P.effect), ao);
 	}
 	st3(lm + 3 * s, out);
+}
+
+// Closest hit of DistanceTestBBT (lighter.cpp:254-289) on the binary tree
+// with the canonical semantics: nearest child first by entry parameter, the
+// segment's far end shrinking to the closest hit found.  Counts as
+// canon_distance does.
+__device__ float canon_segment(const CanonView& C, const RayTri* __restrict__ rtri, f3 A, f3 B,
+                               unsigned long long& nvis, unsigned long long& ntri)
+{
+	float best = 2.0f;
+	const Seg s = make_seg(A, B);
+	nvis++;
+	if (slab(s, C.lo[0], C.lo[1], C.lo[2], C.hi[0], C.hi[1], C.hi[2], 1.0f) == INFINITY)
+		return best;
+	int32_t st[kCanonStack];
+	float sk[kCanonStack];
+	int sp = 0;
+	int32_t ref = 0;
+	for (;;) {
+		if (ref >= 0) {
+			const BvhNode n = C.nodes[ref];
+			const float tl = fminf(best, 1.0f);
+			float ta = slab(s, n.lo0x, n.lo0y, n.lo0z, n.hi0x, n.hi0y, n.hi0z, tl);
+			float tb = slab(s, n.lo1x, n.lo1y, n.lo1z, n.hi1x, n.hi1y, n.hi1z, tl);
+			int32_t a = n.c0, b = n.c1;
+			nvis += 2;
+			if (tb < ta) {
+				float t = ta;
+				ta = tb;
+				tb = t;
+				int32_t u = a;
+				a = b;
+				b = u;
+			}
+			if (ta != INFINITY) {
+				if (tb != INFINITY) {
+					st[sp] = b;
+					sk[sp] = tb;
+					sp++;
+				}
+				ref = a;
+				continue;
+			}
+		} else {
+			const uint32_t f = leaf_first(ref), c = leaf_count(ref);
+			for (uint32_t i = 0; i < c; ++i) {
+				const float r = seg_tri_pre(s.o, s.dir, load_rtri(rtri, f + i));
+				ntri++;
+				if (r < best)
+					best = r;
+			}
+		}
+		bool found = false;
+		while (sp > 0) {
+			--sp;
+			if (sk[sp] <= fminf(best, 1.0f)) {
+				ref = st[sp];
+				found = true;
+				break;
+			}
+		}
+		if (!found)
+			break;
+	}
+	return best;
+}
+
+// The AO factor of every texel (Job_AORender_Inner, lighter.cpp:812-831)
+// with the rays traced by canon_segment: one wave per texel, one lane per
+// ray, the terms added in ray order, as k_ao does.  out: node visits,
+// triangle tests, rays.
+__global__ __launch_bounds__(kBlock) void k_ao_canon(CanonView C, const RayTri* __restrict__ rtri,
+                                                     const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                     const float* __restrict__ randoff,
+                                                     const float2* __restrict__ side, AOParams P, uint32_t S,
+                                                     float* __restrict__ fac_out, unsigned long long* out)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const uint32_t s = blockIdx.x * (kBlock / 64) + wave;
+	if (s >= S)
+		return;
+	f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
+	f3 origin = SP + SN * (kSmall * 2);
+	float ro = randoff[s];
+	float ao = 0;
+	unsigned long long nv = 0, nt = 0, nr = 0;
+	for (int i0 = 0; i0 < P.n; i0 += 64) {
+		const int i = i0 + lane;
+		float term = 0.0f;
+		if (i < P.n) {
+			float angle = (i + ro) * kGoldenAngle;
+			double sa, ca;
+			sincos((double)angle, &sa, &ca);
+			float2 sd = side[i];
+			f3 d = spiral_dir(SN, sd.x, sd.y, (float)ca, (float)sa) * P.distance;
+			float hit = canon_segment(C, rtri, origin, origin + d, nv, nt);
+			nr++;
+			if (hit < 1.0f)
+				term = 1.0f - hit;
+		}
+		const int m = P.n - i0 < 64 ? P.n - i0 : 64;
+		for (int k = 0; k < m; ++k)
+			ao += __shfl(term, k);
+	}
+	atomicAdd(out + 0, nv);
+	atomicAdd(out + 1, nt);
+	atomicAdd(out + 2, nr);
+	if (lane != 0)
+		return;
+	ao /= P.n;
+	ao = tmin(ao * P.multiplier, 1.0f);
+	if (P.falloff)
+		ao = pow_r(ao, P.falloff);
+	fac_out[s] = ao;
 }
 
 // ---------------------------------------------------------------------------
@@ -1621,6 +1737,93 @@ void DeviceScene::reference_counts(const float* tris, size_t ntri, uint64_t out[
 	out[4] = nmarch_;
 }
 
+void DeviceScene::reference_counts_ao(const float* tris, size_t ntri, const float* randoff, int num_samples,
+                                      float distance, float multiplier, float falloff, uint64_t out[5])
+{
+	for (int i = 0; i < 5; ++i)
+		out[i] = 0;
+	if (ntri != ntri_)
+		throw std::runtime_error("reference_counts_ao: the triangles must be those of the device scene");
+	if (nsamples_ == 0 || num_samples <= 0 || ntri == 0)
+		return;
+	BinaryBvh b = build_bvh_binary(tris, ntri, 16, 4);
+	if (b.depth > kCanonStack)
+		throw std::runtime_error("reference_counts_ao: binary tree deeper than the counting stack");
+	std::vector<RayTri> rt(ntri);
+	for (size_t i = 0; i < ntri; ++i) {
+		const float* p = tris + 9 * (size_t)b.order[i];
+		rt[i] = make_ray_tri(ld3(p), ld3(p + 3), ld3(p + 6));
+	}
+	std::vector<float2> side(num_samples);
+	for (int i = 0; i < num_samples; ++i) {
+		float q = (i + 0.5f) / num_samples;
+		float cs = sqrtf(q);
+		side[i] = make_float2(cs, sinf(acosf(cs)));
+	}
+	hipStream_t s = (hipStream_t)stream_;
+	BvhNode* d_nodes = dalloc<BvhNode>(b.nodes.size());
+	RayTri* d_rt = dalloc<RayTri>(ntri);
+	float2* d_side = dalloc<float2>(num_samples);
+	float* d_ro = dalloc<float>(nsamples_);
+	float* d_fp = dalloc<float>(nsamples_);
+	float* d_fc = dalloc<float>(nsamples_);
+	float* d_lm_keep = dalloc<float>(3 * nsamples_);
+	unsigned long long* d_out = dalloc<unsigned long long>(4);
+	struct Free {
+		std::vector<void*> p;
+		~Free()
+		{
+			for (void* q : p)
+				if (q)
+					(void)hipFree(q);
+		}
+	} fr{{d_nodes, d_rt, d_side, d_ro, d_fp, d_fc, d_lm_keep, d_out}};
+	HIPCHECK(hipMemcpyAsync(d_nodes, b.nodes.data(), b.nodes.size() * sizeof(BvhNode), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_rt, rt.data(), ntri * sizeof(RayTri), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_side, side.data(), sizeof(float2) * num_samples, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_ro, randoff, 4 * nsamples_, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemsetAsync(d_out, 0, 4 * sizeof(unsigned long long), s));
+	// the product's factors, with the lightmap put back afterwards
+	HIPCHECK(hipMemcpyAsync(d_lm_keep, d_lm_, 12 * nsamples_, hipMemcpyDeviceToDevice, s));
+	const float black[3] = {0, 0, 0};
+	const bool counting = count_queries;
+	count_queries = false;
+	const StageTimes keep_times = times;
+	ambient_occlusion(randoff, num_samples, distance, multiplier, falloff, 0.0f, black, d_fp);
+	count_queries = counting;
+	times = keep_times;
+	HIPCHECK(hipMemcpyAsync(d_lm_, d_lm_keep, 12 * nsamples_, hipMemcpyDeviceToDevice, s));
+	AOParams P;
+	P.n = num_samples;
+	P.distance = distance;
+	P.multiplier = multiplier;
+	P.falloff = falloff;
+	P.effect = 0.0f;
+	P.color[0] = P.color[1] = P.color[2] = 0.0f;
+	CanonView C;
+	C.nodes = d_nodes;
+	C.dtri = nullptr;
+	C.lo[0] = b.lo.x; C.lo[1] = b.lo.y; C.lo[2] = b.lo.z;
+	C.hi[0] = b.hi.x; C.hi[1] = b.hi.y; C.hi[2] = b.hi.z;
+	const unsigned grid = (unsigned)((nsamples_ + kBlock / 64 - 1) / (kBlock / 64));
+	k_ao_canon<<<grid, kBlock, 0, s>>>(C, d_rt, d_pos_, d_nrm_, d_ro, d_side, P, (uint32_t)nsamples_, d_fc, d_out);
+	HIPCHECK(hipGetLastError());
+	unsigned long long h[4];
+	HIPCHECK(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, s));
+	std::vector<float> fp(nsamples_), fc(nsamples_);
+	HIPCHECK(hipMemcpyAsync(fp.data(), d_fp, 4 * nsamples_, hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipMemcpyAsync(fc.data(), d_fc, 4 * nsamples_, hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	uint64_t differ = 0;
+	for (size_t i = 0; i < nsamples_; ++i)
+		differ += memcmp(&fp[i], &fc[i], 4) != 0;
+	out[0] = h[0];
+	out[1] = h[1];
+	out[2] = h[2];
+	out[3] = differ;
+	out[4] = nsamples_;
+}
+
 void DeviceScene::read_counters()
 {
 	if (!count_queries)
@@ -1692,7 +1895,7 @@ void DeviceScene::get_vis(float* out, size_t n) const
 }
 
 void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float distance, float multiplier,
-                                    float falloff, float effect, const float* color)
+                                    float falloff, float effect, const float* color, float* fac_out)
 {
 	hipStream_t s = (hipStream_t)stream_;
 	if (nsamples_ == 0 || num_samples <= 0)
@@ -1725,11 +1928,11 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 	if (count_queries)
 		k_ao<true><<<ao_grid, kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
 		                                                                  (uint32_t)nsamples_, d_lm_, d_counters_,
-		                                                                  march_stack_depth());
+		                                                                  march_stack_depth(), fac_out);
 	else
 		k_ao<false><<<ao_grid, kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
 		                                                                   (uint32_t)nsamples_, d_lm_, d_counters_,
-		                                                                   march_stack_depth());
+		                                                                   march_stack_depth(), fac_out);
 	HIPCHECK(hipGetLastError());
 	times.ao_ms = tm.stop();
 	read_counters();

@@ -143,3 +143,25 @@ def test_every_march_equals_the_reference_semantics(lib, terrain):
     assert marches > 3_000_000
     assert differ == 0
     assert queries > marches and visits > queries and tests > 0
+
+
+def test_every_ao_factor_equals_the_canonical_traversal(lib, terrain):
+    """AO at full size (64 rays of 2 m for each of the 2,989,242 texels):
+    every texel's factor from the product's traversal (4-wide tree, nearest
+    first) equals, bit for bit, the factor with the rays traced with the
+    canonical closest-hit semantics on a plain binary SAH tree; the closest
+    hit is order independent, so any difference would be a traversal bug."""
+    import ctypes as C
+
+    from helpers import P
+    from lighter_amd import _lib
+    spec, pos, nrm, tris, dev = terrain
+    _lib.check(lib.lgx_SetSamples(dev.h, P(pos), P(nrm), len(pos), None), lib)
+    ro = np.random.default_rng(13).uniform(0, 1, len(pos)).astype(np.float32)
+    out = (C.c_uint64 * 5)()
+    _lib.check(lib.lgx_ReferenceCountsAO(dev.h, P(tris), len(tris), P(ro), 64, 2.0, 1.2, 1.0, out), lib)
+    visits, tests, rays, differ, texels = (int(x) for x in out)
+    print(f"{texels} texels, {rays} rays, {visits} node visits, {tests} triangle tests, {differ} differ")
+    assert texels == len(pos) and rays == 64 * len(pos)
+    assert differ == 0
+    assert visits > rays and tests > 0

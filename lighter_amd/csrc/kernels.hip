@@ -550,16 +550,13 @@ __device__ float shadow_march(const View& V, f3 from, f3 to, float k, unsigned l
 struct CanonView {
 	const BvhNode* __restrict__ nodes;
 	const DistTri* __restrict__ dtri;
-	float lo[3], hi[3]; // scene box (the root's)
 };
 constexpr int kCanonStack = 64; // >= the binary tree's depth, checked at build
 
 __device__ float canon_distance(const CanonView& C, f3 p, unsigned long long& nvis, unsigned long long& ntri)
 {
 	float best = kPenumbraSize;
-	nvis++;
-	if (box_d2(p, C.lo[0], C.lo[1], C.lo[2], C.hi[0], C.hi[1], C.hi[2]) > best * best)
-		return best;
+	nvis++; // the root's own box; node 0 holds its children's (padded) boxes
 	int32_t st[kCanonStack];
 	float sk[kCanonStack];
 	int sp = 0;
@@ -1012,9 +1009,7 @@ __device__ float canon_segment(const CanonView& C, const RayTri* __restrict__ rt
 {
 	float best = 2.0f;
 	const Seg s = make_seg(A, B);
-	nvis++;
-	if (slab(s, C.lo[0], C.lo[1], C.lo[2], C.hi[0], C.hi[1], C.hi[2], 1.0f) == INFINITY)
-		return best;
+	nvis++; // the root's own box; node 0 holds its children's (padded) boxes
 	int32_t st[kCanonStack];
 	float sk[kCanonStack];
 	int sp = 0;
@@ -1723,8 +1718,6 @@ void DeviceScene::reference_counts(const float* tris, size_t ntri, uint64_t out[
 	CanonView C;
 	C.nodes = d_nodes;
 	C.dtri = d_dt;
-	C.lo[0] = b.lo.x; C.lo[1] = b.lo.y; C.lo[2] = b.lo.z;
-	C.hi[0] = b.hi.x; C.hi[1] = b.hi.y; C.hi[2] = b.hi.z;
 	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
 	k_march_canon<<<grid_for(nmarch_), kBlock, 0, s>>>(C, V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch_,
 	                                                   d_vis_, d_out);
@@ -1803,8 +1796,6 @@ void DeviceScene::reference_counts_ao(const float* tris, size_t ntri, const floa
 	CanonView C;
 	C.nodes = d_nodes;
 	C.dtri = nullptr;
-	C.lo[0] = b.lo.x; C.lo[1] = b.lo.y; C.lo[2] = b.lo.z;
-	C.hi[0] = b.hi.x; C.hi[1] = b.hi.y; C.hi[2] = b.hi.z;
 	const unsigned grid = (unsigned)((nsamples_ + kBlock / 64 - 1) / (kBlock / 64));
 	k_ao_canon<<<grid, kBlock, 0, s>>>(C, d_rt, d_pos_, d_nrm_, d_ro, d_side, P, (uint32_t)nsamples_, d_fc, d_out);
 	HIPCHECK(hipGetLastError());

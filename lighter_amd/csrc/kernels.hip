@@ -550,13 +550,27 @@ __device__ float shadow_march(const View& V, f3 from, f3 to, float k, unsigned l
 struct CanonView {
 	const BvhNode* __restrict__ nodes;
 	const DistTri* __restrict__ dtri;
+	float lo[3], hi[3]; // the root's box, padded as the builder pads node boxes
 };
+
+// The builder's outward padding of a box (bvh.cpp), for the root's box.
+inline void canon_root_box(const BinaryBvh& b, float lo[3], float hi[3])
+{
+	const float l[3] = {b.lo.x, b.lo.y, b.lo.z}, h[3] = {b.hi.x, b.hi.y, b.hi.z};
+	for (int a = 0; a < 3; ++a) {
+		const float m = 1e-5f * std::max(fabsf(l[a]), fabsf(h[a])) + 1e-6f;
+		lo[a] = l[a] - m;
+		hi[a] = h[a] + m;
+	}
+}
 constexpr int kCanonStack = 64; // >= the binary tree's depth, checked at build
 
 __device__ float canon_distance(const CanonView& C, f3 p, unsigned long long& nvis, unsigned long long& ntri)
 {
 	float best = kPenumbraSize;
-	nvis++; // the root's own box; node 0 holds its children's (padded) boxes
+	nvis++; // the root's box
+	if (box_d2(p, C.lo[0], C.lo[1], C.lo[2], C.hi[0], C.hi[1], C.hi[2]) > best * best)
+		return best;
 	int32_t st[kCanonStack];
 	float sk[kCanonStack];
 	int sp = 0;
@@ -1009,7 +1023,9 @@ __device__ float canon_segment(const CanonView& C, const RayTri* __restrict__ rt
 {
 	float best = 2.0f;
 	const Seg s = make_seg(A, B);
-	nvis++; // the root's own box; node 0 holds its children's (padded) boxes
+	nvis++; // the root's box
+	if (slab(s, C.lo[0], C.lo[1], C.lo[2], C.hi[0], C.hi[1], C.hi[2], 1.0f) == INFINITY)
+		return best;
 	int32_t st[kCanonStack];
 	float sk[kCanonStack];
 	int sp = 0;
@@ -1718,6 +1734,7 @@ void DeviceScene::reference_counts(const float* tris, size_t ntri, uint64_t out[
 	CanonView C;
 	C.nodes = d_nodes;
 	C.dtri = d_dt;
+	canon_root_box(b, C.lo, C.hi);
 	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
 	k_march_canon<<<grid_for(nmarch_), kBlock, 0, s>>>(C, V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch_,
 	                                                   d_vis_, d_out);
@@ -1796,6 +1813,7 @@ void DeviceScene::reference_counts_ao(const float* tris, size_t ntri, const floa
 	CanonView C;
 	C.nodes = d_nodes;
 	C.dtri = nullptr;
+	canon_root_box(b, C.lo, C.hi);
 	const unsigned grid = (unsigned)((nsamples_ + kBlock / 64 - 1) / (kBlock / 64));
 	k_ao_canon<<<grid, kBlock, 0, s>>>(C, d_rt, d_pos_, d_nrm_, d_ro, d_side, P, (uint32_t)nsamples_, d_fc, d_out);
 	HIPCHECK(hipGetLastError());

@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -81,9 +82,12 @@ public:
 
 	// --- ambient occlusion (lighter.cpp:799-841) -----------------------
 	// fac_out (device, one float per sample, optional) receives each
-	// texel's AO factor before the colour blend
+	// texel's AO factor before the colour blend.  With between, the texels
+	// go in chunks and between(fraction done) is called after each; it
+	// returns false to stop (the remaining texels keep their colour).
 	void ambient_occlusion(const float* randoff, int num_samples, float distance, float multiplier, float falloff,
-	                       float effect, const float* color, float* fac_out = nullptr);
+	                       float effect, const float* color, float* fac_out = nullptr,
+	                       const std::function<bool(float)>& between = {});
 	// The AO factors again with the rays traced on a plain binary SAH tree
 	// with the canonical closest-hit semantics (measurement only; the
 	// lightmap is left as it was): out[0] node visits, out[1] triangle

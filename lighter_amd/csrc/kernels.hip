@@ -682,6 +682,7 @@ __global__ __launch_bounds__(kBlock) void k_march_canon(CanonView C, View V, con
 // (-9 % march time on the terrain workload).  Only the order in which
 // marches are taken changes, not any result.
 constexpr int kGateRounds = 8;
+constexpr size_t kAOChunk = size_t(1) << 20; // texels per AO launch when progress is followed
 constexpr uint32_t kMarchBatch = 64; // march-list positions a wave claims at a time
 static_assert(kMarchBatch >= 64, "a refill of all 64 lanes must fit in one claim");
 
@@ -965,13 +966,13 @@ template <bool COUNT>
 // 191 M rays; 8 waves spill and lose
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_ao(View V, const float* __restrict__ pos, const float* __restrict__ nrm,
                                                const float* __restrict__ randoff, const float2* __restrict__ side,
-                                               AOParams P, uint32_t S, float* __restrict__ lm,
+                                               AOParams P, uint32_t s0, uint32_t S, float* __restrict__ lm,
                                                unsigned long long* cnt, int depth, float* __restrict__ fac_out)
 {
 	extern __shared__ __align__(16) unsigned char lds_stack[];
 	LdsStack st = lds_stack_of_lane(lds_stack, depth);
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const uint32_t s = blockIdx.x * (kBlock / 64) + wave;
+	const uint32_t s = s0 + blockIdx.x * (kBlock / 64) + wave; // texels [s0, S)
 	if (s >= S)
 		return; // whole waves: no barrier below
 	f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
@@ -1904,7 +1905,8 @@ void DeviceScene::get_vis(float* out, size_t n) const
 }
 
 void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float distance, float multiplier,
-                                    float falloff, float effect, const float* color, float* fac_out)
+                                    float falloff, float effect, const float* color, float* fac_out,
+                                    const std::function<bool(float)>& between)
 {
 	hipStream_t s = (hipStream_t)stream_;
 	if (nsamples_ == 0 || num_samples <= 0)
@@ -1932,17 +1934,28 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 	P.color[0] = color[0];
 	P.color[1] = color[1];
 	P.color[2] = color[2];
-	const unsigned ao_grid = (unsigned)((nsamples_ + kBlock / 64 - 1) / (kBlock / 64)); // a wave per texel
+	// one launch, or, when the caller follows progress, chunks of texels
+	// with the caller asked between them whether to go on
+	const size_t chunk = between ? kAOChunk : nsamples_;
 	EventTimer tm(s);
-	if (count_queries)
-		k_ao<true><<<ao_grid, kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
-		                                                                  (uint32_t)nsamples_, d_lm_, d_counters_,
-		                                                                  march_stack_depth(), fac_out);
-	else
-		k_ao<false><<<ao_grid, kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P,
-		                                                                   (uint32_t)nsamples_, d_lm_, d_counters_,
-		                                                                   march_stack_depth(), fac_out);
-	HIPCHECK(hipGetLastError());
+	for (size_t b = 0; b < nsamples_; b += chunk) {
+		const size_t e = std::min(nsamples_, b + chunk);
+		const unsigned ao_grid = (unsigned)((e - b + kBlock / 64 - 1) / (kBlock / 64)); // a wave per texel
+		if (count_queries)
+			k_ao<true><<<ao_grid, kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P, (uint32_t)b,
+			                                                      (uint32_t)e, d_lm_, d_counters_,
+			                                                      march_stack_depth(), fac_out);
+		else
+			k_ao<false><<<ao_grid, kBlock, march_lds_bytes(), s>>>(V, d_pos_, d_nrm_, d_ro, d_side, P, (uint32_t)b,
+			                                                       (uint32_t)e, d_lm_, d_counters_,
+			                                                       march_stack_depth(), fac_out);
+		HIPCHECK(hipGetLastError());
+		if (between) {
+			HIPCHECK(hipStreamSynchronize(s));
+			if (!between((float)e / (float)nsamples_))
+				break;
+		}
+	}
 	times.ao_ms = tm.stop();
 	read_counters();
 	dfree(d_side);

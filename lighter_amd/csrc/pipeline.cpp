@@ -636,9 +636,14 @@ void stage_ao(ltr_Scene* S)
 	std::vector<float> ro(S->total_samples);
 	for (size_t i = 0; i < ro.size(); ++i)
 		ro[i] = (float)rand() / (float)RAND_MAX;
+	// chunks of texels: progress after each, and an abort stops between them
 	S->dev->ambient_occlusion(ro.data(), cfg.ao_num_samples, cfg.ao_distance, cfg.ao_multiplier, cfg.ao_falloff,
-	                          cfg.ao_effect, cfg.ao_color_rgb);
+	                          cfg.ao_effect, cfg.ao_color_rgb, nullptr, [S](float done) {
+		                          S->completion.store(done);
+		                          return !S->abort.load();
+	                          });
 	S->times.ao_ms = S->dev->times.ao_ms;
+	check_abort(S);
 	S->completion.store(1.0f);
 }
 

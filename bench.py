@@ -166,6 +166,19 @@ def export_vs_host(lib, b):
     return out
 
 
+def link_reference_counts(lib, dev, tris):
+    """The last lgx_RadLinks' candidate pairs (kept: it ran with counting on)
+    tested again with the canonical any-hit walk on a plain binary SAH tree:
+    node visits, triangle tests, pairs, and links that differ."""
+    out = (C.c_uint64 * 5)()
+    _lib.check(lib.lgx_ReferenceCountsLinks(dev, P(tris), len(tris), out), lib)
+    v, t, pairs, diff, links = (int(x) for x in out)
+    log(f"[bench] link visibility, reference-shaped: {pairs} candidate pairs, {v} node visits, {t} triangle tests; "
+        f"{diff} of them differ from the product's links ({links} links)")
+    return dict(candidate_pairs=pairs, node_visits=v, triangle_tests=t, links=links, links_differing=diff,
+                bytes=v * 32 + t * 36)
+
+
 def make_lights(spec):
     arr = (_lib.LgxLight * len(spec.lights))()
     for i, L in enumerate(spec.lights):
@@ -471,11 +484,14 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
     pn = np.array([normalize_f32(v) for v in nrm], np.float32)
     nl = C.c_size_t(0)
     t0 = time.perf_counter()
+    lib.lgx_SetCounting(dev, 1)  # keeps the candidate pairs for the reference-shaped pass
     _lib.check(lib.lgx_RadLinks(dev, P(pos), P(pn), N, C.byref(nl)), lib)
+    lib.lgx_SetCounting(dev, 0)
     tm = (C.c_double * 6)()
     lib.lgx_DeviceTimes(dev, tm)
     link_ms = tm[3]
     log(f"[bench] C2: {N} patches, {nl.value} links ({link_ms:.1f} ms GPU, {time.perf_counter() - t0:.1f} s wall)")
+    link_refc = link_reference_counts(lib, dev, tris)
     links_cmp = links_bench(lib, nth) if with_cpu else None
     alb = spec.albedo
     diff = np.array([alb.get(int(p), (1.0, 1.0, 1.0)) for p in rad[:, 2]], np.float32)
@@ -525,6 +541,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
     out = dict(workload=f"C2 room radiosity: {N} patches (texels of the 10x10x4 m room), {R} right-hand sides "
                         f"({setups} seeded lighting setups x RGB), {bounces} bounces", patches=N, rhs=R,
                bounces=bounces, links=int(nl.value), link_density=round(2 * nl.value / (N * (N - 1)), 5),
+               link_reference_counts=link_refc,
                stored_tile_fraction=round(tiles, 4), link_generation_s=link_ms / 1000.0, forms=res,
                fp64_vs_oracle_rel_l2=pin,
                mfma_bf16_speedup_vs_dense_f32=round(res["dense_f32"]["s_per_bounce"] / mf, 2),
@@ -790,9 +807,12 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
     rad_dev = lib.lgx_DeviceCreate(P(tris), len(tris))
     _lib.check(lib.lgx_SetSamples(rad_dev, P(ppos), P(np.ascontiguousarray(nrm[:N])), N, None), lib)
     nl = C.c_size_t(0)
+    lib.lgx_SetCounting(rad_dev, 1)  # keeps the candidate pairs for the reference-shaped pass
     _lib.check(lib.lgx_RadLinks(rad_dev, P(ppos), P(pn), N, C.byref(nl)), lib)
+    lib.lgx_SetCounting(rad_dev, 0)
     lib.lgx_DeviceTimes(rad_dev, tm)
     link_ms = tm[3]
+    link_refc = link_reference_counts(lib, rad_dev, tris)
     lid = (items >> 26).astype(np.int64)
     smp = (items & ((1 << 26) - 1)).astype(np.int64)
     wv = np.where(w.view(np.uint32) == 0xFFFFFFFF, 0.0, w.astype(np.float64))
@@ -824,6 +844,7 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
         log(f"[bench] C4 radiosity {key}: {per:.3f} ms/bounce, err {forms[key]['rel_l2_err_vs_fp64']:.2e}")
     tiles = tile_fraction(lib, rad_dev)
     out["radiosity"] = dict(patches=N, rhs=R, bounces=bounces, links=int(nl.value), link_generation_ms=round(link_ms, 1),
+                            link_reference_counts=link_refc,
                             stored_tile_fraction=round(tiles, 4), forms=forms)
     sp = forms["sparse_f32"]["s_per_bounce"]
     best = min((v["s_per_bounce"], k) for k, v in forms.items()

@@ -180,7 +180,8 @@ LTRAPI int lgx_RadTiles(lgx_Device* dev, uint64_t out[2]);
 LTRAPI int lgx_DirectLightGrad(lgx_Device* dev, const double* lam3, double* dcolor3);
 /* Traversal counters (node visits, full triangle tests, march steps,
  * queries, marches, triangles culled by the distance query's box pre-test)
- * collected by the next stage when enabled. */
+ * collected by the next stage when enabled; with counting on, lgx_RadLinks
+ * also keeps its candidate pairs for lgx_ReferenceCountsLinks. */
 LTRAPI int lgx_SetCounting(lgx_Device* dev, int on);
 LTRAPI int lgx_ReadCounters(lgx_Device* dev, uint64_t out[6]);
 /* No counterpart in the reference (measurement only). The last
@@ -201,6 +202,13 @@ LTRAPI int lgx_ReferenceCounts(lgx_Device* dev, const float* tris, size_t ntri, 
  * samples. */
 LTRAPI int lgx_ReferenceCountsAO(lgx_Device* dev, const float* tris, size_t ntri, const float* randoff,
                                  int num_samples, float distance, float multiplier, float falloff, uint64_t out[5]);
+/* No counterpart in the reference (measurement only). The occlusion test of
+ * every candidate pair of the last lgx_RadLinks (run with lgx_SetCounting
+ * on, which keeps the pairs) again with the canonical any-hit walk (nearest
+ * child first, ending at the first hit with r < 1) on the same plain binary
+ * SAH tree: out[0] node visits, out[1] triangle tests, out[2] pairs tested,
+ * out[3] pairs whose link differs from the product's, out[4] links. */
+LTRAPI int lgx_ReferenceCountsLinks(lgx_Device* dev, const float* tris, size_t ntri, uint64_t out[5]);
 /* Last stage times in ms: direct, direct march, AO, links, bounces,
  * direct-light gradient. */
 LTRAPI int lgx_DeviceTimes(lgx_Device* dev, double out[6]);

@@ -75,6 +75,7 @@ LGX_SIGNATURES = {
     "lgx_ReferenceCounts": (C.c_int, [vp, fp, C.c_size_t, C.POINTER(C.c_uint64)]),
     "lgx_ReferenceCountsAO": (C.c_int, [vp, fp, C.c_size_t, fp, C.c_int, C.c_float, C.c_float, C.c_float,
                                         C.POINTER(C.c_uint64)]),
+    "lgx_ReferenceCountsLinks": (C.c_int, [vp, fp, C.c_size_t, C.POINTER(C.c_uint64)]),
     "lgx_DeviceTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
 }
 

@@ -755,6 +755,11 @@ int lgx_ReferenceCountsAO(lgx_Device* d, const float* tris, size_t ntri, const f
 	});
 }
 
+int lgx_ReferenceCountsLinks(lgx_Device* d, const float* tris, size_t ntri, uint64_t out[5])
+{
+	return guarded([&] { d->dev.reference_counts_links(tris, ntri, out); });
+}
+
 int lgx_ReadCounters(lgx_Device* d, uint64_t out[6])
 {
 	out[0] = d->dev.counters.nodes;

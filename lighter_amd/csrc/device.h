@@ -95,6 +95,12 @@ public:
 	// product's, out[4] texels.
 	void reference_counts_ao(const float* tris, size_t ntri, const float* randoff, int num_samples, float distance,
 	                         float multiplier, float falloff, uint64_t out[5]);
+	// The occlusion test of every candidate pair of the last link search
+	// (run with counting on, which keeps the pairs) with the canonical
+	// any-hit walk on the same binary tree: out[0] node visits, out[1]
+	// triangle tests, out[2] pairs tested, out[3] pairs whose link differs
+	// from the product's, out[4] links.
+	void reference_counts_links(const float* tris, size_t ntri, uint64_t out[5]);
 
 	// --- radiosity (lighter.cpp:667-797) ---------------------------------
 	// Patches = samples.  rad_links builds the symmetric link structure of
@@ -164,6 +170,8 @@ private:
 	uint32_t* d_skey_ = nullptr; // 30-bit Morton code of each sample position (march ordering)
 	float* d_nrm_ = nullptr;
 	float* d_lm_ = nullptr;
+	void* d_cand_ = nullptr; // candidate pairs (uint2) of the last link search, kept in counting mode
+	uint64_t ncand_ = 0;
 	size_t nsamples_ = 0;
 	// direct light
 	float* d_w_ = nullptr;

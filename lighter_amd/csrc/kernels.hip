@@ -1019,6 +1019,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
 // with the canonical semantics: nearest child first by entry parameter, the
 // segment's far end shrinking to the closest hit found.  Counts as
 // canon_distance does.
+// ANY = true: any hit, the walk ending at the first hit with r < 1 (the
+// reference's VisibilityTest, lighter.cpp:112-147).
+template <bool ANY>
 __device__ float canon_segment(const CanonView& C, const RayTri* __restrict__ rtri, f3 A, f3 B,
                                unsigned long long& nvis, unsigned long long& ntri)
 {
@@ -1063,6 +1066,8 @@ __device__ float canon_segment(const CanonView& C, const RayTri* __restrict__ rt
 				ntri++;
 				if (r < best)
 					best = r;
+				if (ANY && best < 1.0f)
+					return best;
 			}
 		}
 		bool found = false;
@@ -1108,7 +1113,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_canon(CanonView C, const RayTri* 
 			sincos((double)angle, &sa, &ca);
 			float2 sd = side[i];
 			f3 d = spiral_dir(SN, sd.x, sd.y, (float)ca, (float)sa) * P.distance;
-			float hit = canon_segment(C, rtri, origin, origin + d, nv, nt);
+			float hit = canon_segment<false>(C, rtri, origin, origin + d, nv, nt);
 			nr++;
 			if (hit < 1.0f)
 				term = 1.0f - hit;
@@ -1127,6 +1132,29 @@ __global__ __launch_bounds__(kBlock) void k_ao_canon(CanonView C, const RayTri* 
 	if (P.falloff)
 		ao = pow_r(ao, P.falloff);
 	fac_out[s] = ao;
+}
+
+// The occlusion test of every candidate pair of the last link search
+// (Int_RDGenLinks, lighter.cpp:751) with the canonical any-hit walk; out:
+// node visits, triangle tests, pairs whose link bit differs from the
+// product's.
+__global__ __launch_bounds__(kBlock) void k_link_canon(CanonView C, const RayTri* __restrict__ rtri,
+                                                       const float* __restrict__ pos,
+                                                       const uint2* __restrict__ cand, uint64_t ncand,
+                                                       const uint32_t* __restrict__ bits, uint64_t words,
+                                                       unsigned long long* out)
+{
+	const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= ncand)
+		return;
+	const uint2 c = cand[k];
+	unsigned long long nv = 0, nt = 0;
+	const bool occluded = canon_segment<true>(C, rtri, ld3(pos + 3 * c.x), ld3(pos + 3 * c.y), nv, nt) < 1.0f;
+	const bool linked = (bits[(uint64_t)c.x * words + (c.y >> 5)] >> (c.y & 31)) & 1u;
+	atomicAdd(out + 0, nv);
+	atomicAdd(out + 1, nt);
+	if (occluded != linked)
+		atomicAdd(out + 2, 1ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -1426,6 +1454,8 @@ DeviceScene::~DeviceScene()
 
 void DeviceScene::free_all()
 {
+	dfree(d_cand_);
+	ncand_ = 0;
 	dfree(d_nodes_);
 	dfree(d_dtri_);
 	dfree(d_rtri_);
@@ -1834,6 +1864,57 @@ void DeviceScene::reference_counts_ao(const float* tris, size_t ntri, const floa
 	out[4] = nsamples_;
 }
 
+void DeviceScene::reference_counts_links(const float* tris, size_t ntri, uint64_t out[5])
+{
+	for (int i = 0; i < 5; ++i)
+		out[i] = 0;
+	if (ntri != ntri_)
+		throw std::runtime_error("reference_counts_links: the triangles must be those of the device scene");
+	if (!d_cand_)
+		throw std::runtime_error("reference_counts_links: run the link search with counting on first");
+	if (ncand_ == 0 || ntri == 0)
+		return;
+	BinaryBvh b = build_bvh_binary(tris, ntri, 16, 4);
+	if (b.depth > kCanonStack)
+		throw std::runtime_error("reference_counts_links: binary tree deeper than the counting stack");
+	std::vector<RayTri> rt(ntri);
+	for (size_t i = 0; i < ntri; ++i) {
+		const float* p = tris + 9 * (size_t)b.order[i];
+		rt[i] = make_ray_tri(ld3(p), ld3(p + 3), ld3(p + 6));
+	}
+	hipStream_t s = (hipStream_t)stream_;
+	BvhNode* d_nodes = dalloc<BvhNode>(b.nodes.size());
+	RayTri* d_rt = dalloc<RayTri>(ntri);
+	unsigned long long* d_out = dalloc<unsigned long long>(3);
+	struct Free {
+		std::vector<void*> p;
+		~Free()
+		{
+			for (void* q : p)
+				if (q)
+					(void)hipFree(q);
+		}
+	} fr{{d_nodes, d_rt, d_out}};
+	HIPCHECK(hipMemcpyAsync(d_nodes, b.nodes.data(), b.nodes.size() * sizeof(BvhNode), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_rt, rt.data(), ntri * sizeof(RayTri), hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemsetAsync(d_out, 0, 3 * sizeof(unsigned long long), s));
+	CanonView C;
+	C.nodes = d_nodes;
+	C.dtri = nullptr;
+	canon_root_box(b, C.lo, C.hi);
+	k_link_canon<<<grid_for(ncand_), kBlock, 0, s>>>(C, d_rt, rad.pos, (const uint2*)d_cand_, ncand_, rad.bits,
+	                                                 rad.words, d_out);
+	HIPCHECK(hipGetLastError());
+	unsigned long long h[3];
+	HIPCHECK(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	out[0] = h[0];
+	out[1] = h[1];
+	out[2] = ncand_;
+	out[3] = h[2];
+	out[4] = rad.nnz / 2;
+}
+
 void DeviceScene::read_counters()
 {
 	if (!count_queries)
@@ -2098,7 +2179,14 @@ size_t DeviceScene::rad_links()
 			HIPCHECK(hipGetLastError());
 		}
 		HIPCHECK(hipStreamSynchronize(s));
-		dfree(cand);
+		dfree(d_cand_);
+		ncand_ = 0;
+		if (count_queries) { // kept for reference_counts_links
+			d_cand_ = (void*)cand;
+			ncand_ = ncand;
+		} else {
+			dfree(cand);
+		}
 	}
 	// symmetric CSR
 	uint32_t* cnt = dalloc<uint32_t>(N + 1);

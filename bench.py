@@ -139,7 +139,7 @@ def placement_vs_host(lib, spec, nthreads, b_gpu, pos_gpu):
 
 def export_vs_host(lib, b):
     """The lightmap export ("exporting lightmaps") of prepared scene b on the
-    GPU, then again with the host restatement (its outputs follow the
+    GPU, then again with the host restatement (its outputs replace the
     first call's): both wall times and whether every image is identical.
     The scene's lightmaps hold the ambient colour here (the device-level
     light runs of this bench do not write them back), so this times the
@@ -149,15 +149,16 @@ def export_vs_host(lib, b):
     gpu_s = time.perf_counter() - t0
     t = (C.c_double * 5)()
     lib.lgx_FinalizeTimes(b.scene, t)
+    outs_gpu = b.outputs()
     _lib.check(lib.lgx_SetHostStages(b.scene, 2), lib)  # host export
     t0 = time.perf_counter()
-    _lib.check(lib.lgx_StageFinalize(b.scene), lib)
+    _lib.check(lib.lgx_StageFinalize(b.scene), lib)  # replaces the output list
     host_s = time.perf_counter() - t0
-    outs = b.outputs()
-    n = len(outs) // 2
-    same = len(outs) == 2 * n and all(
-        outs[k][:3] == outs[n + k][:3] and np.array_equal(outs[k][3].view(np.uint32), outs[n + k][3].view(np.uint32))
-        for k in range(n))
+    outs_host = b.outputs()
+    n = len(outs_gpu)
+    same = len(outs_host) == n and all(
+        g[:3] == h[:3] and np.array_equal(g[3].view(np.uint32), h[3].view(np.uint32))
+        for g, h in zip(outs_gpu, outs_host))
     out = dict(images=n, texels=int(t[4]), kernels_ms=round(t[2], 3), pack_ms=round(t[0], 1),
                upload_ms=round(t[1], 1), download_ms=round(t[3], 1), stage_s_gpu=round(gpu_s, 3),
                stage_s_host=round(host_s, 3), identical_images=bool(same))

@@ -5,7 +5,7 @@
  * reference baker's public header (archo5/lighter, lighter.h:1-264), with the
  * same memory layout, so that applications written against the reference
  * link against liblighter_amd.so unchanged.  The layout is fixed by the ABI;
- * the sizes and offsets below are asserted in lighter_amd/csrc/abi_check.cpp.
+ * the sizes and offsets below are asserted in lighter_amd/csrc/api.cpp.
  *
  *   struct               size   (x86-64)
  *   ltr_MeshPartInfo       72

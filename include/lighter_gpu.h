@@ -29,7 +29,9 @@ LTRAPI const char* lgx_LastError(void);
 LTRAPI int lgx_HasDevice(void);
 
 /* ---- staged bake of an ltr_Scene (synchronous, calling thread) -------- */
-/* Stages "transforming spatial data" .. "generating samples"
+/* Starts a staged bake: drops the samples, lightmaps and device state of an
+ * earlier bake (as ltr_Start does), then runs the stages
+ * "transforming spatial data" .. "generating samples"
  * (lighter.cpp:1050-1073): transforms, collision set-up and rasterisation
  * on the host threads; the sample placement (OffsetSample and the
  * correction in front of overlapping faces, lighter.cpp:436-466) on the GPU
@@ -62,7 +64,9 @@ LTRAPI int lgx_StageRadiosity(ltr_Scene* scene);
 LTRAPI int lgx_StageAO(ltr_Scene* scene);
 /* "exporting lightmaps" (lighter.cpp:1138-1140, 849-1044): fills the work
  * outputs; the images (extrapolation, blur, downsample) on the GPU unless
- * lgx_SetHostStages selected the host, the normal-map data on the host. */
+ * lgx_SetHostStages selected the host, the normal-map data on the host.
+ * Each export replaces the output list; the images of an earlier export
+ * stay allocated until ltr_DestroyScene. */
 LTRAPI int lgx_StageFinalize(ltr_Scene* scene);
 
 LTRAPI int lgx_InstanceCount(ltr_Scene* scene);

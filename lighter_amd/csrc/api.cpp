@@ -155,10 +155,11 @@ void ltr_DestroyScene(ltr_Scene* scene)
 		delete mi;
 	for (ltr_Mesh* m : scene->meshes)
 		delete m;
-	for (ltr_WorkOutput& o : scene->outputs) {
-		delete[] o.lightmap_rgb;
-		delete[] o.normals_xyzf;
-	}
+	for (auto* list : {&scene->outputs, &scene->retired})
+		for (ltr_WorkOutput& o : *list) {
+			delete[] o.lightmap_rgb;
+			delete[] o.normals_xyzf;
+		}
 	delete scene;
 }
 
@@ -168,6 +169,10 @@ void ltr_Start(ltr_Scene* scene)
 	if (scene->stage.load() != nullptr && scene->worker.joinable())
 		return; // a bake is already running
 	join_bake(scene);
+	// every bake starts from the scene as added: samples, lightmaps and
+	// device state of an earlier bake are dropped (its output images stay
+	// allocated until ltr_DestroyScene)
+	reset_bake(scene);
 	scene->nthreads = std::max(std::min(scene->config.max_num_threads, online_cpus()), 1);
 	scene->abort.store(false);
 	scene->failed.store(false);
@@ -322,6 +327,7 @@ int lgx_Prepare(ltr_Scene* S)
 	return guarded([&] {
 		staged_begin(S);
 		S->abort.store(false);
+		reset_bake(S);
 		prepare(S);
 		S->stage.store(nullptr);
 	});

@@ -276,14 +276,24 @@ void finalize_gpu(const std::vector<FinImage>& imgs, const float* d_rgb, float b
 	memcpy(host.p + o_dfb, dfb.data(), n * 4);
 	if (!kernel.empty())
 		memcpy(host.p + o_kern, kernel.data(), kernel.size() * 4);
+	std::atomic<bool> bad_loc{false};
 	parallel_items(nthreads, (size_t)n, [&](size_t k) {
 		const FinImage& im = imgs[k];
+		const uint64_t np = (uint64_t)im.w * im.h;
 		uint32_t* map = (uint32_t*)(host.p + o_map) + s_off[k];
-		for (size_t i = 0; i < im.nsamples; ++i)
+		for (size_t i = 0; i < im.nsamples; ++i) {
+			if (im.loc[i] >= np) {
+				bad_loc.store(true);
+				map[i] = 0;
+				continue;
+			}
 			map[i] = (uint32_t)(inst[k].px_off + im.loc[i]);
+		}
 		if (!d_rgb)
 			memcpy((float*)(host.p + o_lm) + 3 * s_off[k], im.rgb, im.nsamples * 12);
 	});
+	if (bad_loc.load())
+		throw std::runtime_error("finalize: a sample's texel index lies outside its instance's atlas");
 	tm.pack_ms = ms_since(t0);
 
 	t0 = Clock::now();

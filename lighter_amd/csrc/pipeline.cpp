@@ -330,6 +330,18 @@ static void gen_samples(ltr_Scene* S)
 	});
 }
 
+// Per-bake state back to that of a scene that has not been baked: the next
+// preparation runs every host stage again (the reference's ltr_Start re-runs
+// them too, lighter.cpp:1046-1073), so lightmaps restart from the ambient
+// colour and no normal-map contribution carries over.
+void reset_bake(ltr_Scene* S)
+{
+	S->dev.reset();
+	S->prepared = false;
+	for (Instance* mi : S->insts)
+		mi->contrib.clear();
+}
+
 void prepare(ltr_Scene* S)
 {
 	if (S->prepared)
@@ -702,14 +714,18 @@ static void downsample2x(float* dst, unsigned dw, unsigned dh, const float* src,
 }
 
 // The image of one instance on the host (lighter.cpp:866-958): the samples'
-// colours in their texels, extrapolation, blur, downsample (which updates
-// the instance's lightmap size).  Instances run in parallel.
+// colours in their texels, extrapolation, blur, downsample (the exported
+// size goes to out_w/out_h; lm_w/lm_h keep the atlas size the samples'
+// texel indices refer to).  Instances run in parallel.
 static float* image_host(ltr_Scene* S, size_t mid)
 {
 	Instance* mi = S->insts[mid];
 	const ltr_Config& cfg = S->config;
 	uint32_t w = mi->lm_w, h = mi->lm_h;
 	size_t npx = (size_t)w * h;
+	for (uint32_t loc : mi->s_loc)
+		if (loc >= npx)
+			throw std::runtime_error("export: a sample's texel index lies outside its instance's atlas");
 	float* rgb = new float[npx * 3]();
 	std::vector<char> set(npx * 2, 0);
 	for (size_t i = 0; i < mi->s_loc.size(); ++i) {
@@ -772,9 +788,11 @@ static float* image_host(ltr_Scene* S, size_t mid)
 		downsample2x(ds, hw, hh, rgb, w, h);
 		delete[] rgb;
 		rgb = ds;
-		mi->lm_w = hw;
-		mi->lm_h = hh;
+		w = hw;
+		h = hh;
 	}
+	mi->out_w = w;
+	mi->out_h = h;
 	return rgb;
 }
 
@@ -808,7 +826,7 @@ static ltr_WorkOutput output_of(ltr_Scene* S, size_t mid, float* rgb)
 			d = lerpf(1.0f, d, tmin(length(cn) / length(t.normal), 1.0f));
 			t.mindot *= d;
 		}
-		size_t nn = (size_t)mi->lm_w * mi->lm_h;
+		size_t nn = (size_t)mi->out_w * mi->out_h;
 		normals = new float[nn * 4]();
 		for (size_t i = 0; i < mi->s_loc.size(); ++i) {
 			size_t loc = (size_t)mi->s_loc[i];
@@ -829,8 +847,8 @@ static ltr_WorkOutput output_of(ltr_Scene* S, size_t mid, float* rgb)
 	wo.inst_ident_size = mi->ident.size();
 	wo.lightmap_rgb = rgb;
 	wo.normals_xyzf = normals;
-	wo.width = mi->lm_w;
-	wo.height = mi->lm_h;
+	wo.width = mi->out_w;
+	wo.height = mi->out_h;
 	return wo;
 }
 
@@ -872,13 +890,17 @@ static void images_gpu(ltr_Scene* S, std::vector<float*>& rgb)
 		d_rgb = S->dev->lightmap_device() + 3 * S->inst_off[1];
 	finalize_gpu(imgs, d_rgb, cfg.blur_size, kernel, cfg.ds2x != 0, S->nthreads, &S->fin_times);
 	for (size_t q = 0; q < ids.size(); ++q) {
-		S->insts[ids[q]]->lm_w = imgs[q].final_w;
-		S->insts[ids[q]]->lm_h = imgs[q].final_h;
+		S->insts[ids[q]]->out_w = imgs[q].final_w;
+		S->insts[ids[q]]->out_h = imgs[q].final_h;
 	}
 }
 
 void finalize(ltr_Scene* S)
 {
+	// an export replaces the output list; the images of an earlier one stay
+	// allocated (the caller may hold them) until the scene is destroyed
+	S->retired.insert(S->retired.end(), S->outputs.begin(), S->outputs.end());
+	S->outputs.clear();
 	const size_t n = S->insts.size();
 	std::vector<ltr_WorkOutput> outs(n);
 	for (size_t mid = 0; mid < n; ++mid) {

@@ -17,6 +17,7 @@ struct AbortBake {};
 constexpr size_t kMaxPatches = 262144;
 
 void bake(ltr_Scene* S);                  // the whole pipeline (ltr_Start's thread)
+void reset_bake(ltr_Scene* S);            // forget the last bake's samples, lightmaps and device state
 void prepare(ltr_Scene* S);               // host stages up to sample generation
 void ensure_device(ltr_Scene* S);         // upload triangles, samples, lightmaps
 void download_lightmaps(ltr_Scene* S);

@@ -31,7 +31,11 @@ struct Instance {
 	std::string ident;
 	float importance = 0;
 	Mat4 matrix{};
+	// atlas size from the size callback (full resolution, as the samples'
+	// texel indices use it); out_w/out_h: the exported image's size (halved
+	// by ds2x), set by each export
 	uint32_t lm_w = 0, lm_h = 0;
+	uint32_t out_w = 0, out_h = 0;
 	bool shadow = false;
 	bool samplecont = false;
 	// world-space vertex data
@@ -75,6 +79,9 @@ struct ltr_Scene {
 	std::vector<ltr_SampleInfo> samples;
 	lgx::BoxTree inst_tree;
 	std::vector<ltr_WorkOutput> outputs;
+	// images of earlier exports: the caller may still hold their pointers,
+	// so they stay allocated until ltr_DestroyScene
+	std::vector<ltr_WorkOutput> retired;
 
 	// bake state
 	std::atomic<const char*> stage{"not started"};

@@ -157,17 +157,64 @@ def test_bake_without_device_reports_failed(lib):
 
 def test_abort_stops_bake(lib):
     """ltr_Abort is a no-op in the reference (lighter.cpp:1154-1157); here the
-    bake stops at the next stage boundary and reports finished."""
+    bake stops at the next stage boundary and reports finished with no
+    outputs.  The abort is issued from the size callback, which the first
+    stage calls (lighter.cpp:325-330), so it lands before any later stage
+    whatever the thread timing."""
     spec = scenes.room_scene(lm=64)
     spec.lights = []
     b = scenes.BuiltScene(lib, spec)
     lib.lgx_SetHostStages(b.scene, 3)  # placement and export on the host: no GPU needed here
+    calls = []
+
+    def size_fn(cfgp, mesh, mesh_size, inst, inst_size, area, importance, out):
+        calls.append(1)
+        lib.ltr_Abort(b.scene)
+        out[0] = out[1] = 64
+        return 1
+
+    cb = api.SIZE_FN(size_fn)
+    cfg = api.Config()
+    lib.ltr_GetConfig(C.byref(cfg), b.scene)
+    cfg.size_fn = C.cast(cb, C.c_void_p).value
+    lib.ltr_SetConfig(b.scene, C.byref(cfg))
     lib.ltr_Start(b.scene)
-    lib.ltr_Abort(b.scene)
     st = api.WorkStatus()
     while lib.ltr_GetStatus(b.scene, C.byref(st)):
         lib.ltr_Sleep(1)
-    assert st.stage in (b"finished",)
+    assert calls, "the size callback did not run"
+    assert st.stage == b"finished"
+    info = api.WorkOutputInfo()
+    lib.ltr_GetWorkOutputInfo(b.scene, C.byref(info))
+    assert info.lightmap_count == 0
+    b.destroy()
+
+
+def test_second_bake_gives_the_same_images(lib):
+    """A second ltr_Start bakes the scene again from its inputs: the same
+    images at the same (downsampled) size, and no more outputs than the
+    first bake made.  ds2x halves the exported image while the samples keep
+    their full-resolution texel indices; a re-export must not scatter those
+    into the halved image.  Placement and export on the host, no lights:
+    no GPU stage runs."""
+    spec = scenes.room_scene(lm=64, bounce_count=0)
+    spec.lights = []
+    b = scenes.BuiltScene(lib, spec, dict(ds2x=1, blur_size=1.0))
+    assert lib.lgx_SetHostStages(b.scene, 3) == 1
+    b.bake()
+    first = b.outputs()
+    assert len(first) == 1 and first[0][1] == first[0][2] == 32
+    b.bake()
+    second = b.outputs()
+    assert len(second) == len(first)
+    for f, s in zip(first, second):
+        assert f[:3] == s[:3]
+        np.testing.assert_array_equal(f[3].view(np.uint32), s[3].view(np.uint32))
+    # a staged export after the bake uses the full-resolution atlas again
+    assert lib.lgx_StageFinalize(b.scene) == 1, _lib.last_error(lib)
+    third = b.outputs()
+    assert [o[:3] for o in third] == [o[:3] for o in first]
+    np.testing.assert_array_equal(third[0][3].view(np.uint32), first[0][3].view(np.uint32))
     b.destroy()
 
 

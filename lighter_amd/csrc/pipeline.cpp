@@ -61,49 +61,95 @@ static void check_abort(ltr_Scene* S)
 }
 
 // ---------------------------------------------------------------------------
+// triangles of a mesh part
+// ---------------------------------------------------------------------------
+// The reference forms a part's triangles with `tri < index_count` in steps
+// of 3 over the mesh's index array from the part's first index, vertex
+// indices relative to the part's first vertex (lighter.cpp:41-47, 308-317,
+// 367-373).  A part whose index count is not a multiple of 3 (MeshAddPart
+// refuses only counts 1 and 2, lighter.cpp:1237) therefore completes its
+// last triangle with the indices that follow it in the mesh, the next
+// part's.  fn(a, b, c) receives mesh vertex indices.  Where such a read
+// would leave the mesh's index array, or an index the mesh's vertex arrays,
+// the reference reads past the end of its vectors (undefined); that
+// triangle is skipped.
+template <class F> static void part_triangles(const ltr_Mesh* mesh, const MeshPart& mp, F&& fn)
+{
+	const size_t ni = mesh->indices.size(), nv = mesh->vpos.size();
+	for (uint32_t t = 0; t < mp.icount; t += 3) {
+		const size_t first = (size_t)mp.ioff + t;
+		if (first + 3 > ni)
+			break;
+		const size_t a = (size_t)mp.voff + mesh->indices[first], b = (size_t)mp.voff + mesh->indices[first + 1],
+		             c = (size_t)mp.voff + mesh->indices[first + 2];
+		if (a >= nv || b >= nv || c >= nv)
+			continue;
+		fn((uint32_t)a, (uint32_t)b, (uint32_t)c);
+	}
+}
+
+// ---------------------------------------------------------------------------
 // transforming spatial data (lighter.cpp:292-341)
 // ---------------------------------------------------------------------------
+// World-space vertices of an instance: positions by its matrix, normals by
+// the inverse transpose, renormalised.
+static void world_vertices(Instance* mi)
+{
+	const ltr_Mesh* mesh = mi->mesh;
+	const Mat4 to_world = mi->matrix, for_normals = normal_matrix(mi->matrix);
+	mi->vpos.clear();
+	mi->vnrm.clear();
+	for (f3 p : mesh->vpos)
+		mi->vpos.push_back(mat4_transform(to_world, p, 1.0f));
+	for (f3 n : mesh->vnrm)
+		mi->vnrm.push_back(normalized(mat4_transform(for_normals, n, 0.0f)));
+}
+
+// Atlas size from the size callback, given the instance's world-space
+// surface area (triangles summed in part and index order); the config's
+// default size when there is no callback or it declines.
+static void atlas_size(ltr_Scene* S, Instance* mi)
+{
+	const ltr_Mesh* mesh = mi->mesh;
+	float area = 0.0f;
+	for (const MeshPart& mp : mesh->parts)
+		part_triangles(mesh, mp, [&](uint32_t a, uint32_t b, uint32_t c) {
+			area += triangle_area(mi->vpos[a], mi->vpos[b], mi->vpos[c]);
+		});
+	ltr_Config& cfg = S->config;
+	u32 wh[2] = {cfg.default_width, cfg.default_height};
+	const bool sized = cfg.size_fn && cfg.size_fn(&cfg, mesh->ident.c_str(), mesh->ident.size(), mi->ident.c_str(),
+	                                              mi->ident.size(), area, mi->importance, wh);
+	mi->lm_w = sized ? wh[0] : cfg.default_width;
+	mi->lm_h = sized ? wh[1] : cfg.default_height;
+}
+
+// Per-vertex texture coordinates: the first set as given, the second in
+// texel units of the atlas (texel centres at integers).
+static void atlas_coords(Instance* mi)
+{
+	const ltr_Mesh* mesh = mi->mesh;
+	const f2 texels = mk2((float)mi->lm_w, (float)mi->lm_h), centre = mk2(0.5f, 0.5f);
+	mi->vtex.assign(mesh->vtex1.begin(), mesh->vtex1.end());
+	mi->ltex.resize(mesh->vtex2.size());
+	for (size_t i = 0; i < mesh->vtex2.size(); ++i)
+		mi->ltex[i] = mesh->vtex2[i] * texels - centre;
+}
+
 static void pre_xform(ltr_Scene* S, Instance* mi)
 {
 	if (mi->samplecont)
 		return;
-	ltr_Mesh* mesh = mi->mesh;
-	mi->vpos.resize(mesh->vpos.size());
-	mi->vnrm.resize(mesh->vnrm.size());
-	mi->vtex.resize(mesh->vtex1.size());
-	mi->ltex.resize(mesh->vtex2.size());
-	for (size_t i = 0; i < mesh->vpos.size(); ++i)
-		mi->vpos[i] = mat4_transform(mi->matrix, mesh->vpos[i], 1.0f);
-	Mat4 nm = normal_matrix(mi->matrix);
-	for (size_t i = 0; i < mesh->vnrm.size(); ++i)
-		mi->vnrm[i] = normalized(mat4_transform(nm, mesh->vnrm[i], 0.0f));
-
-	float total_area = 0.0f;
-	for (const MeshPart& mp : mesh->parts) {
-		const f3* vb = mi->vpos.data() + mp.voff;
-		const uint32_t* ib = mesh->indices.data() + mp.ioff;
-		for (uint32_t t = 0; t + 2 < mp.icount; t += 3)
-			total_area += triangle_area(vb[ib[t]], vb[ib[t + 1]], vb[ib[t + 2]]);
-	}
-	ltr_Config& cfg = S->config;
-	u32 sz[2] = {cfg.default_width, cfg.default_height};
-	if (!cfg.size_fn || !cfg.size_fn(&cfg, mesh->ident.c_str(), mesh->ident.size(), mi->ident.c_str(),
-	                                 mi->ident.size(), total_area, mi->importance, sz)) {
-		sz[0] = cfg.default_width;
-		sz[1] = cfg.default_height;
-	}
-	mi->lm_w = sz[0];
-	mi->lm_h = sz[1];
-	f2 lsize = mk2((float)mi->lm_w, (float)mi->lm_h);
-	for (size_t i = 0; i < mi->ltex.size(); ++i) {
-		mi->vtex[i] = mesh->vtex1[i];
-		mi->ltex[i] = mesh->vtex2[i] * lsize - mk2(0.5f, 0.5f);
-	}
+	world_vertices(mi);
+	atlas_size(S, mi);
+	atlas_coords(mi);
 }
 
 // ---------------------------------------------------------------------------
 // generating data structures (lighter.cpp:349-384, 1060-1069)
 // ---------------------------------------------------------------------------
+// The instance's collision set: triangles of its shadow-casting parts that
+// are not degenerate (Triangle::CheckIsUseful, lighter_int.hpp:604-608).
 static void col_info(ltr_Scene* S, Instance* mi)
 {
 	(void)S;
@@ -112,17 +158,15 @@ static void col_info(ltr_Scene* S, Instance* mi)
 		return;
 	}
 	std::vector<Tri> tris;
-	ltr_Mesh* mesh = mi->mesh;
+	const ltr_Mesh* mesh = mi->mesh;
 	for (const MeshPart& mp : mesh->parts) {
 		if (!mp.shadow)
 			continue;
-		const f3* vb = mi->vpos.data() + mp.voff;
-		const uint32_t* ib = mesh->indices.data() + mp.ioff;
-		for (uint32_t t = 0; t + 2 < mp.icount; t += 3) {
-			Tri T{vb[ib[t]], vb[ib[t + 1]], vb[ib[t + 2]]};
+		part_triangles(mesh, mp, [&](uint32_t a, uint32_t b, uint32_t c) {
+			const Tri T{mi->vpos[a], mi->vpos[b], mi->vpos[c]};
 			if (tri_useful(T))
 				tris.push_back(T);
-		}
+		});
 	}
 	if (!tris.empty())
 		mi->split.set(tris.data(), tris.size());
@@ -170,11 +214,7 @@ static void rasterize_instance(Instance* mi, float margin, f3* R1, f3* R2, f4* R
 {
 	ltr_Mesh* mesh = mi->mesh;
 	for (uint32_t part = 0; part < mesh->parts.size(); ++part) {
-		const MeshPart& mp = mesh->parts[part];
-		uint32_t O = mp.voff;
-		const uint32_t* ib = mesh->indices.data() + mp.ioff;
-		for (uint32_t t = 0; t + 2 < mp.icount; t += 3) {
-			uint32_t a = O + ib[t], b = O + ib[t + 1], c = O + ib[t + 2];
+		part_triangles(mesh, mesh->parts[part], [&](uint32_t a, uint32_t b, uint32_t c) {
 			float area = sample_area(mi->ltex[a], mi->ltex[b], mi->ltex[c], mi->vpos[a], mi->vpos[b], mi->vpos[c]);
 			f4 ia{mi->vtex[a].x, mi->vtex[a].y, (float)part, area};
 			f4 ibv{mi->vtex[b].x, mi->vtex[b].y, (float)part, area};
@@ -182,7 +222,7 @@ static void rasterize_instance(Instance* mi, float margin, f3* R1, f3* R2, f4* R
 			rasterize_triangle(R1, R2, R3, (int32_t)mi->lm_w, (int32_t)mi->lm_h, margin, mi->ltex[a], mi->ltex[b],
 			                   mi->ltex[c], mi->vpos[a], mi->vpos[b], mi->vpos[c], mi->vnrm[a], mi->vnrm[b],
 			                   mi->vnrm[c], ia, ibv, ic);
-		}
+		});
 	}
 }
 

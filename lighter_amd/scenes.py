@@ -473,6 +473,76 @@ def basic_scene():
     return SceneSpec("basic", [mesh], lights, dict(max_num_threads=1))
 
 
+def sliver_scene(seed=SEED_BASE + 0x51, lm=64):
+    """Edge cases of the triangle set (no counterpart among the reference's
+    scenarios): a 4 x 4 m floor, 24 standing slivers whose cross products
+    have a component within a few ulps of the 0.001 threshold of
+    Triangle::CheckIsUseful (lighter_int.hpp:604-608; some are kept, some
+    dropped), and a part of 7 indices whose last triangle takes its two
+    missing indices from the next part (lighter.cpp:41-47,367-373; index
+    counts other than 1 and 2 are accepted, lighter.cpp:1237).  One point
+    light and a sun."""
+    rng = np.random.default_rng(seed)
+    f32 = np.float32
+    # floor: 4 x 4 quads
+    n = 4
+    g = np.linspace(-2, 2, n + 1, dtype=f32)
+    xx, yy = np.meshgrid(g, g)
+    fpos = np.stack([xx.ravel(), yy.ravel(), np.zeros((n + 1) ** 2, f32)], 1).astype(f32)
+    fnrm = np.tile(np.array([0, 0, 1], f32), ((n + 1) ** 2, 1))
+    ft2 = (0.02 + 0.46 * (fpos[:, :2] + 2) / 4).astype(f32)
+    fidx = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            fidx += [a, a + 1, a + n + 2, a + n + 2, a + n + 1, a]
+    floor = Part(fpos, fnrm, ft2.copy(), ft2, np.array(fidx, np.uint32), 1)
+    # slivers: P1, P1 + L e_u, P1 + h e_v with L h within ulps of 0.001; the
+    # cross product then has the single component +-L h on the third axis
+    spos, snrm, st2, sidx = [], [], [], []
+    axes = [(0, 2, 1), (2, 1, 0), (1, 2, 0)]  # (u axis, v axis, normal axis)
+    for k in range(24):
+        u, v, w = axes[k % 3]
+        L = f32(rng.uniform(0.2, 0.6))
+        target = np.nextafter(f32(0.001), f32(1 if k % 2 else -1), dtype=f32)
+        for _ in range(k % 4):
+            target = np.nextafter(target, f32(1 if k % 2 else -1), dtype=f32)
+        h = f32(target / L)
+        P1 = np.array([rng.uniform(-1.6, 1.6), rng.uniform(-1.6, 1.6), rng.uniform(0.2, 1.0)], f32)
+        if w == 2:
+            P1[2] = f32(0.5 + 0.02 * k)
+        P2 = P1.copy(); P2[u] = f32(P2[u] + L)
+        P3 = P1.copy(); P3[v] = f32(P3[v] + h)
+        nn = np.zeros(3, f32); nn[w] = 1
+        base = len(spos)
+        spos += [P1, P2, P3]
+        snrm += [nn, nn, nn]
+        cx, cy = 0.52 + 0.45 * (k % 6) / 6, 0.02 + 0.45 * (k // 6) / 4
+        st2 += [(cx, cy), (cx + 0.06, cy), (cx, cy + 0.09)]
+        sidx += [base, base + 1, base + 2]
+    slivers = Part(np.array(spos, f32), np.array(snrm, f32), np.array(st2, f32), np.array(st2, f32),
+                   np.array(sidx, np.uint32), 1)
+    # a ragged part: a standing quad (vertices 0-3) plus vertices 4, 5 that
+    # only the incomplete last triangle (4, next part's first two indices)
+    # reaches
+    rp = np.array([[-1.5, 1.2, 0.0], [-0.5, 1.2, 0.0], [-0.5, 1.2, 1.0], [-1.5, 1.2, 1.0],
+                   [0.5, -1.5, 0.0], [1.5, -1.5, 0.0]], f32)
+    rn = np.tile(np.array([0, -1, 0], f32), (6, 1))
+    rt2 = np.array([[0.52, 0.55], [0.72, 0.55], [0.72, 0.75], [0.52, 0.75], [0.76, 0.55], [0.95, 0.55]], f32)
+    ragged = Part(rp, rn, rt2.copy(), rt2, np.array([0, 1, 2, 2, 3, 0, 4], np.uint32), 1)
+    # the part after it: a small standing triangle whose first two indices
+    # (5, 2) are also valid in the ragged part's vertex range
+    tp = np.array([[1.0, 0.5, 0.0], [1.4, 0.5, 0.0], [1.2, 0.5, 0.8], [1.0, 0.9, 0.0], [1.4, 0.9, 0.0],
+                   [1.2, 0.9, 0.8]], f32)
+    tn = np.tile(np.array([0, -1, 0], f32), (6, 1))
+    tt2 = np.array([[0.52, 0.80], [0.60, 0.80], [0.56, 0.90], [0.64, 0.80], [0.72, 0.80], [0.68, 0.90]], f32)
+    tail = Part(tp, tn, tt2.copy(), tt2, np.array([5, 2, 0, 3, 4, 5], np.uint32), 1)
+    mesh = Mesh("slivers", [floor, slivers, ragged, tail], [Instance(IDENTITY.copy(), 1.0, 1, "")])
+    lights = [point_light((0.3, 0.2, 2.5), (0.9, 0.8, 0.7), 8.0, 1.0, 0.1, 4),
+              dir_light((0.3, 0.2, 1.0), (0.4, 0.4, 0.5), 10.0, radius=0.05)]
+    return SceneSpec("slivers", [mesh], lights, dict(max_num_threads=1, blur_size=0.0), lm_size=lm)
+
+
 # --------------------------------------------------------------------------
 # building a scene through the ltr_* API
 # --------------------------------------------------------------------------

@@ -243,6 +243,7 @@ class Reference:
             ("rp_seg_tri", C.c_float, [fp] * 5),
             ("rp_spiral", None, [fp, C.c_float, C.c_int, C.c_int, fp]),
             ("rp_point_tri_dist_capped", C.c_float, [fp] * 4),
+            ("rp_tri_useful", C.c_int, [fp]),
         ]:
             fn = getattr(L, name)
             fn.restype = res

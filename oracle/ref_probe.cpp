@@ -317,6 +317,13 @@ void rp_spiral(const float* dir, float randoff, int i, int n, float* out)
 	out[0] = v.x; out[1] = v.y; out[2] = v.z;
 }
 
+// Triangle::CheckIsUseful (lighter_int.hpp:604-608) on one triangle, 9 floats.
+int rp_tri_useful(const float* t9)
+{
+	Triangle T = {V3P(t9), V3P(t9 + 3), V3P(t9 + 6)};
+	return T.CheckIsUseful() ? 1 : 0;
+}
+
 // Point-triangle distance as the reference's distance query sees it: a
 // one-triangle TriTree queried with the initial radius MAX_PENUMBRA_SIZE.
 float rp_point_tri_dist_capped(const float* p, const float* t0, const float* t1, const float* t2)

@@ -214,3 +214,24 @@ def test_full_bake_matches_reference(lib, ref, libc, variant):
         assert close.mean() >= 0.995, close.mean()
     bp.destroy()
     br.destroy()
+
+
+def test_second_bake_gives_the_same_images_and_normals(lib):
+    """Two ltr_Start bakes of one scene: the second starts again from the
+    ambient colour and drops the first bake's normal-map contributions, so
+    images and normal data are bit-identical; ds2x halves the export while
+    the samples keep their full-resolution texel indices."""
+    spec = scenes.room_scene(lm=32, n_boxes=6, wall_tess=4, bounce_count=0)
+    bp = scenes.BuiltScene(lib, spec, dict(ds2x=1, generate_normalmap_data=1))
+    t1 = bp.bake()
+    first = bp.outputs()
+    t2 = bp.bake()
+    second = bp.outputs()
+    assert "failed" not in t1 and "failed" not in t2
+    assert len(first) == len(second) == 1
+    for f, s in zip(first, second):
+        assert f[:3] == s[:3] and f[1] == 16
+        assert np.array_equal(f[3].view(np.uint32), s[3].view(np.uint32))
+        assert f[4] is not None and np.array_equal(f[4].view(np.uint32), s[4].view(np.uint32))
+        assert f[3].max() > 0.05
+    bp.destroy()

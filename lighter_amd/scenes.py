@@ -92,34 +92,61 @@ class Chart:
         return float(np.linalg.norm(self.du)), float(np.linalg.norm(self.dv))
 
 
-def _shelf_pack(sizes, W, H, pad):
-    """Shelf-pack rectangles (w, h) in pixels; returns corners or None."""
-    order = sorted(range(len(sizes)), key=lambda i: -sizes[i][1])
-    out = [None] * len(sizes)
-    x = y = pad
-    row_h = 0
-    for i in order:
-        w, h = sizes[i]
-        if x + w + pad > W:
-            x = pad
-            y += row_h + pad
-            row_h = 0
-        if x + w + pad > W or y + h + pad > H:
+@dc.dataclass
+class QuadSet:
+    """Many flat 1 x 1 charts of one part (box faces), as arrays: corner,
+    u edge and v edge of each quad (float64, (n, 3))."""
+    origin: np.ndarray
+    du: np.ndarray
+    dv: np.ndarray
+    part: int
+
+    def __len__(self):
+        return len(self.origin)
+
+
+def _shelf_pack(wh, W, H, pad):
+    """Shelf-pack rectangles wh (n, 2) in pixels, tallest first (ties in
+    input order); returns the corners (n, 2) or None.  Rows fill left to
+    right while pad + sum(w + pad) <= W; a row is as tall as its first
+    (tallest) rectangle; rows stack from y = pad with pad between them."""
+    n = len(wh)
+    order = np.argsort(-wh[:, 1], kind="stable")
+    w = wh[order, 0]
+    h = wh[order, 1]
+    cw = np.cumsum(w + pad)
+    out = np.empty((n, 2), np.int64)
+    start, y, base = 0, pad, 0
+    while start < n:
+        end = int(np.searchsorted(cw, base + W - pad, side="right"))
+        if end <= start or y + h[start] + pad > H:
             return None
-        out[i] = (x, y)
-        x += w + pad
-        row_h = max(row_h, h)
+        out[order[start:end], 0] = pad + cw[start:end] - (w[start:end] + pad) - base
+        out[order[start:end], 1] = y
+        y += int(h[start]) + pad
+        base = int(cw[end - 1])
+        start = end
     return out
 
 
-def pack_charts(charts, W, H, pad=2):
-    """Pick the largest texels-per-metre scale at which all charts fit."""
+def _chart_metric(charts, quads=()):
+    m = [(c.size_m[0], c.size_m[1]) for c in charts]
+    arr = [np.array(m, np.float64).reshape(-1, 2)]
+    for q in quads:
+        arr.append(np.stack([np.sqrt((q.du * q.du).sum(-1)), np.sqrt((q.dv * q.dv).sum(-1))], -1))
+    return np.concatenate(arr)
+
+
+def pack_charts(charts, W, H, pad=2, quads=()):
+    """Pick the largest texels-per-metre scale at which all charts (then the
+    quads of each QuadSet) fit: 40 bisection steps over [0, 4096]; each
+    chart gets max(1, int(size_m * scale)) pixels per side."""
     lo, hi = 0.0, 4096.0
     best = None
-    metric = [c.size_m for c in charts]
+    metric = _chart_metric(charts, quads)
     for _ in range(40):
         s = 0.5 * (lo + hi)
-        sizes = [(max(1, int(mu * s)), max(1, int(mv * s))) for mu, mv in metric]
+        sizes = np.maximum(1, (metric * s).astype(np.int64))
         pos = _shelf_pack(sizes, W, H, pad)
         if pos is None:
             hi = s
@@ -130,15 +157,11 @@ def pack_charts(charts, W, H, pad=2):
     return best
 
 
-def _quad_batch(lst, W, H):
+def _quad_arrays(O, DU, DV, rect, W, H):
     """Vertices and indices of many 1x1 charts at once, in the per-chart
     layout: vertices (a, b) = (0,0), (1,0), (0,1), (1,1), triangles
-    (v00, v10, v11), (v11, v01, v00)."""
-    n = len(lst)
-    O = np.array([c.origin for c, *_ in lst], np.float64)
-    DU = np.array([c.du for c, *_ in lst], np.float64)
-    DV = np.array([c.dv for c, *_ in lst], np.float64)
-    rect = np.array([(x0, y0, wp, hp) for _, x0, y0, wp, hp in lst], np.float64)
+    (v00, v10, v11), (v11, v01, v00).  rect: (x0, y0, w, h) per quad."""
+    n = len(O)
     A = np.array([0.0, 1.0, 0.0, 1.0])
     B = np.array([0.0, 0.0, 1.0, 1.0])
     P = O[:, None, :] + A[None, :, None] * DU[:, None, :] + B[None, :, None] * DV[:, None, :]
@@ -152,10 +175,26 @@ def _quad_batch(lst, W, H):
     return verts, idx
 
 
-def charts_to_part_arrays(charts, W, H, pad=2):
-    """Tessellate charts and assign UV2 atlas coordinates; returns per-part arrays."""
-    pos_rect, sizes = pack_charts(charts, W, H, pad)
+def _quad_batch(lst, W, H):
+    """_quad_arrays over (chart, x0, y0, w, h) tuples of 1x1 charts."""
+    O = np.array([c.origin for c, *_ in lst], np.float64)
+    DU = np.array([c.du for c, *_ in lst], np.float64)
+    DV = np.array([c.dv for c, *_ in lst], np.float64)
+    rect = np.array([(x0, y0, wp, hp) for _, x0, y0, wp, hp in lst], np.float64)
+    return _quad_arrays(O, DU, DV, rect, W, H)
+
+
+def charts_to_part_arrays(charts, W, H, pad=2, quads=()):
+    """Tessellate charts (and the quads of each QuadSet, packed after them)
+    and assign UV2 atlas coordinates; returns per-part arrays."""
+    pos_rect, sizes = pack_charts(charts, W, H, pad, quads)
     by_part = {}
+    k = len(charts)
+    for q in quads:
+        assert all(c.part != q.part for c in charts), "a QuadSet's part holds no other chart"
+        rect = np.concatenate([pos_rect[k:k + len(q)], sizes[k:k + len(q)]], 1).astype(np.float64)
+        by_part.setdefault(q.part, []).append(_quad_arrays(q.origin, q.du, q.dv, rect, W, H))
+        k += len(q)
     # parts made only of single flat quads (boxes) are tessellated in one batch
     quad_parts = {}
     for c in charts:
@@ -230,6 +269,25 @@ def box_charts(lo, hi, part, with_bottom=True):
     if with_bottom:
         ch.append(Chart(lo + Y, X, -Y, 1, 1, part))  # bottom, -z
     return ch
+
+
+def box_quads(lo, hi, part, with_bottom=True):
+    """box_charts of many boxes at once (lo, hi: (n, 3)): the faces of each
+    box in box_charts' order, box after box, as one QuadSet."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    d = hi - lo
+    z = np.zeros(len(lo))
+    X = np.stack([d[:, 0], z, z], 1)
+    Y = np.stack([z, d[:, 1], z], 1)
+    Z = np.stack([z, z, d[:, 2]], 1)
+    faces = [(lo + Z, X, Y), (lo, X, Z), (lo + Y + X, -X, Z), (lo + Y, -Y, Z), (lo + X, Y, Z)]
+    if with_bottom:
+        faces.append((lo + Y, X, -Y))
+    O = np.stack([f[0] for f in faces], 1).reshape(-1, 3)
+    DU = np.stack([f[1] for f in faces], 1).reshape(-1, 3)
+    DV = np.stack([f[2] for f in faces], 1).reshape(-1, 3)
+    return QuadSet(O, DU, DV, part)
 
 
 def room_charts(size=(10.0, 10.0, 4.0), tess=8, part=0):
@@ -357,6 +415,7 @@ def terrain_scene(seed=SEED_BASE + 1, grid=512, extent=200.0, amplitude=20.0, n_
         i, j = min(int(fx), grid - 1), min(int(fy), grid - 1)
         return H[j, i]
 
+    tile_boxes = [[[] for _ in range(tiles)] for _ in range(tiles)]
     for b in range(n_boxes):
         size = g.uniform(box_size[0], box_size[1], 3)
         cx = g.uniform(size[0], extent - size[0])
@@ -365,11 +424,13 @@ def terrain_scene(seed=SEED_BASE + 1, grid=512, extent=200.0, amplitude=20.0, n_
         lo = np.array([cx - size[0] / 2, cy - size[1] / 2, z0])
         tx = min(int(cx / (q * cell)), tiles - 1)
         ty = min(int(cy / (q * cell)), tiles - 1)
-        tile_charts[ty][tx] += box_charts(lo, lo + size, part=1, with_bottom=True)
+        tile_boxes[ty][tx].append((lo, lo + size))
     meshes = []
     for ty in range(tiles):
         for tx in range(tiles):
-            parts = charts_to_part_arrays(tile_charts[ty][tx], lm, lm)
+            bx = tile_boxes[ty][tx]
+            quads = [box_quads([lo for lo, _ in bx], [hi for _, hi in bx], part=1)] if bx else []
+            parts = charts_to_part_arrays(tile_charts[ty][tx], lm, lm, quads=quads)
             ident = f"tile_{tx}_{ty}"
             meshes.append(Mesh(ident, parts, [Instance(IDENTITY.copy(), 1.0, 1, ident)]))
     lights = []
@@ -420,20 +481,27 @@ def city_scene(seed=SEED_BASE + 4, extent=1000.0, grid=256, amplitude=6.0, n_box
     sizes = np.minimum(g.lognormal(1.0, 0.6, (n_boxes, 3)), 0.9 * pitch * np.array([1, 1, 8]))
     jitter = g.uniform(-0.5, 0.5, (n_boxes, 2))
     tile_m = q * cell
-    for b in range(n_boxes):
-        gx, gy = b % side, b // side
-        size = sizes[b]
-        slack = np.maximum(pitch - size[:2], 0.0)
-        cx = (gx + 0.5) * pitch + jitter[b, 0] * slack[0]
-        cy = (gy + 0.5) * pitch + jitter[b, 1] * slack[1]
-        lo = np.array([cx - size[0] / 2, cy - size[1] / 2, height_at(cx, cy) - 0.3])
-        tx = min(int(cx / tile_m), tiles - 1)
-        ty = min(int(cy / tile_m), tiles - 1)
-        tile_charts[ty][tx] += box_charts(lo, lo + size, part=1, with_bottom=True)
+    # every box at once: grid cell, jittered centre within the cell's slack,
+    # floor at the terrain height under the centre minus 0.3 m
+    b = np.arange(n_boxes)
+    gx, gy = (b % side).astype(np.float64), (b // side).astype(np.float64)
+    slack = np.maximum(pitch - sizes[:, :2], 0.0)
+    cx = (gx + 0.5) * pitch + jitter[:, 0] * slack[:, 0]
+    cy = (gy + 0.5) * pitch + jitter[:, 1] * slack[:, 1]
+    hz = H[np.minimum((cy / cell).astype(np.int64), grid - 1), np.minimum((cx / cell).astype(np.int64), grid - 1)]
+    lo = np.stack([cx - sizes[:, 0] / 2, cy - sizes[:, 1] / 2, hz - 0.3], 1)
+    hi = lo + sizes
+    tile = (np.minimum((cy / tile_m).astype(np.int64), tiles - 1) * tiles
+            + np.minimum((cx / tile_m).astype(np.int64), tiles - 1))
+    order = np.argsort(tile, kind="stable")
+    bounds = np.searchsorted(tile[order], np.arange(tiles * tiles + 1))
     meshes = []
     for ty in range(tiles):
         for tx in range(tiles):
-            parts = charts_to_part_arrays(tile_charts[ty][tx], lm, lm)
+            t = ty * tiles + tx
+            sel = order[bounds[t]:bounds[t + 1]]
+            quads = [box_quads(lo[sel], hi[sel], part=1)] if len(sel) else []
+            parts = charts_to_part_arrays(tile_charts[ty][tx], lm, lm, quads=quads)
             ident = f"block_{tx}_{ty}"
             meshes.append(Mesh(ident, parts, [Instance(IDENTITY.copy(), 1.0, 1, ident)]))
     lights = []

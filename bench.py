@@ -3,20 +3,24 @@
 
 Headline: shadow rays per second (one ray = one soft-shadow march of a
 (texel, light) pair with a nonzero shading factor, lighter.cpp:485-600) for
-the direct-lighting stage of the terrain workload: 512x512 value-noise
-heightfield over 200 m (524,288 tris) plus 2,000 boxes (24,000 tris) in
-16x16 tile instances with 128x128 lightmaps each (2048x2048 texels in total),
-32 point/spot lights and one sun (march length 300 m).
+the direct-lighting stage of the 2M-triangle city block: a 1 km value-noise
+heightfield plus 155,000 lognormal boxes (1,991,072 triangles) in 16 x 16
+tile instances with 256 x 256 lightmaps each (4096 x 4096 texels in total)
+and 64 point/spot lights.
 
-A step is one pass of the direct-light stage (march kernel + in-order
-accumulation) over all work items, with the scene, samples and work items
-resident in HBM.  Timing: HIP events around each pass on the library's
-stream, plus a host wall clock around the K timed steps.
+A step is one pass of the direct-light stage (shading gate, march kernel,
+in-order accumulation) over all work items, with the scene, samples and
+work items resident in HBM.  Timing: a host wall clock around the K timed
+steps (each step returns after its work has finished on the device), and
+HIP events around each march launch on the library's stream.
 
-The JSON line also carries the dominant kernel's roofline figure (algorithmic
-bytes per launch from traversal counters / average kernel time) and the
-reference CPU baker's shadow-march rate on the host cores, measured on a
-bounded sample of the same work items (built from the reference sources).
+The JSON line also carries the march kernel's roofline figure (algorithmic
+bytes per launch from reference-shaped traversal counts / average kernel
+time) and the reference CPU baker's shadow-march rate on the host cores,
+on the same marches (built from the reference sources).  Extra keys: the
+rest of the city bake (AO, radiosity forms, colour gradient, export,
+placement), the terrain workload's direct light, the room radiosity forms
+and AO, and the gradient workload.
 """
 from __future__ import annotations
 
@@ -32,7 +36,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from lighter_amd import _lib, scenes  # noqa: E402
+from lighter_amd import _lib, scenes, workloads  # noqa: E402
+from lighter_amd.workloads import host_cores, make_lights  # noqa: E402
 
 fp = C.POINTER(C.c_float)
 u32p = C.POINTER(C.c_uint32)
@@ -53,53 +58,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_threads():
-    try:
-        return max(1, min(16, len(os.sched_getaffinity(0))))
-    except AttributeError:
-        return max(1, min(16, os.cpu_count() or 1))
-
-
 def prepare_workload(lib, spec, nthreads, radinfo=None):
-    """Host stages through the product; samples, shadow triangles and the
-    direct-light work items of every light.  radinfo (a list) receives the
-    per-texel radiosity info (uv, part id, world area) when given."""
-    b = scenes.BuiltScene(lib, spec, dict(max_num_threads=nthreads))
+    """workloads.direct_workload, with the placement figures attached."""
     t0 = time.perf_counter()
-    _lib.check(lib.lgx_Prepare(b.scene), lib)
-    b.prepare_s = time.perf_counter() - t0
-    b.placement = placement_info(lib, b)
-    n_inst = lib.lgx_InstanceCount(b.scene)
-    pos, nrm, tris, counts = [], [], [], []
-    for i in range(n_inst):
-        info = (C.c_int32 * 4)()
-        lib.lgx_InstanceInfo(b.scene, i, info)
-        n, nt = info[2], info[3]
-        p = np.zeros((n, 3), np.float32)
-        q = np.zeros((n, 3), np.float32)
-        r = np.zeros((n, 4), np.float32) if radinfo is not None else None
-        lib.lgx_GetSamples(b.scene, i, P(p), P(q), None, P(r) if r is not None else None)
-        if radinfo is not None:
-            radinfo.append(r)
-        t = np.zeros((nt, 9), np.float32)
-        if nt:
-            lib.lgx_GetTriangles(b.scene, i, P(t))
-        pos.append(p)
-        nrm.append(q)
-        tris.append(t)
-        counts.append(n)
-    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-    # work items per light: every sample of every instance the light reaches
-    items, light_off = [], [0]
-    buf = (C.c_int32 * (n_inst + 1))()
-    for l in range(len(spec.lights)):
-        k = lib.lgx_LightInstances(b.scene, l, buf, n_inst + 1)
-        ids = list(buf[:k])
-        s = np.concatenate([np.arange(off[i], off[i + 1], dtype=np.uint32) for i in ids])
-        items.append((np.uint32(l) << np.uint32(26)) | s)
-        light_off.append(light_off[-1] + len(s))
-    return (b, np.concatenate(pos), np.concatenate(nrm), np.concatenate(tris),
-            np.concatenate(items).astype(np.uint32), np.array(light_off, np.uint32))
+    w = workloads.direct_workload(lib, spec, nthreads, radinfo)
+    w[0].placement = placement_info(lib, w[0])
+    return w
 
 
 def placement_info(lib, b):
@@ -178,28 +142,6 @@ def link_reference_counts(lib, dev, tris):
         f"{diff} of them differ from the product's links ({links} links)")
     return dict(candidate_pairs=pairs, node_visits=v, triangle_tests=t, links=links, links_differing=diff,
                 bytes=v * 32 + t * 36)
-
-
-def make_lights(spec):
-    arr = (_lib.LgxLight * len(spec.lights))()
-    for i, L in enumerate(spec.lights):
-        d = np.asarray(L.get("direction", [0, 0, 0]), np.float32)
-        l2 = np.float32(d[0] * d[0]) + np.float32(d[1] * d[1])
-        l2 = np.float32(l2 + np.float32(d[2] * d[2]))
-        if l2 > 0:
-            d = (d * (np.float32(1) / np.sqrt(l2, dtype=np.float32))).astype(np.float32)
-        a = arr[i]
-        a.type = L["type"]
-        a.pos[:] = L.get("position", [0, 0, 0])
-        a.dir[:] = [float(x) for x in d]
-        a.color[:] = L.get("color_rgb", [0, 0, 0])
-        a.range = L.get("range", 0)
-        a.power = L.get("power", 0)
-        a.radius = L.get("light_radius", 0)
-        a.spot_angle_out = L.get("spot_angle_out", 0)
-        a.spot_angle_in = L.get("spot_angle_in", 0)
-        a.spot_curve = L.get("spot_curve", 0)
-    return arr
 
 
 def cpu_baseline(spec, pos, nrm, items, w, lights, target_s, nthreads, ao=None):
@@ -454,10 +396,25 @@ def ao_bench(lib, dev, spec, S, nth, with_cpu, tris):
     return out
 
 
+def reference_bounce_seconds(rowmap, other, fac, diff, E, area, N):
+    """CPU leg: the reference's Int_RDBounce loop (lighter.cpp:767-797,
+    restated in C in the oracle) for one RGB setup on the same links, one
+    thread as the reference runs it; seconds per bounce, or None."""
+    try:
+        from oracle.checkers import Oracle
+        orc = Oracle()
+        c13 = np.concatenate([diff, E[:, :3], E[:, :3], np.zeros((N, 3)), area[:, None]], 1).astype(np.float32)
+        t0 = time.perf_counter()
+        orc.rad_bounce_f32(rowmap, other, fac, c13)
+        return time.perf_counter() - t0
+    except Exception as e:  # the checker is optional on a bare install
+        log(f"[bench] reference bounce loop skipped: {e}")
+        return None
+
+
 def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with_cpu=True):
     """C2: radiosity over the room's texels as patches, R right-hand sides
     (setups x RGB) built from the lights' direct contributions."""
-    from oracle.checkers import normalize_f32
     spec, b, pos, nrm, rad, tris, N, _ = room_patches(lib, nth, n_patches, 200, bounce_count=bounces,
                                                       ao_distance=2.0, ao_samples=64, name="C2")
     dev = lib.lgx_DeviceCreate(P(tris), len(tris))
@@ -482,7 +439,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
             E[:, 3 * s + c] = W @ cols[s, :, c]
     E = np.ascontiguousarray(E[:, :R])
     # links (patch normals normalised again, as Int_RDGenLinks does)
-    pn = np.array([normalize_f32(v) for v in nrm], np.float32)
+    pn = scenes.normalize_rows_f32(nrm)
     nl = C.c_size_t(0)
     t0 = time.perf_counter()
     lib.lgx_SetCounting(dev, 1)  # keeps the candidate pairs for the reference-shaped pass
@@ -509,22 +466,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
     ref64, t64 = run(0)
     res = {}
     rowmap, other, fac = read_links(lib, dev, N, nl.value)
-    # pin the GPU FP64 series against the C oracle on the first setup (RGB)
-    pin = None
-    try:
-        from oracle.checkers import Oracle
-        orc = Oracle()
-        t0 = time.perf_counter()
-        exp = orc.rad_series_f64(rowmap, other, fac, area, diff, E[:, :3], bounces)
-        pin = float(np.linalg.norm(ref64[:, :3] - exp) / np.linalg.norm(exp))
-        # the reference's own FP32 bounce loop (one RGB setup), restated in C
-        c13 = np.concatenate([diff, E[:, :3], E[:, :3], np.zeros((N, 3)), area[:, None]], 1).astype(np.float32)
-        t0 = time.perf_counter()
-        c13 = orc.rad_bounce_f32(rowmap, other, fac, c13)
-        cpu_bounce_s = time.perf_counter() - t0
-    except Exception as e:  # the checker is optional on a bare install
-        log(f"[bench] oracle pin skipped: {e}")
-        cpu_bounce_s = None
+    cpu_bounce_s = reference_bounce_seconds(rowmap, other, fac, diff, E, area, N) if with_cpu else None
     nrm64 = np.linalg.norm(ref64)
     for form, refine in [(0, 0), (1, 0), (2, 0), (3, 0), (3, 1), (4, 0), (4, 1)]:
         out, t = (ref64, t64) if form == 0 else run(form, refine)
@@ -544,7 +486,6 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
                bounces=bounces, links=int(nl.value), link_density=round(2 * nl.value / (N * (N - 1)), 5),
                link_reference_counts=link_refc,
                stored_tile_fraction=round(tiles, 4), link_generation_s=link_ms / 1000.0, forms=res,
-               fp64_vs_oracle_rel_l2=pin,
                mfma_bf16_speedup_vs_dense_f32=round(res["dense_f32"]["s_per_bounce"] / mf, 2),
                mfma_f16_refined_speedup_vs_dense_f32=round(
                    res["dense_f32"]["s_per_bounce"] / res["dense_f16_mfma_refined"]["s_per_bounce"], 2),
@@ -570,7 +511,6 @@ def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
     against the sparse FP64 form's gradient.  Second part: the colour
     gradient over the checkpointed visibility of a ckpt_lm^2 atlas (every
     texel x 64 lights)."""
-    from oracle.checkers import normalize_f32
     kw = dict(seed=scenes.SEED_BASE + 3, n_boxes=31, wall_tess=40, n_point=64, with_sun=False, bounce_count=bounces,
               name="C3")
     spec, b, pos, nrm, rad, tris, N, lm = room_patches(lib, nth, n_patches, 256, **kw)
@@ -589,7 +529,7 @@ def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
     lib.lgx_ReadWeights(dev, P(w), None, len(items))
     w[w.view(np.uint32) == 0xFFFFFFFF] = 0
     W = w.reshape(L, N).T.astype(np.float64)  # N x L
-    pn = np.array([normalize_f32(v) for v in nrm], np.float32)
+    pn = scenes.normalize_rows_f32(nrm)
     nl = C.c_size_t(0)
     _lib.check(lib.lgx_RadLinks(dev, P(pos), P(pn), N, C.byref(nl)), lib)
     lib.lgx_DeviceTimes(dev, tm)
@@ -691,122 +631,169 @@ def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
     return out
 
 
-def prepare_with_progress(lib, spec, nth, every_s=20.0, radinfo=None):
-    """prepare_workload, logging the host stage every every_s seconds (the
-    2M-triangle scene spends minutes in the reference's sample generation)."""
-    import threading
-    res = {}
-    th = threading.Thread(target=lambda: res.update(out=prepare_workload(lib, spec, nth, radinfo)))
-    t0 = time.perf_counter()
-    th.start()
-    while th.is_alive():
-        th.join(every_s)
-        if th.is_alive():
-            log(f"[bench] host stages running ({time.perf_counter() - t0:.0f} s)")
-    return res["out"]
+class DirectLeg:
+    """One direct-light workload on the device: the host stages through the
+    product, the device scene, the work items, W untimed and K timed passes
+    of the direct-light stage, the roofline of the march kernel and the
+    reference's shadow march on a sample of the same marches."""
+
+    def __init__(self, lib, spec, nth, radinfo=False):
+        self.lib, self.spec, self.nth = lib, spec, nth
+        t0 = time.perf_counter()
+        self.radinfo = [] if radinfo else None
+        (self.built, self.pos, self.nrm, self.tris, self.items,
+         self.light_off) = prepare_workload(lib, spec, nth, self.radinfo)
+        if self.radinfo is not None:
+            self.radinfo = np.concatenate(self.radinfo)
+        self.host_s = time.perf_counter() - t0
+        self.S = len(self.pos)
+        log(f"[bench] {spec.name} host stages: {self.S} samples, {len(self.items)} work items ({self.host_s:.1f} s)")
+        t0 = time.perf_counter()
+        self.dev = lib.lgx_DeviceCreate(P(self.tris), len(self.tris))
+        if not self.dev:
+            raise RuntimeError(_lib.last_error(lib))
+        self.device_create_s = time.perf_counter() - t0
+        info = (C.c_int64 * 3)()
+        lib.lgx_DeviceInfo(self.dev, info)
+        self.ntri, self.nodes_bvh, self.depth = int(info[0]), int(info[1]), int(info[2])
+        log(f"[bench] {spec.name} device scene (BVH build, triangle forms, upload): {self.device_create_s:.2f} s; "
+            f"{self.ntri} tris, {self.nodes_bvh} nodes, depth {self.depth}")
+        _lib.check(lib.lgx_SetSamples(self.dev, P(self.pos), P(self.nrm), self.S, None), lib)
+        self.lights = make_lights(spec)
+        self.setup(keep_vis=1)
+
+    def setup(self, keep_vis):
+        _lib.check(self.lib.lgx_DirectLightSetup(self.dev, self.lights, len(self.lights),
+                                                 self.items.ctypes.data_as(u32p), len(self.items),
+                                                 self.light_off.ctypes.data_as(u32p), keep_vis), self.lib)
+
+    def run(self, steps, warmup):
+        """Counters from one instrumented pass, then warmup + timed passes:
+        wall time around the K passes, HIP-event time of each march launch."""
+        lib = self.lib
+        lib.lgx_SetCounting(self.dev, 1)
+        _lib.check(lib.lgx_DirectLightRun(self.dev), lib)
+        cnt = (C.c_uint64 * 6)()
+        lib.lgx_ReadCounters(self.dev, cnt)
+        lib.lgx_SetCounting(self.dev, 0)
+        self.nodes, self.ntests, self.qsteps, self.queries, self.marches, self.npre = (int(x) for x in cnt)
+        tm = (C.c_double * 6)()
+        for _ in range(warmup):
+            _lib.check(lib.lgx_DirectLightRun(self.dev), lib)
+        march_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _lib.check(lib.lgx_DirectLightRun(self.dev), lib)
+            lib.lgx_DeviceTimes(self.dev, tm)
+            march_ms.append(tm[1])
+        wall = time.perf_counter() - t0
+        self.steps, self.warmup = steps, warmup
+        self.ms_per_step = wall * 1000.0 / steps
+        self.march_avg = float(np.mean(march_ms))
+        self.value = self.marches / (self.ms_per_step / 1000.0)
+        log(f"[bench] {self.spec.name} direct light: {steps} steps, {self.ms_per_step:.2f} ms/step, march kernel "
+            f"{self.march_avg:.2f} ms; {self.marches} marches, {self.qsteps} distance queries: "
+            f"{self.value / 1e6:.1f} M shadow rays/s")
+        self.w = np.zeros(len(self.items), np.float32)
+        self.vis = np.zeros(len(self.items), np.float32)
+        _lib.check(lib.lgx_ReadWeights(self.dev, P(self.w), P(self.vis), len(self.items)), lib)
+
+    def roofline(self, workload_key):
+        """Algorithmic bytes of one march launch: the reference-shaped count
+        (the same marches on a plain binary SAH tree, 16 bins, <= 4
+        triangles per leaf, a radius-2 query per step: node visits x 32 B +
+        triangle tests x 36 B) plus the per-item I/O, over the average
+        HIP-event duration of the timed launches."""
+        lib = self.lib
+        io_bytes = len(self.items) * (4 + 24 + 4 + 4)  # item, sample pos+nrm, weight, shadow factor
+        refc = (C.c_uint64 * 5)()
+        _lib.check(lib.lgx_ReferenceCounts(self.dev, P(self.tris), len(self.tris), refc), lib)
+        rvis, rtri, rq, rbad, rm = (int(x) for x in refc)
+        alg = rvis * 32 + rtri * 36 + io_bytes
+        achieved = alg / (self.march_avg / 1000.0) / 1e9
+        own = self.nodes * NODE_BYTES + self.ntests * DIST_TRI_BYTES + self.npre * TRI_BOX_BYTES + io_bytes
+        log(f"[bench] {self.spec.name} reference-shaped counts: {rvis} node visits, {rtri} triangle tests, {rq} "
+            f"queries over {rm} marches ({rbad} shadow factors differ from the product's): "
+            f"{achieved:.0f} GB/s, {achieved / HBM_PEAK_GBS:.3f} of HBM peak")
+        self.factors_differing = rbad
+        return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic("k_march", dict(workload=workload_key, triangles=self.ntri,
+                                                             work_items=int(len(self.items)),
+                                                             marches=int(self.marches))),
+                    kernel="k_march", alg_bytes_per_launch=int(alg), kernel_ms=round(self.march_avg, 3),
+                    note="reference-shaped count: the same marches on a plain binary SAH tree (16 bins, <= 4 "
+                         "triangles per leaf) with one radius-2 distance query per step, node visits x 32 B + "
+                         "triangle tests x 36 B, plus 36 B of I/O per item, over the march kernel's average "
+                         "HIP-event duration; traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE",
+                    reference_counts=dict(node_visits=rvis, triangle_tests=rtri, distance_queries=rq, marches=rm,
+                                          factors_differing=rbad),
+                    this_build=dict(node_visits=self.nodes, triangle_tests=self.ntests, box_pretests=self.npre,
+                                    distance_queries=self.qsteps, bytes_per_launch=int(own),
+                                    achieved_gbs=round(own / (self.march_avg / 1000.0) / 1e9, 1),
+                                    note="4-wide nodes x 112 B + full triangle tests x 160 B + box pre-tests x "
+                                         "48 B + I/O on this build's own tree and query bounds"))
+
+    def cpu(self, seconds, cores_info, ao=None):
+        """CPU leg: the reference's CalcInvShadowFactor on a seeded sample of
+        the marched pairs (all of them when they fit in `seconds`), and the
+        bit-level agreement of the sampled shadow factors."""
+        cpu, samp = cpu_baseline(self.spec, self.pos, self.nrm, self.items, self.w, self.lights, seconds,
+                                 self.nth, ao=ao)
+        parity = None
+        if cpu is not None:
+            cpu.update(cpu_model=cores_info.get("cpu_model"), affinity_cpus=cores_info.get("affinity_cpus"))
+            sel, refvis = samp
+            parity = dict(sampled=int(len(sel)),
+                          bitexact_fraction=round(float(np.mean(self.vis[sel].view(np.uint32)
+                                                                == refvis.view(np.uint32))), 6),
+                          shadowed_agreement=round(float(np.mean((self.vis[sel] == 0) == (refvis == 0))), 6))
+            log(f"[bench] {self.spec.name} reference march {cpu['value']:.0f} rays/s on {cpu['cores']} threads; "
+                f"GPU/CPU {self.value / cpu['value']:.0f}x; sampled parity {parity}")
+        return cpu, parity
+
+    def close(self):
+        if self.dev:
+            self.lib.lgx_DeviceDestroy(self.dev)
+            self.dev = None
 
 
-def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=65536, bounces=4):
-    """C4, full bake at scale: 2M-triangle city block, 4096^2 atlas, 64 lights:
-    direct-light shadow rays/s (with the reference's CPU march on a sample),
-    64 AO rays per texel, 65,536-patch radiosity in the MFMA forms with
-    refinement, and the gradient with respect to the 64 light colours."""
-    t0 = time.perf_counter()
-    spec = scenes.city_scene()
-    log(f"[bench] C4 scene: {spec.num_triangles} tris, {len(spec.lights)} lights ({time.perf_counter() - t0:.1f} s)")
-    t0 = time.perf_counter()
-    radinfo = []
-    built, pos, nrm, tris, items, light_off = prepare_with_progress(lib, spec, nth, radinfo=radinfo)
-    radinfo = np.concatenate(radinfo)
-    host_s = time.perf_counter() - t0
-    S = len(pos)
-    log(f"[bench] C4 host stages: {S} samples, {len(items)} work items ({host_s:.1f} s)")
-    t0 = time.perf_counter()
-    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
-    if not dev:
-        raise RuntimeError(_lib.last_error(lib))
-    device_create_s = time.perf_counter() - t0
-    log(f"[bench] C4 device scene (BVH build, triangle forms, upload): {device_create_s:.2f} s")
-    info = (C.c_int64 * 3)()
-    lib.lgx_DeviceInfo(dev, info)
-    _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), S, None), lib)
-    lights = make_lights(spec)
-    _lib.check(lib.lgx_DirectLightSetup(dev, lights, len(lights), items.ctypes.data_as(u32p), len(items),
-                                        light_off.ctypes.data_as(u32p), 1), lib)
-    lib.lgx_SetCounting(dev, 1)
-    _lib.check(lib.lgx_DirectLightRun(dev), lib)
-    cnt = (C.c_uint64 * 6)()
-    lib.lgx_ReadCounters(dev, cnt)
-    nodes, ntris, qsteps, queries, marches, npre = list(cnt)
-    lib.lgx_SetCounting(dev, 0)
-    tm = (C.c_double * 6)()
-    for _ in range(warmup):
-        _lib.check(lib.lgx_DirectLightRun(dev), lib)
-    march_ms = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        _lib.check(lib.lgx_DirectLightRun(dev), lib)
-        lib.lgx_DeviceTimes(dev, tm)
-        march_ms.append(tm[1])
-    ms_step = (time.perf_counter() - t0) * 1000.0 / steps
-    rays = marches / (ms_step / 1000.0)
-    march_avg = float(np.mean(march_ms))
-    refc = (C.c_uint64 * 5)()
-    _lib.check(lib.lgx_ReferenceCounts(dev, P(tris), len(tris), refc), lib)
-    rvis, rtri, rq, rbad, rm = (int(x) for x in refc)
-    alg = rvis * 32 + rtri * 36 + len(items) * 36
-    log(f"[bench] C4 direct: BVH {info[1]} nodes depth {info[2]}; {marches} marches, {ms_step:.1f} ms/step "
-        f"(march {march_avg:.1f} ms): {rays / 1e6:.1f} M rays/s; reference-shaped counts {rvis} node visits, "
-        f"{rtri} triangle tests, {rq} queries ({rbad} factors differ)")
-    w = np.zeros(len(items), np.float32)
-    vis = np.zeros(len(items), np.float32)
-    _lib.check(lib.lgx_ReadWeights(dev, P(w), P(vis), len(items)), lib)
-    out = dict(workload=f"C4 city block: {spec.num_triangles} tris (1 km heightfield + 155,000 lognormal boxes in "
-                        f"16 x 16 tiles), 4096^2 atlas ({S} texels), 64 point/spot lights",
-               triangles=int(info[0]), samples=S, work_items=int(len(items)), host_stages_s=round(host_s, 1),
-               device_create_s=round(device_create_s, 2),
-               direct=dict(shadow_rays_per_s=round(rays, 1), marches=int(marches), ms_per_step=round(ms_step, 3),
-                           march_kernel_ms=round(march_avg, 3),
-                           roofline=dict(bound="hbm", achieved=round(alg / (march_avg / 1e3) / 1e9, 1),
-                                         peak=HBM_PEAK_GBS, unit="GB/s",
-                                         frac=round(alg / (march_avg / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                                         reference_counts=dict(node_visits=rvis, triangle_tests=rtri,
-                                                               distance_queries=rq, marches=rm,
-                                                               factors_differing=rbad))))
-    # ambient occlusion: 64 rays per texel, 2 m
+def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
+    """The rest of C4 on its scene: 64 AO rays of 2 m per texel, the colour
+    gradient over every work item, 65,536-patch radiosity in every form,
+    the lightmap export and the sample placement beside their host forms."""
+    S, dev, tris, items = leg.S, leg.dev, leg.tris, leg.items
+    out = {}
     rng = np.random.default_rng(0x4C540004)
     ro = rng.uniform(0, 1, S).astype(np.float32)
     col = np.zeros(3, np.float32)
+    tm = (C.c_double * 6)()
     _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
     _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
     lib.lgx_DeviceTimes(dev, tm)
     out["ao"] = dict(rays=S * 64, ms=round(tm[2], 3), rays_per_s=round(S * 64 / (tm[2] / 1e3), 1),
                      roofline=ao_roofline(lib, dev, ro, S, 64, 2.0, tm[2], tris))
-    rc = out['ao']['roofline']['reference_counts']
-    log(f"[bench] C4 AO: {S * 64} rays in {tm[2]:.1f} ms; {out['ao']['roofline']['achieved']} GB/s of "
-        f"reference-shaped bytes ({rc['node_visits']} node visits, {rc['triangle_tests']} triangle tests; "
-        f"{rc['factors_differing']} of {rc['texels']} factors differ from the product's)")
+    rc = out["ao"]["roofline"]["reference_counts"]
+    log(f"[bench] C4 AO: {S * 64} rays in {tm[2]:.1f} ms ({out['ao']['rays_per_s'] / 1e9:.2f} G rays/s); "
+        f"{rc['factors_differing']} of {rc['texels']} factors differ from the canonical traversal's")
     # colour gradient of L = 1/2 |direct - target|^2 over every texel (64 lights)
     lm = np.zeros((S, 3), np.float32)
     lib.lgx_ReadLightmap(dev, P(lm))
     lam = np.ascontiguousarray(lm.astype(np.float64) * rng.uniform(-0.3, 0.3, (S, 1)))
-    dcol = np.zeros((len(lights), 3))
+    dcol = np.zeros((len(leg.lights), 3))
     dp = C.POINTER(C.c_double)
-    _lib.check(lib.lgx_DirectLightSetup(dev, lights, len(lights), items.ctypes.data_as(u32p), len(items),
-                                        light_off.ctypes.data_as(u32p), 0), lib)
+    leg.setup(keep_vis=0)
     _lib.check(lib.lgx_DirectLightRun(dev), lib)
     _lib.check(lib.lgx_DirectLightGrad(dev, lam.ctypes.data_as(dp), dcol.ctypes.data_as(dp)), lib)
     lib.lgx_DeviceTimes(dev, tm)
-    out["colour_gradient"] = dict(ms=round(tm[5], 4), items=int(len(items)),
-                                  weights_bytes=int(len(items) * 4))
+    out["colour_gradient"] = dict(ms=round(tm[5], 4), items=int(len(items)), weights_bytes=int(len(items) * 4))
     log(f"[bench] C4 colour gradient over {len(items)} items: {tm[5]:.3f} ms")
     # radiosity on the first n_patches texels, 192 right-hand sides from the lights' weights
-    from oracle.checkers import normalize_f32
     N = min(n_patches, S)
-    pn = np.array([normalize_f32(v) for v in nrm[:N]], np.float32)
-    ppos = np.ascontiguousarray(pos[:N])
+    pn = scenes.normalize_rows_f32(leg.nrm[:N])
+    ppos = np.ascontiguousarray(leg.pos[:N])
     rad_dev = lib.lgx_DeviceCreate(P(tris), len(tris))
-    _lib.check(lib.lgx_SetSamples(rad_dev, P(ppos), P(np.ascontiguousarray(nrm[:N])), N, None), lib)
+    _lib.check(lib.lgx_SetSamples(rad_dev, P(ppos), P(np.ascontiguousarray(leg.nrm[:N])), N, None), lib)
     nl = C.c_size_t(0)
     lib.lgx_SetCounting(rad_dev, 1)  # keeps the candidate pairs for the reference-shaped pass
     _lib.check(lib.lgx_RadLinks(rad_dev, P(ppos), P(pn), N, C.byref(nl)), lib)
@@ -816,16 +803,16 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
     link_refc = link_reference_counts(lib, rad_dev, tris)
     lid = (items >> 26).astype(np.int64)
     smp = (items & ((1 << 26) - 1)).astype(np.int64)
-    wv = np.where(w.view(np.uint32) == 0xFFFFFFFF, 0.0, w.astype(np.float64))
+    wv = np.where(leg.w.view(np.uint32) == 0xFFFFFFFF, 0.0, leg.w.astype(np.float64))
     sel = smp < N
-    Wp = np.zeros((N, len(lights)))
+    Wp = np.zeros((N, len(leg.lights)))
     np.add.at(Wp, (smp[sel], lid[sel]), wv[sel])
-    cols = rng.uniform(0.5, 1.0, (64, len(lights), 3))
+    cols = rng.uniform(0.5, 1.0, (64, len(leg.lights), 3))
     # the lights' direct contributions plus a small uniform emission, so that
     # no column is empty where the lights do not reach the first patches
-    E = np.ascontiguousarray(np.concatenate([Wp @ cols[s] for s in range(64)], 1) + 0.05)
+    E = np.ascontiguousarray(np.concatenate([Wp @ cols[k] for k in range(64)], 1) + 0.05)
     diff = np.full((N, 3), 0.6, np.float32)
-    area = np.ascontiguousarray(radinfo[:N, 3])  # world area per texel, as the reference's patches carry
+    area = np.ascontiguousarray(leg.radinfo[:N, 3])  # world area per texel, as the reference's patches carry
     R = E.shape[1]
 
     def run(form, refine=0):
@@ -841,51 +828,45 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
         o, t = (ref64, t64) if form == 0 else run(form, refine)
         key = FORMS[form] + ("_refined" if refine else "")
         per = t[2] if not refine else (t[1] + t[3]) / bounces
-        forms[key] = dict(s_per_bounce=per / 1e3, rel_l2_err_vs_fp64=float(np.linalg.norm(o - ref64) / np.linalg.norm(ref64)))
+        forms[key] = dict(s_per_bounce=per / 1e3,
+                          rel_l2_err_vs_fp64=float(np.linalg.norm(o - ref64) / np.linalg.norm(ref64)))
         log(f"[bench] C4 radiosity {key}: {per:.3f} ms/bounce, err {forms[key]['rel_l2_err_vs_fp64']:.2e}")
     tiles = tile_fraction(lib, rad_dev)
-    out["radiosity"] = dict(patches=N, rhs=R, bounces=bounces, links=int(nl.value), link_generation_ms=round(link_ms, 1),
-                            link_reference_counts=link_refc,
-                            stored_tile_fraction=round(tiles, 4), forms=forms)
+    rad = dict(patches=N, rhs=R, bounces=bounces, links=int(nl.value), link_generation_ms=round(link_ms, 1),
+               link_reference_counts=link_refc, stored_tile_fraction=round(tiles, 4), forms=forms)
     sp = forms["sparse_f32"]["s_per_bounce"]
     best = min((v["s_per_bounce"], k) for k, v in forms.items()
                if "mfma" in k and v["rel_l2_err_vs_fp64"] <= 1e-4)
-    out["radiosity"]["mfma_at_1e-4"] = dict(form=best[1], s_per_bounce=best[0],
-                                            speedup_vs_sparse_f32_gather=round(sp / best[0], 2),
-                                            speedup_vs_dense_f32=round(forms["dense_f32"]["s_per_bounce"] / best[0], 2))
-    log(f"[bench] C4 radiosity: {tiles:.4f} of the 256 x 32 tiles stored; best MFMA form at <= 1e-4: {best[1]}, "
-        f"{sp / best[0]:.2f}x the sparse FP32 gather")
+    rad["mfma_at_1e-4"] = dict(form=best[1], s_per_bounce=best[0],
+                               speedup_vs_sparse_f32_gather=round(sp / best[0], 2),
+                               speedup_vs_dense_f32=round(forms["dense_f32"]["s_per_bounce"] / best[0], 2))
+    log(f"[bench] C4 radiosity: {tiles:.4f} of the 256 x 32 tiles stored; best MFMA form at <= 1e-4: {best[1]}")
     if with_cpu:
-        # the reference's FP32 bounce loop (Int_RDBounce, restated in the
-        # oracle) on the same links for one RGB setup, one thread as it runs
-        try:
-            from oracle.checkers import Oracle
-            orc = Oracle()
-            rowmap, other, fac = read_links(lib, rad_dev, N, nl.value)
-            c13 = np.concatenate([diff, E[:, :3], E[:, :3], np.zeros((N, 3)), area[:, None]], 1).astype(np.float32)
-            t0 = time.perf_counter()
-            orc.rad_bounce_f32(rowmap, other, fac, c13)
-            cpu_b = time.perf_counter() - t0
+        rowmap, other, fac = read_links(lib, rad_dev, N, nl.value)
+        cpu_b = reference_bounce_seconds(rowmap, other, fac, diff, E, area, N)
+        if cpu_b:
             sets = R / 3.0
-            out["radiosity"]["cpu_port_s_per_bounce_rgb"] = cpu_b
-            out["radiosity"]["mfma_vs_cpu_bounce"] = dict(
+            rad["cpu_port_s_per_bounce_rgb"] = cpu_b
+            rad["mfma_vs_cpu_bounce"] = dict(
                 note=f"GPU seconds per bounce for all {R} right-hand sides ({sets:.0f} RGB setups) against the "
                      "reference loop's seconds per bounce for one RGB setup times the number of setups",
                 speedup=round(cpu_b * sets / best[0], 1))
-            log(f"[bench] C4 reference bounce loop: {cpu_b * 1e3:.1f} ms per RGB bounce; "
-                f"{best[1]} {cpu_b * sets / best[0]:.0f}x for {sets:.0f} setups")
-        except Exception as e:  # the checker is optional on a bare install
-            log(f"[bench] C4 bounce baseline skipped: {e}")
+    out["radiosity"] = rad
     lib.lgx_DeviceDestroy(rad_dev)
     if with_cpu:
+        # the reference's AO ray loop on a sample of texels, and the device's
+        # AO over a lightmap of ones (multiplier 1, no falloff, effect 0,
+        # black), which leaves 1 - factor in every channel
         ao_ref = {}
-        cpu, samp = cpu_baseline(spec, pos, nrm, items, w, lights, cpu_seconds, nth,
-                                 ao=(ro, 64, 2.0, cpu_seconds, ao_ref))
-        out["cpu_baseline"] = cpu
+        from oracle.checkers import Reference, reference_available
+        if reference_available():
+            ref = Reference()
+            br = scenes.BuiltScene(ref.lib, leg.spec, dict(max_num_threads=nth))
+            ref.lib.rp_prepare_collision(br.scene, nth)
+            ao_baseline(ref, br, leg.pos, leg.nrm, nth, ro, 64, 2.0, cpu_seconds, ao_ref)
+            br.destroy()
         if ao_ref:
-            # the device's AO over a lightmap of ones (multiplier 1, no
-            # falloff, effect 0, black) leaves 1 - factor in every channel
-            _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), S, P(np.ones((S, 3), np.float32))), lib)
+            _lib.check(lib.lgx_SetSamples(dev, P(leg.pos), P(leg.nrm), S, P(np.ones((S, 3), np.float32))), lib)
             _lib.check(lib.lgx_AmbientOcclusion(dev, P(ro), 64, 2.0, 1.0, 0.0, 0.0, P(col)), lib)
             lm1 = np.zeros((S, 3), np.float32)
             lib.lgx_ReadLightmap(dev, P(lm1))
@@ -896,23 +877,20 @@ def c4_bench(lib, nth, steps, warmup, cpu_seconds, with_cpu=True, n_patches=6553
             out["ao"]["parity_sample"] = dict(texels=int(len(got)),
                                               bitexact_fraction=float(np.mean(got.view(np.uint32)
                                                                               == exp.view(np.uint32))),
+                                              within_1_255=float(np.mean(np.abs(got - exp) <= 1.0 / 255)),
                                               max_abs_diff=float(np.max(np.abs(got - exp))))
-            log(f"[bench] C4 AO cpu baseline {ao_ref['baseline']['value'] / 1e6:.2f} M rays/s; GPU/CPU "
+            log(f"[bench] C4 AO reference {ao_ref['baseline']['value'] / 1e6:.2f} M rays/s; GPU/CPU "
                 f"{out['ao']['speedup_vs_cpu']}x; parity {out['ao']['parity_sample']}")
-        if cpu:
-            out["direct"]["speedup_vs_cpu"] = round(rays / cpu["value"], 1)
-            sel_, refvis = samp
-            out["parity_sample"] = dict(sampled=int(len(sel_)),
-                                        bitexact_fraction=float(np.mean(vis[sel_].view(np.uint32)
-                                                                        == refvis.view(np.uint32))),
-                                        shadowed_agreement=float(np.mean((vis[sel_] == 0) == (refvis == 0))))
-            log(f"[bench] C4 cpu baseline {cpu['value']:.0f} rays/s; GPU/CPU {rays / cpu['value']:.0f}x; "
-                f"parity {out['parity_sample']}")
-    lib.lgx_DeviceDestroy(dev)
-    out["lightmap_export"] = export_vs_host(lib, built)
-    built.destroy()
-    out["sample_placement"] = placement_vs_host(lib, spec, nth, built, pos)
+    out["lightmap_export"] = export_vs_host(lib, leg.built)
+    out["sample_placement"] = placement_vs_host(lib, leg.spec, nth, leg.built, leg.pos)
     return out
+
+
+C4_WORKLOAD = ("C4 city block direct light: 1 km heightfield (131,072 tris) + 155,000 lognormal(1, 0.6) m "
+               "boxes (1,860,000 tris) = 1,991,072 tris in 16 x 16 tile instances, 4096^2 texels in total "
+               "(256^2 per tile), 64 point/spot lights with 16 soft-shadow samples each")
+C1_WORKLOAD = ("C1 terrain direct light: 512x512 heightfield + 2000 boxes (548,288 tris) in 16x16 tiles, "
+               "2048x2048 texels, 32 point/spot + sun (range 300 m)")
 
 
 def measured_traffic(kernel, workload):
@@ -923,27 +901,27 @@ def measured_traffic(kernel, workload):
     try:
         with open(path) as f:
             t = json.load(f)
-        k = t[kernel]
-        if k["workload"] != workload:
-            return None
-        return int(round((k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"]) * 1024))
-    except (OSError, KeyError, ValueError):
-        return None
+        for k in t[kernel]:
+            if k["workload"] == workload:
+                return int(round((k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"]) * 1024))
+    except (OSError, KeyError, ValueError, TypeError):
+        pass
+    return None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--small", action="store_true", help="reduced terrain for a quick check")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--headline-only", action="store_true", help="only the C4 direct-light line (no extra legs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c4-extras", action="store_true", help="skip C4's AO, radiosity, gradient, export legs")
+    ap.add_argument("--no-c1", action="store_true", help="skip the C1 terrain leg")
     ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity (C2) leg")
     ap.add_argument("--radiosity-patches", type=int, default=32768)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (C3) leg")
-    ap.add_argument("--c4", action="store_true",
-                    help="only the 2M-triangle full-bake configuration (several minutes of host stages)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (scales the HBM byte counters)")
     args = ap.parse_args()
@@ -960,133 +938,59 @@ def main():
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path "
               "is a design only and is not run.", file=sys.stderr)
         sys.exit(2)
+    if args.headline_only:
+        args.no_c4_extras = args.no_c1 = args.no_radiosity = args.no_adjoint = True
+    with_cpu = not args.no_cpu_baseline
 
     lib = _lib.load()
     _lib.check(lib.lgx_HasDevice(), lib)
-    nth = host_threads()
-    if args.c4:
-        out = c4_bench(lib, nth, args.steps, args.warmup, args.cpu_seconds, with_cpu=not args.no_cpu_baseline)
-        print(json.dumps(dict(metric=METRIC, config="C4", n_gpus=1, c4=out)), flush=True)
-        return
-    t0 = time.perf_counter()
-    if args.small:
-        spec = scenes.terrain_scene(grid=128, extent=50.0, n_boxes=100, tiles=4, lm_total=512, n_lights=8,
-                                    name="C1-small")
-    else:
-        spec = scenes.terrain_scene()
-    log(f"[bench] scene {spec.name}: {spec.num_triangles} tris, {len(spec.lights)} lights "
-        f"({time.perf_counter() - t0:.1f} s)")
-    t0 = time.perf_counter()
-    built, pos, nrm, tris, items, light_off = prepare_workload(lib, spec, nth)
-    log(f"[bench] host stages: {len(pos)} samples, {len(items)} work items ({time.perf_counter() - t0:.1f} s)")
+    nth, cores = host_cores()
+    log(f"[bench] host: {cores}")
 
+    # headline: C4 direct light
     t0 = time.perf_counter()
-    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
-    if not dev:
-        raise RuntimeError(_lib.last_error(lib))
-    device_create_s = time.perf_counter() - t0
-    info = (C.c_int64 * 3)()
-    lib.lgx_DeviceInfo(dev, info)
-    log(f"[bench] device scene (BVH build, triangle forms, upload): {device_create_s:.2f} s")
-    _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), len(pos), None), lib)
-    lights = make_lights(spec)
-    _lib.check(lib.lgx_DirectLightSetup(dev, lights, len(lights), items.ctypes.data_as(u32p), len(items),
-                                        light_off.ctypes.data_as(u32p), 1), lib)
-    # instrumented pass: traversal counters for the algorithmic byte count
-    lib.lgx_SetCounting(dev, 1)
-    _lib.check(lib.lgx_DirectLightRun(dev), lib)
-    cnt = (C.c_uint64 * 6)()
-    lib.lgx_ReadCounters(dev, cnt)
-    nodes, ntris, steps, queries, marches, npre = list(cnt)
-    lib.lgx_SetCounting(dev, 0)
-    log(f"[bench] BVH {info[0]} tris, {info[1]} nodes, depth {info[2]}; marches {marches}, steps {steps}, "
-        f"node visits {nodes}, triangle tests {ntris} (+{npre} culled by the box pre-test)")
+    spec = scenes.city_scene()
+    log(f"[bench] C4 scene: {spec.num_triangles} tris, {len(spec.lights)} lights ({time.perf_counter() - t0:.1f} s)")
+    leg = DirectLeg(lib, spec, nth, radinfo=not args.no_c4_extras)
+    leg.run(args.steps, args.warmup)
+    roofline = leg.roofline("C4")
+    cpu, parity = leg.cpu(args.cpu_seconds, cores) if with_cpu else (None, None)
+    extras = {}
+    if not args.no_c4_extras:
+        extras["c4"] = c4_extras(lib, leg, nth, args.cpu_seconds, with_cpu)
+    head = dict(value=leg.value, ms=leg.ms_per_step, march=leg.march_avg, marches=leg.marches, qsteps=leg.qsteps,
+                S=leg.S, items=len(leg.items), ntri=leg.ntri, host_s=leg.host_s, dev_s=leg.device_create_s)
+    leg.close()
+    leg.built.destroy()
 
-    tm = (C.c_double * 6)()
-    for _ in range(args.warmup):
-        _lib.check(lib.lgx_DirectLightRun(dev), lib)
-    march_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _lib.check(lib.lgx_DirectLightRun(dev), lib)
-        lib.lgx_DeviceTimes(dev, tm)
-        march_ms.append(tm[1])
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1000.0 / args.steps
-    march_avg = float(np.mean(march_ms))
-    value = marches / (ms_per_step / 1000.0)
-    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, march kernel {march_avg:.2f} ms avg")
-
-    # algorithmic bytes of one march-kernel launch: the reference-shaped
-    # count (the same marches on a plain binary SAH tree, 16 bins, <= 4
-    # triangles per leaf, a radius-2 query per step: node visits x 32 B +
-    # triangle tests x 36 B) plus the per-item I/O
-    io_bytes = len(items) * (4 + 24 + 4 + 4)  # item, sample pos+nrm, weight, shadow factor
-    refc = (C.c_uint64 * 5)()
-    _lib.check(lib.lgx_ReferenceCounts(dev, P(tris), len(tris), refc), lib)
-    rvis, rtri, rq, rbad, rm = (int(x) for x in refc)
-    alg_bytes = rvis * 32 + rtri * 36 + io_bytes
-    achieved = alg_bytes / (march_avg / 1000.0) / 1e9
-    own_bytes = nodes * NODE_BYTES + ntris * DIST_TRI_BYTES + npre * TRI_BOX_BYTES + io_bytes
-    log(f"[bench] reference-shaped counts: {rvis} node visits, {rtri} triangle tests, {rq} queries over {rm} "
-        f"marches ({rbad} shadow factors differ from the product's); this build's tree: {nodes} wide-node visits, "
-        f"{ntris} triangle tests, {steps} queries")
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=measured_traffic("k_march", dict(triangles=int(info[0]), work_items=int(len(items)),
-                                                             marches=int(marches), node_visits=int(nodes),
-                                                             triangle_tests=int(ntris))),
-                    kernel="k_march", alg_bytes_per_launch=int(alg_bytes),
-                    note="reference-shaped count: the same marches on a plain binary SAH tree (16 bins, <= 4 "
-                         "triangles per leaf) with one radius-2 distance query per step, node visits x 32 B + "
-                         "triangle tests x 36 B, plus 36 B of I/O per item; HBM traffic per launch from "
-                         "rocprofv3 FETCH_SIZE/WRITE_SIZE",
-                    reference_counts=dict(node_visits=rvis, triangle_tests=rtri, distance_queries=rq, marches=rm,
-                                          factors_differing=rbad),
-                    this_build=dict(node_visits=int(nodes), triangle_tests=int(ntris), box_pretests=int(npre),
-                                    distance_queries=int(steps), bytes_per_launch=int(own_bytes),
-                                    achieved_gbs=round(own_bytes / (march_avg / 1000.0) / 1e9, 1),
-                                    note="4-wide nodes x 112 B + full triangle tests x 160 B + box pre-tests x "
-                                         "48 B + I/O on this build's own tree and query bounds"))
-
-    w = np.zeros(len(items), np.float32)
-    vis = np.zeros(len(items), np.float32)
-    _lib.check(lib.lgx_ReadWeights(dev, P(w), P(vis), len(items)), lib)
-    cpu = None
-    parity = None
-    if not args.no_cpu_baseline:
-        cpu, samp = cpu_baseline(spec, pos, nrm, items, w, lights, args.cpu_seconds, nth)
-        if samp is not None:
-            sel, refvis = samp
-            agree = float(np.mean(vis[sel].view(np.uint32) == refvis.view(np.uint32)))
-            hitmiss = float(np.mean((vis[sel] == 0) == (refvis == 0)))
-            parity = dict(sampled=int(len(sel)), bitexact_fraction=round(agree, 6),
-                          shadowed_agreement=round(hitmiss, 6))
-            log(f"[bench] cpu baseline {cpu['value']:.0f} rays/s; sampled parity {parity}")
-    lib.lgx_DeviceDestroy(dev)
-    export = export_vs_host(lib, built)
-    built.destroy()
-    placement = placement_vs_host(lib, spec, nth, built, pos)
-    radio = None
+    if not args.no_c1:
+        s1 = scenes.terrain_scene()
+        c1 = DirectLeg(lib, s1, nth)
+        c1.run(args.steps, args.warmup)
+        c1_roof = c1.roofline("C1")
+        c1_cpu, c1_par = c1.cpu(min(args.cpu_seconds, 5.0), cores) if with_cpu else (None, None)
+        extras["c1"] = dict(workload=C1_WORKLOAD, shadow_rays_per_s=round(c1.value, 1),
+                            ms_per_step=round(c1.ms_per_step, 3), marches=int(c1.marches),
+                            march_kernel_ms=round(c1.march_avg, 3), roofline=c1_roof, cpu_baseline=c1_cpu,
+                            parity_sample=c1_par,
+                            speedup_vs_cpu=round(c1.value / c1_cpu["value"], 1) if c1_cpu else None)
+        c1.close()
+        c1.built.destroy()
     if not args.no_radiosity:
-        radio = radiosity_bench(lib, nth, n_patches=args.radiosity_patches, with_cpu=not args.no_cpu_baseline)
-    adjoint = None
+        extras["radiosity"] = radiosity_bench(lib, nth, n_patches=args.radiosity_patches, with_cpu=with_cpu)
     if not args.no_adjoint:
-        adjoint = adjoint_bench(lib, nth)
+        extras["adjoint"] = adjoint_bench(lib, nth)
 
-    out = dict(metric=METRIC, value=round(value, 1), unit="shadow rays/s", n_gpus=1, steps=args.steps,
-               warmup=args.warmup, ms_per_step=round(ms_per_step, 3), higher_is_better=True, scaling="weak",
+    out = dict(metric=METRIC, value=round(head["value"], 1), unit="shadow rays/s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=round(head["ms"], 3), higher_is_better=True, scaling="weak",
                vs_baseline=None, dtype="f32", data="synthetic",
-               config=dict(workload=("C1 terrain direct light: 512x512 heightfield + 2000 boxes "
-                                     "(548,288 tris) in 16x16 tiles, 2048x2048 texels, 32 point/spot + sun "
-                                     "(range 300 m)") if not args.small else "C1-small",
-                           triangles=int(info[0]), samples=int(len(pos)), lights=len(spec.lights),
-                           work_items=int(len(items)), shadow_rays_per_step=int(marches),
-                           march_steps_per_step=int(steps), march_kernel_ms=round(march_avg, 3),
-                           host_stages_s=round(built.prepare_s, 2), device_create_s=round(device_create_s, 2)),
-               roofline=roofline, cpu_baseline=cpu, parity_sample=parity, sample_placement=placement,
-               lightmap_export=export,
-               radiosity=radio, adjoint=adjoint)
+               config=dict(workload=C4_WORKLOAD, triangles=head["ntri"], samples=head["S"], lights=len(spec.lights),
+                           work_items=head["items"], shadow_rays_per_step=head["marches"],
+                           march_steps_per_step=head["qsteps"], march_kernel_ms=round(head["march"], 3),
+                           host_stages_s=round(head["host_s"], 2), device_create_s=round(head["dev_s"], 2)),
+               roofline=roofline, cpu_baseline=cpu, parity_sample=parity,
+               speedup_vs_cpu=round(head["value"] / cpu["value"], 1) if cpu else None, host=cores)
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 

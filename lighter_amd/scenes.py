@@ -74,6 +74,19 @@ class SceneSpec:
 IDENTITY = np.eye(4, dtype=np.float32)
 
 
+def normalize_rows_f32(v):
+    """Vec3::Normalized (lighter_int.hpp:404-415) of each row in float32:
+    v * (1 / sqrt((x x + y y) + z z)), zero rows stay zero.  Int_RDGenLinks
+    normalises the patch normals this way (lighter.cpp:683)."""
+    v = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+    l2 = (v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2]
+    out = np.zeros_like(v)
+    nz = l2 != 0
+    inv = np.float32(1.0) / np.sqrt(l2[nz])
+    out[nz] = v[nz] * inv[:, None]
+    return out
+
+
 # --------------------------------------------------------------------------
 # charts and atlas packing
 # --------------------------------------------------------------------------

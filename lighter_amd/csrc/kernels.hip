@@ -10,6 +10,8 @@
 //  * occluded(A,B): some triangle hit with r < 1 on the same shrunk segment
 //    (lighter.cpp:112-147).
 #include <hip/hip_runtime.h>
+#include <mutex>
+#include <stdlib.h>
 
 #include <math.h>
 #include <stdio.h>
@@ -1380,8 +1382,57 @@ __global__ void k_scan_add(uint64_t* out, uint64_t n, const uint64_t* sums)
 // ===========================================================================
 static inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// The HIP runtime draws from (or reseeds) the C library's random generator
+// while it initialises (the stream, memory and code-object set-up behind the
+// process's first device scene).  A bake replays that generator's sequence
+// for the AO rotation offsets, one draw per texel as the reference's randf()
+// makes them (lighter.cpp:819, lighter_int.hpp:271), so the runtime is
+// initialised once per process, on the first GPU request, with a private
+// generator state switched in; the caller's sequence is left where it was.
+namespace {
+struct PrivateRandState {
+	char buf[128];
+	char* saved;
+	PrivateRandState() { saved = initstate(0x4C54u, buf, sizeof(buf)); }
+	~PrivateRandState() { setstate(saved); }
+};
+} // namespace
+
+__global__ void k_warm(uint32_t* p)
+{
+	p[threadIdx.x] = threadIdx.x;
+}
+
+// A stream, device and pinned memory and one kernel launch: the runtime's
+// one-time set-up, best effort (errors surface again in the calls that need
+// the runtime).
+static void init_runtime_once()
+{
+	hipStream_t s = nullptr;
+	if (hipStreamCreate(&s) != hipSuccess)
+		return;
+	void* d = nullptr;
+	if (hipMalloc(&d, 256) == hipSuccess) {
+		k_warm<<<1, 64, 0, s>>>((uint32_t*)d);
+		(void)hipStreamSynchronize(s);
+		(void)hipFree(d);
+	}
+	void* h = nullptr;
+	if (hipHostMalloc(&h, 256, hipHostMallocDefault) == hipSuccess)
+		(void)hipHostFree(h);
+	(void)hipStreamDestroy(s);
+}
+
 void require_gpu()
 {
+	static std::once_flag once;
+	std::call_once(once, [] {
+		PrivateRandState private_rand;
+		int n = 0;
+		if (hipGetDeviceCount(&n) == hipSuccess && n > 0)
+			init_runtime_once();
+		(void)hipGetLastError();
+	});
 	int n = 0;
 	hipError_t e = hipGetDeviceCount(&n);
 	if (e != hipSuccess || n == 0)

@@ -135,6 +135,22 @@ LTRAPI int lgx_DirectLight(lgx_Device* dev, const lgx_Light* lights, int nlights
 LTRAPI int lgx_DirectLightSetup(lgx_Device* dev, const lgx_Light* lights, int nlights, const uint32_t* items,
                                 size_t nitems, const uint32_t* light_off, int keep_vis);
 LTRAPI int lgx_DirectLightRun(lgx_Device* dev);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The pass of lgx_DirectLight (Job_LMRender_*_Inner,
+ * lighter.cpp:485-600) with no device allocation: the work items (d_items,
+ * (light << 26) | sample grouped by light), the per-item weights
+ * (d_weights) and shadow factors (d_vis, or NULL) are device buffers of the
+ * caller on the device's GPU; lights and light_off (nlights + 1 entries) are
+ * host arrays.  The lights' contributions are added to the device's
+ * lightmap (lgx_SetSamples) in light order, as lgx_DirectLight adds them,
+ * and the weights and factors are those lgx_ReadWeights would return.  The
+ * kernels' scratch (per-item shading factors, the march list, counters) is
+ * a file-scope device array for 4,194,304 work items (32 MiB + 4 KiB), one
+ * per GPU for the life of the process: more items are refused, and calls
+ * that overlap on one GPU race on it.  Synchronous. */
+LTRAPI int lgx_DirectLightNoAlloc(lgx_Device* dev, const lgx_Light* lights, int nlights, const uint32_t* d_items,
+                                  size_t nitems, const uint32_t* light_off, float* d_weights, float* d_vis);
 /* Per item: weight multiplying the light colour (bits 0xFFFFFFFF where the
  * reference skips the texel) and, if kept, the shadow factor. */
 LTRAPI int lgx_ReadWeights(lgx_Device* dev, float* weights, float* vis, size_t n);

@@ -58,6 +58,8 @@ LGX_SIGNATURES = {
     "lgx_DirectLight": (C.c_int, [vp, C.POINTER(LgxLight), C.c_int, u32p, C.c_size_t, u32p, C.c_int]),
     "lgx_DirectLightSetup": (C.c_int, [vp, C.POINTER(LgxLight), C.c_int, u32p, C.c_size_t, u32p, C.c_int]),
     "lgx_DirectLightRun": (C.c_int, [vp]),
+    "lgx_DirectLightNoAlloc": (C.c_int, [vp, C.POINTER(LgxLight), C.c_int, C.c_void_p, C.c_size_t, u32p,
+                                        C.c_void_p, C.c_void_p]),
     "lgx_ReadWeights": (C.c_int, [vp, fp, fp, C.c_size_t]),
     "lgx_AmbientOcclusion": (C.c_int, [vp, fp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp]),
     "lgx_RadLinks": (C.c_int, [vp, fp, fp, C.c_size_t, C.POINTER(C.c_size_t)]),

@@ -622,6 +622,33 @@ int lgx_DirectLightSetup(lgx_Device* d, const lgx_Light* lights, int nlights, co
 	});
 }
 
+int lgx_DirectLightNoAlloc(lgx_Device* d, const lgx_Light* lights, int nlights, const uint32_t* d_items,
+                           size_t nitems, const uint32_t* light_off, float* d_weights, float* d_vis)
+{
+	return guarded([&] {
+		if (nlights < 0 || nlights > 64)
+			throw std::runtime_error("lgx_DirectLightNoAlloc: 0..64 lights per call");
+		if (!light_off || (nitems && (!d_items || !d_weights)) || (nlights && !lights))
+			throw std::runtime_error("lgx_DirectLightNoAlloc: null array");
+		DevLight dl[64];
+		for (int l = 0; l < nlights; ++l) {
+			Light L;
+			L.type = (uint32_t)lights[l].type;
+			L.pos = ld3(lights[l].pos);
+			L.dir = ld3(lights[l].dir);
+			L.color = ld3(lights[l].color);
+			L.range = lights[l].range;
+			L.power = lights[l].power;
+			L.radius = lights[l].radius;
+			L.spot_out = lights[l].spot_angle_out;
+			L.spot_in = lights[l].spot_angle_in;
+			L.spot_curve = lights[l].spot_curve;
+			dl[l] = to_dev_light(L);
+		}
+		d->dev.run_direct_noalloc(dl, nlights, d_items, nitems, light_off, d_weights, d_vis);
+	});
+}
+
 int lgx_ReadWeights(lgx_Device* d, float* w, float* vis, size_t n)
 {
 	return guarded([&] {

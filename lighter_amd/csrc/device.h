@@ -24,6 +24,12 @@ struct DevLight {
 	float spot_curve;
 };
 
+// Work items the allocation-free direct-light pass takes at most: its
+// scratch array holds 8 bytes per item (32 MiB) plus a few KiB.
+constexpr size_t kNoAllocMaxItems = (size_t)1 << 22;
+// GPUs per process whose scratch address is kept.
+constexpr int kMaxDevices = 64;
+
 // Per-stage timing in milliseconds measured with HIP events on the stream
 // the kernels run on.
 struct StageTimes {
@@ -71,6 +77,13 @@ public:
 	void set_direct_work(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
 	                     const std::vector<uint32_t>& light_off, bool keep_weights);
 	void run_direct();
+	// The same pass without device allocations, on the caller's device
+	// buffers (items, weights, optional shadow factors) with host lights and
+	// light offsets; the kernels' scratch is a file-scope device array of
+	// kNoAllocMaxItems items, shared by every call on one GPU (calls that
+	// overlap on one GPU race).  Synchronous.
+	void run_direct_noalloc(const DevLight* lights, int nlights, const uint32_t* d_items, size_t nitems,
+	                        const uint32_t* light_off, float* d_w, float* d_vis);
 	const float* weights_device() const { return d_w_; }
 	void get_weights(float* out, size_t n) const;
 	// reverse mode of the last direct-light run: dcolor3[l] = sum over light
@@ -189,6 +202,7 @@ private:
 	size_t nitems_ = 0;
 	size_t nlinks_ = 0;
 	unsigned long long* d_counters_ = nullptr;
+	void* d_scratch_ = nullptr; // this GPU's allocation-free scratch array
 };
 
 // Throws std::runtime_error unless a gfx950-class GPU is usable.

@@ -1463,6 +1463,67 @@ template <class T> static void dfree(T*& p)
 	p = nullptr;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free direct-light pass (run_direct_noalloc): the
+// buffers run_direct allocates (per-item shading factor, the march list, the
+// march count and list cursor, the traversal counters, the lights and their
+// item ranges) carved out of one file-scope device array.  Each GPU holds
+// its own copy for the life of the process; every call on a GPU uses that
+// copy, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocGateOff = 0;                                          // float per item
+constexpr size_t kNoAllocListOff = kNoAllocGateOff + 4 * kNoAllocMaxItems;      // uint32 per item
+constexpr size_t kNoAllocHeadOff = kNoAllocListOff + 4 * kNoAllocMaxItems;      // count, cursor, bad-item flag
+constexpr size_t kNoAllocCntOff = kNoAllocHeadOff + 256;                        // traversal counters
+constexpr size_t kNoAllocLightOff = kNoAllocCntOff + 256;                       // 64 lights
+constexpr size_t kNoAllocOffOff = kNoAllocLightOff + 64 * sizeof(DevLight);     // 65 item offsets
+constexpr size_t kNoAllocBytes = kNoAllocOffOff + 512;
+static_assert(kNumCounters * 8 <= 256, "counters fit their slot");
+static_assert(kNoAllocLightOff % 256 == 0 && kNoAllocOffOff % 256 == 0, "aligned slots");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on the current GPU, looked up once per device
+// (the array's name is only a host-side stand-in for it).
+static void* noalloc_scratch()
+{
+	static std::mutex mu;
+	static void* addr[kMaxDevices] = {};
+	int dev = 0;
+	HIPCHECK(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices)
+		throw std::runtime_error("device ordinal beyond the supported " + std::to_string(kMaxDevices));
+	std::lock_guard<std::mutex> g(mu);
+	if (!addr[dev]) {
+		PrivateRandState private_rand; // the lookup may set up the device's code object
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+			throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+		addr[dev] = p;
+	}
+	return addr[dev];
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v)
+{
+	for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+		p[i] = v;
+}
+
+// Sets *bad when an item names a light other than its segment's or a sample
+// beyond the device's samples (the kernels index samples by the items).
+__global__ void k_check_items(const uint32_t* __restrict__ items, uint32_t nitems, const uint32_t* __restrict__ off,
+                              uint32_t nlights, uint32_t nsamples, uint32_t* bad)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nitems)
+		return;
+	const uint32_t code = items[i], l = code >> kSampleBits;
+	if (l >= nlights || i < off[l] || i >= off[l + 1] || (code & kSampleMask) >= nsamples)
+		atomicOr(bad, 1u);
+}
+
 DeviceScene::DeviceScene()
 {
 	require_gpu();
@@ -1471,6 +1532,7 @@ DeviceScene::DeviceScene()
 	stream_ = s;
 	d_counters_ = dalloc<unsigned long long>(kNumCounters);
 	d_mcount_ = dalloc<uint32_t>(2);
+	d_scratch_ = noalloc_scratch();
 }
 
 // Blocks of the persistent march kernel: as many as can be resident at once.
@@ -1775,6 +1837,86 @@ void DeviceScene::run_direct()
 	}
 	times.direct_ms = all.stop();
 	read_counters();
+}
+
+// [p, p + bytes) must lie inside one device allocation of the current GPU.
+static void check_device_range(const void* p, size_t bytes, const char* what)
+{
+	hipDeviceptr_t base = nullptr;
+	size_t size = 0;
+	const hipError_t e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p);
+	if (e != hipSuccess)
+		(void)hipGetLastError(); // the failed lookup must not surface as a later launch's error
+	if (e != hipSuccess || !base || (const char*)p + bytes > (const char*)base + size)
+		throw std::runtime_error(std::string("direct_light_noalloc: ") + what +
+		                         " is not a device buffer of the current GPU large enough for the work items");
+}
+
+void DeviceScene::run_direct_noalloc(const DevLight* lights, int nlights, const uint32_t* d_items, size_t nitems,
+                                     const uint32_t* light_off, float* d_w, float* d_vis)
+{
+	if (nitems > kNoAllocMaxItems)
+		throw std::runtime_error("direct_light_noalloc: " + std::to_string(nitems) +
+		                         " work items exceed the scratch's maximum of " + std::to_string(kNoAllocMaxItems));
+	if (nlights < 0 || nlights > 64)
+		throw std::runtime_error("direct_light_noalloc: 0..64 lights per call");
+	if (light_off[0] != 0 || light_off[nlights] != nitems)
+		throw std::runtime_error("direct_light_noalloc: light offsets do not match the work items");
+	for (int l = 0; l < nlights; ++l)
+		if (light_off[l + 1] < light_off[l])
+			throw std::runtime_error("direct_light_noalloc: light offsets must not decrease");
+	if (nitems == 0)
+		return;
+	if (!nsamples_)
+		throw std::runtime_error("direct_light_noalloc: the device holds no samples");
+	check_device_range(d_items, 4 * nitems, "the work-item array");
+	check_device_range(d_w, 4 * nitems, "the weight array");
+	if (d_vis)
+		check_device_range(d_vis, 4 * nitems, "the shadow-factor array");
+	hipStream_t s = (hipStream_t)stream_;
+	char* sc = (char*)d_scratch_;
+	float* gate = (float*)(sc + kNoAllocGateOff);
+	uint32_t* mlist = (uint32_t*)(sc + kNoAllocListOff);
+	uint32_t* head = (uint32_t*)(sc + kNoAllocHeadOff); // [0] march count, [1] list cursor, [2] bad item
+	unsigned long long* cnt = (unsigned long long*)(sc + kNoAllocCntOff);
+	DevLight* dl = (DevLight*)(sc + kNoAllocLightOff);
+	uint32_t* doff = (uint32_t*)(sc + kNoAllocOffOff);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	if (nlights)
+		HIPCHECK(hipMemcpyAsync(dl, lights, sizeof(DevLight) * nlights, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(doff, light_off, 4 * (size_t)(nlights + 1), hipMemcpyHostToDevice, s));
+	// counters and flags start at zero (the array keeps the last call's contents)
+	k_fill_u32<<<1, 64, 0, s>>>(head, 4, 0u);
+	k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)cnt, 2 * kNumCounters, 0u);
+	k_check_items<<<grid_for(nitems), kBlock, 0, s>>>(d_items, (uint32_t)nitems, doff, (uint32_t)nlights,
+	                                                  (uint32_t)nsamples_, head + 2);
+	HIPCHECK(hipGetLastError());
+	uint32_t h[3] = {0, 0, 0};
+	HIPCHECK(hipMemcpyAsync(h, head, sizeof(h), hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	if (h[2])
+		throw std::runtime_error("direct_light_noalloc: a work item names another light or a sample beyond the "
+		                         "device's samples");
+	k_direct_gate<<<(unsigned)((nitems + kGateRounds * kBlock - 1) / (kGateRounds * kBlock)), kBlock, 0, s>>>(
+	    d_pos_, d_nrm_, dl, d_items, (uint32_t)nitems, gate, d_w, d_vis, mlist, head, d_skey_);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(h, head, 4, hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	const uint32_t nmarch = h[0];
+	if (nmarch) {
+		k_march<false><<<persistent_blocks(), kBlock, march_lds_bytes(), s>>>(
+		    V, d_pos_, d_nrm_, dl, d_items, mlist, nmarch, gate, d_w, d_vis, head + 1, cnt, march_stack_depth());
+		HIPCHECK(hipGetLastError());
+	}
+	for (int l = 0; l < nlights; ++l) {
+		const uint32_t b = light_off[l], e = light_off[l + 1];
+		if (e > b) {
+			k_accumulate<<<grid_for(e - b), kBlock, 0, s>>>(d_items, d_w, b, e, lights[l].color[0], lights[l].color[1],
+			                                                lights[l].color[2], d_lm_);
+			HIPCHECK(hipGetLastError());
+		}
+	}
+	HIPCHECK(hipStreamSynchronize(s));
 }
 
 void DeviceScene::reference_counts(const float* tris, size_t ntri, uint64_t out[5])

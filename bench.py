@@ -232,7 +232,48 @@ def ao_baseline(ref, b, pos, nrm, nthreads, ro, nsamples, distance, target_s, ou
     out["texels"], out["factor"] = idx, fac
 
 
-FORMS = {0: "sparse_f64", 1: "sparse_f32", 2: "dense_f32", 3: "dense_bf16_mfma", 4: "dense_f16_mfma"}
+FORMS = {0: "sparse_f64", 1: "sparse_f32", 2: "dense_f32", 3: "tiles_bf16_mfma", 4: "tiles_f16_mfma",
+         5: "tiles_f32_mfma", 6: "tiles_f16x3_mfma", 7: "tiles_bf16x3_mfma"}
+FORM_RUNS = [(0, 0), (1, 0), (2, 0), (5, 0), (3, 0), (3, 1), (4, 0), (4, 1), (6, 0), (7, 0)]
+FP32_FORMS = ("sparse_f32", "dense_f32", "tiles_f32_mfma")
+
+
+FP32_MATRIX_PEAK_TFLOPS = 157.3
+
+
+def tile_counts(lib, dev):
+    out = (C.c_uint64 * 2)()
+    _lib.check(lib.lgx_RadTiles(dev, out), lib)
+    return int(out[0]), int(out[1])
+
+
+def form_rooflines(forms, stored, Np, R=192):
+    """Rooflines of the tile forms per bounce: the FP32 tiles against the
+    FP32 matrix-core peak (flops = stored tiles x 256 x 32 x R x 2); the
+    16-bit tiles against HBM (bytes = the matrix planes' stored tiles x 16 KB
+    + the operand planes R x Np x 2 B + the FP32 result R x Np x 4 B)."""
+    for key, v in forms.items():
+        sec = v["s_per_bounce"]
+        if key == "tiles_f32_mfma":
+            fl = stored * 256 * 32 * R * 2
+            v["roofline"] = dict(bound="mfma", achieved=round(fl / sec / 1e12, 2), peak=FP32_MATRIX_PEAK_TFLOPS,
+                                 unit="TFLOP/s", frac=round(fl / sec / 1e12 / FP32_MATRIX_PEAK_TFLOPS, 4))
+        elif key in ("tiles_bf16_mfma", "tiles_f16_mfma", "tiles_f16x3_mfma", "tiles_bf16x3_mfma"):
+            planes = 2 if "x3" in key else 1
+            by = planes * (stored * 16384 + R * Np * 2) + R * Np * 4
+            v["roofline"] = dict(bound="hbm", achieved=round(by / sec / 1e9, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                                 frac=round(by / sec / 1e9 / HBM_PEAK_GBS, 4))
+
+
+def compare_at_1e4(forms):
+    """The fastest 16-bit MFMA form at <= 1e-4 relative error against FP64,
+    and its speed against every FP32 form and the fastest of them."""
+    best = min((v["s_per_bounce"], k) for k, v in forms.items()
+               if ("bf16" in k or "f16" in k) and v["rel_l2_err_vs_fp64"] <= 1e-4)
+    fp32 = min((forms[k]["s_per_bounce"], k) for k in FP32_FORMS)
+    return dict(form=best[1], s_per_bounce=best[0], rel_l2_err_vs_fp64=forms[best[1]]["rel_l2_err_vs_fp64"],
+                fastest_fp32_form=fp32[1], speedup_vs_fastest_fp32=round(fp32[0] / best[0], 2),
+                speedup_vs=dict((k, round(forms[k]["s_per_bounce"] / best[0], 2)) for k in FP32_FORMS))
 
 
 def read_links(lib, dev, N, nlinks):
@@ -468,7 +509,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
     rowmap, other, fac = read_links(lib, dev, N, nl.value)
     cpu_bounce_s = reference_bounce_seconds(rowmap, other, fac, diff, E, area, N) if with_cpu else None
     nrm64 = np.linalg.norm(ref64)
-    for form, refine in [(0, 0), (1, 0), (2, 0), (3, 0), (3, 1), (4, 0), (4, 1)]:
+    for form, refine in FORM_RUNS:
         out, t = (ref64, t64) if form == 0 else run(form, refine)
         key = FORMS[form] + ("_refined" if refine else "")
         per_bounce = t[2] if not refine else (t[1] + t[3]) / bounces
@@ -476,19 +517,18 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
                         series_ms=round(t[1], 3), refine_ms=round(t[3], 3), build_ms=round(t[0], 3))
         log(f"[bench] {key}: {per_bounce:.3f} ms/bounce, err {res[key]['rel_l2_err_vs_fp64']:.2e}")
     tiles = tile_fraction(lib, dev)
+    Np = max(512, ((N + 255) // 256) * 256)
+    form_rooflines(res, tile_counts(lib, dev)[0], Np)
     lib.lgx_DeviceDestroy(dev)
     b.destroy()
-    Np = max(512, ((N + 255) // 256) * 256)
-    mf = res["dense_bf16_mfma"]["s_per_bounce"]
+    mf = res["tiles_bf16_mfma"]["s_per_bounce"]
     gemm_bytes = tiles * Np * Np * 2 + 2 * 192 * Np * 2
     out = dict(workload=f"C2 room radiosity: {N} patches (texels of the 10x10x4 m room), {R} right-hand sides "
                         f"({setups} seeded lighting setups x RGB), {bounces} bounces", patches=N, rhs=R,
                bounces=bounces, links=int(nl.value), link_density=round(2 * nl.value / (N * (N - 1)), 5),
                link_reference_counts=link_refc,
                stored_tile_fraction=round(tiles, 4), link_generation_s=link_ms / 1000.0, forms=res,
-               mfma_bf16_speedup_vs_dense_f32=round(res["dense_f32"]["s_per_bounce"] / mf, 2),
-               mfma_f16_refined_speedup_vs_dense_f32=round(
-                   res["dense_f32"]["s_per_bounce"] / res["dense_f16_mfma_refined"]["s_per_bounce"], 2),
+               mfma_at_1e4=compare_at_1e4(res),
                mfma_roofline=dict(bound="hbm", achieved=round(gemm_bytes / mf / 1e9, 1), peak=HBM_PEAK_GBS,
                                   unit="GB/s", frac=round(gemm_bytes / mf / 1e9 / HBM_PEAK_GBS, 4),
                                   note="BF16 matrix (its stored tiles) once per bounce + right-hand sides in and out"),
@@ -576,6 +616,7 @@ def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
 
     g4 = gradient(4)   # builds the FP16 matrix
     g4 = gradient(4)   # timed with the matrix resident
+    g6 = gradient(6)   # the compensated FP16 form (same matrix store)
     g0 = gradient(0)
     rel = lambda a, b_: float(np.linalg.norm(a - b_) / max(np.linalg.norm(b_), 1e-300))
     gpu_ms = g4["fwd_series_ms"] + g4["rev_fwd_ms"] + g4["rev_sweep_ms"] + g4["color_grad_ms"]
@@ -594,8 +635,13 @@ def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
                                  reverse_recompute=round(g4["rev_fwd_ms"], 3),
                                  reverse_sweep=round(g4["rev_sweep_ms"], 3),
                                  colour_gradient=round(g4["color_grad_ms"], 3)),
-               form="dense FP16 MFMA (FP32 accumulation)",
+               form="tiles FP16 MFMA (FP32 accumulation)",
                rel_err_vs_fp64=dict(colours=rel(g4["dcol"], g0["dcol"]), albedo=rel(g4["dalb"], g0["dalb"])),
+               f16x3=dict(form="tiles FP16 x 3 (error-compensated product every bounce, forward and reverse)",
+                          gradient_ms_gpu=round(g6["fwd_series_ms"] + g6["rev_fwd_ms"] + g6["rev_sweep_ms"]
+                                                + g6["color_grad_ms"], 3),
+                          rel_err_vs_fp64=dict(colours=rel(g6["dcol"], g0["dcol"]),
+                                               albedo=rel(g6["dalb"], g0["dalb"]))),
                fp64_gradient_ms_gpu=round(g0["fwd_series_ms"] + g0["rev_fwd_ms"] + g0["rev_sweep_ms"]
                                           + g0["color_grad_ms"], 3))
     # colour gradient over the checkpointed visibility of a ckpt_lm^2 atlas
@@ -824,7 +870,7 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
 
     ref64, t64 = run(0)
     forms = {}
-    for form, refine in [(0, 0), (1, 0), (2, 0), (3, 0), (3, 1), (4, 0), (4, 1)]:
+    for form, refine in FORM_RUNS:
         o, t = (ref64, t64) if form == 0 else run(form, refine)
         key = FORMS[form] + ("_refined" if refine else "")
         per = t[2] if not refine else (t[1] + t[3]) / bounces
@@ -832,15 +878,15 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
                           rel_l2_err_vs_fp64=float(np.linalg.norm(o - ref64) / np.linalg.norm(ref64)))
         log(f"[bench] C4 radiosity {key}: {per:.3f} ms/bounce, err {forms[key]['rel_l2_err_vs_fp64']:.2e}")
     tiles = tile_fraction(lib, rad_dev)
+    form_rooflines(forms, tile_counts(lib, rad_dev)[0], max(512, ((N + 255) // 256) * 256))
     rad = dict(patches=N, rhs=R, bounces=bounces, links=int(nl.value), link_generation_ms=round(link_ms, 1),
                link_reference_counts=link_refc, stored_tile_fraction=round(tiles, 4), forms=forms)
-    sp = forms["sparse_f32"]["s_per_bounce"]
-    best = min((v["s_per_bounce"], k) for k, v in forms.items()
-               if "mfma" in k and v["rel_l2_err_vs_fp64"] <= 1e-4)
-    rad["mfma_at_1e-4"] = dict(form=best[1], s_per_bounce=best[0],
-                               speedup_vs_sparse_f32_gather=round(sp / best[0], 2),
-                               speedup_vs_dense_f32=round(forms["dense_f32"]["s_per_bounce"] / best[0], 2))
-    log(f"[bench] C4 radiosity: {tiles:.4f} of the 256 x 32 tiles stored; best MFMA form at <= 1e-4: {best[1]}")
+    cmp = compare_at_1e4(forms)
+    rad["mfma_at_1e4"] = cmp
+    best = (cmp["s_per_bounce"], cmp["form"])
+    log(f"[bench] C4 radiosity: {tiles:.4f} of the 256 x 32 tiles stored; fastest 16-bit form at <= 1e-4: "
+        f"{cmp['form']} {cmp['s_per_bounce'] * 1e3:.3f} ms/bounce, {cmp['speedup_vs_fastest_fp32']}x the fastest "
+        f"FP32 form ({cmp['fastest_fp32_form']})")
     if with_cpu:
         rowmap, other, fac = read_links(lib, rad_dev, N, nl.value)
         cpu_b = reference_bounce_seconds(rowmap, other, fac, diff, E, area, N)
@@ -852,6 +898,7 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
                      "reference loop's seconds per bounce for one RGB setup times the number of setups",
                 speedup=round(cpu_b * sets / best[0], 1))
     out["radiosity"] = rad
+    out["adjoint"] = c4_adjoint(lib, leg, rad_dev, N, Wp, diff, area, bounces, rng)
     lib.lgx_DeviceDestroy(rad_dev)
     if with_cpu:
         # the reference's AO ray loop on a sample of texels, and the device's
@@ -884,6 +931,61 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
     out["lightmap_export"] = export_vs_host(lib, leg.built)
     out["sample_placement"] = placement_vs_host(lib, leg.spec, nth, leg.built, leg.pos)
     return out
+
+
+def c4_adjoint(lib, leg, rad_dev, N, Wp, diff, area, bounces, rng):
+    """Gradient of L = 1/2 |T - target|^2 over the radiosity tile (the first
+    N texels, 4 bounces) with respect to all 64 light colours (RGB): E =
+    Wp col from the checkpointed direct-light weights, T = sum_k K^k E;
+    lgx_RadSolveGrad gives lam = dL/dE, lgx_DirectLightGrad the colours'
+    gradient from lam over every work item (lam is zero beyond the tile).
+    The FP16 x 3 form against the sparse FP64 form's gradient."""
+    dp = C.POINTER(C.c_double)
+    L = len(leg.lights)
+    col0 = np.array([[leg.lights[l].color[c] for c in range(3)] for l in range(L)], np.float64)
+    # the target is the bake at 40-60 % of the colours: the residual keeps one
+    # sign, so the gradient's sums over texels do not cancel
+    target_col = col0 * rng.uniform(0.4, 0.6, col0.shape)
+    tm = (C.c_double * 6)()
+
+    def solve(form, col):
+        E = np.ascontiguousarray(Wp @ col + 0.05)
+        T = np.zeros_like(E)
+        t = (C.c_double * 4)()
+        _lib.check(lib.lgx_RadSolve(rad_dev, E.ctypes.data_as(dp), 3, P(diff), P(area), form, bounces, 0,
+                                    T.ctypes.data_as(dp), t), lib)
+        return E, T
+
+    _, target = solve(0, target_col)
+    out = {}
+    for form in (6, 0):
+        t0 = time.perf_counter()
+        E, T = solve(form, col0)
+        g = np.ascontiguousarray(T - target)
+        tot, lam, dalb = np.zeros_like(E), np.zeros_like(E), np.zeros_like(E)
+        t = (C.c_double * 4)()
+        _lib.check(lib.lgx_RadSolveGrad(rad_dev, E.ctypes.data_as(dp), 3, P(diff), P(area), form, bounces,
+                                        g.ctypes.data_as(dp), tot.ctypes.data_as(dp), lam.ctypes.data_as(dp),
+                                        dalb.ctypes.data_as(dp), t), lib)
+        lam_all = np.zeros((leg.S, 3))
+        lam_all[:N] = lam
+        dcol = np.zeros((L, 3))
+        _lib.check(lib.lgx_DirectLightGrad(leg.dev, lam_all.ctypes.data_as(dp), dcol.ctypes.data_as(dp)), lib)
+        lib.lgx_DeviceTimes(leg.dev, tm)
+        out[form] = dict(dcol=dcol, wall_ms=(time.perf_counter() - t0) * 1e3, forward_ms=t[1],
+                         reverse_ms=t[3], colour_grad_ms=tm[5])
+    g6, g0 = out[6], out[0]
+    rel = float(np.linalg.norm(g6["dcol"] - g0["dcol"]) / np.linalg.norm(g0["dcol"]))
+    res = dict(parameters=f"{L} light colours x RGB", patches=N, bounces=bounces, form="tiles_f16x3_mfma",
+               gradient_ms_gpu=round(g6["forward_ms"] + g6["reverse_ms"] + g6["colour_grad_ms"], 3),
+               breakdown_ms=dict(forward_series=round(g6["forward_ms"], 3),
+                                 reverse_sweep=round(g6["reverse_ms"], 3),
+                                 colour_gradient=round(g6["colour_grad_ms"], 3)),
+               gradient_ms_wall=round(g6["wall_ms"], 1), rel_err_vs_fp64=rel,
+               fp64_gradient_ms_gpu=round(g0["forward_ms"] + g0["reverse_ms"] + g0["colour_grad_ms"], 3))
+    log(f"[bench] C4 gradient w.r.t. {L} light colours through the radiosity tile: {res['gradient_ms_gpu']} ms "
+        f"(FP16 x 3), rel err vs FP64 {rel:.2e}")
+    return res
 
 
 C4_WORKLOAD = ("C4 city block direct light: 1 km heightfield (131,072 tris) + 155,000 lognormal(1, 0.6) m "

@@ -171,14 +171,19 @@ LTRAPI int lgx_RadBounce(lgx_Device* dev, const float* diffuse3, const float* em
  * K = F diag(area) diag(diffuse_{c % 3}) for column c, the closed form of
  * bounce_count rounds of Int_RDBounce (lighter.cpp:767-797, 1104-1123).
  * E and total are N x R row-major FP64.  form: 0 sparse FP64 gather,
- * 1 sparse FP32 gather, 2 dense FP32, 3 dense BF16 on MFMA, 4 dense FP16 on
- * MFMA (FP32 accumulation); refine: refinement sweeps for the dense forms
- * with an FP64 residual from the error-compensated (hi + lo) product.
- * times_ms: dense build, series, per bounce, refinement. */
+ * 1 sparse FP32 gather, 2 dense FP32, 3 BF16 on MFMA and 4 FP16 on MFMA over
+ * the 256 x 32 tiles of the matrix that hold a link (patches in Morton
+ * order), 5 FP32 on the FP32 matrix cores over the same tiles, 6 FP16 and
+ * 7 BF16 with the error-compensated product (hi + lo planes of matrix and
+ * operand, three MFMA products) on every bounce; every form accumulates in
+ * FP32 or FP64.  refine: refinement sweeps of forms 3 and 4 with an FP64
+ * residual from the error-compensated product (an error for other forms).
+ * times_ms: matrix build, series, per bounce, refinement. */
 LTRAPI int lgx_RadSolve(lgx_Device* dev, const double* E, uint32_t R, const float* diffuse3, const float* area,
                         int form, int bounces, int refine, double* total, double times_ms[4]);
 /* Reverse mode of lgx_RadSolve (no counterpart in the reference, which has
- * no derivatives): for the upstream gradient g = dL/dtotal (N x R FP64),
+ * no derivatives), in any of its forms (forward and reverse products both
+ * the form's own): for the upstream gradient g = dL/dtotal (N x R FP64),
  * total again, lam = dL/dE (which is also dL/d emission) and
  * dalb = dL/d diffuse per column (column c holds the derivative through the
  * albedo of channel c % 3; sum the columns of one channel).  The adjoint

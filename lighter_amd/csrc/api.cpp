@@ -698,7 +698,7 @@ int lgx_RadSolve(lgx_Device* d, const double* E, uint32_t R, const float* diffus
                  int bounces, int refine, double* total, double times_ms[4])
 {
 	return guarded([&] {
-		if (form < kSparseF64 || form > kDenseF16)
+		if (form < 0 || form >= kNumForms)
 			throw std::runtime_error("lgx_RadSolve: unknown form");
 		if (bounces < 0 || refine < 0)
 			throw std::runtime_error("lgx_RadSolve: negative count");
@@ -724,7 +724,7 @@ int lgx_RadSolveGrad(lgx_Device* d, const double* E, uint32_t R, const float* di
                      double times_ms[4])
 {
 	return guarded([&] {
-		if (form < kSparseF64 || form > kDenseF16)
+		if (form < 0 || form >= kNumForms)
 			throw std::runtime_error("lgx_RadSolveGrad: unknown form");
 		if (bounces < 0)
 			throw std::runtime_error("lgx_RadSolveGrad: negative bounce count");

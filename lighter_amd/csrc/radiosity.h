@@ -16,7 +16,11 @@ enum RadForm {
 	kDenseF32 = 2,  // dense H = F in FP32, plain FP32 FMAs
 	kDenseBF16 = 3, // H in BF16 (the tiles that hold links), MFMA with FP32 accumulation
 	kDenseF16 = 4,  // H in FP16 (power-of-two scaled, the tiles that hold links), MFMA with FP32 accumulation
+	kTileF32 = 5,   // H in FP32 as the same stored tiles, FP32 matrix-core MFMA (the fair FP32 form)
+	kF16x3 = 6,     // FP16 hi + lo planes of H and the operand, three MFMA products per tile, every bounce
+	kBF16x3 = 7,    // the same in BF16
 };
+constexpr int kNumForms = 8;
 
 struct RadTimes {
 	double build_ms = 0;      // dense matrix construction
@@ -49,19 +53,26 @@ private:
 	void release();
 	void build_tiles();
 	void ensure_work();
+	void ensure_part();
 	uint32_t ksplit(uint32_t rows_per_block, uint32_t min_blocks) const;
 	void apply_dense(int form, const float* xT, float* yT, bool split, bool pre);
+	void apply_tiles_f32(const float* xT, float* yT, bool pre);
 	void apply_op(int form, const double* x, double* y, bool pre);
 	DeviceScene& dev;
 	void* s_ = nullptr;
 	float* g32_ = nullptr;
 	uint16_t* g16_ = nullptr;   // 16-bit matrix (hi plane)
 	uint16_t* g16lo_ = nullptr; // remainder plane for the compensated product
+	float* g32t_ = nullptr;     // FP32 tile store of form kTileF32 (same tiles as the 16-bit forms)
 	int32_t* tmap_ = nullptr;   // tile (256-row block, k stage) -> place in the 16-bit store, or -1
 	uint32_t* tks_ = nullptr;   // k stage of each stored tile (row blocks in order, stages ascending)
-	uint32_t* chunks_ = nullptr; // work runs {row block, first tile, end tile, slot}
-	uint32_t* nslot_ = nullptr;  // runs (partial-sum slots) per row block
+	uint32_t* chunks_ = nullptr; // work segments {row block, first tile, end tile, slot}
+	uint32_t* segoff_ = nullptr; // first segment of each persistent block (nblocks_ + 1 entries)
+	uint32_t* nslot_ = nullptr;  // segments (partial-sum slots) per row block
 	uint64_t ntiles_ = 0, nchunks_ = 0;
+	uint32_t nblocks_ = 1;  // persistent blocks of the tile products (one per compute unit)
+	uint32_t maxslot_ = 0;  // most segments of one row block
+	uint32_t part_slots_ = 0; // slots part_ holds
 	bool tiles_built_ = false;
 	uint32_t* perm_ = nullptr; // position -> patch (Morton order, padding last)
 	uint32_t* pinv_ = nullptr; // patch -> position
@@ -79,6 +90,8 @@ private:
 	float* yT_ = nullptr;
 	float* rm32a_ = nullptr;
 	float* rm32b_ = nullptr;
+	float* rmx_ = nullptr;  // sparse FP32 forms: the operand (x diffuse) area prepared once per product
+	double* rmx64_ = nullptr; // the same for the FP64 form
 	unsigned* cmax_ = nullptr; // column maxima of the FP16 operand (its scales), per product
 };
 

@@ -73,6 +73,27 @@ template <class T> static void dfree(T*& p)
 	p = nullptr;
 }
 
+// A device array owned for one solve: freed on every exit, errors included.
+template <class T> struct DevBuf {
+	T* p = nullptr;
+	DevBuf() = default;
+	explicit DevBuf(size_t n) : p(dalloc<T>(n)) {}
+	DevBuf(const DevBuf&) = delete;
+	DevBuf& operator=(const DevBuf&) = delete;
+	DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+	DevBuf& operator=(DevBuf&& o) noexcept
+	{
+		if (this != &o) {
+			dfree(p);
+			p = o.p;
+			o.p = nullptr;
+		}
+		return *this;
+	}
+	~DevBuf() { dfree(p); }
+	operator T*() const { return p; }
+};
+
 // ---------------------------------------------------------------------------
 // conversions
 // ---------------------------------------------------------------------------
@@ -307,26 +328,24 @@ __global__ __launch_bounds__(kB) void k_prep_x32(const float* __restrict__ xT, c
 }
 
 // ---------------------------------------------------------------------------
-// MFMA GEMM: part[s][c][i] = sum_{j in split s} G[i][j] * Xq[c][j]
-//
-// Block: 4 waves, each RT row tiles of 16 x all 12 column tiles (R padded to
-// 192), v_mfma_f32_16x16x32 (bf16 or f16) with FP32 accumulators.  Lane l
-// supplies row (l & 15) of A and column (l & 15) of B with the same eight k
-// indices 8*(l>>4)+e, so the k order inside an instruction is immaterial;
-// output element i of a 16x16 tile sits at row 4*(l>>4)+i, column l & 15.
-//
-// k advances in stages of 32.  A stage (the block's matrix rows x 32 and the
-// right-hand-side chunk 192 x 32, per plane) goes HBM -> LDS by
-// global_load_lds_dwordx4 (no register staging), PD stages in flight in a
-// ring of PD LDS buffers.  LDS holds 16-row x 64-byte groups of 1 KB, one
-// DMA instruction each; the 16-byte pieces of a row are permuted by
-// piece_swz(row) so that the rows one ds_read_b128 serves together fall in
-// distinct banks.  Each wave fetches its own matrix rows and a quarter of
-// the right-hand-side chunk, waits on its own DMA count, then a barrier makes
-// every wave's part visible.
+// MFMA building blocks of the 16-bit tile products: v_mfma_f32_16x16x32
+// (bf16 or f16) with FP32 accumulators; lane l supplies row (l & 15) of A and
+// column (l & 15) of B with the same eight k indices 8 (l >> 4) + e, so the k
+// order inside an instruction is immaterial; output element i of a 16 x 16
+// tile sits at row 4 (l >> 4) + i, column l & 15.  k advances in stages of
+// 32; operand stages go HBM -> LDS by global_load_lds_dwordx4, 1 KB (one
+// 16-row x 64-byte group) per instruction, and the 16-byte pieces of a row
+// are permuted by piece_swz(row) so that the rows one ds_read_b128 serves
+// together fall in distinct banks.
 // ---------------------------------------------------------------------------
 constexpr int kKS = 32; // k per stage
 constexpr int kMaxPD = 6; // deepest LDS ring
+
+// The 16-bit matrix and operand are stored as LDS images (see k_tile_scatter
+// / k_prep_x16): 1 KB groups of 16 rows x 64 bytes.
+constexpr uint64_t kTileElems = 512;                         // 16-bit elements per 1 KB group
+constexpr uint64_t kGStageElems = 16 * kTileElems;           // a stored 256-row x 32 tile
+constexpr uint64_t kXStageElems = (kRPad / 16) * kTileElems; // an operand stage (192 x 32)
 
 template <int P>
 __device__ __forceinline__ f32x4 mfma16(i16x8 a, i16x8 b, f32x4 c)
@@ -366,179 +385,273 @@ __device__ __forceinline__ void dma16(const uint16_t* src, unsigned char* dst)
 	__builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)dst, 16, 0, 0);
 }
 
-template <int NPL, int RT>
-struct GemmShape {
-	static constexpr int kRows = 64 * RT;                       // block rows
-	static constexpr int kXBytes = kRPad * kKS * 2;             // right-hand-side plane per stage
-	static constexpr int kGBytes = kRows * kKS * 2;             // matrix plane per stage
-	static constexpr int kStage = NPL * (kXBytes + kGBytes);    // LDS bytes per stage
-	static constexpr int kOps = NPL * (kColTiles / 4 + RT);     // DMA instructions per wave per stage
+// ---------------------------------------------------------------------------
+// The same product with the matrix fragments loaded straight into registers.
+//
+// Block: 4 waves over one 256-row block (64 rows = 4 row tiles per wave) x
+// all 12 column tiles.  The matrix is per-wave data: each lane loads its A
+// fragment (16 bytes per row tile and plane) from the tile store into
+// registers D stages ahead; in the store's LDS-image layout lane l's piece
+// (row l & 15, k piece l >> 4) lies at 16 (4 row + (piece ^ piece_swz(row)))
+// bytes into the row tile's 1 KB group, so a wave's loads of one row tile
+// cover one contiguous KB.  The operand stage (192 x 32 per plane, 12 KB) is
+// shared by the block's waves: it goes HBM -> LDS by global_load_lds_dwordx4
+// into a ring of D slots (one variable each, so that the compiler's
+// wait-count tracking tells the slots apart), each wave copying 3 of the 12
+// groups per plane.
+// Stage s (ring slot s % D, unrolled so that every slot is a constant):
+// wait until this wave's loads and copies of stage s have landed (the
+// D - 1 later stages may still be in flight), barrier (every wave's copies
+// are in), read the operand fragments, wait for those reads and barrier
+// (every wave is done with the slot), copy stage s + D into the slot,
+// multiply, load stage s + D's matrix fragments into the registers just
+// used.  Past the end the last stage is fetched again (never used), which
+// keeps every stage's wait count exact.
+// ---------------------------------------------------------------------------
+template <int NPL, int D>
+struct TileLane {
+	static constexpr int kXPlane = kRPad * kKS * 2; // operand plane per stage (12 KB)
+	static constexpr int kOps = NPL * (3 + 4);      // loads and copies per wave per stage
+	const uint16_t *G, *Glo, *xq, *xlo;
+	const uint32_t* lst; // k stages of the run's tiles
+	uint64_t first;      // the run's first tile
+	uint32_t aofs;       // element offset of this lane's A piece in a 1 KB group
+	int rofs;            // byte offset of this lane's B fragment in a 1 KB group
+	int wave, lane;
+	unsigned char* slot[kMaxPD];
 };
 
-// Issue stage st (counted from the split's first stage) into LDS buffer
-// base.  The 16-bit matrix and operand are stored as LDS images (see
-// k_tile_scatter / k_prep_x16): every DMA instruction copies one contiguous
-// KB, and a block's stage is one contiguous tile.
-constexpr uint64_t kTileElems = 512;             // 16-bit elements per 1 KB group
-constexpr uint64_t kGStageElems = 16 * kTileElems; // a 256-row stage of the matrix
-constexpr uint64_t kXStageElems = (kRPad / 16) * kTileElems; // a stage of the operand
-
-template <int NPL, int RT>
-__device__ __forceinline__ void gemm_issue(const uint16_t* G, const uint16_t* Glo, const uint16_t* xq,
-                                           const uint16_t* xlo, uint64_t xsrc, uint64_t gsrc, uint64_t gstride,
-                                           const uint32_t* lst, uint32_t st, unsigned char* base, int wave)
+__device__ __forceinline__ void lds_reads_done()
 {
-	typedef GemmShape<NPL, RT> S;
-	const uint64_t ks = lst[st]; // k stage of the block's st-th listed tile
+	asm volatile("" ::: "memory");
+	__builtin_amdgcn_s_waitcnt(0xC07F); // lgkmcnt(0), vmcnt and expcnt left alone
+	asm volatile("" ::: "memory");
+}
+
+template <int NPL, int D>
+__device__ __forceinline__ void tile_copy_operand(const TileLane<NPL, D>& L, uint32_t st, unsigned char* slot)
+{
+	const uint64_t ks = L.lst[st];
 #pragma unroll
 	for (int pl = 0; pl < NPL; ++pl) {
-		const uint16_t* X = pl ? xlo : xq;
-		const uint16_t* M = pl ? Glo : G;
-		unsigned char* xb = base + pl * S::kXBytes + wave * (kColTiles / 4) * 1024;
-		unsigned char* gb = base + NPL * S::kXBytes + pl * S::kGBytes + wave * RT * 1024;
+		const uint16_t* X = (pl ? L.xlo : L.xq) + ks * kXStageElems + 8 * L.lane;
 #pragma unroll
-		for (int j = 0; j < kColTiles / 4; ++j)
-			dma16(X + xsrc + ks * kXStageElems + j * kTileElems, xb + j * 1024);
-#pragma unroll
-		for (int t = 0; t < RT; ++t)
-			dma16(M + gsrc + (uint64_t)st * gstride + t * kTileElems, gb + t * 1024);
+		for (int j = 0; j < 3; ++j)
+			dma16(X + (uint64_t)(3 * L.wave + j) * kTileElems,
+			      slot + pl * TileLane<NPL, D>::kXPlane + (3 * L.wave + j) * 1024);
 	}
 }
 
-template <int NPL, int RT>
-struct GemmLane {
-	uint64_t xsrc, gsrc; // DMA source offsets (elements) of this lane at the split's first stage
-	uint64_t gstride;    // matrix elements from one stage to the next
-	const uint32_t* lst; // k stages of the split's tiles
-	int wave, rofs;      // wave index; ds_read_b128 offset of the fragment in a 1 KB group
-	unsigned char* buf[kMaxPD]; // the LDS stage ring
-};
-
-// one k stage (buffer B of PD): wait for it, free the previous buffer, issue
-// the stage PD - 1 ahead into that buffer, multiply
-template <int P, int NPL, int RT, int PD, int B>
-__device__ __forceinline__ void gemm_step(int s, int nst, const GemmLane<NPL, RT>& L, const uint16_t* G,
-                                          const uint16_t* Glo, const uint16_t* xq, const uint16_t* xlo,
-                                          uint64_t Np, f32x4 (&acc)[RT][kColTiles])
+template <int NPL, int D>
+__device__ __forceinline__ void tile_load_matrix(const TileLane<NPL, D>& L, uint32_t st, i16x8 (&a)[NPL][4])
 {
-	typedef GemmShape<NPL, RT> S;
-	// stage s has landed once at most the later PD - 2 stages' DMAs are pending
-	wait_vm<S::kOps * (PD - 2)>();
-	block_barrier(); // every wave's part of stage s is in; stage s-1's buffer is free
-	// all of this stage's fragments are read before the next DMA is issued:
-	// the compiler cannot tell that DMA's buffer from this one and would
-	// otherwise wait for every DMA in flight
-	const unsigned char* base = L.buf[B];
-	i16x8 a[NPL][RT], bx[NPL][kColTiles];
+	const uint64_t base = (L.first + st) * kGStageElems + (uint64_t)(4 * L.wave) * kTileElems + L.aofs;
 #pragma unroll
 	for (int pl = 0; pl < NPL; ++pl) {
+		const uint16_t* M = pl ? L.Glo : L.G;
 #pragma unroll
-		for (int t = 0; t < RT; ++t)
-			a[pl][t] = *reinterpret_cast<const i16x8*>(base + NPL * S::kXBytes + pl * S::kGBytes +
-			                                           (L.wave * RT + t) * 1024 + L.rofs);
+		for (int t = 0; t < 4; ++t)
+			a[pl][t] = *reinterpret_cast<const i16x8*>(M + base + t * kTileElems);
+	}
+}
+
+template <int P, int NPL, int D, int B>
+__device__ __forceinline__ void tile_step(int s, int nst, const TileLane<NPL, D>& L, i16x8 (&a)[D][NPL][4],
+                                          f32x4 (&acc)[4][kColTiles])
+{
+	typedef TileLane<NPL, D> T;
+	wait_vm<T::kOps * (D - 1)>();
+	block_barrier();
+	unsigned char* slot = L.slot[B];
+	i16x8 bx[NPL][kColTiles];
+#pragma unroll
+	for (int pl = 0; pl < NPL; ++pl)
 #pragma unroll
 		for (int c = 0; c < kColTiles; ++c)
-			bx[pl][c] = *reinterpret_cast<const i16x8*>(base + pl * S::kXBytes + c * 1024 + L.rofs);
-	}
-	// always PD - 1 stages in flight: past the end, the last stage is fetched
-	// again into the freed buffer (never read), which keeps the wait counts
-	// above exact
-	constexpr int BN = (B + PD - 1) % PD;
-	const int ahead = s + PD - 1 < nst ? s + PD - 1 : nst - 1;
-	gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, L.gstride, L.lst, ahead, L.buf[BN], L.wave);
+			bx[pl][c] = *reinterpret_cast<const i16x8*>(slot + pl * T::kXPlane + c * 1024 + L.rofs);
+	lds_reads_done();
+	block_barrier();
+	const int ahead = s + D < nst ? s + D : nst - 1;
+	tile_copy_operand<NPL, D>(L, (uint32_t)ahead, slot);
 #pragma unroll
 	for (int c = 0; c < kColTiles; ++c) {
 #pragma unroll
-		for (int t = 0; t < RT; ++t)
-			acc[t][c] = mfma16<P>(a[0][t], bx[0][c], acc[t][c]);
+		for (int t = 0; t < 4; ++t)
+			acc[t][c] = mfma16<P>(a[B][0][t], bx[0][c], acc[t][c]);
 		if (NPL == 2) {
 			// error-compensated: + G_hi X_lo + G_lo X_hi
 #pragma unroll
-			for (int t = 0; t < RT; ++t) {
-				acc[t][c] = mfma16<P>(a[0][t], bx[NPL - 1][c], acc[t][c]);
-				acc[t][c] = mfma16<P>(a[NPL - 1][t], bx[0][c], acc[t][c]);
+			for (int t = 0; t < 4; ++t) {
+				acc[t][c] = mfma16<P>(a[B][0][t], bx[NPL - 1][c], acc[t][c]);
+				acc[t][c] = mfma16<P>(a[B][NPL - 1][t], bx[0][c], acc[t][c]);
 			}
 		}
 	}
+	tile_load_matrix<NPL, D>(L, (uint32_t)ahead, a[B]);
 }
 
-template <int P, int NPL, int RT, int PD, int... Bs>
-__device__ __forceinline__ void gemm_steps(int s, const GemmLane<NPL, RT>& L, const uint16_t* G,
-                                           const uint16_t* Glo, const uint16_t* xq, const uint16_t* xlo,
-                                           uint64_t Np, f32x4 (&acc)[RT][kColTiles], int nst,
-                                           std::integer_sequence<int, Bs...>)
+template <int P, int NPL, int D, int... Bs>
+__device__ __forceinline__ void tile_steps(int s, int nst, const TileLane<NPL, D>& L, i16x8 (&a)[D][NPL][4],
+                                           f32x4 (&acc)[4][kColTiles], std::integer_sequence<int, Bs...>)
 {
-	(gemm_step<P, NPL, RT, PD, Bs>(s + Bs, nst, L, G, Glo, xq, xlo, Np, acc), ...);
+	(tile_step<P, NPL, D, Bs>(s + Bs, nst, L, a, acc), ...);
 }
 
-template <int NPL, int RT, int... Bs>
-__device__ __forceinline__ void gemm_prologue(int nst, const GemmLane<NPL, RT>& L, const uint16_t* G,
-                                              const uint16_t* Glo, const uint16_t* xq, const uint16_t* xlo,
-                                              uint64_t Np, std::integer_sequence<int, Bs...>)
+template <int P, int NPL, int D, int... Bs>
+__device__ __forceinline__ void tile_tail(int s, int nst, const TileLane<NPL, D>& L, i16x8 (&a)[D][NPL][4],
+                                          f32x4 (&acc)[4][kColTiles], std::integer_sequence<int, Bs...>)
 {
-	// stage Bs into buffer Bs (the last stage again where there are fewer)
-	(gemm_issue<NPL, RT>(G, Glo, xq, xlo, L.xsrc, L.gsrc, L.gstride, L.lst, Bs < nst ? Bs : nst - 1, L.buf[Bs],
-	                     L.wave),
+	((s + Bs < nst ? tile_step<P, NPL, D, Bs>(s + Bs, nst, L, a, acc) : void()), ...);
+}
+
+template <int NPL, int D, int... Bs>
+__device__ __forceinline__ void tile_prologue(int nst, const TileLane<NPL, D>& L, i16x8 (&a)[D][NPL][4],
+                                              std::integer_sequence<int, Bs...>)
+{
+	((tile_copy_operand<NPL, D>(L, (uint32_t)(Bs < nst ? Bs : nst - 1), L.slot[Bs]),
+	  tile_load_matrix<NPL, D>(L, (uint32_t)(Bs < nst ? Bs : nst - 1), a[Bs])),
 	 ...);
 }
 
-template <int P, int NPL, int RT, int PD, int... Bs>
-__device__ __forceinline__ void gemm_tail(int s, int nst, const GemmLane<NPL, RT>& L, const uint16_t* G,
-                                          const uint16_t* Glo, const uint16_t* xq, const uint16_t* xlo,
-                                          uint64_t Np, f32x4 (&acc)[RT][kColTiles],
-                                          std::integer_sequence<int, Bs...>)
+template <int P, int NPL, int D>
+__global__ __launch_bounds__(kB) void k_gemm_tiles(const uint16_t* __restrict__ G, const uint16_t* __restrict__ Glo,
+                                                   const uint16_t* __restrict__ xq, const uint16_t* __restrict__ xlo,
+                                                   const uint4* __restrict__ chunks,
+                                                   const uint32_t* __restrict__ seg_off,
+                                                   const uint32_t* __restrict__ tks, uint32_t Np,
+                                                   float* __restrict__ part)
 {
-	((s + Bs < nst ? gemm_step<P, NPL, RT, PD, Bs>(s + Bs, nst, L, G, Glo, xq, xlo, Np, acc) : void()), ...);
-}
-
-// y = G x over the stored tiles.  A chunk {rb, tbeg, tend, slot} is a run of
-// 256-row block rb's stored tiles (tks[t]: the k stage of tile t); blocks
-// 256 / kRows at a time take its rows and write their partial sums to the
-// slot's plane of part.
-template <int P, int NPL, int RT, int PD>
-__global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G, const uint16_t* __restrict__ Glo,
-                                                  const uint16_t* __restrict__ xq, const uint16_t* __restrict__ xlo,
-                                                  const uint4* __restrict__ chunks, const uint32_t* __restrict__ tks,
-                                                  uint32_t Np, float* __restrict__ part)
-{
-	typedef GemmShape<NPL, RT> S;
-	static_assert(PD >= 2, "pipeline depth");
+	typedef TileLane<NPL, D> T;
+	static_assert(D >= 2 && D <= kMaxPD, "ring depth");
+	constexpr int kSlot = NPL * T::kXPlane;
+	__shared__ __align__(1024) unsigned char lds0[kSlot], lds1[kSlot], lds2[D > 2 ? kSlot : 16],
+	    lds3[D > 3 ? kSlot : 16], lds4[D > 4 ? kSlot : 16], lds5[D > 5 ? kSlot : 16];
 	const int tid = threadIdx.x, lane = tid & 63;
 	const int r = lane & 15, g = lane >> 4;
-	// one variable per stage buffer: the compiler's wait-count tracking can
-	// then tell the DMA into one buffer from the reads of another (with a
-	// single array every LDS read would wait for every DMA in flight)
-	static_assert(PD <= kMaxPD, "pipeline depth");
-	__shared__ __align__(1024) unsigned char lds0[S::kStage], lds1[S::kStage], lds2[PD > 2 ? S::kStage : 16],
-	    lds3[PD > 3 ? S::kStage : 16], lds4[PD > 4 ? S::kStage : 16], lds5[PD > 5 ? S::kStage : 16];
-	GemmLane<NPL, RT> L;
-	L.buf[0] = lds0;
-	L.buf[1] = lds1;
-	L.buf[2] = lds2;
-	L.buf[3] = lds3;
-	L.buf[4] = lds4;
-	L.buf[5] = lds5;
+	T L;
+	L.slot[0] = lds0;
+	L.slot[1] = lds1;
+	L.slot[2] = lds2;
+	L.slot[3] = lds3;
+	L.slot[4] = lds4;
+	L.slot[5] = lds5;
+	L.G = G;
+	L.Glo = Glo;
+	L.xq = xq;
+	L.xlo = xlo;
 	L.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-	constexpr uint32_t kBPC = 256 / S::kRows; // blocks per chunk
-	// XCD-aware order: block b runs on XCD b % 8, so XCD x is given the x-th
-	// contiguous eighth of the work; neighbouring chunks (neighbouring
-	// patches in Morton order, hence overlapping operand stages) then share
-	// that XCD's L2
+	L.lane = lane;
+	// XCD-aware order, as in k_gemm_mfma
 	const uint32_t nb = gridDim.x, x = blockIdx.x % 8, per = nb / 8, rem = nb % 8;
 	const uint32_t bid = x * per + (x < rem ? x : rem) + blockIdx.x / 8;
-	const uint4 ch = chunks[bid / kBPC];
-	const uint32_t row0 = ch.x * 256 + (bid % kBPC) * S::kRows; // the block's first row
-	const uint32_t i0 = row0 + L.wave * 16 * RT;                       // this wave's first row
-	const uint32_t gbase = (row0 & 255) >> 4;
-	const int nst = (int)(ch.z - ch.y);
-	// DMA sources in the LDS-image layouts: lane l copies the l-th 16 bytes
-	// of each 1 KB group.  Stored tile t (16 KB) holds group g at
-	// (t * 16 + g) KB; operand stage ks, column group cg at (ks * 12 + cg) KB.
-	L.lst = tks + ch.y;
-	L.gstride = kGStageElems;
-	L.gsrc = ((uint64_t)ch.y * 16 + gbase + (uint64_t)L.wave * RT) * kTileElems + 8 * lane;
-	L.xsrc = ((uint64_t)(kColTiles / 4) * L.wave) * kTileElems + 8 * lane;
+	L.aofs = 8u * (uint32_t)(4 * r + (g ^ piece_swz(r)));
 	L.rofs = r * 64 + 16 * (g ^ piece_swz(r));
+	for (uint32_t q = seg_off[bid]; q < seg_off[bid + 1]; ++q) {
+		const uint4 ch = chunks[q];
+		const uint32_t i0 = ch.x * 256 + L.wave * 64;
+		const int nst = (int)(ch.z - ch.y);
+		L.lst = tks + ch.y;
+		L.first = ch.y;
+		f32x4 acc[4][kColTiles];
+#pragma unroll
+		for (int t = 0; t < 4; ++t)
+#pragma unroll
+			for (int c = 0; c < kColTiles; ++c)
+				acc[t][c] = f32x4{0, 0, 0, 0};
+		if (nst > 0) {
+			i16x8 a[D][NPL][4];
+			tile_prologue<NPL, D>(nst, L, a, std::make_integer_sequence<int, D>{});
+			int s = 0;
+			for (; s + D <= nst; s += D)
+				tile_steps<P, NPL, D>(s, nst, L, a, acc, std::make_integer_sequence<int, D>{});
+			tile_tail<P, NPL, D>(s, nst, L, a, acc, std::make_integer_sequence<int, D - 1>{});
+			wait_vm<0>();    // no copy lands after the segment (or the block) is done with the ring
+			block_barrier(); // every wave has finished with the ring before the next segment refills it
+		}
+		float* out = part + (uint64_t)ch.w * kRPad * Np;
+#pragma unroll
+		for (int t = 0; t < 4; ++t)
+#pragma unroll
+			for (int c = 0; c < kColTiles; ++c) {
+				const uint32_t col = c * 16 + r, i = i0 + t * 16 + 4 * g;
+				*reinterpret_cast<float4*>(out + (uint64_t)col * Np + i) =
+				    make_float4(acc[t][c][0], acc[t][c][1], acc[t][c][2], acc[t][c][3]);
+			}
+	}
+}
+
+constexpr int kTilesD1 = 4, kTilesD2 = 3; // ring depths of the plain and compensated products
+
+constexpr uint32_t kMaxSlots = 16; // partial-sum planes allocated at least
+constexpr uint64_t kMinShare = 16;  // fewest tiles per persistent block of the tile products
+
+// ---------------------------------------------------------------------------
+// Tile-sparse FP32 GEMM on the FP32 matrix cores: the stored tiles, work runs
+// and slots of the 16-bit forms, values in FP32.  v_mfma_f32_16x16x4_f32:
+// lane l supplies A[row l & 15][k] and B[k][column l & 15] for one k of the
+// instruction's four; over the eight instructions of a 32-wide k stage, lane
+// group g = l >> 4 takes k = 8g .. 8g + 7 in turn, for A and B alike, so the
+// four k of every instruction match.  Output element i of a 16 x 16 tile
+// sits at row 4 (l >> 4) + i, column l & 15, as for the 16-bit MFMA.
+//
+// LDS images (and the HBM layouts, so that every DMA copies one contiguous
+// KB): 2 KB groups of 16 rows x 128 bytes, the eight 16-byte pieces of a row
+// at slot piece ^ (row & 7), so that the rows one ds_read_b128 serves
+// together fall in distinct banks.  A stored tile is 16 groups (256 rows x
+// 32 k, 32 KB); an operand stage 12 groups (192 columns x 32 k, 24 KB).
+// Block: 4 waves x 64 rows (4 row tiles) x all 192 columns; k stages
+// through a two-buffer LDS ring (112 KB), the next stage's DMA in flight
+// while the current one is multiplied (a stage is 384 MFMAs per wave, the
+// kernel is bound by FP32 matrix throughput, not by its loads).
+// ---------------------------------------------------------------------------
+constexpr uint64_t kT32Group = 512;                      // floats per 2 KB group
+constexpr uint64_t kT32Tile = 16 * kT32Group;            // a stored tile
+constexpr uint64_t kT32XStage = (kRPad / 16) * kT32Group; // an operand stage
+constexpr int kT32XBytes = (int)(kT32XStage * 4);          // 24 KB
+constexpr int kT32Stage = kT32XBytes + (int)(kT32Tile * 4); // 56 KB
+
+__device__ __forceinline__ int t32_slot(int piece, int row) { return piece ^ (row & 7); }
+
+__device__ __forceinline__ void dma16f(const float* src, unsigned char* dst)
+{
+	__builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)dst, 16, 0, 0);
+}
+
+// stage st of the run (tile first + st, operand stage lst[st]) into buf
+__device__ __forceinline__ void t32_issue(const float* G, const float* xq, uint64_t first, const uint32_t* lst,
+                                          uint32_t st, unsigned char* buf, int wave, int lane)
+{
+	const uint64_t ks = lst[st];
+	const float* xs = xq + ks * kT32XStage + 4 * lane;
+	const float* gs = G + (first + st) * kT32Tile + 4 * lane;
+#pragma unroll
+	for (int j = 0; j < 6; ++j) // the operand's 24 KB: 6 per wave
+		dma16f(xs + (uint64_t)(6 * wave + j) * 256, buf + (6 * wave + j) * 1024);
+#pragma unroll
+	for (int j = 0; j < 8; ++j) // the wave's own 64 rows: 4 groups of 2 KB
+		dma16f(gs + (uint64_t)(8 * wave + j) * 256, buf + kT32XBytes + (8 * wave + j) * 1024);
+}
+
+__global__ __launch_bounds__(kB) void k_gemm_f32t(const float* __restrict__ G, const float* __restrict__ xq,
+                                                  const uint4* __restrict__ chunks,
+                                                  const uint32_t* __restrict__ seg_off,
+                                                  const uint32_t* __restrict__ tks, uint32_t Np,
+                                                  float* __restrict__ part)
+{
+	constexpr int RT = 4; // row tiles per wave
+	__shared__ __align__(1024) unsigned char lds0[kT32Stage], lds1[kT32Stage];
+	const int tid = threadIdx.x, lane = tid & 63;
+	const int r = lane & 15, g = lane >> 4;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	// XCD-aware order, as in k_gemm_mfma
+	const uint32_t nb = gridDim.x, x = blockIdx.x % 8, per = nb / 8, rem = nb % 8;
+	const uint32_t bid = x * per + (x < rem ? x : rem) + blockIdx.x / 8;
+	const int s0 = t32_slot(2 * g, r) * 16, s1 = t32_slot(2 * g + 1, r) * 16;
+	for (uint32_t q = seg_off[bid]; q < seg_off[bid + 1]; ++q) {
+	const uint4 ch = chunks[q];
+	const uint32_t i0 = ch.x * 256 + wave * 16 * RT;
+	const int nst = (int)(ch.z - ch.y);
+	const uint32_t* lst = tks + ch.y;
 
 	f32x4 acc[RT][kColTiles];
 #pragma unroll
@@ -547,18 +660,41 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 		for (int c = 0; c < kColTiles; ++c)
 			acc[t][c] = f32x4{0, 0, 0, 0};
 
-	if (nst > 0) { // a split without tiles writes zeros
-		// prologue: stages 0 .. PD-2 into buffers 0 .. PD-2
-		gemm_prologue<NPL, RT>(nst, L, G, Glo, xq, xlo, Np, std::make_integer_sequence<int, PD - 1>{});
-		int s = 0;
-		for (; s + PD <= nst; s += PD)
-			gemm_steps<P, NPL, RT, PD>(s, L, G, Glo, xq, xlo, Np, acc, nst, std::make_integer_sequence<int, PD>{});
-		// fewer than PD stages left
-		gemm_tail<P, NPL, RT, PD>(s, nst, L, G, Glo, xq, xlo, Np, acc, std::make_integer_sequence<int, PD - 1>{});
-		wait_vm<0>(); // no DMA may land after the block has left its LDS
+	for (int st = 0; st < nst; ++st) {
+		unsigned char* cur = (st & 1) ? lds1 : lds0;
+		unsigned char* nxt = (st & 1) ? lds0 : lds1;
+		if (st == 0)
+			t32_issue(G, xq, ch.y, lst, 0, cur, wave, lane);
+		wait_vm<0>();    // this wave's part of stage st has landed
+		block_barrier(); // every wave's part is in; stage st-1's buffer is free
+		float4 a[RT][2], b[kColTiles][2];
+#pragma unroll
+		for (int t = 0; t < RT; ++t) {
+			const unsigned char* rowp = cur + kT32XBytes + (wave * RT + t) * 2048 + r * 128;
+			a[t][0] = *reinterpret_cast<const float4*>(rowp + s0);
+			a[t][1] = *reinterpret_cast<const float4*>(rowp + s1);
+		}
+#pragma unroll
+		for (int c = 0; c < kColTiles; ++c) {
+			const unsigned char* colp = cur + c * 2048 + r * 128;
+			b[c][0] = *reinterpret_cast<const float4*>(colp + s0);
+			b[c][1] = *reinterpret_cast<const float4*>(colp + s1);
+		}
+		// every fragment is read before the next DMA is issued (see gemm_step)
+		if (st + 1 < nst)
+			t32_issue(G, xq, ch.y, lst, st + 1, nxt, wave, lane);
+#pragma unroll
+		for (int e = 0; e < 8; ++e)
+#pragma unroll
+			for (int t = 0; t < RT; ++t)
+#pragma unroll
+				for (int c = 0; c < kColTiles; ++c) {
+					const float av = e < 4 ? a[t][0][e] : a[t][1][e - 4];
+					const float bv = e < 4 ? b[c][0][e] : b[c][1][e - 4];
+					acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t][c], 0, 0, 0);
+				}
 	}
-
-	// part[slot][c][i]: column-major per slot
+	block_barrier(); // every wave is done with both buffers before the next segment refills them
 	float* out = part + (uint64_t)ch.w * kRPad * Np;
 #pragma unroll
 	for (int t = 0; t < RT; ++t)
@@ -568,12 +704,56 @@ __global__ __launch_bounds__(kB) void k_gemm_mfma(const uint16_t* __restrict__ G
 			*reinterpret_cast<float4*>(out + (uint64_t)col * Np + i) =
 			    make_float4(acc[t][c][0], acc[t][c][1], acc[t][c][2], acc[t][c][3]);
 		}
+	}
 }
 
-// the two instantiations: plain (64 rows per wave, 4 stages in flight) and
-// error-compensated hi + lo (32 rows per wave, 3 stages)
-constexpr int kPlainRT = 4, kPlainPD = 4, kSplitRT = 2, kSplitPD = 3;
-constexpr uint32_t kMaxSlots = 16; // partial-sum slots per 256-row block
+// FP32 tile store values from the symmetric CSR links (zeroed first); one
+// wave per row, as k_tile_scatter
+__global__ __launch_bounds__(kB) void k_tile_scatter_f32(uint32_t N, uint32_t Np, const uint32_t* __restrict__ rowptr,
+                                                         const uint32_t* __restrict__ col,
+                                                         const float* __restrict__ val,
+                                                         const uint32_t* __restrict__ pinv,
+                                                         const int32_t* __restrict__ tmap, float* __restrict__ G)
+{
+	const uint32_t i = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (i >= N)
+		return;
+	const uint32_t ip = pinv[i], rr = ip & 15;
+	const uint64_t rowbase = (uint64_t)(ip >> 8) * (Np / 32);
+	const uint64_t inrow = (uint64_t)((ip & 255) >> 4) * kT32Group + rr * 32;
+	for (uint32_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
+		const uint32_t jp = pinv[col[k]], kk = jp & 31;
+		const int32_t tile = tmap[rowbase + (jp >> 5)];
+		if (tile < 0)
+			continue; // not reached: the flags came from the same links
+		G[(uint64_t)tile * kT32Tile + inrow + t32_slot(kk >> 2, rr) * 4 + (kk & 3)] = val[k];
+	}
+}
+
+// FP32 operand of the tile form at position j (patch perm[j], Morton
+// order): (x[c][j] * diffuse[perm[j]][c % 3]) * area[perm[j]] (pre) or
+// x[c][j], in the LDS-image layout: stage j / 32, column group c / 16, row
+// c % 16, piece (j % 32) / 4 at slot piece ^ (c % 8), element j % 4
+__global__ __launch_bounds__(kB) void k_prep_x32t(const float* __restrict__ xT, const float* __restrict__ diffuse,
+                                                  const float* __restrict__ area, const uint32_t* __restrict__ perm,
+                                                  uint32_t N, uint32_t Np, uint32_t R, int pre, float* __restrict__ xq)
+{
+	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
+	if (t >= (uint64_t)kRPad * Np)
+		return;
+	const uint32_t c = (uint32_t)(t / Np), j = (uint32_t)(t % Np);
+	const uint32_t src = perm[j];
+	float v = 0.0f;
+	if (c < R && src < N) {
+		v = xT[(uint64_t)c * Np + j];
+		if (pre)
+			v = (v * diffuse[3 * src + c % 3]) * area[src];
+	}
+	const uint32_t kk = j & 31, cc = c & 15;
+	xq[((uint64_t)(j >> 5) * (kRPad / 16) + (c >> 4)) * kT32Group + cc * 32 + t32_slot(kk >> 2, cc) * 4 + (kk & 3)] =
+	    v;
+}
 
 // ---------------------------------------------------------------------------
 // plain FP32 GEMM with the same blocking: 256 threads, each 8 rows x 12
@@ -666,34 +846,65 @@ __global__ __launch_bounds__(kB) void k_reduce_split(const float* __restrict__ p
 }
 
 // ---------------------------------------------------------------------------
-// sparse gather over the symmetric CSR links, one wave per row, lanes over
-// columns: y[i][c] = sum_k ((x[j][c] * diffuse[j][c%3]) * area_j) * f_k
-// (PRE, the reference's operation order) or sum_k x[j][c] * f_k
+// sparse gather over the symmetric CSR links.  The operand is prepared once
+// per product, xp[j][c] = (x[j][c] * diffuse[j][c % 3]) * area_j (PRE, the
+// reference's operation order) or x[j][c]; then one wave per row, lanes
+// over columns (up to 3 per lane for R <= 192), the row's links taken 64 at
+// a time (one coalesced load of columns and factors, broadcast lane by lane):
+// y[i][c] = sum_k xp[col_k][c] * f_k in link order.
 // ---------------------------------------------------------------------------
 template <class T, bool PRE>
-__global__ __launch_bounds__(kB) void k_spmm(uint32_t N, uint32_t R, const uint32_t* __restrict__ rowptr,
-                                             const uint32_t* __restrict__ col, const float* __restrict__ val,
-                                             const float* __restrict__ diffuse, const float* __restrict__ area,
-                                             const T* __restrict__ x, T* __restrict__ y)
+__global__ __launch_bounds__(kB) void k_prep_rows(uint64_t n, uint32_t R, const T* __restrict__ x,
+                                                  const float* __restrict__ diffuse, const float* __restrict__ area,
+                                                  T* __restrict__ xp)
 {
-	uint32_t i = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
-	int lane = threadIdx.x & 63;
+	const uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
+	if (t >= n)
+		return;
+	if (PRE) {
+		const uint32_t j = (uint32_t)(t / R), c = (uint32_t)(t % R);
+		xp[t] = (x[t] * (T)diffuse[3 * j + c % 3]) * (T)area[j];
+	} else {
+		xp[t] = x[t];
+	}
+}
+
+template <class T>
+__global__ __launch_bounds__(kB) void k_gather(uint32_t N, uint32_t R, const uint32_t* __restrict__ rowptr,
+                                               const uint32_t* __restrict__ col, const float* __restrict__ val,
+                                               const T* __restrict__ xp, T* __restrict__ y)
+{
+	const uint32_t i = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
 	if (i >= N)
 		return;
-	for (uint32_t c0 = 0; c0 < R; c0 += 64) {
-		uint32_t c = c0 + lane;
-		T acc = 0;
-		if (c < R) {
-			for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-				uint32_t j = col[k];
-				if (PRE)
-					acc += ((x[(uint64_t)j * R + c] * (T)diffuse[3 * j + c % 3]) * (T)area[j]) * (T)val[k];
-				else
-					acc += x[(uint64_t)j * R + c] * (T)val[k];
-			}
-			y[(uint64_t)i * R + c] = acc;
+	const uint32_t c0 = lane, c1 = lane + 64, c2 = lane + 128;
+	const bool u0 = c0 < R, u1 = c1 < R, u2 = c2 < R;
+	T a0 = 0, a1 = 0, a2 = 0;
+	const uint32_t kb = rowptr[i], ke = rowptr[i + 1];
+	for (uint32_t k0 = kb; k0 < ke; k0 += 64) {
+		const uint32_t n = min(64u, ke - k0);
+		const uint32_t mj = (uint32_t)lane < n ? col[k0 + lane] : 0u;
+		const float mf = (uint32_t)lane < n ? val[k0 + lane] : 0.0f;
+		for (uint32_t q = 0; q < n; ++q) {
+			const uint32_t j = (uint32_t)__shfl((int)mj, (int)q);
+			const T f = (T)__shfl(mf, (int)q);
+			const T* row = xp + (uint64_t)j * R;
+			if (u0)
+				a0 += row[c0] * f;
+			if (u1)
+				a1 += row[c1] * f;
+			if (u2)
+				a2 += row[c2] * f;
 		}
 	}
+	T* out = y + (uint64_t)i * R;
+	if (u0)
+		out[c0] = a0;
+	if (u1)
+		out[c1] = a1;
+	if (u2)
+		out[c2] = a2;
 }
 
 // layout changes between the row-major (sparse) and column-major (dense) sets
@@ -810,9 +1021,11 @@ void RadSolver::release()
 	dfree(g32_);
 	dfree(g16_);
 	dfree(g16lo_);
+	dfree(g32t_);
 	dfree(tmap_);
 	dfree(tks_);
 	dfree(chunks_);
+	dfree(segoff_);
 	dfree(nslot_);
 	dfree(perm_);
 	dfree(pinv_);
@@ -826,6 +1039,8 @@ void RadSolver::release()
 	dfree(yT_);
 	dfree(rm32a_);
 	dfree(rm32b_);
+	dfree(rmx_);
+	dfree(rmx64_);
 	dfree(cmax_);
 }
 
@@ -836,6 +1051,8 @@ void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3,
 	if (R_ != R) {
 		dfree(rm32a_);
 		dfree(rm32b_);
+		dfree(rmx_);
+		dfree(rmx64_);
 	}
 	R = R_;
 	hipStream_t s = (hipStream_t)s_;
@@ -914,14 +1131,15 @@ void RadSolver::build_tiles()
 		HIPCHECK(hipMemcpyAsync(pinv_, pinv.data(), 4 * (size_t)Np, hipMemcpyHostToDevice, s));
 		HIPCHECK(hipStreamSynchronize(s));
 	}
-	uint8_t* df = dalloc<uint8_t>(n);
-	HIPCHECK(hipMemsetAsync(df, 0, n, s));
-	k_tile_flags<<<gridn(N, kB / 64), kB, 0, s>>>(N, Np, dev.rad.rowptr, dev.rad.col, pinv_, df);
-	HIPCHECK(hipGetLastError());
 	std::vector<uint8_t> f(n);
-	HIPCHECK(hipMemcpyAsync(f.data(), df, n, hipMemcpyDeviceToHost, s));
-	HIPCHECK(hipStreamSynchronize(s));
-	dfree(df);
+	{
+		DevBuf<uint8_t> df(n);
+		HIPCHECK(hipMemsetAsync(df, 0, n, s));
+		k_tile_flags<<<gridn(N, kB / 64), kB, 0, s>>>(N, Np, dev.rad.rowptr, dev.rad.col, pinv_, df);
+		HIPCHECK(hipGetLastError());
+		HIPCHECK(hipMemcpyAsync(f.data(), df, n, hipMemcpyDeviceToHost, s));
+		HIPCHECK(hipStreamSynchronize(s));
+	}
 	std::vector<int32_t> map(n, -1);
 	std::vector<uint32_t> off(nrb + 1, 0), ks;
 	uint32_t maxcnt = 0;
@@ -935,29 +1153,39 @@ void RadSolver::build_tiles()
 		maxcnt = std::max(maxcnt, off[rb + 1] - off[rb]);
 	}
 	ntiles_ = ks.size();
-	// Work chunks: a row block's list in runs of about ntiles / (2 CUs)
-	// tiles (at most kMaxSlots runs per block), so that the blocks with the
-	// most links do not set the product's time; each run writes its own
-	// partial-sum slot.
-	const uint64_t run = std::max<uint64_t>(
-	    {8, (ntiles_ + 2 * ncu_ - 1) / (2 * ncu_), ((uint64_t)maxcnt + kMaxSlots - 1) / kMaxSlots});
-	std::vector<uint32_t> ch, nsl(nrb, 0);
-	for (uint32_t rb = 0; rb < nrb; ++rb) {
-		const uint64_t cnt = off[rb + 1] - off[rb];
-		const uint32_t nch = (uint32_t)((cnt + run - 1) / run);
-		for (uint32_t q = 0; q < nch; ++q) {
-			ch.push_back(rb);
-			ch.push_back(off[rb] + (uint32_t)(cnt * q / nch));
-			ch.push_back(off[rb] + (uint32_t)(cnt * (q + 1) / nch));
-			ch.push_back(q);
+	// Work: the stored tiles in row-block order, split into nblocks_ equal
+	// contiguous shares, one per persistent block (the products hold one
+	// block per compute unit); a share is cut where it crosses from one row
+	// block to the next, and each piece (segment {row block, first tile, end
+	// tile, slot}) writes its own partial-sum slot of its row block.  Every
+	// block then multiplies the same number of tiles.
+	(void)maxcnt;
+	// at least kMinShare tiles per block: small stores use fewer blocks
+	// rather than many one-tile pieces (every piece is a partial-sum slot)
+	nblocks_ = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ncu_, (ntiles_ + kMinShare - 1) / kMinShare));
+	std::vector<uint32_t> ch, segoff(nblocks_ + 1, 0), nsl(nrb, 0);
+	for (uint32_t b = 0, rb = 0; b < nblocks_; ++b) {
+		uint64_t t0 = ntiles_ * b / nblocks_;
+		const uint64_t t1 = ntiles_ * (b + 1) / nblocks_;
+		segoff[b] = (uint32_t)(ch.size() / 4);
+		while (t0 < t1) {
+			while (off[rb + 1] <= t0)
+				++rb; // the row block holding tile t0 (empty ones skipped)
+			const uint64_t e = std::min<uint64_t>(t1, off[rb + 1]);
+			ch.insert(ch.end(), {rb, (uint32_t)t0, (uint32_t)e, nsl[rb]++});
+			t0 = e;
 		}
-		nsl[rb] = nch;
 	}
+	segoff[nblocks_] = (uint32_t)(ch.size() / 4);
+	maxslot_ = nsl.empty() ? 0 : *std::max_element(nsl.begin(), nsl.end());
+	ensure_part(); // the products write maxslot_ partial-sum planes
 	nchunks_ = ch.size() / 4;
 	tmap_ = dalloc<int32_t>(n);
 	tks_ = dalloc<uint32_t>(ntiles_);
 	chunks_ = dalloc<uint32_t>(ch.size());
+	segoff_ = dalloc<uint32_t>(segoff.size());
 	nslot_ = dalloc<uint32_t>(nrb);
+	HIPCHECK(hipMemcpyAsync(segoff_, segoff.data(), 4 * segoff.size(), hipMemcpyHostToDevice, s));
 	HIPCHECK(hipMemcpyAsync(tmap_, map.data(), 4 * n, hipMemcpyHostToDevice, s));
 	if (ntiles_)
 		HIPCHECK(hipMemcpyAsync(tks_, ks.data(), 4 * ntiles_, hipMemcpyHostToDevice, s));
@@ -968,16 +1196,30 @@ void RadSolver::build_tiles()
 	tiles_built_ = true;
 }
 
+// Storage of a form's matrix (the compensated forms multiply the 16-bit
+// stores with their remainder planes) and whether its product is the
+// error-compensated one.
+static int storage_of(int form) { return form == kF16x3 ? kDenseF16 : form == kBF16x3 ? kDenseBF16 : form; }
+static bool split_of(int form) { return form == kF16x3 || form == kBF16x3; }
+
 void RadSolver::build_dense(int form)
 {
 	hipStream_t s = (hipStream_t)s_;
+	const int st = storage_of(form);
 	uint64_t threads = (uint64_t)Np * (Np / 32);
 	Ev ev(s);
-	if (form == kDenseF32) {
+	if (st == kDenseF32) {
 		if (!g32_)
 			g32_ = dalloc<float>((uint64_t)Np * Np);
 		k_dense_build<<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos, dev.rad.nrm,
 		                                            g32_);
+	} else if (st == kTileF32) {
+		build_tiles();
+		if (!g32t_)
+			g32t_ = dalloc<float>(ntiles_ * kT32Tile);
+		HIPCHECK(hipMemsetAsync(g32t_, 0, 4 * ntiles_ * kT32Tile, s));
+		k_tile_scatter_f32<<<gridn(N, kB / 64), kB, 0, s>>>(N, Np, dev.rad.rowptr, dev.rad.col, dev.rad.val, pinv_,
+		                                                    tmap_, g32t_);
 	} else {
 		build_tiles();
 		if (!g16_)
@@ -987,16 +1229,15 @@ void RadSolver::build_dense(int form)
 		HIPCHECK(hipMemsetAsync(g16_, 0, 2 * ntiles_ * kGStageElems, s));
 		HIPCHECK(hipMemsetAsync(g16lo_, 0, 2 * ntiles_ * kGStageElems, s));
 		gscale_ = 1.0f;
-		if (form == kDenseF16) {
+		if (st == kDenseF16) {
 			// scale so that the largest factor sits near 2^13: small factors
 			// stay in FP16's normal range
-			unsigned* dm = dalloc<unsigned>(1);
+			DevBuf<unsigned> dm(1);
 			HIPCHECK(hipMemsetAsync(dm, 0, 4, s));
 			k_max_link<<<2048, kB, 0, s>>>(dev.rad.nnz, dev.rad.val, dm);
 			unsigned hm = 0;
 			HIPCHECK(hipMemcpyAsync(&hm, dm, 4, hipMemcpyDeviceToHost, s));
 			HIPCHECK(hipStreamSynchronize(s));
-			dfree(dm);
 			float mx;
 			memcpy(&mx, &hm, 4);
 			gscale_ = mx > 0 ? ldexpf(1.0f, 14 - ilogbf(mx) - 1) : 1.0f;
@@ -1009,7 +1250,7 @@ void RadSolver::build_dense(int form)
 	}
 	HIPCHECK(hipGetLastError());
 	build_ms = ev.stop();
-	built_form_ = form;
+	built_form_ = st;
 }
 
 uint32_t RadSolver::ksplit(uint32_t rows_per_block, uint32_t min_blocks) const
@@ -1021,13 +1262,25 @@ uint32_t RadSolver::ksplit(uint32_t rows_per_block, uint32_t min_blocks) const
 	return s;
 }
 
-// work buffers of the dense products (kept across solves)
+// Partial-sum planes (kRPad x Np floats each): as many as the dense FP32
+// form's k split or the tile products' most segments of one row block.
+void RadSolver::ensure_part()
+{
+	const uint32_t need = std::max({ksplit(kTileM, 2048), kMaxSlots, maxslot_});
+	if (part_ && need <= part_slots_)
+		return;
+	dfree(part_);
+	part_slots_ = 0;
+	part_ = dalloc<float>((uint64_t)need * kRPad * Np);
+	part_slots_ = need;
+}
+
+// work buffers of the products (kept across solves)
 void RadSolver::ensure_work()
 {
 	const uint64_t n = (uint64_t)kRPad * Np;
-	if (!part_) {
-		const uint32_t kmax = std::max(ksplit(kTileM, 2048), kMaxSlots);
-		part_ = dalloc<float>((uint64_t)kmax * n);
+	ensure_part();
+	if (!xq_) {
 		xq_ = dalloc<float>(n); // FP32 operand, or the 16-bit one in its first half
 		xqlo_ = dalloc<uint16_t>(n);
 		xT_ = dalloc<float>(n);
@@ -1037,18 +1290,41 @@ void RadSolver::ensure_work()
 	if (!rm32a_) {
 		rm32a_ = dalloc<float>((uint64_t)Np * R);
 		rm32b_ = dalloc<float>((uint64_t)Np * R);
+		rmx_ = dalloc<float>((uint64_t)Np * R);
+		rmx64_ = dalloc<double>((uint64_t)Np * R);
 	}
 }
 
+// y = K x (pre) or H x over the FP32 tile store (Morton order in and out)
+void RadSolver::apply_tiles_f32(const float* xT, float* yT, bool pre)
+{
+	hipStream_t s = (hipStream_t)s_;
+	const uint64_t nq = (uint64_t)kRPad * Np;
+	if (part_slots_ < maxslot_)
+		throw std::runtime_error("RadSolver: partial-sum planes fewer than the tile segments need");
+	k_prep_x32t<<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, perm_, N, Np, R, pre ? 1 : 0, xq_);
+	if (nchunks_)
+		k_gemm_f32t<<<nblocks_, kB, 0, s>>>(g32t_, xq_, reinterpret_cast<const uint4*>(chunks_), segoff_, tks_, Np,
+		                                     part_);
+	k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, 0, nslot_, Np, R, nullptr, 1.0f, yT);
+	HIPCHECK(hipGetLastError());
+}
+
 // y (column-major, R x Np, FP32) = K x (pre) or H x for the dense forms; the
-// 16-bit forms take and give their vectors in Morton order (position p holds
-// patch perm_[p]), the FP32 form in patch order.
+// tile forms take and give their vectors in Morton order (position p holds
+// patch perm_[p]), the dense FP32 form in patch order.
 // split: the error-compensated product of the 16-bit forms (hi + lo planes).
 void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bool pre)
 {
 	hipStream_t s = (hipStream_t)s_;
+	const int st = storage_of(form);
+	split = split || split_of(form);
 	uint64_t nq = (uint64_t)kRPad * Np;
-	if (form == kDenseF32) {
+	if (st == kTileF32) {
+		apply_tiles_f32(xT, yT, pre);
+		return;
+	}
+	if (st == kDenseF32) {
 		uint32_t ks = ksplit(kTileM, 2048);
 		dim3 grid(Np / kTileM, ks);
 		k_prep_x32<<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, N, Np, R, pre ? 1 : 0, xq_);
@@ -1058,7 +1334,7 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 		uint16_t* q = reinterpret_cast<uint16_t*>(xq_);
 		uint16_t* lo = split ? xqlo_ : nullptr;
 		const unsigned* cm = nullptr;
-		if (form == kDenseF16) {
+		if (st == kDenseF16) {
 			HIPCHECK(hipMemsetAsync(cmax_, 0, 4 * kRPad, s));
 			k_col_max<<<dim3(gridn(Np, 2048), R), kB, 0, s>>>(xT, diff_, area_, perm_, N, Np, pre ? 1 : 0, cmax_);
 			cm = cmax_;
@@ -1066,20 +1342,21 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 		} else {
 			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cm, perm_, N, Np, R, pre ? 1 : 0, q, lo);
 		}
+		if (part_slots_ < maxslot_)
+			throw std::runtime_error("RadSolver: partial-sum planes fewer than the tile segments need");
 		if (nchunks_) {
-			const uint32_t bpc = 256 / (64 * (split ? kSplitRT : kPlainRT));
-			const dim3 grid((unsigned)(nchunks_ * bpc));
+			const dim3 grid(nblocks_);
 			const uint4* chs = reinterpret_cast<const uint4*>(chunks_);
-			if (form == kDenseF16 && split)
-				k_gemm_mfma<1, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, chs, tks_, Np, part_);
-			else if (form == kDenseF16)
-				k_gemm_mfma<1, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, chs, tks_, Np,
-				                                                          part_);
+			if (st == kDenseF16 && split)
+				k_gemm_tiles<1, 2, kTilesD2><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, chs, segoff_, tks_, Np, part_);
+			else if (st == kDenseF16)
+				k_gemm_tiles<1, 1, kTilesD1><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, chs, segoff_, tks_, Np,
+				                                                 part_);
 			else if (split)
-				k_gemm_mfma<0, 2, kSplitRT, kSplitPD><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, chs, tks_, Np, part_);
+				k_gemm_tiles<0, 2, kTilesD2><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, chs, segoff_, tks_, Np, part_);
 			else
-				k_gemm_mfma<0, 1, kPlainRT, kPlainPD><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, chs, tks_, Np,
-				                                                          part_);
+				k_gemm_tiles<0, 1, kTilesD1><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, chs, segoff_, tks_, Np,
+				                                                 part_);
 		}
 		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, 0, nslot_, Np, R, cm, gscale_, yT);
 	}
@@ -1097,18 +1374,20 @@ void RadSolver::apply_op(int form, const double* x, double* y, bool pre)
 	const float* vl = dev.rad.val;
 	if (form == kSparseF64) {
 		if (pre)
-			k_spmm<double, true><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, x, y);
+			k_prep_rows<double, true><<<gridn(nR), kB, 0, s>>>(nR, R, x, diff_, area_, rmx64_);
 		else
-			k_spmm<double, false><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, x, y);
+			k_prep_rows<double, false><<<gridn(nR), kB, 0, s>>>(nR, R, x, diff_, area_, rmx64_);
+		k_gather<double><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, rmx64_, y);
 	} else if (form == kSparseF32) {
 		k_copy_cast<double, float><<<gridn(nR), kB, 0, s>>>(nR, x, rm32a_);
 		if (pre)
-			k_spmm<float, true><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, rm32a_, rm32b_);
+			k_prep_rows<float, true><<<gridn(nR), kB, 0, s>>>(nR, R, rm32a_, diff_, area_, rmx_);
 		else
-			k_spmm<float, false><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, diff_, area_, rm32a_, rm32b_);
+			k_prep_rows<float, false><<<gridn(nR), kB, 0, s>>>(nR, R, rm32a_, diff_, area_, rmx_);
+		k_gather<float><<<sgrid, kB, 0, s>>>(N, R, rp, cl, vl, rmx_, rm32b_);
 		k_copy_cast<float, double><<<gridn(nR), kB, 0, s>>>(nR, rm32b_, y);
 	} else {
-		const bool mo = form != kDenseF32; // Morton order
+		const bool mo = storage_of(form) != kDenseF32; // Morton order
 		k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(x, N, Np, R, mo ? perm_ : nullptr, xT_);
 		apply_dense(form, xT_, yT_, false, pre);
 		k_to_rowmajor<float, double><<<gridn(nR), kB, 0, s>>>(yT_, N, Np, R, mo ? pinv_ : nullptr, y);
@@ -1118,15 +1397,19 @@ void RadSolver::apply_op(int form, const double* x, double* y, bool pre)
 
 void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 {
+	if (form < 0 || form >= kNumForms)
+		throw std::runtime_error("RadSolver: unknown form " + std::to_string(form));
+	if (refine > 0 && form != kDenseBF16 && form != kDenseF16)
+		throw std::runtime_error("RadSolver: refinement sweeps apply to the plain 16-bit forms (3, 4)");
 	hipStream_t s = (hipStream_t)s_;
 	uint64_t nR = (uint64_t)N * R, nRp = (uint64_t)R * Np;
 	times = RadTimes{};
 	ensure_work();
 	// FP64 running total
-	double* tot = dalloc<double>(nR);
+	DevBuf<double> tot(nR);
 	HIPCHECK(hipMemcpyAsync(tot, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
 	if (form == kSparseF64 || form == kSparseF32) {
-		double *x = dalloc<double>(nR), *y = dalloc<double>(nR);
+		DevBuf<double> x(nR), y(nR);
 		HIPCHECK(hipMemcpyAsync(x, E_, 8 * nR, hipMemcpyDeviceToDevice, s));
 		Ev ev(s);
 		for (int b = 0; b < bounces; ++b) {
@@ -1136,19 +1419,18 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		}
 		times.series_ms = ev.stop();
 		times.per_bounce_ms = times.series_ms / std::max(1, bounces);
-		dfree(x);
-		dfree(y);
 	} else {
-		if (built_form_ != form)
+		if (built_form_ != storage_of(form))
 			build_dense(form);
 		times.build_ms = build_ms;
 		// z_k (column-major FP32), k = 0..b
-		std::vector<float*> z(bounces + 1);
-		for (auto& p : z)
-			p = dalloc<float>(nRp);
-		// the 16-bit forms keep z in Morton order
-		const uint32_t* po = form != kDenseF32 ? perm_ : nullptr;
-		const uint32_t* pi = form != kDenseF32 ? pinv_ : nullptr;
+		std::vector<DevBuf<float>> z;
+		for (int b = 0; b <= bounces; ++b)
+			z.emplace_back(nRp);
+		// the tile forms keep z in Morton order
+		const bool mo = storage_of(form) != kDenseF32;
+		const uint32_t* po = mo ? perm_ : nullptr;
+		const uint32_t* pi = mo ? pinv_ : nullptr;
 		k_to_colmajor<double, float><<<gridn(nRp), kB, 0, s>>>(E_, N, Np, R, po, z[0]);
 		{
 			Ev ev(s);
@@ -1163,18 +1445,15 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		// the difference taken in FP64; corrections d_b = Kdense d_{b-1} + rho_b
 		// (d_0 = 0: z_0 = E is exact) with the plain 16-bit product; then
 		// z += d.  z_{b-1} is corrected only after rho_b has used its old value.
-		const bool half = form != kDenseF32;
-		double* zr = dalloc<double>(nR);
-		double* rho = refine > 0 ? dalloc<double>(nRp) : nullptr;
-		float* kz = refine > 0 ? dalloc<float>(nRp) : nullptr;
-		float* d = refine > 0 ? dalloc<float>(nRp) : nullptr;
-		float* dprev = refine > 0 ? dalloc<float>(nRp) : nullptr;
-		{
+		DevBuf<double> zr(nR);
+		if (refine > 0) {
+			DevBuf<double> rho(nRp);
+			DevBuf<float> kz(nRp), d(nRp), dprev(nRp);
 			Ev ev(s);
 			for (int it = 0; it < refine; ++it) {
 				HIPCHECK(hipMemsetAsync(dprev, 0, 4 * nRp, s));
 				for (int b = 1; b <= bounces; ++b) {
-					apply_dense(form, z[b - 1], kz, half, true);                          // K z_{b-1}
+					apply_dense(form, z[b - 1], kz, true, true);                          // K z_{b-1}
 					k_resid64<<<gridn(nRp), kB, 0, s>>>(nRp, kz, z[b], rho);             // rho_b (FP64)
 					apply_dense(form, dprev, d, false, true);                             // Kdense d_{b-1}
 					k_axpy<double, float><<<gridn(nRp), kB, 0, s>>>(nRp, rho, d);        // d_b
@@ -1191,18 +1470,10 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 			k_to_rowmajor<float, double><<<gridn(nR), kB, 0, s>>>(z[b], N, Np, R, pi, zr);
 			k_axpy<double, double><<<gridn(nR), kB, 0, s>>>(nR, zr, tot);
 		}
-		for (auto& p : z)
-			dfree(p);
-		dfree(zr);
-		dfree(kz);
-		dfree(rho);
-		dfree(d);
-		dfree(dprev);
 	}
 	HIPCHECK(hipGetLastError());
 	HIPCHECK(hipMemcpyAsync(total_out, tot, 8 * nR, hipMemcpyDeviceToHost, s));
 	HIPCHECK(hipStreamSynchronize(s));
-	dfree(tot);
 }
 
 // Reverse mode of T = sum_{k=0..B} K^k E, K = H diag(area) diag(diffuse_c).
@@ -1212,26 +1483,24 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 //   dalb += area * u_k * S_{k-1}
 //   ybar_{k-1} = diffuse_c * area * u_k
 //   lam = sum_k ybar_k        (= dL/dE; also dL/d emission)
+// Every product, forward and reverse, is the form's own (the compensated
+// forms' three MFMA products included).
 void RadSolver::solve_grad(int form, int bounces, const double* g, double* total_out, double* lam_out,
                            double* dalb_out)
 {
+	if (form < 0 || form >= kNumForms)
+		throw std::runtime_error("RadSolver: unknown form " + std::to_string(form));
 	hipStream_t s = (hipStream_t)s_;
 	const uint64_t nR = (uint64_t)N * R;
 	times = RadTimes{};
 	ensure_work();
-	if (form >= kDenseF32 && built_form_ != form)
+	if (form >= kDenseF32 && built_form_ != storage_of(form))
 		build_dense(form);
 	const int B = std::max(0, bounces);
-	std::vector<double*> S(std::max(1, B));
-	for (auto& p : S)
-		p = dalloc<double>(nR);
-	double* z = dalloc<double>(nR);
-	double* zn = dalloc<double>(nR);
-	double* tot = dalloc<double>(nR);
-	double* ybar = dalloc<double>(nR);
-	double* u = dalloc<double>(nR);
-	double* lam = dalloc<double>(nR);
-	double* dalb = dalloc<double>(nR);
+	std::vector<DevBuf<double>> S;
+	for (int k = 0; k < std::max(1, B); ++k)
+		S.emplace_back(nR);
+	DevBuf<double> z(nR), zn(nR), tot(nR), ybar(nR), u(nR), lam(nR), dalb(nR);
 	Ev ev(s);
 	// forward: S_0 = E, z_k = K z_{k-1}, S_k = S_{k-1} + z_k
 	HIPCHECK(hipMemcpyAsync(S[0], E_, 8 * nR, hipMemcpyDeviceToDevice, s));
@@ -1261,15 +1530,6 @@ void RadSolver::solve_grad(int form, int bounces, const double* g, double* total
 	HIPCHECK(hipMemcpyAsync(lam_out, lam, 8 * nR, hipMemcpyDeviceToHost, s));
 	HIPCHECK(hipMemcpyAsync(dalb_out, dalb, 8 * nR, hipMemcpyDeviceToHost, s));
 	HIPCHECK(hipStreamSynchronize(s));
-	for (auto& p : S)
-		dfree(p);
-	dfree(z);
-	dfree(zn);
-	dfree(tot);
-	dfree(ybar);
-	dfree(u);
-	dfree(lam);
-	dfree(dalb);
 }
 
 } // namespace lgx

@@ -15,7 +15,8 @@ shadowed pairs (a step closer than 0.001) are the same for k = 0, k and 2k,
 k = 0 gives 0 or 1, and the factor does not grow with k.
 Radiosity over the city's first 65,536 texels (one 62.5 m tile), one RGB
 setup: every form against the oracle's FP64 series on the same links; the
-refined BF16 and FP16 MFMA forms at or below 1e-4.
+refined BF16 and FP16 MFMA forms and the compensated (hi + lo) forms at or
+below 1e-4, the FP32 forms at or below 1e-5.
 Colour gradient of an L2 loss over every texel against central differences
 of the GPU bake on 20 light intensities, to 1e-3.
 """
@@ -218,7 +219,7 @@ def city_radiosity(lib, city):
 
 
 @pytest.mark.parametrize("form,refine,tol", [(0, 0, 1e-12), (1, 0, 1e-5), (2, 0, 1e-5), (3, 1, 1e-4),
-                                             (4, 1, 1e-4)])
+                                             (4, 1, 1e-4), (5, 0, 1e-5), (6, 0, 1e-5), (7, 0, 1e-4)])
 def test_c4_radiosity_tile_against_fp64_oracle(lib, orc, city_radiosity, form, refine, tol):
     """sum_{k<=4} K^k E over the 65,536-patch tile (12 M links) in each form
     against the oracle's FP64 series on the same link list."""

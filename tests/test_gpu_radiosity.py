@@ -4,8 +4,10 @@ total = sum_{k=0..b} K^k E, K = F diag(area) diag(diffuse_{c%3}) (closed form
 of lighter.cpp:767-797, 1104-1123) for many right-hand sides.  Tolerances
 (relative L2 error against FP64), written per form:
   sparse FP64 1e-12, sparse FP32 1e-5, dense FP32 1e-5,
-  dense BF16 (MFMA) 3e-2 unrefined / 1e-4 after one refinement sweep,
-  dense FP16 (MFMA) 3e-3 unrefined / 1e-5 after one refinement sweep.
+  BF16 tiles (MFMA) 3e-2 unrefined / 1e-4 after one refinement sweep,
+  FP16 tiles (MFMA) 3e-3 unrefined / 1e-5 after one refinement sweep,
+  FP32 tiles (FP32 matrix cores) 1e-5,
+  FP16 x 3 (compensated product every bounce) 1e-5, BF16 x 3 1e-4.
 """
 import ctypes as C
 
@@ -14,18 +16,17 @@ import pytest
 
 from helpers import Device, P, all_samples_and_tris, prepared
 from lighter_amd import _lib, scenes
-from oracle.checkers import normalize_f32
 
 pytestmark = pytest.mark.gpu
 
-TOL = {0: 1e-12, 1: 1e-5, 2: 1e-5, 3: 3e-2, 4: 3e-3}
+TOL = {0: 1e-12, 1: 1e-5, 2: 1e-5, 3: 3e-2, 4: 3e-3, 5: 1e-5, 6: 1e-5, 7: 1e-4}
 TOL_REFINED = {3: 1e-4, 4: 1e-5}
 
 
 def scene_links(lib, spec):
     b = prepared(lib, spec)
     pos, nrm, tris = all_samples_and_tris(lib, b)
-    pn = np.array([normalize_f32(v) for v in nrm], np.float32)
+    pn = scenes.normalize_rows_f32(nrm)
     dev = Device(lib, tris)
     nl = C.c_size_t(0)
     _lib.check(lib.lgx_RadLinks(dev.h, P(pos), P(pn), len(pos), C.byref(nl)), lib)
@@ -69,7 +70,7 @@ def solve(lib, dev, E, diff, area, form, bounces, refine=0):
     return out, list(t)
 
 
-@pytest.mark.parametrize("form", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("form", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_solver_forms_against_fp64(lib, orc, links, form):
     dev, N, rowmap, other, fac = links
     rng = np.random.default_rng(30 + form)
@@ -101,7 +102,7 @@ def test_full_width_192_columns(lib, orc, links):
     area = rng.uniform(0.005, 0.05, N).astype(np.float32)
     E = rng.uniform(0, 1, (N, R))
     exp = orc.rad_series_f64(rowmap, other, fac, area, diff, E, 3)
-    for form, tol in [(2, 1e-5), (3, 3e-2), (4, 3e-3)]:
+    for form, tol in [(1, 1e-5), (2, 1e-5), (3, 3e-2), (4, 3e-3), (5, 1e-5), (6, 1e-5), (7, 1e-4)]:
         got, _ = solve(lib, dev, E, diff, area, form, 3)
         assert np.linalg.norm(got - exp) / np.linalg.norm(exp) < tol, form
 
@@ -125,7 +126,7 @@ def tiles_holding_links(N, rowmap, other, order):
     return len(np.unique(t)), (Np // 256) * nks
 
 
-@pytest.mark.parametrize("form", [3, 4])
+@pytest.mark.parametrize("form", [3, 4, 5, 6, 7])
 def test_tile_skipping_on_a_city_block(lib, orc, city_links, form):
     """The 16-bit forms multiply only the tiles that hold links; on a city
     block most tiles are empty, and the series and its refinement still meet
@@ -148,9 +149,10 @@ def test_tile_skipping_on_a_city_block(lib, orc, city_links, form):
     # the Morton order stores fewer tiles than the patches' own order would
     unordered, total2 = tiles_holding_links(N, rowmap, other, np.arange(N))
     assert total == total2 and 0 < stored < unordered and stored < 0.5 * total, (stored, unordered, total)
-    got1, _ = solve(lib, dev, E, diff, area, form, bounces, refine=1)
-    rel1 = np.linalg.norm(got1 - exp) / np.linalg.norm(exp)
-    assert rel1 < TOL_REFINED[form], (form, rel, rel1)
+    if form in TOL_REFINED:
+        got1, _ = solve(lib, dev, E, diff, area, form, bounces, refine=1)
+        rel1 = np.linalg.norm(got1 - exp) / np.linalg.norm(exp)
+        assert rel1 < TOL_REFINED[form], (form, rel, rel1)
     # rows without links keep their emission exactly
     lonely = np.ones(N, bool)
     lonely[np.repeat(np.arange(N), rowmap[1::2])] = False
@@ -177,8 +179,8 @@ def test_patches_without_links_keep_their_emission(lib):
         E = g.uniform(0, 1, (n, 5))
         diff = np.full((n, 3), 0.5, np.float32)
         area = np.full(n, 0.1, np.float32)
-        for form in range(5):
-            got, _ = solve(lib, dev, E, diff, area, form, 3, refine=1 if form >= 2 else 0)
+        for form in range(8):
+            got, _ = solve(lib, dev, E, diff, area, form, 3, refine=1 if form in (3, 4) else 0)
             assert np.array_equal(got, E), form
         assert stored_tiles(lib, dev)[0] == 0
     finally:
@@ -200,8 +202,8 @@ def test_zero_and_one_patch(lib, n):
         E = np.full((n, 3), 0.25)
         diff = np.full((max(n, 1), 3), 0.5, np.float32)
         area = np.full(max(n, 1), 0.1, np.float32)
-        for form in range(5):
-            got, _ = solve(lib, dev, E, diff, area, form, 2, refine=1 if form >= 2 else 0)
+        for form in range(8):
+            got, _ = solve(lib, dev, E, diff, area, form, 2, refine=1 if form in (3, 4) else 0)
             assert got.shape == (n, 3) and np.array_equal(got, E), form
     finally:
         dev.close()
@@ -221,7 +223,7 @@ def test_every_link_equals_the_canonical_any_hit(lib, name):
                                  n_lights=2)
     b = prepared(lib, spec)
     pos, nrm, tris = all_samples_and_tris(lib, b)
-    pn = np.array([normalize_f32(v) for v in nrm], np.float32)
+    pn = scenes.normalize_rows_f32(nrm)
     dev = Device(lib, tris)
     nl = C.c_size_t(0)
     lib.lgx_SetCounting(dev.h, 1)

@@ -54,8 +54,11 @@ private:
 	void build_tiles();
 	void ensure_work();
 	void ensure_part();
+	void ensure_perm_colors();
 	uint32_t ksplit(uint32_t rows_per_block, uint32_t min_blocks) const;
-	void apply_dense(int form, const float* xT, float* yT, bool split, bool pre);
+	// chain: the next call is apply_dense(form, yT, .., split, pre), so the
+	// reduction prepares its operand (BF16) or its column scales (FP16)
+	void apply_dense(int form, const float* xT, float* yT, bool split, bool pre, bool chain = false);
 	void apply_tiles_f32(const float* xT, float* yT, bool pre);
 	void apply_op(int form, const double* x, double* y, bool pre);
 	DeviceScene& dev;
@@ -92,7 +95,13 @@ private:
 	float* rm32b_ = nullptr;
 	float* rmx_ = nullptr;  // sparse FP32 forms: the operand (x diffuse) area prepared once per product
 	double* rmx64_ = nullptr; // the same for the FP64 form
-	unsigned* cmax_ = nullptr; // column maxima of the FP16 operand (its scales), per product
+	unsigned* cmax_ = nullptr; // column maxima of the FP16 operand (its scales): two planes of kRPad
+	int cmax_cur_ = 0;         // the plane of the current operand
+	const float* chain_x_ = nullptr; // operand (or its scales) already prepared from this vector ..
+	int chain_key_ = -1;             // .. for this storage / split / pre
+	float* dperm_ = nullptr;   // diffuse at each Morton position (3 per position, padding zero)
+	float* aperm_ = nullptr;   // area at each Morton position
+	bool pcol_ok_ = false;     // dperm_ / aperm_ hold the current problem's colours
 };
 
 } // namespace lgx

@@ -247,65 +247,144 @@ __device__ __forceinline__ float col_scale_of(const unsigned* cmax, uint32_t c)
 	return (m > 0.0f && m < INFINITY) ? ldexpf(1.0f, 14 - ilogbf(m) - 1) : 1.0f;
 }
 
-// Column maxima of the operand before scaling: block (x, c) takes positions
-// x * 2048 .. of column c.  A maximum does not depend on the order of the
-// atomics, so the result is deterministic; cmax starts at zero.
-__global__ __launch_bounds__(kB) void k_col_max(const float* __restrict__ xT, const float* __restrict__ diffuse,
-                                                const float* __restrict__ area, const uint32_t* __restrict__ perm,
-                                                uint32_t N, uint32_t Np, int pre, unsigned* __restrict__ cmax)
+// Diffuse and area of the patch at each Morton position (padding: zero).
+__global__ __launch_bounds__(kB) void k_permute_colors(const float* __restrict__ diffuse,
+                                                       const float* __restrict__ area,
+                                                       const uint32_t* __restrict__ perm, uint32_t N, uint32_t Np,
+                                                       float* __restrict__ dperm, float* __restrict__ aperm)
 {
-	const uint32_t c = blockIdx.y, p0 = blockIdx.x * 2048, p1 = min(Np, p0 + 2048);
-	float m = 0.0f;
-	for (uint32_t p = p0 + threadIdx.x; p < p1; p += kB) {
-		const uint32_t src = perm[p];
-		if (src < N) {
-			float v = xT[(uint64_t)c * Np + p];
-			if (pre)
-				v = (v * diffuse[3 * src + c % 3]) * area[src];
-			m = fmaxf(m, fabsf(v));
-		}
-	}
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1)
-		m = fmaxf(m, __shfl_xor(m, o));
-	if ((threadIdx.x & 63) == 0)
-		atomicMax(cmax + c, __float_as_uint(m)); // non-negative floats order like their bits
+	const uint32_t p = blockIdx.x * kB + threadIdx.x;
+	if (p >= Np)
+		return;
+	const uint32_t src = perm[p];
+	const bool in = src < N;
+	dperm[3 * p + 0] = in ? diffuse[3 * src + 0] : 0.0f;
+	dperm[3 * p + 1] = in ? diffuse[3 * src + 1] : 0.0f;
+	dperm[3 * p + 2] = in ? diffuse[3 * src + 2] : 0.0f;
+	aperm[p] = in ? area[src] : 0.0f;
 }
 
 // ---------------------------------------------------------------------------
-// right-hand-side preparation at position j (patch perm[j], Morton order):
-// Xq[c][j] = (x[c][j] * diffuse[perm[j]][c % 3]) * area[perm[j]] * s_c for
-// K = H diag(area) diag(diffuse_c) (pre), or x[c][j] * s_c for the bare H
-// (the adjoint sweep); s_c is the column scale
+// Operand passes of the 16-bit tile products, one kernel for three jobs:
+//   source   the column-major FP32 vector x (positions in Morton order), or
+//            the sum of a product's partial-sum planes (slots 0 .. nslot of
+//            the row block, in slot order) divided by the product's scales;
+//   y        that source written out (the product's result);
+//   cmax_out column maxima of |w| by atomicMax (non-negative floats order
+//            like their bits, so the result does not depend on the order of
+//            the atomics; the buffer starts at zero);
+//   xq / xlo w times the column scale from cmax_op, rounded to 16 bits, and
+//            the remainder, as LDS images: stage j / 32, column group c / 16,
+//            row c % 16, piece (j % 32) / 8 at slot piece ^ piece_swz(c % 16);
+// where w = (x[c][j] * diffuse_j[c % 3]) * area_j for K = H diag(area)
+// diag(diffuse_c) (pre), or x[c][j] for the bare H (the adjoint sweep).  The
+// padding positions have zero diffuse and area (with pre off they hold zero
+// in x); columns R .. 191 are zero.
+// Block: one 16-column group x one 256-position row block; wave: two k
+// stages; lane: column r = lane >> 2 of the group, piece pc = lane & 3, so a
+// wave writes one contiguous 1 KB group per stage.  Fusing the reduction of
+// one product with the operand of the next (the series z_b = K z_{b-1})
+// saves a pass over z: the BF16 forms take the operand straight from the
+// reduction, the FP16 forms the column maxima that set its scales.
 // ---------------------------------------------------------------------------
+struct OperandPass {
+	const float* part = nullptr;      // partial-sum planes (null: read x)
+	const uint32_t* nslot = nullptr;  // slots per row block
+	const unsigned* cmax_in = nullptr; // the product's operand scales (FP16)
+	float gscale = 1.0f;              // the product's matrix scale
+	const float* x = nullptr;
+	float* y = nullptr;
+	const float* dperm = nullptr;
+	const float* aperm = nullptr;
+	int pre = 0;
+	unsigned* cmax_out = nullptr;
+	const unsigned* cmax_op = nullptr; // scales of the operand written here (FP16)
+	uint16_t* xq = nullptr;
+	uint16_t* xlo = nullptr;
+	uint32_t Np = 0, R = 0;
+};
+
 template <int P>
-__global__ __launch_bounds__(kB) void k_prep_x16(const float* __restrict__ xT, const float* __restrict__ diffuse,
-                                                 const float* __restrict__ area, const unsigned* __restrict__ cmax,
-                                                 const uint32_t* __restrict__ perm, uint32_t N, uint32_t Np,
-                                                 uint32_t R, int pre, uint16_t* __restrict__ xq,
-                                                 uint16_t* __restrict__ xlo)
+__global__ __launch_bounds__(kB) void k_operand(const OperandPass a)
 {
-	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
-	if (t >= (uint64_t)kRPad * Np)
-		return;
-	uint32_t c = (uint32_t)(t / Np), j = (uint32_t)(t % Np);
-	const uint32_t src = perm[j];
-	float v = 0.0f;
-	if (c < R && src < N) {
-		v = xT[(uint64_t)c * Np + j];
-		if (pre)
-			v = (v * diffuse[3 * src + c % 3]) * area[src];
-		v *= col_scale_of(cmax, c);
+	const uint32_t cg = blockIdx.x % (kRPad / 16), rb = blockIdx.x / (kRPad / 16);
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane >> 2, pc = lane & 3;
+	const uint32_t c = cg * 16 + r;
+	const bool live = c < a.R;
+	const uint64_t cbase = (uint64_t)c * a.Np;
+	float inv = 1.0f;
+	uint32_t ns = 0;
+	if (a.part) {
+		ns = a.nslot[rb];
+		inv = 1.0f / (col_scale_of(a.cmax_in, c) * a.gscale); // powers of two: exact
 	}
-	uint16_t h = Prec<P>::cvt(v);
-	// LDS-image layout: stage j / 32, column group c / 16, row c % 16, piece
-	// (j % 32) / 8 at slot piece ^ piece_swz(c % 16), element j % 8
-	const uint32_t kk = j & 31, r = c & 15;
-	const uint64_t o = (((uint64_t)(j >> 5) * (kRPad / 16) + (c >> 4)) * 64 + r * 4 + ((kk >> 3) ^ piece_swz(r))) * 8 +
-	                   (kk & 7);
-	xq[o] = h;
-	if (xlo)
-		xlo[o] = Prec<P>::cvt(v - from16<P>(h));
+	const float sc = a.xq ? col_scale_of(a.cmax_op, c) : 1.0f;
+	float m = 0.0f;
+#pragma unroll
+	for (int k = 0; k < 2; ++k) {
+		const uint32_t stage = rb * 8 + wave * 2 + k, j0 = stage * 32 + pc * 8;
+		float v[8];
+#pragma unroll
+		for (int e = 0; e < 8; ++e)
+			v[e] = 0.0f;
+		if (a.part) {
+			for (uint32_t q = 0; q < ns; ++q) {
+				const float* src = a.part + (uint64_t)q * kRPad * a.Np + cbase + j0;
+				const float4 u0 = *reinterpret_cast<const float4*>(src);
+				const float4 u1 = *reinterpret_cast<const float4*>(src + 4);
+				v[0] += u0.x, v[1] += u0.y, v[2] += u0.z, v[3] += u0.w;
+				v[4] += u1.x, v[5] += u1.y, v[6] += u1.z, v[7] += u1.w;
+			}
+#pragma unroll
+			for (int e = 0; e < 8; ++e)
+				v[e] *= inv;
+		} else if (live) {
+			const float4 u0 = *reinterpret_cast<const float4*>(a.x + cbase + j0);
+			const float4 u1 = *reinterpret_cast<const float4*>(a.x + cbase + j0 + 4);
+			v[0] = u0.x, v[1] = u0.y, v[2] = u0.z, v[3] = u0.w, v[4] = u1.x, v[5] = u1.y, v[6] = u1.z, v[7] = u1.w;
+		}
+		if (a.y && live) {
+			*reinterpret_cast<float4*>(a.y + cbase + j0) = make_float4(v[0], v[1], v[2], v[3]);
+			*reinterpret_cast<float4*>(a.y + cbase + j0 + 4) = make_float4(v[4], v[5], v[6], v[7]);
+		}
+		if (!a.xq && !a.cmax_out)
+			continue;
+#pragma unroll
+		for (int e = 0; e < 8; ++e) {
+			if (!live)
+				v[e] = 0.0f;
+			else if (a.pre)
+				v[e] = (v[e] * a.dperm[3 * (j0 + e) + c % 3]) * a.aperm[j0 + e];
+			m = fmaxf(m, fabsf(v[e]));
+		}
+		if (a.xq) {
+			uint32_t h[4], l[4];
+#pragma unroll
+			for (int e = 0; e < 8; e += 2) {
+				const float w0 = v[e] * sc, w1 = v[e + 1] * sc;
+				const uint16_t h0 = Prec<P>::cvt(w0), h1 = Prec<P>::cvt(w1);
+				h[e / 2] = h0 | (uint32_t)h1 << 16;
+				l[e / 2] = Prec<P>::cvt(w0 - from16<P>(h0)) | (uint32_t)Prec<P>::cvt(w1 - from16<P>(h1)) << 16;
+			}
+			const uint64_t o = (((uint64_t)stage * (kRPad / 16) + cg) * 64 + r * 4 + (pc ^ piece_swz(r))) * 8;
+			*reinterpret_cast<uint4*>(a.xq + o) = make_uint4(h[0], h[1], h[2], h[3]);
+			if (a.xlo)
+				*reinterpret_cast<uint4*>(a.xlo + o) = make_uint4(l[0], l[1], l[2], l[3]);
+		}
+	}
+	if (a.cmax_out) {
+		m = fmaxf(m, __shfl_xor(m, 1));
+		m = fmaxf(m, __shfl_xor(m, 2));
+		__shared__ float wm[4][16];
+		if (pc == 0)
+			wm[wave][r] = m;
+		__syncthreads();
+		if (threadIdx.x < 16) {
+			const float b = fmaxf(fmaxf(wm[0][threadIdx.x], wm[1][threadIdx.x]), fmaxf(wm[2][threadIdx.x], wm[3][threadIdx.x]));
+			if (b > 0.0f)
+				atomicMax(a.cmax_out + cg * 16 + threadIdx.x, __float_as_uint(b));
+		}
+	}
 }
 
 // FP32 operand of the dense FP32 form, patch order: Xq[c][j] =
@@ -342,7 +421,7 @@ constexpr int kKS = 32; // k per stage
 constexpr int kMaxPD = 6; // deepest LDS ring
 
 // The 16-bit matrix and operand are stored as LDS images (see k_tile_scatter
-// / k_prep_x16): 1 KB groups of 16 rows x 64 bytes.
+// / k_operand): 1 KB groups of 16 rows x 64 bytes.
 constexpr uint64_t kTileElems = 512;                         // 16-bit elements per 1 KB group
 constexpr uint64_t kGStageElems = 16 * kTileElems;           // a stored 256-row x 32 tile
 constexpr uint64_t kXStageElems = (kRPad / 16) * kTileElems; // an operand stage (192 x 32)
@@ -406,7 +485,9 @@ __device__ __forceinline__ void dma16(const uint16_t* src, unsigned char* dst)
 // (every wave is done with the slot), copy stage s + D into the slot,
 // multiply, load stage s + D's matrix fragments into the registers just
 // used.  Past the end the last stage is fetched again (never used), which
-// keeps every stage's wait count exact.
+// keeps every stage's wait count exact.  The matrix is read once per
+// product, so its loads are non-temporal: they leave the L2 to the operand
+// stages, which every row block of the same k stage reads again.
 // ---------------------------------------------------------------------------
 template <int NPL, int D>
 struct TileLane {
@@ -451,7 +532,7 @@ __device__ __forceinline__ void tile_load_matrix(const TileLane<NPL, D>& L, uint
 		const uint16_t* M = pl ? L.Glo : L.G;
 #pragma unroll
 		for (int t = 0; t < 4; ++t)
-			a[pl][t] = *reinterpret_cast<const i16x8*>(M + base + t * kTileElems);
+			a[pl][t] = __builtin_nontemporal_load(reinterpret_cast<const i16x8*>(M + base + t * kTileElems));
 	}
 }
 
@@ -731,28 +812,32 @@ __global__ __launch_bounds__(kB) void k_tile_scatter_f32(uint32_t N, uint32_t Np
 	}
 }
 
-// FP32 operand of the tile form at position j (patch perm[j], Morton
-// order): (x[c][j] * diffuse[perm[j]][c % 3]) * area[perm[j]] (pre) or
+// FP32 operand of the tile form at Morton positions j .. j + 3 of column c,
+// one 16-byte piece per lane: (x[c][j] * diffuse_j[c % 3]) * area_j (pre) or
 // x[c][j], in the LDS-image layout: stage j / 32, column group c / 16, row
-// c % 16, piece (j % 32) / 4 at slot piece ^ (c % 8), element j % 4
-__global__ __launch_bounds__(kB) void k_prep_x32t(const float* __restrict__ xT, const float* __restrict__ diffuse,
-                                                  const float* __restrict__ area, const uint32_t* __restrict__ perm,
-                                                  uint32_t N, uint32_t Np, uint32_t R, int pre, float* __restrict__ xq)
+// c % 16, piece (j % 32) / 4 at slot piece ^ (c % 8)
+__global__ __launch_bounds__(kB) void k_prep_x32t(const float* __restrict__ xT, const float* __restrict__ dperm,
+                                                  const float* __restrict__ aperm, uint32_t Np, uint32_t R, int pre,
+                                                  float* __restrict__ xq)
 {
-	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
-	if (t >= (uint64_t)kRPad * Np)
+	const uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
+	const uint32_t pieces = Np / 4;
+	if (t >= (uint64_t)kRPad * pieces)
 		return;
-	const uint32_t c = (uint32_t)(t / Np), j = (uint32_t)(t % Np);
-	const uint32_t src = perm[j];
-	float v = 0.0f;
-	if (c < R && src < N) {
-		v = xT[(uint64_t)c * Np + j];
-		if (pre)
-			v = (v * diffuse[3 * src + c % 3]) * area[src];
+	const uint32_t c = (uint32_t)(t / pieces), j0 = (uint32_t)(t % pieces) * 4;
+	float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	if (c < R) {
+		v = *reinterpret_cast<const float4*>(xT + (uint64_t)c * Np + j0);
+		if (pre) {
+			v.x = (v.x * dperm[3 * (j0 + 0) + c % 3]) * aperm[j0 + 0];
+			v.y = (v.y * dperm[3 * (j0 + 1) + c % 3]) * aperm[j0 + 1];
+			v.z = (v.z * dperm[3 * (j0 + 2) + c % 3]) * aperm[j0 + 2];
+			v.w = (v.w * dperm[3 * (j0 + 3) + c % 3]) * aperm[j0 + 3];
+		}
 	}
-	const uint32_t kk = j & 31, cc = c & 15;
-	xq[((uint64_t)(j >> 5) * (kRPad / 16) + (c >> 4)) * kT32Group + cc * 32 + t32_slot(kk >> 2, cc) * 4 + (kk & 3)] =
-	    v;
+	const uint32_t kk = j0 & 31, cc = c & 15;
+	*reinterpret_cast<float4*>(xq + ((uint64_t)(j0 >> 5) * (kRPad / 16) + (c >> 4)) * kT32Group + cc * 32 +
+	                           t32_slot(kk >> 2, cc) * 4) = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1018,6 +1103,7 @@ RadSolver::~RadSolver() { release(); }
 
 void RadSolver::release()
 {
+	chain_x_ = nullptr;
 	dfree(g32_);
 	dfree(g16_);
 	dfree(g16lo_);
@@ -1042,10 +1128,13 @@ void RadSolver::release()
 	dfree(rmx_);
 	dfree(rmx64_);
 	dfree(cmax_);
+	dfree(dperm_);
+	dfree(aperm_);
 }
 
 void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3, const float* area)
 {
+	chain_x_ = nullptr;
 	if (R_ == 0 || R_ > (uint32_t)kRPad)
 		throw std::runtime_error("RadSolver: 1..192 right-hand sides");
 	if (R_ != R) {
@@ -1059,6 +1148,7 @@ void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3,
 	dfree(E_);
 	dfree(diff_);
 	dfree(area_);
+	pcol_ok_ = false;
 	E_ = dalloc<double>((size_t)N * R);
 	diff_ = dalloc<float>(3 * (size_t)N);
 	area_ = dalloc<float>((size_t)Np);
@@ -1285,7 +1375,7 @@ void RadSolver::ensure_work()
 		xqlo_ = dalloc<uint16_t>(n);
 		xT_ = dalloc<float>(n);
 		yT_ = dalloc<float>(n);
-		cmax_ = dalloc<unsigned>(kRPad);
+		cmax_ = dalloc<unsigned>(2 * kRPad);
 	}
 	if (!rm32a_) {
 		rm32a_ = dalloc<float>((uint64_t)Np * R);
@@ -1296,13 +1386,28 @@ void RadSolver::ensure_work()
 }
 
 // y = K x (pre) or H x over the FP32 tile store (Morton order in and out)
+// Diffuse and area in Morton order for the tile forms' operand preparation
+// (after set_problem and the tile build).
+void RadSolver::ensure_perm_colors()
+{
+	if (pcol_ok_)
+		return;
+	if (!dperm_)
+		dperm_ = dalloc<float>(3 * (size_t)Np);
+	if (!aperm_)
+		aperm_ = dalloc<float>((size_t)Np);
+	k_permute_colors<<<gridn(Np), kB, 0, (hipStream_t)s_>>>(diff_, area_, perm_, N, Np, dperm_, aperm_);
+	HIPCHECK(hipGetLastError());
+	pcol_ok_ = true;
+}
+
 void RadSolver::apply_tiles_f32(const float* xT, float* yT, bool pre)
 {
 	hipStream_t s = (hipStream_t)s_;
-	const uint64_t nq = (uint64_t)kRPad * Np;
 	if (part_slots_ < maxslot_)
 		throw std::runtime_error("RadSolver: partial-sum planes fewer than the tile segments need");
-	k_prep_x32t<<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, perm_, N, Np, R, pre ? 1 : 0, xq_);
+	ensure_perm_colors();
+	k_prep_x32t<<<gridn((uint64_t)kRPad * (Np / 4)), kB, 0, s>>>(xT, dperm_, aperm_, Np, R, pre ? 1 : 0, xq_);
 	if (nchunks_)
 		k_gemm_f32t<<<nblocks_, kB, 0, s>>>(g32t_, xq_, reinterpret_cast<const uint4*>(chunks_), segoff_, tks_, Np,
 		                                     part_);
@@ -1314,12 +1419,15 @@ void RadSolver::apply_tiles_f32(const float* xT, float* yT, bool pre)
 // tile forms take and give their vectors in Morton order (position p holds
 // patch perm_[p]), the dense FP32 form in patch order.
 // split: the error-compensated product of the 16-bit forms (hi + lo planes).
-void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bool pre)
+void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bool pre, bool chain)
 {
 	hipStream_t s = (hipStream_t)s_;
 	const int st = storage_of(form);
 	split = split || split_of(form);
 	uint64_t nq = (uint64_t)kRPad * Np;
+	const int key = st * 4 + (split ? 2 : 0) + (pre ? 1 : 0);
+	const bool ready = chain_x_ == xT && chain_key_ == key;
+	chain_x_ = nullptr;
 	if (st == kTileF32) {
 		apply_tiles_f32(xT, yT, pre);
 		return;
@@ -1331,25 +1439,42 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 		k_gemm_f32<<<grid, kB, 0, s>>>(g32_, xq_, Np, ks, part_);
 		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, ks, nullptr, Np, R, nullptr, 1.0f, yT);
 	} else {
+		const bool f16 = st == kDenseF16;
 		uint16_t* q = reinterpret_cast<uint16_t*>(xq_);
 		uint16_t* lo = split ? xqlo_ : nullptr;
-		const unsigned* cm = nullptr;
-		if (st == kDenseF16) {
-			HIPCHECK(hipMemsetAsync(cmax_, 0, 4 * kRPad, s));
-			k_col_max<<<dim3(gridn(Np, 2048), R), kB, 0, s>>>(xT, diff_, area_, perm_, N, Np, pre ? 1 : 0, cmax_);
-			cm = cmax_;
-			k_prep_x16<1><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cm, perm_, N, Np, R, pre ? 1 : 0, q, lo);
-		} else {
-			k_prep_x16<0><<<gridn(nq), kB, 0, s>>>(xT, diff_, area_, cm, perm_, N, Np, R, pre ? 1 : 0, q, lo);
+		unsigned* cm = f16 ? cmax_ + cmax_cur_ * kRPad : nullptr;
+		ensure_perm_colors();
+		const unsigned og = (Np / 256) * (kRPad / 16); // k_operand blocks
+		OperandPass op;
+		op.x = xT;
+		op.dperm = dperm_;
+		op.aperm = aperm_;
+		op.pre = pre ? 1 : 0;
+		op.Np = Np;
+		op.R = R;
+		if (f16 && !ready) { // column maxima first: they set the operand's scales
+			HIPCHECK(hipMemsetAsync(cm, 0, 4 * kRPad, s));
+			op.cmax_out = cm;
+			k_operand<1><<<og, kB, 0, s>>>(op);
+			op.cmax_out = nullptr;
+		}
+		if (f16 || !ready) {
+			op.cmax_op = cm;
+			op.xq = q;
+			op.xlo = lo;
+			if (f16)
+				k_operand<1><<<og, kB, 0, s>>>(op);
+			else
+				k_operand<0><<<og, kB, 0, s>>>(op);
 		}
 		if (part_slots_ < maxslot_)
 			throw std::runtime_error("RadSolver: partial-sum planes fewer than the tile segments need");
 		if (nchunks_) {
 			const dim3 grid(nblocks_);
 			const uint4* chs = reinterpret_cast<const uint4*>(chunks_);
-			if (st == kDenseF16 && split)
+			if (f16 && split)
 				k_gemm_tiles<1, 2, kTilesD2><<<grid, kB, 0, s>>>(g16_, g16lo_, q, lo, chs, segoff_, tks_, Np, part_);
-			else if (st == kDenseF16)
+			else if (f16)
 				k_gemm_tiles<1, 1, kTilesD1><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, chs, segoff_, tks_, Np,
 				                                                 part_);
 			else if (split)
@@ -1358,7 +1483,35 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 				k_gemm_tiles<0, 1, kTilesD1><<<grid, kB, 0, s>>>(g16_, nullptr, q, nullptr, chs, segoff_, tks_, Np,
 				                                                 part_);
 		}
-		k_reduce_split<<<gridn((uint64_t)R * Np), kB, 0, s>>>(part_, 0, nslot_, Np, R, cm, gscale_, yT);
+		// the reduction; chained, it also prepares the next product's operand
+		// (BF16) or the column maxima that scale it (FP16)
+		OperandPass rd;
+		rd.part = part_;
+		rd.nslot = nslot_;
+		rd.cmax_in = cm;
+		rd.gscale = gscale_;
+		rd.y = yT;
+		rd.dperm = dperm_;
+		rd.aperm = aperm_;
+		rd.pre = pre ? 1 : 0;
+		rd.Np = Np;
+		rd.R = R;
+		if (chain && f16) {
+			cmax_cur_ ^= 1;
+			rd.cmax_out = cmax_ + cmax_cur_ * kRPad;
+			HIPCHECK(hipMemsetAsync(rd.cmax_out, 0, 4 * kRPad, s));
+		} else if (chain) {
+			rd.xq = q;
+			rd.xlo = lo;
+		}
+		if (f16)
+			k_operand<1><<<og, kB, 0, s>>>(rd);
+		else
+			k_operand<0><<<og, kB, 0, s>>>(rd);
+		if (chain) {
+			chain_x_ = yT;
+			chain_key_ = key;
+		}
 	}
 	HIPCHECK(hipGetLastError());
 }
@@ -1397,6 +1550,7 @@ void RadSolver::apply_op(int form, const double* x, double* y, bool pre)
 
 void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 {
+	chain_x_ = nullptr;
 	if (form < 0 || form >= kNumForms)
 		throw std::runtime_error("RadSolver: unknown form " + std::to_string(form));
 	if (refine > 0 && form != kDenseBF16 && form != kDenseF16)
@@ -1435,7 +1589,7 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 		{
 			Ev ev(s);
 			for (int b = 1; b <= bounces; ++b)
-				apply_dense(form, z[b - 1], z[b], false, true);
+				apply_dense(form, z[b - 1], z[b], false, true, b < bounces);
 			times.series_ms = ev.stop();
 		}
 		times.per_bounce_ms = times.series_ms / std::max(1, bounces);
@@ -1488,6 +1642,7 @@ void RadSolver::solve(int form, int bounces, int refine, double* total_out)
 void RadSolver::solve_grad(int form, int bounces, const double* g, double* total_out, double* lam_out,
                            double* dalb_out)
 {
+	chain_x_ = nullptr;
 	if (form < 0 || form >= kNumForms)
 		throw std::runtime_error("RadSolver: unknown form " + std::to_string(form));
 	hipStream_t s = (hipStream_t)s_;

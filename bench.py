@@ -834,12 +834,20 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
     lib.lgx_DeviceTimes(dev, tm)
     out["colour_gradient"] = dict(ms=round(tm[5], 4), items=int(len(items)), weights_bytes=int(len(items) * 4))
     log(f"[bench] C4 colour gradient over {len(items)} items: {tm[5]:.3f} ms")
-    # radiosity on the first n_patches texels, 192 right-hand sides from the lights' weights
+    # radiosity on the aligned window of n_patches consecutive texels that
+    # receives the most direct light, 192 right-hand sides from the lights'
+    # weights
     N = min(n_patches, S)
-    pn = scenes.normalize_rows_f32(leg.nrm[:N])
-    ppos = np.ascontiguousarray(leg.pos[:N])
+    lid = (items >> 26).astype(np.int64)
+    smp = (items & ((1 << 26) - 1)).astype(np.int64)
+    wv = np.where(leg.w.view(np.uint32) == 0xFFFFFFFF, 0.0, leg.w.astype(np.float64))
+    lit = np.bincount(smp, weights=wv, minlength=S)
+    starts = np.arange(0, S - N + 1, N)
+    s0 = int(starts[np.argmax([lit[a:a + N].sum() for a in starts])])
+    pn = scenes.normalize_rows_f32(leg.nrm[s0:s0 + N])
+    ppos = np.ascontiguousarray(leg.pos[s0:s0 + N])
     rad_dev = lib.lgx_DeviceCreate(P(tris), len(tris))
-    _lib.check(lib.lgx_SetSamples(rad_dev, P(ppos), P(np.ascontiguousarray(leg.nrm[:N])), N, None), lib)
+    _lib.check(lib.lgx_SetSamples(rad_dev, P(ppos), P(np.ascontiguousarray(leg.nrm[s0:s0 + N])), N, None), lib)
     nl = C.c_size_t(0)
     lib.lgx_SetCounting(rad_dev, 1)  # keeps the candidate pairs for the reference-shaped pass
     _lib.check(lib.lgx_RadLinks(rad_dev, P(ppos), P(pn), N, C.byref(nl)), lib)
@@ -847,18 +855,15 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
     lib.lgx_DeviceTimes(rad_dev, tm)
     link_ms = tm[3]
     link_refc = link_reference_counts(lib, rad_dev, tris)
-    lid = (items >> 26).astype(np.int64)
-    smp = (items & ((1 << 26) - 1)).astype(np.int64)
-    wv = np.where(leg.w.view(np.uint32) == 0xFFFFFFFF, 0.0, leg.w.astype(np.float64))
-    sel = smp < N
+    sel = (smp >= s0) & (smp < s0 + N)
     Wp = np.zeros((N, len(leg.lights)))
-    np.add.at(Wp, (smp[sel], lid[sel]), wv[sel])
+    np.add.at(Wp, (smp[sel] - s0, lid[sel]), wv[sel])
     cols = rng.uniform(0.5, 1.0, (64, len(leg.lights), 3))
     # the lights' direct contributions plus a small uniform emission, so that
-    # no column is empty where the lights do not reach the first patches
+    # no column is empty where the lights do not reach a patch
     E = np.ascontiguousarray(np.concatenate([Wp @ cols[k] for k in range(64)], 1) + 0.05)
     diff = np.full((N, 3), 0.6, np.float32)
-    area = np.ascontiguousarray(leg.radinfo[:N, 3])  # world area per texel, as the reference's patches carry
+    area = np.ascontiguousarray(leg.radinfo[s0:s0 + N, 3])  # world area per texel, as the reference's patches carry
     R = E.shape[1]
 
     def run(form, refine=0):
@@ -879,12 +884,14 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
         log(f"[bench] C4 radiosity {key}: {per:.3f} ms/bounce, err {forms[key]['rel_l2_err_vs_fp64']:.2e}")
     tiles = tile_fraction(lib, rad_dev)
     form_rooflines(forms, tile_counts(lib, rad_dev)[0], max(512, ((N + 255) // 256) * 256))
-    rad = dict(patches=N, rhs=R, bounces=bounces, links=int(nl.value), link_generation_ms=round(link_ms, 1),
+    lit_share = float((Wp @ cols[0]).sum() / E[:, :3].sum())  # direct light's share of E
+    rad = dict(patches=N, first_patch=s0, direct_light_share_of_emission=round(lit_share, 4), rhs=R, bounces=bounces, links=int(nl.value), link_generation_ms=round(link_ms, 1),
                link_reference_counts=link_refc, stored_tile_fraction=round(tiles, 4), forms=forms)
     cmp = compare_at_1e4(forms)
     rad["mfma_at_1e4"] = cmp
     best = (cmp["s_per_bounce"], cmp["form"])
-    log(f"[bench] C4 radiosity: {tiles:.4f} of the 256 x 32 tiles stored; fastest 16-bit form at <= 1e-4: "
+    log(f"[bench] C4 radiosity on texels {s0} .. {s0 + N - 1} (direct light {lit_share:.1%} of the emission): "
+        f"{tiles:.4f} of the 256 x 32 tiles stored; fastest 16-bit form at <= 1e-4: "
         f"{cmp['form']} {cmp['s_per_bounce'] * 1e3:.3f} ms/bounce, {cmp['speedup_vs_fastest_fp32']}x the fastest "
         f"FP32 form ({cmp['fastest_fp32_form']})")
     if with_cpu:
@@ -898,7 +905,7 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
                      "reference loop's seconds per bounce for one RGB setup times the number of setups",
                 speedup=round(cpu_b * sets / best[0], 1))
     out["radiosity"] = rad
-    out["adjoint"] = c4_adjoint(lib, leg, rad_dev, N, Wp, diff, area, bounces, rng)
+    out["adjoint"] = c4_adjoint(lib, leg, rad_dev, s0, N, Wp, diff, area, bounces, rng)
     lib.lgx_DeviceDestroy(rad_dev)
     if with_cpu:
         # the reference's AO ray loop on a sample of texels, and the device's
@@ -933,23 +940,27 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
     return out
 
 
-def c4_adjoint(lib, leg, rad_dev, N, Wp, diff, area, bounces, rng):
-    """Gradient of L = 1/2 |T - target|^2 over the radiosity tile (the first
-    N texels, 4 bounces) with respect to all 64 light colours (RGB): E =
+def c4_adjoint(lib, leg, rad_dev, s0, N, Wp, diff, area, bounces, rng):
+    """Gradient of L = 1/2 |T - target|^2 over the radiosity tile (texels
+    s0 .. s0 + N - 1, 4 bounces) with respect to all 64 light colours (RGB): E =
     Wp col from the checkpointed direct-light weights, T = sum_k K^k E;
     lgx_RadSolveGrad gives lam = dL/dE, lgx_DirectLightGrad the colours'
     gradient from lam over every work item (lam is zero beyond the tile).
-    The FP16 x 3 form against the sparse FP64 form's gradient."""
+    The FP16 tile form and the FP16 x 3 form against the sparse FP64 form's
+    gradient."""
     dp = C.POINTER(C.c_double)
     L = len(leg.lights)
     col0 = np.array([[leg.lights[l].color[c] for c in range(3)] for l in range(L)], np.float64)
     # the target is the bake at 40-60 % of the colours: the residual keeps one
-    # sign, so the gradient's sums over texels do not cancel
+    # sign, so the gradient's sums over texels do not cancel.  E is the direct
+    # light alone: with a uniform floor in E the residual would be the small
+    # difference of two large totals, and every form's gradient would measure
+    # the rounding of that floor's bounces instead of the light's.
     target_col = col0 * rng.uniform(0.4, 0.6, col0.shape)
     tm = (C.c_double * 6)()
 
     def solve(form, col):
-        E = np.ascontiguousarray(Wp @ col + 0.05)
+        E = np.ascontiguousarray(Wp @ col)
         T = np.zeros_like(E)
         t = (C.c_double * 4)()
         _lib.check(lib.lgx_RadSolve(rad_dev, E.ctypes.data_as(dp), 3, P(diff), P(area), form, bounces, 0,
@@ -958,7 +969,7 @@ def c4_adjoint(lib, leg, rad_dev, N, Wp, diff, area, bounces, rng):
 
     _, target = solve(0, target_col)
     out = {}
-    for form in (6, 0):
+    for form in (4, 4, 6, 0):  # form 4 twice: the first call builds its matrix store
         t0 = time.perf_counter()
         E, T = solve(form, col0)
         g = np.ascontiguousarray(T - target)
@@ -968,23 +979,26 @@ def c4_adjoint(lib, leg, rad_dev, N, Wp, diff, area, bounces, rng):
                                         g.ctypes.data_as(dp), tot.ctypes.data_as(dp), lam.ctypes.data_as(dp),
                                         dalb.ctypes.data_as(dp), t), lib)
         lam_all = np.zeros((leg.S, 3))
-        lam_all[:N] = lam
+        lam_all[s0:s0 + N] = lam
         dcol = np.zeros((L, 3))
         _lib.check(lib.lgx_DirectLightGrad(leg.dev, lam_all.ctypes.data_as(dp), dcol.ctypes.data_as(dp)), lib)
         lib.lgx_DeviceTimes(leg.dev, tm)
         out[form] = dict(dcol=dcol, wall_ms=(time.perf_counter() - t0) * 1e3, forward_ms=t[1],
                          reverse_ms=t[3], colour_grad_ms=tm[5])
-    g6, g0 = out[6], out[0]
-    rel = float(np.linalg.norm(g6["dcol"] - g0["dcol"]) / np.linalg.norm(g0["dcol"]))
-    res = dict(parameters=f"{L} light colours x RGB", patches=N, bounces=bounces, form="tiles_f16x3_mfma",
-               gradient_ms_gpu=round(g6["forward_ms"] + g6["reverse_ms"] + g6["colour_grad_ms"], 3),
-               breakdown_ms=dict(forward_series=round(g6["forward_ms"], 3),
-                                 reverse_sweep=round(g6["reverse_ms"], 3),
-                                 colour_gradient=round(g6["colour_grad_ms"], 3)),
-               gradient_ms_wall=round(g6["wall_ms"], 1), rel_err_vs_fp64=rel,
-               fp64_gradient_ms_gpu=round(g0["forward_ms"] + g0["reverse_ms"] + g0["colour_grad_ms"], 3))
+    g4, g6, g0 = out[4], out[6], out[0]
+    rel = lambda g: float(np.linalg.norm(g["dcol"] - g0["dcol"]) / np.linalg.norm(g0["dcol"]))
+    ms = lambda g: round(g["forward_ms"] + g["reverse_ms"] + g["colour_grad_ms"], 3)
+    res = dict(parameters=f"{L} light colours x RGB", patches=N, bounces=bounces, form="tiles_f16_mfma",
+               gradient_ms_gpu=ms(g4),
+               breakdown_ms=dict(forward_series=round(g4["forward_ms"], 3),
+                                 reverse_sweep=round(g4["reverse_ms"], 3),
+                                 colour_gradient=round(g4["colour_grad_ms"], 3)),
+               gradient_ms_wall=round(g4["wall_ms"], 1), rel_err_vs_fp64=rel(g4),
+               f16x3=dict(form="tiles_f16x3_mfma", gradient_ms_gpu=ms(g6), rel_err_vs_fp64=rel(g6)),
+               fp64_gradient_ms_gpu=ms(g0))
     log(f"[bench] C4 gradient w.r.t. {L} light colours through the radiosity tile: {res['gradient_ms_gpu']} ms "
-        f"(FP16 x 3), rel err vs FP64 {rel:.2e}")
+        f"(FP16 tiles, largest factors in FP32), rel err vs FP64 {rel(g4):.2e}; FP16 x 3 {ms(g6)} ms, "
+        f"{rel(g6):.2e}")
     return res
 
 

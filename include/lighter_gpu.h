@@ -176,7 +176,11 @@ LTRAPI int lgx_RadBounce(lgx_Device* dev, const float* diffuse3, const float* em
  * order), 5 FP32 on the FP32 matrix cores over the same tiles, 6 FP16 and
  * 7 BF16 with the error-compensated product (hi + lo planes of matrix and
  * operand, three MFMA products) on every bounce; every form accumulates in
- * FP32 or FP64.  refine: refinement sweeps of forms 3 and 4 with an FP64
+ * FP32 or FP64.  The 16-bit forms (3, 4, 6, 7) keep the largest factors out
+ * of the tiles (the top binades of the factors' exponents, at most 12 below
+ * the largest and at most max(nnz / 4096, 256) factors: the near-coincident
+ * pairs where two surfaces meet) and apply them in FP32 to the FP32
+ * operand.  refine: refinement sweeps of forms 3 and 4 with an FP64
  * residual from the error-compensated product (an error for other forms).
  * times_ms: matrix build, series, per bounce, refinement. */
 LTRAPI int lgx_RadSolve(lgx_Device* dev, const double* E, uint32_t R, const float* diffuse3, const float* area,

@@ -14,8 +14,9 @@ enum RadForm {
 	kSparseF64 = 0, // gather over the links, FP64 (the accuracy reference on the GPU)
 	kSparseF32 = 1, // gather over the links, FP32
 	kDenseF32 = 2,  // dense H = F in FP32, plain FP32 FMAs
-	kDenseBF16 = 3, // H in BF16 (the tiles that hold links), MFMA with FP32 accumulation
-	kDenseF16 = 4,  // H in FP16 (power-of-two scaled, the tiles that hold links), MFMA with FP32 accumulation
+	kDenseBF16 = 3, // H in BF16 (the tiles that hold links), MFMA with FP32 accumulation; outliers in FP32
+	kDenseF16 = 4,  // H in FP16 (power-of-two scaled, the tiles that hold links), MFMA with FP32 accumulation;
+	                // outliers in FP32
 	kTileF32 = 5,   // H in FP32 as the same stored tiles, FP32 matrix-core MFMA (the fair FP32 form)
 	kF16x3 = 6,     // FP16 hi + lo planes of H and the operand, three MFMA products per tile, every bounce
 	kBF16x3 = 7,    // the same in BF16
@@ -47,11 +48,13 @@ public:
 	double build_ms = 0;
 	uint32_t N = 0, Np = 0, R = 0;
 	uint64_t ntiles() const { return ntiles_; } // stored 256 x 32 tiles of the 16-bit forms
+	uint64_t noutliers() const { return nout_; } // factors the 16-bit forms apply in FP32
 	bool ntiles_built() const { return tiles_built_; }
 
 private:
 	void release();
 	void build_tiles();
+	void build_outliers();
 	void ensure_work();
 	void ensure_part();
 	void ensure_perm_colors();
@@ -102,6 +105,15 @@ private:
 	float* dperm_ = nullptr;   // diffuse at each Morton position (3 per position, padding zero)
 	float* aperm_ = nullptr;   // area at each Morton position
 	bool pcol_ok_ = false;     // dperm_ / aperm_ hold the current problem's colours
+	// outlier split of the 16-bit forms: factors >= othr_ are applied in FP32
+	// by the reduction (CSR by Morton position), the rest go to the tiles
+	uint32_t* orow_ = nullptr;
+	uint32_t* ocol_ = nullptr;
+	float* oval_ = nullptr;
+	uint64_t nout_ = 0;
+	float othr_ = 0.0f;
+	int bulk_exp_ = -1000; // largest factor exponent left in the tiles
+	bool out_built_ = false;
 };
 
 } // namespace lgx

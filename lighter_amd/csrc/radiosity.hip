@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kB) void k_tile_flags(uint32_t N, uint32_t Np, cons
 }
 
 // Values into the tile store (zeroed first), scaled by a power of two, and
-// the remainder planes.  LDS-image layout of a tile: row r = i' & 255 in
+// the remainder planes; factors >= othr (the outliers) stay out.  LDS-image layout of a tile: row r = i' & 255 in
 // 1 KB group r >> 4 at 64 bytes per row; column kk = j' & 31 in 16-byte piece
 // kk >> 3 at slot (kk >> 3) ^ piece_swz(r & 15), element kk & 7.
 template <int P>
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kB) void k_tile_scatter(uint32_t N, uint32_t Np, co
                                                      const uint32_t* __restrict__ col,
                                                      const float* __restrict__ val,
                                                      const uint32_t* __restrict__ pinv,
-                                                     const int32_t* __restrict__ tmap, float scale,
+                                                     const int32_t* __restrict__ tmap, float scale, float othr,
                                                      uint16_t* __restrict__ G, uint16_t* __restrict__ Glo)
 {
 	const uint32_t i = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
@@ -209,8 +209,8 @@ __global__ __launch_bounds__(kB) void k_tile_scatter(uint32_t N, uint32_t Np, co
 	for (uint32_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
 		const uint32_t jp = pinv[col[k]], kk = jp & 31;
 		const int32_t tile = tmap[rowbase + (jp >> 5)];
-		if (tile < 0)
-			continue; // not reached: the flags came from the same links
+		if (tile < 0 || val[k] >= othr)
+			continue; // (not reached: the flags came from the same links); an outlier
 		const uint64_t o = (uint64_t)tile * 8192 + inrow + (uint32_t)(((kk >> 3) ^ piece_swz(rr)) * 8) + (kk & 7);
 		const float a = val[k] * scale;
 		const uint16_t h = Prec<P>::cvt(a);
@@ -219,18 +219,81 @@ __global__ __launch_bounds__(kB) void k_tile_scatter(uint32_t N, uint32_t Np, co
 	}
 }
 
-// largest link factor (for the FP16 scale); grid-stride over the stored
-// links, one atomic per wave
-__global__ __launch_bounds__(kB) void k_max_link(uint64_t nnz, const float* __restrict__ val, unsigned* out)
+// ---------------------------------------------------------------------------
+// Outlier split of the 16-bit forms.  A handful of near-coincident patch
+// pairs at the edges where two surfaces meet have factors 10^5 - 10^6 times
+// the median (1 / r^4 in the form factor), and they dominate both the
+// radiosity total and the rounding error of a 16-bit product: rounded to
+// FP16 they alone leave a relative error near 2^-12 in the total, however
+// many links average the rest.  The factors in the top binades (at most
+// kOutBinades below the largest and at most max(nnz / kOutCapDiv,
+// kOutCapMin) of them, whole binades only) leave the tiles and are applied
+// in FP32 by the product's reduction, from the FP32 operand:
+// y = H_bulk16 x16 + H_out x.
+// ---------------------------------------------------------------------------
+constexpr int kHistBias = 160, kHistBins = 320; // bin = ilogb(f) + kHistBias (FP32 subnormals included)
+constexpr int kOutBinades = 12;
+constexpr uint64_t kOutCapDiv = 4096, kOutCapMin = 256;
+
+__global__ __launch_bounds__(kB) void k_link_hist(uint64_t nnz, const float* __restrict__ val,
+                                                  unsigned* __restrict__ hist)
 {
-	float m = 0.0f;
-	for (uint64_t k = (uint64_t)blockIdx.x * kB + threadIdx.x; k < nnz; k += (uint64_t)gridDim.x * kB)
-		m = fmaxf(m, val[k]);
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1)
-		m = fmaxf(m, __shfl_xor(m, o));
-	if ((threadIdx.x & 63) == 0)
-		atomicMax(out, __float_as_uint(m)); // non-negative floats order like their bits
+	__shared__ unsigned h[kHistBins];
+	for (int b = threadIdx.x; b < kHistBins; b += kB)
+		h[b] = 0;
+	__syncthreads();
+	for (uint64_t k = (uint64_t)blockIdx.x * kB + threadIdx.x; k < nnz; k += (uint64_t)gridDim.x * kB) {
+		const float v = val[k];
+		if (v > 0.0f && v < INFINITY)
+			atomicAdd(&h[ilogbf(v) + kHistBias], 1u);
+	}
+	__syncthreads();
+	for (int b = threadIdx.x; b < kHistBins; b += kB)
+		if (h[b])
+			atomicAdd(hist + b, h[b]);
+}
+
+// outliers per Morton position (one wave per patch row; cnt starts at zero)
+__global__ __launch_bounds__(kB) void k_outlier_count(uint32_t N, const uint32_t* __restrict__ rowptr,
+                                                      const float* __restrict__ val, float thr,
+                                                      const uint32_t* __restrict__ pinv, uint32_t* __restrict__ cnt)
+{
+	const uint32_t i = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (i >= N)
+		return;
+	uint32_t n = 0;
+	for (uint32_t k0 = rowptr[i]; k0 < rowptr[i + 1]; k0 += 64) {
+		const uint32_t k = k0 + lane;
+		n += (uint32_t)__popcll(__ballot(k < rowptr[i + 1] && val[k] >= thr));
+	}
+	if (lane == 0)
+		cnt[pinv[i]] = n;
+}
+
+// the outliers of each row in link order: other position and factor
+__global__ __launch_bounds__(kB) void k_outlier_fill(uint32_t N, const uint32_t* __restrict__ rowptr,
+                                                     const uint32_t* __restrict__ col, const float* __restrict__ val,
+                                                     float thr, const uint32_t* __restrict__ pinv,
+                                                     const uint32_t* __restrict__ orow, uint32_t* __restrict__ ocol,
+                                                     float* __restrict__ oval)
+{
+	const uint32_t i = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (i >= N)
+		return;
+	uint32_t o = orow[pinv[i]];
+	for (uint32_t k0 = rowptr[i]; k0 < rowptr[i + 1]; k0 += 64) {
+		const uint32_t k = k0 + lane;
+		const bool hit = k < rowptr[i + 1] && val[k] >= thr;
+		const uint64_t m = __ballot(hit);
+		if (hit) {
+			const uint32_t at = o + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+			ocol[at] = pinv[col[k]];
+			oval[at] = val[k];
+		}
+		o += (uint32_t)__popcll(m);
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -268,7 +331,8 @@ __global__ __launch_bounds__(kB) void k_permute_colors(const float* __restrict__
 // Operand passes of the 16-bit tile products, one kernel for three jobs:
 //   source   the column-major FP32 vector x (positions in Morton order), or
 //            the sum of a product's partial-sum planes (slots 0 .. nslot of
-//            the row block, in slot order) divided by the product's scales;
+//            the row block, in slot order) divided by the product's scales,
+//            plus the outliers' products in FP32 (in CSR order);
 //   y        that source written out (the product's result);
 //   cmax_out column maxima of |w| by atomicMax (non-negative floats order
 //            like their bits, so the result does not depend on the order of
@@ -301,6 +365,12 @@ struct OperandPass {
 	const unsigned* cmax_op = nullptr; // scales of the operand written here (FP16)
 	uint16_t* xq = nullptr;
 	uint16_t* xlo = nullptr;
+	// reduction: + H_out w_in, the outliers (CSR by Morton position) times
+	// the product's FP32 operand w_in (from its input vector xin)
+	const uint32_t* orow = nullptr;
+	const uint32_t* ocol = nullptr;
+	const float* oval = nullptr;
+	const float* xin = nullptr;
 	uint32_t Np = 0, R = 0;
 };
 
@@ -338,6 +408,16 @@ __global__ __launch_bounds__(kB) void k_operand(const OperandPass a)
 #pragma unroll
 			for (int e = 0; e < 8; ++e)
 				v[e] *= inv;
+			if (a.orow && live && a.orow[j0] != a.orow[j0 + 8]) {
+				for (int e = 0; e < 8; ++e)
+					for (uint32_t k = a.orow[j0 + e]; k < a.orow[j0 + e + 1]; ++k) {
+						const uint32_t q = a.ocol[k];
+						float w = a.xin[cbase + q];
+						if (a.pre)
+							w = (w * a.dperm[3 * q + c % 3]) * a.aperm[q];
+						v[e] += a.oval[k] * w;
+					}
+			}
 		} else if (live) {
 			const float4 u0 = *reinterpret_cast<const float4*>(a.x + cbase + j0);
 			const float4 u1 = *reinterpret_cast<const float4*>(a.x + cbase + j0 + 4);
@@ -1130,6 +1210,9 @@ void RadSolver::release()
 	dfree(cmax_);
 	dfree(dperm_);
 	dfree(aperm_);
+	dfree(orow_);
+	dfree(ocol_);
+	dfree(oval_);
 }
 
 void RadSolver::set_problem(const double* E, uint32_t R_, const float* diffuse3, const float* area)
@@ -1292,6 +1375,62 @@ void RadSolver::build_tiles()
 static int storage_of(int form) { return form == kF16x3 ? kDenseF16 : form == kBF16x3 ? kDenseBF16 : form; }
 static bool split_of(int form) { return form == kF16x3 || form == kBF16x3; }
 
+// The outlier threshold from the factors' exponent histogram (whole binades
+// from the top, within kOutBinades of the largest factor and at most
+// max(nnz / kOutCapDiv, kOutCapMin) factors), the outliers' CSR by Morton
+// position, and the largest exponent left in the tiles.
+void RadSolver::build_outliers()
+{
+	if (out_built_)
+		return;
+	hipStream_t s = (hipStream_t)s_;
+	DevBuf<unsigned> dh(kHistBins);
+	HIPCHECK(hipMemsetAsync(dh, 0, 4 * kHistBins, s));
+	k_link_hist<<<1024, kB, 0, s>>>(dev.rad.nnz, dev.rad.val, dh);
+	std::vector<unsigned> h(kHistBins);
+	HIPCHECK(hipMemcpyAsync(h.data(), dh, 4 * kHistBins, hipMemcpyDeviceToHost, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	int top = kHistBins - 1;
+	while (top >= 0 && !h[top])
+		--top;
+	const uint64_t cap = std::max(dev.rad.nnz / kOutCapDiv, kOutCapMin);
+	int lowest = top + 1; // lowest outlier bin
+	uint64_t cum = 0;
+	for (int b = top; b >= 0 && b > top - kOutBinades && cum + h[b] <= cap; --b) {
+		cum += h[b];
+		lowest = b;
+	}
+	othr_ = (top >= 0 && lowest <= top) ? ldexpf(1.0f, lowest - kHistBias) : INFINITY;
+	int bulk = lowest - 1;
+	while (bulk >= 0 && !h[bulk])
+		--bulk;
+	bulk_exp_ = bulk >= 0 ? bulk - kHistBias : -1000;
+	nout_ = cum;
+	dfree(orow_);
+	dfree(ocol_);
+	dfree(oval_);
+	if (nout_) {
+		DevBuf<uint32_t> cnt(Np);
+		HIPCHECK(hipMemsetAsync(cnt, 0, 4 * (size_t)Np, s));
+		k_outlier_count<<<gridn(N, kB / 64), kB, 0, s>>>(N, dev.rad.rowptr, dev.rad.val, othr_, pinv_, cnt);
+		std::vector<uint32_t> c(Np), off(Np + 1, 0);
+		HIPCHECK(hipMemcpyAsync(c.data(), cnt, 4 * (size_t)Np, hipMemcpyDeviceToHost, s));
+		HIPCHECK(hipStreamSynchronize(s));
+		for (uint32_t p = 0; p < Np; ++p)
+			off[p + 1] = off[p] + c[p];
+		if (off[Np] != nout_)
+			throw std::runtime_error("RadSolver: outlier count differs from the histogram");
+		orow_ = dalloc<uint32_t>(Np + 1);
+		ocol_ = dalloc<uint32_t>(nout_);
+		oval_ = dalloc<float>(nout_);
+		HIPCHECK(hipMemcpyAsync(orow_, off.data(), 4 * ((size_t)Np + 1), hipMemcpyHostToDevice, s));
+		k_outlier_fill<<<gridn(N, kB / 64), kB, 0, s>>>(N, dev.rad.rowptr, dev.rad.col, dev.rad.val, othr_, pinv_,
+		                                                 orow_, ocol_, oval_);
+		HIPCHECK(hipStreamSynchronize(s)); // off lives on this stack
+	}
+	out_built_ = true;
+}
+
 void RadSolver::build_dense(int form)
 {
 	hipStream_t s = (hipStream_t)s_;
@@ -1318,24 +1457,17 @@ void RadSolver::build_dense(int form)
 			g16lo_ = dalloc<uint16_t>(ntiles_ * kGStageElems);
 		HIPCHECK(hipMemsetAsync(g16_, 0, 2 * ntiles_ * kGStageElems, s));
 		HIPCHECK(hipMemsetAsync(g16lo_, 0, 2 * ntiles_ * kGStageElems, s));
+		build_outliers();
 		gscale_ = 1.0f;
 		if (st == kDenseF16) {
-			// scale so that the largest factor sits near 2^13: small factors
-			// stay in FP16's normal range
-			DevBuf<unsigned> dm(1);
-			HIPCHECK(hipMemsetAsync(dm, 0, 4, s));
-			k_max_link<<<2048, kB, 0, s>>>(dev.rad.nnz, dev.rad.val, dm);
-			unsigned hm = 0;
-			HIPCHECK(hipMemcpyAsync(&hm, dm, 4, hipMemcpyDeviceToHost, s));
-			HIPCHECK(hipStreamSynchronize(s));
-			float mx;
-			memcpy(&mx, &hm, 4);
-			gscale_ = mx > 0 ? ldexpf(1.0f, 14 - ilogbf(mx) - 1) : 1.0f;
+			// scale so that the largest factor in the tiles sits near 2^13:
+			// small factors stay in FP16's normal range
+			gscale_ = bulk_exp_ > -1000 ? ldexpf(1.0f, 13 - bulk_exp_) : 1.0f;
 			k_tile_scatter<1><<<gridn(N, kB / 64), kB, 0, s>>>(N, Np, dev.rad.rowptr, dev.rad.col, dev.rad.val,
-			                                                   pinv_, tmap_, gscale_, g16_, g16lo_);
+			                                                   pinv_, tmap_, gscale_, othr_, g16_, g16lo_);
 		} else {
 			k_tile_scatter<0><<<gridn(N, kB / 64), kB, 0, s>>>(N, Np, dev.rad.rowptr, dev.rad.col, dev.rad.val,
-			                                                   pinv_, tmap_, 1.0f, g16_, g16lo_);
+			                                                   pinv_, tmap_, 1.0f, othr_, g16_, g16lo_);
 		}
 	}
 	HIPCHECK(hipGetLastError());
@@ -1496,6 +1628,12 @@ void RadSolver::apply_dense(int form, const float* xT, float* yT, bool split, bo
 		rd.pre = pre ? 1 : 0;
 		rd.Np = Np;
 		rd.R = R;
+		if (nout_) {
+			rd.orow = orow_;
+			rd.ocol = ocol_;
+			rd.oval = oval_;
+			rd.xin = xT;
+		}
 		if (chain && f16) {
 			cmax_cur_ ^= 1;
 			rd.cmax_out = cmax_ + cmax_cur_ * kRPad;

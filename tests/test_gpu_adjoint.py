@@ -16,8 +16,10 @@ light) and on what the mathematics fixes:
   16-bit product on every bounce, forward and reverse).
 
 Tolerances (relative to the norm of the reference quantity), per form:
-  sparse FP64 1e-10, sparse FP32 1e-5, dense FP32 1e-5, BF16 tiles 3e-2,
-  FP16 tiles 3e-3, FP32 tiles 1e-5, FP16 x 3 1e-5, BF16 x 3 1e-4.
+  sparse FP64 1e-10, sparse FP32 1e-5, dense FP32 1e-5, BF16 tiles 1e-3,
+  FP16 tiles 3e-4, FP32 tiles 1e-5, FP16 x 3 1e-5, BF16 x 3 1e-5.
+The plain 16-bit forms' widest error is the albedo derivative (16-bit link
+factors in both sweeps); total, adjoint identity and lam stay near 1e-6.
 """
 import ctypes as C
 
@@ -31,7 +33,7 @@ from lighter_amd import _lib, scenes
 
 pytestmark = pytest.mark.gpu
 
-TOL = {0: 1e-10, 1: 1e-5, 2: 1e-5, 3: 3e-2, 4: 3e-3, 5: 1e-5, 6: 1e-5, 7: 1e-4}
+TOL = {0: 1e-10, 1: 1e-5, 2: 1e-5, 3: 1e-3, 4: 3e-4, 5: 1e-5, 6: 1e-5, 7: 1e-5}
 dp = C.POINTER(C.c_double)
 u32p = C.POINTER(C.c_uint32)
 

@@ -15,8 +15,9 @@ shadowed pairs (a step closer than 0.001) are the same for k = 0, k and 2k,
 k = 0 gives 0 or 1, and the factor does not grow with k.
 Radiosity over the city's first 65,536 texels (one 62.5 m tile), one RGB
 setup: every form against the oracle's FP64 series on the same links; the
-refined BF16 and FP16 MFMA forms and the compensated (hi + lo) forms at or
-below 1e-4, the FP32 forms at or below 1e-5.
+plain BF16 and FP16 MFMA forms (largest factors in FP32, the rest in 16
+bits) at or below 2e-6, refined or compensated (hi + lo) at or below 1e-6,
+the FP32 forms at or below 1e-6.
 Colour gradient of an L2 loss over every texel against central differences
 of the GPU bake on 20 light intensities, to 1e-3.
 """
@@ -218,8 +219,9 @@ def city_radiosity(lib, city):
     lib.lgx_DeviceDestroy(dev)
 
 
-@pytest.mark.parametrize("form,refine,tol", [(0, 0, 1e-12), (1, 0, 1e-5), (2, 0, 1e-5), (3, 1, 1e-4),
-                                             (4, 1, 1e-4), (5, 0, 1e-5), (6, 0, 1e-5), (7, 0, 1e-4)])
+@pytest.mark.parametrize("form,refine,tol", [(0, 0, 1e-12), (1, 0, 1e-6), (2, 0, 1e-6), (3, 0, 2e-6),
+                                             (3, 1, 1e-6), (4, 0, 2e-6), (4, 1, 1e-6), (5, 0, 1e-6),
+                                             (6, 0, 1e-6), (7, 0, 1e-6)])
 def test_c4_radiosity_tile_against_fp64_oracle(lib, orc, city_radiosity, form, refine, tol):
     """sum_{k<=4} K^k E over the 65,536-patch tile (12 M links) in each form
     against the oracle's FP64 series on the same link list."""

@@ -67,19 +67,21 @@ def prepare_workload(lib, spec, nthreads, radinfo=None):
 
 
 def placement_info(lib, b):
-    """Times of the last GPU sample placement of scene b."""
+    """Times of the last GPU rasterisation and sample placement of scene b."""
     t = (C.c_double * 5)()
     lib.lgx_PlacementTimes(b.scene, t)
+    r = (C.c_double * 3)()
+    lib.lgx_RasterTimes(b.scene, r)
     return dict(samples=int(t[4]), kernel_ms=round(t[2], 3), flatten_ms=round(t[0], 1), upload_ms=round(t[1], 1),
-                download_ms=round(t[3], 1))
+                download_ms=round(t[3], 1), raster_kernels_ms=round(r[0], 3), raster_texels=int(r[1]))
 
 
 def placement_vs_host(lib, spec, nthreads, b_gpu, pos_gpu):
-    """The preparation again with the host restatement of the sample
-    placement: its wall time beside the GPU-placed run's, and whether every
-    sample position is the same."""
+    """The preparation again with the host restatements of the
+    rasterisation and the sample placement: its wall time beside the GPU
+    run's, and whether every sample position is the same."""
     b = scenes.BuiltScene(lib, spec, dict(max_num_threads=nthreads))
-    _lib.check(lib.lgx_SetHostStages(b.scene, 1), lib)  # host placement
+    _lib.check(lib.lgx_SetHostStages(b.scene, 5), lib)  # host rasterisation and placement
     t0 = time.perf_counter()
     _lib.check(lib.lgx_Prepare(b.scene), lib)
     host_s = time.perf_counter() - t0
@@ -93,11 +95,11 @@ def placement_vs_host(lib, spec, nthreads, b_gpu, pos_gpu):
     b.destroy()
     pos = np.concatenate(pos)
     same = pos.shape == pos_gpu.shape and bool(np.array_equal(pos.view(np.uint32), pos_gpu.view(np.uint32)))
-    out = dict(b_gpu.placement, prepare_s_gpu_placement=round(b_gpu.prepare_s, 2),
-               prepare_s_host_placement=round(host_s, 2), host_threads=nthreads, identical_samples=same)
-    log(f"[bench] sample placement: GPU kernel {out['kernel_ms']} ms for {out['samples']} samples; preparation "
-        f"{b_gpu.prepare_s:.2f} s (GPU placement) vs {host_s:.2f} s (host placement, {nthreads} threads); "
-        f"identical: {same}")
+    out = dict(b_gpu.placement, prepare_s_gpu=round(b_gpu.prepare_s, 2), prepare_s_host=round(host_s, 2),
+               host_threads=nthreads, identical_samples=same)
+    log(f"[bench] rasterisation and sample placement: GPU kernels {out['raster_kernels_ms']} ms (rasterisation, "
+        f"{out['raster_texels']} texels) + {out['kernel_ms']} ms (placement, {out['samples']} samples); preparation "
+        f"{b_gpu.prepare_s:.2f} s (GPU) vs {host_s:.2f} s (host, {nthreads} threads); identical: {same}")
     return out
 
 

@@ -39,15 +39,22 @@ LTRAPI int lgx_HasDevice(void);
  * place need no GPU. */
 LTRAPI int lgx_Prepare(ltr_Scene* scene);
 /* No counterpart in the reference (it runs these stages on its worker
- * threads). Two host stages of the reference also run on the GPU here: the
- * sample placement (lighter.cpp:436-466) and the lightmap export's image
- * work (extrapolation, blur, downsample; lighter.cpp:849-958). Both use the
- * GPU by default, also under ltr_Start; a bit set in mask selects the host
- * restatement instead, which the GPU forms are tested against. Takes effect
- * at the next preparation or export. */
+ * threads). Three host stages of the reference also run on the GPU here:
+ * the lightmap rasterisation and sample compaction (lighter.cpp:32-66,
+ * 392-420), the sample placement (lighter.cpp:436-466) and the lightmap
+ * export's image work (extrapolation, blur, downsample;
+ * lighter.cpp:849-958). They use the GPU by default, also under ltr_Start;
+ * a bit set in mask selects the host restatement instead, which the GPU
+ * forms are tested against. Takes effect at the next preparation or
+ * export. */
 #define LGX_HOST_PLACEMENT 1
 #define LGX_HOST_FINALIZE 2
+#define LGX_HOST_RASTER 4
 LTRAPI int lgx_SetHostStages(ltr_Scene* scene, int mask);
+/* No counterpart in the reference. Last GPU rasterisation: out[0] kernels
+ * ms (HIP events, the count read-back between them included), out[1] atlas
+ * texels, out[2] samples (zeros after a host rasterisation). */
+LTRAPI int lgx_RasterTimes(ltr_Scene* scene, double out[3]);
 /* No counterpart in the reference. Last GPU placement: out[0] flatten ms,
  * out[1] upload ms, out[2] kernel ms (HIP events), out[3] download ms,
  * out[4] samples placed (zeros after a host placement). */

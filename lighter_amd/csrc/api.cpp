@@ -379,13 +379,23 @@ int lgx_StageFinalize(ltr_Scene* S)
 int lgx_SetHostStages(ltr_Scene* S, int mask)
 {
 	return guarded([&] {
-		if (mask & ~(LGX_HOST_PLACEMENT | LGX_HOST_FINALIZE))
+		if (mask & ~(LGX_HOST_PLACEMENT | LGX_HOST_FINALIZE | LGX_HOST_RASTER))
 			throw std::runtime_error("lgx_SetHostStages: unknown bits in mask");
 		if (S->worker.joinable() && S->stage.load() != nullptr)
 			throw std::runtime_error("a bake started with ltr_Start is running");
 		S->host_place = (mask & LGX_HOST_PLACEMENT) != 0;
 		S->host_finalize = (mask & LGX_HOST_FINALIZE) != 0;
+		S->host_raster = (mask & LGX_HOST_RASTER) != 0;
 	});
+}
+
+int lgx_RasterTimes(ltr_Scene* S, double out[3])
+{
+	const RasterTimes& t = S->raster_times;
+	out[0] = t.kernel_ms;
+	out[1] = (double)t.texels;
+	out[2] = (double)t.samples;
+	return 1;
 }
 
 int lgx_FinalizeTimes(ltr_Scene* S, double out[5])

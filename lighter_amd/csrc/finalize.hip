@@ -5,18 +5,15 @@
 // a block belongs to one instance, found by a binary search of the
 // instances' first blocks (the same for every lane of the block).
 //  * k_fin_scatter: sample colours into their texels, texel flags set;
-//  * k_fin_extrap (three rounds): an unset texel takes the mean of its set
-//    4-neighbours, each extrapolated linearly from the texel beyond it when
-//    that one is set too (lerp(c2, c1, 2)); flags of the round's start are
-//    read and the round's new flags written to a second array, so a round
-//    reads only texels it does not write, as lighter.cpp:879-940 does in
-//    scan order;
-//  * k_fin_conv (twice): Gaussian taps over the row with clamped edges,
-//    written transposed (Convolve_Transpose, lighter_math.cpp:367-398): the
-//    second pass transposes back;
-//  * k_fin_ds: the 2x2 mean with wrap-around (Downsample2X,
-//    lighter_math.cpp:400-424).
-// Sums run in the host restatement's order with -ffp-contract=off.
+//  * k_fin_extrap (three rounds): extrapolate_texel at every unset texel;
+//    flags of the round's start are read and the round's new flags written
+//    to a second array, so a round reads only texels it does not write, as
+//    lighter.cpp:879-940 does in scan order;
+//  * k_fin_conv (twice): blur_texel along the rows, written transposed
+//    (lighter_math.cpp:367-398): the second pass transposes back;
+//  * k_fin_ds: half_texel, the 2x2 mean with wrap-around
+//    (lighter_math.cpp:400-424).
+// The per-texel functions (texel.h) are the host export's too.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -28,6 +25,7 @@
 #include <thread>
 
 #include "finalize.h"
+#include "texel.h"
 #include "vmath.h"
 
 namespace lgx {
@@ -81,46 +79,18 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_extrap(const FinInst* __restr
 	const int k = fin_inst(fb, ninst, blockIdx.x);
 	const FinInst I = inst[k];
 	const uint64_t p = (uint64_t)(blockIdx.x - fb[k]) * kFinBlock + threadIdx.x;
-	const uint32_t w = I.w, h = I.h;
-	if (p >= (uint64_t)w * h)
+	if (p >= (uint64_t)I.w * I.h || set0[I.px_off + p])
 		return;
-	const uint32_t x = (uint32_t)(p % w), y = (uint32_t)(p / w);
-	const uint8_t* st = set0 + I.px_off;
-	float* img = rgb + 3 * I.px_off;
-	if (st[p])
-		return;
-	auto isset = [&](uint32_t xx, uint32_t yy) { return xx < w && yy < h && st[xx + (uint64_t)yy * w]; };
-	auto col = [&](uint32_t xx, uint32_t yy) { return ld3(img + (xx + (uint64_t)yy * w) * 3); };
-	f3 c = mk3(0, 0, 0);
-	int count = 0;
-	if (isset(x - 1, y)) {
-		c += isset(x - 2, y) ? lerp3(col(x - 2, y), col(x - 1, y), 2.0f) : col(x - 1, y);
-		count++;
-	}
-	if (isset(x + 1, y)) {
-		c += isset(x + 2, y) ? lerp3(col(x + 2, y), col(x + 1, y), 2.0f) : col(x + 1, y);
-		count++;
-	}
-	if (isset(x, y - 1)) {
-		c += isset(x, y - 2) ? lerp3(col(x, y - 2), col(x, y - 1), 2.0f) : col(x, y - 1);
-		count++;
-	}
-	if (isset(x, y + 1)) {
-		c += isset(x, y + 2) ? lerp3(col(x, y + 2), col(x, y + 1), 2.0f) : col(x, y + 1);
-		count++;
-	}
-	if (count) {
-		c = c / (float)count;
-		float* o = img + p * 3;
-		o[0] = tmax(c.x, 0.0f);
-		o[1] = tmax(c.y, 0.0f);
-		o[2] = tmax(c.z, 0.0f);
+	const AtlasView view{rgb + 3 * I.px_off, set0 + I.px_off, I.w, I.h};
+	f3 c;
+	if (extrapolate_texel(view, (uint32_t)(p % I.w), (uint32_t)(p / I.w), &c)) {
+		st3(rgb + 3 * (I.px_off + p), c);
 		set1[I.px_off + p] = 1;
 	}
 }
 
-// One convolve-and-transpose pass over every atlas: rows of length W
-// (w on the first pass, h on the second), H rows.
+// One blur pass over every atlas: rows of length W (w on the first pass, h
+// on the second), H rows, each result written transposed.
 __global__ __launch_bounds__(kFinBlock) void k_fin_conv(const FinInst* __restrict__ inst,
                                                         const uint32_t* __restrict__ fb, int ninst,
                                                         const float* __restrict__ src, float* __restrict__ dst,
@@ -132,22 +102,9 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_conv(const FinInst* __restric
 	const uint64_t p = (uint64_t)(blockIdx.x - fb[k]) * kFinBlock + threadIdx.x;
 	if (p >= (uint64_t)W * H)
 		return;
-	const int x = (int)(p % W);
+	const uint32_t x = (uint32_t)(p % W);
 	const uint64_t y = p / W;
-	const float* row = src + (I.px_off + y * W) * 3;
-	float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-	for (int i = -ext; i <= ext; ++i) {
-		const float kq = kern[i + ext];
-		int xi = x + i;
-		xi = xi < 0 ? 0 : (xi > (int)W - 1 ? (int)W - 1 : xi);
-		s0 += row[xi * 3 + 0] * kq;
-		s1 += row[xi * 3 + 1] * kq;
-		s2 += row[xi * 3 + 2] * kq;
-	}
-	float* dp = dst + (I.px_off + y + (uint64_t)H * x) * 3;
-	dp[0] = s0;
-	dp[1] = s1;
-	dp[2] = s2;
+	st3(dst + 3 * (I.px_off + y + (uint64_t)H * x), blur_texel(src + 3 * (I.px_off + y * W), W, (int)x, kern, ext));
 }
 
 __global__ __launch_bounds__(kFinBlock) void k_fin_ds(const FinInst* __restrict__ inst,
@@ -159,14 +116,8 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_ds(const FinInst* __restrict_
 	const uint64_t p = (uint64_t)(blockIdx.x - dfb[k]) * kFinBlock + threadIdx.x;
 	if (p >= (uint64_t)I.hw * I.hh)
 		return;
-	const unsigned x = (unsigned)(p % I.hw), y = (unsigned)(p / I.hw);
-	const unsigned sw = I.w, sh = I.h;
-	const unsigned x0 = (x * 2) % sw, y0 = (y * 2) % sh, x1 = (x * 2 + 1) % sw, y1 = (y * 2 + 1) % sh;
-	const float* s = src + 3 * I.px_off;
-	const f3 avg = (ld3(s + (x0 + (uint64_t)y0 * sw) * 3) + ld3(s + (x1 + (uint64_t)y0 * sw) * 3) +
-	                ld3(s + (x0 + (uint64_t)y1 * sw) * 3) + ld3(s + (x1 + (uint64_t)y1 * sw) * 3)) *
-	               0.25f;
-	st3(dst + 3 * (I.ds_off + x + (uint64_t)y * I.hw), avg);
+	const uint32_t x = (uint32_t)(p % I.hw), y = (uint32_t)(p / I.hw);
+	st3(dst + 3 * (I.ds_off + x + (uint64_t)y * I.hw), half_texel(src + 3 * I.px_off, I.w, I.h, x, y));
 }
 
 // ===========================================================================

@@ -489,50 +489,4 @@ float SplitTree::intersect(f3 from, f3 to, f3* outnormal) const
 	return node_intersect(0, from, to, outnormal);
 }
 
-// ---------------------------------------------------------------------------
-void rasterize_triangle(f3* img1, f3* img2, f4* img3, int32_t width, int32_t height, float margin, f2 p1,
-                        f2 p2, f2 p3, f3 va1, f3 va2, f3 va3, f3 vb1, f3 vb2, f3 vb3, f4 vc1, f4 vc2, f4 vc3)
-{
-	int32_t maxX = (int32_t)(tmax(p1.x, tmax(p2.x, p3.x)) + margin);
-	int32_t minX = (int32_t)(tmin(p1.x, tmin(p2.x, p3.x)) - margin);
-	int32_t maxY = (int32_t)(tmax(p1.y, tmax(p2.y, p3.y)) + margin);
-	int32_t minY = (int32_t)(tmin(p1.y, tmin(p2.y, p3.y)) - margin);
-	if (maxX < 0 || minX >= width || maxY < 0 || minY >= height)
-		return;
-	if (minX < 0) minX = 0;
-	if (maxX >= width) maxX = width - 1;
-	if (minY < 0) minY = 0;
-	if (maxY >= height) maxY = height - 1;
-
-	f2 e12 = p2 - p1, e13 = p3 - p1;
-	float vc = cross2(e12, e13);
-	if (vc == 0)
-		return;
-	f2 n1 = normalized2(perp2(e12));
-	f2 n2 = normalized2(perp2(p3 - p2));
-	f2 n3 = -normalized2(perp2(e13));
-	float d1 = dot2(n1, p1), d2 = dot2(n2, p2), d3 = dot2(n3, p3);
-	f3 a12 = va2 - va1, a13 = va3 - va1;
-	f3 b12 = vb2 - vb1, b13 = vb3 - vb1;
-	f4 c12 = vc2 - vc1, c13 = vc3 - vc1;
-	for (int32_t x = minX; x <= maxX; x++) {
-		for (int32_t y = minY; y <= maxY; y++) {
-			f2 q = mk2(x - p1.x, y - p1.y);
-			float s = cross2(q, e13) / vc;
-			float t = cross2(e12, q) / vc;
-			s = tmax(tmin(s, 1.0f), 0.0f);
-			t = tmax(tmin(t, 1.0f), 0.0f);
-			f2 p = mk2((float)x, (float)y);
-			float pd1 = dot2(p, n1), pd2 = dot2(p, n2), pd3 = dot2(p, n3);
-			if ((pd1 <= d1 + margin && pd2 <= d2 + margin && pd3 <= d3 + margin) ||
-			    (pd1 + margin >= d1 && pd2 + margin >= d2 && pd3 + margin >= d3)) {
-				size_t o = (size_t)x + (size_t)width * (size_t)y;
-				img1[o] = va1 + a12 * s + a13 * t;
-				img2[o] = vb1 + b12 * s + b13 * t;
-				img3[o] = vc1 + c12 * s + c13 * t;
-			}
-		}
-	}
-}
-
 } // namespace lgx

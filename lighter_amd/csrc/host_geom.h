@@ -203,13 +203,4 @@ private:
 	std::vector<Node> nodes_;
 };
 
-// ---------------------------------------------------------------------------
-// Lightmap rasterisation (lighter_math.cpp:245-302): writes interpolated
-// position, normal and (uv0, part, area) into every texel whose centre lies
-// within `margin` texels of the triangle.
-// ---------------------------------------------------------------------------
-void rasterize_triangle(f3* img_pos, f3* img_nrm, f4* img_info, int32_t width, int32_t height, float margin,
-                        f2 p1, f2 p2, f2 p3, f3 va1, f3 va2, f3 va3, f3 vb1, f3 vb2, f3 vb3, f4 vc1, f4 vc2,
-                        f4 vc3);
-
 } // namespace lgx

@@ -7,6 +7,8 @@
 // finalisation (lighter.cpp:292-482, 849-1044) so that the samples the
 // kernels see and the images the caller gets back are the reference's.
 #include "pipeline.h"
+#include "raster_gpu.h"
+#include "texel.h"
 
 #include <float.h>
 #include <math.h>
@@ -210,18 +212,21 @@ static float scene_split_hit(ltr_Scene* S, f3 A, f3 B, f3* outnormal)
 // ---------------------------------------------------------------------------
 // generating samples (lighter.cpp:32-66, 392-476)
 // ---------------------------------------------------------------------------
-static void rasterize_instance(Instance* mi, float margin, f3* R1, f3* R2, f4* R3)
+// The instance's triangles as the rasteriser draws them, in part and index
+// order: texel-space corners, world positions and normals, and
+// (uv0, part, texel area) per corner.
+static void raster_tris(Instance* mi, std::vector<RasterTri>& out)
 {
 	ltr_Mesh* mesh = mi->mesh;
 	for (uint32_t part = 0; part < mesh->parts.size(); ++part) {
 		part_triangles(mesh, mesh->parts[part], [&](uint32_t a, uint32_t b, uint32_t c) {
-			float area = sample_area(mi->ltex[a], mi->ltex[b], mi->ltex[c], mi->vpos[a], mi->vpos[b], mi->vpos[c]);
-			f4 ia{mi->vtex[a].x, mi->vtex[a].y, (float)part, area};
-			f4 ibv{mi->vtex[b].x, mi->vtex[b].y, (float)part, area};
-			f4 ic{mi->vtex[c].x, mi->vtex[c].y, (float)part, area};
-			rasterize_triangle(R1, R2, R3, (int32_t)mi->lm_w, (int32_t)mi->lm_h, margin, mi->ltex[a], mi->ltex[b],
-			                   mi->ltex[c], mi->vpos[a], mi->vpos[b], mi->vpos[c], mi->vnrm[a], mi->vnrm[b],
-			                   mi->vnrm[c], ia, ibv, ic);
+			const float area = sample_area(mi->ltex[a], mi->ltex[b], mi->ltex[c], mi->vpos[a], mi->vpos[b],
+			                               mi->vpos[c]);
+			const f4 ia{mi->vtex[a].x, mi->vtex[a].y, (float)part, area};
+			const f4 ib{mi->vtex[b].x, mi->vtex[b].y, (float)part, area};
+			const f4 ic{mi->vtex[c].x, mi->vtex[c].y, (float)part, area};
+			out.push_back(raster_setup(mi->ltex[a], mi->ltex[b], mi->ltex[c], mi->vpos[a], mi->vpos[b], mi->vpos[c],
+			                           mi->vnrm[a], mi->vnrm[b], mi->vnrm[c], ia, ib, ic));
 		});
 	}
 }
@@ -235,31 +240,77 @@ struct RasterSamples {
 	std::vector<float> dist; // square root of the world area per texel
 };
 
-static void raster_samples(Instance* mi, RasterSamples& out)
+// The host rasteriser: both passes of raster.h's rule in drawing order over
+// the instance's atlas, then the texels with a nonzero normal in texel order.
+static void raster_samples_host(Instance* mi, RasterSamples& out)
 {
-	size_t n = (size_t)mi->lm_w * mi->lm_h;
-	std::vector<f3> R1(n, mk3(0, 0, 0)), R2(n, mk3(0, 0, 0));
-	std::vector<f4> R3(n, f4{0, 0, 0, 0});
-	rasterize_instance(mi, 0.5f + kSmall, R1.data(), R2.data(), R3.data());
-	rasterize_instance(mi, 0.0f, R1.data(), R2.data(), R3.data());
-	size_t count = 0;
-	for (size_t i = 0; i < n; ++i)
-		if (!is_zero(R2[i]))
-			count++;
-	out.pos.reserve(count);
-	out.nrm.reserve(count);
-	out.info.reserve(count);
-	out.loc.reserve(count);
-	out.dist.reserve(count);
+	std::vector<RasterTri> tris;
+	raster_tris(mi, tris);
+	const int32_t w = (int32_t)mi->lm_w, h = (int32_t)mi->lm_h;
+	const size_t n = (size_t)mi->lm_w * mi->lm_h;
+	std::vector<f3> pos(n, mk3(0, 0, 0)), nrm(n, mk3(0, 0, 0));
+	std::vector<f4> inf(n, f4{0, 0, 0, 0});
+	for (const float margin : {kRasterMargin0, 0.0f})
+		for (const RasterTri& t : tris) {
+			PixelBox b;
+			if (!raster_box(t, margin, w, h, &b))
+				continue;
+			for (int32_t y = b.y0; y <= b.y1; ++y)
+				for (int32_t x = b.x0; x <= b.x1; ++x)
+					if (raster_covers(t, margin, x, y)) {
+						const size_t o = (size_t)x + (size_t)w * (size_t)y;
+						raster_values(t, x, y, &pos[o], &nrm[o], &inf[o]);
+					}
+		}
 	for (size_t i = 0; i < n; ++i) {
-		if (is_zero(R2[i]))
+		if (is_zero(nrm[i]))
 			continue;
-		out.pos.push_back(R1[i]);
-		out.nrm.push_back(normalized(R2[i]));
-		out.info.push_back(R3[i]);
+		out.pos.push_back(pos[i]);
+		out.nrm.push_back(normalized(nrm[i]));
+		out.info.push_back(inf[i]);
 		out.loc.push_back((uint32_t)i);
-		out.dist.push_back(sqrtf(R3[i].w));
+		out.dist.push_back(sqrtf(inf[i].w));
 	}
+}
+
+// Every instance rasterised in one set of GPU launches (raster.hip); the
+// same samples as raster_samples_host's.
+static void raster_samples_gpu(ltr_Scene* S, std::vector<RasterSamples>& raw)
+{
+	std::vector<size_t> ids;
+	for (size_t i = 0; i < S->insts.size(); ++i)
+		if (!S->insts[i]->samplecont)
+			ids.push_back(i);
+	std::vector<std::vector<RasterTri>> per(ids.size());
+	parallel_for(S->nthreads, ids.size(), [&](size_t q) { raster_tris(S->insts[ids[q]], per[q]); });
+	std::vector<RasterTri> tris;
+	std::vector<uint32_t> tri_atlas;
+	std::vector<RasterAtlasSize> sizes;
+	for (size_t q = 0; q < ids.size(); ++q) {
+		tris.insert(tris.end(), per[q].begin(), per[q].end());
+		tri_atlas.insert(tri_atlas.end(), per[q].size(), (uint32_t)q);
+		sizes.push_back(RasterAtlasSize{S->insts[ids[q]]->lm_w, S->insts[ids[q]]->lm_h});
+	}
+	if (ids.empty())
+		return;
+	require_gpu();
+	std::vector<RasterOut> out;
+	rasterize_gpu(tris, tri_atlas, sizes, out, &S->raster_times);
+	parallel_for(S->nthreads, ids.size(), [&](size_t q) {
+		const RasterOut& o = out[q];
+		RasterSamples& r = raw[ids[q]];
+		r.pos.resize(o.count);
+		r.nrm.resize(o.count);
+		r.info.resize(o.count);
+		r.loc = o.loc;
+		r.dist.resize(o.count);
+		for (size_t i = 0; i < o.count; ++i) {
+			r.pos[i] = ld3(&o.pos[3 * i]);
+			r.nrm[i] = ld3(&o.nrm[3 * i]);
+			r.info[i] = f4{o.inf[4 * i], o.inf[4 * i + 1], o.inf[4 * i + 2], o.inf[4 * i + 3]};
+			r.dist[i] = sqrtf(r.info[i].w);
+		}
+	});
 }
 
 // The host restatement of the placement of one instance's samples
@@ -343,22 +394,24 @@ static void commit_samples(ltr_Scene* S, Instance* mi, RasterSamples& r)
 	mi->lightmap.assign(mi->s_pos.size(), ld3(S->config.ambient_color));
 }
 
-// generating samples (lighter.cpp:392-476): rasterisation and compaction per
-// instance on the host threads, placement on the GPU (or, when selected,
-// the host restatement)
+// generating samples (lighter.cpp:392-476): rasterisation and compaction and
+// placement on the GPU (or, when selected, their host restatements)
 static void gen_samples(ltr_Scene* S)
 {
 	const int nt = S->nthreads;
 	std::vector<RasterSamples> raw(S->insts.size());
-	parallel_for(nt, S->insts.size(), [&](size_t i) {
-		Instance* mi = S->insts[i];
-		if (mi->samplecont) {
+	for (Instance* mi : S->insts)
+		if (mi->samplecont)
 			for (auto& c : mi->lightmap)
 				c = ld3(S->config.ambient_color);
-			return;
-		}
-		raster_samples(mi, raw[i]);
-	});
+	S->raster_times = RasterTimes{};
+	if (S->host_raster)
+		parallel_for(nt, S->insts.size(), [&](size_t i) {
+			if (!S->insts[i]->samplecont)
+				raster_samples_host(S->insts[i], raw[i]);
+		});
+	else
+		raster_samples_gpu(S, raw);
 	check_abort(S);
 	if (S->host_place)
 		parallel_for(nt, S->insts.size(), [&](size_t i) { place_host(S, raw[i]); });
@@ -702,137 +755,68 @@ void stage_ao(ltr_Scene* S)
 // ---------------------------------------------------------------------------
 // exporting lightmaps (lighter.cpp:849-1044, lighter_math.cpp:357-424)
 // ---------------------------------------------------------------------------
-static void gaussian_kernel(float* out, int ext, float radius)
-{
-	float sum = 0.0f;
-	float mc = 1.0f / sqrtf(2.0f * kPiF * radius * radius);
-	for (int i = -ext; i <= ext; ++i) {
-		float x = (float)i / radius;
-		sum += out[i + ext] = expf(-0.5f * (x * x)) * mc;
-	}
-	for (int i = -ext; i <= ext; ++i)
-		out[i + ext] /= sum;
-}
-
-static void convolve_transpose(const float* src, float* dst, uint32_t width, uint32_t height, int ext,
-                               const float* kernel, float* tmp)
-{
-	for (uint32_t y = 0; y < height; ++y) {
-		memcpy(tmp + ext * 3, src + (size_t)y * width * 3, sizeof(float) * width * 3);
-		for (int i = 0; i < ext; ++i)
-			for (int c = 0; c < 3; ++c) {
-				tmp[i * 3 + c] = tmp[ext * 3 + c];
-				tmp[(ext + width + i) * 3 + c] = tmp[(ext + width - 1) * 3 + c];
-			}
-		for (uint32_t x = 0; x < width; ++x) {
-			const float* sp = tmp + (x + ext) * 3;
-			float* dp = dst + ((size_t)y + (size_t)height * x) * 3;
-			float sum[3] = {0.0f, 0.0f, 0.0f};
-			for (int i = -ext; i <= ext; ++i) {
-				float kq = kernel[i + ext];
-				sum[0] += sp[i * 3 + 0] * kq;
-				sum[1] += sp[i * 3 + 1] * kq;
-				sum[2] += sp[i * 3 + 2] * kq;
-			}
-			dp[0] = sum[0];
-			dp[1] = sum[1];
-			dp[2] = sum[2];
-		}
-	}
-}
-
-static void downsample2x(float* dst, unsigned dw, unsigned dh, const float* src, unsigned sw, unsigned sh)
-{
-	for (unsigned y = 0; y < dh; ++y)
-		for (unsigned x = 0; x < dw; ++x) {
-			unsigned x0 = (x * 2) % sw, y0 = (y * 2) % sh, x1 = (x * 2 + 1) % sw, y1 = (y * 2 + 1) % sh;
-			f3 avg = (ld3(src + (x0 + y0 * sw) * 3) + ld3(src + (x1 + y0 * sw) * 3) + ld3(src + (x0 + y1 * sw) * 3) +
-			          ld3(src + (x1 + y1 * sw) * 3)) *
-			         0.25f;
-			st3(dst + (x + y * dw) * 3, avg);
-		}
-}
-
-// The image of one instance on the host (lighter.cpp:866-958): the samples'
-// colours in their texels, extrapolation, blur, downsample (the exported
-// size goes to out_w/out_h; lm_w/lm_h keep the atlas size the samples'
-// texel indices refer to).  Instances run in parallel.
+// The image of one instance on the host (lighter.cpp:866-958), texel by
+// texel through the functions the GPU export runs (texel.h): the samples'
+// colours in their texels, three rounds of extrapolation, the separable
+// blur, the 2x downsample.  The exported size goes to out_w/out_h; lm_w/lm_h
+// keep the atlas size the samples' texel indices refer to.  Instances run in
+// parallel.
 static float* image_host(ltr_Scene* S, size_t mid)
 {
 	Instance* mi = S->insts[mid];
 	const ltr_Config& cfg = S->config;
 	uint32_t w = mi->lm_w, h = mi->lm_h;
-	size_t npx = (size_t)w * h;
+	const size_t npx = (size_t)w * h;
 	for (uint32_t loc : mi->s_loc)
 		if (loc >= npx)
 			throw std::runtime_error("export: a sample's texel index lies outside its instance's atlas");
-	float* rgb = new float[npx * 3]();
-	std::vector<char> set(npx * 2, 0);
+	std::vector<float> img(3 * npx, 0.0f);
+	std::vector<uint8_t> flag(npx, 0);
 	for (size_t i = 0; i < mi->s_loc.size(); ++i) {
-		st3(rgb + (size_t)mi->s_loc[i] * 3, mi->lightmap[i]);
-		set[mi->s_loc[i]] = 1;
+		st3(&img[3 * (size_t)mi->s_loc[i]], mi->lightmap[i]);
+		flag[mi->s_loc[i]] = 1;
 	}
-	// three rounds of edge extrapolation into unset texels
-	auto isset = [&](uint32_t x, uint32_t y) { return x < w && y < h && set[x + (size_t)y * w]; };
-	auto col = [&](uint32_t x, uint32_t y) { return ld3(rgb + (x + (size_t)y * w) * 3); };
-	for (int its = 1; its <= 3; ++its) {
-		memcpy(set.data() + npx, set.data(), npx);
-		for (uint32_t y = 0; y < h; ++y) {
+	// each round reads the flags it started with and sets the new ones apart
+	for (int round = 0; round < 3; ++round) {
+		std::vector<uint8_t> grown(flag);
+		const AtlasView view{img.data(), flag.data(), w, h};
+		for (uint32_t y = 0; y < h; ++y)
 			for (uint32_t x = 0; x < w; ++x) {
-				if (set[x + (size_t)y * w])
-					continue;
-				f3 c = mk3(0, 0, 0);
-				int count = 0;
-				if (isset(x - 1, y)) {
-					c += isset(x - 2, y) ? lerp3(col(x - 2, y), col(x - 1, y), 2.0f) : col(x - 1, y);
-					count++;
-				}
-				if (isset(x + 1, y)) {
-					c += isset(x + 2, y) ? lerp3(col(x + 2, y), col(x + 1, y), 2.0f) : col(x + 1, y);
-					count++;
-				}
-				if (isset(x, y - 1)) {
-					c += isset(x, y - 2) ? lerp3(col(x, y - 2), col(x, y - 1), 2.0f) : col(x, y - 1);
-					count++;
-				}
-				if (isset(x, y + 1)) {
-					c += isset(x, y + 2) ? lerp3(col(x, y + 2), col(x, y + 1), 2.0f) : col(x, y + 1);
-					count++;
-				}
-				if (count) {
-					c = c / (float)count;
-					float* o = rgb + (x + (size_t)y * w) * 3;
-					o[0] = tmax(c.x, 0.0f);
-					o[1] = tmax(c.y, 0.0f);
-					o[2] = tmax(c.z, 0.0f);
-					set[x + (size_t)y * w + npx] = 1;
+				const size_t t = x + (size_t)y * w;
+				f3 c;
+				if (!flag[t] && extrapolate_texel(view, x, y, &c)) {
+					st3(&img[3 * t], c);
+					grown[t] = 1;
 				}
 			}
-		}
-		memcpy(set.data(), set.data() + npx, npx);
+		flag.swap(grown);
 	}
 	if (cfg.blur_size) {
-		int ext = (int)ceilf(cfg.blur_size);
-		size_t bufsz = ((size_t)std::max(w, h) + ext * 2) * 3;
-		std::vector<float> tmp(npx * 3 + bufsz + ext * 2 + 1);
-		float* img_tmp = tmp.data();
-		float* blur_tmp = img_tmp + npx * 3;
-		float* kernel = blur_tmp + bufsz;
-		gaussian_kernel(kernel, ext, cfg.blur_size);
-		convolve_transpose(rgb, img_tmp, w, h, ext, kernel, blur_tmp);
-		convolve_transpose(img_tmp, rgb, h, w, ext, kernel, blur_tmp);
+		const int ext = (int)ceilf(cfg.blur_size);
+		const std::vector<float> taps = blur_taps(ext, cfg.blur_size);
+		std::vector<float> across(3 * npx);
+		// rows of w texels into columns, then rows of h texels back
+		for (uint32_t y = 0; y < h; ++y)
+			for (uint32_t x = 0; x < w; ++x)
+				st3(&across[3 * (y + (size_t)h * x)], blur_texel(&img[3 * (size_t)y * w], w, (int)x, taps.data(), ext));
+		for (uint32_t x = 0; x < w; ++x)
+			for (uint32_t y = 0; y < h; ++y)
+				st3(&img[3 * (x + (size_t)w * y)], blur_texel(&across[3 * (size_t)x * h], h, (int)y, taps.data(), ext));
 	}
 	if (cfg.ds2x) {
-		unsigned hw = std::max(w / 2, 1u), hh = std::max(h / 2, 1u);
-		float* ds = new float[(size_t)hw * hh * 3];
-		downsample2x(ds, hw, hh, rgb, w, h);
-		delete[] rgb;
-		rgb = ds;
+		const uint32_t hw = std::max(w / 2, 1u), hh = std::max(h / 2, 1u);
+		std::vector<float> half(3 * (size_t)hw * hh);
+		for (uint32_t y = 0; y < hh; ++y)
+			for (uint32_t x = 0; x < hw; ++x)
+				st3(&half[3 * (x + (size_t)y * hw)], half_texel(img.data(), w, h, x, y));
+		img.swap(half);
 		w = hw;
 		h = hh;
 	}
 	mi->out_w = w;
 	mi->out_h = h;
+	float* rgb = new float[img.size()];
+	memcpy(rgb, img.data(), sizeof(float) * img.size());
 	return rgb;
 }
 
@@ -918,11 +902,8 @@ static void images_gpu(ltr_Scene* S, std::vector<float*>& rgb)
 		return;
 	require_gpu();
 	std::vector<float> kernel;
-	if (cfg.blur_size) {
-		int ext = (int)ceilf(cfg.blur_size);
-		kernel.resize(2 * ext + 1);
-		gaussian_kernel(kernel.data(), ext, cfg.blur_size);
-	}
+	if (cfg.blur_size)
+		kernel = blur_taps((int)ceilf(cfg.blur_size), cfg.blur_size);
 	// the colours stay on the device when a GPU stage left them there: the
 	// images' samples follow the sample container's in the device order
 	const float* d_rgb = nullptr;

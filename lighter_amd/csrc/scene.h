@@ -13,6 +13,7 @@
 #include "host_geom.h"
 #include "finalize.h"
 #include "place.h"
+#include "raster_gpu.h"
 
 namespace lgx {
 
@@ -99,9 +100,10 @@ struct ltr_Scene {
 	bool prepared = false;
 	std::vector<float> vis; // per work item shadow factor, when normal-map data is requested
 	lgx::StageTimes times;
-	// sample placement and lightmap export: on the GPU unless the host
-	// restatement is selected
-	bool host_place = false, host_finalize = false;
+	// rasterisation, sample placement and lightmap export: on the GPU unless
+	// the host restatement is selected
+	bool host_raster = false, host_place = false, host_finalize = false;
+	lgx::RasterTimes raster_times;
 	lgx::PlaceTimes place_times;
 	lgx::FinTimes fin_times;
 };

@@ -48,14 +48,18 @@ def prod_lightmap(lib, scene, i):
 
 
 # lgx_SetHostStages bits (include/lighter_gpu.h)
-GPU_PLACEMENT, HOST_PLACEMENT, HOST_FINALIZE = 0, 1, 2
+PLACE_BIT, FINALIZE_BIT, RASTER_BIT = 1, 2, 4
+GPU_PLACEMENT = 0                        # rasterisation and placement on the GPU
+HOST_PLACEMENT = PLACE_BIT | RASTER_BIT  # the host preparation: no GPU stage
+HOST_FINALIZE = FINALIZE_BIT
 HOST_STAGES = HOST_PLACEMENT | HOST_FINALIZE
 
 
 def prepared(lib, spec, placement=GPU_PLACEMENT, **config):
     """Build spec through the product library and run the preparation
-    stages, placing the samples on the GPU (the default) or with the host
-    restatement (placement = HOST_PLACEMENT)."""
+    stages: rasterisation and sample placement on the GPU (the default) or
+    with the host restatements (placement = HOST_PLACEMENT), or any other
+    lgx_SetHostStages mask."""
     b = scenes.BuiltScene(lib, spec, config or None)
     _lib.check(lib.lgx_SetHostStages(b.scene, placement), lib)
     _lib.check(lib.lgx_Prepare(b.scene), lib)

@@ -130,7 +130,7 @@ def test_samples_only_scene_without_lights_finishes(lib):
     spec.samples = [((0, 0, 1), (0, 0, 1)), ((1, 1, 1), (0, 1, 0))]
     spec.config = dict(spec.config, ambient_color=[0.25, 0.5, 0.75])
     b = scenes.BuiltScene(lib, spec)
-    lib.lgx_SetHostStages(b.scene, 3)  # placement and export on the host
+    lib.lgx_SetHostStages(b.scene, 7)  # rasterisation, placement and export on the host
     times = b.bake()
     assert "rendering lightmaps" not in times
     outs = b.outputs()
@@ -164,7 +164,7 @@ def test_abort_stops_bake(lib):
     spec = scenes.room_scene(lm=64)
     spec.lights = []
     b = scenes.BuiltScene(lib, spec)
-    lib.lgx_SetHostStages(b.scene, 3)  # placement and export on the host: no GPU needed here
+    lib.lgx_SetHostStages(b.scene, 7)  # rasterisation, placement and export on the host: no GPU needed here
     calls = []
 
     def size_fn(cfgp, mesh, mesh_size, inst, inst_size, area, importance, out):
@@ -195,12 +195,12 @@ def test_second_bake_gives_the_same_images(lib):
     images at the same (downsampled) size, and no more outputs than the
     first bake made.  ds2x halves the exported image while the samples keep
     their full-resolution texel indices; a re-export must not scatter those
-    into the halved image.  Placement and export on the host, no lights:
-    no GPU stage runs."""
+    into the halved image.  Rasterisation, placement and export on the
+    host, no lights: no GPU stage runs."""
     spec = scenes.room_scene(lm=64, bounce_count=0)
     spec.lights = []
     b = scenes.BuiltScene(lib, spec, dict(ds2x=1, blur_size=1.0))
-    assert lib.lgx_SetHostStages(b.scene, 3) == 1
+    assert lib.lgx_SetHostStages(b.scene, 7) == 1
     b.bake()
     first = b.outputs()
     assert len(first) == 1 and first[0][1] == first[0][2] == 32
@@ -219,15 +219,15 @@ def test_second_bake_gives_the_same_images(lib):
 
 
 def test_host_stage_switch(lib):
-    """lgx_SetHostStages takes the two documented bits and refuses others;
-    a scene baked with both host forms and no lights needs no GPU stage, and
-    reports no GPU placement or export times."""
+    """lgx_SetHostStages takes the three documented bits and refuses others;
+    a scene baked with every host form and no lights needs no GPU stage, and
+    reports no GPU rasterisation, placement or export times."""
     spec = scenes.room_scene(lm=64, bounce_count=0)
     spec.lights = []
     b = scenes.BuiltScene(lib, spec)
-    assert lib.lgx_SetHostStages(b.scene, 4) == 0
+    assert lib.lgx_SetHostStages(b.scene, 8) == 0
     assert "unknown bits" in _lib.last_error(lib)
-    assert lib.lgx_SetHostStages(b.scene, 3) == 1
+    assert lib.lgx_SetHostStages(b.scene, 7) == 1
     st = api.WorkStatus()
     lib.ltr_Start(b.scene)
     while lib.ltr_GetStatus(b.scene, C.byref(st)):
@@ -238,6 +238,8 @@ def test_host_stage_switch(lib):
     assert list(t) == [0.0] * 5
     lib.lgx_FinalizeTimes(b.scene, t)
     assert list(t) == [0.0] * 5
+    lib.lgx_RasterTimes(b.scene, t)
+    assert list(t)[:3] == [0.0] * 3
     outs = b.outputs()
     assert len(outs) == 1 and outs[0][1] == outs[0][2] == 64
     b.destroy()

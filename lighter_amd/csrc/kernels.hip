@@ -953,6 +953,12 @@ __global__ __launch_bounds__(kBlock) void k_color_grad(const uint32_t* __restric
 // ---------------------------------------------------------------------------
 // ambient occlusion (lighter.cpp:799-841)
 // ---------------------------------------------------------------------------
+// a value that every lane of the wave holds, kept in a scalar register
+__device__ __forceinline__ float wave_uniform(float x)
+{
+	return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
 struct AOParams {
 	int n;
 	float distance, multiplier, falloff, effect;
@@ -974,12 +980,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
 	extern __shared__ __align__(16) unsigned char lds_stack[];
 	LdsStack st = lds_stack_of_lane(lds_stack, depth);
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const uint32_t s = s0 + blockIdx.x * (kBlock / 64) + wave; // texels [s0, S)
+	// The wave's texel is uniform, so its normal, ray origin and rotation
+	// offset stay in scalar registers across the traversals: in vector
+	// registers they pushed the kernel past its 80-register budget, and the
+	// spills wrote about 1.5 KB per wave to scratch memory.
+	const uint32_t s = __builtin_amdgcn_readfirstlane(s0 + blockIdx.x * (kBlock / 64) + wave); // texels [s0, S)
 	if (s >= S)
 		return; // whole waves: no barrier below
-	f3 SP = ld3(pos + 3 * s), SN = ld3(nrm + 3 * s);
-	f3 origin = SP + SN * (kSmall * 2);
-	float ro = randoff[s];
+	const f3 SP = ld3(pos + 3 * s), SNv = ld3(nrm + 3 * s);
+	const f3 SN = mk3(wave_uniform(SNv.x), wave_uniform(SNv.y), wave_uniform(SNv.z));
+	const f3 ov = SP + SN * (kSmall * 2);
+	const f3 origin = mk3(wave_uniform(ov.x), wave_uniform(ov.y), wave_uniform(ov.z));
+	const float ro = wave_uniform(randoff[s]);
 	float ao = 0;
 	for (int i0 = 0; i0 < P.n; i0 += 64) {
 		const int i = i0 + lane;

@@ -142,6 +142,13 @@ LTRAPI int lgx_DirectLight(lgx_Device* dev, const lgx_Light* lights, int nlights
 LTRAPI int lgx_DirectLightSetup(lgx_Device* dev, const lgx_Light* lights, int nlights, const uint32_t* items,
                                 size_t nitems, const uint32_t* light_off, int keep_vis);
 LTRAPI int lgx_DirectLightRun(lgx_Device* dev);
+/* Abort of the direct-light march from inside the kernel (ltr_Abort, which
+ * is a no-op in the reference, lighter.cpp:1154-1157, stops a bake's march
+ * the same way through the scene's worker).  While on is set, every
+ * lgx_DirectLight / lgx_DirectLightRun on this device stops in its march
+ * kernel at the first claim of work, leaves the lightmap as it was and
+ * returns 0 with the error "direct light aborted"; on = 0 clears it. */
+LTRAPI int lgx_DeviceAbort(lgx_Device* dev, int on);
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  The pass of lgx_DirectLight (Job_LMRender_*_Inner,

@@ -33,6 +33,7 @@ LGX_SIGNATURES = {
     "lgx_PlacementTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
     "lgx_FinalizeTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
     "lgx_RasterTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
+    "lgx_DeviceAbort": (C.c_int, [vp, C.c_int]),
     "lgx_StageDirect": (C.c_int, [vp]),
     "lgx_StageRadiosity": (C.c_int, [vp]),
     "lgx_StageAO": (C.c_int, [vp]),

@@ -598,7 +598,15 @@ int lgx_DirectLight(lgx_Device* d, const lgx_Light* lights, int nlights, const u
 
 int lgx_DirectLightRun(lgx_Device* d)
 {
-	return guarded([&] { d->dev.run_direct(); });
+	return guarded([&] {
+		if (!d->dev.run_direct())
+			throw std::runtime_error("direct light aborted");
+	});
+}
+
+int lgx_DeviceAbort(lgx_Device* d, int on)
+{
+	return guarded([&] { d->dev.request_abort(on != 0); });
 }
 
 int lgx_DirectLightSetup(lgx_Device* d, const lgx_Light* lights, int nlights, const uint32_t* items, size_t nitems,

@@ -71,12 +71,22 @@ public:
 	// order; light_off has lights.size()+1 entries.  The march kernel writes
 	// one weight per item (bit pattern 0xFFFFFFFF where the reference skips
 	// the texel); the accumulation adds color * weight per light in order.
-	void direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
-	                  const std::vector<uint32_t>& light_off, bool keep_weights);
+	// keep_going (optional) is called about every millisecond while the
+	// march kernel runs, with the fraction of marches started; when it
+	// returns false the kernel is stopped from the inside (every wave leaves
+	// at its next claim of work).  Returns false when the run was aborted,
+	// by keep_going or by request_abort: the lightmap then keeps its colours
+	// and the weights are incomplete.
+	bool direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
+	                  const std::vector<uint32_t>& light_off, bool keep_weights,
+	                  const std::function<bool(float)>& keep_going = {});
 	// the same in two steps: upload the work once, run it (repeatedly)
 	void set_direct_work(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
 	                     const std::vector<uint32_t>& light_off, bool keep_weights);
-	void run_direct();
+	bool run_direct(const std::function<bool(float)>& keep_going = {});
+	// While set, every direct-light run stops inside its march kernel at the
+	// first claim of work and returns false.
+	void request_abort(bool on) { abort_requested_ = on; }
 	// The same pass without device allocations, on the caller's device
 	// buffers (items, weights, optional shadow factors) with host lights and
 	// light offsets; the kernels' scratch is a file-scope device array of
@@ -193,7 +203,7 @@ private:
 	DevLight* d_lights_ = nullptr;
 	float* d_gate_ = nullptr;     // shading factor per item (marched items)
 	uint32_t* d_mlist_ = nullptr; // compacted march list
-	uint32_t* d_mcount_ = nullptr; // [0] march count, [1] persistent-kernel cursor
+	uint32_t* d_mcount_ = nullptr; // [0] march count, [1] persistent-kernel cursor, [2] abort flag
 	uint32_t nmarch_ = 0;
 	unsigned persistent_blocks();
 	unsigned persist_blocks_ = 0;
@@ -203,6 +213,13 @@ private:
 	size_t nlinks_ = 0;
 	unsigned long long* d_counters_ = nullptr;
 	void* d_scratch_ = nullptr; // this GPU's allocation-free scratch array
+	// march control in pinned, coherent host memory: [0] abort flag (host
+	// writes, kernel reads), [1] march-list position last claimed (kernel
+	// writes, host reads); ctl_dev_ is its device address
+	uint32_t* ctl_host_ = nullptr;
+	uint32_t* ctl_dev_ = nullptr;
+	bool abort_requested_ = false;
+	bool wait_march(uint32_t nmarch, const std::function<bool(float)>& keep_going);
 };
 
 // Throws std::runtime_error unless a gfx950-class GPU is usable.

@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -807,7 +808,11 @@ __global__ __launch_bounds__(kBlock) void k_direct_gate(const float* __restrict_
 // Persistent shadow-march kernel: every lane pulls a new march from the list
 // as soon as its current one ends (one atomic per wave per refill), so the
 // very uneven march lengths do not idle the wave.  Every wave leaves once the
-// list is exhausted and its lanes' marches have ended.  (A variant that
+// list is exhausted and its lanes' marches have ended.  ctl (optional, host
+// memory): the claims of positions 0, 64 x kMarchBatch, .. post the
+// position to ctl[1] (progress) and, when ctl[0] is set, raise the device
+// flag next[1]; a wave that finds next[1] set at a claim leaves at once
+// (abort), so every wave stops within one claim of the first such post.  (A variant that
 // interleaves single traversal actions of all lanes in one loop measured
 // 37% slower on the terrain workload.)
 // The traversal stack's first levels live in LDS (dynamic, kBlock * depth * 6
@@ -820,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void k_march(View V, const float* __restric
                                                   const uint32_t* __restrict__ mlist, uint32_t nmarch,
                                                   const float* __restrict__ gate, float* __restrict__ wout,
                                                   float* __restrict__ vout, uint32_t* __restrict__ next,
-                                                  unsigned long long* cnt, int depth)
+                                                  unsigned long long* cnt, int depth, uint32_t* ctl)
 {
 	extern __shared__ __align__(16) unsigned char lds_stack[];
 	LdsStack st = lds_stack_of_lane(lds_stack, depth);
@@ -845,10 +850,25 @@ __global__ __launch_bounds__(kBlock) void k_march(View V, const float* __restric
 				idx = rbeg + rank;
 				rbeg += cnt_need;
 			} else {
-				uint32_t nb = 0;
-				if (lane == 0)
+				uint32_t nb = 0, stop = 0;
+				if (lane == 0) {
+					// every claim reads the device abort flag next[1] (an L2
+					// access in flight together with the claim's atomic);
+					// one claim in 64 also makes the host round trip:
+					// progress out, the host's abort flag in, passed on to
+					// next[1] (a host round trip at every claim cost 3 %)
+					if (ctl)
+						stop = __hip_atomic_load(next + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 					nb = atomicAdd(next, kMarchBatch);
+					if (ctl && ((nb / kMarchBatch) & 63u) == 0) {
+						__hip_atomic_store(ctl + 1, nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+						if (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+							__hip_atomic_store(next + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+					}
+				}
 				nb = (uint32_t)__shfl((int)nb, 0);
+				if (__shfl((int)stop, 0))
+					break; // aborted: the wave leaves its marches unfinished
 				idx = rank < avail ? rbeg + rank : nb + (rank - avail);
 				rbeg = nb + (cnt_need - avail);
 				rend = nb + kMarchBatch;
@@ -1543,8 +1563,13 @@ DeviceScene::DeviceScene()
 	HIPCHECK(hipStreamCreate(&s));
 	stream_ = s;
 	d_counters_ = dalloc<unsigned long long>(kNumCounters);
-	d_mcount_ = dalloc<uint32_t>(2);
+	d_mcount_ = dalloc<uint32_t>(3);
 	d_scratch_ = noalloc_scratch();
+	// the march's abort flag and progress word: pinned, coherent host memory
+	// the kernel reads and writes while it runs
+	HIPCHECK(hipHostMalloc((void**)&ctl_host_, 2 * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent));
+	ctl_host_[0] = ctl_host_[1] = 0u;
+	HIPCHECK(hipHostGetDevicePointer((void**)&ctl_dev_, ctl_host_, 0));
 }
 
 // Blocks of the persistent march kernel: as many as can be resident at once.
@@ -1575,6 +1600,8 @@ DeviceScene::~DeviceScene()
 	dfree(d_mcount_);
 	if (stream_)
 		(void)hipStreamDestroy((hipStream_t)stream_);
+	if (ctl_host_)
+		(void)hipHostFree(ctl_host_);
 }
 
 void DeviceScene::free_all()
@@ -1808,14 +1835,51 @@ void DeviceScene::set_direct_work(const std::vector<DevLight>& lights, const std
 	h_light_off_ = light_off;
 }
 
-void DeviceScene::run_direct()
+// While the march kernel runs: progress to keep_going (the fraction of the
+// march list claimed) every millisecond, and the abort flag raised when it
+// returns false.  True when the march was aborted (also when the device's
+// abort request was set before the launch: the kernel then leaves at once).
+bool DeviceScene::wait_march(uint32_t nmarch, const std::function<bool(float)>& keep_going)
+{
+	if (abort_requested_)
+		return true;
+	if (!keep_going)
+		return false;
+	hipStream_t s = (hipStream_t)stream_;
+	volatile uint32_t* ctl = ctl_host_;
+	bool stop = false;
+	for (;;) {
+		const hipError_t e = hipStreamQuery(s);
+		if (e == hipSuccess)
+			break;
+		if (e != hipErrorNotReady)
+			HIPCHECK(e);
+		if (!stop) {
+			const float f = std::min(1.0f, (float)ctl[1] / (float)nmarch);
+			if (!keep_going(f)) {
+				stop = true;
+				ctl[0] = 1u;
+			}
+		}
+		std::this_thread::sleep_for(std::chrono::milliseconds(1));
+	}
+	return stop;
+}
+
+bool DeviceScene::run_direct(const std::function<bool(float)>& keep_going)
 {
 	hipStream_t s = (hipStream_t)stream_;
 	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	bool aborted = false;
+	ctl_host_[0] = 0u; // read by the kernel after the launch below
+	ctl_host_[1] = 0u;
 	HIPCHECK(hipMemsetAsync(d_counters_, 0, kNumCounters * sizeof(unsigned long long), s));
 	EventTimer all(s);
 	if (nitems_) {
 		HIPCHECK(hipMemsetAsync(d_mcount_, 0, 2 * sizeof(uint32_t), s));
+		// the device abort flag starts raised under request_abort: every wave
+		// then leaves at its first claim, before any march
+		HIPCHECK(hipMemsetAsync(d_mcount_ + 2, abort_requested_ ? 1 : 0, sizeof(uint32_t), s));
 		k_direct_gate<<<(unsigned)((nitems_ + kGateRounds * kBlock - 1) / (kGateRounds * kBlock)), kBlock, 0, s>>>(
 		    d_pos_, d_nrm_, d_lights_, d_items_, (uint32_t)nitems_, d_gate_, d_w_, d_vis_, d_mlist_, d_mcount_, d_skey_);
 		HIPCHECK(hipGetLastError());
@@ -1827,17 +1891,26 @@ void DeviceScene::run_direct()
 		if (nmarch) {
 			unsigned blocks = persistent_blocks();
 			const size_t lds = march_lds_bytes();
+			// the abort and progress path only for runs that can be stopped
+			// (it costs up to 2 % of the march on the city block)
+			uint32_t* ctl = (keep_going || abort_requested_) ? ctl_dev_ : nullptr;
 			if (count_queries)
 				k_march<true><<<blocks, kBlock, lds, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch,
 				                                          d_gate_, d_w_, d_vis_, d_mcount_ + 1, d_counters_,
-				                                          march_stack_depth());
+				                                          march_stack_depth(), ctl);
 			else
 				k_march<false><<<blocks, kBlock, lds, s>>>(V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch,
 				                                           d_gate_, d_w_, d_vis_, d_mcount_ + 1, d_counters_,
-				                                           march_stack_depth());
+				                                           march_stack_depth(), ctl);
 			HIPCHECK(hipGetLastError());
+			aborted = wait_march(nmarch, keep_going);
 		}
 		times.direct_march_ms = tm.stop();
+	}
+	if (aborted) { // the lightmap keeps its colours
+		times.direct_ms = all.stop();
+		read_counters();
+		return false;
 	}
 	for (size_t l = 0; l < h_lights_.size(); ++l) {
 		uint32_t b = h_light_off_[l], e = h_light_off_[l + 1];
@@ -1849,6 +1922,7 @@ void DeviceScene::run_direct()
 	}
 	times.direct_ms = all.stop();
 	read_counters();
+	return true;
 }
 
 // [p, p + bytes) must lie inside one device allocation of the current GPU.
@@ -1917,7 +1991,8 @@ void DeviceScene::run_direct_noalloc(const DevLight* lights, int nlights, const 
 	const uint32_t nmarch = h[0];
 	if (nmarch) {
 		k_march<false><<<persistent_blocks(), kBlock, march_lds_bytes(), s>>>(
-		    V, d_pos_, d_nrm_, dl, d_items, mlist, nmarch, gate, d_w, d_vis, head + 1, cnt, march_stack_depth());
+		    V, d_pos_, d_nrm_, dl, d_items, mlist, nmarch, gate, d_w, d_vis, head + 1, cnt, march_stack_depth(),
+		    nullptr);
 		HIPCHECK(hipGetLastError());
 	}
 	for (int l = 0; l < nlights; ++l) {
@@ -2134,11 +2209,12 @@ void DeviceScene::read_counters()
 	counters.pretests = c[5];
 }
 
-void DeviceScene::direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
-                               const std::vector<uint32_t>& light_off, bool keep_weights)
+bool DeviceScene::direct_light(const std::vector<DevLight>& lights, const std::vector<uint32_t>& items,
+                               const std::vector<uint32_t>& light_off, bool keep_weights,
+                               const std::function<bool(float)>& keep_going)
 {
 	set_direct_work(lights, items, light_off, keep_weights);
-	run_direct();
+	return run_direct(keep_going);
 }
 
 void DeviceScene::direct_grad(const double* lam3, double* dcolor3)

@@ -631,7 +631,14 @@ void stage_direct(ltr_Scene* S)
 			off.push_back((uint32_t)items.size());
 			ids_of.push_back(std::move(ids));
 		}
-		S->dev->direct_light(dl, items, off, want_vis);
+		// progress and abort while the march kernel runs
+		const float n_lights = (float)S->lights.size();
+		const bool done = S->dev->direct_light(dl, items, off, want_vis, [&](float f) {
+			S->completion.store(((float)l0 + f * (float)(l1 - l0)) / n_lights);
+			return !S->abort.load();
+		});
+		if (!done)
+			throw AbortBake{};
 		S->times.direct_ms += S->dev->times.direct_ms;
 		S->times.direct_march_ms += S->dev->times.direct_march_ms;
 		if (want_vis) {

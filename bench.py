@@ -812,6 +812,14 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
     the lightmap export and the sample placement beside their host forms."""
     S, dev, tris, items = leg.S, leg.dev, leg.tris, leg.items
     out = {}
+    # the device scene's BVH: the GPU build against the host build
+    bv = (C.c_double * 6)()
+    _lib.check(lib.lgx_BvhCompare(P(tris), len(tris), bv), lib)
+    out["bvh_build"] = dict(triangles=int(len(tris)), wide_nodes=int(bv[5]), host_ms=round(bv[2], 1),
+                            gpu_ms=round(bv[3], 1), gpu_device_ms=round(bv[4], 1),
+                            identical_tree=bool(bv[0] == 1 and bv[1] == 1))
+    log(f"[bench] C4 BVH build: GPU {bv[3]:.1f} ms ({bv[4]:.1f} ms device, collapse on the host) vs host "
+        f"{bv[2]:.1f} ms; identical tree: {out['bvh_build']['identical_tree']}")
     rng = np.random.default_rng(0x4C540004)
     ro = rng.uniform(0, 1, S).astype(np.float32)
     col = np.zeros(3, np.float32)

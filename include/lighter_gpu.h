@@ -142,6 +142,15 @@ LTRAPI int lgx_DirectLight(lgx_Device* dev, const lgx_Light* lights, int nlights
 LTRAPI int lgx_DirectLightSetup(lgx_Device* dev, const lgx_Light* lights, int nlights, const uint32_t* items,
                                 size_t nitems, const uint32_t* light_off, int keep_vis);
 LTRAPI int lgx_DirectLightRun(lgx_Device* dev);
+/* No counterpart in the reference (its AABB tree builds on the host,
+ * lighter_math.cpp:653-801; the device scene's BVH, bvh.h, is this
+ * library's).  Builds the device scene's BVH of a triangle soup (9 floats
+ * each) twice, with the host build and with the GPU build that
+ * lgx_DeviceCreate uses: out[0] = 1 when the 4-wide node arrays are
+ * identical byte for byte, out[1] = 1 when the leaf orders and depths are,
+ * out[2] host build ms, out[3] GPU build ms (wall, collapse included),
+ * out[4] GPU kernels and level read-backs ms (HIP events), out[5] nodes. */
+LTRAPI int lgx_BvhCompare(const float* tris, size_t ntri, double out[6]);
 /* Abort of the direct-light march from inside the kernel (ltr_Abort, which
  * is a no-op in the reference, lighter.cpp:1154-1157, stops a bake's march
  * the same way through the scene's worker).  While on is set, every

@@ -34,6 +34,7 @@ LGX_SIGNATURES = {
     "lgx_FinalizeTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
     "lgx_RasterTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
     "lgx_DeviceAbort": (C.c_int, [vp, C.c_int]),
+    "lgx_BvhCompare": (C.c_int, [fp, C.c_size_t, C.POINTER(C.c_double)]),
     "lgx_StageDirect": (C.c_int, [vp]),
     "lgx_StageRadiosity": (C.c_int, [vp]),
     "lgx_StageAO": (C.c_int, [vp]),

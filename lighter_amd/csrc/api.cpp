@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../include/lighter_gpu.h"
+#include "bvh.h"
 #include "pipeline.h"
 #include "radiosity.h"
 
@@ -601,6 +602,14 @@ int lgx_DirectLightRun(lgx_Device* d)
 	return guarded([&] {
 		if (!d->dev.run_direct())
 			throw std::runtime_error("direct light aborted");
+	});
+}
+
+int lgx_BvhCompare(const float* tris, size_t ntri, double out[6])
+{
+	return guarded([&] {
+		require_gpu();
+		compare_bvh_builds(tris, ntri, out);
 	});
 }
 

@@ -46,11 +46,9 @@ struct Aabb {
 	}
 };
 
-constexpr int kBins = 32; // bins of the build the kernels traverse (at most kMaxBins)
+constexpr int kBins = kSahBins; // bins of the build the kernels traverse (at most kMaxBins)
 constexpr int kMaxBins = 32;
-// Below this depth SAH splits; deeper subtrees split at the median so the
-// total depth stays under the kernels' traversal stack (kStackSize).
-constexpr int kMedianDepth = 40;
+static_assert(kBins <= kMaxBins, "bins");
 
 struct Builder {
 	const float* tris;
@@ -131,7 +129,8 @@ struct Builder {
 		return o;
 	}
 
-	void set_child(BvhNode& n, int which, const Aabb& b, int32_t ref)
+	void set_child(BvhNode& n, int which, const Aabb& b, int32_t ref) { set_child_of(n, which, b, ref); }
+	static void set_child_of(BvhNode& n, int which, const Aabb& b, int32_t ref)
 	{
 		Aabb p = padded(b);
 		if (which == 0) {
@@ -241,22 +240,16 @@ struct Builder {
 				}
 			}
 		}
+		// Order-independent rules throughout (bins, stable partition, the
+		// index split), so the GPU build (bvh_gpu.hip) gives the same tree.
 		uint32_t mid;
-		if (depth >= kMedianDepth) {
-			// bound the depth: balanced split along the widest centroid axis
-			int a = 0;
-			for (int k = 1; k < 3; ++k)
-				if (cb.hi[k] - cb.lo[k] > cb.hi[a] - cb.lo[a])
-					a = k;
+		if (depth >= kSahMaxDepth || best_axis < 0) {
+			// bound the depth; coincident centroids: split by position
 			mid = b + n / 2;
-			std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e,
-			                 [&](uint32_t x, uint32_t y) { return cen[3 * x + a] < cen[3 * y + a]; });
-		} else if (best_axis < 0) {
-			mid = b + n / 2; // coincident centroids: split by index
 		} else {
 			float ext = cb.hi[best_axis] - cb.lo[best_axis];
 			float sc = bins / ext;
-			auto it = std::partition(idx.begin() + b, idx.begin() + e, [&](uint32_t t) {
+			auto it = std::stable_partition(idx.begin() + b, idx.begin() + e, [&](uint32_t t) {
 				int k = std::min(bins - 1, (int)((cen[3 * t + best_axis] - cb.lo[best_axis]) * sc));
 				return k <= best_split;
 			});
@@ -501,6 +494,31 @@ static void build_binary(Builder& B, const Aabb& all, size_t ntri)
 	}
 }
 
+void collapse_bvh(const std::vector<BvhNode>& bin, BvhBuild& out)
+{
+	out.nodes.clear();
+	Collapser W{bin, out.nodes};
+	out.nodes.reserve(bin.size() / 2 + 1);
+	W.collapse(0, 0);
+	out.depth = W.max_depth;
+}
+
+BvhNode bvh_node(const float lo0[3], const float hi0[3], int32_t c0, const float lo1[3], const float hi1[3],
+                 int32_t c1)
+{
+	BvhNode n{};
+	Aabb a, b;
+	for (int k = 0; k < 3; ++k) {
+		a.lo[k] = lo0[k];
+		a.hi[k] = hi0[k];
+		b.lo[k] = lo1[k];
+		b.hi[k] = hi1[k];
+	}
+	Builder::set_child_of(n, 0, a, c0);
+	Builder::set_child_of(n, 1, b, c1);
+	return n;
+}
+
 BvhBuild build_bvh(const float* tris, size_t ntri)
 {
 	Builder B;
@@ -513,13 +531,10 @@ BvhBuild build_bvh(const float* tris, size_t ntri)
 		return out;
 	}
 	build_binary(B, all, ntri);
-	Collapser W{B.nodes, out.nodes};
-	out.nodes.reserve(B.nodes.size() / 2 + 1);
-	W.collapse(0, 0);
+	collapse_bvh(B.nodes, out);
 	out.order = std::move(B.idx);
 	out.lo = mk3(all.lo[0], all.lo[1], all.lo[2]);
 	out.hi = mk3(all.hi[0], all.hi[1], all.hi[2]);
-	out.depth = W.max_depth;
 	return out;
 }
 

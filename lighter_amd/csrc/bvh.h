@@ -46,6 +46,8 @@ static_assert(sizeof(WideNode) == 128, "wide node size");
 constexpr float kFarBox = 1e37f; // corner of the unused slots' degenerate box
 
 constexpr int kLeafMax = 2; // measured on the terrain workload: 2 beats 1, 4 and 8
+constexpr int kSahBins = 32;     // SAH bins per axis of the build the kernels traverse
+constexpr int kSahMaxDepth = 40; // from this binary depth on, subtrees split by position (depth bound)
 
 LHD int32_t leaf_ref(uint32_t first, uint32_t count) { return ~(int32_t)((first << 4) | (count - 1)); }
 LHD uint32_t leaf_first(int32_t ref) { return ((uint32_t)~ref) >> 4; }
@@ -58,8 +60,32 @@ struct BvhBuild {
 	int depth = 0;               // levels of wide nodes on the longest path
 };
 
-// tris: 9 floats per triangle.
+// tris: 9 floats per triangle.  Binned SAH (kSahBins per axis, the split of
+// least SAH cost over the axes with a centroid extent), stable partitions,
+// a split by position where all centroids coincide or below kSahMaxDepth,
+// leaves of at most kLeafMax triangles; then collapsed into wide nodes.
 BvhBuild build_bvh(const float* tris, size_t ntri);
+
+// The same build with the binary tree and the leaf order made on the GPU
+// (bvh_gpu.hip) from the triangles on the device (d_tris, 9 floats each;
+// stream: a hipStream_t): every node, box and leaf of the host build,
+// collapsed on the host.  d_order (ntri entries, device) receives the leaf
+// order; kernel_ms the device build's time (HIP events).
+BvhBuild build_bvh_gpu(const float* d_tris, size_t ntri, uint32_t* d_order, void* stream, double* kernel_ms);
+
+// Both builds of one triangle soup (host array): out[0] = 1 when the wide
+// node arrays are identical byte for byte, out[1] = 1 when the leaf orders
+// and depths are, out[2] host ms, out[3] GPU ms (wall), out[4] GPU kernels
+// and read-backs ms, out[5] wide nodes.
+void compare_bvh_builds(const float* tris, size_t ntri, double out[6]);
+
+// The collapse of a binary tree (root at 0) into 4-wide nodes; sets nodes
+// and depth of out.
+void collapse_bvh(const std::vector<BvhNode>& bin, BvhBuild& out);
+// A binary node from its children's (unpadded) boxes and references, the
+// boxes padded as the build pads them.
+BvhNode bvh_node(const float lo0[3], const float hi0[3], int32_t c0, const float lo1[3], const float hi1[3],
+                 int32_t c1);
 
 // A plain binary binned-SAH tree (bins per axis, at most leaf_max triangles
 // per leaf), not collapsed: the reference tree shape the traversal byte

@@ -34,6 +34,7 @@ constexpr int kMaxDevices = 64;
 // the kernels run on.
 struct StageTimes {
 	double direct_ms = 0, direct_march_ms = 0, ao_ms = 0, links_ms = 0, bounce_ms = 0, grad_ms = 0;
+	double bvh_ms = 0; // the device's BVH build (kernels and the level read-backs)
 };
 
 // Optional traversal counters (when enabled): node visits and triangle

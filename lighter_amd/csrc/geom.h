@@ -59,7 +59,7 @@ struct alignas(16) RayTri {
 	float nx, ny, nz, D;
 };
 
-inline RayTri make_ray_tri(f3 P1, f3 P2, f3 P3)
+LHD RayTri make_ray_tri(f3 P1, f3 P2, f3 P3)
 {
 	f3 u = P2 - P1, v = P3 - P1, n = cross(u, v);
 	float uu = dot(u, u), uv = dot(u, v), vv = dot(v, v);
@@ -140,7 +140,7 @@ struct alignas(16) DistTri {
 	float e2x, e2y, e2z, e2d;     // edge normal 2, en2 . t2
 };
 
-inline DistTri make_dist_tri(f3 t0, f3 t1, f3 t2)
+LHD DistTri make_dist_tri(f3 t0, f3 t1, f3 t2)
 {
 	f3 nrm = normalized(cross(t1 - t0, t2 - t0));
 	f3 tan0 = normalized(t1 - t0), tan1 = normalized(t2 - t1), tan2 = normalized(t0 - t2);

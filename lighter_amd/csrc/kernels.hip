@@ -1495,6 +1495,15 @@ template <class T> static void dfree(T*& p)
 	p = nullptr;
 }
 
+// a device array owned for one scope
+template <class T> struct DevArray {
+	T* p = nullptr;
+	explicit DevArray(size_t n) : p(dalloc<T>(n)) {}
+	DevArray(const DevArray&) = delete;
+	DevArray& operator=(const DevArray&) = delete;
+	~DevArray() { dfree(p); }
+};
+
 // ---------------------------------------------------------------------------
 // Scratch of the allocation-free direct-light pass (run_direct_noalloc): the
 // buffers run_direct allocates (per-item shading factor, the march list, the
@@ -1649,61 +1658,54 @@ static View make_view(void* nodes, void* dtri, void* rtri, size_t ntri, const fl
 	return v;
 }
 
+// The triangles' precomputed forms in leaf order (make_dist_tri and
+// make_ray_tri, the host's functions, on the device).
+__global__ __launch_bounds__(kBlock) void k_tri_forms(const float* __restrict__ tris,
+                                                      const uint32_t* __restrict__ order, size_t n,
+                                                      DistTri* __restrict__ dt, RayTri* __restrict__ rt)
+{
+	const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n)
+		return;
+	const float* p = tris + 9 * (size_t)order[i];
+	const f3 a = ld3(p), c = ld3(p + 3), e = ld3(p + 6);
+	dt[i] = make_dist_tri(a, c, e);
+	rt[i] = make_ray_tri(a, c, e);
+}
+
 void DeviceScene::set_triangles(const float* tris, size_t ntri)
 {
 	dfree(d_nodes_);
 	dfree(d_dtri_);
 	dfree(d_rtri_);
 	ntri_ = ntri;
+	times.bvh_ms = 0;
 	if (ntri == 0) {
 		nnodes_ = 0;
 		return;
 	}
 	if (ntri >= (1u << 27))
 		throw std::runtime_error("too many triangles for the leaf encoding");
-	BvhBuild b = build_bvh(tris, ntri);
+	hipStream_t s = (hipStream_t)stream_;
+	// triangles up, the tree built on the device (the host build's tree,
+	// bvh_gpu.hip), the triangle forms made there in leaf order
+	DevArray<float> d_tris(9 * ntri);
+	DevArray<uint32_t> d_order(ntri);
+	HIPCHECK(hipMemcpyAsync(d_tris.p, tris, 36 * ntri, hipMemcpyHostToDevice, s));
+	double kms = 0;
+	BvhBuild b = build_bvh_gpu(d_tris.p, ntri, d_order.p, stream_, &kms);
+	times.bvh_ms = kms;
 	if (3 * b.depth + 1 > kStackSize)
 		throw std::runtime_error("BVH deeper than the traversal stack");
 	bvh_depth_ = b.depth;
 	persist_blocks_ = 0; // the march kernel's LDS stack follows the depth
 	nnodes_ = b.nodes.size();
-	// the triangles' precomputed forms, in leaf order, written by the host
-	// threads straight into pinned memory (one DMA each)
-	const size_t nb = nnodes_ * sizeof(WideNode), db = ntri * sizeof(DistTri), rb = ntri * sizeof(RayTri);
-	char* pin = nullptr;
-	HIPCHECK(hipHostMalloc((void**)&pin, nb + db + rb, hipHostMallocDefault));
-	struct PinGuard {
-		char* p;
-		~PinGuard() { (void)hipHostFree(p); }
-	} pg{pin};
-	memcpy(pin, b.nodes.data(), nb);
-	DistTri* dt = (DistTri*)(pin + nb);
-	RayTri* rt = (RayTri*)(pin + nb + db);
-	{
-		const unsigned hw = std::thread::hardware_concurrency();
-		const size_t nt = std::min<size_t>(std::min(16u, std::max(1u, hw)), (ntri + 4095) / 4096);
-		auto fill = [&](size_t t) {
-			for (size_t i = ntri * t / nt; i < ntri * (t + 1) / nt; ++i) {
-				const float* p = tris + 9 * (size_t)b.order[i];
-				f3 a = ld3(p), c = ld3(p + 3), e = ld3(p + 6);
-				dt[i] = make_dist_tri(a, c, e);
-				rt[i] = make_ray_tri(a, c, e);
-			}
-		};
-		std::vector<std::thread> th;
-		for (size_t t = 1; t < nt; ++t)
-			th.emplace_back(fill, t);
-		fill(0);
-		for (auto& x : th)
-			x.join();
-	}
 	d_nodes_ = dalloc<WideNode>(nnodes_);
 	d_dtri_ = dalloc<DistTri>(ntri);
 	d_rtri_ = dalloc<RayTri>(ntri);
-	hipStream_t s = (hipStream_t)stream_;
-	HIPCHECK(hipMemcpyAsync(d_nodes_, pin, nb, hipMemcpyHostToDevice, s));
-	HIPCHECK(hipMemcpyAsync(d_dtri_, dt, db, hipMemcpyHostToDevice, s));
-	HIPCHECK(hipMemcpyAsync(d_rtri_, rt, rb, hipMemcpyHostToDevice, s));
+	HIPCHECK(hipMemcpyAsync(d_nodes_, b.nodes.data(), sizeof(WideNode) * nnodes_, hipMemcpyHostToDevice, s));
+	k_tri_forms<<<grid_for(ntri), kBlock, 0, s>>>(d_tris.p, d_order.p, ntri, (DistTri*)d_dtri_, (RayTri*)d_rtri_);
+	HIPCHECK(hipGetLastError());
 	HIPCHECK(hipStreamSynchronize(s));
 	lo_[0] = b.lo.x; lo_[1] = b.lo.y; lo_[2] = b.lo.z;
 	hi_[0] = b.hi.x; hi_[1] = b.hi.y; hi_[2] = b.hi.z;

@@ -104,6 +104,19 @@ public:
 	void get_vis(float* out, size_t n) const;
 	size_t num_items() const { return nitems_; }
 
+	// --- normal-map data (lighter.cpp:505-513, 959-1030) ----------------
+	// Keeps the last direct-light run's shadow factors (run with kept
+	// weights) with its lights; first_item[l * ninst + i] is the item of
+	// instance i's first sample in light l's items, or -1.
+	void nm_keep_batch(const std::vector<DevLight>& lights, const std::vector<int32_t>& first_item, size_t ninst);
+	size_t nm_lights() const { return nm_lights_.size(); }
+	void nm_clear();
+	// Per sample: the normal (normalised mean of the sample normal scaled by
+	// ambient_bright and the kept contributions, in light order) and the
+	// min-dot term, 4 floats into out4; inst_off: first sample of each
+	// instance plus the total.
+	void nm_compute(const std::vector<uint32_t>& inst_off, float ambient_bright, float* out4);
+
 	// --- ambient occlusion (lighter.cpp:799-841) -----------------------
 	// fac_out (device, one float per sample, optional) receives each
 	// texel's AO factor before the colour blend.  With between, the texels
@@ -217,6 +230,12 @@ private:
 	// march control in pinned, coherent host memory: [0] abort flag (host
 	// writes, kernel reads), [1] march-list position last claimed (kernel
 	// writes, host reads); ctl_dev_ is its device address
+	std::vector<DevLight> nm_lights_;      // every kept light, in order
+	std::vector<int32_t> nm_first_;        // per kept light: first item per instance
+	std::vector<float*> nm_vis_;           // per kept batch: its shadow factors (device)
+	std::vector<uint32_t> nm_light_batch_; // batch of each kept light
+	std::vector<size_t> nm_batch_items_;   // items of each kept batch
+	size_t nm_ninst_ = 0;
 	uint32_t* ctl_host_ = nullptr;
 	uint32_t* ctl_dev_ = nullptr;
 	bool abort_requested_ = false;

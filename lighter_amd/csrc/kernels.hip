@@ -27,6 +27,7 @@
 #include "bvh.h"
 #include "device.h"
 #include "geom.h"
+#include "normals.h"
 
 namespace lgx {
 
@@ -1615,6 +1616,7 @@ DeviceScene::~DeviceScene()
 
 void DeviceScene::free_all()
 {
+	nm_clear();
 	dfree(d_cand_);
 	ncand_ = 0;
 	dfree(d_nodes_);
@@ -2217,6 +2219,132 @@ bool DeviceScene::direct_light(const std::vector<DevLight>& lights, const std::v
 {
 	set_direct_work(lights, items, light_off, keep_weights);
 	return run_direct(keep_going);
+}
+
+// ---------------------------------------------------------------------------
+// Normal-map data on the GPU (lighter.cpp:505-513, 959-1030): one lane per
+// sample walks the kept lights in order twice, as the host restatement
+// walks its contribution list: first the sum of the scaled sample normal
+// and the contributions and their count, then, against the mean, the
+// min-dot product (normals.h serves both sides).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_nm_data(const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                    uint32_t n, const uint32_t* __restrict__ inst_off, int ninst,
+                                                    const DevLight* __restrict__ lights, int nl,
+                                                    const int32_t* __restrict__ first,
+                                                    const float* const* __restrict__ visp, float ab,
+                                                    float* __restrict__ out)
+{
+	const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+	if (s >= n)
+		return;
+	int lo = 0, hi = ninst - 1; // the sample's instance: last with inst_off <= s
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (inst_off[mid] <= s)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	const int inst = lo;
+	const uint32_t within = s - inst_off[inst];
+	const f3 SP = ld3(pos + 3 * (size_t)s), SN = ld3(nrm + 3 * (size_t)s);
+	f3 sum = SN * ab;
+	int count = 1;
+	for (int l = 0; l < nl; ++l) {
+		const int32_t f = first[(size_t)l * ninst + inst];
+		f3 c;
+		if (f >= 0 && nm_contrib(lights[l], SP, SN, visp[l][(uint32_t)f + within], &c)) {
+			sum += c;
+			++count;
+		}
+	}
+	const f3 mean = sum / (float)count;
+	float md = 1.0f;
+	for (int l = 0; l < nl; ++l) {
+		const int32_t f = first[(size_t)l * ninst + inst];
+		f3 c;
+		if (f >= 0 && nm_contrib(lights[l], SP, SN, visp[l][(uint32_t)f + within], &c))
+			md *= nm_mindot_factor(mean, c);
+	}
+	const f3 N = normalized(mean);
+	out[4 * (size_t)s + 0] = N.x;
+	out[4 * (size_t)s + 1] = N.y;
+	out[4 * (size_t)s + 2] = N.z;
+	out[4 * (size_t)s + 3] = md;
+}
+
+void DeviceScene::nm_clear()
+{
+	for (float*& p : nm_vis_)
+		dfree(p);
+	nm_vis_.clear();
+	nm_lights_.clear();
+	nm_first_.clear();
+	nm_light_batch_.clear();
+	nm_batch_items_.clear();
+	nm_ninst_ = 0;
+}
+
+void DeviceScene::nm_keep_batch(const std::vector<DevLight>& lights, const std::vector<int32_t>& first_item,
+                                size_t ninst)
+{
+	if (!d_vis_)
+		throw std::runtime_error("nm_keep_batch: the direct-light run kept no shadow factors");
+	if (first_item.size() != lights.size() * ninst || (nm_ninst_ && nm_ninst_ != ninst))
+		throw std::runtime_error("nm_keep_batch: item table does not match the lights and instances");
+	for (int32_t f : first_item)
+		if (f >= 0 && (size_t)f >= nitems_)
+			throw std::runtime_error("nm_keep_batch: item offset beyond the run's items");
+	nm_ninst_ = ninst;
+	float* v = dalloc<float>(nitems_);
+	hipStream_t s = (hipStream_t)stream_;
+	HIPCHECK(hipMemcpyAsync(v, d_vis_, 4 * std::max<size_t>(nitems_, 1), hipMemcpyDeviceToDevice, s));
+	HIPCHECK(hipStreamSynchronize(s));
+	nm_vis_.push_back(v);
+	nm_batch_items_.push_back(nitems_);
+	for (size_t l = 0; l < lights.size(); ++l) {
+		nm_lights_.push_back(lights[l]);
+		nm_light_batch_.push_back((uint32_t)(nm_vis_.size() - 1));
+	}
+	nm_first_.insert(nm_first_.end(), first_item.begin(), first_item.end());
+}
+
+void DeviceScene::nm_compute(const std::vector<uint32_t>& inst_off, float ambient_bright, float* out4)
+{
+	const size_t ni = inst_off.size() - 1;
+	if (inst_off.size() < 2 || ni != nm_ninst_ || inst_off.back() != nsamples_)
+		throw std::runtime_error("nm_compute: instance offsets do not match the kept batches or the samples");
+	// every kept item must lie inside its batch: first + instance size <= items
+	const int nl = (int)nm_lights_.size();
+	for (int l = 0; l < nl; ++l)
+		for (size_t i = 0; i < ni; ++i) {
+			const int32_t f = nm_first_[(size_t)l * ni + i];
+			if (f >= 0 && (size_t)f + (inst_off[i + 1] - inst_off[i]) > nm_batch_items_[nm_light_batch_[l]])
+				throw std::runtime_error("nm_compute: an instance's items run past its batch");
+		}
+	hipStream_t s = (hipStream_t)stream_;
+	std::vector<const float*> visp(nl);
+	for (int l = 0; l < nl; ++l)
+		visp[l] = nm_vis_[nm_light_batch_[l]];
+	DevArray<uint32_t> d_off(inst_off.size());
+	DevArray<DevLight> d_l(std::max(nl, 1));
+	DevArray<int32_t> d_first(std::max<size_t>(nm_first_.size(), 1));
+	DevArray<const float*> d_visp(std::max(nl, 1));
+	DevArray<float> d_out(4 * std::max<size_t>(nsamples_, 1));
+	HIPCHECK(hipMemcpyAsync(d_off.p, inst_off.data(), 4 * inst_off.size(), hipMemcpyHostToDevice, s));
+	if (nl) {
+		HIPCHECK(hipMemcpyAsync(d_l.p, nm_lights_.data(), sizeof(DevLight) * nl, hipMemcpyHostToDevice, s));
+		HIPCHECK(hipMemcpyAsync(d_first.p, nm_first_.data(), 4 * nm_first_.size(), hipMemcpyHostToDevice, s));
+		HIPCHECK(hipMemcpyAsync(d_visp.p, visp.data(), sizeof(float*) * nl, hipMemcpyHostToDevice, s));
+	}
+	if (nsamples_) {
+		k_nm_data<<<grid_for(nsamples_), kBlock, 0, s>>>(d_pos_, d_nrm_, (uint32_t)nsamples_, d_off.p, (int)ni, d_l.p,
+		                                                 nl, d_first.p, d_visp.p, ambient_bright, d_out.p);
+		HIPCHECK(hipGetLastError());
+		HIPCHECK(hipMemcpyAsync(out4, d_out.p, 16 * nsamples_, hipMemcpyDeviceToHost, s));
+	}
+	HIPCHECK(hipStreamSynchronize(s));
 }
 
 void DeviceScene::direct_grad(const double* lam3, double* dcolor3)

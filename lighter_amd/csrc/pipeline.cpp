@@ -7,6 +7,7 @@
 // finalisation (lighter.cpp:292-482, 849-1044) so that the samples the
 // kernels see and the images the caller gets back are the reference's.
 #include "pipeline.h"
+#include "normals.h"
 #include "raster_gpu.h"
 #include "texel.h"
 
@@ -560,49 +561,18 @@ DevLight to_dev_light(const Light& L)
 static float brightness(f3 c) { return (c.x + c.y + c.z) * (1.0f / 3.0f); }
 
 // Normal-map light contributions of one light (lighter.cpp:505-513,
-// 543-551, 590-598, 656-664), from the shadow factors the GPU computed.
+// 543-551, 590-598, 656-664), from the shadow factors the GPU computed:
+// the host restatement of the GPU pass (nm_contrib, normals.h).
 static void collect_contribs(ltr_Scene* S, const Light& L, const std::vector<int32_t>& ids, const float* vis)
 {
-	float out_rad = L.spot_out / 180.0f * kPiF, in_rad = L.spot_in / 180.0f * kPiF;
-	if (in_rad == out_rad)
-		in_rad -= kSmall;
-	float diff = in_rad - out_rad;
+	const DevLight D = to_dev_light(L);
 	size_t k = 0;
 	for (int32_t id : ids) {
 		Instance* mi = S->insts[id];
 		for (size_t i = 0; i < mi->lightmap.size(); ++i, ++k) {
-			if (mi->samplecont)
-				continue;
-			f3 SP = mi->s_pos[i], SN = mi->s_nrm[i];
-			float fv = vis[k];
-			if (L.type == LTR_LT_DIRECT) {
-				float fn = tmax(0.0f, dot(L.dir, SN));
-				float factor = fv * fn * brightness(L.color);
-				if (factor > 0)
-					mi->contrib.push_back(LightContrib{L.dir * factor, (int32_t)i});
-				continue;
-			}
-			f3 s2l = L.pos - SP;
-			float dist = length(s2l);
-			if (dist)
-				s2l = s2l / dist;
-			float fd = powf(1 - tmin(1.0f, dist / L.range), L.power);
-			float fn = tmax(0.0f, dot(s2l, SN));
-			float factor;
-			if (L.type == LTR_LT_SPOT) {
-				float ang = acosf(tmin(1.0f, dot(s2l, -L.dir)));
-				float fdir = tmax(0.0f, tmin(1.0f, (ang - out_rad) / diff));
-				fdir = powf(fdir, L.spot_curve);
-				if (fd * fn * fdir <= 0)
-					continue;
-				factor = fd * fdir * fv * fn * brightness(L.color);
-			} else {
-				if (fd * fn <= 0)
-					continue;
-				factor = fd * fv * fn * brightness(L.color);
-			}
-			if (factor > 0)
-				mi->contrib.push_back(LightContrib{s2l * factor, (int32_t)i});
+			f3 c;
+			if (!mi->samplecont && nm_contrib(D, mi->s_pos[i], mi->s_nrm[i], vis[k], &c))
+				mi->contrib.push_back(LightContrib{c, (int32_t)i});
 		}
 	}
 }
@@ -614,6 +584,8 @@ void stage_direct(ltr_Scene* S)
 {
 	ensure_device(S);
 	const bool want_vis = S->config.generate_normalmap_data != 0;
+	S->dev->nm_clear();
+	S->nm_data.clear();
 	S->times.direct_ms = S->times.direct_march_ms = 0;
 	for (size_t l0 = 0; l0 < S->lights.size(); l0 += 64) {
 		size_t l1 = std::min(S->lights.size(), l0 + 64);
@@ -641,11 +613,24 @@ void stage_direct(ltr_Scene* S)
 			throw AbortBake{};
 		S->times.direct_ms += S->dev->times.direct_ms;
 		S->times.direct_march_ms += S->dev->times.direct_march_ms;
-		if (want_vis) {
+		if (want_vis && S->host_finalize) {
 			std::vector<float> vis(items.size());
 			S->dev->get_vis(vis.data(), vis.size());
 			for (size_t l = l0; l < l1; ++l)
 				collect_contribs(S, S->lights[l], ids_of[l - l0], vis.data() + off[l - l0]);
+		} else if (want_vis) {
+			// the GPU export's normal-map pass reads these shadow factors:
+			// per light, each instance's first item (or -1)
+			const size_t ni = S->insts.size();
+			std::vector<int32_t> first((l1 - l0) * ni, -1);
+			for (size_t l = l0; l < l1; ++l) {
+				uint32_t at = off[l - l0];
+				for (int32_t id : ids_of[l - l0]) {
+					first[(l - l0) * ni + id] = (int32_t)at;
+					at += (uint32_t)(S->inst_off[id + 1] - S->inst_off[id]);
+				}
+			}
+			S->dev->nm_keep_batch(dl, first, ni);
 		}
 		S->completion.store((float)l1 / (float)S->lights.size());
 		check_abort(S);
@@ -834,7 +819,17 @@ static ltr_WorkOutput output_of(ltr_Scene* S, size_t mid, float* rgb)
 	Instance* mi = S->insts[mid];
 	const ltr_Config& cfg = S->config;
 	float* normals = nullptr;
-	if (cfg.generate_normalmap_data) {
+	if (cfg.generate_normalmap_data && !S->nm_data.empty()) {
+		// from the GPU pass: normal and min-dot term per sample
+		const size_t nn = (size_t)mi->out_w * mi->out_h, base = S->inst_off[mid];
+		normals = new float[nn * 4]();
+		for (size_t i = 0; i < mi->s_loc.size(); ++i) {
+			const size_t loc = (size_t)mi->s_loc[i];
+			if (loc >= nn)
+				continue; // after ds2x the full-resolution texel index is out of range
+			memcpy(normals + loc * 4, &S->nm_data[4 * (base + i)], 4 * sizeof(float));
+		}
+	} else if (cfg.generate_normalmap_data) {
 		struct Sum {
 			f3 normal;
 			float mindot;
@@ -852,10 +847,7 @@ static ltr_WorkOutput output_of(ltr_Scene* S, size_t mid, float* rgb)
 			c.normal = c.normal / (float)c.count;
 		for (const LightContrib& c : mi->contrib) {
 			Sum& t = cs[c.sid];
-			f3 cn = c.normal;
-			float d = tmax(dot(normalized(t.normal), normalized(cn)), 0.0f);
-			d = lerpf(1.0f, d, tmin(length(cn) / length(t.normal), 1.0f));
-			t.mindot *= d;
+			t.mindot *= nm_mindot_factor(t.normal, c.normal);
 		}
 		size_t nn = (size_t)mi->out_w * mi->out_h;
 		normals = new float[nn * 4]();
@@ -936,6 +928,14 @@ void finalize(ltr_Scene* S)
 		if (mi->samplecont)
 			for (size_t i = 0; i < mi->lightmap.size(); ++i)
 				st3(S->samples[i].out_color, mi->lightmap[i]);
+	}
+	// normal-map data of the GPU pass, when the direct stage kept its
+	// shadow factors on the device (else the host's contributions serve)
+	S->nm_data.clear();
+	if (S->config.generate_normalmap_data && !S->host_finalize && S->dev && S->dev->nm_lights() > 0) {
+		std::vector<uint32_t> off(S->inst_off.begin(), S->inst_off.end());
+		S->nm_data.resize(4 * S->total_samples);
+		S->dev->nm_compute(off, brightness(ld3(S->config.ambient_color)), S->nm_data.data());
 	}
 	std::vector<float*> rgb(n, nullptr);
 	if (S->host_finalize) {

@@ -104,6 +104,7 @@ struct ltr_Scene {
 	// the host restatement is selected
 	bool host_raster = false, host_place = false, host_finalize = false;
 	lgx::RasterTimes raster_times;
+	std::vector<float> nm_data; // normal-map data of the GPU pass: normal xyz + min-dot per sample
 	lgx::PlaceTimes place_times;
 	lgx::FinTimes fin_times;
 };

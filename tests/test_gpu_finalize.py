@@ -66,3 +66,24 @@ def test_gpu_export_equals_host_instances(lib, libc, name, make):
         gpu, t = bake_outputs(lib, libc, spec, 0, **cfg)
         assert_same(host, gpu)
         print(f"{name} {cfg}: {len(gpu)} images, {int(t[4])} texels, kernels {t[2]:.3f} ms")
+
+
+def test_gpu_normal_map_data_equals_host(lib, libc):
+    """Normal-map data (lighter.cpp:505-513, 959-1030) from the GPU pass
+    over the kept shadow factors against the host restatement's
+    contribution lists, bit for bit, with point, spot and directional
+    lights (every light's pow and acos evaluated alike on both sides), with
+    and without the downsample."""
+    spec = scenes.room_scene(lm=64, n_boxes=8, n_point=2, bounce_count=0)
+    spec.lights.append(scenes.spot_light((5, 5, 3.5), (0.1, 0.0, -1.0), (0.9, 0.8, 0.7), 9.0, 50.0, 30.0, 0.7,
+                                         radius=0.2))
+    for ds2x in (0, 1):
+        cfg = dict(generate_normalmap_data=1, ds2x=ds2x, blur_size=0.5)
+        host, _ = bake_outputs(lib, libc, spec, HOST_FINALIZE, **cfg)
+        gpu, _ = bake_outputs(lib, libc, spec, 0, **cfg)
+        assert_same(host, gpu)
+        nm = gpu[0][4]
+        assert nm is not None
+        md = nm.reshape(-1, 4)[:, 3]
+        assert (md < 1).mean() > 0.05  # the min-dot term does something
+

@@ -182,8 +182,16 @@ LTRAPI int lgx_ReadWeights(lgx_Device* dev, float* weights, float* vis, size_t n
 LTRAPI int lgx_AmbientOcclusion(lgx_Device* dev, const float* randoff, int num_samples, float distance,
                                 float multiplier, float falloff, float effect, const float* color3);
 /* Radiosity links among N patches (Int_RDGenLinks pair loop,
- * lighter.cpp:728-759); returns the link count through nlinks. */
+ * lighter.cpp:728-759); returns the link count through nlinks.  The pairs
+ * are found on a uniform grid of cells as wide as the longest distance a
+ * pair can link over (f >= 0.001 needs l^2 <= |nA| |nB| / (0.001 pi)), so
+ * any patch count whose links fit 2^32 CSR entries is taken; with
+ * non-finite positions or normals the search falls back to all pairs. */
 LTRAPI int lgx_RadLinks(lgx_Device* dev, const float* pos3, const float* nrm3, size_t n, size_t* nlinks);
+/* No counterpart in the reference (tests). all_pairs = 1: the next
+ * lgx_RadLinks enumerates all pairs, as the reference's loop does (at most
+ * 524,288 patches), instead of the grid; both give the same links. */
+LTRAPI int lgx_SetLinkSearch(lgx_Device* dev, int all_pairs);
 LTRAPI int lgx_RadReadLinks(lgx_Device* dev, uint32_t* rowmap, uint32_t* other, float* factor);
 /* bounce_count rounds of Int_RDBounce (lighter.cpp:767-797) from emitted
  * light E; total3 receives the accumulated light per patch. */

@@ -696,8 +696,8 @@ int lgx_AmbientOcclusion(lgx_Device* d, const float* randoff, int num_samples, f
 int lgx_RadLinks(lgx_Device* d, const float* pos, const float* nrm, size_t n, size_t* nlinks)
 {
 	return guarded([&] {
-		if (n > kMaxPatches)
-			throw std::runtime_error("lgx_RadLinks: too many patches");
+		if (n >= ((size_t)1 << 32))
+			throw std::runtime_error("lgx_RadLinks: patch indices are 32-bit");
 		d->rad.reset(); // its dense matrix belongs to the old links
 		d->dev.rad_set_patches(pos, nrm, n);
 		size_t nl = d->dev.rad_links();
@@ -794,6 +794,12 @@ int lgx_DirectLightGrad(lgx_Device* d, const double* lam3, double* dcolor3)
 			throw std::runtime_error("lgx_DirectLightGrad: null argument");
 		d->dev.direct_grad(lam3, dcolor3);
 	});
+}
+
+int lgx_SetLinkSearch(lgx_Device* d, int all_pairs)
+{
+	d->dev.links_all_pairs = all_pairs != 0;
+	return 1;
 }
 
 int lgx_SetCounting(lgx_Device* d, int on)

@@ -29,6 +29,15 @@ struct DevLight {
 constexpr size_t kNoAllocMaxItems = (size_t)1 << 22;
 // GPUs per process whose scratch address is kept.
 constexpr int kMaxDevices = 64;
+// Patches the all-pairs link search takes at most: its per-row chunk counts
+// and offsets take 12 bytes per 256 pairs (12 GB here).
+constexpr size_t kMaxAllPairsPatches = 524288;
+
+// Sorts the columns within every row of a CSR structure (links_sort.hip):
+// in -> out, nnz entries, rows given by rowptr (nrows + 1 entries, device),
+// column values below 2^bits.
+void sort_row_columns(const uint32_t* in, uint32_t* out, uint32_t nnz, uint32_t nrows, const uint32_t* rowptr,
+                      int bits, void* stream);
 
 // Per-stage timing in milliseconds measured with HIP events on the stream
 // the kernels run on.
@@ -141,9 +150,16 @@ public:
 
 	// --- radiosity (lighter.cpp:667-797) ---------------------------------
 	// Patches = samples.  rad_links builds the symmetric link structure of
-	// occluded pairs passing the form-factor thresholds.
+	// the pairs that pass the form-factor thresholds and whose segment hits
+	// a shadow caster (the reference's VisibilityTest returns the hit).
+	// Candidates come from a uniform grid of cells as wide as the longest
+	// distance a pair can link over, or, with links_all_pairs set (and for
+	// patches with non-finite positions or normals), from all pairs as the
+	// reference enumerates them (at most kMaxAllPairsPatches patches); both
+	// give the same links.
 	void rad_set_patches(const float* pos, const float* nrm, size_t n);
 	size_t rad_links();
+	bool links_all_pairs = false;
 	size_t rad_num_links() const { return nlinks_; }
 	// Reference link list: rowmap {offset,count} per patch, other, factor.
 	void rad_get_links(uint32_t* rowmap, uint32_t* other, float* factor) const;
@@ -175,8 +191,6 @@ public:
 	struct Rad {
 		float* pos = nullptr;
 		float* nrm = nullptr;
-		uint32_t* bits = nullptr; // N x words symmetric occlusion bitmask
-		size_t words = 0;
 		uint32_t* rowptr = nullptr;
 		uint32_t* col = nullptr;
 		float* val = nullptr;
@@ -207,8 +221,15 @@ private:
 	uint32_t* d_skey_ = nullptr; // 30-bit Morton code of each sample position (march ordering)
 	float* d_nrm_ = nullptr;
 	float* d_lm_ = nullptr;
-	void* d_cand_ = nullptr; // candidate pairs (uint2) of the last link search, kept in counting mode
+	void* d_cand_ = nullptr;    // candidate pairs (uint2) of the last link search, kept in counting mode
+	void* d_clinked_ = nullptr; // and whether each one linked (uint8)
 	uint64_t ncand_ = 0;
+	// link search: patch bounds, the largest squared normal length, all finite
+	float links_lo_[3] = {0, 0, 0}, links_hi_[3] = {0, 0, 0};
+	double links_nmax2_ = 0.0;
+	bool links_finite_ = true;
+	void* link_candidates_all(uint64_t& ncand);  // uint2 pairs (i < j), device memory
+	void* link_candidates_grid(uint64_t& ncand); // the same set from the grid
 	size_t nsamples_ = 0;
 	// direct light
 	float* d_w_ = nullptr;

@@ -1176,7 +1176,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_canon(CanonView C, const RayTri* 
 __global__ __launch_bounds__(kBlock) void k_link_canon(CanonView C, const RayTri* __restrict__ rtri,
                                                        const float* __restrict__ pos,
                                                        const uint2* __restrict__ cand, uint64_t ncand,
-                                                       const uint32_t* __restrict__ bits, uint64_t words,
+                                                       const uint8_t* __restrict__ linked_by_product,
                                                        unsigned long long* out)
 {
 	const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1185,7 +1185,7 @@ __global__ __launch_bounds__(kBlock) void k_link_canon(CanonView C, const RayTri
 	const uint2 c = cand[k];
 	unsigned long long nv = 0, nt = 0;
 	const bool occluded = canon_segment<true>(C, rtri, ld3(pos + 3 * c.x), ld3(pos + 3 * c.y), nv, nt) < 1.0f;
-	const bool linked = (bits[(uint64_t)c.x * words + (c.y >> 5)] >> (c.y & 31)) & 1u;
+	const bool linked = linked_by_product[k] != 0;
 	atomicAdd(out + 0, nv);
 	atomicAdd(out + 1, nt);
 	if (occluded != linked)
@@ -1231,7 +1231,8 @@ __global__ void k_q_occluded(View V, const float* a, const float* b, uint32_t n,
 // radiosity links (lighter.cpp:728-759)
 // ---------------------------------------------------------------------------
 // Form factor of the pair (i < j) exactly as the reference evaluates it, or
-// -1 when the pair fails the geometric thresholds.
+// -1 when the pair fails the geometric thresholds (NaN inputs fail neither
+// comparison, as in the reference, and give a NaN factor).
 __device__ __forceinline__ float pair_factor(f3 ap, f3 an, f3 bp, f3 bn)
 {
 	f3 d = bp - ap;
@@ -1268,7 +1269,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const float* __restrict__ pos,
 		f3 ap = ld3(pos + 3 * i), an = ld3(nrm + 3 * i);
 		for (uint32_t j = j0; j < j1; ++j) {
 			float f = pair_factor(ap, an, ld3(pos + 3 * j), ld3(nrm + 3 * j));
-			if (f >= 0.0f) {
+			if (!(f < 0.0f)) { // -1: failed; NaN passes, as in the reference's comparisons
 				if (WRITE)
 					cand[base + n] = make_uint2(i, j);
 				n++;
@@ -1279,58 +1280,147 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const float* __restrict__ pos,
 		counts[(uint64_t)i * nchunks + c] = n;
 }
 
-__global__ __launch_bounds__(kBlock) void k_link_vis(View V, const float* __restrict__ pos,
-                                                     const uint2* __restrict__ cand, uint64_t ncand,
-                                                     uint32_t* __restrict__ bits, uint64_t words, int depth)
+// ---------------------------------------------------------------------------
+// Link candidates over a uniform grid of the patches.  A pair passes the
+// factor thresholds only when f = dA dB / (pi l^4) >= kSmall, and dA, dB <=
+// |n| l, so l^2 <= |nA| |nB| / (pi kSmall): 17.8 m for unit normals.  With
+// cells at least that wide every pair that can pass lies in the same or a
+// neighbouring cell; those pairs are tested with the arithmetic of the
+// all-pairs search (pair_factor, the lower index as A), so both searches
+// find the same candidate set, in another order.
+// ---------------------------------------------------------------------------
+struct LinkGrid {
+	float lo[3];
+	float inv;     // 1 / cell width
+	uint32_t n[3]; // cells per axis (product below 2^24)
+};
+
+__device__ __forceinline__ uint32_t grid_cell(const LinkGrid& G, f3 p)
+{
+	const float q[3] = {(p.x - G.lo[0]) * G.inv, (p.y - G.lo[1]) * G.inv, (p.z - G.lo[2]) * G.inv};
+	uint32_t c[3];
+#pragma unroll
+	for (int a = 0; a < 3; ++a)
+		c[a] = min(G.n[a] - 1, (uint32_t)floorf(q[a])); // p >= lo: q >= 0
+	return (c[2] * G.n[1] + c[1]) * G.n[0] + c[0];
+}
+
+__global__ void k_grid_count(const float* __restrict__ pos, uint32_t N, LinkGrid G, uint32_t* __restrict__ cell,
+                             uint32_t* __restrict__ ccount)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= N)
+		return;
+	const uint32_t k = grid_cell(G, ld3(pos + 3 * i));
+	cell[i] = k;
+	atomicAdd(ccount + k, 1u);
+}
+
+__global__ void k_grid_fill(uint32_t N, const uint32_t* __restrict__ cell, const uint64_t* __restrict__ cstart,
+                            uint32_t* __restrict__ cfill, uint32_t* __restrict__ order)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= N)
+		return;
+	const uint32_t k = cell[i];
+	order[cstart[k] + atomicAdd(cfill + k, 1u)] = i;
+}
+
+// Pass 0 counts, pass 1 writes the candidates (i, j > i) of row i from the
+// 27 cells around i's cell.
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void k_pairs_grid(const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                       uint32_t N, LinkGrid G, const uint32_t* __restrict__ cell,
+                                                       const uint64_t* __restrict__ cstart,
+                                                       const uint32_t* __restrict__ order,
+                                                       uint32_t* __restrict__ counts,
+                                                       const uint64_t* __restrict__ offs, uint2* __restrict__ cand)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= N)
+		return;
+	const uint32_t k = cell[i];
+	const int cx = (int)(k % G.n[0]), cy = (int)((k / G.n[0]) % G.n[1]), cz = (int)(k / (G.n[0] * G.n[1]));
+	const f3 ap = ld3(pos + 3 * i), an = ld3(nrm + 3 * i);
+	uint32_t n = 0;
+	const uint64_t base = WRITE ? offs[i] : 0;
+	for (int z = max(cz - 1, 0); z <= min(cz + 1, (int)G.n[2] - 1); ++z)
+		for (int y = max(cy - 1, 0); y <= min(cy + 1, (int)G.n[1] - 1); ++y)
+			for (int x = max(cx - 1, 0); x <= min(cx + 1, (int)G.n[0] - 1); ++x) {
+				const uint32_t cc = ((uint32_t)z * G.n[1] + (uint32_t)y) * G.n[0] + (uint32_t)x;
+				const uint64_t e = cstart[cc + 1];
+				for (uint64_t q = cstart[cc]; q < e; ++q) {
+					const uint32_t j = order[q];
+					if (j <= i)
+						continue;
+					if (!(pair_factor(ap, an, ld3(pos + 3 * j), ld3(nrm + 3 * j)) < 0.0f)) {
+						if (WRITE)
+							cand[base + n] = make_uint2(i, j);
+						n++;
+					}
+				}
+			}
+	if (!WRITE)
+		counts[i] = n;
+}
+
+// The occlusion test of each candidate: the pair links when the segment
+// between the patches hits a shadow caster (VisibilityTest returns the
+// any-hit query's hit, lighter.cpp:137-147, and Int_RDGenLinks keeps the
+// pair when it is true, lighter.cpp:751-752); links per row, both ends.
+__global__ __launch_bounds__(kBlock) void k_link_test(View V, const float* __restrict__ pos,
+                                                      const uint2* __restrict__ cand, uint64_t ncand,
+                                                      uint8_t* __restrict__ linked, uint32_t* __restrict__ rcount,
+                                                      int depth)
 {
 	extern __shared__ __align__(16) unsigned char lds_stack[];
 	LdsStack st = lds_stack_of_lane(lds_stack, depth);
-	uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= ncand)
 		return;
-	uint2 c = cand[k];
-	if (q_segment<true, false>(V, ld3(pos + 3 * c.x), ld3(pos + 3 * c.y), nullptr, st) < 1.0f) {
-		atomicOr(bits + (uint64_t)c.x * words + (c.y >> 5), 1u << (c.y & 31));
-		atomicOr(bits + (uint64_t)c.y * words + (c.x >> 5), 1u << (c.x & 31));
+	const uint2 c = cand[k];
+	const bool l = q_segment<true, false>(V, ld3(pos + 3 * c.x), ld3(pos + 3 * c.y), nullptr, st) < 1.0f;
+	linked[k] = l ? 1 : 0;
+	if (l) {
+		atomicAdd(rcount + c.x, 1u);
+		atomicAdd(rcount + c.y, 1u);
 	}
 }
 
-__global__ void k_row_popc(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N, uint32_t* __restrict__ cnt)
+// both directions of every link into their rows (any order within a row)
+__global__ void k_link_scatter(const uint2* __restrict__ cand, const uint8_t* __restrict__ linked, uint64_t ncand,
+                               const uint32_t* __restrict__ rowptr, uint32_t* __restrict__ rfill,
+                               uint32_t* __restrict__ colx)
 {
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= N)
+	const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= ncand || !linked[k])
 		return;
-	uint32_t n = 0;
-	const uint32_t* row = bits + (uint64_t)i * words;
-	for (uint64_t w = 0; w < words; ++w)
-		n += __popc(row[w]);
-	cnt[i] = n;
+	const uint2 c = cand[k];
+	colx[rowptr[c.x] + atomicAdd(rfill + c.x, 1u)] = c.y;
+	colx[rowptr[c.y] + atomicAdd(rfill + c.y, 1u)] = c.x;
 }
 
-__global__ void k_row_fill(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N,
-                           const uint32_t* __restrict__ rowptr, const float* __restrict__ pos,
-                           const float* __restrict__ nrm, uint32_t* __restrict__ col, float* __restrict__ val)
+// factor of every link (columns ascending within a row); the reference
+// evaluates each pair with the lower index as A
+__global__ void k_link_factors(uint32_t N, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                               const float* __restrict__ pos, const float* __restrict__ nrm, float* __restrict__ val)
 {
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= N)
 		return;
-	const uint32_t* row = bits + (uint64_t)i * words;
-	uint32_t o = rowptr[i];
-	f3 ip = ld3(pos + 3 * i), in_ = ld3(nrm + 3 * i);
-	for (uint64_t w = 0; w < words; ++w) {
-		uint32_t m = row[w];
-		while (m) {
-			uint32_t b = __ffs(m) - 1;
-			m &= m - 1;
-			uint32_t j = (uint32_t)(w * 32 + b);
-			f3 jp = ld3(pos + 3 * j), jn = ld3(nrm + 3 * j);
-			// the reference evaluates each pair with the lower index as A
-			float f = (i < j) ? pair_factor(ip, in_, jp, jn) : pair_factor(jp, jn, ip, in_);
-			col[o] = j;
-			val[o] = f;
-			o++;
-		}
+	const f3 ip = ld3(pos + 3 * i), in_ = ld3(nrm + 3 * i);
+	for (uint32_t o = rowptr[i]; o < rowptr[i + 1]; ++o) {
+		const uint32_t j = col[o];
+		const f3 jp = ld3(pos + 3 * j), jn = ld3(nrm + 3 * j);
+		val[o] = (i < j) ? pair_factor(ip, in_, jp, jn) : pair_factor(jp, jn, ip, in_);
 	}
+}
+
+__global__ void k_u64_to_u32(const uint64_t* __restrict__ in, uint64_t n, uint32_t* __restrict__ out)
+{
+	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		out[i] = (uint32_t)in[i];
 }
 
 // One bounce of Int_RDBounce (lighter.cpp:767-797) as a gather over the
@@ -1618,6 +1708,7 @@ void DeviceScene::free_all()
 {
 	nm_clear();
 	dfree(d_cand_);
+	dfree(d_clinked_);
 	ncand_ = 0;
 	dfree(d_nodes_);
 	dfree(d_dtri_);
@@ -1634,7 +1725,6 @@ void DeviceScene::free_all()
 	dfree(d_mlist_);
 	dfree(rad.pos);
 	dfree(rad.nrm);
-	dfree(rad.bits);
 	dfree(rad.rowptr);
 	dfree(rad.col);
 	dfree(rad.val);
@@ -2186,8 +2276,8 @@ void DeviceScene::reference_counts_links(const float* tris, size_t ntri, uint64_
 	C.nodes = d_nodes;
 	C.dtri = nullptr;
 	canon_root_box(b, C.lo, C.hi);
-	k_link_canon<<<grid_for(ncand_), kBlock, 0, s>>>(C, d_rt, rad.pos, (const uint2*)d_cand_, ncand_, rad.bits,
-	                                                 rad.words, d_out);
+	k_link_canon<<<grid_for(ncand_), kBlock, 0, s>>>(C, d_rt, rad.pos, (const uint2*)d_cand_, ncand_,
+	                                                 (const uint8_t*)d_clinked_, d_out);
 	HIPCHECK(hipGetLastError());
 	unsigned long long h[3];
 	HIPCHECK(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -2553,77 +2643,161 @@ void DeviceScene::rad_set_patches(const float* pos, const float* nrm, size_t n)
 		HIPCHECK(hipMemcpy(rad.pos, pos, 12 * n, hipMemcpyHostToDevice));
 		HIPCHECK(hipMemcpy(rad.nrm, nrm, 12 * n, hipMemcpyHostToDevice));
 	}
+	// bounds and the longest normal, for the grid of the link search
+	links_lo_[0] = links_lo_[1] = links_lo_[2] = INFINITY;
+	links_hi_[0] = links_hi_[1] = links_hi_[2] = -INFINITY;
+	links_nmax2_ = 0.0;
+	links_finite_ = true;
+	for (size_t i = 0; i < n; ++i) {
+		double l2 = 0.0;
+		for (int a = 0; a < 3; ++a) {
+			const float p = pos[3 * i + a], q = nrm[3 * i + a];
+			if (!std::isfinite(p) || !std::isfinite(q))
+				links_finite_ = false;
+			links_lo_[a] = std::min(links_lo_[a], p);
+			links_hi_[a] = std::max(links_hi_[a], p);
+			l2 += (double)q * q;
+		}
+		links_nmax2_ = std::max(links_nmax2_, l2);
+	}
+}
+
+// All pairs (i, j > i) that pass the factor thresholds, in chunks of
+// kPairChunk columns per row (the reference's loop, lighter.cpp:728-749).
+void* DeviceScene::link_candidates_all(uint64_t& ncand)
+{
+	hipStream_t s = (hipStream_t)stream_;
+	const uint32_t N = (uint32_t)npatches;
+	if (npatches > kMaxAllPairsPatches)
+		throw std::runtime_error("radiosity: the all-pairs link search takes at most " +
+		                         std::to_string(kMaxAllPairsPatches) + " patches (" + std::to_string(npatches) +
+		                         (links_finite_ ? " given)" : " given; positions or normals are not finite)"));
+	const uint32_t nchunks = (N + kPairChunk - 1) / kPairChunk;
+	const uint64_t ncounts = (uint64_t)N * nchunks;
+	DevArray<uint32_t> counts(ncounts);
+	DevArray<uint64_t> offs(ncounts);
+	const dim3 g(grid_for(N), nchunks);
+	k_pairs<false><<<g, kBlock, 0, s>>>(rad.pos, rad.nrm, N, nchunks, counts.p, nullptr, nullptr);
+	HIPCHECK(hipGetLastError());
+	scan_u32_to_u64(counts.p, ncounts, offs.p, &ncand, s);
+	uint2* cand = dalloc<uint2>(ncand);
+	k_pairs<true><<<g, kBlock, 0, s>>>(rad.pos, rad.nrm, N, nchunks, nullptr, offs.p, cand);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize(s));
+	return cand;
+}
+
+// The same candidate set from the 27 grid cells around each patch.
+void* DeviceScene::link_candidates_grid(uint64_t& ncand)
+{
+	hipStream_t s = (hipStream_t)stream_;
+	const uint32_t N = (uint32_t)npatches;
+	ncand = 0;
+	// the longest linking distance (1 per mille margin over the float
+	// rounding of the factor), the cell width 1e-4 above it
+	const double r2 = links_nmax2_ / (M_PI * (double)kSmall) * 1.001;
+	if (!(r2 > 0.0))
+		return dalloc<uint2>(0); // zero normals: no pair passes
+	double h = std::sqrt(r2) * 1.0001;
+	LinkGrid G;
+	uint64_t cells = 0;
+	for (;;) {
+		cells = 1;
+		for (int a = 0; a < 3; ++a) {
+			G.lo[a] = links_lo_[a];
+			G.n[a] = (uint32_t)std::floor(((double)links_hi_[a] - links_lo_[a]) / h) + 1;
+			cells *= G.n[a];
+		}
+		if (cells <= (1u << 24))
+			break;
+		h *= 1.26; // wider cells keep every linking pair in neighbouring cells
+	}
+	G.inv = (float)(1.0 / h);
+	const uint32_t nc = (uint32_t)cells;
+	DevArray<uint32_t> cell(N), ccount(nc + 1), order(N), counts(N);
+	DevArray<uint64_t> cstart(nc + 1), offs(N);
+	HIPCHECK(hipMemsetAsync(ccount.p, 0, 4 * ((size_t)nc + 1), s));
+	k_grid_count<<<grid_for(N), kBlock, 0, s>>>(rad.pos, N, G, cell.p, ccount.p);
+	HIPCHECK(hipGetLastError());
+	uint64_t total = 0;
+	scan_u32_to_u64(ccount.p, (uint64_t)nc + 1, cstart.p, &total, s);
+	HIPCHECK(hipMemsetAsync(ccount.p, 0, 4 * ((size_t)nc + 1), s));
+	k_grid_fill<<<grid_for(N), kBlock, 0, s>>>(N, cell.p, cstart.p, ccount.p, order.p);
+	k_pairs_grid<false><<<grid_for(N), kBlock, 0, s>>>(rad.pos, rad.nrm, N, G, cell.p, cstart.p, order.p, counts.p,
+	                                                   nullptr, nullptr);
+	HIPCHECK(hipGetLastError());
+	scan_u32_to_u64(counts.p, N, offs.p, &ncand, s);
+	uint2* cand = dalloc<uint2>(ncand);
+	k_pairs_grid<true><<<grid_for(N), kBlock, 0, s>>>(rad.pos, rad.nrm, N, G, cell.p, cstart.p, order.p, nullptr,
+	                                                  offs.p, cand);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize(s));
+	return cand;
 }
 
 size_t DeviceScene::rad_links()
 {
 	hipStream_t s = (hipStream_t)stream_;
-	uint32_t N = (uint32_t)npatches;
+	const uint32_t N = (uint32_t)npatches;
 	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
-	dfree(rad.bits);
 	dfree(rad.rowptr);
 	dfree(rad.col);
 	dfree(rad.val);
-	rad.words = (N + 31) / 32;
-	rad.bits = dalloc<uint32_t>((size_t)N * rad.words);
-	HIPCHECK(hipMemsetAsync(rad.bits, 0, (size_t)N * rad.words * 4, s));
+	dfree(d_cand_);
+	dfree(d_clinked_);
+	ncand_ = 0;
+	rad.nnz = 0;
 	rad.rowptr = dalloc<uint32_t>(N + 1);
 	EventTimer tm(s);
-	if (N > 1) {
-		uint32_t nchunks = (N + kPairChunk - 1) / kPairChunk;
-		uint64_t ncounts = (uint64_t)N * nchunks;
-		uint32_t* counts = dalloc<uint32_t>(ncounts);
-		uint64_t* offs = dalloc<uint64_t>(ncounts);
-		dim3 g(grid_for(N), nchunks);
-		k_pairs<false><<<g, kBlock, 0, s>>>(rad.pos, rad.nrm, N, nchunks, counts, nullptr, nullptr);
+	uint64_t ncand = 0;
+	uint2* cand = nullptr;
+	if (N > 1)
+		cand = (uint2*)((links_all_pairs || !links_finite_) ? link_candidates_all(ncand) : link_candidates_grid(ncand));
+	// occlusion test of every candidate, links per row (both ends)
+	DevArray<uint32_t> cnt((uint64_t)N + 1);
+	HIPCHECK(hipMemsetAsync(cnt.p, 0, 4 * ((size_t)N + 1), s));
+	uint8_t* linked = dalloc<uint8_t>(ncand);
+	if (ncand) {
+		k_link_test<<<grid_for(ncand), kBlock, march_lds_bytes(), s>>>(V, rad.pos, cand, ncand, linked, cnt.p,
+		                                                               march_stack_depth());
 		HIPCHECK(hipGetLastError());
-		uint64_t ncand = 0;
-		scan_u32_to_u64(counts, ncounts, offs, &ncand, s);
-		uint2* cand = dalloc<uint2>(ncand);
-		k_pairs<true><<<g, kBlock, 0, s>>>(rad.pos, rad.nrm, N, nchunks, nullptr, offs, cand);
-		HIPCHECK(hipGetLastError());
-		dfree(counts);
-		dfree(offs);
-		if (ncand) {
-			k_link_vis<<<grid_for(ncand), kBlock, march_lds_bytes(), s>>>(V, rad.pos, cand, ncand, rad.bits,
-			                                                              rad.words, march_stack_depth());
-			HIPCHECK(hipGetLastError());
-		}
-		HIPCHECK(hipStreamSynchronize(s));
-		dfree(d_cand_);
-		ncand_ = 0;
-		if (count_queries) { // kept for reference_counts_links
-			d_cand_ = (void*)cand;
-			ncand_ = ncand;
-		} else {
-			dfree(cand);
-		}
 	}
-	// symmetric CSR
-	uint32_t* cnt = dalloc<uint32_t>(N + 1);
-	HIPCHECK(hipMemsetAsync(cnt, 0, 4 * (N + 1), s));
-	if (N)
-		k_row_popc<<<grid_for(N), kBlock, 0, s>>>(rad.bits, rad.words, N, cnt);
-	uint64_t* rp64 = dalloc<uint64_t>(N + 1);
+	// symmetric CSR, columns ascending within a row
 	uint64_t nnz = 0;
-	scan_u32_to_u64(cnt, (uint64_t)N + 1, rp64, &nnz, s);
-	std::vector<uint64_t> h64(N + 1);
-	HIPCHECK(hipMemcpy(h64.data(), rp64, 8 * (N + 1), hipMemcpyDeviceToHost));
-	if (nnz >= (1ull << 32))
-		throw std::runtime_error("radiosity: more than 2^32 link entries");
-	std::vector<uint32_t> h32(N + 1);
-	for (uint32_t i = 0; i <= N; ++i)
-		h32[i] = (uint32_t)h64[i];
-	HIPCHECK(hipMemcpy(rad.rowptr, h32.data(), 4 * (N + 1), hipMemcpyHostToDevice));
-	dfree(cnt);
-	dfree(rp64);
+	{
+		DevArray<uint64_t> rp64((uint64_t)N + 1);
+		scan_u32_to_u64(cnt.p, (uint64_t)N + 1, rp64.p, &nnz, s);
+		if (nnz >= (1ull << 32))
+			throw std::runtime_error("radiosity: more than 2^32 link entries");
+		k_u64_to_u32<<<grid_for((uint64_t)N + 1), kBlock, 0, s>>>(rp64.p, (uint64_t)N + 1, rad.rowptr);
+		HIPCHECK(hipGetLastError());
+		HIPCHECK(hipStreamSynchronize(s));
+	}
 	rad.nnz = nnz;
 	rad.col = dalloc<uint32_t>(nnz);
 	rad.val = dalloc<float>(nnz);
-	if (N)
-		k_row_fill<<<grid_for(N), kBlock, 0, s>>>(rad.bits, rad.words, N, rad.rowptr, rad.pos, rad.nrm, rad.col,
-		                                          rad.val);
-	HIPCHECK(hipGetLastError());
+	if (nnz) {
+		DevArray<uint32_t> colx(nnz);
+		HIPCHECK(hipMemsetAsync(cnt.p, 0, 4 * ((size_t)N + 1), s));
+		k_link_scatter<<<grid_for(ncand), kBlock, 0, s>>>(cand, linked, ncand, rad.rowptr, cnt.p, colx.p);
+		HIPCHECK(hipGetLastError());
+		int bits = 1;
+		while (bits < 32 && (1ull << bits) < N)
+			++bits;
+		sort_row_columns(colx.p, rad.col, (uint32_t)nnz, N, rad.rowptr, bits, s);
+		k_link_factors<<<grid_for(N), kBlock, 0, s>>>(N, rad.rowptr, rad.col, rad.pos, rad.nrm, rad.val);
+		HIPCHECK(hipGetLastError());
+		HIPCHECK(hipStreamSynchronize(s)); // before colx goes
+	}
+	HIPCHECK(hipStreamSynchronize(s));
+	if (count_queries) { // kept for reference_counts_links
+		d_cand_ = (void*)cand;
+		d_clinked_ = (void*)linked;
+		ncand_ = ncand;
+	} else {
+		dfree(cand);
+		dfree(linked);
+	}
 	times.links_ms = tm.stop();
 	nlinks_ = nnz / 2;
 	return nlinks_;

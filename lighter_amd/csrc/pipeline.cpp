@@ -644,9 +644,8 @@ void stage_radiosity(ltr_Scene* S)
 {
 	ensure_device(S);
 	size_t N = S->total_samples;
-	if (N > kMaxPatches)
-		throw std::runtime_error("radiosity: " + std::to_string(N) + " patches exceeds the supported " +
-		                         std::to_string(kMaxPatches));
+	if (N >= ((size_t)1 << 32))
+		throw std::runtime_error("radiosity: patch indices are 32-bit (" + std::to_string(N) + " patches)");
 	download_lightmaps(S);
 	std::vector<float> pos(3 * N), nrm(3 * N), diff(3 * N), emit(3 * N), area(N);
 	const ltr_Config& cfg = S->config;

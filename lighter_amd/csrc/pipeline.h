@@ -12,10 +12,6 @@ namespace lgx {
 // Thrown between stages when ltr_Abort was called.
 struct AbortBake {};
 
-// Largest radiosity patch count: the pair-visibility bitmask takes N^2/8
-// bytes of device memory (8.6 GB at this limit).
-constexpr size_t kMaxPatches = 262144;
-
 void bake(ltr_Scene* S);                  // the whole pipeline (ltr_Start's thread)
 void reset_bake(ltr_Scene* S);            // forget the last bake's samples, lightmaps and device state
 void prepare(ltr_Scene* S);               // host stages up to sample generation

@@ -121,17 +121,6 @@ template <> struct Prec<1> { // FP16
 	static __device__ __forceinline__ uint16_t cvt(float x) { return to_f16(x); }
 };
 
-// form factor of the pair, lower index as A (lighter.cpp:740-749)
-__device__ __forceinline__ float link_factor(const float* pos, const float* nrm, uint32_t i, uint32_t j)
-{
-	uint32_t a = i < j ? i : j, b = i < j ? j : i;
-	f3 d = ld3(pos + 3 * b) - ld3(pos + 3 * a);
-	float da = dot(ld3(nrm + 3 * a), d);
-	float db = dot(ld3(nrm + 3 * b), -d);
-	float l2 = lensq(d);
-	return da * db / (l2 * l2 * kPiF);
-}
-
 template <int P>
 __device__ __forceinline__ float from16(uint16_t v)
 {
@@ -144,29 +133,19 @@ __device__ __forceinline__ float from16(uint16_t v)
 
 // ---------------------------------------------------------------------------
 // dense FP32 matrix H[i][j] = f_ij (the symmetric link factors, zero off the
-// links), row-major Np x Np, from the occlusion bitmask; one thread per
-// 32-column word
+// links), row-major Np x Np, from the link CSR: one thread per row writes
+// its links into the zeroed matrix
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kB) void k_dense_build(const uint32_t* __restrict__ bits, uint64_t words, uint32_t N,
-                                                    uint32_t Np, const float* __restrict__ pos,
-                                                    const float* __restrict__ nrm, float* __restrict__ G)
+__global__ __launch_bounds__(kB) void k_dense_build(uint32_t N, uint32_t Np, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ col, const float* __restrict__ val,
+                                                    float* __restrict__ G)
 {
-	uint64_t t = (uint64_t)blockIdx.x * kB + threadIdx.x;
-	uint64_t wpr = Np / 32; // words per padded row
-	if (t >= (uint64_t)Np * wpr)
+	const uint32_t i = blockIdx.x * kB + threadIdx.x;
+	if (i >= N)
 		return;
-	uint32_t i = (uint32_t)(t / wpr), w = (uint32_t)(t % wpr);
-	uint32_t m = (i < N && w < words) ? bits[(uint64_t)i * words + w] : 0u;
-	float v[32];
-#pragma unroll
-	for (int b = 0; b < 32; ++b) {
-		uint32_t j = w * 32 + b;
-		v[b] = ((m >> b) & 1u) ? link_factor(pos, nrm, i, j) : 0.0f;
-	}
-	float4* o = reinterpret_cast<float4*>(G + (uint64_t)i * Np + (uint64_t)w * 32);
-#pragma unroll
-	for (int q = 0; q < 8; ++q)
-		o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+	float* row = G + (uint64_t)i * Np;
+	for (uint32_t o = rowptr[i]; o < rowptr[i + 1]; ++o)
+		row[col[o]] = val[o];
 }
 
 // ---------------------------------------------------------------------------
@@ -1435,13 +1414,13 @@ void RadSolver::build_dense(int form)
 {
 	hipStream_t s = (hipStream_t)s_;
 	const int st = storage_of(form);
-	uint64_t threads = (uint64_t)Np * (Np / 32);
 	Ev ev(s);
 	if (st == kDenseF32) {
 		if (!g32_)
 			g32_ = dalloc<float>((uint64_t)Np * Np);
-		k_dense_build<<<gridn(threads), kB, 0, s>>>(dev.rad.bits, dev.rad.words, N, Np, dev.rad.pos, dev.rad.nrm,
-		                                            g32_);
+		HIPCHECK(hipMemsetAsync(g32_, 0, 4 * (uint64_t)Np * Np, s));
+		if (N)
+			k_dense_build<<<gridn(N), kB, 0, s>>>(N, Np, dev.rad.rowptr, dev.rad.col, dev.rad.val, g32_);
 	} else if (st == kTileF32) {
 		build_tiles();
 		if (!g32t_)

@@ -36,7 +36,7 @@ constexpr size_t kMaxAllPairsPatches = 524288;
 // Sorts the columns within every row of a CSR structure (links_sort.hip):
 // in -> out, nnz entries, rows given by rowptr (nrows + 1 entries, device),
 // column values below 2^bits.
-void sort_row_columns(const uint32_t* in, uint32_t* out, uint32_t nnz, uint32_t nrows, const uint32_t* rowptr,
+void sort_row_columns(const uint32_t* in, uint32_t* out, uint64_t nnz, uint32_t nrows, const uint64_t* rowptr,
                       int bits, void* stream);
 
 // Per-stage timing in milliseconds measured with HIP events on the stream
@@ -191,7 +191,8 @@ public:
 	struct Rad {
 		float* pos = nullptr;
 		float* nrm = nullptr;
-		uint32_t* rowptr = nullptr;
+		uint64_t* rowptr64 = nullptr; // N + 1 row offsets of the symmetric CSR
+		uint32_t* rowptr = nullptr;   // the same in 32 bits, while nnz < 2^32 (the solver forms)
 		uint32_t* col = nullptr;
 		float* val = nullptr;
 		size_t nnz = 0;
@@ -200,6 +201,7 @@ public:
 		float* total = nullptr;
 		float* out = nullptr;
 		float* in = nullptr;
+		float* x = nullptr; // (out * diffuse) * area per patch, 4 floats (the bounce's gather operand)
 	};
 	Rad rad;
 	size_t npatches = 0;

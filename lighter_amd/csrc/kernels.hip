@@ -1389,7 +1389,7 @@ __global__ __launch_bounds__(kBlock) void k_link_test(View V, const float* __res
 
 // both directions of every link into their rows (any order within a row)
 __global__ void k_link_scatter(const uint2* __restrict__ cand, const uint8_t* __restrict__ linked, uint64_t ncand,
-                               const uint32_t* __restrict__ rowptr, uint32_t* __restrict__ rfill,
+                               const uint64_t* __restrict__ rowptr, uint32_t* __restrict__ rfill,
                                uint32_t* __restrict__ colx)
 {
 	const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1402,14 +1402,14 @@ __global__ void k_link_scatter(const uint2* __restrict__ cand, const uint8_t* __
 
 // factor of every link (columns ascending within a row); the reference
 // evaluates each pair with the lower index as A
-__global__ void k_link_factors(uint32_t N, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+__global__ void k_link_factors(uint32_t N, const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
                                const float* __restrict__ pos, const float* __restrict__ nrm, float* __restrict__ val)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= N)
 		return;
 	const f3 ip = ld3(pos + 3 * i), in_ = ld3(nrm + 3 * i);
-	for (uint32_t o = rowptr[i]; o < rowptr[i + 1]; ++o) {
+	for (uint64_t o = rowptr[i]; o < rowptr[i + 1]; ++o) {
 		const uint32_t j = col[o];
 		const f3 jp = ld3(pos + 3 * j), jn = ld3(nrm + 3 * j);
 		val[o] = (i < j) ? pair_factor(ip, in_, jp, jn) : pair_factor(jp, jn, ip, in_);
@@ -1424,23 +1424,52 @@ __global__ void k_u64_to_u32(const uint64_t* __restrict__ in, uint64_t n, uint32
 }
 
 // One bounce of Int_RDBounce (lighter.cpp:767-797) as a gather over the
-// symmetric links: in_i = sum_j ((out_j * diffuse_j) * area_j) * f_ij.
-__global__ __launch_bounds__(kBlock) void k_bounce3(uint32_t N, const uint32_t* __restrict__ rowptr,
-                                                    const uint32_t* __restrict__ col, const float* __restrict__ val,
-                                                    const float* __restrict__ diffuse, const float* __restrict__ area,
-                                                    const float* __restrict__ out, float* __restrict__ in,
-                                                    float* __restrict__ total)
+// symmetric links: in_i = sum_j ((out_j * diffuse_j) * area_j) * f_ij.  The
+// factor-free part x_j = (out_j * diffuse_j) * area_j is formed once per
+// patch (the same products in the same order as the reference's per-link
+// expression); one wave per row then sums x_j * f_ij over the row's links,
+// lane l taking links l, l + 64, .., and the lanes' partial sums meet in a
+// fixed butterfly (deterministic, and rows of thousands of links read
+// their columns and factors in coalesced runs).
+__global__ void k_bounce_prep(uint32_t N, const float* __restrict__ out, const float* __restrict__ diffuse,
+                              const float* __restrict__ area, float4* __restrict__ x)
 {
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= N)
 		return;
-	f3 acc = mk3(0, 0, 0);
-	for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-		uint32_t j = col[k];
-		acc += ((ld3(out + 3 * j) * ld3(diffuse + 3 * j)) * area[j]) * val[k];
+	const f3 v = (ld3(out + 3 * i) * ld3(diffuse + 3 * i)) * area[i];
+	x[i] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounce_rows(uint32_t N, const uint64_t* __restrict__ rowptr,
+                                                        const uint32_t* __restrict__ col,
+                                                        const float* __restrict__ val, const float4* __restrict__ x,
+                                                        float* __restrict__ in, float* __restrict__ total)
+{
+	const uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (i >= N)
+		return; // the whole wave
+	float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+	const uint64_t e = rowptr[i + 1];
+	for (uint64_t k = rowptr[i] + lane; k < e; k += 64) {
+		const float4 v = x[col[k]];
+		const float f = val[k];
+		a0 += v.x * f;
+		a1 += v.y * f;
+		a2 += v.z * f;
 	}
-	st3(in + 3 * i, acc);
-	st3(total + 3 * i, ld3(total + 3 * i) + acc);
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) {
+		a0 += __shfl_xor(a0, o);
+		a1 += __shfl_xor(a1, o);
+		a2 += __shfl_xor(a2, o);
+	}
+	if (lane == 0) {
+		const f3 acc = mk3(a0, a1, a2);
+		st3(in + 3 * i, acc);
+		st3(total + 3 * i, ld3(total + 3 * i) + acc);
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -1504,6 +1533,7 @@ __global__ void k_scan_add(uint64_t* out, uint64_t n, const uint64_t* sums)
 // host side
 // ===========================================================================
 static inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+static inline unsigned gridn_waves(uint64_t n) { return (unsigned)((n + kBlock / 64 - 1) / (kBlock / 64)); }
 
 // The HIP runtime draws from (or reseeds) the C library's random generator
 // while it initialises (the stream, memory and code-object set-up behind the
@@ -1726,6 +1756,7 @@ void DeviceScene::free_all()
 	dfree(rad.pos);
 	dfree(rad.nrm);
 	dfree(rad.rowptr);
+	dfree(rad.rowptr64);
 	dfree(rad.col);
 	dfree(rad.val);
 	dfree(rad.diffuse);
@@ -1733,6 +1764,7 @@ void DeviceScene::free_all()
 	dfree(rad.total);
 	dfree(rad.out);
 	dfree(rad.in);
+	dfree(rad.x);
 }
 
 static View make_view(void* nodes, void* dtri, void* rtri, size_t ntri, const float* lo, const float* hi)
@@ -2741,13 +2773,14 @@ size_t DeviceScene::rad_links()
 	const uint32_t N = (uint32_t)npatches;
 	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
 	dfree(rad.rowptr);
+	dfree(rad.rowptr64);
 	dfree(rad.col);
 	dfree(rad.val);
 	dfree(d_cand_);
 	dfree(d_clinked_);
 	ncand_ = 0;
 	rad.nnz = 0;
-	rad.rowptr = dalloc<uint32_t>(N + 1);
+	rad.rowptr64 = dalloc<uint64_t>((uint64_t)N + 1);
 	EventTimer tm(s);
 	uint64_t ncand = 0;
 	uint2* cand = nullptr;
@@ -2762,30 +2795,29 @@ size_t DeviceScene::rad_links()
 		                                                               march_stack_depth());
 		HIPCHECK(hipGetLastError());
 	}
-	// symmetric CSR, columns ascending within a row
+	// symmetric CSR, columns ascending within a row; 32-bit row offsets too
+	// while they fit (the solver forms take those)
 	uint64_t nnz = 0;
-	{
-		DevArray<uint64_t> rp64((uint64_t)N + 1);
-		scan_u32_to_u64(cnt.p, (uint64_t)N + 1, rp64.p, &nnz, s);
-		if (nnz >= (1ull << 32))
-			throw std::runtime_error("radiosity: more than 2^32 link entries");
-		k_u64_to_u32<<<grid_for((uint64_t)N + 1), kBlock, 0, s>>>(rp64.p, (uint64_t)N + 1, rad.rowptr);
+	scan_u32_to_u64(cnt.p, (uint64_t)N + 1, rad.rowptr64, &nnz, s);
+	if (nnz < (1ull << 32)) {
+		rad.rowptr = dalloc<uint32_t>((uint64_t)N + 1);
+		k_u64_to_u32<<<grid_for((uint64_t)N + 1), kBlock, 0, s>>>(rad.rowptr64, (uint64_t)N + 1, rad.rowptr);
 		HIPCHECK(hipGetLastError());
-		HIPCHECK(hipStreamSynchronize(s));
 	}
+	HIPCHECK(hipStreamSynchronize(s));
 	rad.nnz = nnz;
 	rad.col = dalloc<uint32_t>(nnz);
 	rad.val = dalloc<float>(nnz);
 	if (nnz) {
 		DevArray<uint32_t> colx(nnz);
 		HIPCHECK(hipMemsetAsync(cnt.p, 0, 4 * ((size_t)N + 1), s));
-		k_link_scatter<<<grid_for(ncand), kBlock, 0, s>>>(cand, linked, ncand, rad.rowptr, cnt.p, colx.p);
+		k_link_scatter<<<grid_for(ncand), kBlock, 0, s>>>(cand, linked, ncand, rad.rowptr64, cnt.p, colx.p);
 		HIPCHECK(hipGetLastError());
 		int bits = 1;
 		while (bits < 32 && (1ull << bits) < N)
 			++bits;
-		sort_row_columns(colx.p, rad.col, (uint32_t)nnz, N, rad.rowptr, bits, s);
-		k_link_factors<<<grid_for(N), kBlock, 0, s>>>(N, rad.rowptr, rad.col, rad.pos, rad.nrm, rad.val);
+		sort_row_columns(colx.p, rad.col, nnz, N, rad.rowptr64, bits, s);
+		k_link_factors<<<grid_for(N), kBlock, 0, s>>>(N, rad.rowptr64, rad.col, rad.pos, rad.nrm, rad.val);
 		HIPCHECK(hipGetLastError());
 		HIPCHECK(hipStreamSynchronize(s)); // before colx goes
 	}
@@ -2806,6 +2838,9 @@ size_t DeviceScene::rad_links()
 void DeviceScene::rad_get_csr(std::vector<uint32_t>& rowptr, std::vector<uint32_t>& col,
                               std::vector<float>& val) const
 {
+	if (!rad.rowptr)
+		throw std::runtime_error("radiosity: the link CSR holds " + std::to_string(rad.nnz) +
+		                         " entries; reading it takes fewer than 2^32");
 	rowptr.resize(npatches + 1);
 	col.resize(rad.nnz);
 	val.resize(rad.nnz);
@@ -2845,11 +2880,13 @@ void DeviceScene::rad_set_colors(const float* diffuse3, const float* emit3, cons
 	dfree(rad.total);
 	dfree(rad.out);
 	dfree(rad.in);
+	dfree(rad.x);
 	rad.diffuse = dalloc<float>(3 * n);
 	rad.area = dalloc<float>(n);
 	rad.total = dalloc<float>(3 * n);
 	rad.out = dalloc<float>(3 * n);
 	rad.in = dalloc<float>(3 * n);
+	rad.x = dalloc<float>(4 * n);
 	if (n) {
 		HIPCHECK(hipMemcpy(rad.diffuse, diffuse3, 12 * n, hipMemcpyHostToDevice));
 		HIPCHECK(hipMemcpy(rad.area, area, 4 * n, hipMemcpyHostToDevice));
@@ -2864,8 +2901,9 @@ void DeviceScene::rad_bounce(int count)
 	uint32_t N = (uint32_t)npatches;
 	EventTimer tm(s);
 	for (int b = 0; b < count && N; ++b) {
-		k_bounce3<<<grid_for(N), kBlock, 0, s>>>(N, rad.rowptr, rad.col, rad.val, rad.diffuse, rad.area, rad.out,
-		                                         rad.in, rad.total);
+		k_bounce_prep<<<grid_for(N), kBlock, 0, s>>>(N, rad.out, rad.diffuse, rad.area, (float4*)rad.x);
+		k_bounce_rows<<<gridn_waves(N), kBlock, 0, s>>>(N, rad.rowptr64, rad.col, rad.val, (const float4*)rad.x,
+		                                                 rad.in, rad.total);
 		HIPCHECK(hipGetLastError());
 		std::swap(rad.out, rad.in);
 	}

@@ -1145,8 +1145,11 @@ struct Ev {
 
 RadSolver::RadSolver(DeviceScene& d) : dev(d)
 {
-	if (!d.rad.rowptr)
+	if (!d.rad.rowptr64)
 		throw std::runtime_error("RadSolver: build the links first");
+	if (!d.rad.rowptr)
+		throw std::runtime_error("RadSolver: the solver forms take fewer than 2^32 link entries (" +
+		                         std::to_string(d.rad.nnz) + " given)");
 	N = (uint32_t)d.npatches;
 	Np = ((N + kPadM - 1) / kPadM) * kPadM;
 	if (Np < 512)

@@ -917,6 +917,7 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
     out["radiosity"] = rad
     out["adjoint"] = c4_adjoint(lib, leg, rad_dev, s0, N, Wp, diff, area, bounces, rng)
     lib.lgx_DeviceDestroy(rad_dev)
+    out["radiosity_whole_scene"] = whole_scene_radiosity(lib, leg, tris, rng)
     if with_cpu:
         # the reference's AO ray loop on a sample of texels, and the device's
         # AO over a lightmap of ones (multiplier 1, no falloff, effect 0,
@@ -948,6 +949,38 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
     out["lightmap_export"] = export_vs_host(lib, leg.built)
     out["sample_placement"] = placement_vs_host(lib, leg.spec, nth, leg.built, leg.pos)
     return out
+
+
+def whole_scene_radiosity(lib, leg, tris, rng):
+    """The bake's radiosity stage at C4's full size: links among all
+    11,917,038 texels (Int_RDGenLinks' pairs, found on the uniform grid, the
+    CSR with 64-bit row offsets) and one bounce over them for one RGB setup
+    (diffuse 0.6, world area per texel, seeded emission)."""
+    S = len(leg.pos)
+    d = lib.lgx_DeviceCreate(P(tris), len(tris))
+    try:
+        pn = scenes.normalize_rows_f32(leg.nrm)
+        nl = C.c_size_t(0)
+        _lib.check(lib.lgx_RadLinks(d, P(leg.pos), P(pn), S, C.byref(nl)), lib)
+        tm = (C.c_double * 6)()
+        lib.lgx_DeviceTimes(d, tm)
+        link_ms = tm[3]
+        diff = np.full((S, 3), 0.6, np.float32)
+        emit = np.ascontiguousarray(rng.uniform(0, 1, (S, 3)).astype(np.float32))
+        area = np.ascontiguousarray(leg.radinfo[:, 3])
+        total = np.zeros((S, 3), np.float32)
+        _lib.check(lib.lgx_RadBounce(d, P(diff), P(emit), P(area), 1, P(total)), lib)
+        lib.lgx_DeviceTimes(d, tm)
+        entries = 2 * int(nl.value)
+        res = dict(patches=int(S), links=int(nl.value), csr_entries=entries, link_search_ms=round(link_ms, 1),
+                   bounce_ms=round(tm[4], 2), csr_gb=round(entries * 8 / 1e9, 1),
+                   bounce_csr_tb_per_s=round(entries * 8 / (tm[4] / 1e3) / 1e12, 2),
+                   total_finite=bool(np.isfinite(total).all()))
+        log(f"[bench] C4 whole-scene radiosity: {nl.value} links among {S} texels in {link_ms:.0f} ms, "
+            f"one bounce {tm[4]:.1f} ms ({res['bounce_csr_tb_per_s']} TB/s of CSR)")
+        return res
+    finally:
+        lib.lgx_DeviceDestroy(d)
 
 
 def c4_adjoint(lib, leg, rad_dev, s0, N, Wp, diff, area, bounces, rng):

@@ -59,6 +59,16 @@ LTRAPI int lgx_RasterTimes(ltr_Scene* scene, double out[3]);
  * out[1] upload ms, out[2] kernel ms (HIP events), out[3] download ms,
  * out[4] samples placed (zeros after a host placement). */
 LTRAPI int lgx_PlacementTimes(ltr_Scene* scene, double out[5]);
+/* No counterpart in the reference. Wall ms of the last preparation's steps:
+ * out[0] transforms, out[1] collision set-up (each instance's split tree
+ * and triangle tree), out[2] the instance tree, out[3] sample generation
+ * (rasterisation and placement). */
+LTRAPI int lgx_PrepareTimes(ltr_Scene* scene, double out[4]);
+/* No counterpart in the reference (tests). The split tree (BSP) of n
+ * triangles (9 floats each; lighter_math.cpp:428-488) built with the pruned
+ * farthest-pair search and with the reference's full pair scan: out[0] 1
+ * when the two trees are identical, out[1] and out[2] their build ms. */
+LTRAPI int lgx_SplitTreeCompare(const float* tris9, size_t n, double out[3]);
 /* No counterpart in the reference. Last GPU export: out[0] pack ms, out[1]
  * upload ms, out[2] kernels ms (HIP events), out[3] download and unpack ms,
  * out[4] atlas texels (zeros after a host export). */
@@ -70,8 +80,8 @@ LTRAPI int lgx_StageRadiosity(ltr_Scene* scene);
 /* "rendering ambient occlusion" (lighter.cpp:1126-1136). */
 LTRAPI int lgx_StageAO(ltr_Scene* scene);
 /* "exporting lightmaps" (lighter.cpp:1138-1140, 849-1044): fills the work
- * outputs; the images (extrapolation, blur, downsample) on the GPU unless
- * lgx_SetHostStages selected the host, the normal-map data on the host.
+ * outputs; the images (extrapolation, blur, downsample) and the normal-map
+ * data on the GPU unless lgx_SetHostStages selected the host.
  * Each export replaces the output list; the images of an earlier export
  * stay allocated until ltr_DestroyScene. */
 LTRAPI int lgx_StageFinalize(ltr_Scene* scene);
@@ -188,10 +198,12 @@ LTRAPI int lgx_AmbientOcclusion(lgx_Device* dev, const float* randoff, int num_s
  * any patch count whose links fit 2^32 CSR entries is taken; with
  * non-finite positions or normals the search falls back to all pairs. */
 LTRAPI int lgx_RadLinks(lgx_Device* dev, const float* pos3, const float* nrm3, size_t n, size_t* nlinks);
-/* No counterpart in the reference (tests). all_pairs = 1: the next
- * lgx_RadLinks enumerates all pairs, as the reference's loop does (at most
- * 524,288 patches), instead of the grid; both give the same links. */
-LTRAPI int lgx_SetLinkSearch(lgx_Device* dev, int all_pairs);
+/* No counterpart in the reference (tests). How the next lgx_RadLinks
+ * finds its candidate pairs: mode 0 (default) the grid, unless it would
+ * test more than N^2 / 4 pairs (a scene within one cell width), then all
+ * pairs; 1 all pairs, as the reference's loop does (at most 524,288
+ * patches); 2 the grid.  Every mode gives the same links. */
+LTRAPI int lgx_SetLinkSearch(lgx_Device* dev, int mode);
 LTRAPI int lgx_RadReadLinks(lgx_Device* dev, uint32_t* rowmap, uint32_t* other, float* factor);
 /* bounce_count rounds of Int_RDBounce (lighter.cpp:767-797) from emitted
  * light E; total3 receives the accumulated light per patch. */

@@ -421,6 +421,25 @@ int lgx_PlacementTimes(ltr_Scene* S, double out[5])
 	return 1;
 }
 
+int lgx_SplitTreeCompare(const float* tris9, size_t n, double out[3])
+{
+	return guarded([&] {
+		std::vector<Tri> t(n);
+		for (size_t i = 0; i < n; ++i)
+			t[i] = Tri{mk3(tris9[9 * i], tris9[9 * i + 1], tris9[9 * i + 2]),
+			           mk3(tris9[9 * i + 3], tris9[9 * i + 4], tris9[9 * i + 5]),
+			           mk3(tris9[9 * i + 6], tris9[9 * i + 7], tris9[9 * i + 8])};
+		out[0] = split_tree_compare(t.data(), n, &out[1], &out[2]) ? 1.0 : 0.0;
+	});
+}
+
+int lgx_PrepareTimes(ltr_Scene* S, double out[4])
+{
+	for (int i = 0; i < 4; ++i)
+		out[i] = S->prep_ms[i];
+	return 1;
+}
+
 int lgx_InstanceCount(ltr_Scene* S) { return (int)S->insts.size(); }
 
 int lgx_InstanceInfo(ltr_Scene* S, int inst, int32_t out[4])
@@ -796,10 +815,13 @@ int lgx_DirectLightGrad(lgx_Device* d, const double* lam3, double* dcolor3)
 	});
 }
 
-int lgx_SetLinkSearch(lgx_Device* d, int all_pairs)
+int lgx_SetLinkSearch(lgx_Device* d, int mode)
 {
-	d->dev.links_all_pairs = all_pairs != 0;
-	return 1;
+	return guarded([&] {
+		if (mode < 0 || mode > 2)
+			throw std::runtime_error("lgx_SetLinkSearch: mode 0 (automatic), 1 (all pairs) or 2 (grid)");
+		d->dev.links_mode = mode;
+	});
 }
 
 int lgx_SetCounting(lgx_Device* d, int on)

@@ -153,13 +153,14 @@ public:
 	// the pairs that pass the form-factor thresholds and whose segment hits
 	// a shadow caster (the reference's VisibilityTest returns the hit).
 	// Candidates come from a uniform grid of cells as wide as the longest
-	// distance a pair can link over, or, with links_all_pairs set (and for
-	// patches with non-finite positions or normals), from all pairs as the
-	// reference enumerates them (at most kMaxAllPairsPatches patches); both
-	// give the same links.
+	// distance a pair can link over, or from all pairs as the reference
+	// enumerates them (at most kMaxAllPairsPatches patches); both give the
+	// same links.  links_mode 0: the grid, unless it would test more than
+	// N^2 / 4 pairs (then all pairs); 1: all pairs; 2: the grid.  Patches
+	// with non-finite positions or normals always take all pairs.
 	void rad_set_patches(const float* pos, const float* nrm, size_t n);
 	size_t rad_links();
-	bool links_all_pairs = false;
+	int links_mode = 0;
 	size_t rad_num_links() const { return nlinks_; }
 	// Reference link list: rowmap {offset,count} per patch, other, factor.
 	void rad_get_links(uint32_t* rowmap, uint32_t* other, float* factor) const;
@@ -230,8 +231,10 @@ private:
 	float links_lo_[3] = {0, 0, 0}, links_hi_[3] = {0, 0, 0};
 	double links_nmax2_ = 0.0;
 	bool links_finite_ = true;
-	void* link_candidates_all(uint64_t& ncand);  // uint2 pairs (i < j), device memory
-	void* link_candidates_grid(uint64_t& ncand); // the same set from the grid
+	// candidate pairs (uint2, i < j, device memory), grouped by row i;
+	// coff (N + 1, device) receives the first candidate of each row
+	void* link_candidates_all(uint64_t& ncand, uint64_t* coff);
+	void* link_candidates_grid(uint64_t& ncand, uint64_t* coff); // the same set from the grid
 	size_t nsamples_ = 0;
 	// direct light
 	float* d_w_ = nullptr;

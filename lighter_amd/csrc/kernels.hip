@@ -1327,7 +1327,7 @@ __global__ void k_grid_fill(uint32_t N, const uint32_t* __restrict__ cell, const
 }
 
 // Pass 0 counts, pass 1 writes the candidates (i, j > i) of row i from the
-// 27 cells around i's cell.
+// 27 cells around i's cell; thread t takes the t-th patch in cell order.
 template <bool WRITE>
 __global__ __launch_bounds__(kBlock) void k_pairs_grid(const float* __restrict__ pos, const float* __restrict__ nrm,
                                                        uint32_t N, LinkGrid G, const uint32_t* __restrict__ cell,
@@ -1336,9 +1336,10 @@ __global__ __launch_bounds__(kBlock) void k_pairs_grid(const float* __restrict__
                                                        uint32_t* __restrict__ counts,
                                                        const uint64_t* __restrict__ offs, uint2* __restrict__ cand)
 {
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= N)
+	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= N)
 		return;
+	const uint32_t i = order[t]; // neighbouring lanes share cells: they walk the same lists
 	const uint32_t k = cell[i];
 	const int cx = (int)(k % G.n[0]), cy = (int)((k / G.n[0]) % G.n[1]), cz = (int)(k / (G.n[0] * G.n[1]));
 	const f3 ap = ld3(pos + 3 * i), an = ld3(nrm + 3 * i);
@@ -1367,10 +1368,12 @@ __global__ __launch_bounds__(kBlock) void k_pairs_grid(const float* __restrict__
 // The occlusion test of each candidate: the pair links when the segment
 // between the patches hits a shadow caster (VisibilityTest returns the
 // any-hit query's hit, lighter.cpp:137-147, and Int_RDGenLinks keeps the
-// pair when it is true, lighter.cpp:751-752); links per row, both ends.
+// pair when it is true, lighter.cpp:751-752).  Each link (i, j > i) also
+// counts toward row j's links below the diagonal; the candidates of one row
+// are contiguous, so the lanes of a wave add to different rows.
 __global__ __launch_bounds__(kBlock) void k_link_test(View V, const float* __restrict__ pos,
                                                       const uint2* __restrict__ cand, uint64_t ncand,
-                                                      uint8_t* __restrict__ linked, uint32_t* __restrict__ rcount,
+                                                      uint8_t* __restrict__ linked, uint32_t* __restrict__ lower,
                                                       int depth)
 {
 	extern __shared__ __align__(16) unsigned char lds_stack[];
@@ -1381,39 +1384,87 @@ __global__ __launch_bounds__(kBlock) void k_link_test(View V, const float* __res
 	const uint2 c = cand[k];
 	const bool l = q_segment<true, false>(V, ld3(pos + 3 * c.x), ld3(pos + 3 * c.y), nullptr, st) < 1.0f;
 	linked[k] = l ? 1 : 0;
-	if (l) {
-		atomicAdd(rcount + c.x, 1u);
-		atomicAdd(rcount + c.y, 1u);
-	}
+	if (l)
+		atomicAdd(lower + c.y, 1u);
 }
 
-// both directions of every link into their rows (any order within a row)
-__global__ void k_link_scatter(const uint2* __restrict__ cand, const uint8_t* __restrict__ linked, uint64_t ncand,
-                               const uint64_t* __restrict__ rowptr, uint32_t* __restrict__ rfill,
-                               uint32_t* __restrict__ colx)
+// Links of row i above the diagonal: its own linked candidates, one wave
+// per row.  With colx null, their count goes to upper[i]; otherwise they
+// are written after the row's lower part, in candidate order.
+__global__ __launch_bounds__(kBlock) void k_link_upper(uint32_t N, const uint64_t* __restrict__ coff,
+                                                       const uint2* __restrict__ cand,
+                                                       const uint8_t* __restrict__ linked,
+                                                       uint32_t* __restrict__ upper,
+                                                       const uint64_t* __restrict__ rowptr,
+                                                       const uint32_t* __restrict__ lower, uint32_t* __restrict__ colx)
+{
+	const uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (i >= N)
+		return; // the whole wave
+	const uint64_t e = coff[i + 1];
+	uint64_t at = colx ? rowptr[i] + lower[i] : 0;
+	for (uint64_t k0 = coff[i]; k0 < e; k0 += 64) {
+		const uint64_t k = k0 + lane;
+		const bool l = k < e && linked[k];
+		const uint64_t m = __ballot(l);
+		if (colx && l)
+			colx[at + __popcll(m & ((1ull << lane) - 1ull))] = cand[k].y;
+		at += __popcll(m);
+	}
+	if (!colx && lane == 0)
+		upper[i] = (uint32_t)at;
+}
+
+// Links of row j below the diagonal: every linked candidate (i, j) puts i
+// into row j's lower part (any order; the row sort orders it)
+__global__ void k_link_lower(const uint2* __restrict__ cand, const uint8_t* __restrict__ linked, uint64_t ncand,
+                             const uint64_t* __restrict__ rowptr, uint32_t* __restrict__ lfill,
+                             uint32_t* __restrict__ colx)
 {
 	const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= ncand || !linked[k])
 		return;
 	const uint2 c = cand[k];
-	colx[rowptr[c.x] + atomicAdd(rfill + c.x, 1u)] = c.y;
-	colx[rowptr[c.y] + atomicAdd(rfill + c.y, 1u)] = c.x;
+	colx[rowptr[c.y] + atomicAdd(lfill + c.y, 1u)] = c.x;
 }
 
-// factor of every link (columns ascending within a row); the reference
-// evaluates each pair with the lower index as A
-__global__ void k_link_factors(uint32_t N, const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
-                               const float* __restrict__ pos, const float* __restrict__ nrm, float* __restrict__ val)
+__global__ void k_row_sum(uint32_t N, const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                          uint32_t* __restrict__ out)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < N)
+		out[i] = a[i] + b[i];
+}
+
+// factor of every link (columns ascending within a row), one wave per row;
+// the reference evaluates each pair with the lower index as A
+__global__ __launch_bounds__(kBlock) void k_link_factors(uint32_t N, const uint64_t* __restrict__ rowptr,
+                                                         const uint32_t* __restrict__ col,
+                                                         const float* __restrict__ pos,
+                                                         const float* __restrict__ nrm, float* __restrict__ val)
+{
+	const uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
 	if (i >= N)
 		return;
 	const f3 ip = ld3(pos + 3 * i), in_ = ld3(nrm + 3 * i);
-	for (uint64_t o = rowptr[i]; o < rowptr[i + 1]; ++o) {
+	const uint64_t e = rowptr[i + 1];
+	for (uint64_t o = rowptr[i] + lane; o < e; o += 64) {
 		const uint32_t j = col[o];
 		const f3 jp = ld3(pos + 3 * j), jn = ld3(nrm + 3 * j);
 		val[o] = (i < j) ? pair_factor(ip, in_, jp, jn) : pair_factor(jp, jn, ip, in_);
 	}
+}
+
+// first candidate of every row from the all-pairs search's per-(row, chunk)
+// offsets
+__global__ void k_row_first(uint32_t N, uint32_t nchunks, const uint64_t* __restrict__ offs, uint64_t total,
+                            uint64_t* __restrict__ coff)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i <= N)
+		coff[i] = i < N ? offs[(uint64_t)i * nchunks] : total;
 }
 
 __global__ void k_u64_to_u32(const uint64_t* __restrict__ in, uint64_t n, uint32_t* __restrict__ out)
@@ -2696,7 +2747,7 @@ void DeviceScene::rad_set_patches(const float* pos, const float* nrm, size_t n)
 
 // All pairs (i, j > i) that pass the factor thresholds, in chunks of
 // kPairChunk columns per row (the reference's loop, lighter.cpp:728-749).
-void* DeviceScene::link_candidates_all(uint64_t& ncand)
+void* DeviceScene::link_candidates_all(uint64_t& ncand, uint64_t* coff)
 {
 	hipStream_t s = (hipStream_t)stream_;
 	const uint32_t N = (uint32_t)npatches;
@@ -2714,13 +2765,14 @@ void* DeviceScene::link_candidates_all(uint64_t& ncand)
 	scan_u32_to_u64(counts.p, ncounts, offs.p, &ncand, s);
 	uint2* cand = dalloc<uint2>(ncand);
 	k_pairs<true><<<g, kBlock, 0, s>>>(rad.pos, rad.nrm, N, nchunks, nullptr, offs.p, cand);
+	k_row_first<<<grid_for((uint64_t)N + 1), kBlock, 0, s>>>(N, nchunks, offs.p, ncand, coff);
 	HIPCHECK(hipGetLastError());
 	HIPCHECK(hipStreamSynchronize(s));
 	return cand;
 }
 
 // The same candidate set from the 27 grid cells around each patch.
-void* DeviceScene::link_candidates_grid(uint64_t& ncand)
+void* DeviceScene::link_candidates_grid(uint64_t& ncand, uint64_t* coff)
 {
 	hipStream_t s = (hipStream_t)stream_;
 	const uint32_t N = (uint32_t)npatches;
@@ -2728,8 +2780,11 @@ void* DeviceScene::link_candidates_grid(uint64_t& ncand)
 	// the longest linking distance (1 per mille margin over the float
 	// rounding of the factor), the cell width 1e-4 above it
 	const double r2 = links_nmax2_ / (M_PI * (double)kSmall) * 1.001;
-	if (!(r2 > 0.0))
-		return dalloc<uint2>(0); // zero normals: no pair passes
+	if (!(r2 > 0.0)) { // zero normals: no pair passes
+		HIPCHECK(hipMemsetAsync(coff, 0, 8 * ((size_t)N + 1), s));
+		HIPCHECK(hipStreamSynchronize(s));
+		return dalloc<uint2>(0);
+	}
 	double h = std::sqrt(r2) * 1.0001;
 	LinkGrid G;
 	uint64_t cells = 0;
@@ -2746,22 +2801,47 @@ void* DeviceScene::link_candidates_grid(uint64_t& ncand)
 	}
 	G.inv = (float)(1.0 / h);
 	const uint32_t nc = (uint32_t)cells;
-	DevArray<uint32_t> cell(N), ccount(nc + 1), order(N), counts(N);
-	DevArray<uint64_t> cstart(nc + 1), offs(N);
+	DevArray<uint32_t> cell(N), ccount(nc + 1), order(N), counts((uint64_t)N + 1);
+	DevArray<uint64_t> cstart(nc + 1);
 	HIPCHECK(hipMemsetAsync(ccount.p, 0, 4 * ((size_t)nc + 1), s));
 	k_grid_count<<<grid_for(N), kBlock, 0, s>>>(rad.pos, N, G, cell.p, ccount.p);
 	HIPCHECK(hipGetLastError());
+	if (links_mode == 0 && npatches <= kMaxAllPairsPatches) {
+		// the pairs the grid tests (both orders, each patch against its 27
+		// cells) against the all-pairs loop's N^2 / 2: where the grid does
+		// not prune (a room within one cell width), the loop is faster
+		std::vector<uint32_t> h(nc);
+		HIPCHECK(hipMemcpyAsync(h.data(), ccount.p, 4 * (size_t)nc, hipMemcpyDeviceToHost, s));
+		HIPCHECK(hipStreamSynchronize(s));
+		double tests = 0;
+		for (uint32_t z = 0; z < G.n[2]; ++z)
+			for (uint32_t y = 0; y < G.n[1]; ++y)
+				for (uint32_t x = 0; x < G.n[0]; ++x) {
+					const uint32_t c = (z * G.n[1] + y) * G.n[0] + x;
+					if (!h[c])
+						continue;
+					double nb = 0;
+					for (uint32_t zz = z ? z - 1 : 0; zz <= std::min(z + 1, G.n[2] - 1); ++zz)
+						for (uint32_t yy = y ? y - 1 : 0; yy <= std::min(y + 1, G.n[1] - 1); ++yy)
+							for (uint32_t xx = x ? x - 1 : 0; xx <= std::min(x + 1, G.n[0] - 1); ++xx)
+								nb += h[(zz * G.n[1] + yy) * G.n[0] + xx];
+					tests += (double)h[c] * nb;
+				}
+		if (tests > 0.25 * (double)N * (double)N)
+			return link_candidates_all(ncand, coff);
+	}
 	uint64_t total = 0;
 	scan_u32_to_u64(ccount.p, (uint64_t)nc + 1, cstart.p, &total, s);
 	HIPCHECK(hipMemsetAsync(ccount.p, 0, 4 * ((size_t)nc + 1), s));
 	k_grid_fill<<<grid_for(N), kBlock, 0, s>>>(N, cell.p, cstart.p, ccount.p, order.p);
+	HIPCHECK(hipMemsetAsync(counts.p + N, 0, 4, s));
 	k_pairs_grid<false><<<grid_for(N), kBlock, 0, s>>>(rad.pos, rad.nrm, N, G, cell.p, cstart.p, order.p, counts.p,
 	                                                   nullptr, nullptr);
 	HIPCHECK(hipGetLastError());
-	scan_u32_to_u64(counts.p, N, offs.p, &ncand, s);
+	scan_u32_to_u64(counts.p, (uint64_t)N + 1, coff, &ncand, s);
 	uint2* cand = dalloc<uint2>(ncand);
 	k_pairs_grid<true><<<grid_for(N), kBlock, 0, s>>>(rad.pos, rad.nrm, N, G, cell.p, cstart.p, order.p, nullptr,
-	                                                  offs.p, cand);
+	                                                  coff, cand);
 	HIPCHECK(hipGetLastError());
 	HIPCHECK(hipStreamSynchronize(s));
 	return cand;
@@ -2784,15 +2864,27 @@ size_t DeviceScene::rad_links()
 	EventTimer tm(s);
 	uint64_t ncand = 0;
 	uint2* cand = nullptr;
-	if (N > 1)
-		cand = (uint2*)((links_all_pairs || !links_finite_) ? link_candidates_all(ncand) : link_candidates_grid(ncand));
-	// occlusion test of every candidate, links per row (both ends)
-	DevArray<uint32_t> cnt((uint64_t)N + 1);
+	DevArray<uint64_t> coff((uint64_t)N + 1); // first candidate of each row
+	if (N > 1) {
+		cand = (uint2*)((links_mode == 1 || !links_finite_) ? link_candidates_all(ncand, coff.p)
+		                                                    : link_candidates_grid(ncand, coff.p));
+	} else {
+		HIPCHECK(hipMemsetAsync(coff.p, 0, 8 * ((size_t)N + 1), s));
+	}
+	// occlusion test of every candidate; links per row below the diagonal
+	// (atomics from the other end) and above it (the row's own candidates)
+	DevArray<uint32_t> lower((uint64_t)N + 1), upper((uint64_t)N + 1), cnt((uint64_t)N + 1);
+	HIPCHECK(hipMemsetAsync(lower.p, 0, 4 * ((size_t)N + 1), s));
 	HIPCHECK(hipMemsetAsync(cnt.p, 0, 4 * ((size_t)N + 1), s));
 	uint8_t* linked = dalloc<uint8_t>(ncand);
 	if (ncand) {
-		k_link_test<<<grid_for(ncand), kBlock, march_lds_bytes(), s>>>(V, rad.pos, cand, ncand, linked, cnt.p,
+		k_link_test<<<grid_for(ncand), kBlock, march_lds_bytes(), s>>>(V, rad.pos, cand, ncand, linked, lower.p,
 		                                                               march_stack_depth());
+		HIPCHECK(hipGetLastError());
+	}
+	if (N) {
+		k_link_upper<<<gridn_waves(N), kBlock, 0, s>>>(N, coff.p, cand, linked, upper.p, nullptr, nullptr, nullptr);
+		k_row_sum<<<grid_for(N), kBlock, 0, s>>>(N, upper.p, lower.p, cnt.p);
 		HIPCHECK(hipGetLastError());
 	}
 	// symmetric CSR, columns ascending within a row; 32-bit row offsets too
@@ -2810,14 +2902,16 @@ size_t DeviceScene::rad_links()
 	rad.val = dalloc<float>(nnz);
 	if (nnz) {
 		DevArray<uint32_t> colx(nnz);
-		HIPCHECK(hipMemsetAsync(cnt.p, 0, 4 * ((size_t)N + 1), s));
-		k_link_scatter<<<grid_for(ncand), kBlock, 0, s>>>(cand, linked, ncand, rad.rowptr64, cnt.p, colx.p);
+		k_link_upper<<<gridn_waves(N), kBlock, 0, s>>>(N, coff.p, cand, linked, nullptr, rad.rowptr64, lower.p,
+		                                                colx.p);
+		HIPCHECK(hipMemsetAsync(upper.p, 0, 4 * ((size_t)N + 1), s)); // the lower parts' fill counters
+		k_link_lower<<<grid_for(ncand), kBlock, 0, s>>>(cand, linked, ncand, rad.rowptr64, upper.p, colx.p);
 		HIPCHECK(hipGetLastError());
 		int bits = 1;
 		while (bits < 32 && (1ull << bits) < N)
 			++bits;
 		sort_row_columns(colx.p, rad.col, nnz, N, rad.rowptr64, bits, s);
-		k_link_factors<<<grid_for(N), kBlock, 0, s>>>(N, rad.rowptr64, rad.col, rad.pos, rad.nrm, rad.val);
+		k_link_factors<<<gridn_waves(N), kBlock, 0, s>>>(N, rad.rowptr64, rad.col, rad.pos, rad.nrm, rad.val);
 		HIPCHECK(hipGetLastError());
 		HIPCHECK(hipStreamSynchronize(s)); // before colx goes
 	}

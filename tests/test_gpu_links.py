@@ -6,10 +6,11 @@ only pass the factor threshold f = dA dB / (pi l^4) >= 0.001 when l^2 <=
 wide (every such pair lies in the same or a neighbouring cell) with the
 per-pair arithmetic of the all-pairs loop, and assembles the symmetric CSR
 by a per-row sort.  lgx_SetLinkSearch(dev, 1) switches to the all-pairs
-enumeration.  Both must give the same links and factors bit for bit, also
-above the 262,144 patches the earlier pair bitmask limited the search to;
-patches with non-finite positions or normals take the all-pairs path, whose
-links must then match the oracle's loop.
+enumeration, mode 2 forces the grid, and the default takes all pairs where
+the grid would not prune.  Every mode must give the same links and factors
+bit for bit, also above the 262,144 patches the earlier pair bitmask
+limited the search to; patches with non-finite positions or normals take
+the all-pairs path, whose links must then match the oracle's loop.
 """
 import ctypes as C
 
@@ -23,8 +24,9 @@ pytestmark = pytest.mark.gpu
 u32p = C.POINTER(C.c_uint32)
 
 
-def links(lib, dev, pos, nrm, all_pairs):
-    _lib.check(lib.lgx_SetLinkSearch(dev.h, 1 if all_pairs else 0), lib)
+def links(lib, dev, pos, nrm, mode):
+    """mode 0: automatic, 1: all pairs, 2: the grid"""
+    _lib.check(lib.lgx_SetLinkSearch(dev.h, mode), lib)
     nl = C.c_size_t(0)
     _lib.check(lib.lgx_RadLinks(dev.h, P(pos), P(nrm), len(pos), C.byref(nl)), lib)
     rowmap = np.zeros(2 * len(pos), np.uint32)
@@ -53,8 +55,10 @@ def test_grid_search_finds_the_all_pairs_links(lib, name):
         assert len(pos) > 262144
     dev = Device(lib, tris)
     try:
-        rm_g, ot_g, fa_g = links(lib, dev, pos, pn, all_pairs=False)
-        rm_a, ot_a, fa_a = links(lib, dev, pos, pn, all_pairs=True)
+        rm_g, ot_g, fa_g = links(lib, dev, pos, pn, 2)
+        rm_a, ot_a, fa_a = links(lib, dev, pos, pn, 1)
+        rm_d, ot_d, fa_d = links(lib, dev, pos, pn, 0)
+        assert np.array_equal(rm_d, rm_a) and np.array_equal(ot_d, ot_a)
         print(f"{name}: {len(pos)} patches, {len(ot_g)} links")
         assert len(ot_g) > 1000
         assert np.array_equal(rm_g, rm_a)
@@ -84,7 +88,7 @@ def test_non_finite_normals_take_the_all_pairs_loop(lib, orc):
     pn[17] = (np.inf, 0.0, 0.0)
     dev = Device(lib, tris)
     try:
-        rm, ot, fa = links(lib, dev, pos, pn, all_pairs=False)
+        rm, ot, fa = links(lib, dev, pos, pn, 2)
         rm2, ot2, fa2 = orc.rad_links(tris, pos, pn)
         assert np.array_equal(rm, rm2)
         assert np.array_equal(ot, ot2)

@@ -77,6 +77,8 @@ LGX_SIGNATURES = {
     "lgx_DirectLightGrad": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "lgx_SetCounting": (C.c_int, [vp, C.c_int]),
     "lgx_SetLinkSearch": (C.c_int, [vp, C.c_int]),
+    "lgx_PrepareTimes": (C.c_int, [vp, C.POINTER(C.c_double)]),
+    "lgx_SplitTreeCompare": (C.c_int, [C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_double)]),
     "lgx_ReadCounters": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
     "lgx_ReferenceCounts": (C.c_int, [vp, fp, C.c_size_t, C.POINTER(C.c_uint64)]),
     "lgx_ReferenceCountsAO": (C.c_int, [vp, fp, C.c_size_t, fp, C.c_int, C.c_float, C.c_float, C.c_float,

@@ -2,9 +2,12 @@
 #include "host_geom.h"
 
 #include <float.h>
+#include <math.h>
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <unordered_set>
 
 namespace lgx {
@@ -272,8 +275,9 @@ void TriSet::offset_sample(f3& P, f3 N, float dist) const
 static constexpr size_t kSplitMaxTris = 16;
 static constexpr int kSplitMaxDepth = 32;
 
-void SplitTree::set(const Tri* tris, size_t count)
+void SplitTree::set(const Tri* tris, size_t count, bool brute_pairs)
 {
+	brute_pairs_ = brute_pairs;
 	nodes_.clear();
 	nodes_.emplace_back();
 	nodes_[0].tris.assign(tris, tris + count);
@@ -318,6 +322,97 @@ inline uint32_t canon_bits(float f)
 }
 } // namespace
 
+// pts[j] - pts[i] of the pair i < j whose float squared length is the
+// largest, the first such pair in (i, j) order (lighter_math.cpp:451-460
+// scans every pair and keeps a pair only when it is strictly longer).  With
+// brute set, or any coordinate not finite, every pair is scanned.  Otherwise
+// the scan is restricted to the points that can be in such a pair: L, the
+// float squared length of a real pair (the longest among the extreme points
+// along 7 directions), is at most the maximum M; a point whose distance to
+// the farthest corner of the points' box is below sqrt(L) (with a margin
+// for the float rounding of the difference and the sum of squares) is in no
+// pair reaching M.  Every pair reaching M therefore joins two surviving
+// points, and scanning those in their original order finds the same first
+// pair, with the same float difference.
+static f3 farthest_pair_dir(const std::vector<f3>& pts, bool brute)
+{
+	auto scan = [&](const std::vector<uint32_t>* idx) {
+		const size_t n = idx ? idx->size() : pts.size();
+		f3 dir = mk3(0, 0, 0);
+		float best = 0;
+		for (size_t a = 0; a < n; ++a) {
+			const f3 pi = pts[idx ? (*idx)[a] : a];
+			for (size_t b = a + 1; b < n; ++b) {
+				const f3 nd = pts[idx ? (*idx)[b] : b] - pi;
+				const float l = lensq(nd);
+				if (l > best) {
+					dir = nd;
+					best = l;
+				}
+			}
+		}
+		return dir;
+	};
+	bool finite = true;
+	for (const f3& p : pts)
+		finite = finite && std::isfinite(p.x) && std::isfinite(p.y) && std::isfinite(p.z);
+	if (brute || !finite || pts.size() < 64)
+		return scan(nullptr);
+	double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+	// extreme points along x, y, z and the four diagonals
+	static const int dirs[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}};
+	size_t emin[7], emax[7];
+	double vmin[7], vmax[7];
+	for (int k = 0; k < 7; ++k) {
+		emin[k] = emax[k] = 0;
+		vmin[k] = INFINITY;
+		vmax[k] = -INFINITY;
+	}
+	for (size_t i = 0; i < pts.size(); ++i) {
+		const double c[3] = {pts[i].x, pts[i].y, pts[i].z};
+		for (int a = 0; a < 3; ++a) {
+			lo[a] = std::min(lo[a], c[a]);
+			hi[a] = std::max(hi[a], c[a]);
+		}
+		for (int k = 0; k < 7; ++k) {
+			const double v = dirs[k][0] * c[0] + dirs[k][1] * c[1] + dirs[k][2] * c[2];
+			if (v < vmin[k]) {
+				vmin[k] = v;
+				emin[k] = i;
+			}
+			if (v > vmax[k]) {
+				vmax[k] = v;
+				emax[k] = i;
+			}
+		}
+	}
+	std::vector<size_t> ext;
+	for (int k = 0; k < 7; ++k) {
+		ext.push_back(emin[k]);
+		ext.push_back(emax[k]);
+	}
+	float L = 0;
+	for (size_t a = 0; a < ext.size(); ++a)
+		for (size_t b = 0; b < ext.size(); ++b)
+			if (ext[a] < ext[b])
+				L = std::max(L, lensq(pts[ext[b]] - pts[ext[a]]));
+	// the rounding of each difference (1 + 2^-24) and of the squares and
+	// sums (four more roundings) stays far inside this margin
+	const double margin = 1.0 + 1e-5;
+	std::vector<uint32_t> keep;
+	for (size_t i = 0; i < pts.size(); ++i) {
+		const double c[3] = {pts[i].x, pts[i].y, pts[i].z};
+		double ub = 0;
+		for (int a = 0; a < 3; ++a) {
+			const double e = std::max(hi[a] - c[a], c[a] - lo[a]);
+			ub += e * e;
+		}
+		if (ub * margin >= (double)L)
+			keep.push_back((uint32_t)i);
+	}
+	return scan(&keep);
+}
+
 bool SplitTree::pick_plane(Node& node)
 {
 	// unique vertices in first-seen order, equality as float ==
@@ -337,19 +432,10 @@ bool SplitTree::pick_plane(Node& node)
 		add(t.p2);
 		add(t.p3);
 	}
-	f3 center = mk3(0, 0, 0), dir = mk3(0, 0, 0);
-	float best = 0;
-	for (size_t i = 0; i < pts.size(); ++i) {
+	f3 center = mk3(0, 0, 0);
+	for (size_t i = 0; i < pts.size(); ++i)
 		center += pts[i];
-		for (size_t j = i + 1; j < pts.size(); ++j) {
-			f3 nd = pts[j] - pts[i];
-			float l = lensq(nd);
-			if (l > best) {
-				dir = nd;
-				best = l;
-			}
-		}
-	}
+	const f3 dir = farthest_pair_dir(pts, brute_pairs_);
 	if (!pts.empty())
 		center = center / (float)pts.size();
 	node.N = normalized(dir);
@@ -487,6 +573,31 @@ float SplitTree::intersect(f3 from, f3 to, f3* outnormal) const
 	if (nodes_.empty())
 		return 2.0f;
 	return node_intersect(0, from, to, outnormal);
+}
+
+// The split tree of a triangle set built with the pruned and with the full
+// scan of vertex pairs; true when both give the same flat tree (nodes and
+// leaf triangles, byte for byte).
+bool split_tree_compare(const Tri* tris, size_t count, double* ms_pruned, double* ms_full)
+{
+	using clk = std::chrono::steady_clock;
+	SplitTree a, b;
+	auto t0 = clk::now();
+	a.set(tris, count, false);
+	auto t1 = clk::now();
+	b.set(tris, count, true);
+	auto t2 = clk::now();
+	if (ms_pruned)
+		*ms_pruned = std::chrono::duration<double, std::milli>(t1 - t0).count();
+	if (ms_full)
+		*ms_full = std::chrono::duration<double, std::milli>(t2 - t1).count();
+	if (a.num_nodes() != b.num_nodes() || a.num_leaf_tris() != b.num_leaf_tris())
+		return false;
+	std::vector<FlatSplitNode> na(a.num_nodes()), nb(b.num_nodes());
+	std::vector<FlatSplitTri> ta(a.num_leaf_tris()), tb(b.num_leaf_tris());
+	const int da = a.flatten(na.data(), ta.data()), db = b.flatten(nb.data(), tb.data());
+	return da == db && memcmp(na.data(), nb.data(), na.size() * sizeof(FlatSplitNode)) == 0 &&
+	       memcmp(ta.data(), tb.data(), ta.size() * sizeof(FlatSplitTri)) == 0;
 }
 
 } // namespace lgx

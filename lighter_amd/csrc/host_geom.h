@@ -174,7 +174,9 @@ struct FlatSplitTri {
 
 class SplitTree {
 public:
-	void set(const Tri* tris, size_t count);
+	// brute_pairs: pick every plane with the full scan of vertex pairs (the
+	// reference's loop) instead of the pruned scan; both give the same tree
+	void set(const Tri* tris, size_t count, bool brute_pairs = false);
 	bool empty() const { return nodes_.empty(); }
 	size_t num_nodes() const { return nodes_.size(); }
 	size_t num_leaf_tris() const;
@@ -188,8 +190,8 @@ public:
 
 private:
 	struct Node {
-		f3 N;
-		float D;
+		f3 N = {0, 0, 0}; // the plane (leaves that were never split keep zeros)
+		float D = 0;
 		int32_t front = -1, back = -1;
 		std::vector<Tri> tris;
 		// leaves: the segment-test operands of the triangles with a usable
@@ -199,8 +201,13 @@ private:
 	};
 	void split(int32_t node, int depth);
 	bool pick_plane(Node& n);
+	bool brute_pairs_ = false; // scan every vertex pair when picking a plane (tests)
 	float node_intersect(int32_t node, f3 from, f3 to, f3* outnormal) const;
 	std::vector<Node> nodes_;
 };
+
+// Builds the split tree of a triangle set with the pruned and with the full
+// scan of vertex pairs; true when the two flat trees are identical.
+bool split_tree_compare(const Tri* tris, size_t count, double* ms_pruned, double* ms_full);
 
 } // namespace lgx

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <functional>
 #include <stdexcept>
 #include <thread>
@@ -441,23 +442,35 @@ void prepare(ltr_Scene* S)
 	if (S->prepared)
 		return;
 	int nt = S->nthreads;
+	using clk = std::chrono::steady_clock;
+	auto ms_since = [](clk::time_point t0) {
+		return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+	};
+	auto t0 = clk::now();
 	set_stage(S, "transforming spatial data");
 	parallel_for(nt, S->insts.size(), [&](size_t i) { pre_xform(S, S->insts[i]); });
+	S->prep_ms[0] = ms_since(t0);
 	check_abort(S);
 
+	t0 = clk::now();
 	set_stage(S, "generating data structures");
 	parallel_for(nt, S->insts.size(), [&](size_t i) { col_info(S, S->insts[i]); });
+	S->prep_ms[1] = ms_since(t0);
 	S->completion.store(0.99f);
+	t0 = clk::now();
 	std::vector<Box> bbs;
 	for (Instance* mi : S->insts) {
 		const BoxTree::Node& r = mi->tris.tree().root();
 		bbs.push_back(Box{r.mn, r.mx});
 	}
 	S->inst_tree.build(bbs.data(), bbs.size());
+	S->prep_ms[2] = ms_since(t0);
 	check_abort(S);
 
+	t0 = clk::now();
 	set_stage(S, "generating samples");
 	gen_samples(S);
+	S->prep_ms[3] = ms_since(t0);
 	check_abort(S);
 
 	S->inst_off.assign(S->insts.size() + 1, 0);

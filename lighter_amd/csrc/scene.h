@@ -106,5 +106,8 @@ struct ltr_Scene {
 	lgx::RasterTimes raster_times;
 	std::vector<float> nm_data; // normal-map data of the GPU pass: normal xyz + min-dot per sample
 	lgx::PlaceTimes place_times;
+	// wall ms of the preparation steps: transforms, collision set-up (the
+	// instances' split and triangle trees), the instance tree, samples
+	double prep_ms[4] = {0, 0, 0, 0};
 	lgx::FinTimes fin_times;
 };

@@ -415,17 +415,35 @@ static f3 farthest_pair_dir(const std::vector<f3>& pts, bool brute)
 
 bool SplitTree::pick_plane(Node& node)
 {
-	// unique vertices in first-seen order, equality as float ==
+	// unique vertices in first-seen order, equality as float == (+0 and -0
+	// alike, NaN never equal): an open-addressing table of indices into
+	// pts, reused by the thread's later nodes
 	std::vector<f3> pts;
-	std::unordered_set<PointKey, PointKeyHash> seen;
+	pts.reserve(3 * node.tris.size());
+	static thread_local std::vector<uint32_t> slots;
+	size_t cap = 16;
+	while (cap < 6 * node.tris.size())
+		cap <<= 1;
+	slots.assign(cap, 0u);
+	const size_t mask = cap - 1;
+	PointKeyHash hash;
 	auto add = [&](f3 p) {
 		if (p.x != p.x || p.y != p.y || p.z != p.z) {
-			pts.push_back(p); // NaN never compares equal
+			pts.push_back(p);
 			return;
 		}
-		PointKey k{canon_bits(p.x), canon_bits(p.y), canon_bits(p.z)};
-		if (seen.insert(k).second)
-			pts.push_back(p);
+		const PointKey k{canon_bits(p.x), canon_bits(p.y), canon_bits(p.z)};
+		for (size_t h = hash(k) & mask;; h = (h + 1) & mask) {
+			const uint32_t e = slots[h];
+			if (e == 0) {
+				pts.push_back(p);
+				slots[h] = (uint32_t)pts.size();
+				return;
+			}
+			const f3 q = pts[e - 1];
+			if (PointKey{canon_bits(q.x), canon_bits(q.y), canon_bits(q.z)} == k)
+				return;
+		}
 	};
 	for (const Tri& t : node.tris) {
 		add(t.p1);

@@ -72,12 +72,14 @@ def placement_info(lib, b):
     lib.lgx_PlacementTimes(b.scene, t)
     r = (C.c_double * 3)()
     lib.lgx_RasterTimes(b.scene, r)
-    p = (C.c_double * 4)()
+    p = (C.c_double * 7)()
     lib.lgx_PrepareTimes(b.scene, p)
     return dict(samples=int(t[4]), kernel_ms=round(t[2], 3), flatten_ms=round(t[0], 1), upload_ms=round(t[1], 1),
                 download_ms=round(t[3], 1), raster_kernels_ms=round(r[0], 3), raster_texels=int(r[1]),
                 prepare_steps_ms=dict(transforms=round(p[0], 1), collision_setup=round(p[1], 1),
-                                      instance_tree=round(p[2], 1), samples=round(p[3], 1)))
+                                      instance_tree=round(p[2], 1), samples=round(p[3], 1),
+                                      samples_rasterisation=round(p[4], 1), samples_placement=round(p[5], 1),
+                                      samples_commit=round(p[6], 1)))
 
 
 def placement_vs_host(lib, spec, nthreads, b_gpu, pos_gpu):

@@ -61,9 +61,10 @@ LTRAPI int lgx_RasterTimes(ltr_Scene* scene, double out[3]);
 LTRAPI int lgx_PlacementTimes(ltr_Scene* scene, double out[5]);
 /* No counterpart in the reference. Wall ms of the last preparation's steps:
  * out[0] transforms, out[1] collision set-up (each instance's split tree
- * and triangle tree), out[2] the instance tree, out[3] sample generation
- * (rasterisation and placement). */
-LTRAPI int lgx_PrepareTimes(ltr_Scene* scene, double out[4]);
+ * and triangle tree), out[2] the instance tree, out[3] sample generation,
+ * and within it out[4] rasterisation, out[5] placement, out[6] the
+ * samples' commit to their instances. */
+LTRAPI int lgx_PrepareTimes(ltr_Scene* scene, double out[7]);
 /* No counterpart in the reference (tests). The split tree (BSP) of n
  * triangles (9 floats each; lighter_math.cpp:428-488) built with the pruned
  * farthest-pair search and with the reference's full pair scan: out[0] 1

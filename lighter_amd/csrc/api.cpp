@@ -433,9 +433,9 @@ int lgx_SplitTreeCompare(const float* tris9, size_t n, double out[3])
 	});
 }
 
-int lgx_PrepareTimes(ltr_Scene* S, double out[4])
+int lgx_PrepareTimes(ltr_Scene* S, double out[7])
 {
-	for (int i = 0; i < 4; ++i)
+	for (int i = 0; i < 7; ++i)
 		out[i] = S->prep_ms[i];
 	return 1;
 }

@@ -407,6 +407,8 @@ static void gen_samples(ltr_Scene* S)
 			for (auto& c : mi->lightmap)
 				c = ld3(S->config.ambient_color);
 	S->raster_times = RasterTimes{};
+	using clk = std::chrono::steady_clock;
+	auto t0 = clk::now();
 	if (S->host_raster)
 		parallel_for(nt, S->insts.size(), [&](size_t i) {
 			if (!S->insts[i]->samplecont)
@@ -414,15 +416,21 @@ static void gen_samples(ltr_Scene* S)
 		});
 	else
 		raster_samples_gpu(S, raw);
+	auto t1 = clk::now();
 	check_abort(S);
 	if (S->host_place)
 		parallel_for(nt, S->insts.size(), [&](size_t i) { place_host(S, raw[i]); });
 	else
 		place_gpu(S, raw);
+	auto t2 = clk::now();
 	parallel_for(nt, S->insts.size(), [&](size_t i) {
 		if (!S->insts[i]->samplecont)
 			commit_samples(S, S->insts[i], raw[i]);
 	});
+	auto t3 = clk::now();
+	S->prep_ms[4] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+	S->prep_ms[5] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+	S->prep_ms[6] = std::chrono::duration<double, std::milli>(t3 - t2).count();
 }
 
 // Per-bake state back to that of a scene that has not been baked: the next

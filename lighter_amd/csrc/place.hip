@@ -12,9 +12,14 @@
 //    reaches it and the near side found no hit below 1, which a stack of
 //    node indices and "result 2" markers reproduces (the value of the last
 //    subtree searched is the tree's result, and a hit below 1 ends the walk).
-// Stacks are per lane and sized from the trees' depths, checked on the host
-// before the launch.  This is a preprocessing step that runs once per bake;
-// its cost is dominated by divergent tree walks, not by memory traffic.
+// The BoxTrees are stored in pre-order (node n's first child is n + 1, its
+// second child ch follows the first child's subtree), so their walks need
+// no stack: every node carries the node that follows its subtree, and a
+// walk goes to n + 1 after a node with children and to that node after a
+// leaf or a culled box, which visits the nodes in the stack walk's order.
+// The split-tree walk keeps a per-lane stack, sized from the trees' depths
+// and checked on the host before the launch.  This is a preprocessing step
+// that runs once per bake; its cost is dominated by divergent tree walks.
 #include <hip/hip_runtime.h>
 
 #include <string.h>
@@ -39,12 +44,26 @@ namespace lgx {
 	} while (0)
 
 constexpr int kPlaceBlock = 256;
-// Stack entries a walk needs equal the tree's depth (nodes on the longest
-// root-to-leaf path): BoxTrees stop splitting at depth 16 and SplitTrees at
-// 32 (lighter_math.cpp:654, 430-435), so 17 and 33 suffice; both are
-// checked per tree before the launch.
-constexpr int kBoxStack = 24;
+// Stack entries the split-tree walk needs equal the tree's depth (nodes on
+// the longest root-to-leaf path): SplitTrees stop splitting at depth 32
+// (lighter_math.cpp:430-435), so 33 suffice; checked per tree before the
+// launch.
 constexpr int kSplitStack = 40;
+
+// A BoxTree node for the stackless walk: link = (the node after this
+// node's subtree) << 1 | (the node has children); kWalkEnd after the root's
+// subtree.
+struct PlaceNode {
+	f3 mn, mx;
+	int32_t ido;
+	uint32_t link;
+};
+constexpr uint32_t kWalkEnd = 0x3FFFFFFFu;
+
+__host__ __device__ __forceinline__ uint32_t walk_next(uint32_t n, uint32_t link)
+{
+	return (link & 1u) ? n + 1 : link >> 1;
+}
 
 struct PlaceTri {
 	f3 p1, p2, p3;
@@ -58,13 +77,13 @@ struct PlaceInst {
 };
 
 struct PlaceView {
-	const BoxTree::Node* __restrict__ nodes;
+	const PlaceNode* __restrict__ nodes;
 	const int32_t* __restrict__ items;
 	const PlaceTri* __restrict__ tris;
 	const FlatSplitNode* __restrict__ snodes;
 	const FlatSplitTri* __restrict__ stris;
 	const PlaceInst* __restrict__ inst;
-	const BoxTree::Node* __restrict__ inodes; // instance tree
+	const PlaceNode* __restrict__ inodes; // instance tree
 	const int32_t* __restrict__ iitems;
 	int32_t ninst;
 };
@@ -77,14 +96,12 @@ __host__ __device__ float pl_ray_dist(const PlaceView& V, const PlaceInst& I, f3
 	r.set(from, to);
 	float closest = 2.0f;
 	int32_t hit = -1;
-	int32_t st[kBoxStack];
-	int sp = 0;
-	st[sp++] = 0;
-	while (sp) {
-		const int32_t n = st[--sp];
-		const BoxTree::Node N = V.nodes[I.node_off + n];
-		if (box_disjoint(r.qmn, r.qmx, N.mn, N.mx) || !ray_box(r, N.mn, N.mx))
+	for (uint32_t n = 0; n != kWalkEnd;) {
+		const PlaceNode N = V.nodes[I.node_off + n];
+		if (box_disjoint(r.qmn, r.qmx, N.mn, N.mx) || !ray_box(r, N.mn, N.mx)) {
+			n = N.link >> 1;
 			continue;
+		}
 		if (N.ido != -1) {
 			const int32_t* it = V.items + I.item_off + N.ido;
 			const int32_t c = it[0];
@@ -98,10 +115,7 @@ __host__ __device__ float pl_ray_dist(const PlaceView& V, const PlaceInst& I, f3
 				}
 			}
 		}
-		if (N.ch != -1) {
-			st[sp++] = N.ch;
-			st[sp++] = n + 1;
-		}
+		n = walk_next(n, N.link);
 	}
 	if (hit != -1)
 		*tid = hit;
@@ -113,14 +127,12 @@ __host__ __device__ void pl_offset(const PlaceView& V, const PlaceInst& I, f3& P
 {
 	const f3 d3 = mk3(dist, dist, dist);
 	const f3 qmn = P - d3, qmx = P + d3;
-	int32_t st[kBoxStack];
-	int sp = 0;
-	st[sp++] = 0;
-	while (sp) {
-		const int32_t n = st[--sp];
-		const BoxTree::Node B = V.nodes[I.node_off + n];
-		if (box_disjoint(qmn, qmx, B.mn, B.mx))
+	for (uint32_t n = 0; n != kWalkEnd;) {
+		const PlaceNode B = V.nodes[I.node_off + n];
+		if (box_disjoint(qmn, qmx, B.mn, B.mx)) {
+			n = B.link >> 1;
 			continue;
+		}
 		if (B.ido != -1) {
 			const int32_t* it = V.items + I.item_off + B.ido;
 			const int32_t c = it[0];
@@ -137,14 +149,13 @@ __host__ __device__ void pl_offset(const PlaceView& V, const PlaceInst& I, f3& P
 					P = pnew;
 			}
 		}
-		if (B.ch != -1) {
-			st[sp++] = B.ch;
-			st[sp++] = n + 1;
-		}
+		n = walk_next(n, B.link);
 	}
 }
 
-// SplitTree::intersect (lighter_math.cpp:490-533).
+// SplitTree::intersect (lighter_math.cpp:490-533).  The stack's entries
+// are node indices or -1 (the far side of a split, not searched); it lives
+// in private memory (a 16-bit copy in LDS measured slower).
 __host__ __device__ float pl_split(const PlaceView& V, const PlaceInst& I, f3 from, f3 to, f3* outnormal)
 {
 	if (I.nsplit == 0)
@@ -207,14 +218,12 @@ __host__ __device__ float pl_scene_hit(const PlaceView& V, f3 A, f3 B, f3* outno
 	RaySeg r;
 	r.set(mA, mB);
 	float closest = 2.0f;
-	int32_t st[kBoxStack];
-	int sp = 0;
-	st[sp++] = 0;
-	while (sp) {
-		const int32_t n = st[--sp];
-		const BoxTree::Node N = V.inodes[n];
-		if (box_disjoint(r.qmn, r.qmx, N.mn, N.mx) || !ray_box(r, N.mn, N.mx))
+	for (uint32_t n = 0; n != kWalkEnd;) {
+		const PlaceNode N = V.inodes[n];
+		if (box_disjoint(r.qmn, r.qmx, N.mn, N.mx) || !ray_box(r, N.mn, N.mx)) {
+			n = N.link >> 1;
 			continue;
+		}
 		if (N.ido != -1) {
 			const int32_t* it = V.iitems + N.ido;
 			const int32_t c = it[0];
@@ -234,10 +243,7 @@ __host__ __device__ float pl_scene_hit(const PlaceView& V, f3 A, f3 B, f3* outno
 				}
 			}
 		}
-		if (N.ch != -1) {
-			st[sp++] = N.ch;
-			st[sp++] = n + 1;
-		}
+		n = walk_next(n, N.link);
 	}
 	return closest;
 }
@@ -269,13 +275,13 @@ __host__ __device__ f3 pl_place(const PlaceView& V, f3 P, f3 N, float d, float m
 	return P;
 }
 
-// 8 waves per SIMD (64 VGPRs, the walk stacks and spills in scratch): the
-// divergent walks hide their latency with more waves.  C4 city block:
-// 358 ms at the compiler's 3 waves, 235 ms at 5, 222 ms at 6, 196 ms at 8.
-__global__ __launch_bounds__(kPlaceBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_place(PlaceView V, float* __restrict__ pos,
-                                                       const float* __restrict__ nrm,
-                                                       const float* __restrict__ dist, uint32_t n, float max_corr,
-                                                       float corr_min_dot)
+// 8 waves per SIMD (64 VGPRs): the divergent walks hide their latency with
+// more waves.  C4 city block with the BoxTree walks on stacks: 358 ms at
+// the compiler's 3 waves, 235 ms at 5, 222 ms at 6, 196 ms at 8; with the
+// stackless walks 165 ms at 8 and 176 ms at 6.
+__global__ __launch_bounds__(kPlaceBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_place(
+    PlaceView V, float* __restrict__ pos, const float* __restrict__ nrm, const float* __restrict__ dist, uint32_t n,
+    float max_corr, float corr_min_dot)
 {
 	const uint32_t i = blockIdx.x * kPlaceBlock + threadIdx.x;
 	if (i >= n)
@@ -334,6 +340,29 @@ struct PlaceLayout {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// A BoxTree's nodes for the stackless walk.  Parents come before their
+// children, so one pass in index order gives every node the node after its
+// subtree: the first child's is the second child, the second child's is
+// its parent's.
+void walk_nodes(const std::vector<BoxTree::Node>& in, PlaceNode* out)
+{
+	const size_t n = in.size();
+	if (n == 0)
+		return;
+	std::vector<uint32_t> after(n, kWalkEnd);
+	for (size_t k = 0; k < n; ++k) {
+		const int32_t ch = in[k].ch;
+		if (ch == -1)
+			continue;
+		if ((size_t)ch <= k + 1 || (size_t)ch >= n)
+			throw std::runtime_error("sample placement: a tree node's children are not in pre-order");
+		after[k + 1] = (uint32_t)ch;
+		after[ch] = after[k];
+	}
+	for (size_t k = 0; k < n; ++k)
+		out[k] = PlaceNode{in[k].mn, in[k].mx, in[k].ido, (after[k] << 1) | (in[k].ch != -1 ? 1u : 0u)};
+}
+
 // Sizes, offsets and the instance records (serial; sizes only).
 PlaceLayout plan(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, size_t n,
                  std::vector<PlaceInst>& inst)
@@ -358,7 +387,7 @@ PlaceLayout plan(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tr
 		nt += insts[k].tris->tris().size();
 		sn += insts[k].split->num_nodes();
 		stt += insts[k].split->num_leaf_tris();
-		if (nn > 0x7FFFFFFF || ni > 0x7FFFFFFF || nt > 0x7FFFFFFF || sn > 0x7FFFFFFF || stt > 0x7FFFFFFF)
+		if (nn >= kWalkEnd || ni > 0x7FFFFFFF || nt > 0x7FFFFFFF || sn > 0x7FFFFFFF || stt > 0x7FFFFFFF)
 			throw std::runtime_error("sample placement: collision data beyond 2^31 entries");
 	}
 	PlaceLayout L;
@@ -369,12 +398,12 @@ PlaceLayout plan(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tr
 		return at;
 	};
 	L.inst = take(inst.size() * sizeof(PlaceInst));
-	L.nodes = take(nn * sizeof(BoxTree::Node));
+	L.nodes = take(nn * sizeof(PlaceNode));
 	L.items = take(ni * sizeof(int32_t));
 	L.tris = take(nt * sizeof(PlaceTri));
 	L.snodes = take(sn * sizeof(FlatSplitNode));
 	L.stris = take(stt * sizeof(FlatSplitTri));
-	L.inodes = take(inst_tree.nodes().size() * sizeof(BoxTree::Node));
+	L.inodes = take(inst_tree.nodes().size() * sizeof(PlaceNode));
 	L.iitems = take(inst_tree.items().size() * sizeof(int32_t));
 	L.pos = take(3 * n * sizeof(float));
 	L.nrm = take(3 * n * sizeof(float));
@@ -389,16 +418,16 @@ void fill(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, con
           const PlaceLayout& L, char* base, int nthreads)
 {
 	memcpy(base + L.inst, inst.data(), inst.size() * sizeof(PlaceInst));
-	std::copy(inst_tree.nodes().begin(), inst_tree.nodes().end(), (BoxTree::Node*)(base + L.inodes));
+	walk_nodes(inst_tree.nodes(), (PlaceNode*)(base + L.inodes));
 	std::copy(inst_tree.items().begin(), inst_tree.items().end(), (int32_t*)(base + L.iitems));
 	std::atomic<size_t> next{0};
-	std::atomic<int> box_depth{inst_tree.depth()}, split_depth{0};
+	std::atomic<int> split_depth{0};
 	auto work = [&]() {
 		for (size_t k; (k = next.fetch_add(1)) < insts.size();) {
 			const TriSet& ts = *insts[k].tris;
 			const BoxTree& bt = ts.tree();
 			const PlaceInst& I = inst[k];
-			std::copy(bt.nodes().begin(), bt.nodes().end(), (BoxTree::Node*)(base + L.nodes) + I.node_off);
+			walk_nodes(bt.nodes(), (PlaceNode*)(base + L.nodes) + I.node_off);
 			std::copy(bt.items().begin(), bt.items().end(), (int32_t*)(base + L.items) + I.item_off);
 			const std::vector<Tri>& tv = ts.tris();
 			const std::vector<Box>& bv = ts.boxes();
@@ -407,9 +436,6 @@ void fill(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, con
 				tris[j] = PlaceTri{tv[j].p1, tv[j].p2, tv[j].p3, bv[j].mn, bv[j].mx};
 			const int sd = insts[k].split->flatten((FlatSplitNode*)(base + L.snodes) + I.split_off,
 			                                       (FlatSplitTri*)(base + L.stris) + I.stri_off);
-			const int bd = bt.depth();
-			for (int cur = box_depth.load(); bd > cur && !box_depth.compare_exchange_weak(cur, bd);) {
-			}
 			for (int cur = split_depth.load(); sd > cur && !split_depth.compare_exchange_weak(cur, sd);) {
 			}
 		}
@@ -421,7 +447,6 @@ void fill(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, con
 	work();
 	for (auto& t : th)
 		t.join();
-	check_depth(box_depth.load(), kBoxStack, "triangle or instance tree");
 	check_depth(split_depth.load(), kSplitStack, "split tree");
 }
 
@@ -429,13 +454,13 @@ void fill(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, con
 PlaceView view_of(const char* base, const PlaceLayout& L, int32_t ninst)
 {
 	PlaceView V;
-	V.nodes = (const BoxTree::Node*)(base + L.nodes);
+	V.nodes = (const PlaceNode*)(base + L.nodes);
 	V.items = (const int32_t*)(base + L.items);
 	V.tris = (const PlaceTri*)(base + L.tris);
 	V.snodes = (const FlatSplitNode*)(base + L.snodes);
 	V.stris = (const FlatSplitTri*)(base + L.stris);
 	V.inst = (const PlaceInst*)(base + L.inst);
-	V.inodes = (const BoxTree::Node*)(base + L.inodes);
+	V.inodes = (const PlaceNode*)(base + L.inodes);
 	V.iitems = (const int32_t*)(base + L.iitems);
 	V.ninst = ninst;
 	return V;

@@ -107,7 +107,8 @@ struct ltr_Scene {
 	std::vector<float> nm_data; // normal-map data of the GPU pass: normal xyz + min-dot per sample
 	lgx::PlaceTimes place_times;
 	// wall ms of the preparation steps: transforms, collision set-up (the
-	// instances' split and triangle trees), the instance tree, samples
-	double prep_ms[4] = {0, 0, 0, 0};
+	// instances' split and triangle trees), the instance tree, samples; and
+	// within the samples: rasterisation, placement, the samples' commit
+	double prep_ms[7] = {0, 0, 0, 0, 0, 0, 0};
 	lgx::FinTimes fin_times;
 };

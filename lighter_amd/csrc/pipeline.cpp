@@ -217,9 +217,13 @@ static float scene_split_hit(ltr_Scene* S, f3 A, f3 B, f3* outnormal)
 // The instance's triangles as the rasteriser draws them, in part and index
 // order: texel-space corners, world positions and normals, and
 // (uv0, part, texel area) per corner.
-static void raster_tris(Instance* mi, std::vector<RasterTri>& out)
+// The drawing-order triangles of one instance, each given to emit (a
+// RasterTri); returns their count.
+template <class Emit>
+static size_t raster_tris_each(Instance* mi, Emit&& emit)
 {
 	ltr_Mesh* mesh = mi->mesh;
+	size_t n = 0;
 	for (uint32_t part = 0; part < mesh->parts.size(); ++part) {
 		part_triangles(mesh, mesh->parts[part], [&](uint32_t a, uint32_t b, uint32_t c) {
 			const float area = sample_area(mi->ltex[a], mi->ltex[b], mi->ltex[c], mi->vpos[a], mi->vpos[b],
@@ -227,10 +231,24 @@ static void raster_tris(Instance* mi, std::vector<RasterTri>& out)
 			const f4 ia{mi->vtex[a].x, mi->vtex[a].y, (float)part, area};
 			const f4 ib{mi->vtex[b].x, mi->vtex[b].y, (float)part, area};
 			const f4 ic{mi->vtex[c].x, mi->vtex[c].y, (float)part, area};
-			out.push_back(raster_setup(mi->ltex[a], mi->ltex[b], mi->ltex[c], mi->vpos[a], mi->vpos[b], mi->vpos[c],
-			                           mi->vnrm[a], mi->vnrm[b], mi->vnrm[c], ia, ib, ic));
+			emit(n++, raster_setup(mi->ltex[a], mi->ltex[b], mi->ltex[c], mi->vpos[a], mi->vpos[b], mi->vpos[c],
+			                       mi->vnrm[a], mi->vnrm[b], mi->vnrm[c], ia, ib, ic));
 		});
 	}
+	return n;
+}
+
+static size_t count_raster_tris(Instance* mi)
+{
+	size_t n = 0;
+	for (const MeshPart& mp : mi->mesh->parts)
+		part_triangles(mi->mesh, mp, [&](uint32_t, uint32_t, uint32_t) { ++n; });
+	return n;
+}
+
+static void raster_tris(Instance* mi, std::vector<RasterTri>& out)
+{
+	raster_tris_each(mi, [&](size_t, const RasterTri& t) { out.push_back(t); });
 }
 
 // Rasterised texels of one instance with a nonzero normal, in texel order:
@@ -283,33 +301,49 @@ static void raster_samples_gpu(ltr_Scene* S, std::vector<RasterSamples>& raw)
 	for (size_t i = 0; i < S->insts.size(); ++i)
 		if (!S->insts[i]->samplecont)
 			ids.push_back(i);
-	std::vector<std::vector<RasterTri>> per(ids.size());
-	parallel_for(S->nthreads, ids.size(), [&](size_t q) { raster_tris(S->insts[ids[q]], per[q]); });
-	std::vector<RasterTri> tris;
-	std::vector<uint32_t> tri_atlas;
+	// every instance's triangles in one array, each instance filling its own
+	// range in parallel
+	std::vector<size_t> first(ids.size() + 1, 0);
+	std::vector<size_t> cnt(ids.size());
+	parallel_for(S->nthreads, ids.size(), [&](size_t q) { cnt[q] = count_raster_tris(S->insts[ids[q]]); });
+	for (size_t q = 0; q < ids.size(); ++q)
+		first[q + 1] = first[q] + cnt[q];
+	std::vector<RasterTri> tris(first.back());
+	std::vector<uint32_t> tri_atlas(first.back());
 	std::vector<RasterAtlasSize> sizes;
-	for (size_t q = 0; q < ids.size(); ++q) {
-		tris.insert(tris.end(), per[q].begin(), per[q].end());
-		tri_atlas.insert(tri_atlas.end(), per[q].size(), (uint32_t)q);
+	for (size_t q = 0; q < ids.size(); ++q)
 		sizes.push_back(RasterAtlasSize{S->insts[ids[q]]->lm_w, S->insts[ids[q]]->lm_h});
-	}
+	std::atomic<bool> miscount{false};
+	parallel_for(S->nthreads, ids.size(), [&](size_t q) {
+		RasterTri* dst = tris.data() + first[q];
+		const size_t n = raster_tris_each(S->insts[ids[q]], [&](size_t k, const RasterTri& t) {
+			if (k < cnt[q])
+				dst[k] = t;
+		});
+		if (n != cnt[q])
+			miscount.store(true);
+		std::fill(tri_atlas.begin() + first[q], tri_atlas.begin() + first[q + 1], (uint32_t)q);
+	});
+	if (miscount.load())
+		throw std::runtime_error("rasterisation: an instance's triangle count changed between two passes");
 	if (ids.empty())
 		return;
 	require_gpu();
-	std::vector<RasterOut> out;
+	RasterOut out;
 	rasterize_gpu(tris, tri_atlas, sizes, out, &S->raster_times);
 	parallel_for(S->nthreads, ids.size(), [&](size_t q) {
-		const RasterOut& o = out[q];
+		const uint64_t f = out.first[q], c = out.count[q];
 		RasterSamples& r = raw[ids[q]];
-		r.pos.resize(o.count);
-		r.nrm.resize(o.count);
-		r.info.resize(o.count);
-		r.loc = o.loc;
-		r.dist.resize(o.count);
-		for (size_t i = 0; i < o.count; ++i) {
-			r.pos[i] = ld3(&o.pos[3 * i]);
-			r.nrm[i] = ld3(&o.nrm[3 * i]);
-			r.info[i] = f4{o.inf[4 * i], o.inf[4 * i + 1], o.inf[4 * i + 2], o.inf[4 * i + 3]};
+		r.pos.resize(c);
+		r.nrm.resize(c);
+		r.info.resize(c);
+		r.loc.assign(out.loc.get() + f, out.loc.get() + f + c);
+		r.dist.resize(c);
+		for (size_t i = 0; i < c; ++i) {
+			const size_t k = f + i;
+			r.pos[i] = ld3(&out.pos[3 * k]);
+			r.nrm[i] = ld3(&out.nrm[3 * k]);
+			r.info[i] = f4{out.inf[4 * k], out.inf[4 * k + 1], out.inf[4 * k + 2], out.inf[4 * k + 3]};
 			r.dist[i] = sqrtf(r.info[i].w);
 		}
 	});

@@ -157,11 +157,13 @@ __global__ __launch_bounds__(kRasBlock) void k_raster_emit(const RasterTri* __re
 } // namespace
 
 void rasterize_gpu(const std::vector<RasterTri>& tris, const std::vector<uint32_t>& tri_atlas,
-                   const std::vector<RasterAtlasSize>& sizes, std::vector<RasterOut>& out, RasterTimes* times)
+                   const std::vector<RasterAtlasSize>& sizes, RasterOut& out, RasterTimes* times)
 {
 	hipEvent_t e0, e1;
 	const size_t na = sizes.size();
-	out.assign(na, RasterOut{});
+	out = RasterOut{};
+	out.first.assign(na, 0);
+	out.count.assign(na, 0);
 	if (na == 0)
 		return;
 	if (tri_atlas.size() != tris.size())
@@ -235,9 +237,10 @@ void rasterize_gpu(const std::vector<RasterTri>& tris, const std::vector<uint32_
 	for (size_t a = 0; a < na; ++a) {
 		const uint64_t b0 = first_block[a], b1 = a + 1 < na ? first_block[a + 1] : nblocks;
 		const uint64_t first = b0 < nblocks ? boff[b0] : total, end = b1 < nblocks ? boff[b1] : total;
-		out[a].first = first;
-		out[a].count = end - first;
+		out.first[a] = first;
+		out.count[a] = end - first;
 	}
+	out.total = total;
 	DevArr<float> d_pos(3 * total), d_nrm(3 * total), d_inf(4 * total);
 	DevArr<uint32_t> d_loc(total);
 	RAS_CHECK(hipMemcpyAsync(d_boff.p, boff.data(), 8 * nblocks, hipMemcpyHostToDevice, s));
@@ -246,18 +249,16 @@ void rasterize_gpu(const std::vector<RasterTri>& tris, const std::vector<uint32_
 	                                                     d_inf.p, d_loc.p);
 	RAS_CHECK(hipGetLastError());
 	RAS_CHECK(hipEventRecord(e1, s));
-	for (size_t a = 0; a < na; ++a) {
-		RasterOut& o = out[a];
-		o.pos.resize(3 * o.count);
-		o.nrm.resize(3 * o.count);
-		o.inf.resize(4 * o.count);
-		o.loc.resize(o.count);
-		if (!o.count)
-			continue;
-		RAS_CHECK(hipMemcpyAsync(o.pos.data(), d_pos.p + 3 * o.first, 12 * o.count, hipMemcpyDeviceToHost, s));
-		RAS_CHECK(hipMemcpyAsync(o.nrm.data(), d_nrm.p + 3 * o.first, 12 * o.count, hipMemcpyDeviceToHost, s));
-		RAS_CHECK(hipMemcpyAsync(o.inf.data(), d_inf.p + 4 * o.first, 16 * o.count, hipMemcpyDeviceToHost, s));
-		RAS_CHECK(hipMemcpyAsync(o.loc.data(), d_loc.p + o.first, 4 * o.count, hipMemcpyDeviceToHost, s));
+	// one copy per array (host buffers left uninitialised: every element is copied)
+	out.pos.reset(new float[3 * total + 1]);
+	out.nrm.reset(new float[3 * total + 1]);
+	out.inf.reset(new float[4 * total + 1]);
+	out.loc.reset(new uint32_t[total + 1]);
+	if (total) {
+		RAS_CHECK(hipMemcpyAsync(out.pos.get(), d_pos.p, 12 * total, hipMemcpyDeviceToHost, s));
+		RAS_CHECK(hipMemcpyAsync(out.nrm.get(), d_nrm.p, 12 * total, hipMemcpyDeviceToHost, s));
+		RAS_CHECK(hipMemcpyAsync(out.inf.get(), d_inf.p, 16 * total, hipMemcpyDeviceToHost, s));
+		RAS_CHECK(hipMemcpyAsync(out.loc.get(), d_loc.p, 4 * total, hipMemcpyDeviceToHost, s));
 	}
 	RAS_CHECK(hipStreamSynchronize(s));
 	if (times) {

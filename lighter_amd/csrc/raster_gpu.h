@@ -7,6 +7,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <memory>
 #include <vector>
 
 #include "raster.h"
@@ -17,12 +18,14 @@ struct RasterAtlasSize {
 	uint32_t w, h;
 };
 
-// One atlas's samples: position, normalised normal, (uv0, part, area) and
-// the texel index x + y w.
+// Every atlas's samples, atlas after atlas: position (3 floats), normalised
+// normal (3), (uv0, part, area) (4) and the texel index x + y w; atlas a's
+// samples are first[a] .. first[a] + count[a] - 1.
 struct RasterOut {
-	uint64_t first = 0, count = 0;
-	std::vector<float> pos, nrm, inf;
-	std::vector<uint32_t> loc;
+	std::vector<uint64_t> first, count;
+	std::unique_ptr<float[]> pos, nrm, inf;
+	std::unique_ptr<uint32_t[]> loc;
+	uint64_t total = 0;
 };
 
 struct RasterTimes {
@@ -34,6 +37,6 @@ struct RasterTimes {
 // the atlases one after another; tri_atlas: the atlas of each triangle.
 // Throws std::runtime_error on HIP errors and malformed input.
 void rasterize_gpu(const std::vector<RasterTri>& tris, const std::vector<uint32_t>& tri_atlas,
-                   const std::vector<RasterAtlasSize>& sizes, std::vector<RasterOut>& out, RasterTimes* times);
+                   const std::vector<RasterAtlasSize>& sizes, RasterOut& out, RasterTimes* times);
 
 } // namespace lgx

@@ -399,26 +399,15 @@ static void place_gpu(ltr_Scene* S, std::vector<RasterSamples>& raw)
 	if (total == 0)
 		return;
 	require_gpu();
-	std::vector<float> pos(3 * total), nrm(3 * total), dist(total);
-	size_t k = 0;
-	for (const RasterSamples& r : raw) {
-		for (size_t i = 0; i < r.pos.size(); ++i, ++k) {
-			st3(&pos[3 * k], r.pos[i]);
-			st3(&nrm[3 * k], r.nrm[i]);
-			dist[k] = r.dist[i];
-		}
-	}
+	std::vector<PlaceSamples> runs;
+	for (RasterSamples& r : raw)
+		runs.push_back(PlaceSamples{r.pos.data(), r.nrm.data(), r.dist.data(), r.pos.size()});
 	std::vector<PlaceInstance> pi;
 	for (Instance* mi : S->insts)
 		pi.push_back(PlaceInstance{&mi->tris, &mi->split, mi->shadow});
 	const ltr_Config& cfg = S->config;
 	float corr_min_dot = cosf(cfg.max_correct_angle / 180.0f * kPiF);
-	place_samples_gpu(pi, S->inst_tree, pos.data(), nrm.data(), dist.data(), total, cfg.max_correct_dist,
-	                  corr_min_dot, S->nthreads, &S->place_times);
-	k = 0;
-	for (RasterSamples& r : raw)
-		for (size_t i = 0; i < r.pos.size(); ++i, ++k)
-			r.pos[i] = ld3(&pos[3 * k]);
+	place_samples_gpu(pi, S->inst_tree, runs, cfg.max_correct_dist, corr_min_dot, S->nthreads, &S->place_times);
 }
 
 static void commit_samples(ltr_Scene* S, Instance* mi, RasterSamples& r)

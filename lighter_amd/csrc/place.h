@@ -29,14 +29,22 @@ struct PlaceTimes {
 	size_t samples = 0;
 };
 
-// pos: 3 floats per sample, moved in place; nrm: 3 floats per sample
-// (normalised); dist: per sample, the square root of the texel's world area.
-// max_corr and corr_min_dot as the config gives them (corr_min_dot is the
-// cosine of max_correct_angle).  The trees are flattened on nthreads host
+// One run of samples (one instance's, say): pos moved in place, nrm
+// (normalised) and dist (the square root of the texel's world area), n each.
+struct PlaceSamples {
+	f3* pos;
+	const f3* nrm;
+	const float* dist;
+	size_t n;
+};
+
+// Places every run's samples in one launch.  max_corr and corr_min_dot as
+// the config gives them (corr_min_dot is the cosine of max_correct_angle).
+// The trees are flattened and the runs packed and unpacked on nthreads host
 // threads.  Throws std::runtime_error on HIP errors or when a tree is deeper
 // than the kernel's stacks.
-void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, float* pos,
-                       const float* nrm, const float* dist, size_t n, float max_corr, float corr_min_dot,
-                       int nthreads, PlaceTimes* times);
+void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree,
+                       const std::vector<PlaceSamples>& runs, float max_corr, float corr_min_dot, int nthreads,
+                       PlaceTimes* times);
 
 } // namespace lgx

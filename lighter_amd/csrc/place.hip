@@ -468,11 +468,33 @@ PlaceView view_of(const char* base, const PlaceLayout& L, int32_t ninst)
 
 } // namespace
 
-void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree, float* pos,
-                       const float* nrm, const float* dist, size_t n, float max_corr, float corr_min_dot,
-                       int nthreads, PlaceTimes* times)
+// f(k) for k in [0, n) on nthreads threads
+template <class F>
+static void run_parallel(int nthreads, size_t n, F&& f)
+{
+	const int nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, nthreads), n));
+	std::atomic<size_t> next{0};
+	auto work = [&]() {
+		for (size_t k; (k = next.fetch_add(1)) < n;)
+			f(k);
+	};
+	std::vector<std::thread> th;
+	for (int t = 1; t < nt; ++t)
+		th.emplace_back(work);
+	work();
+	for (auto& t : th)
+		t.join();
+}
+
+void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& inst_tree,
+                       const std::vector<PlaceSamples>& runs, float max_corr, float corr_min_dot, int nthreads,
+                       PlaceTimes* times)
 {
 	PlaceTimes tm;
+	std::vector<size_t> first(runs.size() + 1, 0);
+	for (size_t r = 0; r < runs.size(); ++r)
+		first[r + 1] = first[r] + runs[r].n;
+	const size_t n = first.back();
 	tm.samples = n;
 	if (n == 0) {
 		if (times)
@@ -486,9 +508,18 @@ void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& i
 	const PlaceLayout L = plan(insts, inst_tree, n, inst);
 	PinnedBlock host(L.bytes);
 	fill(insts, inst_tree, inst, L, host.p, nthreads);
-	memcpy(host.p + L.pos, pos, 3 * n * sizeof(float));
-	memcpy(host.p + L.nrm, nrm, 3 * n * sizeof(float));
-	memcpy(host.p + L.dist, dist, n * sizeof(float));
+	float* hpos = (float*)(host.p + L.pos);
+	float* hnrm = (float*)(host.p + L.nrm);
+	float* hdist = (float*)(host.p + L.dist);
+	run_parallel(nthreads, runs.size(), [&](size_t r) {
+		const PlaceSamples& R = runs[r];
+		for (size_t i = 0; i < R.n; ++i) {
+			const size_t k = first[r] + i;
+			st3(hpos + 3 * k, R.pos[i]);
+			st3(hnrm + 3 * k, R.nrm[i]);
+			hdist[k] = R.dist[i];
+		}
+	});
 	tm.flatten_ms = ms_since(t0);
 
 	t0 = Clock::now();
@@ -520,8 +551,12 @@ void place_samples_gpu(const std::vector<PlaceInstance>& insts, const BoxTree& i
 	PLACE_CHECK(se);
 	tm.kernel_ms = kms;
 	t0 = Clock::now();
-	PLACE_CHECK(hipMemcpy(host.p + L.pos, d_pos, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
-	memcpy(pos, host.p + L.pos, 3 * n * sizeof(float));
+	PLACE_CHECK(hipMemcpy(hpos, d_pos, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+	run_parallel(nthreads, runs.size(), [&](size_t r) {
+		const PlaceSamples& R = runs[r];
+		for (size_t i = 0; i < R.n; ++i)
+			R.pos[i] = ld3(hpos + 3 * (first[r] + i));
+	});
 	tm.download_ms = ms_since(t0);
 	if (times)
 		*times = tm;

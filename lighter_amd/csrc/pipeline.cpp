@@ -21,6 +21,7 @@
 #include <atomic>
 #include <chrono>
 #include <functional>
+#include <memory>
 #include <stdexcept>
 #include <thread>
 
@@ -308,21 +309,22 @@ static void raster_samples_gpu(ltr_Scene* S, std::vector<RasterSamples>& raw)
 	parallel_for(S->nthreads, ids.size(), [&](size_t q) { cnt[q] = count_raster_tris(S->insts[ids[q]]); });
 	for (size_t q = 0; q < ids.size(); ++q)
 		first[q + 1] = first[q] + cnt[q];
-	std::vector<RasterTri> tris(first.back());
-	std::vector<uint32_t> tri_atlas(first.back());
+	// left uninitialised: every element is written below
+	std::unique_ptr<RasterTri[]> tris(new RasterTri[first.back() + 1]);
+	std::unique_ptr<uint32_t[]> tri_atlas(new uint32_t[first.back() + 1]);
 	std::vector<RasterAtlasSize> sizes;
 	for (size_t q = 0; q < ids.size(); ++q)
 		sizes.push_back(RasterAtlasSize{S->insts[ids[q]]->lm_w, S->insts[ids[q]]->lm_h});
 	std::atomic<bool> miscount{false};
 	parallel_for(S->nthreads, ids.size(), [&](size_t q) {
-		RasterTri* dst = tris.data() + first[q];
+		RasterTri* dst = tris.get() + first[q];
 		const size_t n = raster_tris_each(S->insts[ids[q]], [&](size_t k, const RasterTri& t) {
 			if (k < cnt[q])
 				dst[k] = t;
 		});
 		if (n != cnt[q])
 			miscount.store(true);
-		std::fill(tri_atlas.begin() + first[q], tri_atlas.begin() + first[q + 1], (uint32_t)q);
+		std::fill(tri_atlas.get() + first[q], tri_atlas.get() + first[q + 1], (uint32_t)q);
 	});
 	if (miscount.load())
 		throw std::runtime_error("rasterisation: an instance's triangle count changed between two passes");
@@ -330,7 +332,7 @@ static void raster_samples_gpu(ltr_Scene* S, std::vector<RasterSamples>& raw)
 		return;
 	require_gpu();
 	RasterOut out;
-	rasterize_gpu(tris, tri_atlas, sizes, out, &S->raster_times);
+	rasterize_gpu(tris.get(), tri_atlas.get(), first.back(), sizes, out, &S->raster_times);
 	parallel_for(S->nthreads, ids.size(), [&](size_t q) {
 		const uint64_t f = out.first[q], c = out.count[q];
 		RasterSamples& r = raw[ids[q]];

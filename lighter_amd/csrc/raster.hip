@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kRasBlock) void k_raster_emit(const RasterTri* __re
 
 } // namespace
 
-void rasterize_gpu(const std::vector<RasterTri>& tris, const std::vector<uint32_t>& tri_atlas,
+void rasterize_gpu(const RasterTri* tris, const uint32_t* tri_atlas, size_t ntris,
                    const std::vector<RasterAtlasSize>& sizes, RasterOut& out, RasterTimes* times)
 {
 	hipEvent_t e0, e1;
@@ -166,9 +166,7 @@ void rasterize_gpu(const std::vector<RasterTri>& tris, const std::vector<uint32_
 	out.count.assign(na, 0);
 	if (na == 0)
 		return;
-	if (tri_atlas.size() != tris.size())
-		throw std::runtime_error("rasterize_gpu: one atlas index per triangle");
-	if (tris.size() >= 0x7fffffffu)
+	if (ntris >= 0x7fffffffu)
 		throw std::runtime_error("rasterize_gpu: too many triangles");
 	// atlases back to back, each from a block boundary
 	std::vector<RasAtlas> atlas(na);
@@ -179,15 +177,15 @@ void rasterize_gpu(const std::vector<RasterTri>& tris, const std::vector<uint32_
 		first_block[a] = (uint32_t)(px / kRasBlock);
 		px += ((uint64_t)sizes[a].w * sizes[a].h + kRasBlock - 1) / kRasBlock * kRasBlock;
 	}
-	for (uint32_t t : tri_atlas)
-		if (t >= na)
+	for (size_t t = 0; t < ntris; ++t)
+		if (tri_atlas[t] >= na)
 			throw std::runtime_error("rasterize_gpu: triangle of an unknown atlas");
 	const uint64_t nblocks = px / kRasBlock;
 	if (nblocks == 0)
 		return;
 	if (nblocks >= 0xffffffffull)
 		throw std::runtime_error("rasterize_gpu: atlases too large");
-	const uint32_t ntri = (uint32_t)tris.size();
+	const uint32_t ntri = (uint32_t)ntris;
 	hipStream_t s;
 	RAS_CHECK(hipStreamCreate(&s));
 	struct StreamGuard {
@@ -204,13 +202,13 @@ void rasterize_gpu(const std::vector<RasterTri>& tris, const std::vector<uint32_
 			(void)hipEventDestroy(b);
 		}
 	} eg{e0, e1};
-	DevArr<RasterTri> d_tris(tris.size());
-	DevArr<uint32_t> d_tatlas(tris.size()), d_first(na), d_owner(px), d_count(nblocks);
+	DevArr<RasterTri> d_tris(ntris);
+	DevArr<uint32_t> d_tatlas(ntris), d_first(na), d_owner(px), d_count(nblocks);
 	DevArr<RasAtlas> d_atlas(na);
 	DevArr<uint64_t> d_boff(nblocks);
 	if (ntri) {
-		RAS_CHECK(hipMemcpyAsync(d_tris.p, tris.data(), sizeof(RasterTri) * ntri, hipMemcpyHostToDevice, s));
-		RAS_CHECK(hipMemcpyAsync(d_tatlas.p, tri_atlas.data(), 4 * (size_t)ntri, hipMemcpyHostToDevice, s));
+		RAS_CHECK(hipMemcpyAsync(d_tris.p, tris, sizeof(RasterTri) * ntri, hipMemcpyHostToDevice, s));
+		RAS_CHECK(hipMemcpyAsync(d_tatlas.p, tri_atlas, 4 * (size_t)ntri, hipMemcpyHostToDevice, s));
 	}
 	RAS_CHECK(hipMemcpyAsync(d_atlas.p, atlas.data(), sizeof(RasAtlas) * na, hipMemcpyHostToDevice, s));
 	RAS_CHECK(hipMemcpyAsync(d_first.p, first_block.data(), 4 * na, hipMemcpyHostToDevice, s));

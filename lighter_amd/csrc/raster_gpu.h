@@ -36,7 +36,7 @@ struct RasterTimes {
 // tris: every atlas's triangles in drawing order (part and index order),
 // the atlases one after another; tri_atlas: the atlas of each triangle.
 // Throws std::runtime_error on HIP errors and malformed input.
-void rasterize_gpu(const std::vector<RasterTri>& tris, const std::vector<uint32_t>& tri_atlas,
+void rasterize_gpu(const RasterTri* tris, const uint32_t* tri_atlas, size_t ntris,
                    const std::vector<RasterAtlasSize>& sizes, RasterOut& out, RasterTimes* times);
 
 } // namespace lgx

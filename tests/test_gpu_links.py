@@ -130,3 +130,47 @@ def test_bake_with_bounces_above_the_old_patch_limit(lib):
         assert gained > 100
     finally:
         b.destroy()
+
+
+def test_c1_links_beyond_32_bit_offsets(lib):
+    """C1's 2,989,242 texels have about 2.2 G links, 4.5 G CSR entries: the
+    row offsets go 64-bit and the per-row sort runs in pieces of at most
+    2^31 entries.  The search is repeatable, one bounce only adds light, and
+    the entry points that hand out 32-bit offsets (the reference's link
+    list, the solver forms) refuse with an error instead of wrapping."""
+    from lighter_amd import workloads
+    rad = []
+    b, pos, nrm, tris, _, _ = workloads.direct_workload(lib, scenes.terrain_scene(), 16, rad)
+    rad = np.concatenate(rad)
+    pn = scenes.normalize_rows_f32(nrm)
+    N = len(pos)
+    dev = Device(lib, tris)
+    try:
+        counts = []
+        for _ in range(2):
+            nl = C.c_size_t(0)
+            _lib.check(lib.lgx_RadLinks(dev.h, P(pos), P(pn), N, C.byref(nl)), lib)
+            counts.append(nl.value)
+        print(f"C1: {N} patches, {counts[0]} links")
+        assert counts[0] == counts[1] and 2 * counts[0] >= 2 ** 32
+        diff = np.full((N, 3), 0.6, np.float32)
+        emit = np.ascontiguousarray(np.random.default_rng(3).uniform(0, 1, (N, 3)).astype(np.float32))
+        area = np.ascontiguousarray(rad[:, 3])
+        total = np.zeros((N, 3), np.float32)
+        _lib.check(lib.lgx_RadBounce(dev.h, P(diff), P(emit), P(area), 1, P(total)), lib)
+        assert np.all(np.isfinite(total)) and np.all(total >= emit) and np.mean(total > emit) > 0.5
+        rowmap = np.zeros(2 * N, np.uint32)
+        one = np.zeros(1, np.uint32)
+        assert lib.lgx_RadReadLinks(dev.h, rowmap.ctypes.data_as(u32p), one.ctypes.data_as(u32p),
+                                    P(np.zeros(1, np.float32))) == 0
+        assert "2^32" in _lib.last_error(lib)
+        E = np.ones((N, 3))
+        out = np.zeros((N, 3))
+        t = (C.c_double * 4)()
+        dp = C.POINTER(C.c_double)
+        assert lib.lgx_RadSolve(dev.h, E.ctypes.data_as(dp), 3, P(diff), P(area), 3, 1, 0,
+                                out.ctypes.data_as(dp), t) == 0
+        assert "2^32" in _lib.last_error(lib)
+    finally:
+        dev.close()
+        b.destroy()

@@ -924,6 +924,7 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
     out["adjoint"] = c4_adjoint(lib, leg, rad_dev, s0, N, Wp, diff, area, bounces, rng)
     lib.lgx_DeviceDestroy(rad_dev)
     out["radiosity_whole_scene"] = whole_scene_radiosity(lib, leg, tris, rng)
+    out["bake_end_to_end"] = whole_bake(lib, leg.spec, nth)
     if with_cpu:
         # the reference's AO ray loop on a sample of texels, and the device's
         # AO over a lightmap of ones (multiplier 1, no falloff, effect 0,
@@ -987,6 +988,27 @@ def whole_scene_radiosity(lib, leg, tris, rng):
         return res
     finally:
         lib.lgx_DeviceDestroy(d)
+
+
+def whole_bake(lib, spec, nth):
+    """The C4 scene baked through the drop-in API (ltr_Start, polled with
+    ltr_GetStatus): preparation, direct light, one radiosity bounce over
+    all texels, 64 AO rays of 2 m per texel and the export; wall seconds per
+    stage as the status reports them."""
+    b = scenes.BuiltScene(lib, spec, dict(max_num_threads=nth, bounce_count=1, ao_num_samples=64,
+                                          ao_distance=2.0))
+    try:
+        t0 = time.perf_counter()
+        stages = b.bake(poll_ms=2)
+        wall = time.perf_counter() - t0
+        outs = b.outputs()
+        res = dict(config="bounce_count 1, ao_num_samples 64, ao_distance 2 m", wall_s=round(wall, 2),
+                   stages_s={k: round(v, 3) for k, v in stages.items()}, outputs=len(outs),
+                   outputs_finite=bool(all(np.isfinite(o[3]).all() for o in outs)))
+        log(f"[bench] C4 bake through ltr_Start (one bounce, AO 64 rays): {wall:.2f} s; {res['stages_s']}")
+        return res
+    finally:
+        b.destroy()
 
 
 def c4_adjoint(lib, leg, rad_dev, s0, N, Wp, diff, area, bounces, rng):

@@ -636,19 +636,31 @@ void stage_direct(ltr_Scene* S)
 	for (size_t l0 = 0; l0 < S->lights.size(); l0 += 64) {
 		size_t l1 = std::min(S->lights.size(), l0 + 64);
 		std::vector<DevLight> dl;
-		std::vector<uint32_t> items, off;
+		std::vector<uint32_t> off;
 		std::vector<std::vector<int32_t>> ids_of;
 		off.push_back(0);
 		for (size_t l = l0; l < l1; ++l) {
 			const Light& L = S->lights[l];
 			dl.push_back(to_dev_light(L));
 			std::vector<int32_t> ids = light_instances(S, L);
+			size_t n = off.back();
 			for (int32_t id : ids)
-				for (size_t s = S->inst_off[id]; s < S->inst_off[id + 1]; ++s)
-					items.push_back((uint32_t)((l - l0) << 26) | (uint32_t)s);
-			off.push_back((uint32_t)items.size());
+				n += S->inst_off[id + 1] - S->inst_off[id];
+			if (n >= ((size_t)1 << 32))
+				throw std::runtime_error("direct light: more than 2^32 work items in one batch of lights");
+			off.push_back((uint32_t)n);
 			ids_of.push_back(std::move(ids));
 		}
+		// (light << 26) | sample for every sample of every instance the
+		// light reaches, in light order and instance order; one light per
+		// task
+		std::vector<uint32_t> items(off.back());
+		parallel_for(S->nthreads, l1 - l0, [&](size_t q) {
+			uint32_t* dst = items.data() + off[q];
+			for (int32_t id : ids_of[q])
+				for (size_t s = S->inst_off[id]; s < S->inst_off[id + 1]; ++s)
+					*dst++ = (uint32_t)(q << 26) | (uint32_t)s;
+		});
 		// progress and abort while the march kernel runs
 		const float n_lights = (float)S->lights.size();
 		const bool done = S->dev->direct_light(dl, items, off, want_vis, [&](float f) {

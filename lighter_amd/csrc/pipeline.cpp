@@ -707,8 +707,12 @@ void stage_radiosity(ltr_Scene* S)
 	download_lightmaps(S);
 	std::vector<float> pos(3 * N), nrm(3 * N), diff(3 * N), emit(3 * N), area(N);
 	const ltr_Config& cfg = S->config;
-	size_t k = 0;
-	for (Instance* mi : S->insts) {
+	// each instance's patches at its samples' offsets; in parallel unless
+	// the caller's sample_fn is called, which the reference calls from one
+	// thread (its thread safety is not promised)
+	auto patches_of = [&](size_t m) {
+		Instance* mi = S->insts[m];
+		size_t k = S->inst_off[m];
 		for (size_t i = 0; i < mi->lightmap.size(); ++i, ++k) {
 			f3 P = mi->s_pos[i];
 			f3 Nn = normalized(mi->s_nrm[i]);
@@ -746,7 +750,8 @@ void stage_radiosity(ltr_Scene* S)
 			st3(&emit[3 * k], e);
 			area[k] = a;
 		}
-	}
+	};
+	parallel_for(cfg.sample_fn ? 1 : S->nthreads, S->insts.size(), patches_of);
 	check_abort(S);
 	DeviceScene& D = *S->dev;
 	D.rad_set_patches(pos.data(), nrm.data(), N);
@@ -767,13 +772,12 @@ void stage_radiosity(ltr_Scene* S)
 	set_stage(S, "committing radiosity");
 	std::vector<float> total(3 * N);
 	D.rad_get_total(total.data());
-	k = 0;
-	for (size_t m = 0; m < S->insts.size(); ++m) {
+	parallel_for(S->nthreads, S->insts.size(), [&](size_t m) {
 		Instance* mi = S->insts[m];
-		for (size_t i = 0; i < mi->lightmap.size(); ++i, ++k)
+		for (size_t i = 0, k = S->inst_off[m]; i < mi->lightmap.size(); ++i, ++k)
 			mi->lightmap[i] = ld3(&total[3 * k]);
-		S->completion.store((m + 1) / (float)S->insts.size());
-	}
+	});
+	S->completion.store(1.0f);
 	upload_lightmaps(S);
 }
 

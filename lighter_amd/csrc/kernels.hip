@@ -2794,20 +2794,23 @@ void* DeviceScene::link_candidates_grid(uint64_t& ncand, uint64_t* coff)
 	double h = std::sqrt(r2) / 2 * 1.0001;
 	LinkGrid G;
 	G.reach = 2;
-	uint64_t cells = 0;
-	for (;;) {
+	double na[3], cells = 0;
+	for (;;) { // sized in double: a large extent over a short distance must not overflow the counts
 		cells = 1;
 		for (int a = 0; a < 3; ++a) {
-			G.lo[a] = links_lo_[a];
-			G.n[a] = (uint32_t)std::floor(((double)links_hi_[a] - links_lo_[a]) / h) + 1;
-			cells *= G.n[a];
+			na[a] = std::floor(((double)links_hi_[a] - links_lo_[a]) / h) + 1;
+			cells *= na[a];
 		}
-		if (cells <= (1u << 24))
+		if (cells <= (double)(1u << 24))
 			break;
 		h *= 1.26; // wider cells keep every linking pair within reach
 	}
+	for (int a = 0; a < 3; ++a) {
+		G.lo[a] = links_lo_[a];
+		G.n[a] = (uint32_t)na[a];
+	}
 	G.inv = (float)(1.0 / h);
-	const uint32_t nc = (uint32_t)cells;
+	const uint32_t nc = G.n[0] * G.n[1] * G.n[2];
 	DevArray<uint32_t> cell(N), ccount(nc + 1), order(N), counts((uint64_t)N + 1);
 	DevArray<uint64_t> cstart(nc + 1);
 	HIPCHECK(hipMemsetAsync(ccount.p, 0, 4 * ((size_t)nc + 1), s));

@@ -1353,7 +1353,28 @@ __global__ __launch_bounds__(kBlock) void k_pairs_grid(const float* __restrict__
 			for (int x = max(cx - q, 0); x <= min(cx + q, (int)G.n[0] - 1); ++x) {
 				const uint32_t cc = ((uint32_t)z * G.n[1] + (uint32_t)y) * G.n[0] + (uint32_t)x;
 				const uint64_t e = cstart[cc + 1];
-				for (uint64_t q = cstart[cc]; q < e; ++q) {
+				uint64_t q = cstart[cc];
+				// four patches per round: their loads are in flight together
+				for (; q + 4 <= e; q += 4) {
+					uint32_t j[4];
+					f3 bp[4], bn[4];
+#pragma unroll
+					for (int u = 0; u < 4; ++u)
+						j[u] = order[q + u];
+#pragma unroll
+					for (int u = 0; u < 4; ++u) {
+						bp[u] = ld3(pos + 3 * j[u]);
+						bn[u] = ld3(nrm + 3 * j[u]);
+					}
+#pragma unroll
+					for (int u = 0; u < 4; ++u)
+						if (j[u] > i && !(pair_factor(ap, an, bp[u], bn[u]) < 0.0f)) {
+							if (WRITE)
+								cand[base + n] = make_uint2(i, j[u]);
+							n++;
+						}
+				}
+				for (; q < e; ++q) {
 					const uint32_t j = order[q];
 					if (j <= i)
 						continue;

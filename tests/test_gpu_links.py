@@ -8,9 +8,9 @@ per-pair arithmetic of the all-pairs loop, and assembles the symmetric CSR
 by a per-row sort.  lgx_SetLinkSearch(dev, 1) switches to the all-pairs
 enumeration, mode 2 forces the grid, and the default takes all pairs where
 the grid would not prune.  Every mode must give the same links and factors
-bit for bit, also above the 262,144 patches the earlier pair bitmask
-limited the search to; patches with non-finite positions or normals take
-the all-pairs path, whose links must then match the oracle's loop.
+bit for bit, also above 262,144 patches (where a pair bitmask would take
+N^2 / 8 = 8.6 GB); patches with non-finite positions or normals take the
+all-pairs path, whose links must then match the oracle's loop.
 """
 import ctypes as C
 
@@ -40,7 +40,7 @@ SCENES = {
     "room_96": lambda: scenes.room_scene(lm=96, n_boxes=20, bounce_count=1),
     "city_block": lambda: scenes.city_scene(extent=200.0, grid=32, amplitude=2.0, n_boxes=144, tiles=4,
                                             lm_total=160, n_lights=2),
-    # 316,375 patches: above the 262,144 the pair bitmask allowed
+    # 316,375 patches: above 262,144
     "city_block_316k": lambda: scenes.city_scene(extent=200.0, grid=32, amplitude=2.0, n_boxes=144, tiles=4,
                                                  lm_total=640, n_lights=2),
 }
@@ -102,9 +102,9 @@ def test_non_finite_normals_take_the_all_pairs_loop(lib, orc):
 
 
 def test_bake_with_bounces_above_the_old_patch_limit(lib):
-    """ltr_Start with bounce_count = 1 on 316,375 patches (the pair bitmask
-    refused this bake): it completes, the radiosity stage finds links, and
-    the bounce only adds light to the direct-light lightmaps."""
+    """The staged bake with bounce_count = 1 on 316,375 patches (above
+    262,144): it completes, the radiosity stage finds links, and the bounce
+    only adds light to the direct-light lightmaps."""
     spec = scenes.city_scene(extent=200.0, grid=32, amplitude=2.0, n_boxes=144, tiles=4, lm_total=640,
                              n_lights=2)
     b = prepared(lib, spec, bounce_count=1)

@@ -138,8 +138,11 @@ __device__ __forceinline__ RayTri load_rtri(const RayTri* t, uint32_t i)
 	return r;
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // True when the triangle cannot lower the distance bound `best`: the box of
-// its three vertices (DistTri quads 4-6) lies farther than best + margin.
+// its three vertices (DistTri quads 4-6, qa..qc) lies farther than best +
+// margin.
 // The distance the reference's formula returns is the distance to a point of
 // the triangle (a vertex, a point of an edge line inside the edge's span, or
 // the projection into the face), up to rounding in its region tests of a few
@@ -147,10 +150,8 @@ __device__ __forceinline__ RayTri load_rtri(const RayTri* t, uint32_t i)
 // the margin (1e-5 of the coordinates' magnitude, hundreds of ulps) covers
 // it, and a skipped triangle could only have returned a value >= best, which
 // leaves the minimum unchanged.
-__device__ __forceinline__ bool tri_box_beyond(f3 p, const DistTri* t, float best)
+__device__ __forceinline__ bool tri_box_beyond_q(f3 p, v4f qa, v4f qb, v4f qc, float best)
 {
-	const float4* q = reinterpret_cast<const float4*>(t);
-	const float4 qa = q[4], qb = q[5], qc = q[6];
 	// vertices (qa.w, qb.x, qb.y), (qb.z, qb.w, qc.x), (qc.y, qc.z, qc.w)
 	float lx = fminf(fminf(qa.w, qb.z), qc.y), hx = fmaxf(fmaxf(qa.w, qb.z), qc.y);
 	float ly = fminf(fminf(qb.x, qb.w), qc.z), hy = fmaxf(fmaxf(qb.x, qb.w), qc.z);
@@ -274,11 +275,32 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 		} else {
 			uint32_t f = leaf_first(ref), c = leaf_count(ref);
 			for (uint32_t i = 0; i < c; ++i) {
-				if (tri_box_beyond(p, V.dtri + f + i, best)) {
+				// the plane, tangents and vertices (quads 0-6) in one round
+				// trip with the box pre-test's vertices; the edge normals
+				// (quads 7-9) only where the vertex regions do not decide.
+				// The empty asm takes the seven quads as register operands
+				// here, so the compiler issues their loads together instead
+				// of moving quads 0-3 behind the pre-test's branch (a second
+				// round trip to memory for almost every triangle).
+				const v4f* q = reinterpret_cast<const v4f*>(V.dtri + f + i);
+				const v4f q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6];
+				asm volatile("" ::"v"(q0), "v"(q1), "v"(q2), "v"(q3), "v"(q4), "v"(q5), "v"(q6));
+				if (tri_box_beyond_q(p, q4, q5, q6, best)) {
 					ncull++;
 					continue;
 				}
-				DistTri T = load_dtri(V.dtri, f + i);
+				DistTri T;
+				v4f* o = reinterpret_cast<v4f*>(&T);
+				o[0] = q0;
+				o[1] = q1;
+				o[2] = q2;
+				o[3] = q3;
+				o[4] = q4;
+				o[5] = q5;
+				o[6] = q6;
+				o[7] = q[7];
+				o[8] = q[8];
+				o[9] = q[9];
 				float d = dist_tri_pre(p, T);
 				nt++;
 				if (d < best)

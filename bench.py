@@ -352,7 +352,7 @@ def links_bench(lib, nth, lm=64):
     br.destroy()
     N = len(geo)
     pairs = N * (N - 1) // 2
-    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    dev = device_create(lib, tris)
     pos = np.ascontiguousarray(geo[:, :3])
     nrm = np.ascontiguousarray(geo[:, 3:])
     nl = C.c_size_t(0)
@@ -466,7 +466,7 @@ def radiosity_bench(lib, nth, n_patches=32768, R=192, bounces=8, setups=64, with
     (setups x RGB) built from the lights' direct contributions."""
     spec, b, pos, nrm, rad, tris, N, _ = room_patches(lib, nth, n_patches, 200, bounce_count=bounces,
                                                       ao_distance=2.0, ao_samples=64, name="C2")
-    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    dev = device_create(lib, tris)
     # direct light of each light on the patches
     _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), N, None), lib)
     lights = make_lights(spec)
@@ -562,7 +562,7 @@ def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
     kw = dict(seed=scenes.SEED_BASE + 3, n_boxes=31, wall_tess=40, n_point=64, with_sun=False, bounce_count=bounces,
               name="C3")
     spec, b, pos, nrm, rad, tris, N, lm = room_patches(lib, nth, n_patches, 256, **kw)
-    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    dev = device_create(lib, tris)
     _lib.check(lib.lgx_SetSamples(dev, P(pos), P(nrm), N, None), lib)
     lights = make_lights(spec)
     L = len(lights)
@@ -657,7 +657,7 @@ def adjoint_bench(lib, nth, n_patches=65536, bounces=4, ckpt_lm=1024):
         spec2 = scenes.room_scene(lm=ckpt_lm, **kw)
         b2, pos2, nrm2, tris2, items2, off2 = prepare_workload(lib, spec2, nth)
         S = len(pos2)
-        dev2 = lib.lgx_DeviceCreate(P(tris2), len(tris2))
+        dev2 = device_create(lib, tris2)
         _lib.check(lib.lgx_SetSamples(dev2, P(pos2), P(nrm2), S, None), lib)
         lights2 = make_lights(spec2)
         _lib.check(lib.lgx_DirectLight(dev2, lights2, len(lights2), items2.ctypes.data_as(u32p), len(items2),
@@ -703,7 +703,7 @@ class DirectLeg:
         self.S = len(self.pos)
         log(f"[bench] {spec.name} host stages: {self.S} samples, {len(self.items)} work items ({self.host_s:.1f} s)")
         t0 = time.perf_counter()
-        self.dev = lib.lgx_DeviceCreate(P(self.tris), len(self.tris))
+        self.dev = device_create(lib, self.tris)
         if not self.dev:
             raise RuntimeError(_lib.last_error(lib))
         self.device_create_s = time.perf_counter() - t0
@@ -862,7 +862,7 @@ def c4_extras(lib, leg, nth, cpu_seconds, with_cpu, n_patches=65536, bounces=4):
     s0 = int(starts[np.argmax([lit[a:a + N].sum() for a in starts])])
     pn = scenes.normalize_rows_f32(leg.nrm[s0:s0 + N])
     ppos = np.ascontiguousarray(leg.pos[s0:s0 + N])
-    rad_dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    rad_dev = device_create(lib, tris)
     _lib.check(lib.lgx_SetSamples(rad_dev, P(ppos), P(np.ascontiguousarray(leg.nrm[s0:s0 + N])), N, None), lib)
     nl = C.c_size_t(0)
     lib.lgx_SetCounting(rad_dev, 1)  # keeps the candidate pairs for the reference-shaped pass
@@ -964,7 +964,7 @@ def whole_scene_radiosity(lib, leg, tris, rng):
     CSR with 64-bit row offsets) and one bounce over them for one RGB setup
     (diffuse 0.6, world area per texel, seeded emission)."""
     S = len(leg.pos)
-    d = lib.lgx_DeviceCreate(P(tris), len(tris))
+    d = device_create(lib, tris)
     try:
         pn = scenes.normalize_rows_f32(leg.nrm)
         nl = C.c_size_t(0)
@@ -1096,6 +1096,17 @@ def measured_traffic(kernel, workload):
     return None
 
 
+ENTRY_GRID = True  # --no-entry-grid: every traversal starts at the root (for comparison)
+
+
+def device_create(lib, tris):
+    """lgx_DeviceCreate on a float32 (n, 9) triangle array."""
+    dev = lib.lgx_DeviceCreate(P(tris), len(tris))
+    if dev and not ENTRY_GRID:
+        _lib.check(lib.lgx_SetEntryGrid(dev, 0, None), lib)
+    return dev
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1103,6 +1114,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--headline-only", action="store_true", help="only the C4 direct-light line (no extra legs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-entry-grid", action="store_true",
+                    help="start every traversal at the root instead of its entry-grid cell (same results)")
     ap.add_argument("--no-c4-extras", action="store_true", help="skip C4's AO, radiosity, gradient, export legs")
     ap.add_argument("--no-c1", action="store_true", help="skip the C1 terrain leg")
     ap.add_argument("--no-radiosity", action="store_true", help="skip the radiosity (C2) leg")
@@ -1125,6 +1138,8 @@ def main():
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path "
               "is a design only and is not run.", file=sys.stderr)
         sys.exit(2)
+    global ENTRY_GRID
+    ENTRY_GRID = not args.no_entry_grid
     if args.headline_only:
         args.no_c4_extras = args.no_c1 = args.no_radiosity = args.no_adjoint = True
     with_cpu = not args.no_cpu_baseline

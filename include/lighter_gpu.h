@@ -128,6 +128,15 @@ LTRAPI lgx_Device* lgx_DeviceCreate(const float* tris9, size_t ntri);
 LTRAPI void lgx_DeviceDestroy(lgx_Device* dev);
 /* out: triangles, BVH nodes, BVH depth. */
 LTRAPI int lgx_DeviceInfo(lgx_Device* dev, int64_t out[3]);
+/* The entry grid (no counterpart in the reference, whose queries start at
+ * the root of each BBT, lighter_math.cpp:653-781): a uniform grid over the
+ * scene whose cells each hold a node listing at most four subtrees that
+ * contain every triangle within 2 m of the cell; distance queries (radius
+ * <= 2) and segments of at most 2 m start at their cell's node.  Results
+ * are the same with it on (the default) or off (on = 0: every query starts
+ * at the root).  cells (may be null) receives the grid's cell count, 0
+ * when the scene has none (no triangles, or a non-finite scene box). */
+LTRAPI int lgx_SetEntryGrid(lgx_Device* dev, int on, int64_t* cells);
 
 /* Scene queries, one result per query:
  *  distance: min(2, nearest shadow-triangle distance)   (lighter.cpp:183-188)

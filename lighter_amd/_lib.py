@@ -52,6 +52,7 @@ LGX_SIGNATURES = {
     "lgx_DeviceCreate": (vp, [fp, C.c_size_t]),
     "lgx_DeviceDestroy": (None, [vp]),
     "lgx_DeviceInfo": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+    "lgx_SetEntryGrid": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64)]),
     "lgx_QueryDistance": (C.c_int, [vp, fp, C.c_size_t, fp]),
     "lgx_QueryShadow": (C.c_int, [vp, fp, fp, fp, C.c_size_t, fp, u32p]),
     "lgx_QueryClosest": (C.c_int, [vp, fp, fp, C.c_size_t, fp]),

@@ -575,6 +575,15 @@ int lgx_DeviceInfo(lgx_Device* d, int64_t out[3])
 	return 1;
 }
 
+int lgx_SetEntryGrid(lgx_Device* d, int on, int64_t* cells)
+{
+	return guarded([&] {
+		d->dev.set_entry_grid(on != 0);
+		if (cells)
+			*cells = (int64_t)d->dev.grid_cells();
+	});
+}
+
 int lgx_QueryDistance(lgx_Device* d, const float* p, size_t n, float* out)
 {
 	return guarded([&] { d->dev.query_distance(p, n, out); });

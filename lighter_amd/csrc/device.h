@@ -14,6 +14,14 @@ namespace lgx {
 
 // Light as stored after ltr_LightAdd (direction normalised), as the kernels
 // read it.
+// Entry grid of the traversals (kernels.hip): cubic cells of size g from
+// lo, n cells per axis; pad grows a cell to the region its entry node covers.
+struct GridSpec {
+	float lo[3], g;
+	int n[3];
+	float pad;
+};
+
 struct DevLight {
 	int32_t type;
 	float pos[3];
@@ -75,6 +83,10 @@ public:
 	int march_stack_depth() const;
 	size_t march_lds_bytes() const;
 	size_t bvh_nodes() const { return nnodes_; }
+	// on = false: every query starts at the root (the entry grid's answers
+	// are the same; this is for comparing them)
+	void set_entry_grid(bool on) { use_grid_ = on; }
+	size_t grid_cells() const { return grid_base_ ? (size_t)grid_.n[0] * grid_.n[1] * grid_.n[2] : 0; }
 
 	// --- direct light (lighter.cpp:485-600, 1075-1096) -------------------
 	// items: (light << 26) | sample, grouped by light in ascending light
@@ -213,7 +225,10 @@ private:
 	void read_counters();
 	void* stream_ = nullptr;
 	// BVH and triangles
-	void* d_nodes_ = nullptr;
+	void* d_nodes_ = nullptr; // the tree's nodes, then the entry grid's cells
+	GridSpec grid_{};
+	int32_t grid_base_ = 0; // index of the first cell node (0: no grid)
+	bool use_grid_ = true;
 	void* d_dtri_ = nullptr;
 	void* d_rtri_ = nullptr;
 	size_t ntri_ = 0, nnodes_ = 0;

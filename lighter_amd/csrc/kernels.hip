@@ -54,7 +54,38 @@ struct View {
 	// scene box grown by the distance-query radius plus a safety margin:
 	// outside it every distance query returns exactly 2
 	float xlo[3], xhi[3];
+	// entry grid: cell (ix, iy, iz) of size 1 / ginv from glo; its entry node
+	// is nodes[gbase + (iz * gn[1] + iy) * gn[0] + ix]; gbase = 0: no grid
+	float glo[3], ginv;
+	int gn[3];
+	int32_t gbase;
 };
+
+// The entry grid.  Every query of the kernels here that is local (a distance
+// query of radius <= kGridReach, a segment no longer than kGridReach) can
+// only be answered by triangles that meet the ball of that radius around
+// its point (the segment's start).  Each cell of a uniform grid over the
+// scene holds a virtual wide node (after the tree's nodes) whose children
+// are a cut of the tree: at most four subtrees such that every triangle
+// meeting the cell grown by kGridReach (plus a rounding margin) lies in one
+// of them.  A local query starts at its cell's node instead of the root, so
+// it skips the levels above the cut and still meets every triangle that
+// can change its answer; its result is the same minimum.
+constexpr float kGridReach = kPenumbraSize;
+constexpr uint32_t kGridCells = 1u << 20; // cells at most (128 bytes each)
+
+// entry node of a query at p (the root outside the grid)
+__device__ __forceinline__ int32_t entry_node(const View& V, f3 p)
+{
+	if (!V.gbase)
+		return 0;
+	const float fx = (p.x - V.glo[0]) * V.ginv, fy = (p.y - V.glo[1]) * V.ginv, fz = (p.z - V.glo[2]) * V.ginv;
+	if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)V.gn[0] && fy < (float)V.gn[1] &&
+	      fz < (float)V.gn[2]))
+		return 0;
+	const int ix = min((int)fx, V.gn[0] - 1), iy = min((int)fy, V.gn[1] - 1), iz = min((int)fz, V.gn[2] - 1);
+	return V.gbase + (iz * V.gn[1] + iy) * V.gn[0] + ix;
+}
 
 // ---------------------------------------------------------------------------
 // transcendental helpers: evaluated in double and rounded once, which is the
@@ -248,7 +279,7 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 	if (!V.has_tris)
 		return best;
 	int sp = 0;
-	int32_t ref = 0;
+	int32_t ref = radius <= kGridReach ? entry_node(V, p) : 0;
 	unsigned nn = 0, nt = 0, ncull = 0;
 	for (;;) {
 		if (ref >= 0) {
@@ -368,7 +399,10 @@ __device__ __forceinline__ float slab(const Seg& s, float lx, float ly, float lz
 }
 
 // ANY = true: returns 0 on the first hit with r < 1 (occlusion), else 2.
-template <bool ANY, bool COUNT, class Stack>
+// GRID = false: the walk starts at the root even for a short segment (the
+// AO kernel: its 64 rays per texel read the top levels from the cache, and
+// the cell's node, one more miss per texel, cost more than those levels).
+template <bool ANY, bool COUNT, class Stack, bool GRID = true>
 __device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt, Stack& st)
 {
 	float best = 2.0f;
@@ -376,7 +410,8 @@ __device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt, S
 		return best;
 	Seg s = make_seg(A, B);
 	int sp = 0;
-	int32_t ref = 0;
+	const f3 AB = B - A;
+	int32_t ref = GRID && dot(AB, AB) <= kGridReach * kGridReach ? entry_node(V, A) : 0;
 	unsigned nn = 0, nt = 0;
 	for (;;) {
 		if (ref >= 0) {
@@ -1045,7 +1080,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
 			sincos((double)angle, &sa, &ca);
 			float2 sd = side[i];
 			f3 d = spiral_dir(SN, sd.x, sd.y, (float)ca, (float)sa) * P.distance;
-			float hit = q_segment<false, COUNT>(V, origin, origin + d, cnt, st);
+			float hit = q_segment<false, COUNT, LdsStack, false>(V, origin, origin + d, cnt, st);
 			if (hit < 1.0f)
 				term = 1.0f - hit; // a miss adds nothing (ao + 0 = ao)
 		}
@@ -1864,9 +1899,16 @@ void DeviceScene::free_all()
 	dfree(rad.x);
 }
 
-static View make_view(void* nodes, void* dtri, void* rtri, size_t ntri, const float* lo, const float* hi)
+static View make_view(void* nodes, void* dtri, void* rtri, size_t ntri, const float* lo, const float* hi,
+                      const GridSpec& G, int32_t gbase)
 {
 	View v;
+	for (int a = 0; a < 3; ++a) {
+		v.glo[a] = G.lo[a];
+		v.gn[a] = G.n[a];
+	}
+	v.ginv = 1.0f / G.g;
+	v.gbase = gbase;
 	v.nodes = (const WideNode*)nodes;
 	v.dtri = (const DistTri*)dtri;
 	v.rtri = (const RayTri*)rtri;
@@ -1892,6 +1934,139 @@ __global__ __launch_bounds__(kBlock) void k_tri_forms(const float* __restrict__ 
 	const f3 a = ld3(p), c = ld3(p + 3), e = ld3(p + 6);
 	dt[i] = make_dist_tri(a, c, e);
 	rt[i] = make_ray_tri(a, c, e);
+}
+
+__device__ __forceinline__ bool box_meets(const float* lo, const float* hi, float lx, float ly, float lz, float hx,
+                                          float hy, float hz)
+{
+	return lx <= hi[0] && hx >= lo[0] && ly <= hi[1] && hy >= lo[1] && lz <= hi[2] && hz >= lo[2];
+}
+
+// One lane per grid cell: its virtual node.  Starting from the root's
+// children that meet the cell's region (the cell grown by pad), the largest
+// inner node of the set is replaced by its children that meet the region
+// while the set stays within four nodes; a node whose children would not
+// fit stays.  Children that miss the region hold no triangle meeting it (a
+// node's box holds its subtree's triangles; boxes with a NaN bound hold only
+// triangles whose distances are NaN, which lower no minimum).
+__global__ __launch_bounds__(256) void k_entry_cells(const WideNode* __restrict__ nodes, GridSpec G,
+                                                     WideNode* __restrict__ cells)
+{
+	const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+	const uint32_t total = (uint32_t)G.n[0] * G.n[1] * G.n[2];
+	if (c >= total)
+		return;
+	const int ix = (int)(c % (uint32_t)G.n[0]), iy = (int)((c / (uint32_t)G.n[0]) % (uint32_t)G.n[1]),
+	          iz = (int)(c / ((uint32_t)G.n[0] * G.n[1]));
+	const int ic[3] = {ix, iy, iz};
+	float rlo[3], rhi[3];
+	for (int a = 0; a < 3; ++a) {
+		rlo[a] = G.lo[a] + (float)ic[a] * G.g - G.pad;
+		rhi[a] = G.lo[a] + (float)(ic[a] + 1) * G.g + G.pad;
+	}
+	float blo[4][3], bhi[4][3];
+	int32_t ref[4];
+	bool open[4];
+	int k = 0;
+	{
+		const WideNode& r = nodes[0];
+		for (int j = 0; j < 4; ++j)
+			if (box_meets(rlo, rhi, r.lox[j], r.loy[j], r.loz[j], r.hix[j], r.hiy[j], r.hiz[j])) {
+				blo[k][0] = r.lox[j]; blo[k][1] = r.loy[j]; blo[k][2] = r.loz[j];
+				bhi[k][0] = r.hix[j]; bhi[k][1] = r.hiy[j]; bhi[k][2] = r.hiz[j];
+				ref[k] = r.child[j];
+				open[k] = r.child[j] >= 0;
+				++k;
+			}
+	}
+	for (;;) {
+		int e = -1;
+		float area = -1.0f;
+		for (int i = 0; i < k; ++i) {
+			if (!open[i])
+				continue;
+			const float dx = bhi[i][0] - blo[i][0], dy = bhi[i][1] - blo[i][1], dz = bhi[i][2] - blo[i][2];
+			const float s2 = dx * dy + dy * dz + dz * dx;
+			if (e < 0 || s2 > area) {
+				e = i;
+				area = s2;
+			}
+		}
+		if (e < 0)
+			break;
+		const WideNode& n = nodes[ref[e]];
+		int take[4], m = 0;
+		for (int j = 0; j < 4; ++j)
+			if (box_meets(rlo, rhi, n.lox[j], n.loy[j], n.loz[j], n.hix[j], n.hiy[j], n.hiz[j]))
+				take[m++] = j;
+		if (k - 1 + m > 4) {
+			open[e] = false;
+			continue;
+		}
+		// entry e is replaced by the last entry, then the children appended
+		--k;
+		for (int a = 0; a < 3; ++a) {
+			blo[e][a] = blo[k][a];
+			bhi[e][a] = bhi[k][a];
+		}
+		ref[e] = ref[k];
+		open[e] = open[k];
+		for (int t = 0; t < m; ++t) {
+			const int j = take[t];
+			blo[k][0] = n.lox[j]; blo[k][1] = n.loy[j]; blo[k][2] = n.loz[j];
+			bhi[k][0] = n.hix[j]; bhi[k][1] = n.hiy[j]; bhi[k][2] = n.hiz[j];
+			ref[k] = n.child[j];
+			open[k] = n.child[j] >= 0;
+			++k;
+		}
+	}
+	WideNode w;
+	for (int j = 0; j < 4; ++j) {
+		const bool use = j < k;
+		w.lox[j] = use ? blo[j][0] : kFarBox;
+		w.loy[j] = use ? blo[j][1] : kFarBox;
+		w.loz[j] = use ? blo[j][2] : kFarBox;
+		w.hix[j] = use ? bhi[j][0] : kFarBox;
+		w.hiy[j] = use ? bhi[j][1] : kFarBox;
+		w.hiz[j] = use ? bhi[j][2] : kFarBox;
+		w.child[j] = use ? ref[j] : leaf_ref(0, 1);
+		w.pad[j] = 0;
+	}
+	cells[c] = w;
+}
+
+// Sizes the entry grid over the scene box grown by kGridReach: cubic cells,
+// at most kGridCells of them; false when the box is not finite.
+static bool grid_spec(const float lo[3], const float hi[3], GridSpec& G)
+{
+	double e[3], vol = 1.0, mx = 0.0;
+	for (int a = 0; a < 3; ++a) {
+		if (!std::isfinite(lo[a]) || !std::isfinite(hi[a]) || !(hi[a] >= lo[a]))
+			return false;
+		G.lo[a] = (float)((double)lo[a] - kGridReach);
+		e[a] = (double)hi[a] - lo[a] + 2.0 * kGridReach;
+		vol *= e[a];
+		mx = std::max(mx, std::max(fabs((double)lo[a]), fabs((double)hi[a])));
+	}
+	double g = std::cbrt(vol / kGridCells);
+	for (int it = 0; it < 64; ++it) {
+		double cells = 1.0;
+		for (int a = 0; a < 3; ++a)
+			cells *= std::max(1.0, std::ceil(e[a] / g));
+		if (cells <= kGridCells)
+			break;
+		g *= 1.05;
+	}
+	for (int a = 0; a < 3; ++a)
+		G.n[a] = (int)std::max(1.0, std::ceil(e[a] / g));
+	if ((double)G.n[0] * G.n[1] * G.n[2] > kGridCells)
+		return false;
+	G.g = (float)g;
+	// rounding of the cell index and of the region's bounds: a query point
+	// lies within 1e-3 of a cell plus 1e-5 of the coordinates' magnitude of
+	// its cell; distances within the same relative bound of the radius
+	G.pad = kGridReach * (1.0f + 1e-4f) + 1e-3f * G.g + 1e-5f * (float)mx + 1e-3f;
+	return true;
 }
 
 void DeviceScene::set_triangles(const float* tris, size_t ntri)
@@ -1921,15 +2096,26 @@ void DeviceScene::set_triangles(const float* tris, size_t ntri)
 	bvh_depth_ = b.depth;
 	persist_blocks_ = 0; // the march kernel's LDS stack follows the depth
 	nnodes_ = b.nodes.size();
-	d_nodes_ = dalloc<WideNode>(nnodes_);
+	lo_[0] = b.lo.x; lo_[1] = b.lo.y; lo_[2] = b.lo.z;
+	hi_[0] = b.hi.x; hi_[1] = b.hi.y; hi_[2] = b.hi.z;
+	GridSpec G{};
+	const bool grid = grid_spec(lo_, hi_, G) && nnodes_ + (size_t)G.n[0] * G.n[1] * G.n[2] < (size_t)INT32_MAX;
+	const size_t ncells = grid ? (size_t)G.n[0] * G.n[1] * G.n[2] : 0;
+	d_nodes_ = dalloc<WideNode>(nnodes_ + ncells);
 	d_dtri_ = dalloc<DistTri>(ntri);
 	d_rtri_ = dalloc<RayTri>(ntri);
 	HIPCHECK(hipMemcpyAsync(d_nodes_, b.nodes.data(), sizeof(WideNode) * nnodes_, hipMemcpyHostToDevice, s));
 	k_tri_forms<<<grid_for(ntri), kBlock, 0, s>>>(d_tris.p, d_order.p, ntri, (DistTri*)d_dtri_, (RayTri*)d_rtri_);
 	HIPCHECK(hipGetLastError());
+	grid_ = G;
+	grid_base_ = 0;
+	if (grid) {
+		k_entry_cells<<<(unsigned)((ncells + 255) / 256), 256, 0, s>>>((const WideNode*)d_nodes_, G,
+		                                                               (WideNode*)d_nodes_ + nnodes_);
+		HIPCHECK(hipGetLastError());
+		grid_base_ = (int32_t)nnodes_;
+	}
 	HIPCHECK(hipStreamSynchronize(s));
-	lo_[0] = b.lo.x; lo_[1] = b.lo.y; lo_[2] = b.lo.z;
-	hi_[0] = b.hi.x; hi_[1] = b.hi.y; hi_[2] = b.hi.z;
 }
 
 void DeviceScene::set_samples(const float* pos, const float* nrm, size_t n)
@@ -2092,7 +2278,7 @@ bool DeviceScene::wait_march(uint32_t nmarch, const std::function<bool(float)>& 
 bool DeviceScene::run_direct(const std::function<bool(float)>& keep_going)
 {
 	hipStream_t s = (hipStream_t)stream_;
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
 	bool aborted = false;
 	ctl_host_[0] = 0u; // read by the kernel after the launch below
 	ctl_host_[1] = 0u;
@@ -2190,7 +2376,7 @@ void DeviceScene::run_direct_noalloc(const DevLight* lights, int nlights, const 
 	unsigned long long* cnt = (unsigned long long*)(sc + kNoAllocCntOff);
 	DevLight* dl = (DevLight*)(sc + kNoAllocLightOff);
 	uint32_t* doff = (uint32_t*)(sc + kNoAllocOffOff);
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
 	if (nlights)
 		HIPCHECK(hipMemcpyAsync(dl, lights, sizeof(DevLight) * nlights, hipMemcpyHostToDevice, s));
 	HIPCHECK(hipMemcpyAsync(doff, light_off, 4 * (size_t)(nlights + 1), hipMemcpyHostToDevice, s));
@@ -2269,7 +2455,7 @@ void DeviceScene::reference_counts(const float* tris, size_t ntri, uint64_t out[
 	C.nodes = d_nodes;
 	C.dtri = d_dt;
 	canon_root_box(b, C.lo, C.hi);
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
 	k_march_canon<<<grid_for(nmarch_), kBlock, 0, s>>>(C, V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch_,
 	                                                   d_vis_, d_out);
 	HIPCHECK(hipGetLastError());
@@ -2622,7 +2808,7 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 	hipStream_t s = (hipStream_t)stream_;
 	if (nsamples_ == 0 || num_samples <= 0)
 		return;
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
 	// (i + 0.5) / n, its square root and sin(acos(.)) depend only on (i, n):
 	// evaluated here with the C library the reference uses
 	std::vector<float2> side(num_samples);
@@ -2676,7 +2862,7 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 // ---------------------------------------------------------------------------
 void DeviceScene::query_distance(const float* pts, size_t n, float* out)
 {
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
 	float *dp = dalloc<float>(3 * n), *dout = dalloc<float>(n);
 	HIPCHECK(hipMemcpy(dp, pts, 12 * n, hipMemcpyHostToDevice));
 	k_q_distance<<<grid_for(n), kBlock, 0, (hipStream_t)stream_>>>(V, dp, (uint32_t)n, dout);
@@ -2690,7 +2876,7 @@ void DeviceScene::query_distance(const float* pts, size_t n, float* out)
 void DeviceScene::query_shadow(const float* from, const float* to, const float* k, size_t n, float* out,
                                uint32_t* steps)
 {
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
 	float *da = dalloc<float>(3 * n), *db = dalloc<float>(3 * n), *dk = dalloc<float>(n), *dout = dalloc<float>(n);
 	uint32_t* dst = dalloc<uint32_t>(n);
 	HIPCHECK(hipMemcpy(da, from, 12 * n, hipMemcpyHostToDevice));
@@ -2711,7 +2897,7 @@ void DeviceScene::query_shadow(const float* from, const float* to, const float* 
 
 void DeviceScene::query_closest(const float* a, const float* b, size_t n, float* out)
 {
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
 	float *da = dalloc<float>(3 * n), *db = dalloc<float>(3 * n), *dout = dalloc<float>(n);
 	HIPCHECK(hipMemcpy(da, a, 12 * n, hipMemcpyHostToDevice));
 	HIPCHECK(hipMemcpy(db, b, 12 * n, hipMemcpyHostToDevice));
@@ -2726,7 +2912,7 @@ void DeviceScene::query_closest(const float* a, const float* b, size_t n, float*
 
 void DeviceScene::query_occluded(const float* a, const float* b, size_t n, uint8_t* out)
 {
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
 	float *da = dalloc<float>(3 * n), *db = dalloc<float>(3 * n);
 	uint8_t* dout = dalloc<uint8_t>(n);
 	HIPCHECK(hipMemcpy(da, a, 12 * n, hipMemcpyHostToDevice));
@@ -2906,7 +3092,7 @@ size_t DeviceScene::rad_links()
 {
 	hipStream_t s = (hipStream_t)stream_;
 	const uint32_t N = (uint32_t)npatches;
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
 	dfree(rad.rowptr);
 	dfree(rad.rowptr64);
 	dfree(rad.col);

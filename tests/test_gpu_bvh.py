@@ -7,13 +7,18 @@ Soups: the room, the sliver scene (near-degenerate triangles), the terrain
 workload (548,288 triangles), a city block, coincident centroids (every
 split by position), tiny soups (1-3 triangles: the leaf-only root) and a
 random soup.  That the tree answers every query exactly is checked by the
-query, march and AO tests, which run on this build."""
+query, march and AO tests, which run on this build.
+
+Local queries (distance queries of radius <= 2, segments of at most 2 m)
+start at their cell of the entry grid, whose node lists at most four
+subtrees holding every triangle within 2 m of the cell: every answer must
+equal, bit for bit, the answer of the walk from the root."""
 import ctypes as C
 
 import numpy as np
 import pytest
 
-from helpers import all_samples_and_tris, prepared
+from helpers import Device, all_samples_and_tris, bits, prepared
 from lighter_amd import _lib, scenes
 
 pytestmark = pytest.mark.gpu
@@ -56,3 +61,61 @@ def test_gpu_bvh_special_soups(lib):
     for name, t in soups.items():
         out = _compare(lib, t)
         assert out[0] == 1 and out[1] == 1, (name, out)
+
+
+def _entry_grid(lib, dev, on):
+    cells = C.c_int64(0)
+    _lib.check(lib.lgx_SetEntryGrid(dev.h, on, C.byref(cells)), lib)
+    return cells.value
+
+
+@pytest.mark.parametrize("name", ["room", "terrain", "city_block", "multiscale"])
+def test_entry_grid_answers_as_the_root(lib, name):
+    """Distance queries, closest-hit and occlusion segments (short ones from
+    the grid, long ones from the root) and shadow marches with the entry
+    grid on and off: identical bits.  Points lie near the surfaces, in the
+    open, on cell faces and outside the grid."""
+    rng = np.random.default_rng(80)
+    if name == "multiscale":
+        # small triangles scattered over 100 km beside metre-sized ones
+        tiny = (rng.uniform(-5e4, 5e4, (4000, 1, 3)) + rng.uniform(0, 1e-3, (4000, 3, 3))).reshape(-1, 9)
+        tris = np.ascontiguousarray(np.concatenate([tiny, rng.uniform(-5, 5, (4000, 9))]), np.float32)
+        a = tris[rng.choice(len(tris), 20000), :3] + rng.normal(0, 1.0, (20000, 3))
+    else:
+        spec = {"room": lambda: scenes.room_scene(lm=48),
+                "terrain": scenes.terrain_scene,
+                "city_block": lambda: scenes.city_scene(extent=200.0, grid=64, n_boxes=12000, tiles=4,
+                                                        lm_total=1024, n_lights=4)}[name]()
+        b = prepared(lib, spec)
+        pos, nrm, tris = all_samples_and_tris(lib, b)
+        b.destroy()
+        idx = rng.choice(len(pos), min(len(pos), 20000), replace=False)
+        a = pos[idx] + nrm[idx] * rng.uniform(0.0, 1.5, (len(idx), 1))
+    t = tris.reshape(-1, 3)
+    lo, hi = t.min(0), t.max(0)
+    # open space, and points around and beyond the grown scene box
+    a = np.concatenate([a, rng.uniform(lo - 3, hi + 3, (5000, 3))])
+    a = np.ascontiguousarray(a, np.float32)
+    d = rng.normal(size=a.shape)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    near = np.ascontiguousarray(a + d * rng.uniform(0.05, 2.0, (len(a), 1)), np.float32)
+    far = np.ascontiguousarray(a + d * rng.uniform(2.0, 40.0, (len(a), 1)), np.float32)
+    k = rng.choice([0.0, 0.5, 2.0], len(a)).astype(np.float32)
+    dev = Device(lib, tris)
+    try:
+        res = []
+        for on in (0, 1):
+            cells = _entry_grid(lib, dev, on)
+            res.append((dev.distance(a), dev.closest(a, near), dev.occluded(a, near), dev.closest(a, far),
+                        dev.occluded(a, far), dev.shadow(a, far, k)))
+        assert cells > 1000, cells
+        for x, y in zip(res[0][:5], res[1][:5]):
+            assert np.array_equal(bits(x) if x.dtype == np.float32 else x, bits(y) if y.dtype == np.float32 else y)
+        assert np.array_equal(bits(res[0][5][0]), bits(res[1][5][0]))
+        assert np.array_equal(res[0][5][1], res[1][5][1])
+        dist = res[1][0]
+        assert (dist < 2).mean() > 0.2 and (dist == 2).mean() > 0.05
+        assert 0 < res[1][2].mean() < 1
+        print(f"{name}: {len(tris)} triangles, {cells} cells")
+    finally:
+        dev.close()

@@ -710,8 +710,12 @@ class DirectLeg:
         info = (C.c_int64 * 3)()
         lib.lgx_DeviceInfo(self.dev, info)
         self.ntri, self.nodes_bvh, self.depth = int(info[0]), int(info[1]), int(info[2])
-        log(f"[bench] {spec.name} device scene (BVH build, triangle forms, upload): {self.device_create_s:.2f} s; "
-            f"{self.ntri} tris, {self.nodes_bvh} nodes, depth {self.depth}")
+        cells = C.c_int64(0)
+        _lib.check(lib.lgx_SetEntryGrid(self.dev, 1 if ENTRY_GRID else 0, C.byref(cells)), lib)
+        self.grid_cells = int(cells.value) if ENTRY_GRID else 0
+        log(f"[bench] {spec.name} device scene (BVH build, triangle forms, entry grid, upload): "
+            f"{self.device_create_s:.2f} s; {self.ntri} tris, {self.nodes_bvh} nodes, depth {self.depth}, "
+            f"{self.grid_cells} entry-grid cells")
         _lib.check(lib.lgx_SetSamples(self.dev, P(self.pos), P(self.nrm), self.S, None), lib)
         self.lights = make_lights(spec)
         self.setup(keep_vis=1)
@@ -1161,7 +1165,8 @@ def main():
     if not args.no_c4_extras:
         extras["c4"] = c4_extras(lib, leg, nth, args.cpu_seconds, with_cpu)
     head = dict(value=leg.value, ms=leg.ms_per_step, march=leg.march_avg, marches=leg.marches, qsteps=leg.qsteps,
-                S=leg.S, items=len(leg.items), ntri=leg.ntri, host_s=leg.host_s, dev_s=leg.device_create_s)
+                S=leg.S, items=len(leg.items), ntri=leg.ntri, host_s=leg.host_s, dev_s=leg.device_create_s,
+                cells=leg.grid_cells)
     leg.close()
     leg.built.destroy()
 
@@ -1189,7 +1194,8 @@ def main():
                config=dict(workload=C4_WORKLOAD, triangles=head["ntri"], samples=head["S"], lights=len(spec.lights),
                            work_items=head["items"], shadow_rays_per_step=head["marches"],
                            march_steps_per_step=head["qsteps"], march_kernel_ms=round(head["march"], 3),
-                           host_stages_s=round(head["host_s"], 2), device_create_s=round(head["dev_s"], 2)),
+                           host_stages_s=round(head["host_s"], 2), device_create_s=round(head["dev_s"], 2),
+                           entry_grid_cells=head["cells"]),
                roofline=roofline, cpu_baseline=cpu, parity_sample=parity,
                speedup_vs_cpu=round(head["value"] / cpu["value"], 1) if cpu else None, host=cores)
     out.update(extras)

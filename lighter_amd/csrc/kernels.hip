@@ -72,7 +72,8 @@ struct View {
 // it skips the levels above the cut and still meets every triangle that
 // can change its answer; its result is the same minimum.
 constexpr float kGridReach = kPenumbraSize;
-constexpr uint32_t kGridCells = 1u << 20; // cells at most (128 bytes each)
+constexpr uint32_t kGridCells = 1u << 22; // cells at most (128 bytes each, 512 MB)
+constexpr float kGridMinCell = 1.0f;      // metres: finer cells gain nothing against the 2 m reach
 
 // entry node of a query at p (the root outside the grid)
 __device__ __forceinline__ int32_t entry_node(const View& V, f3 p)
@@ -2035,8 +2036,9 @@ __global__ __launch_bounds__(256) void k_entry_cells(const WideNode* __restrict_
 	cells[c] = w;
 }
 
-// Sizes the entry grid over the scene box grown by kGridReach: cubic cells,
-// at most kGridCells of them; false when the box is not finite.
+// Sizes the entry grid over the scene box grown by kGridReach: cubic cells
+// of at least kGridMinCell, at most kGridCells of them; false when the box
+// is not finite.
 static bool grid_spec(const float lo[3], const float hi[3], GridSpec& G)
 {
 	double e[3], vol = 1.0, mx = 0.0;
@@ -2048,7 +2050,7 @@ static bool grid_spec(const float lo[3], const float hi[3], GridSpec& G)
 		vol *= e[a];
 		mx = std::max(mx, std::max(fabs((double)lo[a]), fabs((double)hi[a])));
 	}
-	double g = std::cbrt(vol / kGridCells);
+	double g = std::max((double)kGridMinCell, std::cbrt(vol / kGridCells));
 	for (int it = 0; it < 64; ++it) {
 		double cells = 1.0;
 		for (int a = 0; a < 3; ++a)

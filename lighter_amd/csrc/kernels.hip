@@ -400,11 +400,11 @@ __device__ __forceinline__ float slab(const Seg& s, float lx, float ly, float lz
 }
 
 // ANY = true: returns 0 on the first hit with r < 1 (occlusion), else 2.
-// GRID = false: the walk starts at the root even for a short segment (the
-// AO kernel: its 64 rays per texel read the top levels from the cache, and
-// the cell's node, one more miss per texel, cost more than those levels).
-template <bool ANY, bool COUNT, class Stack, bool GRID = true>
-__device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt, Stack& st)
+// start >= 0: the walk starts at that node, which the caller took from the
+// entry grid at A for a segment of at most kGridReach; -1: at A's cell for
+// such a segment, else at the root.
+template <bool ANY, bool COUNT, class Stack>
+__device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt, Stack& st, int32_t start = -1)
 {
 	float best = 2.0f;
 	if (!V.has_tris)
@@ -412,7 +412,7 @@ __device__ float q_segment(const View& V, f3 A, f3 B, unsigned long long* cnt, S
 	Seg s = make_seg(A, B);
 	int sp = 0;
 	const f3 AB = B - A;
-	int32_t ref = GRID && dot(AB, AB) <= kGridReach * kGridReach ? entry_node(V, A) : 0;
+	int32_t ref = start >= 0 ? start : dot(AB, AB) <= kGridReach * kGridReach ? entry_node(V, A) : 0;
 	unsigned nn = 0, nt = 0;
 	for (;;) {
 		if (ref >= 0) {
@@ -1071,6 +1071,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
 	const f3 ov = SP + SN * (kSmall * 2);
 	const f3 origin = mk3(wave_uniform(ov.x), wave_uniform(ov.y), wave_uniform(ov.z));
 	const float ro = wave_uniform(randoff[s]);
+	// The rays (at most P.distance <= kGridReach long) start at the origin's
+	// entry-grid cell; its node is fetched here, before the directions are
+	// computed, so that the fetch overlaps that arithmetic (taken at the
+	// first walk's start instead, it was one exposed miss per texel and
+	// cost more than the levels it skips).
+	const int32_t start = P.distance <= kGridReach ? entry_node(V, origin) : 0;
+	const uint32_t touch = reinterpret_cast<const uint32_t*>(V.nodes + start)[lane & 31];
 	float ao = 0;
 	for (int i0 = 0; i0 < P.n; i0 += 64) {
 		const int i = i0 + lane;
@@ -1081,7 +1088,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
 			sincos((double)angle, &sa, &ca);
 			float2 sd = side[i];
 			f3 d = spiral_dir(SN, sd.x, sd.y, (float)ca, (float)sa) * P.distance;
-			float hit = q_segment<false, COUNT, LdsStack, false>(V, origin, origin + d, cnt, st);
+			asm volatile("" ::"v"(touch));
+			float hit = q_segment<false, COUNT>(V, origin, origin + d, cnt, st, start);
 			if (hit < 1.0f)
 				term = 1.0f - hit; // a miss adds nothing (ao + 0 = ao)
 		}

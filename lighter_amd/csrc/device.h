@@ -228,6 +228,7 @@ private:
 	void* d_nodes_ = nullptr; // the tree's nodes, then the entry grid's cells
 	GridSpec grid_{};
 	int32_t grid_base_ = 0; // index of the first cell node (0: no grid)
+	int32_t grid_fine_ = 0; // the fine cuts' first cell node (0: none)
 	bool use_grid_ = true;
 	void* d_dtri_ = nullptr;
 	void* d_rtri_ = nullptr;

@@ -59,6 +59,9 @@ struct View {
 	float glo[3], ginv;
 	int gn[3];
 	int32_t gbase;
+	// the same cells again from gfine on, each a cut for the cell grown by
+	// kGridFineReach only, for distance queries of radius <= 1 (0: none)
+	int32_t gfine;
 };
 
 // The entry grid.  Every query of the kernels here that is local (a distance
@@ -68,15 +71,20 @@ struct View {
 // scene holds a virtual wide node (after the tree's nodes) whose children
 // are a cut of the tree: at most four subtrees such that every triangle
 // meeting the cell grown by kGridReach (plus a rounding margin) lies in one
-// of them.  A local query starts at its cell's node instead of the root, so
-// it skips the levels above the cut and still meets every triangle that
-// can change its answer; its result is the same minimum.
+// of them.  A second node per cell (from gfine on) is the same cut for the
+// cell grown by kGridFineReach only, for distance queries of radius <= 1:
+// a smaller region, a deeper cut.  A local query starts at its cell's node
+// instead of the root, so it skips the levels above the cut and still
+// meets every triangle that can change its answer; its result is the same
+// minimum.
 constexpr float kGridReach = kPenumbraSize;
-constexpr uint32_t kGridCells = 1u << 22; // cells at most (128 bytes each, 512 MB)
+constexpr float kGridFineReach = kPenumbraStep; // 1 m: the radius of many march queries
+constexpr uint32_t kGridCells = 1u << 22; // cells at most (two nodes of 128 bytes each, 1 GB)
 constexpr float kGridMinCell = 1.0f;      // metres: finer cells gain nothing against the 2 m reach
 
-// entry node of a query at p (the root outside the grid)
-__device__ __forceinline__ int32_t entry_node(const View& V, f3 p)
+// entry node of a query at p (the root outside the grid); fine: the query's
+// radius is at most kGridFineReach
+__device__ __forceinline__ int32_t entry_node(const View& V, f3 p, bool fine = false)
 {
 	if (!V.gbase)
 		return 0;
@@ -85,7 +93,7 @@ __device__ __forceinline__ int32_t entry_node(const View& V, f3 p)
 	      fz < (float)V.gn[2]))
 		return 0;
 	const int ix = min((int)fx, V.gn[0] - 1), iy = min((int)fy, V.gn[1] - 1), iz = min((int)fz, V.gn[2] - 1);
-	return V.gbase + (iz * V.gn[1] + iy) * V.gn[0] + ix;
+	return (fine && V.gfine ? V.gfine : V.gbase) + (iz * V.gn[1] + iy) * V.gn[0] + ix;
 }
 
 // ---------------------------------------------------------------------------
@@ -280,7 +288,7 @@ __device__ float q_distance(const View& V, f3 p, float radius, unsigned long lon
 	if (!V.has_tris)
 		return best;
 	int sp = 0;
-	int32_t ref = radius <= kGridReach ? entry_node(V, p) : 0;
+	int32_t ref = radius <= kGridReach ? entry_node(V, p, radius <= kGridFineReach) : 0;
 	unsigned nn = 0, nt = 0, ncull = 0;
 	for (;;) {
 		if (ref >= 0) {
@@ -1909,9 +1917,10 @@ void DeviceScene::free_all()
 }
 
 static View make_view(void* nodes, void* dtri, void* rtri, size_t ntri, const float* lo, const float* hi,
-                      const GridSpec& G, int32_t gbase)
+                      const GridSpec& G, int32_t gbase, int32_t gfine)
 {
 	View v;
+	v.gfine = gfine;
 	for (int a = 0; a < 3; ++a) {
 		v.glo[a] = G.lo[a];
 		v.gn[a] = G.n[a];
@@ -2111,7 +2120,8 @@ void DeviceScene::set_triangles(const float* tris, size_t ntri)
 	GridSpec G{};
 	const bool grid = grid_spec(lo_, hi_, G) && nnodes_ + (size_t)G.n[0] * G.n[1] * G.n[2] < (size_t)INT32_MAX;
 	const size_t ncells = grid ? (size_t)G.n[0] * G.n[1] * G.n[2] : 0;
-	d_nodes_ = dalloc<WideNode>(nnodes_ + ncells);
+	const bool fine = grid && nnodes_ + 2 * ncells < (size_t)INT32_MAX;
+	d_nodes_ = dalloc<WideNode>(nnodes_ + (fine ? 2 : 1) * ncells);
 	d_dtri_ = dalloc<DistTri>(ntri);
 	d_rtri_ = dalloc<RayTri>(ntri);
 	HIPCHECK(hipMemcpyAsync(d_nodes_, b.nodes.data(), sizeof(WideNode) * nnodes_, hipMemcpyHostToDevice, s));
@@ -2124,6 +2134,15 @@ void DeviceScene::set_triangles(const float* tris, size_t ntri)
 		                                                               (WideNode*)d_nodes_ + nnodes_);
 		HIPCHECK(hipGetLastError());
 		grid_base_ = (int32_t)nnodes_;
+	}
+	grid_fine_ = 0;
+	if (fine) {
+		GridSpec F = G;
+		F.pad = G.pad - kGridReach * (1.0f + 1e-4f) + kGridFineReach * (1.0f + 1e-4f);
+		k_entry_cells<<<(unsigned)((ncells + 255) / 256), 256, 0, s>>>((const WideNode*)d_nodes_, F,
+		                                                               (WideNode*)d_nodes_ + nnodes_ + ncells);
+		HIPCHECK(hipGetLastError());
+		grid_fine_ = (int32_t)(nnodes_ + ncells);
 	}
 	HIPCHECK(hipStreamSynchronize(s));
 }
@@ -2288,7 +2307,7 @@ bool DeviceScene::wait_march(uint32_t nmarch, const std::function<bool(float)>& 
 bool DeviceScene::run_direct(const std::function<bool(float)>& keep_going)
 {
 	hipStream_t s = (hipStream_t)stream_;
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0, use_grid_ ? grid_fine_ : 0);
 	bool aborted = false;
 	ctl_host_[0] = 0u; // read by the kernel after the launch below
 	ctl_host_[1] = 0u;
@@ -2386,7 +2405,7 @@ void DeviceScene::run_direct_noalloc(const DevLight* lights, int nlights, const 
 	unsigned long long* cnt = (unsigned long long*)(sc + kNoAllocCntOff);
 	DevLight* dl = (DevLight*)(sc + kNoAllocLightOff);
 	uint32_t* doff = (uint32_t*)(sc + kNoAllocOffOff);
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0, use_grid_ ? grid_fine_ : 0);
 	if (nlights)
 		HIPCHECK(hipMemcpyAsync(dl, lights, sizeof(DevLight) * nlights, hipMemcpyHostToDevice, s));
 	HIPCHECK(hipMemcpyAsync(doff, light_off, 4 * (size_t)(nlights + 1), hipMemcpyHostToDevice, s));
@@ -2465,7 +2484,7 @@ void DeviceScene::reference_counts(const float* tris, size_t ntri, uint64_t out[
 	C.nodes = d_nodes;
 	C.dtri = d_dt;
 	canon_root_box(b, C.lo, C.hi);
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0, use_grid_ ? grid_fine_ : 0);
 	k_march_canon<<<grid_for(nmarch_), kBlock, 0, s>>>(C, V, d_pos_, d_nrm_, d_lights_, d_items_, d_mlist_, nmarch_,
 	                                                   d_vis_, d_out);
 	HIPCHECK(hipGetLastError());
@@ -2818,7 +2837,7 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 	hipStream_t s = (hipStream_t)stream_;
 	if (nsamples_ == 0 || num_samples <= 0)
 		return;
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0, use_grid_ ? grid_fine_ : 0);
 	// (i + 0.5) / n, its square root and sin(acos(.)) depend only on (i, n):
 	// evaluated here with the C library the reference uses
 	std::vector<float2> side(num_samples);
@@ -2872,7 +2891,7 @@ void DeviceScene::ambient_occlusion(const float* randoff, int num_samples, float
 // ---------------------------------------------------------------------------
 void DeviceScene::query_distance(const float* pts, size_t n, float* out)
 {
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0, use_grid_ ? grid_fine_ : 0);
 	float *dp = dalloc<float>(3 * n), *dout = dalloc<float>(n);
 	HIPCHECK(hipMemcpy(dp, pts, 12 * n, hipMemcpyHostToDevice));
 	k_q_distance<<<grid_for(n), kBlock, 0, (hipStream_t)stream_>>>(V, dp, (uint32_t)n, dout);
@@ -2886,7 +2905,7 @@ void DeviceScene::query_distance(const float* pts, size_t n, float* out)
 void DeviceScene::query_shadow(const float* from, const float* to, const float* k, size_t n, float* out,
                                uint32_t* steps)
 {
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0, use_grid_ ? grid_fine_ : 0);
 	float *da = dalloc<float>(3 * n), *db = dalloc<float>(3 * n), *dk = dalloc<float>(n), *dout = dalloc<float>(n);
 	uint32_t* dst = dalloc<uint32_t>(n);
 	HIPCHECK(hipMemcpy(da, from, 12 * n, hipMemcpyHostToDevice));
@@ -2907,7 +2926,7 @@ void DeviceScene::query_shadow(const float* from, const float* to, const float* 
 
 void DeviceScene::query_closest(const float* a, const float* b, size_t n, float* out)
 {
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0, use_grid_ ? grid_fine_ : 0);
 	float *da = dalloc<float>(3 * n), *db = dalloc<float>(3 * n), *dout = dalloc<float>(n);
 	HIPCHECK(hipMemcpy(da, a, 12 * n, hipMemcpyHostToDevice));
 	HIPCHECK(hipMemcpy(db, b, 12 * n, hipMemcpyHostToDevice));
@@ -2922,7 +2941,7 @@ void DeviceScene::query_closest(const float* a, const float* b, size_t n, float*
 
 void DeviceScene::query_occluded(const float* a, const float* b, size_t n, uint8_t* out)
 {
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0, use_grid_ ? grid_fine_ : 0);
 	float *da = dalloc<float>(3 * n), *db = dalloc<float>(3 * n);
 	uint8_t* dout = dalloc<uint8_t>(n);
 	HIPCHECK(hipMemcpy(da, a, 12 * n, hipMemcpyHostToDevice));
@@ -3102,7 +3121,7 @@ size_t DeviceScene::rad_links()
 {
 	hipStream_t s = (hipStream_t)stream_;
 	const uint32_t N = (uint32_t)npatches;
-	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0);
+	View V = make_view(d_nodes_, d_dtri_, d_rtri_, ntri_, lo_, hi_, grid_, use_grid_ ? grid_base_ : 0, use_grid_ ? grid_fine_ : 0);
 	dfree(rad.rowptr);
 	dfree(rad.rowptr64);
 	dfree(rad.col);

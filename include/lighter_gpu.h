@@ -131,8 +131,9 @@ LTRAPI int lgx_DeviceInfo(lgx_Device* dev, int64_t out[3]);
 /* The entry grid (no counterpart in the reference, whose queries start at
  * the root of each BBT, lighter_math.cpp:653-781): a uniform grid over the
  * scene whose cells each hold a node listing at most four subtrees that
- * contain every triangle within 2 m of the cell; distance queries (radius
- * <= 2) and segments of at most 2 m start at their cell's node.  Results
+ * contain every triangle within 2 m of the cell, and one for 1 m; distance
+ * queries (radius <= 2, <= 1) and segments of at most 2 m start at their
+ * cell's node.  Results
  * are the same with it on (the default) or off (on = 0: every query starts
  * at the root).  cells (may be null) receives the grid's cell count, 0
  * when the scene has none (no triangles, or a non-finite scene box). */
